@@ -1,0 +1,134 @@
+"""ctypes binding of libfh.so (include/fh.h).
+
+The library is the only compute path: when it is missing or no HIP device is
+visible, every entry point raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfh.so")
+
+FH_OK = 0
+FH_EINVAL = -1
+FH_ENOMEM = -2
+FH_EDEVICE = -3
+FH_EUNSUPPORTED = -4
+FH_MAX_DIM = 64
+
+# Every symbol include/fh.h declares.
+EXPORTED_SYMBOLS = (
+    "fh_create",
+    "fh_destroy",
+    "fh_last_error",
+    "fh_version",
+    "fh_mst_euclidean_device",
+    "fh_mst_euclidean",
+    "fh_knn",
+    "fh_tree_nnodes",
+    "fh_build_kdtree",
+)
+
+
+class FHStats(ctypes.Structure):
+    _fields_ = [
+        ("rounds", ctypes.c_int32),
+        ("n_levels", ctypes.c_int32),
+        ("n_nodes", ctypes.c_int64),
+        ("initial_components", ctypes.c_int64),
+        ("n_edges", ctypes.c_int64),
+        ("t_tree_ms", ctypes.c_double),
+        ("t_knn_ms", ctypes.c_double),
+        ("t_init_ms", ctypes.c_double),
+        ("t_rounds_ms", ctypes.c_double),
+        ("t_query_ms", ctypes.c_double),
+        ("t_total_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class FHError(RuntimeError):
+    pass
+
+
+_lib = None
+_lock = threading.Lock()
+_ctx = {}
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+
+
+def load() -> ctypes.CDLL:
+    """Load libfh.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `make -C fast_hdbscan_amd/csrc` "
+                "(or __graft_entry__.build()); there is no CPU fallback"
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        L.fh_create.restype = _vp
+        L.fh_create.argtypes = [ctypes.c_int]
+        L.fh_destroy.restype = None
+        L.fh_destroy.argtypes = [_vp]
+        L.fh_last_error.restype = ctypes.c_char_p
+        L.fh_last_error.argtypes = [_vp]
+        L.fh_version.restype = ctypes.c_char_p
+        L.fh_version.argtypes = []
+        mst_args = [_vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(FHStats)]
+        L.fh_mst_euclidean_device.restype = ctypes.c_int
+        L.fh_mst_euclidean_device.argtypes = mst_args
+        L.fh_mst_euclidean.restype = ctypes.c_int
+        L.fh_mst_euclidean.argtypes = mst_args
+        L.fh_knn.restype = ctypes.c_int
+        L.fh_knn.argtypes = [_vp, _vp, _i64, _i32, _vp, _i64, _i32, _vp, _vp]
+        L.fh_tree_nnodes.restype = _i64
+        L.fh_tree_nnodes.argtypes = [_i64]
+        L.fh_build_kdtree.restype = ctypes.c_int
+        L.fh_build_kdtree.argtypes = [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp]
+        _lib = L
+    return _lib
+
+
+def context(device: int = 0):
+    """Per-device context (one HIP stream + cached device buffers)."""
+    L = load()
+    with _lock:
+        c = _ctx.get(device)
+        if c is None:
+            c = L.fh_create(int(device))
+            if not c:
+                raise FHError(f"fh_create({device}) failed: no usable HIP device")
+            _ctx[device] = c
+    return c
+
+
+def check(ctx, code: int) -> None:
+    if code == FH_OK:
+        return
+    msg = load().fh_last_error(ctx).decode()
+    if code == FH_EINVAL:
+        raise ValueError(msg)
+    if code == FH_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    if code == FH_ENOMEM:
+        raise MemoryError(msg)
+    raise FHError(msg)
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)

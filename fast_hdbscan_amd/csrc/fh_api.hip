@@ -1,0 +1,481 @@
+// C-ABI entry points (include/fh.h) and the host driver of the MST path.
+//
+// Per call:
+//   1. KD-tree built on the host exactly as numba_kdtree.build_kdtree, rows
+//      uploaded in tree order.
+//   2. knn kernel: exact (min_samples+1)-NN of every point, core distances.
+//   3. kNN-seeded forest (boruvka.py:486-524): edge choice, cycle skip by
+//      pointer doubling, emit in point order, lock-free union.
+//   4. Boruvka rounds (boruvka.py:610-640) until one component is left:
+//      node purity, nearest out-of-component search with packed atomicMin per
+//      component, per-component selection, cycle skip, emit in order of the
+//      component's lowest point, union + relabel, root compaction.  The host
+//      reads back two integers per round (edges emitted, components left).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fh.h"
+#include "fh_kernels.h"
+#include "kdtree_build.h"
+
+using fh::HostTree;
+
+namespace {
+
+constexpr int64_t kLeafSize = 40;  // numba_kdtree.py:478 default, used by hdbscan.py:466
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Timer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+}  // namespace
+
+struct fh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int host_threads = 0;
+
+    DevBuf<float> xp, box, core_p, core_o, knn_d, w_init, bound, ew, heap_p, queries, out_w, out_core;
+    DevBuf<int> perm, posof, comp_p, node_comp, knn_i, succ, dst_raw, S0, M0, S1, M1, keep, offs, parent, roots,
+        roots2, rootslot, partner, ei, ej, tiles, small, heap_i, flag, out_src, out_dst, out_nbr;
+    DevBuf<int2> range;
+    DevBuf<unsigned long long> best;
+    int *h_small = nullptr;  // pinned readback slots
+    hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+int fail(fh_ctx *c, int code, const std::string &msg) {
+    c->err = msg;
+    return code;
+}
+
+#define FH_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(ctx, FH_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+template <class T>
+int ensure(fh_ctx *ctx, DevBuf<T> &b, size_t count) {
+    if (count == 0) count = 1;
+    if (b.cap >= count) return FH_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        (void)e;
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    // Grow with headroom so a sequence of similar sizes does not reallocate.
+    size_t want = count + count / 8;
+    if (hipMalloc(reinterpret_cast<void **>(&b.p), want * sizeof(T)) != hipSuccess) {
+        b.p = nullptr;
+        return fail(ctx, FH_ENOMEM, "hipMalloc of " + std::to_string(want * sizeof(T)) + " bytes failed");
+    }
+    b.cap = want;
+    return FH_OK;
+}
+
+template <class T>
+void release(DevBuf<T> &b) {
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        (void)e;
+    }
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+#define FH_TRY(x)               \
+    do {                        \
+        int r_ = (x);           \
+        if (r_ != FH_OK) return r_; \
+    } while (0)
+
+float event_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
+    return ms;
+}
+
+// Build the host tree and upload it in the device layout.
+int upload_tree(fh_ctx *ctx, const float *X, int64_t n, int d, HostTree &t) {
+    if (fh::build_host_tree(X, n, d, kLeafSize, ctx->host_threads, t) != 0)
+        return fail(ctx, FH_EINVAL, "tree build rejected its input");
+    if (!t.leaves_last_level) return fail(ctx, FH_EUNSUPPORTED, "tree has a leaf above the last level");
+    if (t.levels > 31) return fail(ctx, FH_EUNSUPPORTED, "tree deeper than 31 levels");
+    const int64_t N = t.n_nodes;
+    FH_TRY(ensure(ctx, ctx->xp, (size_t)n * d));
+    FH_TRY(ensure(ctx, ctx->perm, n));
+    FH_TRY(ensure(ctx, ctx->posof, n));
+    FH_TRY(ensure(ctx, ctx->range, N));
+    FH_TRY(ensure(ctx, ctx->box, (size_t)N * 2 * d));
+    std::vector<int2> rng(N);
+    for (int64_t i = 0; i < N; ++i) rng[i] = make_int2(t.start[i], t.end[i]);
+    hipStream_t st = ctx->stream;
+    FH_HIP(hipMemcpyAsync(ctx->xp.p, t.xp.data(), sizeof(float) * n * d, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->perm.p, t.perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->range.p, rng.data(), sizeof(int2) * N, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->box.p, t.box.data(), sizeof(float) * N * 2 * d, hipMemcpyHostToDevice, st));
+    FH_HIP(fh::launch_posof((int)n, ctx->perm.p, ctx->posof.p, st));
+    // The host vectors are released when the caller's HostTree goes out of
+    // scope; make sure the copies are done first.
+    FH_HIP(hipStreamSynchronize(st));
+    return FH_OK;
+}
+
+int knn_block(int k) {
+    if (k <= 32) return 256;
+    if (k <= 64) return 128;
+    return 64;
+}
+
+// kNN of `nq` query rows against the uploaded tree.  Self-query when
+// query_perm != nullptr (queries = tree-order rows).
+int run_knn(fh_ctx *ctx, const HostTree &t, const float *queries, const int *query_perm, int64_t nq, int k,
+            float *out_d, int *out_i, float *core_o, float *core_p, int core_from_knn) {
+    fh::KnnArgs a{};
+    a.xp = ctx->xp.p;
+    a.range = ctx->range.p;
+    a.box = ctx->box.p;
+    a.perm = ctx->perm.p;
+    a.queries = queries;
+    a.query_perm = query_perm;
+    a.nq = nq;
+    a.d = t.d;
+    a.k = k;
+    a.levels = t.levels;
+    a.out_d = out_d;
+    a.out_i = out_i;
+    a.core_o = core_o;
+    a.core_p = core_p;
+    a.core_from_knn = core_from_knn;
+    if (k <= 128) {
+        a.block = knn_block(k);
+    } else {
+        a.block = 256;
+        FH_TRY(ensure(ctx, ctx->heap_p, (size_t)nq * k));
+        FH_TRY(ensure(ctx, ctx->heap_i, (size_t)nq * k));
+        a.heap_p = ctx->heap_p.p;
+        a.heap_i = ctx->heap_i.p;
+        a.heap_stride = nq;
+    }
+    FH_HIP(fh::launch_knn(a, ctx->stream));
+    return FH_OK;
+}
+
+int read_small(fh_ctx *ctx, int count) {
+    FH_HIP(hipMemcpyAsync(ctx->h_small, ctx->small.p, sizeof(int) * count, hipMemcpyDeviceToHost, ctx->stream));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
+// The MST proper.  X_host: the points on the host (tree build input).
+// Outputs are device pointers.
+int run_mst(fh_ctx *ctx, const float *X_host, int64_t n, int d, int ms, float *core_out, int *nbr_out,
+            int *src_out, int *dst_out, float *w_out, fh_stats *stats) {
+    Timer total;
+    hipStream_t st = ctx->stream;
+    fh_stats S{};
+    FH_HIP(hipEventRecord(ctx->ev[0], st));
+
+    Timer tt;
+    HostTree t;
+    FH_TRY(upload_tree(ctx, X_host, n, d, t));
+    S.t_tree_ms = tt.ms();
+    S.n_levels = t.levels;
+    S.n_nodes = t.n_nodes;
+
+    const int ni = (int)n;
+    const int kq = ms > 1 ? ms + 1 : 2;
+    FH_TRY(ensure(ctx, ctx->knn_d, (size_t)n * kq));
+    FH_TRY(ensure(ctx, ctx->knn_i, (size_t)n * kq));
+    FH_TRY(ensure(ctx, ctx->core_o, n));
+    FH_TRY(ensure(ctx, ctx->core_p, n));
+    FH_TRY(ensure(ctx, ctx->comp_p, n));
+    FH_TRY(ensure(ctx, ctx->node_comp, t.n_nodes));
+    FH_TRY(ensure(ctx, ctx->succ, n));
+    FH_TRY(ensure(ctx, ctx->dst_raw, n));
+    FH_TRY(ensure(ctx, ctx->w_init, n));
+    FH_TRY(ensure(ctx, ctx->S0, n));
+    FH_TRY(ensure(ctx, ctx->M0, n));
+    FH_TRY(ensure(ctx, ctx->S1, n));
+    FH_TRY(ensure(ctx, ctx->M1, n));
+    FH_TRY(ensure(ctx, ctx->keep, n));
+    FH_TRY(ensure(ctx, ctx->offs, n));
+    FH_TRY(ensure(ctx, ctx->flag, n));
+    FH_TRY(ensure(ctx, ctx->parent, n));
+    FH_TRY(ensure(ctx, ctx->roots, n));
+    FH_TRY(ensure(ctx, ctx->roots2, n));
+    FH_TRY(ensure(ctx, ctx->rootslot, n));
+    FH_TRY(ensure(ctx, ctx->partner, n));
+    FH_TRY(ensure(ctx, ctx->bound, n));
+    FH_TRY(ensure(ctx, ctx->best, n));
+    FH_TRY(ensure(ctx, ctx->ei, n));
+    FH_TRY(ensure(ctx, ctx->ej, n));
+    FH_TRY(ensure(ctx, ctx->ew, n));
+    FH_TRY(ensure(ctx, ctx->tiles, fh::scan_tiles(ni) + 1));
+    FH_TRY(ensure(ctx, ctx->small, 8));
+
+    // -- core distances
+    FH_HIP(hipEventRecord(ctx->ev[1], st));
+    FH_TRY(run_knn(ctx, t, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p, ctx->core_p.p,
+                   ms > 1 ? 1 : 0));
+    FH_HIP(hipEventRecord(ctx->ev[2], st));
+
+    // -- kNN-seeded forest
+    fh::CycleBufs cb{ctx->S0.p, ctx->M0.p, ctx->S1.p, ctx->M1.p};
+    FH_HIP(fh::launch_init_edges(ni, kq, ctx->knn_i.p, ctx->knn_d.p, ctx->core_o.p, ctx->succ.p, ctx->dst_raw.p,
+                                 ctx->w_init.p, st));
+    FH_HIP(fh::launch_cycle_keep(ni, ctx->succ.p, cb, ctx->keep.p, st));
+    FH_HIP(fh::launch_excl_scan(ni, ctx->keep.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 0, st));
+    FH_HIP(fh::launch_init_emit_union(ni, ctx->keep.p, ctx->offs.p, ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p,
+                                      ctx->parent.p, src_out, dst_out, w_out, st));
+    FH_HIP(fh::launch_labels_and_root_flags(ni, ctx->perm.p, ctx->parent.p, ctx->comp_p.p, ctx->flag.p, st));
+    FH_HIP(fh::launch_excl_scan(ni, ctx->flag.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 1, st));
+    FH_HIP(fh::launch_root_scatter(ni, ctx->flag.p, ctx->offs.p, ctx->roots.p, ctx->rootslot.p, st));
+    FH_TRY(read_small(ctx, 2));
+    int base = ctx->h_small[0];
+    int C = ctx->h_small[1];
+    S.initial_components = C;
+    FH_HIP(hipEventRecord(ctx->ev[3], st));
+
+    // -- Boruvka rounds
+    fh::QueryArgs qa{};
+    qa.xp = ctx->xp.p;
+    qa.range = ctx->range.p;
+    qa.box = ctx->box.p;
+    qa.perm = ctx->perm.p;
+    qa.core_p = ctx->core_p.p;
+    qa.comp_p = ctx->comp_p.p;
+    qa.node_comp = ctx->node_comp.p;
+    qa.bound = ctx->bound.p;
+    qa.best = ctx->best.p;
+    qa.partner = ctx->partner.p;
+    qa.n = n;
+    qa.d = d;
+    qa.levels = t.levels;
+    double query_ms = 0.0;
+    int rounds = 0;
+    while (C > 1) {
+        FH_HIP(fh::launch_round_reset(C, ctx->roots.p, ctx->bound.p, ctx->best.p, st));
+        FH_HIP(fh::launch_purity(t.levels, ctx->range.p, ctx->comp_p.p, ctx->node_comp.p, st));
+        FH_HIP(hipEventRecord(ctx->ev[5], st));
+        FH_HIP(fh::launch_query(qa, st));
+        FH_HIP(hipEventRecord(ctx->ev[6], st));
+        FH_HIP(fh::launch_select(C, ni, ctx->roots.p, ctx->best.p, ctx->partner.p, ctx->posof.p, ctx->perm.p,
+                                 ctx->comp_p.p, ctx->rootslot.p, ctx->succ.p, ctx->ei.p, ctx->ej.p, ctx->ew.p, st));
+        FH_HIP(fh::launch_cycle_keep(C, ctx->succ.p, cb, ctx->keep.p, st));
+        FH_HIP(fh::launch_excl_scan(C, ctx->keep.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 0, st));
+        FH_HIP(fh::launch_round_emit_union(C, base, ctx->keep.p, ctx->offs.p, ctx->ei.p, ctx->ej.p, ctx->ew.p,
+                                           ctx->roots.p, ctx->succ.p, ctx->parent.p, src_out, dst_out, w_out, st));
+        FH_HIP(fh::launch_relabel_and_flags(ni, C, ctx->roots.p, ctx->parent.p, ctx->comp_p.p, ctx->flag.p, st));
+        FH_HIP(fh::launch_excl_scan(C, ctx->flag.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 1, st));
+        FH_HIP(fh::launch_roots_compact(C, ctx->roots.p, ctx->flag.p, ctx->offs.p, ctx->roots2.p, ctx->rootslot.p,
+                                        st));
+        FH_TRY(read_small(ctx, 2));
+        query_ms += event_ms(ctx->ev[5], ctx->ev[6]);
+        const int emitted = ctx->h_small[0];
+        const int C2 = ctx->h_small[1];
+        if (emitted != C - C2 || C2 >= C)
+            return fail(ctx, FH_EDEVICE,
+                        "Boruvka round made no consistent progress (components " + std::to_string(C) + " -> " +
+                            std::to_string(C2) + ", edges " + std::to_string(emitted) + ")");
+        base += emitted;
+        C = C2;
+        std::swap(ctx->roots, ctx->roots2);
+        ++rounds;
+    }
+    FH_HIP(hipEventRecord(ctx->ev[4], st));
+
+    // -- outputs
+    if (core_out) FH_HIP(hipMemcpyAsync(core_out, ctx->core_o.p, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    if (nbr_out) FH_HIP(fh::launch_neighbors(n, kq, ctx->knn_i.p, nbr_out, st));
+    FH_HIP(hipEventRecord(ctx->ev[7], st));
+    FH_HIP(hipEventSynchronize(ctx->ev[7]));
+
+    S.rounds = rounds;
+    S.n_edges = base;
+    S.t_knn_ms = event_ms(ctx->ev[1], ctx->ev[2]);
+    S.t_init_ms = event_ms(ctx->ev[2], ctx->ev[3]);
+    S.t_rounds_ms = event_ms(ctx->ev[3], ctx->ev[4]);
+    S.t_query_ms = query_ms;
+    S.t_total_ms = total.ms();
+    if (base != n - 1)
+        return fail(ctx, FH_EDEVICE, "spanning tree has " + std::to_string(base) + " edges for " +
+                                         std::to_string(n) + " points");
+    if (stats) *stats = S;
+    return FH_OK;
+}
+
+int check_args(fh_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t ms) {
+    if (!ctx) return FH_EINVAL;
+    if (!X) return fail(ctx, FH_EINVAL, "X is null");
+    if (n < 1) return fail(ctx, FH_EINVAL, "n must be >= 1");
+    if (n > (int64_t(1) << 30)) return fail(ctx, FH_EUNSUPPORTED, "n above 2^30");
+    if (d < 1 || d > FH_MAX_DIM) return fail(ctx, FH_EUNSUPPORTED, "d outside [1, FH_MAX_DIM]");
+    if (ms < 1) return fail(ctx, FH_EINVAL, "min_samples must be >= 1");
+    return FH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fh_version(void) { return "fh 0.1.0 gfx950"; }
+
+fh_ctx *fh_create(int device) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return nullptr;
+    fh_ctx *c = new fh_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    for (auto &e : c->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            fh_destroy(c);
+            return nullptr;
+        }
+    }
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->h_small), 64 * sizeof(int)) != hipSuccess) {
+        c->h_small = nullptr;
+        fh_destroy(c);
+        return nullptr;
+    }
+    return c;
+}
+
+void fh_destroy(fh_ctx *c) {
+    if (!c) return;
+    hipError_t e = hipSetDevice(c->device);
+    (void)e;
+    if (c->stream) e = hipStreamSynchronize(c->stream);
+    release(c->xp); release(c->box); release(c->core_p); release(c->core_o); release(c->knn_d);
+    release(c->w_init); release(c->bound); release(c->ew); release(c->heap_p); release(c->queries);
+    release(c->out_w); release(c->out_core);
+    release(c->perm); release(c->posof); release(c->comp_p); release(c->node_comp); release(c->knn_i);
+    release(c->succ); release(c->dst_raw); release(c->S0); release(c->M0); release(c->S1); release(c->M1);
+    release(c->keep); release(c->offs); release(c->parent); release(c->roots); release(c->roots2);
+    release(c->rootslot); release(c->partner); release(c->ei); release(c->ej); release(c->tiles);
+    release(c->small); release(c->heap_i); release(c->flag); release(c->out_src); release(c->out_dst);
+    release(c->out_nbr); release(c->range); release(c->best);
+    if (c->h_small) e = hipHostFree(c->h_small);
+    for (auto &ev : c->ev)
+        if (ev) e = hipEventDestroy(ev);
+    if (c->stream) e = hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *fh_last_error(const fh_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int64_t fh_tree_nnodes(int64_t n) {
+    if (n < 1) return 0;
+    return (int64_t(1) << fh::tree_levels(n, kLeafSize)) - 1;
+}
+
+int fh_mst_euclidean_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, float *core_sq,
+                            int32_t *nbr, int32_t *src, int32_t *dst, float *w_sq, fh_stats *stats) {
+    FH_TRY(check_args(ctx, X, n, d, min_samples));
+    if (n > 1 && (!src || !dst || !w_sq)) return fail(ctx, FH_EINVAL, "edge outputs are null");
+    FH_HIP(hipSetDevice(ctx->device));
+    // The tree is built on the host: bring the points over once.
+    std::vector<float> host((size_t)n * d);
+    FH_HIP(hipMemcpyAsync(host.data(), X, sizeof(float) * n * d, hipMemcpyDeviceToHost, ctx->stream));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    return run_mst(ctx, host.data(), n, d, min_samples, core_sq, nbr, src, dst, w_sq, stats);
+}
+
+int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, float *core_sq,
+                     int32_t *nbr, int32_t *src, int32_t *dst, float *w_sq, fh_stats *stats) {
+    FH_TRY(check_args(ctx, X, n, d, min_samples));
+    if (n > 1 && (!src || !dst || !w_sq)) return fail(ctx, FH_EINVAL, "edge outputs are null");
+    FH_HIP(hipSetDevice(ctx->device));
+    const int kn = min_samples > 1 ? min_samples : 1;
+    const int64_t m = n - 1;
+    FH_TRY(ensure(ctx, ctx->out_core, n));
+    FH_TRY(ensure(ctx, ctx->out_nbr, (size_t)n * kn));
+    FH_TRY(ensure(ctx, ctx->out_src, m));
+    FH_TRY(ensure(ctx, ctx->out_dst, m));
+    FH_TRY(ensure(ctx, ctx->out_w, m));
+    FH_TRY(run_mst(ctx, X, n, d, min_samples, ctx->out_core.p, ctx->out_nbr.p, ctx->out_src.p, ctx->out_dst.p,
+                   ctx->out_w.p, stats));
+    hipStream_t st = ctx->stream;
+    if (core_sq) FH_HIP(hipMemcpyAsync(core_sq, ctx->out_core.p, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    if (nbr) FH_HIP(hipMemcpyAsync(nbr, ctx->out_nbr.p, sizeof(int) * n * kn, hipMemcpyDeviceToHost, st));
+    if (m > 0) {
+        FH_HIP(hipMemcpyAsync(src, ctx->out_src.p, sizeof(int) * m, hipMemcpyDeviceToHost, st));
+        FH_HIP(hipMemcpyAsync(dst, ctx->out_dst.p, sizeof(int) * m, hipMemcpyDeviceToHost, st));
+        FH_HIP(hipMemcpyAsync(w_sq, ctx->out_w.p, sizeof(float) * m, hipMemcpyDeviceToHost, st));
+    }
+    FH_HIP(hipStreamSynchronize(st));
+    return FH_OK;
+}
+
+int fh_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, const float *Q, int64_t nq, int32_t k, float *dist_sq,
+           int32_t *idx) {
+    FH_TRY(check_args(ctx, X, n, d, 1));
+    if (!Q || !dist_sq || !idx || nq < 0 || k < 1) return fail(ctx, FH_EINVAL, "bad query arguments");
+    if (nq == 0) return FH_OK;
+    FH_HIP(hipSetDevice(ctx->device));
+    HostTree t;
+    FH_TRY(upload_tree(ctx, X, n, d, t));
+    FH_TRY(ensure(ctx, ctx->queries, (size_t)nq * d));
+    FH_TRY(ensure(ctx, ctx->knn_d, (size_t)nq * k));
+    FH_TRY(ensure(ctx, ctx->knn_i, (size_t)nq * k));
+    hipStream_t st = ctx->stream;
+    FH_HIP(hipMemcpyAsync(ctx->queries.p, Q, sizeof(float) * nq * d, hipMemcpyHostToDevice, st));
+    FH_TRY(run_knn(ctx, t, ctx->queries.p, nullptr, nq, k, ctx->knn_d.p, ctx->knn_i.p, nullptr, nullptr, 0));
+    FH_HIP(hipMemcpyAsync(dist_sq, ctx->knn_d.p, sizeof(float) * nq * k, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(idx, ctx->knn_i.p, sizeof(int) * nq * k, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    return FH_OK;
+}
+
+int fh_build_kdtree(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int64_t *idx_array, int64_t *idx_start,
+                    int64_t *idx_end, float *radius, uint8_t *is_leaf, float *node_bounds) {
+    // Host-only: ctx may be NULL (then no error message is kept).
+    if (!X || !idx_array || !idx_start || !idx_end || !radius || !is_leaf || !node_bounds || n < 1 || d < 1 ||
+        n > (int64_t(1) << 30))
+        return ctx ? fail(ctx, FH_EINVAL, "bad fh_build_kdtree arguments") : FH_EINVAL;
+    HostTree t;
+    if (fh::build_host_tree(X, n, d, kLeafSize, ctx ? ctx->host_threads : 0, t) != 0)
+        return ctx ? fail(ctx, FH_EINVAL, "tree build rejected its input") : FH_EINVAL;
+    const int64_t N = t.n_nodes;
+    for (int64_t i = 0; i < n; ++i) idx_array[i] = t.perm[i];
+    for (int64_t i = 0; i < N; ++i) {
+        idx_start[i] = t.start[i];
+        idx_end[i] = t.end[i];
+        radius[i] = t.radius[i];
+        is_leaf[i] = t.leaf[i];
+        for (int j = 0; j < d; ++j) {
+            node_bounds[i * d + j] = t.box[(size_t)i * 2 * d + j];
+            node_bounds[(N + i) * d + j] = t.box[(size_t)i * 2 * d + d + j];
+        }
+    }
+    return FH_OK;
+}
+
+}  // extern "C"

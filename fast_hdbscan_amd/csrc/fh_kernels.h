@@ -1,0 +1,82 @@
+// Launch interface of the gfx950 kernels (fh_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fh {
+
+struct KnnArgs {
+    const float *xp;        // tree-order rows
+    const int2 *range;      // per node (start, end)
+    const float *box;       // per node min[d], max[d]
+    const int *perm;        // tree position -> original index
+    const float *queries;   // nq rows
+    const int *query_perm;  // self-query: original index of each query row; else nullptr
+    int64_t nq;
+    int d;
+    int k;
+    int levels;
+    int block;
+    float *out_d;           // [rows, k] squared distances, ascending
+    int *out_i;             // [rows, k] original indices
+    float *core_o;          // optional: core by original index
+    float *core_p;          // optional: core by query (tree) position
+    int core_from_knn;      // 1: core = last column; 0: zeros (min_samples == 1)
+    float *heap_p;          // global heap slices for large k (nullptr: LDS)
+    int *heap_i;
+    int64_t heap_stride;
+};
+
+struct QueryArgs {
+    const float *xp;
+    const int2 *range;
+    const float *box;
+    const int *perm;
+    const float *core_p;
+    const int *comp_p;
+    const int *node_comp;
+    float *bound;                // per component label
+    unsigned long long *best;    // per component label
+    int *partner;                // per tree position
+    int64_t n;
+    int d;
+    int levels;
+};
+
+struct CycleBufs {
+    int *S0, *M0, *S1, *M1;
+};
+
+hipError_t launch_knn(const KnnArgs &a, hipStream_t st);
+hipError_t launch_query(const QueryArgs &a, hipStream_t st);
+hipError_t launch_posof(int n, const int *perm, int *posof, hipStream_t st);
+hipError_t launch_neighbors(int64_t n, int k, const int *knn_i, int *nbr, hipStream_t st);
+
+int scan_tiles(int m);
+hipError_t launch_excl_scan(int m, const int *in, int *out, int *tile_sum, int *total, hipStream_t st);
+hipError_t launch_cycle_keep(int m, const int *succ, CycleBufs &b, int *keep, hipStream_t st);
+
+hipError_t launch_init_edges(int n, int k, const int *knn_i, const float *knn_d, const float *core_o, int *succ,
+                             int *dst_raw, float *w, hipStream_t st);
+hipError_t launch_init_emit_union(int n, const int *keep, const int *offs, const int *succ, const int *dst_raw,
+                                  const float *w, int *parent, int *src_out, int *dst_out, float *w_out,
+                                  hipStream_t st);
+hipError_t launch_labels_and_root_flags(int n, const int *perm, const int *parent, int *comp_p, int *flag,
+                                        hipStream_t st);
+hipError_t launch_root_scatter(int n, const int *flag, const int *offs, int *roots, int *rootslot, hipStream_t st);
+
+hipError_t launch_round_reset(int c, const int *roots, float *bound, unsigned long long *best, hipStream_t st);
+hipError_t launch_purity(int levels, const int2 *range, const int *comp_p, int *node_comp, hipStream_t st);
+hipError_t launch_select(int c, int n, const int *roots, const unsigned long long *best, const int *partner,
+                         const int *posof, const int *perm, const int *comp_p, const int *rootslot, int *succ,
+                         int *ei, int *ej, float *ew, hipStream_t st);
+hipError_t launch_round_emit_union(int c, int base, const int *keep, const int *offs, const int *ei, const int *ej,
+                                   const float *ew, const int *roots, const int *succ, int *parent, int *src_out,
+                                   int *dst_out, float *w_out, hipStream_t st);
+hipError_t launch_relabel_and_flags(int n, int c, const int *roots, const int *parent, int *comp_p, int *flag,
+                                    hipStream_t st);
+hipError_t launch_roots_compact(int c, const int *roots, const int *flag, const int *offs, int *roots_out,
+                                int *rootslot, hipStream_t st);
+
+}  // namespace fh

@@ -1,0 +1,796 @@
+// gfx950 kernels of the Euclidean mutual-reachability MST.
+//
+// Data layout in HBM (one tree per call):
+//   xp[n*d]        points in KD-tree order (row-major); a leaf is a
+//                  contiguous run of rows, so one wavefront whose lanes sit
+//                  in the same leaf reads the same cache lines
+//   perm[n]        tree position -> original index   (numba idx_array)
+//   posof[n]       original index -> tree position
+//   range[N]       (idx_start, idx_end) per node
+//   box[N*2d]      per node min[d] then max[d]       (numba node_bounds)
+//   core_p[n]      squared core distance by tree position
+//   comp_p[n]      component label by tree position; the label of a
+//                  component is its lowest original point index
+//   node_comp[N]   label of a node whose points are all in one component,
+//                  else -1                          (boruvka.py:98-124)
+//   bound[n], best[n]  per-label pruning bound and packed best edge
+//
+// All leaves of the implicit tree are on its last level (checked on the
+// host), so "leaf" == "depth == levels-1" inside the walks.
+#include "fh_kernels.h"
+
+#include "fh_device.h"
+
+namespace fh {
+
+namespace {
+
+constexpr float kInf = __builtin_huge_valf();
+
+// Depth-first walk of the implicit tree without a stack, visiting children
+// in the reference's order: the child with the smaller box lower bound
+// first, left on ties (numba_kdtree.py:747-760, boruvka.py:228-285).  A bit
+// per depth remembers which child went first; on the way back the sibling's
+// bound is recomputed with the same arithmetic, so the walk makes exactly the
+// reference recursion's sequence of enter/prune decisions.
+template <int D, class V>
+__device__ __forceinline__ void walk(const float *__restrict__ box, int levels, int d, V &v) {
+    const float *x = v.x.p();
+    const int stride = 2 * (D > 0 ? D : d);
+    int node = 0, depth = 0;
+    uint32_t first_right = 0;
+    float lb = box_lb<D>(x, box, d);
+    bool entering = true;
+    for (;;) {
+        if (entering && v.enter(node, lb)) {
+            if (depth == levels - 1) {
+                v.leaf(node);
+            } else {
+                const int l = 2 * node + 1;
+                const float lbl = box_lb<D>(x, box + (size_t)l * stride, d);
+                const float lbr = box_lb<D>(x, box + (size_t)(l + 1) * stride, d);
+                const bool rf = !(lbl <= lbr);
+                ++depth;
+                first_right = rf ? (first_right | (1u << depth)) : (first_right & ~(1u << depth));
+                node = rf ? l + 1 : l;
+                lb = rf ? lbr : lbl;
+                continue;
+            }
+        }
+        if (depth == 0) break;
+        const bool rf = (first_right >> depth) & 1u;
+        const bool is_right = (node & 1) == 0;
+        if (is_right == rf) {
+            node = is_right ? node - 1 : node + 1;
+            lb = box_lb<D>(x, box + (size_t)node * stride, d);
+            entering = true;
+        } else {
+            node = (node - 1) >> 1;
+            --depth;
+            entering = false;
+        }
+    }
+}
+
+// The query point: held in registers for a compile-time dimension, read
+// through a pointer otherwise.
+template <int D>
+struct QueryPoint {
+    float v[D];
+    __device__ __forceinline__ void load(const float *row, int) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) v[j] = row[j];
+    }
+    __device__ __forceinline__ const float *p() const { return v; }
+};
+
+template <>
+struct QueryPoint<0> {
+    const float *ptr;
+    __device__ __forceinline__ void load(const float *row, int) { ptr = row; }
+    __device__ __forceinline__ const float *p() const { return ptr; }
+};
+
+// ---------------------------------------------------------------- kNN ----
+
+template <int D>
+struct KnnVisitor {
+    QueryPoint<D> x;
+    const float *__restrict__ xp;
+    const int2 *__restrict__ range;
+    LdsHeap h;
+    float hmax;
+    int d;
+
+    __device__ __forceinline__ bool enter(int, float lb) const { return !(lb > hmax); }
+
+    __device__ __forceinline__ void leaf(int node) {
+        const int2 r = range[node];
+        const int dd = D > 0 ? D : d;
+        for (int i = r.x; i < r.y; ++i) {
+            const float dist = rdist<D>(x.p(), xp + (size_t)i * dd, d);
+            if (dist < hmax) {
+                h.push(dist, i);
+                hmax = h.P(0);
+            }
+        }
+    }
+};
+
+// One lane per query row (queries in tree order for the self-query so a
+// wavefront's lanes walk nearby paths).  Heap in LDS (use_lds) or in a
+// global scratch slice for large k.
+template <int D>
+__global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
+    extern __shared__ unsigned char smem_raw[];
+    const int tid = threadIdx.x;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + tid;
+    if (q >= a.nq) return;
+    const int k = a.k;
+    LdsHeap h;
+    if (a.heap_p == nullptr) {
+        float *sp = reinterpret_cast<float *>(smem_raw);
+        h.p = sp + tid;
+        h.i = reinterpret_cast<int *>(sp + (size_t)k * blockDim.x) + tid;
+        h.stride = blockDim.x;
+    } else {
+        h.p = a.heap_p + q;
+        h.i = a.heap_i + q;
+        h.stride = (int)a.heap_stride;
+    }
+    h.k = k;
+    h.init();
+
+    const int dd = D > 0 ? D : a.d;
+    KnnVisitor<D> v;
+    v.x.load(a.queries + (size_t)q * dd, a.d);
+    v.xp = a.xp;
+    v.range = a.range;
+    v.h = h;
+    v.hmax = kInf;
+    v.d = a.d;
+    walk<D>(a.box, a.levels, a.d, v);
+    h.sort();
+
+    // Output row: the original index of the query for the self-query, the
+    // query's own row otherwise.
+    const int64_t row = a.query_perm ? a.query_perm[q] : q;
+    for (int j = 0; j < k; ++j) {
+        const int pj = h.I(j);
+        a.out_d[row * k + j] = h.P(j);
+        a.out_i[row * k + j] = pj >= 0 ? a.perm[pj] : -1;
+    }
+    if (a.core_o != nullptr) {
+        const float c = a.core_from_knn ? h.P(k - 1) : 0.0f;
+        a.core_o[row] = c;
+        a.core_p[q] = c;
+    }
+}
+
+// ----------------------------------------------------- Boruvka query ----
+
+template <int D>
+struct BoruvkaVisitor {
+    QueryPoint<D> x;
+    const float *__restrict__ xp;
+    const int2 *__restrict__ range;
+    const float *__restrict__ core_p;
+    const int *__restrict__ comp_p;
+    const int *__restrict__ node_comp;
+    float *bound_c;
+    float core_i;
+    int comp;
+    int d;
+    float best;
+    int best_j;
+    float B;
+
+    // boruvka.py:176-192: prune on the point's own best, on the component
+    // bound (strict comparisons) and on single-component nodes.
+    __device__ __forceinline__ bool enter(int node, float lb) {
+        if (lb > best) return false;
+        B = ld_relaxed(bound_c);
+        if (lb > B || core_i > B) return false;
+        if (node_comp[node] == comp) return false;
+        return true;
+    }
+
+    // boruvka.py:196-212.  The candidate filter uses core_j <= B (the
+    // reference's sequential run has core_j < B because the bound it sees is
+    // still above the component minimum when the winning point runs; with
+    // concurrent lanes the bound may already equal it).
+    __device__ __forceinline__ void leaf(int node) {
+        const int2 r = range[node];
+        const int dd = D > 0 ? D : d;
+        for (int i = r.x; i < r.y; ++i) {
+            if (comp_p[i] == comp) continue;
+            const float cj = core_p[i];
+            if (!(cj <= B)) continue;
+            float dist = rdist<D>(x.p(), xp + (size_t)i * dd, d);
+            dist = dist > core_i ? dist : core_i;
+            dist = dist > cj ? dist : cj;
+            if (dist < best) {
+                best = dist;
+                best_j = i;
+                if (dist < B) {
+                    atomicMin(reinterpret_cast<unsigned int *>(bound_c), __float_as_uint(dist));
+                    B = dist;
+                }
+            }
+        }
+    }
+};
+
+template <int D>
+__global__ __launch_bounds__(256) void boruvka_query_kernel(QueryArgs a) {
+    const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= a.n) return;
+    const int dd = D > 0 ? D : a.d;
+    const int comp = a.comp_p[pos];
+    BoruvkaVisitor<D> v;
+    v.x.load(a.xp + (size_t)pos * dd, a.d);
+    v.xp = a.xp;
+    v.range = a.range;
+    v.core_p = a.core_p;
+    v.comp_p = a.comp_p;
+    v.node_comp = a.node_comp;
+    v.bound_c = a.bound + comp;
+    v.core_i = a.core_p[pos];
+    v.comp = comp;
+    v.d = a.d;
+    v.best = kInf;
+    v.best_j = -1;
+    v.B = kInf;
+    walk<D>(a.box, a.levels, a.d, v);
+    if (v.best_j >= 0) {
+        // Positive floats order like their bit patterns: the packed key
+        // gives the smallest weight, then the lowest original point index
+        // (merge_components keeps the first index at the minimum,
+        // boruvka.py:43-59).
+        const unsigned long long key =
+            ((unsigned long long)__float_as_uint(v.best) << 32) | (unsigned int)a.perm[pos];
+        atomicMin(a.best + comp, key);
+        a.partner[pos] = v.best_j;
+    }
+}
+
+// ------------------------------------------------- kNN-seeded forest ----
+
+// boruvka.py:492-503: the first neighbour column j >= 1 whose core distance
+// does not exceed the point's own.  Negative neighbour indices (rows with
+// fewer real neighbours than k) read core[n-1], as numba's wrap-around
+// indexing does.
+__global__ void init_edges_kernel(int n, int k, const int *__restrict__ knn_i, const float *__restrict__ knn_d,
+                                  const float *__restrict__ core_o, int *succ, int *dst_raw, float *w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float ci = core_o[i];
+    int s = -1, draw = -1;
+    float wi = 0.0f;
+    for (int j = 1; j < k; ++j) {
+        const int kk = knn_i[(size_t)i * k + j];
+        const int kw = kk < 0 ? kk + n : kk;
+        if (ci >= core_o[kw]) {
+            const float dj = knn_d[(size_t)i * k + j];
+            s = kw;
+            draw = kk;
+            wi = ci > dj ? ci : dj;
+            break;
+        }
+    }
+    succ[i] = s;
+    dst_raw[i] = draw;
+    w[i] = wi;
+}
+
+// ----------------------------------------------- cycles by doubling ----
+//
+// In a functional graph (one out-edge per node) every weakly connected
+// piece holds at most one cycle.  Processing edges in key order with a
+// union-find (boruvka.py:509-522, 64-76) skips exactly the cycle edge
+// processed last: the one whose source has the largest key on the cycle.
+// Pointer doubling finds it: after K steps with 2^K >= m, S(v) lies on v's
+// cycle and M(S(v)) is that cycle's largest key.
+
+__global__ void dbl_init_kernel(int m, const int *__restrict__ succ, int *S, int *M) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m) return;
+    S[v] = succ[v];
+    M[v] = v;
+}
+
+__global__ void dbl_step_kernel(int m, const int *__restrict__ S, const int *__restrict__ M, int *S2, int *M2) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m) return;
+    const int s = S[v];
+    const int mv = M[v];
+    if (s < 0) {
+        S2[v] = -1;
+        M2[v] = mv;
+    } else {
+        S2[v] = S[s];
+        const int ms = M[s];
+        M2[v] = mv > ms ? mv : ms;
+    }
+}
+
+// keep[v] = node has an edge and it is not its cycle's skipped edge.
+__global__ void dbl_keep_kernel(int m, const int *__restrict__ succ, const int *__restrict__ S,
+                                const int *__restrict__ M, int *keep) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m) return;
+    const int s = S[v];
+    const bool skip = s >= 0 && M[s] == v;
+    keep[v] = (succ[v] >= 0 && !skip) ? 1 : 0;
+}
+
+// ------------------------------------------------------------ scans ----
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread over the block; returns the block
+// total in `total`.
+__device__ __forceinline__ int block_excl_scan(int v, int &total) {
+    __shared__ int wsum[kScanBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int inc = wave_incl_scan(v);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        const int s = wsum[w];
+        off += (w < wave) ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_tile_kernel(int m, const int *__restrict__ in, int *out,
+                                                                int *tile_sum) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = (base + j < m) ? in[base + j] : 0;
+        s += v[j];
+    }
+    int total;
+    int off = block_excl_scan(s, total);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < m) out[base + j] = off;
+        off += v[j];
+    }
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_sums_kernel(int tiles, int *tile_sum, int *total_out) {
+    int carry = 0;
+    for (int b0 = 0; b0 < tiles; b0 += kScanBlock) {
+        const int t = b0 + threadIdx.x;
+        const int v = t < tiles ? tile_sum[t] : 0;
+        int tot;
+        const int ex = block_excl_scan(v, tot);
+        if (t < tiles) tile_sum[t] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_add_kernel(int m, int *out, const int *__restrict__ tile_sum) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const int add = tile_sum[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+        if (base + j < m) out[base + j] += add;
+}
+
+// ------------------------------------------------------- union-find ----
+//
+// Lock-free union-find over component labels: roots are linked larger
+// under smaller with a CAS, so every root is the lowest original point index
+// of its set (the label convention above).
+
+__device__ __forceinline__ int uf_find(int *parent, int x) {
+    for (;;) {
+        const int p = ld_relaxed(parent + x);
+        if (p == x) return x;
+        x = p;
+    }
+}
+
+__device__ __forceinline__ void uf_unite(int *parent, int a, int b) {
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(parent + a, a, b) == a) return;
+    }
+}
+
+__global__ void iota_kernel(int m, int *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] = i;
+}
+
+// Initial forest: emit kept edges in point order, then union them.
+__global__ void init_emit_kernel(int n, const int *__restrict__ keep, const int *__restrict__ offs,
+                                 const int *__restrict__ dst_raw, const float *__restrict__ w, int *src_out,
+                                 int *dst_out, float *w_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    const int r = offs[i];
+    src_out[r] = i;
+    dst_out[r] = dst_raw[i];
+    w_out[r] = w[i];
+}
+
+__global__ void init_union_kernel(int n, const int *__restrict__ keep, const int *__restrict__ succ, int *parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    uf_unite(parent, i, succ[i]);
+}
+
+__global__ void compress_all_kernel(int n, int *parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    parent[i] = uf_find(parent, i);
+}
+
+__global__ void label_points_kernel(int n, const int *__restrict__ perm, const int *__restrict__ parent,
+                                    int *comp_p) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    comp_p[p] = parent[perm[p]];
+}
+
+__global__ void root_flags_kernel(int n, const int *__restrict__ parent, int *flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = parent[i] == i ? 1 : 0;
+}
+
+__global__ void root_scatter_kernel(int n, const int *__restrict__ flag, const int *__restrict__ offs, int *roots,
+                                    int *rootslot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    roots[offs[i]] = i;
+    rootslot[i] = offs[i];
+}
+
+// --------------------------------------------------- Boruvka rounds ----
+
+__global__ void round_reset_kernel(int c, const int *__restrict__ roots, float *bound,
+                                   unsigned long long *best) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c) return;
+    const int r = roots[s];
+    bound[r] = kInf;
+    best[r] = ~0ull;
+}
+
+// Node purity, boruvka.py:98-124.  A block turns 256 nodes of one level into
+// `steps` levels above them.  At the leaf level the values come from the
+// points; otherwise from node_comp of the level below.
+__global__ __launch_bounds__(256) void purity_kernel(int level, int steps, int leaf_level,
+                                                     const int2 *__restrict__ range, const int *__restrict__ comp_p,
+                                                     int *node_comp) {
+    __shared__ int sv[256];
+    const int count = 1 << level;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    int v = -1;
+    if (idx < count) {
+        const int node = count - 1 + idx;
+        if (leaf_level) {
+            const int2 r = range[node];
+            const int c0 = comp_p[r.x];
+            bool pure = true;
+            for (int i = r.x + 1; i < r.y; ++i)
+                if (comp_p[i] != c0) {
+                    pure = false;
+                    break;
+                }
+            v = pure ? c0 : -1;
+            node_comp[node] = v;
+        } else {
+            v = node_comp[node];
+        }
+    }
+    sv[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 1; h <= steps; ++h) {
+        const int lvl = level - h;
+        const int cnt = 1 << lvl;
+        const int width = 256 >> h;
+        int nv = -1;
+        const bool act = threadIdx.x < width;
+        const int gidx = (blockIdx.x * 256 >> h) + threadIdx.x;
+        if (act) {
+            const int a = sv[2 * threadIdx.x], b = sv[2 * threadIdx.x + 1];
+            nv = (a == b) ? a : -1;
+        }
+        __syncthreads();
+        if (act) {
+            sv[threadIdx.x] = nv;
+            if (gidx < cnt) node_comp[cnt - 1 + gidx] = nv;
+        }
+        __syncthreads();
+    }
+}
+
+// Per component: decode the packed best (weight, lowest point), look up the
+// partner, map the target to its slot.  A component without any candidate
+// (only when every core distance is +inf, i.e. min_samples >= n) keeps the
+// reference's entry (lowest point, -1, inf) whose target resolves like
+// ds_find(-1) = the component of point n-1.
+__global__ void select_kernel(int c, int n, const int *__restrict__ roots, const unsigned long long *__restrict__ best,
+                              const int *__restrict__ partner, const int *__restrict__ posof,
+                              const int *__restrict__ perm, const int *__restrict__ comp_p,
+                              const int *__restrict__ rootslot, int *succ, int *ei, int *ej, float *ew) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c) return;
+    const int r = roots[s];
+    const unsigned long long key = best[r];
+    int i_star, j_star, tgt;
+    float w;
+    if (key == ~0ull) {
+        i_star = r;
+        j_star = -1;
+        w = kInf;
+        tgt = comp_p[posof[n - 1]];
+    } else {
+        i_star = (int)(unsigned int)(key & 0xffffffffull);
+        w = __uint_as_float((unsigned int)(key >> 32));
+        const int pj = partner[posof[i_star]];
+        j_star = perm[pj];
+        tgt = comp_p[pj];
+    }
+    succ[s] = rootslot[tgt];
+    ei[s] = i_star;
+    ej[s] = j_star;
+    ew[s] = w;
+}
+
+__global__ void round_emit_kernel(int c, int base, const int *__restrict__ keep, const int *__restrict__ offs,
+                                  const int *__restrict__ ei, const int *__restrict__ ej,
+                                  const float *__restrict__ ew, int *src_out, int *dst_out, float *w_out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c || !keep[s]) return;
+    const int r = base + offs[s];
+    src_out[r] = ei[s];
+    dst_out[r] = ej[s];
+    w_out[r] = ew[s];
+}
+
+__global__ void round_union_kernel(int c, const int *__restrict__ keep, const int *__restrict__ roots,
+                                   const int *__restrict__ succ, int *parent) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c || !keep[s]) return;
+    uf_unite(parent, roots[s], roots[succ[s]]);
+}
+
+__global__ void compress_roots_kernel(int c, const int *__restrict__ roots, int *parent) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c) return;
+    const int r = roots[s];
+    parent[r] = uf_find(parent, r);
+}
+
+__global__ void relabel_kernel(int n, const int *__restrict__ parent, int *comp_p) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    comp_p[p] = parent[comp_p[p]];
+}
+
+__global__ void still_root_kernel(int c, const int *__restrict__ roots, const int *__restrict__ parent, int *flag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c) return;
+    const int r = roots[s];
+    flag[s] = parent[r] == r ? 1 : 0;
+}
+
+__global__ void roots_compact_kernel(int c, const int *__restrict__ roots, const int *__restrict__ flag,
+                                     const int *__restrict__ offs, int *roots_out, int *rootslot) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c || !flag[s]) return;
+    const int r = roots[s];
+    roots_out[offs[s]] = r;
+    rootslot[r] = offs[s];
+}
+
+__global__ void posof_kernel(int n, const int *__restrict__ perm, int *posof) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    posof[perm[p]] = p;
+}
+
+// Columns 1..k-1 of the sorted kNN rows (boruvka.py:647 neighbors[:, 1:]).
+__global__ void neighbors_kernel(int64_t n, int k, const int *__restrict__ knn_i, int *nbr) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int kn = k - 1;
+    if (t >= n * kn) return;
+    const int64_t row = t / kn;
+    const int col = (int)(t - row * kn);
+    nbr[t] = knn_i[row * k + 1 + col];
+}
+
+inline unsigned blocks(int64_t m, int b) { return (unsigned)((m + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------ launch ----
+
+template <int D>
+static hipError_t launch_knn_t(const KnnArgs &a, hipStream_t st) {
+    const int block = a.block;
+    const size_t shmem = a.heap_p ? 0 : (size_t)a.k * block * (sizeof(float) + sizeof(int));
+    hipLaunchKernelGGL(knn_kernel<D>, dim3(blocks(a.nq, block)), dim3(block), shmem, st, a);
+    return hipGetLastError();
+}
+
+template <int D>
+static hipError_t launch_query_t(const QueryArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(boruvka_query_kernel<D>, dim3(blocks(a.n, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+#define FH_DISPATCH_D(d, fn, ...)                 \
+    switch (d) {                                  \
+        case 1: return fn<1>(__VA_ARGS__);        \
+        case 2: return fn<2>(__VA_ARGS__);        \
+        case 3: return fn<3>(__VA_ARGS__);        \
+        case 4: return fn<4>(__VA_ARGS__);        \
+        case 5: return fn<5>(__VA_ARGS__);        \
+        case 6: return fn<6>(__VA_ARGS__);        \
+        case 7: return fn<7>(__VA_ARGS__);        \
+        case 8: return fn<8>(__VA_ARGS__);        \
+        default: return fn<0>(__VA_ARGS__);       \
+    }
+
+hipError_t launch_knn(const KnnArgs &a, hipStream_t st) { FH_DISPATCH_D(a.d, launch_knn_t, a, st) }
+
+hipError_t launch_query(const QueryArgs &a, hipStream_t st) { FH_DISPATCH_D(a.d, launch_query_t, a, st) }
+
+hipError_t launch_posof(int n, const int *perm, int *posof, hipStream_t st) {
+    hipLaunchKernelGGL(posof_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, perm, posof);
+    return hipGetLastError();
+}
+
+hipError_t launch_neighbors(int64_t n, int k, const int *knn_i, int *nbr, hipStream_t st) {
+    if (k <= 1 || n == 0) return hipSuccess;
+    hipLaunchKernelGGL(neighbors_kernel, dim3(blocks(n * (k - 1), 256)), dim3(256), 0, st, n, k, knn_i, nbr);
+    return hipGetLastError();
+}
+
+int scan_tiles(int m) { return (m + kScanTile - 1) / kScanTile; }
+
+hipError_t launch_excl_scan(int m, const int *in, int *out, int *tile_sum, int *total, hipStream_t st) {
+    const int tiles = scan_tiles(m);
+    if (m == 0) {
+        return hipMemsetAsync(total, 0, sizeof(int), st);
+    }
+    hipLaunchKernelGGL(scan_tile_kernel, dim3(tiles), dim3(kScanBlock), 0, st, m, in, out, tile_sum);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, st, tiles, tile_sum, total);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(tiles), dim3(kScanBlock), 0, st, m, out, tile_sum);
+    return hipGetLastError();
+}
+
+hipError_t launch_cycle_keep(int m, const int *succ, CycleBufs &b, int *keep, hipStream_t st) {
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(dbl_init_kernel, dim3(blocks(m, 256)), dim3(256), 0, st, m, succ, b.S0, b.M0);
+    int steps = 0;
+    while ((int64_t(1) << steps) < m) ++steps;
+    int *S = b.S0, *M = b.M0, *S2 = b.S1, *M2 = b.M1;
+    for (int t = 0; t < steps; ++t) {
+        hipLaunchKernelGGL(dbl_step_kernel, dim3(blocks(m, 256)), dim3(256), 0, st, m, S, M, S2, M2);
+        std::swap(S, S2);
+        std::swap(M, M2);
+    }
+    hipLaunchKernelGGL(dbl_keep_kernel, dim3(blocks(m, 256)), dim3(256), 0, st, m, succ, S, M, keep);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_edges(int n, int k, const int *knn_i, const float *knn_d, const float *core_o, int *succ,
+                             int *dst_raw, float *w, hipStream_t st) {
+    hipLaunchKernelGGL(init_edges_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, k, knn_i, knn_d, core_o, succ,
+                       dst_raw, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_emit_union(int n, const int *keep, const int *offs, const int *succ, const int *dst_raw,
+                                  const float *w, int *parent, int *src_out, int *dst_out, float *w_out,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL(init_emit_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, keep, offs, dst_raw, w,
+                       src_out, dst_out, w_out);
+    hipLaunchKernelGGL(iota_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, parent);
+    hipLaunchKernelGGL(init_union_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, keep, succ, parent);
+    hipLaunchKernelGGL(compress_all_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, parent);
+    return hipGetLastError();
+}
+
+hipError_t launch_labels_and_root_flags(int n, const int *perm, const int *parent, int *comp_p, int *flag,
+                                        hipStream_t st) {
+    hipLaunchKernelGGL(label_points_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, perm, parent, comp_p);
+    hipLaunchKernelGGL(root_flags_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, parent, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_root_scatter(int n, const int *flag, const int *offs, int *roots, int *rootslot, hipStream_t st) {
+    hipLaunchKernelGGL(root_scatter_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, flag, offs, roots, rootslot);
+    return hipGetLastError();
+}
+
+hipError_t launch_round_reset(int c, const int *roots, float *bound, unsigned long long *best, hipStream_t st) {
+    hipLaunchKernelGGL(round_reset_kernel, dim3(blocks(c, 256)), dim3(256), 0, st, c, roots, bound, best);
+    return hipGetLastError();
+}
+
+hipError_t launch_purity(int levels, const int2 *range, const int *comp_p, int *node_comp, hipStream_t st) {
+    int level = levels - 1;
+    bool leaf = true;
+    for (;;) {
+        const int steps = level < 8 ? level : 8;
+        const int count = 1 << level;
+        hipLaunchKernelGGL(purity_kernel, dim3(blocks(count, 256)), dim3(256), 0, st, level, steps, leaf ? 1 : 0,
+                           range, comp_p, node_comp);
+        leaf = false;
+        level -= steps;
+        if (level <= 0) break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_select(int c, int n, const int *roots, const unsigned long long *best, const int *partner,
+                         const int *posof, const int *perm, const int *comp_p, const int *rootslot, int *succ,
+                         int *ei, int *ej, float *ew, hipStream_t st) {
+    hipLaunchKernelGGL(select_kernel, dim3(blocks(c, 256)), dim3(256), 0, st, c, n, roots, best, partner, posof, perm,
+                       comp_p, rootslot, succ, ei, ej, ew);
+    return hipGetLastError();
+}
+
+hipError_t launch_round_emit_union(int c, int base, const int *keep, const int *offs, const int *ei, const int *ej,
+                                   const float *ew, const int *roots, const int *succ, int *parent, int *src_out,
+                                   int *dst_out, float *w_out, hipStream_t st) {
+    hipLaunchKernelGGL(round_emit_kernel, dim3(blocks(c, 256)), dim3(256), 0, st, c, base, keep, offs, ei, ej, ew,
+                       src_out, dst_out, w_out);
+    hipLaunchKernelGGL(round_union_kernel, dim3(blocks(c, 256)), dim3(256), 0, st, c, keep, roots, succ, parent);
+    hipLaunchKernelGGL(compress_roots_kernel, dim3(blocks(c, 256)), dim3(256), 0, st, c, roots, parent);
+    return hipGetLastError();
+}
+
+hipError_t launch_relabel_and_flags(int n, int c, const int *roots, const int *parent, int *comp_p, int *flag,
+                                    hipStream_t st) {
+    hipLaunchKernelGGL(relabel_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, parent, comp_p);
+    hipLaunchKernelGGL(still_root_kernel, dim3(blocks(c, 256)), dim3(256), 0, st, c, roots, parent, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_roots_compact(int c, const int *roots, const int *flag, const int *offs, int *roots_out,
+                                int *rootslot, hipStream_t st) {
+    hipLaunchKernelGGL(roots_compact_kernel, dim3(blocks(c, 256)), dim3(256), 0, st, c, roots, flag, offs, roots_out,
+                       rootslot);
+    return hipGetLastError();
+}
+
+}  // namespace fh

@@ -1,0 +1,222 @@
+"""Host-side mirror of fast_hdbscan's MST seam, backed by the gfx950 library.
+
+``compute_minimum_spanning_tree`` has the signature, argument checks and
+return contract of ``fast_hdbscan.hdbscan.compute_minimum_spanning_tree``
+(hdbscan.py:339-494) for the Euclidean Boruvka branch, so rebinding the
+reference's module attribute routes both ``HDBSCAN`` and ``PLSCAN`` through
+it (``layer_clusters.py:371,381`` imports it at call time).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+__all__ = [
+    "compute_minimum_spanning_tree",
+    "mst_arrays",
+    "mst_device",
+    "parallel_tree_query",
+    "build_kdtree",
+]
+
+
+def _as_points(data) -> np.ndarray:
+    # numba_kdtree.py:481: the tree works on a float32 copy of the data.
+    X = np.ascontiguousarray(np.asarray(data).astype(np.float32))
+    if X.ndim != 2:
+        raise ValueError("data must be a 2-D array")
+    return X
+
+
+def mst_arrays(X: np.ndarray, min_samples: int = 10, device: int = 0, stats: dict | None = None):
+    """Raw library outputs for host float32 points.
+
+    Returns (src int32[n-1], dst int32[n-1], w_sq float32[n-1],
+    core_sq float32[n], neighbors int32[n, max(min_samples, 1)]).
+    """
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, d = X.shape
+    ms = int(min_samples)
+    kn = ms if ms > 1 else 1
+    m = max(n - 1, 0)
+    src = np.empty(max(m, 1), np.int32)
+    dst = np.empty(max(m, 1), np.int32)
+    w = np.empty(max(m, 1), np.float32)
+    core = np.empty(n, np.float32)
+    nbr = np.empty((n, kn), np.int32)
+    st = _lib.FHStats()
+    ctx = _lib.context(device)
+    code = _lib.load().fh_mst_euclidean(
+        ctx, _lib.ptr(X), n, d, ms, _lib.ptr(core), _lib.ptr(nbr), _lib.ptr(src), _lib.ptr(dst), _lib.ptr(w),
+        ctypes.byref(st),
+    )
+    _lib.check(ctx, code)
+    if stats is not None:
+        stats.update(st.as_dict())
+    return src[:m], dst[:m], w[:m], core, nbr
+
+
+def mst_device(X, min_samples: int = 10, stats: dict | None = None, out: dict | None = None):
+    """MST of a float32 torch tensor resident on a HIP device.
+
+    Returns torch tensors on the same device: (src, dst, w_sq, core_sq,
+    neighbors).  ``out`` may hold preallocated tensors under those names.
+    """
+    import torch
+
+    if not (isinstance(X, torch.Tensor) and X.is_cuda):
+        raise TypeError("mst_device expects a torch tensor on a HIP device")
+    if X.dtype != torch.float32 or X.dim() != 2:
+        raise ValueError("mst_device expects a 2-D float32 tensor")
+    X = X.contiguous()
+    n, d = X.shape
+    ms = int(min_samples)
+    kn = ms if ms > 1 else 1
+    m = max(n - 1, 0)
+    dev = X.device
+    o = out if out is not None else {}
+    src = o.get("src") if o.get("src") is not None else torch.empty(max(m, 1), dtype=torch.int32, device=dev)
+    dst = o.get("dst") if o.get("dst") is not None else torch.empty(max(m, 1), dtype=torch.int32, device=dev)
+    w = o.get("w_sq") if o.get("w_sq") is not None else torch.empty(max(m, 1), dtype=torch.float32, device=dev)
+    core = o.get("core_sq") if o.get("core_sq") is not None else torch.empty(n, dtype=torch.float32, device=dev)
+    nbr = o.get("neighbors") if o.get("neighbors") is not None else torch.empty(
+        (n, kn), dtype=torch.int32, device=dev)
+    # The library works on its own stream: inputs must be complete.
+    torch.cuda.current_stream(dev).synchronize()
+    st = _lib.FHStats()
+    ctx = _lib.context(dev.index or 0)
+    code = _lib.load().fh_mst_euclidean_device(
+        ctx, ctypes.c_void_p(X.data_ptr()), n, d, ms, ctypes.c_void_p(core.data_ptr()),
+        ctypes.c_void_p(nbr.data_ptr()), ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+        ctypes.c_void_p(w.data_ptr()), ctypes.byref(st),
+    )
+    _lib.check(ctx, code)
+    if stats is not None:
+        stats.update(st.as_dict())
+    return src[:m], dst[:m], w[:m], core, nbr
+
+
+def _edges_f64(src, dst, w_sq) -> np.ndarray:
+    # boruvka.py:640,646: float64 rows (src, dst, sqrt(weight)), the square
+    # root taken in float64 of the float32 squared weight.
+    edges = np.empty((src.shape[0], 3), dtype=np.float64)
+    edges[:, 0] = src
+    edges[:, 1] = dst
+    edges[:, 2] = np.sqrt(w_sq.astype(np.float64))
+    return edges
+
+
+def compute_minimum_spanning_tree(
+    data,
+    min_samples=10,
+    sample_weights=None,
+    reproducible=False,
+    metric="euclidean",
+    algorithm="boruvka",
+    knn_k=None,
+    cannot_link=None,
+    validate_cannot_link=True,
+    cannot_link_groups=None,
+    metric_kwds=None,
+):
+    """Drop-in for hdbscan.compute_minimum_spanning_tree (hdbscan.py:339-494),
+    Euclidean Boruvka branch.
+
+    Returns ``(mst_edges f64[n-1, 3], neighbors int32[n, k], core_distances
+    float32[n])`` exactly as ``parallel_boruvka`` does (boruvka.py:646-647).
+    ``reproducible`` is accepted and has no effect: the device path is
+    deterministic and equals the reference's single-threaded run.  Data with
+    more than 30 columns stays on the exact KD-tree path (the reference sends
+    it to approximate pynndescent, hdbscan.py:447-463).
+    """
+    # hdbscan.py:409-425
+    if algorithm not in ("boruvka", "kruskal"):
+        raise ValueError("algorithm must be 'boruvka' or 'kruskal'. Got: %s" % algorithm)
+    if cannot_link is not None and cannot_link_groups is not None:
+        raise ValueError(
+            "cannot_link and cannot_link_groups are mutually exclusive. "
+            "Provide one or the other, not both."
+        )
+    _has_cl = cannot_link is not None or cannot_link_groups is not None
+    if _has_cl and algorithm != "kruskal":
+        raise ValueError(
+            "cannot_link constraints are only supported with "
+            "algorithm='kruskal'. Got algorithm=%r." % algorithm
+        )
+    if metric == "precomputed":
+        if sample_weights is not None:
+            raise NotImplementedError("sample_weights is not supported with metric='precomputed'.")
+        raise NotImplementedError("metric='precomputed' is not on the device MST path")
+    if metric != "euclidean":
+        raise NotImplementedError(f"metric={metric!r} is not on the device MST path (euclidean only)")
+    if algorithm == "kruskal":
+        raise NotImplementedError("algorithm='kruskal' is not on the device MST path")
+    if sample_weights is not None:
+        raise NotImplementedError("sample_weights are not on the device MST path yet")
+
+    try:
+        import torch
+
+        if isinstance(data, torch.Tensor) and data.is_cuda:
+            src, dst, w, core_sq, nbr = mst_device(data.float(), min_samples)
+            return (
+                _edges_f64(src.cpu().numpy(), dst.cpu().numpy(), w.cpu().numpy()),
+                nbr.cpu().numpy(),
+                np.sqrt(core_sq.cpu().numpy()),
+            )
+    except ImportError:
+        pass
+
+    X = _as_points(data)
+    src, dst, w, core_sq, nbr = mst_arrays(X, min_samples)
+    return _edges_f64(src, dst, w), nbr, np.sqrt(core_sq)
+
+
+def parallel_tree_query(data, queries, k=10, output_rdist=False, device: int = 0):
+    """numba_kdtree.parallel_tree_query(build_kdtree(data), queries, k,
+    output_rdist) (numba_kdtree.py:774-792) on the device."""
+    X = _as_points(data)
+    Q = _as_points(queries)
+    if Q.shape[1] != X.shape[1]:
+        raise ValueError("queries and data differ in dimension")
+    nq = Q.shape[0]
+    dist = np.empty((nq, int(k)), np.float32)
+    idx = np.empty((nq, int(k)), np.int32)
+    ctx = _lib.context(device)
+    code = _lib.load().fh_knn(ctx, _lib.ptr(X), X.shape[0], X.shape[1], _lib.ptr(Q), nq, int(k), _lib.ptr(dist),
+                              _lib.ptr(idx))
+    _lib.check(ctx, code)
+    if output_rdist:
+        return dist, idx
+    # numba_kdtree.py:792 takes the square root before sorting; for distinct
+    # squared distances that order is the one returned here.
+    return np.sqrt(dist), idx
+
+
+def build_kdtree(data, device: int = 0):
+    """numba_kdtree.build_kdtree(data, leaf_size=40) (numba_kdtree.py:478-521)
+    as a dict of arrays with the reference's dtypes."""
+    X = _as_points(data)
+    n, d = X.shape
+    L = _lib.load()
+    N = int(L.fh_tree_nnodes(n))
+    out = {
+        "data": X,
+        "idx_array": np.empty(n, np.int64),
+        "idx_start": np.empty(N, np.int64),
+        "idx_end": np.empty(N, np.int64),
+        "radius": np.empty(N, np.float32),
+        "is_leaf": np.empty(N, np.uint8),
+        "node_bounds": np.empty((2, N, d), np.float32),
+    }
+    ctx = None  # host-only entry point: no device context needed
+    code = L.fh_build_kdtree(ctx, _lib.ptr(X), n, d, _lib.ptr(out["idx_array"]), _lib.ptr(out["idx_start"]),
+                             _lib.ptr(out["idx_end"]), _lib.ptr(out["radius"]), _lib.ptr(out["is_leaf"]),
+                             _lib.ptr(out["node_bounds"]))
+    if code != _lib.FH_OK:
+        raise ValueError(f"fh_build_kdtree failed with code {code}")
+    out["is_leaf"] = out["is_leaf"].astype(bool)
+    return out

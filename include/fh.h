@@ -1,0 +1,135 @@
+/*
+ * fh.h -- C ABI of the MI355X (gfx950) Euclidean MST library for HDBSCAN.
+ *
+ * The library replaces the Euclidean Boruvka branch of
+ *   fast_hdbscan.hdbscan.compute_minimum_spanning_tree   (hdbscan.py:339-494)
+ * i.e. build_kdtree (numba_kdtree.py:478-521) followed by
+ *   parallel_boruvka (boruvka.py:552-647)
+ * and the standalone kNN service
+ *   parallel_tree_query (numba_kdtree.py:774-792).
+ *
+ * Conventions
+ *   - Plain pointers and sizes only; no exceptions cross the ABI.
+ *   - Every entry point returns FH_OK (0) or a negative FH_E* code; the
+ *     message of the last failure on a context is fh_last_error().
+ *   - Points are float32, row-major n x d.  Distances on the ABI are the
+ *     reference's *squared* float32 distances (rdist); the caller takes the
+ *     square roots exactly where the reference does (boruvka.py:646-647).
+ *   - Edge rows come back in the reference's row order: the kNN-seeded
+ *     initial edges in point order, then each Boruvka round's edges in the
+ *     order of their component's lowest point index (boruvka.py:43-78,
+ *     486-524, 608, 640).
+ *   - A context owns one HIP stream and its device work buffers; it is not
+ *     thread-safe.  All calls are synchronous: outputs are complete on
+ *     return.
+ */
+#ifndef FH_H
+#define FH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FH_OK 0
+#define FH_EINVAL (-1)       /* bad argument (shape, size, null pointer)     */
+#define FH_ENOMEM (-2)       /* device or host allocation failed              */
+#define FH_EDEVICE (-3)      /* HIP runtime / kernel launch error             */
+#define FH_EUNSUPPORTED (-4) /* valid input outside what the kernels handle   */
+
+/* Largest dimension the KD-tree kernels accept.  The reference routes
+ * d > 30 away from its KD-tree Boruvka (hdbscan.py:447-463). */
+#define FH_MAX_DIM 64
+
+typedef struct fh_ctx fh_ctx;
+
+/* Per-call statistics; times are milliseconds measured with HIP events on
+ * the context's stream (host phases with a steady clock). */
+typedef struct {
+    int32_t rounds;           /* Boruvka rounds after the kNN-seeded forest  */
+    int32_t n_levels;         /* KD-tree levels                               */
+    int64_t n_nodes;          /* KD-tree nodes                                */
+    int64_t initial_components; /* components after the kNN-seeded forest    */
+    int64_t n_edges;          /* rows written                                  */
+    double t_tree_ms;         /* tree build (incl. its transfers)             */
+    double t_knn_ms;          /* kNN / core-distance kernel                   */
+    double t_init_ms;         /* kNN-seeded forest                             */
+    double t_rounds_ms;       /* all Boruvka rounds                            */
+    double t_query_ms;        /* sum of the per-round nearest-edge kernels    */
+    double t_total_ms;        /* whole call                                    */
+} fh_stats;
+
+/* Create a context on HIP device `device`.  Returns NULL on failure (no
+ * device, out of memory).  Reference counterpart: none (numba keeps no
+ * state); the context holds what parallel_boruvka allocates per call. */
+fh_ctx *fh_create(int device);
+
+/* Release the context, its stream and device buffers. */
+void fh_destroy(fh_ctx *ctx);
+
+/* Message of the last failing call on ctx (never NULL). */
+const char *fh_last_error(const fh_ctx *ctx);
+
+/* Library version string, e.g. "fh 0.1.0 gfx950". */
+const char *fh_version(void);
+
+/*
+ * Mutual-reachability minimum spanning tree, inputs resident in HBM.
+ *   Replaces: hdbscan.py:466 build_kdtree(data) +
+ *             hdbscan.py:482-494 parallel_boruvka(tree, n_threads,
+ *             min_samples, sample_weights=empty(1), reproducible)
+ *             (boruvka.py:552-647, the sample-weight-free branches).
+ *
+ * All pointers are DEVICE pointers on the context's device:
+ *   X        [n*d]   float32 points, row-major (boruvka casts with
+ *                    astype(float32), numba_kdtree.py:481; pass float32)
+ *   core_sq  [n]     squared core distance of each point: the
+ *                    (min_samples+1)-th smallest rdist including the point
+ *                    itself (boruvka.py:581-584); zeros when min_samples==1
+ *                    (boruvka.py:592)
+ *   nbr      [n*kn]  kn = max(min_samples, 1); kNN columns 1..kn of the
+ *                    sorted query (boruvka.py:647 neighbors[:, 1:]), -1
+ *                    where fewer than kn+1 points exist
+ *   src,dst  [n-1]   edge endpoints (original point indices; dst may be -1
+ *                    only when min_samples >= n, as in the reference)
+ *   w_sq     [n-1]   float32 squared mutual-reachability weight
+ * stats may be NULL.
+ */
+int fh_mst_euclidean_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples,
+                            float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst, float *w_sq,
+                            fh_stats *stats);
+
+/* Same contract with HOST pointers; the library copies in and out. */
+int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, float *core_sq,
+                     int32_t *nbr, int32_t *src, int32_t *dst, float *w_sq, fh_stats *stats);
+
+/*
+ * Exact k nearest neighbours of Q's rows among X's rows.
+ *   Replaces: numba_kdtree.py:774-792 parallel_tree_query(build_kdtree(X),
+ *             Q, k, output_rdist=True): per row ascending squared float32
+ *             distances and int32 indices, in the reference's order for
+ *             ties (its bounded max-heap and deheap_sort); rows with fewer
+ *             than k points are padded with (+inf, -1).
+ * HOST pointers: X [n*d], Q [nq*d], dist_sq [nq*k], idx [nq*k].
+ */
+int fh_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, const float *Q, int64_t nq, int32_t k,
+           float *dist_sq, int32_t *idx);
+
+/*
+ * The KD-tree alone, built exactly as numba_kdtree.build_kdtree(X, 40)
+ * (numba_kdtree.py:478-521).  HOST pointers; host-only work, so ctx may be
+ * NULL.  N = fh_tree_nnodes(n).
+ *   idx_array [n] int64, idx_start [N] int64, idx_end [N] int64,
+ *   radius [N] float32, is_leaf [N] uint8, node_bounds [2*N*d] float32
+ *   laid out [2][N][d] (min then max).
+ */
+int64_t fh_tree_nnodes(int64_t n);
+int fh_build_kdtree(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int64_t *idx_array, int64_t *idx_start,
+                    int64_t *idx_end, float *radius, uint8_t *is_leaf, float *node_bounds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FH_H */

@@ -1,0 +1,727 @@
+/*
+ * fh_oracle.c -- CPU restatement of fast_hdbscan's Euclidean MST hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP product
+ * path is compared against; it is never linked into the product library and
+ * nothing under fast_hdbscan_amd/ may call it.  Only tests/, the smoke check
+ * and the CPU-baseline leg of the benchmark load it.
+ *
+ * Every function below restates one reference function, cited by
+ * file:line in fast_hdbscan/ (reference package version 0.3.2).  The
+ * restatement executes the reference with ONE thread: numba `prange` loops
+ * run sequentially in index order, which is the deterministic semantics the
+ * reference's own test-suite runs under (tests/test_core_graph.py:20 sets
+ * numba.set_num_threads(1) at import).
+ *
+ * Float32 arithmetic: the reference compiles rdist / the box lower bound
+ * with fastmath (numba_kdtree.py:530,555).  On an FMA host the sequential
+ * accumulation `result += diff*diff` contracts to fma(diff, diff, result);
+ * that is the canonical form used here (RDIST_FMA_SEQ).  RDIST_MULADD_SEQ
+ * (separately rounded multiply and add) is kept selectable because no
+ * reference test pins which one numba emits.
+ *
+ * Pinning status of each function is recorded in the header comment of the
+ * Python wrapper (oracle/oracle.py) and in the tests that exercise it.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <omp.h>
+
+#define RDIST_FMA_SEQ 0
+#define RDIST_MULADD_SEQ 1
+
+static int g_rdist_mode = RDIST_FMA_SEQ;
+
+void orc_set_rdist_mode(int mode) { g_rdist_mode = mode; }
+int orc_get_rdist_mode(void) { return g_rdist_mode; }
+
+/* numba_kdtree.py:539-546 (rdist) */
+static inline float rdist(const float *x, const float *y, int d) {
+    float r = 0.0f;
+    if (g_rdist_mode == RDIST_FMA_SEQ) {
+        for (int i = 0; i < d; ++i) {
+            float diff = x[i] - y[i];
+            r = fmaf(diff, diff, r);
+        }
+    } else {
+        for (int i = 0; i < d; ++i) {
+            float diff = x[i] - y[i];
+            volatile float sq = diff * diff;
+            r = r + sq;
+        }
+    }
+    return r;
+}
+
+/* numba_kdtree.py:566-575 (point_to_node_lower_bound_rdist); lo = node
+ * minimum (the reference passes node_bounds[0] as its `upper` argument),
+ * hi = node maximum. */
+static inline float box_lb(const float *lo, const float *hi, const float *pt, int d) {
+    float r = 0.0f;
+    for (int i = 0; i < d; ++i) {
+        float dlo = lo[i] > pt[i] ? lo[i] - pt[i] : 0.0f;
+        float dhi = pt[i] > hi[i] ? pt[i] - hi[i] : 0.0f;
+        float dd = dlo + dhi;
+        if (g_rdist_mode == RDIST_FMA_SEQ) {
+            r = fmaf(dd, dd, r);
+        } else {
+            volatile float sq = dd * dd;
+            r = r + sq;
+        }
+    }
+    return r;
+}
+
+float orc_rdist(const float *x, const float *y, int d) { return rdist(x, y, d); }
+float orc_box_lb(const float *lo, const float *hi, const float *pt, int d) {
+    return box_lb(lo, hi, pt, d);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tree build: numba_kdtree.py:79-521                                        */
+/* ------------------------------------------------------------------------ */
+
+/* numba_kdtree.py:489-490: n_levels = int(log2(max(1, (n-1)/leaf_size)) + 1).
+ * Evaluated in integers: 1 + max{m >= 0 : leaf_size * 2^m <= n-1}. */
+int64_t orc_tree_nlevels(int64_t n, int64_t leaf_size) {
+    if (n - 1 < leaf_size) return 1;
+    int64_t m = 0;
+    while ((leaf_size << (m + 1)) <= n - 1) ++m;
+    return m + 1;
+}
+
+int64_t orc_tree_nnodes(int64_t n, int64_t leaf_size) {
+    return (((int64_t)1) << orc_tree_nlevels(n, leaf_size)) - 1;
+}
+
+typedef struct {
+    const float *X;
+    int d;
+    int64_t *idx;
+} build_ctx;
+
+/* numba_kdtree.py:164-179 */
+static inline int cmp_idx(const build_ctx *c, int axis, int64_t i1, int64_t i2) {
+    float v1 = c->X[i1 * c->d + axis], v2 = c->X[i2 * c->d + axis];
+    if (v1 < v2) return -1;
+    if (v1 > v2) return 1;
+    if (i1 < i2) return -1;
+    if (i1 > i2) return 1;
+    return 0;
+}
+
+static inline void swp(int64_t *a, int64_t i, int64_t j) {
+    int64_t t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+}
+
+/* numba_kdtree.py:192-201 */
+static void insertion_sort(build_ctx *c, int axis, int64_t left, int64_t right) {
+    int64_t *a = c->idx;
+    for (int64_t i = left + 1; i < right; ++i) {
+        int64_t key = a[i];
+        int64_t j = i - 1;
+        while (j >= left && cmp_idx(c, axis, a[j], key) > 0) {
+            a[j + 1] = a[j];
+            --j;
+        }
+        a[j + 1] = key;
+    }
+}
+
+/* numba_kdtree.py:214-245 */
+static void sift_down_idx(build_ctx *c, int axis, int64_t off, int64_t start, int64_t end) {
+    int64_t *a = c->idx;
+    int64_t root = start;
+    while (root * 2 + 1 < end) {
+        int64_t child = root * 2 + 1;
+        int64_t s = root;
+        if (cmp_idx(c, axis, a[off + s], a[off + child]) < 0) s = child;
+        if (child + 1 < end && cmp_idx(c, axis, a[off + s], a[off + child + 1]) < 0) s = child + 1;
+        if (s == root) return;
+        swp(a, off + root, off + s);
+        root = s;
+    }
+}
+
+/* numba_kdtree.py:256-266 */
+static void heapsort_idx(build_ctx *c, int axis, int64_t left, int64_t right) {
+    int64_t size = right - left;
+    for (int64_t i = size / 2 - 1; i >= 0; --i) sift_down_idx(c, axis, left, i, size);
+    for (int64_t i = size - 1; i > 0; --i) {
+        swp(c->idx, left, left + i);
+        sift_down_idx(c, axis, left, 0, i);
+    }
+}
+
+/* numba_kdtree.py:280-299 */
+static int64_t median_of_three(build_ctx *c, int axis, int64_t left, int64_t right) {
+    int64_t *a = c->idx;
+    int64_t mid = (left + right - 1) / 2;
+    int64_t il = a[left], im = a[mid], ir = a[right - 1];
+    if (cmp_idx(c, axis, il, im) > 0) {
+        swp(a, left, mid);
+        int64_t t = il; il = im; im = t;
+    }
+    if (cmp_idx(c, axis, im, ir) > 0) {
+        swp(a, mid, right - 1);
+        int64_t t = im; im = ir; ir = t;
+        if (cmp_idx(c, axis, il, im) > 0) swp(a, left, mid);
+    }
+    return mid;
+}
+
+/* numba_kdtree.py:313-350 */
+static int64_t partition_idx(build_ctx *c, int axis, int64_t left, int64_t right, int64_t pivot) {
+    int64_t *a = c->idx;
+    swp(a, pivot, right - 1);
+    int64_t porig = a[right - 1];
+    int64_t i = left, j = right - 2;
+    for (;;) {
+        while (i <= j && cmp_idx(c, axis, a[i], porig) < 0) ++i;
+        while (i <= j && cmp_idx(c, axis, a[j], porig) >= 0) --j;
+        if (i >= j) break;
+        swp(a, i, j);
+        ++i;
+        --j;
+    }
+    swp(a, i, right - 1);
+    return i;
+}
+
+static inline int64_t floor_log2(int64_t v) { return 63 - __builtin_clzll((unsigned long long)v); }
+
+/* numba_kdtree.py:361-407 */
+static void introselect(build_ctx *c, int axis, int64_t left, int64_t right, int64_t nth) {
+    int64_t size = right - left;
+    if (size <= 16) {
+        insertion_sort(c, axis, left, right);
+        return;
+    }
+    int64_t depth = 2 * floor_log2(size);
+    while (right - left > 16) {
+        if (depth == 0) {
+            heapsort_idx(c, axis, left, right);
+            return;
+        }
+        --depth;
+        int64_t p = median_of_three(c, axis, left, right);
+        int64_t pp = partition_idx(c, axis, left, right, p);
+        if (nth < pp)
+            right = pp;
+        else if (nth > pp)
+            left = pp + 1;
+        else
+            return;
+    }
+    insertion_sort(c, axis, left, right);
+}
+
+typedef struct {
+    int64_t *idx_start, *idx_end;
+    float *radius;
+    uint8_t *is_leaf;
+    float *bounds; /* [2][N][d] */
+    int64_t N;
+} tree_out;
+
+/* numba_kdtree.py:79-115 */
+static void init_node(build_ctx *c, tree_out *t, int64_t node, int64_t s, int64_t e) {
+    int d = c->d;
+    float *lo = t->bounds + node * d;
+    float *hi = t->bounds + t->N * d + node * d;
+    for (int j = 0; j < d; ++j) {
+        lo[j] = INFINITY;
+        hi[j] = -INFINITY;
+    }
+    for (int64_t i = s; i < e; ++i) {
+        const float *row = c->X + c->idx[i] * d;
+        for (int j = 0; j < d; ++j) {
+            lo[j] = fminf(lo[j], row[j]);
+            hi[j] = fmaxf(hi[j], row[j]);
+        }
+    }
+    float r = 0.0f;
+    for (int j = 0; j < d; ++j) {
+        float df = fabsf(hi[j] - lo[j]) * 0.5f;
+        r += df * df;
+    }
+    t->idx_start[node] = s;
+    t->idx_end[node] = e;
+    t->radius[node] = sqrtf(r);
+}
+
+/* numba_kdtree.py:133-152 */
+static int split_dim(build_ctx *c, int64_t s, int64_t e) {
+    int d = c->d, result = 0;
+    float max_spread = 0.0f;
+    for (int j = 0; j < d; ++j) {
+        float mx = c->X[c->idx[s] * d + j], mn = mx;
+        for (int64_t i = s + 1; i < e; ++i) {
+            float v = c->X[c->idx[i] * d + j];
+            mx = fmaxf(mx, v);
+            mn = fminf(mn, v);
+        }
+        float spread = mx - mn;
+        if (spread > max_spread) {
+            max_spread = spread;
+            result = j;
+        }
+    }
+    return result;
+}
+
+/* numba_kdtree.py:414-475 */
+static void build_rec(build_ctx *c, tree_out *t, int64_t s, int64_t e, int64_t node) {
+    int64_t n_mid = (e - s) / 2;
+    init_node(c, t, node, s, e);
+    if (2 * node + 1 >= t->N) {
+        t->is_leaf[node] = 1;
+    } else if (e - s < 2) {
+        t->is_leaf[node] = 1;
+    } else {
+        t->is_leaf[node] = 0;
+        int axis = split_dim(c, s, e);
+        introselect(c, axis, s, e, s + n_mid);
+        build_rec(c, t, s, s + n_mid, 2 * node + 1);
+        build_rec(c, t, s + n_mid, e, 2 * node + 2);
+    }
+}
+
+/* numba_kdtree.py:478-521.  Caller allocates: idx_array[n], idx_start[N],
+ * idx_end[N], radius[N], is_leaf[N], bounds[2*N*d], N = orc_tree_nnodes(). */
+int orc_build_kdtree(const float *X, int64_t n, int d, int64_t leaf_size, int64_t *idx_array,
+                     int64_t *idx_start, int64_t *idx_end, float *radius, uint8_t *is_leaf,
+                     float *bounds) {
+    if (leaf_size < 1) return -1;
+    int64_t N = orc_tree_nnodes(n, leaf_size);
+    for (int64_t i = 0; i < n; ++i) idx_array[i] = i;
+    memset(idx_start, 0, sizeof(int64_t) * N);
+    memset(idx_end, 0, sizeof(int64_t) * N);
+    memset(radius, 0, sizeof(float) * N);
+    memset(is_leaf, 0, N);
+    memset(bounds, 0, sizeof(float) * 2 * N * d);
+    build_ctx c = {X, d, idx_array};
+    tree_out t = {idx_start, idx_end, radius, is_leaf, bounds, N};
+    build_rec(&c, &t, 0, n, 0);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* kNN: numba_kdtree.py:593-792                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const float *X; /* data, original order */
+    int d;
+    const int64_t *idx;
+    const int64_t *idx_start, *idx_end;
+    const uint8_t *is_leaf;
+    const float *bounds;
+    int64_t N;
+} tree_view;
+
+static inline const float *node_lo(const tree_view *t, int64_t node) { return t->bounds + node * t->d; }
+static inline const float *node_hi(const tree_view *t, int64_t node) {
+    return t->bounds + t->N * t->d + node * t->d;
+}
+
+/* numba_kdtree.py:593-635 */
+static int heap_push(float *pri, int32_t *ind, int size, float p, int32_t n) {
+    if (p >= pri[0]) return 0;
+    pri[0] = p;
+    ind[0] = n;
+    int i = 0, isw;
+    for (;;) {
+        int ic1 = 2 * i + 1, ic2 = ic1 + 1;
+        if (ic1 >= size) break;
+        else if (ic2 >= size) {
+            if (pri[ic1] > p) isw = ic1;
+            else break;
+        } else if (pri[ic1] >= pri[ic2]) {
+            if (p < pri[ic1]) isw = ic1;
+            else break;
+        } else {
+            if (p < pri[ic2]) isw = ic2;
+            else break;
+        }
+        pri[i] = pri[isw];
+        ind[i] = ind[isw];
+        i = isw;
+    }
+    pri[i] = p;
+    ind[i] = n;
+    return 1;
+}
+
+/* numba_kdtree.py:647-664 */
+static void siftdown(float *h1, int32_t *h2, int len, int elt) {
+    while (elt * 2 + 1 < len) {
+        int l = elt * 2 + 1, r = l + 1, s = elt;
+        if (h1[s] < h1[l]) s = l;
+        if (r < len && h1[s] < h1[r]) s = r;
+        if (s == elt) break;
+        float tf = h1[elt]; h1[elt] = h1[s]; h1[s] = tf;
+        int32_t ti = h2[elt]; h2[elt] = h2[s]; h2[s] = ti;
+        elt = s;
+    }
+}
+
+/* numba_kdtree.py:668-677, one row */
+static void deheap_row(float *dist, int32_t *ind, int k) {
+    for (int j = k - 1; j > 0; --j) {
+        float tf = dist[0]; dist[0] = dist[j]; dist[j] = tf;
+        int32_t ti = ind[0]; ind[0] = ind[j]; ind[j] = ti;
+        siftdown(dist, ind, j, 0);
+    }
+}
+
+/* numba_kdtree.py:705-762 */
+static void knn_rec(const tree_view *t, int64_t node, const float *pt, float *hp, int32_t *hi, int k,
+                    float lb) {
+    if (lb > hp[0]) return;
+    if (t->is_leaf[node]) {
+        for (int64_t i = t->idx_start[node]; i < t->idx_end[node]; ++i) {
+            int64_t idx = t->idx[i];
+            float dd = rdist(pt, t->X + idx * t->d, t->d);
+            if (dd < hp[0]) heap_push(hp, hi, k, dd, (int32_t)idx);
+        }
+    } else {
+        int64_t l = 2 * node + 1, r = l + 1;
+        float lbl = box_lb(node_lo(t, l), node_hi(t, l), pt, t->d);
+        float lbr = box_lb(node_lo(t, r), node_hi(t, r), pt, t->d);
+        if (lbl <= lbr) {
+            knn_rec(t, l, pt, hp, hi, k, lbl);
+            knn_rec(t, r, pt, hp, hi, k, lbr);
+        } else {
+            knn_rec(t, r, pt, hp, hi, k, lbr);
+            knn_rec(t, l, pt, hp, hi, k, lbl);
+        }
+    }
+}
+
+/* numba_kdtree.py:774-792 with output_rdist=True: squared distances,
+ * ascending per row after deheap_sort.  Q holds nq query rows. */
+int orc_knn(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
+            const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
+            const float *Q, int64_t nq, int k, float *out_d, int32_t *out_i, int nthreads) {
+    (void)n;
+    tree_view t = {X, d, idx_array, idx_start, idx_end, is_leaf, bounds, N};
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t q = 0; q < nq; ++q) {
+        float *hp = out_d + q * k;
+        int32_t *hi = out_i + q * k;
+        for (int j = 0; j < k; ++j) {
+            hp[j] = INFINITY;
+            hi[j] = -1;
+        }
+        const float *pt = Q + q * d;
+        float lb = box_lb(node_lo(&t, 0), node_hi(&t, 0), pt, d);
+        knn_rec(&t, 0, pt, hp, hi, k, lb);
+        deheap_row(hp, hi, k);
+    }
+    return 0;
+}
+
+/* Brute-force kNN with the same rdist arithmetic; rows sorted by
+ * (distance, index).  Checker for orc_knn on tie-free inputs. */
+typedef struct {
+    float d;
+    int32_t i;
+} di_pair;
+
+static int cmp_di(const void *a, const void *b) {
+    const di_pair *x = (const di_pair *)a, *y = (const di_pair *)b;
+    if (x->d < y->d) return -1;
+    if (x->d > y->d) return 1;
+    return (x->i > y->i) - (x->i < y->i);
+}
+
+int orc_knn_brute(const float *X, int64_t n, int d, int k, float *out_d, int32_t *out_i, int nthreads) {
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        di_pair *buf = (di_pair *)malloc(sizeof(di_pair) * (size_t)n);
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t q = 0; q < n; ++q) {
+            for (int64_t j = 0; j < n; ++j) {
+                buf[j].d = rdist(X + q * d, X + j * d, d);
+                buf[j].i = (int32_t)j;
+            }
+            qsort(buf, (size_t)n, sizeof(di_pair), cmp_di);
+            for (int j = 0; j < k; ++j) {
+                out_d[q * k + j] = j < n ? buf[j].d : INFINITY;
+                out_i[q * k + j] = j < n ? buf[j].i : -1;
+            }
+        }
+        free(buf);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Union-find: disjoint_set.py:11-42                                          */
+/* ------------------------------------------------------------------------ */
+
+/* The reference's path-halving statement (disjoint_set.py:23-24) assigns
+ * x first and then parent[x] with the NEW x, which stores parent[x] back
+ * into itself: the find performs no compression.  Roots are what matter;
+ * this plain walk returns the same roots.  Negative indices wrap like
+ * numba's array indexing (x = -1 reads parent[n-1]). */
+static inline int32_t ds_find(const int32_t *parent, int64_t n, int32_t x) {
+    for (;;) {
+        int32_t p = parent[x < 0 ? x + n : x];
+        if (p == x) return x;
+        x = p;
+    }
+}
+
+static void ds_union(int32_t *parent, int32_t *rank, int64_t n, int32_t x, int32_t y) {
+    x = ds_find(parent, n, x);
+    y = ds_find(parent, n, y);
+    if (x == y) return;
+    if (rank[x] < rank[y]) {
+        int32_t t = x; x = y; y = t;
+    }
+    parent[y] = x;
+    if (rank[x] == rank[y]) rank[x] += 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boruvka: boruvka.py:34-647                                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    tree_view t;
+    const float *core;
+    const int64_t *pcomp;
+    const int64_t *ncomp;
+} bq_ctx;
+
+/* boruvka.py:154-287 */
+static void bq_rec(const bq_ctx *c, int64_t node, const float *pt, float *hp, int32_t *hi, float core_i,
+                   int64_t comp_i, float lb, float *B) {
+    const tree_view *t = &c->t;
+    if (lb > hp[0]) return;
+    if (lb > B[0] || core_i > B[0]) return;
+    if (c->ncomp[node] == comp_i) return;
+    if (t->is_leaf[node]) {
+        for (int64_t i = t->idx_start[node]; i < t->idx_end[node]; ++i) {
+            int64_t idx = t->idx[i];
+            if (c->pcomp[idx] != comp_i && c->core[idx] < B[0]) {
+                float dd = rdist(pt, t->X + idx * t->d, t->d);
+                if (core_i > dd) dd = core_i;
+                if (c->core[idx] > dd) dd = c->core[idx];
+                if (dd < hp[0]) {
+                    hp[0] = dd;
+                    hi[0] = (int32_t)idx;
+                    if (dd < B[0]) B[0] = dd;
+                }
+            }
+        }
+    } else {
+        int64_t l = 2 * node + 1, r = l + 1;
+        float lbl = box_lb(node_lo(t, l), node_hi(t, l), pt, t->d);
+        float lbr = box_lb(node_lo(t, r), node_hi(t, r), pt, t->d);
+        if (lbl <= lbr) {
+            bq_rec(c, l, pt, hp, hi, core_i, comp_i, lbl, B);
+            bq_rec(c, r, pt, hp, hi, core_i, comp_i, lbr, B);
+        } else {
+            bq_rec(c, r, pt, hp, hi, core_i, comp_i, lbr, B);
+            bq_rec(c, l, pt, hp, hi, core_i, comp_i, lbl, B);
+        }
+    }
+}
+
+/* boruvka.py:94-124 */
+static void update_component_vectors(const tree_view *t, const int32_t *parent, int64_t n, int64_t *ncomp,
+                                     int64_t *pcomp) {
+    for (int64_t i = 0; i < n; ++i) pcomp[i] = ds_find(parent, n, (int32_t)i);
+    for (int64_t i = t->N - 1; i >= 0; --i) {
+        if (t->is_leaf[i]) {
+            int64_t s = t->idx_start[i], e = t->idx_end[i];
+            int64_t cand = pcomp[t->idx[s]];
+            int pure = 1;
+            for (int64_t j = s + 1; j < e; ++j)
+                if (pcomp[t->idx[j]] != cand) {
+                    pure = 0;
+                    break;
+                }
+            if (pure) ncomp[i] = cand;
+        } else {
+            int64_t l = 2 * i + 1, r = l + 1;
+            if (ncomp[l] == ncomp[r]) ncomp[i] = ncomp[l];
+        }
+    }
+}
+
+typedef struct {
+    int64_t *src, *dst;
+    float *w;
+    int64_t count;
+} edge_list;
+
+static void emit(edge_list *E, int64_t s, int64_t d, float w) {
+    E->src[E->count] = s;
+    E->dst[E->count] = d;
+    E->w[E->count] = w;
+    E->count++;
+}
+
+/* boruvka.py:486-524.  knn rows are k wide, sorted ascending. */
+static void init_from_knn(const int32_t *ki, const float *kd, int64_t n, int k, const float *core,
+                          int32_t *parent, int32_t *rank, edge_list *E) {
+    int64_t *es = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *ed = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    float *ew = (float *)malloc(sizeof(float) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        es[i] = -1;
+        for (int j = 1; j < k; ++j) {
+            int32_t kk = ki[i * k + j];
+            float ck = core[kk < 0 ? kk + n : kk];
+            if (core[i] >= ck) {
+                float w = core[i] > kd[i * k + j] ? core[i] : kd[i * k + j];
+                es[i] = i;
+                ed[i] = kk;
+                ew[i] = w;
+                break;
+            }
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        if (es[i] < 0) continue;
+        int32_t fc = ds_find(parent, n, (int32_t)es[i]);
+        int32_t tc = ds_find(parent, n, (int32_t)ed[i]);
+        if (fc != tc) {
+            emit(E, es[i], ed[i], ew[i]);
+            ds_union(parent, rank, n, fc, tc);
+        }
+    }
+    free(es);
+    free(ed);
+    free(ew);
+}
+
+/* boruvka.py:35-78.  The numba typed dict keyed by component keeps keys in
+ * first-insertion order, i.e. by the lowest point index of each component. */
+static int64_t merge_components(int32_t *parent, int32_t *rank, int64_t n, const int32_t *cand_i,
+                                const float *cand_d, const int64_t *pcomp, edge_list *E, int64_t *slot,
+                                int64_t *order, int64_t *ei, int64_t *ej, float *ew) {
+    int64_t nk = 0;
+    for (int64_t i = 0; i < n; ++i) slot[i] = -1;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t c = pcomp[i];
+        int64_t s = slot[c];
+        if (s >= 0 && !isnan(cand_d[i])) {
+            if (cand_d[i] < ew[s]) {
+                ei[s] = i;
+                ej[s] = cand_i[i];
+                ew[s] = cand_d[i];
+            }
+        } else {
+            if (s < 0) {
+                s = nk++;
+                slot[c] = s;
+                order[s] = c;
+            }
+            ei[s] = i;
+            ej[s] = cand_i[i];
+            ew[s] = cand_d[i];
+        }
+    }
+    int64_t added = 0;
+    for (int64_t s = 0; s < nk; ++s) {
+        int32_t fc = ds_find(parent, n, (int32_t)ei[s]);
+        int32_t tc = ds_find(parent, n, (int32_t)ej[s]);
+        if (fc != tc) {
+            emit(E, ei[s], ej[s], ew[s]);
+            ++added;
+            ds_union(parent, rank, n, fc, tc);
+        }
+    }
+    return added;
+}
+
+/* boruvka.py:552-647 (no sample weights).  Outputs:
+ *   src, dst, w_sq : n-1 rows in the reference's row order, weights are the
+ *                    float32 squared mutual-reachability distances (the
+ *                    caller takes the float64 square root, boruvka.py:646)
+ *   core_sq        : n squared core distances (boruvka.py:584; zeros if
+ *                    min_samples == 1)
+ *   knn_i, knn_d   : n x kq raw sorted kNN rows, kq = max(min_samples+1, 2)
+ *   rounds         : number of Boruvka rounds executed
+ * Returns the number of edges written. */
+int64_t orc_boruvka(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
+                    const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
+                    int min_samples, int64_t *src, int64_t *dst, float *w_sq, float *core_sq, int32_t *knn_i,
+                    float *knn_d, int32_t *rounds, int nthreads) {
+    tree_view t = {X, d, idx_array, idx_start, idx_end, is_leaf, bounds, N};
+    int32_t *parent = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t *rank = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+    int64_t *pcomp = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *ncomp = (int64_t *)malloc(sizeof(int64_t) * (size_t)N);
+    for (int64_t i = 0; i < n; ++i) {
+        parent[i] = (int32_t)i;
+        pcomp[i] = i;
+    }
+    for (int64_t i = 0; i < N; ++i) ncomp[i] = -1;
+
+    int kq = min_samples > 1 ? min_samples + 1 : 2;
+    orc_knn(X, n, d, idx_array, idx_start, idx_end, is_leaf, bounds, N, X, n, kq, knn_d, knn_i, nthreads);
+    for (int64_t i = 0; i < n; ++i) core_sq[i] = min_samples > 1 ? knn_d[i * kq + kq - 1] : 0.0f;
+
+    edge_list E = {src, dst, w_sq, 0};
+    init_from_knn(knn_i, knn_d, n, kq, core_sq, parent, rank, &E);
+    update_component_vectors(&t, parent, n, ncomp, pcomp);
+
+    /* boruvka.py:604: len(np.unique(point_components)) */
+    int64_t n_comp = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (pcomp[i] == i) ++n_comp;
+
+    float *cand_d = (float *)malloc(sizeof(float) * (size_t)n);
+    int32_t *cand_i = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    float *B = (float *)malloc(sizeof(float) * (size_t)n);
+    int64_t *slot = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *order = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *ei = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *ej = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    float *ew = (float *)malloc(sizeof(float) * (size_t)n);
+    bq_ctx c = {t, core_sq, pcomp, ncomp};
+    int32_t r = 0;
+    while (n_comp > 1) {
+        /* boruvka.py:300-331, run sequentially in point order */
+        for (int64_t i = 0; i < n; ++i) {
+            cand_d[i] = INFINITY;
+            cand_i[i] = -1;
+            B[i] = INFINITY;
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            const float *pt = X + i * d;
+            float lb = box_lb(node_lo(&t, 0), node_hi(&t, 0), pt, d);
+            bq_rec(&c, 0, pt, cand_d + i, cand_i + i, core_sq[i], pcomp[i], lb, B + pcomp[i]);
+        }
+        int64_t added = merge_components(parent, rank, n, cand_i, cand_d, pcomp, &E, slot, order, ei, ej, ew);
+        n_comp -= added;
+        update_component_vectors(&t, parent, n, ncomp, pcomp);
+        ++r;
+        if (added == 0 && n_comp > 1) break; /* cannot happen for finite input; guards the loop */
+    }
+    *rounds = r;
+    free(cand_d);
+    free(cand_i);
+    free(B);
+    free(slot);
+    free(order);
+    free(ei);
+    free(ej);
+    free(ew);
+    free(parent);
+    free(rank);
+    free(pcomp);
+    free(ncomp);
+    return E.count;
+}

@@ -1,0 +1,137 @@
+"""Device path vs the CPU oracle, bit for bit, through the C-ABI.
+
+Every case compares the library's outputs with the oracle's sequential
+restatement of parallel_boruvka on the same float32 points:
+core distances, neighbour columns, and the MST rows (source, target,
+squared weight) in row order.
+"""
+import numpy as np
+import pytest
+
+from fast_hdbscan_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(X, ms):
+    from oracle import oracle as O
+
+    return O.boruvka(O.build_kdtree(X), ms)
+
+
+def _device(X, ms):
+    from fast_hdbscan_amd.hdbscan import mst_arrays
+
+    stats = {}
+    out = mst_arrays(X, ms, stats=stats)
+    return out, stats
+
+
+def _check(X, ms):
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    print(f"case n={X.shape[0]} d={X.shape[1]} ms={ms}", flush=True)
+    (src, dst, w, core, nbr), stats = _device(X, ms)
+    ref = _oracle(X, ms)
+    np.testing.assert_array_equal(core, ref.core_sq)
+    np.testing.assert_array_equal(nbr, ref.knn_i[:, 1:])
+    assert src.shape == ref.src.shape
+    np.testing.assert_array_equal(w, ref.w_sq)
+    np.testing.assert_array_equal(src, ref.src)
+    np.testing.assert_array_equal(dst, ref.dst)
+    assert stats["rounds"] == ref.rounds
+    return stats
+
+
+def test_config1_blobs_10k():
+    _check(workloads.blobs_small(10_000, seed=0), 10)
+
+
+@pytest.mark.parametrize("ms", [1, 2, 5, 10, 15])
+def test_blobs_min_samples(ms):
+    _check(workloads.blobs_small(4000, seed=ms), ms)
+
+
+@pytest.mark.parametrize("d", [1, 3, 4, 5, 8, 10])
+def test_dimensions(d):
+    from sklearn.datasets import make_blobs
+
+    X = make_blobs(n_samples=3000, n_features=d, centers=6, random_state=d)[0]
+    _check(X, 5)
+
+
+def test_uniform_reference_fixture():
+    # test_hdbscan.py:169-186 data: np.random.seed(0); rand(150, 2)
+    rng = np.random.RandomState(0)
+    _check(rng.rand(150, 2), 5)
+
+
+def test_all_duplicates():
+    # test_edge_cases.py:20-24: 50 identical 3-D points
+    _check(np.full((50, 3), 1.0), 5)
+
+
+def test_duplicates_injected():
+    X = workloads.blobs_small(5000, seed=3)
+    rng = np.random.default_rng(0)
+    dup = rng.choice(5000, 400, replace=False)
+    X[dup[:200]] = X[dup[200:]]
+    _check(X, 10)
+
+
+def test_integer_grid_ties():
+    g = np.stack(np.meshgrid(np.arange(60), np.arange(50)), -1).reshape(-1, 2).astype(np.float32)
+    g = g[np.random.default_rng(1).permutation(g.shape[0])]
+    _check(g, 4)
+
+
+def test_k_exceeds_n():
+    # test_edge_cases.py:26-31: min_samples=100 with 10 points
+    rng = np.random.RandomState(42)
+    _check(rng.rand(10, 2), 100)
+
+
+def test_high_min_samples():
+    # test_edge_cases.py:46-51
+    rng = np.random.RandomState(42)
+    _check(rng.rand(20, 2), 18)
+
+
+def test_large_k_global_heap():
+    _check(workloads.blobs_small(1500, seed=7), 150)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 41, 81, 82])
+def test_tiny(n):
+    rng = np.random.default_rng(n)
+    _check(rng.standard_normal((n, 2)), 3)
+
+
+def test_one_dimensional():
+    # test_edge_cases.py:33-44
+    rng = np.random.RandomState(42)
+    X = np.concatenate([rng.normal(0, 0.1, 50), rng.normal(5, 0.1, 50)]).reshape(-1, 1)
+    _check(X, 10)
+
+
+def test_anisotropic_200k():
+    _check(workloads.scale_2d(200_000, seed=5), 10)
+
+
+def test_knn_external_queries():
+    from fast_hdbscan_amd.hdbscan import parallel_tree_query
+    from oracle import oracle as O
+
+    X = workloads.blobs_small(20_000, seed=1)
+    Q = workloads.blobs_small(3000, seed=2)
+    d, i = parallel_tree_query(X, Q, k=7, output_rdist=True)
+    rd, ri = O.tree_query(O.build_kdtree(X), Q, 7)
+    np.testing.assert_array_equal(d, rd)
+    np.testing.assert_array_equal(i, ri)
+
+
+def test_repeatable():
+    X = workloads.scale_2d(100_000, seed=9)
+    a = _device(X, 10)[0]
+    b = _device(X, 10)[0]
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)

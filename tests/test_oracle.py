@@ -1,0 +1,303 @@
+"""Pins for the CPU oracle (oracle/fh_oracle.c) -- CPU only.
+
+Each check is chosen so that a plausible restatement mistake fails it:
+closed forms, an independent library (sklearn KDTree, scipy MST), brute
+force, a hand-derived fixture, and an unpruned re-derivation of every
+Boruvka round.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from fast_hdbscan_amd import workloads
+
+
+def _blobs(n, d=2, seed=0, centers=5):
+    from sklearn.datasets import make_blobs
+
+    return make_blobs(n_samples=n, n_features=d, centers=centers, random_state=seed)[0].astype(np.float32)
+
+
+# ----------------------------------------------------------------- tree ----
+
+@pytest.mark.parametrize("n", [1, 2, 40, 41, 42, 80, 81, 82, 161, 1000, 12345, 2_000_000, 10_000_000])
+def test_tree_levels_closed_form(n):
+    # numba_kdtree.py:489-490 evaluated in float64 as the reference does
+    expect = int(np.log2(max(1, (n - 1) / 40)) + 1)
+    assert O.tree_nlevels(n) == expect
+    assert O.tree_nnodes(n) == 2 ** expect - 1
+
+
+def _sk_tree(X):
+    from sklearn.neighbors import KDTree
+
+    t = KDTree(X.astype(np.float64), leaf_size=40)
+    data, idx, nd, nb = t.get_arrays()
+    return idx, nd, nb
+
+
+@pytest.mark.parametrize("case", ["blobs2", "blobs3", "d1", "d5", "dups", "grid"])
+def test_tree_matches_sklearn_structure(case):
+    rng = np.random.default_rng(0)
+    if case == "blobs2":
+        X = _blobs(5000)
+    elif case == "blobs3":
+        X = _blobs(7000, d=3, seed=1)
+    elif case == "d1":
+        X = rng.standard_normal((3000, 1)).astype(np.float32)
+    elif case == "d5":
+        X = _blobs(4000, d=5, seed=2)
+    elif case == "dups":
+        X = _blobs(3000, seed=3)
+        X[1500:] = X[:1500]
+    else:
+        X = np.stack(np.meshgrid(np.arange(50), np.arange(40)), -1).reshape(-1, 2).astype(np.float32)
+        X = X[rng.permutation(X.shape[0])]
+    T = O.build_kdtree(X)
+    idx, nd, nb = _sk_tree(X)
+    np.testing.assert_array_equal(T.idx_start, nd["idx_start"])
+    np.testing.assert_array_equal(T.idx_end, nd["idx_end"])
+    np.testing.assert_array_equal(T.is_leaf, nd["is_leaf"].astype(bool))
+    np.testing.assert_array_equal(T.node_bounds, nb.astype(np.float32))
+    for i in np.nonzero(T.is_leaf)[0]:
+        s, e = T.idx_start[i], T.idx_end[i]
+        assert set(T.idx_array[s:e]) == set(idx[s:e])
+
+
+def test_tree_median_invariant():
+    X = _blobs(20000, d=3, seed=4)
+    T = O.build_kdtree(X)
+    for node in range(T.n_nodes):
+        if T.is_leaf[node]:
+            continue
+        s, e = T.idx_start[node], T.idx_end[node]
+        mid = s + (e - s) // 2
+        lo, hi = T.node_bounds[0, node], T.node_bounds[1, node]
+        spread = hi - lo
+        axis = int(np.argmax(spread))  # first maximum, like the strict '>' scan
+        keys = [(X[i, axis], i) for i in T.idx_array[s:e]]
+        left, right = keys[: mid - s], keys[mid - s:]
+        # the split leaves the n//2 smallest (value, index) keys on the left;
+        # positions inside each side are rearranged again by the children
+        assert max(left) < min(right)
+
+
+def test_tree_permutation():
+    X = _blobs(10000)
+    T = O.build_kdtree(X)
+    assert np.array_equal(np.sort(T.idx_array), np.arange(10000))
+
+
+# ------------------------------------------------------------------ kNN ----
+
+@pytest.mark.parametrize("d,k", [(2, 11), (3, 16), (5, 6), (8, 3)])
+def test_knn_matches_brute_force(d, k):
+    X = _blobs(2500, d=d, seed=d)
+    T = O.build_kdtree(X)
+    td, ti = O.tree_query(T, X, k)
+    bd, bi = O.knn_brute(X, k)
+    np.testing.assert_array_equal(td, bd)
+    # indices agree wherever the row has no exact distance tie
+    ties = np.zeros(X.shape[0], bool)
+    for r in range(X.shape[0]):
+        ties[r] = len(np.unique(bd[r])) < k
+    np.testing.assert_array_equal(ti[~ties], bi[~ties])
+
+
+def test_core_distance_vs_float64_kdtree():
+    from sklearn.neighbors import KDTree
+
+    X = workloads.blobs_small(10_000, seed=0)
+    T = O.build_kdtree(X)
+    td, _ = O.tree_query(T, X, 11)
+    ref, _ = KDTree(X.astype(np.float64)).query(X.astype(np.float64), k=11)
+    np.testing.assert_allclose(np.sqrt(td[:, -1].astype(np.float64)), ref[:, -1], rtol=1e-5, atol=1e-6)
+
+
+def test_rdist_fma_form():
+    # The canonical float32 form: r = d0*d0 rounded, then fma(d_i, d_i, r).
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        x = rng.standard_normal(2).astype(np.float32)
+        y = rng.standard_normal(2).astype(np.float32)
+        d0 = np.float32(x[0] - y[0])
+        d1 = np.float32(x[1] - y[1])
+        # fma in exact arithmetic: float64 holds d1*d1 + r exactly enough for
+        # a single final rounding whenever the exponents are close; use
+        # fractions for an exact reference instead.
+        from fractions import Fraction
+
+        r0 = np.float32(d0 * d0)
+        exact = Fraction(float(d1)) * Fraction(float(d1)) + Fraction(float(r0))
+        want = np.float32(float(exact)) if True else None
+        got = O.lib().orc_rdist(x, y, 2)
+        # float(exact) rounds to float64 first; confirm no double-rounding
+        # ambiguity by checking the neighbours of `want`.
+        cands = [np.nextafter(want, np.float32(-np.inf)), want, np.nextafter(want, np.float32(np.inf))]
+        best = min(cands, key=lambda c: abs(Fraction(float(c)) - exact))
+        assert np.float32(got) == best
+
+
+def test_duplicate_points_heap_layout_by_hand():
+    # test_edge_cases.py:20-24 data.  One leaf (n_levels = 1) holding points
+    # 0..49; every distance is 0.  Tracing simple_heap_push
+    # (numba_kdtree.py:593-635) for pushes 0..5 leaves indices
+    # [5, 2, 4, 0, 1, 3] in heap order (all later pushes are rejected since
+    # 0 >= 0); deheap_sort (668-677) turns that into [2, 4, 0, 1, 3, 5].
+    # initialize_boruvka_from_knn then links every i to column 1 = point 4;
+    # (4, 4) is a self-loop and is skipped.
+    X = np.full((50, 3), 1.0, np.float32)
+    r = O.boruvka(O.build_kdtree(X), 5)
+    for row in r.knn_i:
+        assert list(row) == [2, 4, 0, 1, 3, 5]
+    assert np.all(r.core_sq == 0)
+    expect_src = [i for i in range(50) if i != 4]
+    assert list(r.src) == expect_src
+    assert list(r.dst) == [4] * 49
+    assert np.all(r.w_sq == 0)
+    assert r.rounds == 0
+
+
+# -------------------------------------------------------------- Boruvka ----
+
+def _mrd_matrix(X, core_sq):
+    n = X.shape[0]
+    bd, bi = O.knn_brute(X, n)
+    D = np.empty((n, n), np.float32)
+    rows = np.repeat(np.arange(n), n)
+    D[rows, bi.ravel()] = bd.ravel()
+    return np.maximum(D, np.maximum(core_sq[:, None], core_sq[None, :]))
+
+
+@pytest.mark.parametrize("n,ms,seed", [(600, 5, 0), (900, 10, 1), (500, 1, 2), (700, 3, 3)])
+def test_mst_weights_match_scipy(n, ms, seed):
+    from scipy.sparse.csgraph import minimum_spanning_tree
+
+    X = _blobs(n, seed=seed)
+    r = O.boruvka(O.build_kdtree(X), ms)
+    M = _mrd_matrix(X, r.core_sq).astype(np.float64)
+    # scipy drops explicit zeros; shift every weight by one
+    G = M + 1.0
+    np.fill_diagonal(G, 0.0)
+    ref = np.sort(minimum_spanning_tree(G).tocoo().data - 1.0)
+    assert r.src.shape[0] == n - 1
+    np.testing.assert_array_equal(np.sort(r.w_sq.astype(np.float64)), ref)
+
+
+def _spanning(n, src, dst):
+    parent = list(range(n))
+
+    def find(x):
+        while parent[x] != x:
+            x = parent[x]
+        return x
+
+    for a, b in zip(src, dst):
+        ra, rb = find(int(a)), find(int(b))
+        assert ra != rb, "cycle in MST"
+        parent[ra] = rb
+    return len({find(i) for i in range(n)}) == 1
+
+
+def test_mst_is_spanning_tree():
+    X = workloads.scale_2d(30_000, seed=3)
+    r = O.boruvka(O.build_kdtree(X), 10)
+    assert r.src.shape[0] == X.shape[0] - 1
+    assert _spanning(X.shape[0], r.src, r.dst)
+    # each edge weight is the mutual reachability of its endpoints
+    d = np.array([O.lib().orc_rdist(X[a], X[b], 2) for a, b in zip(r.src[:2000], r.dst[:2000])], np.float32)
+    w = np.maximum(d, np.maximum(r.core_sq[r.src[:2000]], r.core_sq[r.dst[:2000]]))
+    np.testing.assert_array_equal(w, r.w_sq[:2000])
+
+
+def _dfs_order(T, X, i):
+    """All leaf points in the order a full (unpruned) walk from point i visits
+    them: closer child first by box lower bound, left on ties."""
+    order = []
+    lib = O.lib()
+
+    def lb(node):
+        return lib.orc_box_lb(np.ascontiguousarray(T.node_bounds[0, node]),
+                              np.ascontiguousarray(T.node_bounds[1, node]), X[i], X.shape[1])
+
+    def rec(node):
+        if T.is_leaf[node]:
+            order.extend(T.idx_array[T.idx_start[node]:T.idx_end[node]].tolist())
+            return
+        l, r = 2 * node + 1, 2 * node + 2
+        if lb(l) <= lb(r):
+            rec(l)
+            rec(r)
+        else:
+            rec(r)
+            rec(l)
+
+    rec(0)
+    return order
+
+
+@pytest.mark.parametrize("case", ["blobs", "grid"])
+def test_boruvka_rounds_rederived_without_pruning(case):
+    """Re-derive every Boruvka round from its definition (lowest point index
+    at the component minimum; its first partner in unpruned walk order; edges
+    in order of the component's lowest point; the last-processed edge of
+    each cycle skipped) and compare with the pruned restatement."""
+    if case == "blobs":
+        X = _blobs(300, seed=11)
+        ms = 5
+    else:
+        X = np.stack(np.meshgrid(np.arange(15), np.arange(14)), -1).reshape(-1, 2).astype(np.float32)
+        X = X[np.random.default_rng(2).permutation(X.shape[0])]
+        ms = 3
+    n = X.shape[0]
+    T = O.build_kdtree(X)
+    r = O.boruvka(T, ms)
+    core = r.core_sq
+    bd, bi = O.knn_brute(X, n)
+    D = np.empty((n, n), np.float32)
+    D[np.repeat(np.arange(n), n), bi.ravel()] = bd.ravel()
+    M = np.maximum(D, np.maximum(core[:, None], core[None, :]))
+
+    # initial forest from the oracle's own kNN rows (E2), processed in order
+    parent = list(range(n))
+
+    def find(x):
+        while parent[x] != x:
+            x = parent[x]
+        return x
+
+    rows = []
+    for i in range(n):
+        for j in range(1, r.knn_i.shape[1]):
+            kk = int(r.knn_i[i, j])
+            if core[i] >= core[kk]:
+                a, b = find(i), find(kk)
+                if a != b:
+                    rows.append((i, kk, max(core[i], r.knn_d[i, j])))
+                    parent[a] = b
+                break
+    n_init = len(rows)
+    assert list(zip(r.src[:n_init], r.dst[:n_init])) == [(a, b) for a, b, _ in rows]
+
+    orders = {}
+    while len({find(i) for i in range(n)}) > 1:
+        comp = np.array([find(i) for i in range(n)])
+        chosen = {}
+        for c in sorted(set(comp.tolist()), key=lambda c: np.min(np.nonzero(comp == c)[0])):
+            members = np.nonzero(comp == c)[0]
+            others = comp != c
+            best = min(M[i][others].min() for i in members)
+            i_star = min(i for i in members if M[i][others].min() == best)
+            if i_star not in orders:
+                orders[i_star] = _dfs_order(T, X, i_star)
+            j_star = next(j for j in orders[i_star] if comp[j] != c and M[i_star, j] == best)
+            chosen[c] = (i_star, j_star, best)
+        for c, (a, b, w) in chosen.items():
+            fa, fb = find(a), find(b)
+            if fa != fb:
+                rows.append((a, b, w))
+                parent[fa] = fb
+    assert len(rows) == n - 1
+    np.testing.assert_array_equal(r.src, [a for a, _, _ in rows])
+    np.testing.assert_array_equal(r.dst, [b for _, b, _ in rows])
+    np.testing.assert_array_equal(r.w_sq, np.array([w for _, _, w in rows], np.float32))
