@@ -13,7 +13,9 @@
 //      reads back two integers per round (edges emitted, components left).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -21,6 +23,7 @@
 
 #include "../../include/fh.h"
 #include "fh_kernels.h"
+#include "fh_tree.h"
 #include "kdtree_build.h"
 
 using fh::HostTree;
@@ -55,6 +58,16 @@ struct fh_ctx {
         roots2, rootslot, partner, ei, ej, tiles, small, heap_i, flag, out_src, out_dst, out_nbr;
     DevBuf<int2> range;
     DevBuf<unsigned long long> best;
+    // device tree build
+    DevBuf<float> xdev;
+    DevBuf<uint64_t> tkeys;
+    DevBuf<uint32_t> obox;
+    DevBuf<int> taxis, tcnt, gtpos, ltpos, tactive;
+    DevBuf<fh::SegState> tseg;
+    int64_t geom_n = -1;            // n the uploaded node ranges belong to
+    std::vector<int2> geom_range;   // host copy of the node ranges
+    std::vector<int> geom_maxsize;  // largest node per level
+    int geom_levels = 0;
     int *h_small = nullptr;  // pinned readback slots
     hipEvent_t ev[8] = {};
 };
@@ -115,29 +128,111 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Build the host tree and upload it in the device layout.
-int upload_tree(fh_ctx *ctx, const float *X, int64_t n, int d, HostTree &t) {
-    if (fh::build_host_tree(X, n, d, kLeafSize, ctx->host_threads, t) != 0)
-        return fail(ctx, FH_EINVAL, "tree build rejected its input");
-    if (!t.leaves_last_level) return fail(ctx, FH_EUNSUPPORTED, "tree has a leaf above the last level");
-    if (t.levels > 31) return fail(ctx, FH_EUNSUPPORTED, "tree deeper than 31 levels");
-    const int64_t N = t.n_nodes;
+// Node ranges of the implicit tree for n points (numba_kdtree.py:426-473:
+// the left child takes n//2 points), uploaded once per n.
+int tree_geometry(fh_ctx *ctx, int64_t n) {
+    if (ctx->geom_n == n) return FH_OK;
+    const int levels = fh::tree_levels(n, kLeafSize);
+    const int64_t N = (int64_t(1) << levels) - 1;
+    ctx->geom_range.assign(N, make_int2(0, 0));
+    ctx->geom_maxsize.assign(levels, 0);
+    ctx->geom_range[0] = make_int2(0, (int)n);
+    for (int64_t i = 0; i < N; ++i) {
+        const int2 r = ctx->geom_range[i];
+        int lvl = 0;
+        while (((int64_t(1) << (lvl + 1)) - 1) <= i) ++lvl;
+        ctx->geom_maxsize[lvl] = std::max(ctx->geom_maxsize[lvl], r.y - r.x);
+        if (2 * i + 2 < N) {
+            const int mid = r.x + (r.y - r.x) / 2;
+            ctx->geom_range[2 * i + 1] = make_int2(r.x, mid);
+            ctx->geom_range[2 * i + 2] = make_int2(mid, r.y);
+        }
+    }
+    // every node above the last level must hold >= 2 points so that all
+    // leaves sit on the last level (numba_kdtree.py:442-445)
+    for (int64_t i = 0; i < (N - 1) / 2; ++i)
+        if (ctx->geom_range[i].y - ctx->geom_range[i].x < 2)
+            return fail(ctx, FH_EUNSUPPORTED, "tree has a leaf above the last level");
+    if (levels > 31) return fail(ctx, FH_EUNSUPPORTED, "tree deeper than 31 levels");
+    FH_TRY(ensure(ctx, ctx->range, N));
+    FH_HIP(hipMemcpyAsync(ctx->range.p, ctx->geom_range.data(), sizeof(int2) * N, hipMemcpyHostToDevice,
+                          ctx->stream));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->geom_levels = levels;
+    ctx->geom_n = n;
+    return FH_OK;
+}
+
+// Exact KD-tree built on the device from points already in HBM; fills xp,
+// perm, posof, range, box.
+int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *levels_out, int64_t *nodes_out) {
+    FH_TRY(tree_geometry(ctx, n));
+    const int L = ctx->geom_levels;
+    const int64_t N = (int64_t(1) << L) - 1;
+    const int ni = (int)n;
+    const int64_t last_count = int64_t(1) << (L - 1);
     FH_TRY(ensure(ctx, ctx->xp, (size_t)n * d));
     FH_TRY(ensure(ctx, ctx->perm, n));
     FH_TRY(ensure(ctx, ctx->posof, n));
-    FH_TRY(ensure(ctx, ctx->range, N));
     FH_TRY(ensure(ctx, ctx->box, (size_t)N * 2 * d));
-    std::vector<int2> rng(N);
-    for (int64_t i = 0; i < N; ++i) rng[i] = make_int2(t.start[i], t.end[i]);
+    FH_TRY(ensure(ctx, ctx->tkeys, n));
+    FH_TRY(ensure(ctx, ctx->obox, (size_t)N * 2 * d));
+    FH_TRY(ensure(ctx, ctx->taxis, last_count));
+    FH_TRY(ensure(ctx, ctx->tseg, last_count));
+    FH_TRY(ensure(ctx, ctx->tcnt, (size_t)n / fh::tree_chunk() + 2 * last_count + 1));
+    FH_TRY(ensure(ctx, ctx->gtpos, n));
+    FH_TRY(ensure(ctx, ctx->ltpos, n));
+    FH_TRY(ensure(ctx, ctx->tactive, 8));
+    fh::TreeBuildArgs a{};
+    a.X = Xd;
+    a.d = d;
+    a.range = ctx->range.p;
+    a.A = ctx->tkeys.p;
+    a.obox = ctx->obox.p;
+    a.box = ctx->box.p;
+    a.axis = ctx->taxis.p;
+    a.seg = ctx->tseg.p;
+    a.cnt = ctx->tcnt.p;
+    a.gtpos = ctx->gtpos.p;
+    a.ltpos = ctx->ltpos.p;
+    a.active = ctx->tactive.p;
     hipStream_t st = ctx->stream;
-    FH_HIP(hipMemcpyAsync(ctx->xp.p, t.xp.data(), sizeof(float) * n * d, hipMemcpyHostToDevice, st));
-    FH_HIP(hipMemcpyAsync(ctx->perm.p, t.perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    FH_HIP(hipMemcpyAsync(ctx->range.p, rng.data(), sizeof(int2) * N, hipMemcpyHostToDevice, st));
-    FH_HIP(hipMemcpyAsync(ctx->box.p, t.box.data(), sizeof(float) * N * 2 * d, hipMemcpyHostToDevice, st));
-    FH_HIP(fh::launch_posof((int)n, ctx->perm.p, ctx->posof.p, st));
-    // The host vectors are released when the caller's HostTree goes out of
-    // scope; make sure the copies are done first.
-    FH_HIP(hipStreamSynchronize(st));
+    const int small = fh::tree_small_limit();
+    constexpr int kBatch = 4;
+    FH_HIP(fh::tree_identity_keys(ni, a.A, st));
+    for (int l = 0; l < L; ++l) {
+        const int count = 1 << l;
+        const int maxs = ctx->geom_maxsize[l];
+        const bool internal = l < L - 1;
+        FH_HIP(fh::tree_level_bounds(a, l, maxs, 0, st));
+        FH_HIP(fh::tree_level_axis(a, l, internal ? 1 : 0, st));
+        if (!internal) break;
+        FH_HIP(fh::tree_level_keys(a, l, maxs, 0, st));
+        if (maxs > small) {
+            // multi-block steps until every node's active range fits in LDS
+            for (;;) {
+                FH_HIP(hipMemsetAsync(a.active, 0, sizeof(int) * kBatch, st));
+                for (int b = 0; b < kBatch; ++b) {
+                    fh::TreeBuildArgs ab = a;
+                    ab.active = a.active + b;
+                    FH_HIP(fh::tree_big_step(ab, count, maxs, st));
+                }
+                FH_HIP(hipMemcpyAsync(ctx->h_small, a.active, sizeof(int) * kBatch, hipMemcpyDeviceToHost, st));
+                FH_HIP(hipStreamSynchronize(st));
+                if (ctx->h_small[kBatch - 1] == 0) break;
+            }
+            // ranges that hit the depth limit while still large: sort them
+            std::vector<fh::SegState> segs(count);
+            FH_HIP(hipMemcpyAsync(segs.data(), a.seg, sizeof(fh::SegState) * count, hipMemcpyDeviceToHost, st));
+            FH_HIP(hipStreamSynchronize(st));
+            for (int k = 0; k < count; ++k)
+                if (segs[k].flag == fh::SEG_SORT) FH_HIP(fh::tree_sort_range(a.A, segs[k].L, segs[k].R, st));
+        }
+        FH_HIP(fh::tree_small_select(a, count, st));
+    }
+    FH_HIP(fh::tree_finalize(a, ni, ctx->perm.p, ctx->posof.p, ctx->xp.p, st));
+    if (levels_out) *levels_out = L;
+    if (nodes_out) *nodes_out = N;
     return FH_OK;
 }
 
@@ -149,7 +244,7 @@ int knn_block(int k) {
 
 // kNN of `nq` query rows against the uploaded tree.  Self-query when
 // query_perm != nullptr (queries = tree-order rows).
-int run_knn(fh_ctx *ctx, const HostTree &t, const float *queries, const int *query_perm, int64_t nq, int k,
+int run_knn(fh_ctx *ctx, int d, int levels, const float *queries, const int *query_perm, int64_t nq, int k,
             float *out_d, int *out_i, float *core_o, float *core_p, int core_from_knn) {
     fh::KnnArgs a{};
     a.xp = ctx->xp.p;
@@ -159,9 +254,9 @@ int run_knn(fh_ctx *ctx, const HostTree &t, const float *queries, const int *que
     a.queries = queries;
     a.query_perm = query_perm;
     a.nq = nq;
-    a.d = t.d;
+    a.d = d;
     a.k = k;
-    a.levels = t.levels;
+    a.levels = levels;
     a.out_d = out_d;
     a.out_i = out_i;
     a.core_o = core_o;
@@ -187,21 +282,21 @@ int read_small(fh_ctx *ctx, int count) {
     return FH_OK;
 }
 
-// The MST proper.  X_host: the points on the host (tree build input).
-// Outputs are device pointers.
-int run_mst(fh_ctx *ctx, const float *X_host, int64_t n, int d, int ms, float *core_out, int *nbr_out,
-            int *src_out, int *dst_out, float *w_out, fh_stats *stats) {
+// The MST proper.  Xd: the points in HBM (original order).  Outputs are
+// device pointers.
+int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_out, int *nbr_out, int *src_out,
+            int *dst_out, float *w_out, fh_stats *stats) {
     Timer total;
     hipStream_t st = ctx->stream;
     fh_stats S{};
     FH_HIP(hipEventRecord(ctx->ev[0], st));
 
-    Timer tt;
-    HostTree t;
-    FH_TRY(upload_tree(ctx, X_host, n, d, t));
-    S.t_tree_ms = tt.ms();
-    S.n_levels = t.levels;
-    S.n_nodes = t.n_nodes;
+    int levels = 0;
+    int64_t n_nodes = 0;
+    FH_TRY(build_tree_device(ctx, Xd, n, d, &levels, &n_nodes));
+    FH_HIP(hipEventRecord(ctx->ev[1], st));
+    S.n_levels = levels;
+    S.n_nodes = n_nodes;
 
     const int ni = (int)n;
     const int kq = ms > 1 ? ms + 1 : 2;
@@ -210,7 +305,7 @@ int run_mst(fh_ctx *ctx, const float *X_host, int64_t n, int d, int ms, float *c
     FH_TRY(ensure(ctx, ctx->core_o, n));
     FH_TRY(ensure(ctx, ctx->core_p, n));
     FH_TRY(ensure(ctx, ctx->comp_p, n));
-    FH_TRY(ensure(ctx, ctx->node_comp, t.n_nodes));
+    FH_TRY(ensure(ctx, ctx->node_comp, n_nodes));
     FH_TRY(ensure(ctx, ctx->succ, n));
     FH_TRY(ensure(ctx, ctx->dst_raw, n));
     FH_TRY(ensure(ctx, ctx->w_init, n));
@@ -235,8 +330,7 @@ int run_mst(fh_ctx *ctx, const float *X_host, int64_t n, int d, int ms, float *c
     FH_TRY(ensure(ctx, ctx->small, 8));
 
     // -- core distances
-    FH_HIP(hipEventRecord(ctx->ev[1], st));
-    FH_TRY(run_knn(ctx, t, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p, ctx->core_p.p,
+    FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p, ctx->core_p.p,
                    ms > 1 ? 1 : 0));
     FH_HIP(hipEventRecord(ctx->ev[2], st));
 
@@ -271,12 +365,12 @@ int run_mst(fh_ctx *ctx, const float *X_host, int64_t n, int d, int ms, float *c
     qa.partner = ctx->partner.p;
     qa.n = n;
     qa.d = d;
-    qa.levels = t.levels;
+    qa.levels = levels;
     double query_ms = 0.0;
     int rounds = 0;
     while (C > 1) {
         FH_HIP(fh::launch_round_reset(C, ctx->roots.p, ctx->bound.p, ctx->best.p, st));
-        FH_HIP(fh::launch_purity(t.levels, ctx->range.p, ctx->comp_p.p, ctx->node_comp.p, st));
+        FH_HIP(fh::launch_purity(levels, ctx->range.p, ctx->comp_p.p, ctx->node_comp.p, st));
         FH_HIP(hipEventRecord(ctx->ev[5], st));
         FH_HIP(fh::launch_query(qa, st));
         FH_HIP(hipEventRecord(ctx->ev[6], st));
@@ -313,6 +407,7 @@ int run_mst(fh_ctx *ctx, const float *X_host, int64_t n, int d, int ms, float *c
 
     S.rounds = rounds;
     S.n_edges = base;
+    S.t_tree_ms = event_ms(ctx->ev[0], ctx->ev[1]);
     S.t_knn_ms = event_ms(ctx->ev[1], ctx->ev[2]);
     S.t_init_ms = event_ms(ctx->ev[2], ctx->ev[3]);
     S.t_rounds_ms = event_ms(ctx->ev[3], ctx->ev[4]);
@@ -382,6 +477,8 @@ void fh_destroy(fh_ctx *c) {
     release(c->rootslot); release(c->partner); release(c->ei); release(c->ej); release(c->tiles);
     release(c->small); release(c->heap_i); release(c->flag); release(c->out_src); release(c->out_dst);
     release(c->out_nbr); release(c->range); release(c->best);
+    release(c->xdev); release(c->tkeys); release(c->obox); release(c->taxis); release(c->tcnt);
+    release(c->gtpos); release(c->ltpos); release(c->tactive); release(c->tseg);
     if (c->h_small) e = hipHostFree(c->h_small);
     for (auto &ev : c->ev)
         if (ev) e = hipEventDestroy(ev);
@@ -401,11 +498,7 @@ int fh_mst_euclidean_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, i
     FH_TRY(check_args(ctx, X, n, d, min_samples));
     if (n > 1 && (!src || !dst || !w_sq)) return fail(ctx, FH_EINVAL, "edge outputs are null");
     FH_HIP(hipSetDevice(ctx->device));
-    // The tree is built on the host: bring the points over once.
-    std::vector<float> host((size_t)n * d);
-    FH_HIP(hipMemcpyAsync(host.data(), X, sizeof(float) * n * d, hipMemcpyDeviceToHost, ctx->stream));
-    FH_HIP(hipStreamSynchronize(ctx->stream));
-    return run_mst(ctx, host.data(), n, d, min_samples, core_sq, nbr, src, dst, w_sq, stats);
+    return run_mst(ctx, X, n, d, min_samples, core_sq, nbr, src, dst, w_sq, stats);
 }
 
 int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, float *core_sq,
@@ -420,7 +513,9 @@ int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t 
     FH_TRY(ensure(ctx, ctx->out_src, m));
     FH_TRY(ensure(ctx, ctx->out_dst, m));
     FH_TRY(ensure(ctx, ctx->out_w, m));
-    FH_TRY(run_mst(ctx, X, n, d, min_samples, ctx->out_core.p, ctx->out_nbr.p, ctx->out_src.p, ctx->out_dst.p,
+    FH_TRY(ensure(ctx, ctx->xdev, (size_t)n * d));
+    FH_HIP(hipMemcpyAsync(ctx->xdev.p, X, sizeof(float) * n * d, hipMemcpyHostToDevice, ctx->stream));
+    FH_TRY(run_mst(ctx, ctx->xdev.p, n, d, min_samples, ctx->out_core.p, ctx->out_nbr.p, ctx->out_src.p, ctx->out_dst.p,
                    ctx->out_w.p, stats));
     hipStream_t st = ctx->stream;
     if (core_sq) FH_HIP(hipMemcpyAsync(core_sq, ctx->out_core.p, sizeof(float) * n, hipMemcpyDeviceToHost, st));
@@ -440,14 +535,16 @@ int fh_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, const float *Q, in
     if (!Q || !dist_sq || !idx || nq < 0 || k < 1) return fail(ctx, FH_EINVAL, "bad query arguments");
     if (nq == 0) return FH_OK;
     FH_HIP(hipSetDevice(ctx->device));
-    HostTree t;
-    FH_TRY(upload_tree(ctx, X, n, d, t));
+    FH_TRY(ensure(ctx, ctx->xdev, (size_t)n * d));
+    FH_HIP(hipMemcpyAsync(ctx->xdev.p, X, sizeof(float) * n * d, hipMemcpyHostToDevice, ctx->stream));
+    int levels = 0;
+    FH_TRY(build_tree_device(ctx, ctx->xdev.p, n, d, &levels, nullptr));
     FH_TRY(ensure(ctx, ctx->queries, (size_t)nq * d));
     FH_TRY(ensure(ctx, ctx->knn_d, (size_t)nq * k));
     FH_TRY(ensure(ctx, ctx->knn_i, (size_t)nq * k));
     hipStream_t st = ctx->stream;
     FH_HIP(hipMemcpyAsync(ctx->queries.p, Q, sizeof(float) * nq * d, hipMemcpyHostToDevice, st));
-    FH_TRY(run_knn(ctx, t, ctx->queries.p, nullptr, nq, k, ctx->knn_d.p, ctx->knn_i.p, nullptr, nullptr, 0));
+    FH_TRY(run_knn(ctx, d, levels, ctx->queries.p, nullptr, nq, k, ctx->knn_d.p, ctx->knn_i.p, nullptr, nullptr, 0));
     FH_HIP(hipMemcpyAsync(dist_sq, ctx->knn_d.p, sizeof(float) * nq * k, hipMemcpyDeviceToHost, st));
     FH_HIP(hipMemcpyAsync(idx, ctx->knn_i.p, sizeof(int) * nq * k, hipMemcpyDeviceToHost, st));
     FH_HIP(hipStreamSynchronize(st));
@@ -456,24 +553,55 @@ int fh_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, const float *Q, in
 
 int fh_build_kdtree(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int64_t *idx_array, int64_t *idx_start,
                     int64_t *idx_end, float *radius, uint8_t *is_leaf, float *node_bounds) {
-    // Host-only: ctx may be NULL (then no error message is kept).
     if (!X || !idx_array || !idx_start || !idx_end || !radius || !is_leaf || !node_bounds || n < 1 || d < 1 ||
         n > (int64_t(1) << 30))
         return ctx ? fail(ctx, FH_EINVAL, "bad fh_build_kdtree arguments") : FH_EINVAL;
-    HostTree t;
-    if (fh::build_host_tree(X, n, d, kLeafSize, ctx ? ctx->host_threads : 0, t) != 0)
-        return ctx ? fail(ctx, FH_EINVAL, "tree build rejected its input") : FH_EINVAL;
-    const int64_t N = t.n_nodes;
-    for (int64_t i = 0; i < n; ++i) idx_array[i] = t.perm[i];
-    for (int64_t i = 0; i < N; ++i) {
-        idx_start[i] = t.start[i];
-        idx_end[i] = t.end[i];
-        radius[i] = t.radius[i];
-        is_leaf[i] = t.leaf[i];
-        for (int j = 0; j < d; ++j) {
-            node_bounds[i * d + j] = t.box[(size_t)i * 2 * d + j];
-            node_bounds[(N + i) * d + j] = t.box[(size_t)i * 2 * d + d + j];
+    if (!ctx) {
+        // No context: the host construction (same tree, no device needed).
+        HostTree t;
+        if (fh::build_host_tree(X, n, d, kLeafSize, 0, t) != 0) return FH_EINVAL;
+        const int64_t N = t.n_nodes;
+        for (int64_t i = 0; i < n; ++i) idx_array[i] = t.perm[i];
+        for (int64_t i = 0; i < N; ++i) {
+            idx_start[i] = t.start[i];
+            idx_end[i] = t.end[i];
+            radius[i] = t.radius[i];
+            is_leaf[i] = t.leaf[i];
+            for (int j = 0; j < d; ++j) {
+                node_bounds[i * d + j] = t.box[(size_t)i * 2 * d + j];
+                node_bounds[(N + i) * d + j] = t.box[(size_t)i * 2 * d + d + j];
+            }
         }
+        return FH_OK;
+    }
+    if (d > FH_MAX_DIM) return fail(ctx, FH_EUNSUPPORTED, "d above FH_MAX_DIM");
+    FH_HIP(hipSetDevice(ctx->device));
+    FH_TRY(ensure(ctx, ctx->xdev, (size_t)n * d));
+    hipStream_t st = ctx->stream;
+    FH_HIP(hipMemcpyAsync(ctx->xdev.p, X, sizeof(float) * n * d, hipMemcpyHostToDevice, st));
+    int levels = 0;
+    int64_t N = 0;
+    FH_TRY(build_tree_device(ctx, ctx->xdev.p, n, d, &levels, &N));
+    std::vector<int> perm(n);
+    std::vector<float> box((size_t)N * 2 * d);
+    FH_HIP(hipMemcpyAsync(perm.data(), ctx->perm.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(box.data(), ctx->box.p, sizeof(float) * N * 2 * d, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < n; ++i) idx_array[i] = perm[i];
+    const int64_t first_leaf = (N - 1) / 2;
+    for (int64_t i = 0; i < N; ++i) {
+        idx_start[i] = ctx->geom_range[i].x;
+        idx_end[i] = ctx->geom_range[i].y;
+        is_leaf[i] = i >= first_leaf ? 1 : 0;
+        float r = 0.0f;
+        for (int j = 0; j < d; ++j) {
+            const float lo = box[(size_t)i * 2 * d + j], hi = box[(size_t)i * 2 * d + d + j];
+            node_bounds[i * d + j] = lo;
+            node_bounds[(N + i) * d + j] = hi;
+            const float half = std::fabs(hi - lo) * 0.5f;
+            r += half * half;
+        }
+        radius[i] = std::sqrt(r);
     }
     return FH_OK;
 }

@@ -1,0 +1,48 @@
+// Launch interface of the exact device KD-tree build (fh_tree.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fh {
+
+enum SegFlag : int {
+    SEG_LARGE = 0,  // still running multi-block introselect steps
+    SEG_SMALL = 1,  // to be finished in LDS by one work-group
+    SEG_SORT = 2,   // hit the depth limit while large: range must be sorted
+    SEG_DONE = 3,
+};
+
+// Introselect state of one node (numba_kdtree.py:361-386 locals).
+struct SegState {
+    int L, R, nth, depth, flag;
+    int P, S, M;
+    uint64_t pivot;
+};
+
+struct TreeBuildArgs {
+    const float *X;      // points, original order, row-major
+    int d;
+    const int2 *range;   // (idx_start, idx_end) of every node
+    uint64_t *A;         // n keys: (ordered coordinate << 32) | original index
+    uint32_t *obox;      // N * 2d order-preserving bounds scratch
+    float *box;          // N * 2d bounds (min[d], max[d]) per node
+    int *axis;           // per node of the current level
+    SegState *seg;       // per node of the current level
+    int *cnt;            // per (node, chunk) counts
+    int *gtpos, *ltpos;  // n each: exchange partners
+    int *active;         // step counters
+};
+
+int tree_small_limit();
+int tree_chunk();
+hipError_t tree_identity_keys(int n, uint64_t *A, hipStream_t st);
+hipError_t tree_level_bounds(const TreeBuildArgs &a, int level, int max_size, int identity, hipStream_t st);
+hipError_t tree_level_axis(const TreeBuildArgs &a, int level, int internal, hipStream_t st);
+hipError_t tree_level_keys(const TreeBuildArgs &a, int level, int max_size, int identity, hipStream_t st);
+hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStream_t st);
+hipError_t tree_small_select(const TreeBuildArgs &a, int count, hipStream_t st);
+hipError_t tree_sort_range(uint64_t *A, int L, int R, hipStream_t st);
+hipError_t tree_finalize(const TreeBuildArgs &a, int n, int *perm, int *posof, float *xp, hipStream_t st);
+
+}  // namespace fh

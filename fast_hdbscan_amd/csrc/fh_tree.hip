@@ -1,0 +1,588 @@
+// Exact KD-tree construction on gfx950.
+//
+// The tree must equal numba_kdtree.build_kdtree(X, 40) (numba_kdtree.py:
+// 478-521) including the order of points inside every node, which is the
+// by-product of the reference's introselect swap sequence
+// (numba_kdtree.py:280-407).  The build is level-synchronous: at level l all
+// 2^l nodes compute their bounds and split dimension, then run their
+// introselect concurrently.  Each introselect step is replayed exactly:
+//
+//   * median of three (280-299) and the pivot move: one lane per node;
+//   * the two-pointer partition (313-350) pairs the m-th element >= pivot
+//     counted from the left with the m-th element < pivot counted from the
+//     right, for m < M = #(>= pivot) left of P = left + #(< pivot); both ranks
+//     come from a prefix count of "< pivot" flags, so every swap of the
+//     reference's scan happens here as one parallel exchange; the pivot
+//     then swaps with position P;
+//   * ranges of <= 16 (insertion sort, 192-201) and the depth-limit
+//     fallback (heapsort, 256-266) both leave the range sorted; keys are
+//     unique, so any sort reproduces them.
+//
+// Keys are 64-bit: the order-preserving image of the float coordinate on
+// the split axis (-0 folded into +0, which the reference compares equal) in
+// the high word and the original point index in the low word, so one
+// integer comparison is the reference's (value, index) comparison
+// (numba_kdtree.py:164-179) and the low word doubles as the permutation.
+//
+// Nodes larger than kSmall elements run their steps as multi-block kernels
+// over global memory; smaller ones finish inside one work-group in LDS.
+#include <hip/hip_runtime.h>
+
+#include "fh_tree.h"
+
+namespace fh {
+
+namespace {
+
+constexpr int kSmall = 4096;    // largest range finished in LDS
+constexpr int kChunk = 4096;    // elements per block in the multi-block steps
+constexpr int kTB = 256;        // threads per block
+constexpr int kPer = kChunk / kTB;
+
+__device__ __forceinline__ uint32_t f2o(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t o) {
+    uint32_t b = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(b);
+}
+__device__ __forceinline__ uint64_t make_key(float v, uint32_t idx) {
+    if (v == 0.0f) v = 0.0f;  // -0 == +0 in the reference's comparison
+    return ((uint64_t)f2o(v) << 32) | idx;
+}
+__device__ __forceinline__ uint32_t key_idx(uint64_t k) { return (uint32_t)(k & 0xffffffffull); }
+
+__device__ __forceinline__ int floor_log2(int v) { return 31 - __clz(v); }
+
+// Level geometry: node k of level l covers [start(k), end(k)); the host
+// uploads start/end for all nodes (they depend on n only).
+struct LevelRef {
+    const int2 *range;  // all nodes
+    int first;          // 2^l - 1
+    int count;          // 2^l
+    int cps;            // chunks per node
+};
+
+__device__ __forceinline__ int wave_incl(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan (blockDim == kTB); total returned by reference.
+__device__ __forceinline__ int block_excl(int v, int &total, int *ws) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = wave_incl(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kTB / 64; ++i) {
+        int s = ws[i];
+        off += i < w ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - v;
+}
+
+// ---------------------------------------------------------- bounds ----
+
+// Blocks over (node, chunk): per-dimension min / max of the node's points,
+// merged with order-preserving integer atomics (numba_kdtree.py:96-105).
+__global__ __launch_bounds__(kTB) void bounds_kernel(LevelRef lv, const float *__restrict__ X, int d,
+                                                     const uint64_t *__restrict__ A, int identity,
+                                                     uint32_t *obox) {
+    __shared__ uint32_t smin[kTB / 64], smax[kTB / 64];
+    const int k = blockIdx.x / lv.cps, c = blockIdx.x % lv.cps;
+    if (k >= lv.count) return;
+    const int node = lv.first + k;
+    const int2 r = lv.range[node];
+    const int s = r.x + c * kChunk;
+    if (s >= r.y) return;
+    const int e = min(r.y, s + kChunk);
+    for (int j = 0; j < d; ++j) {
+        uint32_t mn = 0xffffffffu, mx = 0u;
+        for (int p = s + threadIdx.x; p < e; p += kTB) {
+            const uint32_t idx = identity ? (uint32_t)p : key_idx(A[p]);
+            const uint32_t o = f2o(X[(size_t)idx * d + j]);
+            mn = min(mn, o);
+            mx = max(mx, o);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, 64));
+            mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            smin[threadIdx.x >> 6] = mn;
+            smax[threadIdx.x >> 6] = mx;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < kTB / 64; ++w) {
+                mn = min(mn, smin[w]);
+                mx = max(mx, smax[w]);
+            }
+            atomicMin(obox + (size_t)node * 2 * d + j, mn);
+            atomicMax(obox + (size_t)node * 2 * d + d + j, mx);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void bounds_init_kernel(int count, int first, int d, uint32_t *obox) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * 2 * d) return;
+    const int k = t / (2 * d), j = t % (2 * d);
+    obox[(size_t)(first + k) * 2 * d + j] = j < d ? 0xffffffffu : 0u;
+}
+
+// Per node: float bounds, split dimension (first strictly largest spread,
+// numba_kdtree.py:133-152) and the introselect state.
+__global__ void axis_kernel(LevelRef lv, int d, const uint32_t *__restrict__ obox, float *box, int *axis,
+                            SegState *seg, int internal) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= lv.count) return;
+    const int node = lv.first + k;
+    float best = 0.0f;
+    int ax = 0;
+    for (int j = 0; j < d; ++j) {
+        const float lo = o2f(obox[(size_t)node * 2 * d + j]);
+        const float hi = o2f(obox[(size_t)node * 2 * d + d + j]);
+        box[(size_t)node * 2 * d + j] = lo;
+        box[(size_t)node * 2 * d + d + j] = hi;
+        const float spread = hi - lo;
+        if (spread > best) {
+            best = spread;
+            ax = j;
+        }
+    }
+    if (!internal) return;
+    axis[k] = ax;
+    const int2 r = lv.range[node];
+    const int size = r.y - r.x;
+    SegState st;
+    st.L = r.x;
+    st.R = r.y;
+    st.nth = r.x + size / 2;
+    st.depth = size > 16 ? 2 * floor_log2(size) : 0;
+    st.flag = size > kSmall ? SEG_LARGE : SEG_SMALL;
+    st.P = 0;
+    st.S = 0;
+    st.M = 0;
+    st.pivot = 0;
+    seg[k] = st;
+}
+
+// Blocks over (node, chunk): keys for the node's split axis.
+__global__ __launch_bounds__(kTB) void keys_kernel(LevelRef lv, const float *__restrict__ X, int d,
+                                                   const int *__restrict__ axis, uint64_t *A, int identity) {
+    const int k = blockIdx.x / lv.cps, c = blockIdx.x % lv.cps;
+    if (k >= lv.count) return;
+    const int2 r = lv.range[lv.first + k];
+    const int s = r.x + c * kChunk;
+    if (s >= r.y) return;
+    const int e = min(r.y, s + kChunk);
+    const int ax = axis[k];
+    for (int p = s + threadIdx.x; p < e; p += kTB) {
+        const uint32_t idx = identity ? (uint32_t)p : key_idx(A[p]);
+        A[p] = make_key(X[(size_t)idx * d + ax], idx);
+    }
+}
+
+// ------------------------------------------- multi-block introselect ----
+
+__device__ __forceinline__ void swap64(uint64_t *A, int i, int j) {
+    uint64_t t = A[i];
+    A[i] = A[j];
+    A[j] = t;
+}
+
+// numba_kdtree.py:280-299 and the pivot move of 313-318.
+__device__ __forceinline__ void median3_and_move(uint64_t *A, int L, int R) {
+    const int mid = (L + R - 1) / 2;
+    if (A[L] > A[mid]) swap64(A, L, mid);
+    if (A[mid] > A[R - 1]) {
+        swap64(A, mid, R - 1);
+        if (A[L] > A[mid]) swap64(A, L, mid);
+    }
+    swap64(A, mid, R - 1);
+}
+
+__global__ void big_pivot_kernel(int count, SegState *seg, uint64_t *A, int *active) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    const int size = st.R - st.L;
+    if (size <= kSmall) {
+        seg[k].flag = SEG_SMALL;
+        return;
+    }
+    if (st.depth == 0) {
+        seg[k].flag = SEG_SORT;
+        return;
+    }
+    st.depth -= 1;
+    median3_and_move(A, st.L, st.R);
+    st.pivot = A[st.R - 1];
+    seg[k] = st;
+    atomicAdd(active, 1);
+}
+
+// Blocks over (segment, chunk of [L, R-1)): count of keys < pivot.
+__global__ __launch_bounds__(kTB) void big_count_kernel(int count, int cps, const SegState *__restrict__ seg,
+                                                        const uint64_t *__restrict__ A, int *cnt) {
+    __shared__ int ws[kTB / 64];
+    const int k = blockIdx.x / cps, c = blockIdx.x % cps;
+    if (k >= count) return;
+    const SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    const int s = st.L + c * kChunk;
+    const int e = min(st.R - 1, s + kChunk);
+    int v = 0;
+    for (int p = s + threadIdx.x; p < e; p += kTB) v += A[p] < st.pivot ? 1 : 0;
+    int tot;
+    block_excl(v, tot, ws);
+    if (threadIdx.x == 0) cnt[(size_t)k * cps + c] = s < e ? tot : 0;
+}
+
+// One block per segment: exclusive scan of its chunk counts; P and S.
+__global__ __launch_bounds__(kTB) void big_scan_kernel(int count, int cps, SegState *seg, int *cnt) {
+    __shared__ int ws[kTB / 64];
+    const int k = blockIdx.x;
+    if (k >= count) return;
+    SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    const int nch = (st.R - 1 - st.L + kChunk - 1) / kChunk;
+    int carry = 0;
+    for (int b0 = 0; b0 < nch; b0 += kTB) {
+        const int t = b0 + threadIdx.x;
+        const int v = t < nch ? cnt[(size_t)k * cps + t] : 0;
+        int tot;
+        const int ex = block_excl(v, tot, ws);
+        if (t < nch) cnt[(size_t)k * cps + t] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        seg[k].S = carry;
+        seg[k].P = st.L + carry;
+        seg[k].M = 0;  // written by the block holding position P when P < R-1
+    }
+}
+
+// Blocks over (segment, chunk): ranks of misplaced keys (the partners of
+// the reference's two-pointer exchanges).
+__global__ __launch_bounds__(kTB) void big_rank_kernel(int count, int cps, SegState *seg,
+                                                       const uint64_t *__restrict__ A, const int *__restrict__ cnt,
+                                                       int *gtpos, int *ltpos) {
+    __shared__ int ws[kTB / 64];
+    const int k = blockIdx.x / cps, c = blockIdx.x % cps;
+    if (k >= count) return;
+    const SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    const int s = st.L + c * kChunk;
+    const int e = min(st.R - 1, s + kChunk);
+    if (s >= e) return;
+    // each thread owns kPer consecutive elements
+    const int base = s + threadIdx.x * kPer;
+    uint32_t ltmask = 0;
+    int v = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int p = base + j;
+        const bool lt = p < e && A[p] < st.pivot;
+        ltmask |= (lt ? 1u : 0u) << j;
+        v += lt ? 1 : 0;
+    }
+    int tot;
+    int before = cnt[(size_t)k * cps + c] + block_excl(v, tot, ws);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int p = base + j;
+        if (p >= e) break;
+        const bool lt = (ltmask >> j) & 1u;
+        if (p == st.P) seg[k].M = (st.P - st.L) - before;
+        if (p < st.P) {
+            if (!lt) gtpos[st.L + (p - st.L) - before] = p;
+        } else if (lt) {
+            ltpos[st.L + (st.S - 1 - before)] = p;
+        }
+        before += lt ? 1 : 0;
+    }
+}
+
+// Blocks over (segment, chunk of pairs): the exchanges.
+__global__ __launch_bounds__(kTB) void big_swap_kernel(int count, int cps, const SegState *__restrict__ seg,
+                                                       uint64_t *A, const int *__restrict__ gtpos,
+                                                       const int *__restrict__ ltpos) {
+    const int k = blockIdx.x / cps, c = blockIdx.x % cps;
+    if (k >= count) return;
+    const SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    const int m0 = c * kChunk;
+    const int m1 = min(st.M, m0 + kChunk);
+    for (int m = m0 + threadIdx.x; m < m1; m += kTB) swap64(A, gtpos[st.L + m], ltpos[st.L + m]);
+}
+
+// Pivot into place (numba_kdtree.py:349) and narrowing (376-383).
+__global__ void big_finish_kernel(int count, SegState *seg, uint64_t *A) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    swap64(A, st.P, st.R - 1);
+    if (st.nth < st.P)
+        st.R = st.P;
+    else if (st.nth > st.P)
+        st.L = st.P + 1;
+    else
+        st.flag = SEG_DONE;
+    seg[k] = st;
+}
+
+// ------------------------------------------------- in-LDS introselect ----
+
+// Ascending sort of n keys in LDS with a bitonic network whose merge steps
+// all point the same way (the first step of each merge compares mirrored
+// positions), so positions >= n behave as +inf and need no storage.
+__device__ void lds_sort(uint64_t *sk, int n) {
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int size = 2; size <= m; size <<= 1) {
+        for (int t = threadIdx.x; t < m / 2; t += kTB) {
+            const int half = size >> 1;
+            const int blk = t / half, off = t % half;
+            const int lo = blk * size + off;
+            const int hi = blk * size + size - 1 - off;
+            if (hi < n && sk[lo] > sk[hi]) {
+                const uint64_t x = sk[lo];
+                sk[lo] = sk[hi];
+                sk[hi] = x;
+            }
+        }
+        __syncthreads();
+        for (int stride = size >> 2; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < m / 2; t += kTB) {
+                const int lo = 2 * stride * (t / stride) + (t % stride);
+                const int hi = lo + stride;
+                if (hi < n && sk[lo] > sk[hi]) {
+                    const uint64_t x = sk[lo];
+                    sk[lo] = sk[hi];
+                    sk[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// One block per node of the level: finish the introselect of segments that
+// are (or became) at most kSmall long.
+__global__ __launch_bounds__(kTB) void small_select_kernel(int count, SegState *seg, uint64_t *A) {
+    __shared__ uint64_t sk[kSmall];
+    __shared__ int gt_s[kSmall / 2], lt_s[kSmall / 2];
+    __shared__ int ws[kTB / 64];
+    __shared__ int sh[4];
+    const int k = blockIdx.x;
+    if (k >= count) return;
+    SegState st = seg[k];
+    if (st.flag != SEG_SMALL) return;
+    const int L0 = st.L, R0 = st.R;
+    const int n0 = R0 - L0;
+    for (int p = threadIdx.x; p < n0; p += kTB) sk[p] = A[L0 + p];
+    __syncthreads();
+    // local coordinates
+    int L = 0, R = n0, nth = st.nth - L0, depth = st.depth;
+    bool sort_rest = false;
+    for (;;) {
+        if (R - L <= 16) {
+            sort_rest = true;
+            break;
+        }
+        if (depth == 0) {
+            sort_rest = true;
+            break;
+        }
+        --depth;
+        if (threadIdx.x == 0) median3_and_move(sk, L, R);
+        __syncthreads();
+        const uint64_t pivot = sk[R - 1];
+        const int len = R - 1 - L;
+        const int per = (len + kTB - 1) / kTB;
+        const int b = L + threadIdx.x * per;
+        int v = 0;
+        for (int j = 0; j < per; ++j) {
+            const int p = b + j;
+            v += (p < R - 1 && sk[p] < pivot) ? 1 : 0;
+        }
+        int S;
+        int before = block_excl(v, S, ws);
+        const int P = L + S;
+        if (threadIdx.x == 0) sh[0] = 0;
+        __syncthreads();
+        for (int j = 0; j < per; ++j) {
+            const int p = b + j;
+            if (p >= R - 1) break;
+            const bool lt = sk[p] < pivot;
+            if (p == P) sh[0] = (P - L) - before;
+            if (p < P) {
+                if (!lt) gt_s[(p - L) - before] = p;
+            } else if (lt) {
+                lt_s[S - 1 - before] = p;
+            }
+            before += lt ? 1 : 0;
+        }
+        __syncthreads();
+        const int M = sh[0];
+        for (int m = threadIdx.x; m < M; m += kTB) {
+            const int a = gt_s[m], c = lt_s[m];
+            const uint64_t t = sk[a];
+            sk[a] = sk[c];
+            sk[c] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint64_t t = sk[P];
+            sk[P] = sk[R - 1];
+            sk[R - 1] = t;
+        }
+        __syncthreads();
+        if (nth < P)
+            R = P;
+        else if (nth > P)
+            L = P + 1;
+        else
+            break;
+    }
+    if (sort_rest) {
+        // insertion sort (<= 16) or heapsort at the depth limit: both leave
+        // [L, R) ascending
+        lds_sort(sk + L, R - L);
+    }
+    for (int p = threadIdx.x; p < n0; p += kTB) A[L0 + p] = sk[p];
+    seg[k].flag = SEG_DONE;
+}
+
+// ----------------------------------------------------------- output ----
+
+__global__ void finalize_kernel(int n, int d, const float *__restrict__ X, const uint64_t *__restrict__ A, int *perm,
+                                int *posof, float *xp) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int idx = (int)key_idx(A[p]);
+    perm[p] = idx;
+    posof[idx] = p;
+    for (int j = 0; j < d; ++j) xp[(size_t)p * d + j] = X[(size_t)idx * d + j];
+}
+
+__global__ void identity_keys_kernel(int n, uint64_t *A) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) A[p] = (uint64_t)p;
+}
+
+// One compare-exchange stage of the same-direction bitonic network over
+// A[L, L+n) in global memory (positions >= n act as +inf).  Used only for a
+// range that reaches the introselect depth limit while still longer than
+// kSmall (the reference then heap-sorts it).
+__global__ void gsort_stage_kernel(uint64_t *A, int n, int m, int size, int stride) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m / 2) return;
+    int lo, hi;
+    if (stride == 0) {
+        const int half = size >> 1;
+        lo = (t / half) * size + (t % half);
+        hi = (t / half) * size + size - 1 - (t % half);
+    } else {
+        lo = 2 * stride * (t / stride) + (t % stride);
+        hi = lo + stride;
+    }
+    if (hi < n && A[lo] > A[hi]) {
+        const uint64_t x = A[lo];
+        A[lo] = A[hi];
+        A[hi] = x;
+    }
+}
+
+inline unsigned nblk(int64_t m, int b) { return (unsigned)((m + b - 1) / b); }
+
+}  // namespace
+
+int tree_small_limit() { return kSmall; }
+int tree_chunk() { return kChunk; }
+
+hipError_t tree_level_bounds(const TreeBuildArgs &a, int level, int max_size, int identity, hipStream_t st) {
+    LevelRef lv{a.range, (1 << level) - 1, 1 << level, (max_size + kChunk - 1) / kChunk};
+    if (lv.cps < 1) lv.cps = 1;
+    hipLaunchKernelGGL(bounds_init_kernel, dim3(nblk((int64_t)lv.count * 2 * a.d, 256)), dim3(256), 0, st, lv.count,
+                       lv.first, a.d, a.obox);
+    hipLaunchKernelGGL(bounds_kernel, dim3((unsigned)(lv.count * lv.cps)), dim3(kTB), 0, st, lv, a.X, a.d, a.A,
+                       identity, a.obox);
+    return hipGetLastError();
+}
+
+hipError_t tree_level_axis(const TreeBuildArgs &a, int level, int internal, hipStream_t st) {
+    LevelRef lv{a.range, (1 << level) - 1, 1 << level, 1};
+    hipLaunchKernelGGL(axis_kernel, dim3(nblk(lv.count, 256)), dim3(256), 0, st, lv, a.d, a.obox, a.box, a.axis,
+                       a.seg, internal);
+    return hipGetLastError();
+}
+
+hipError_t tree_level_keys(const TreeBuildArgs &a, int level, int max_size, int identity, hipStream_t st) {
+    LevelRef lv{a.range, (1 << level) - 1, 1 << level, (max_size + kChunk - 1) / kChunk};
+    if (lv.cps < 1) lv.cps = 1;
+    hipLaunchKernelGGL(keys_kernel, dim3((unsigned)(lv.count * lv.cps)), dim3(kTB), 0, st, lv, a.X, a.d, a.axis, a.A,
+                       identity);
+    return hipGetLastError();
+}
+
+hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStream_t st) {
+    const int cps = (max_size + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(big_pivot_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.active);
+    hipLaunchKernelGGL(big_count_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
+                       a.cnt);
+    hipLaunchKernelGGL(big_scan_kernel, dim3((unsigned)count), dim3(kTB), 0, st, count, cps, a.seg, a.cnt);
+    hipLaunchKernelGGL(big_rank_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
+                       a.cnt, a.gtpos, a.ltpos);
+    hipLaunchKernelGGL(big_swap_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
+                       a.gtpos, a.ltpos);
+    hipLaunchKernelGGL(big_finish_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A);
+    return hipGetLastError();
+}
+
+hipError_t tree_small_select(const TreeBuildArgs &a, int count, hipStream_t st) {
+    hipLaunchKernelGGL(small_select_kernel, dim3((unsigned)count), dim3(kTB), 0, st, count, a.seg, a.A);
+    return hipGetLastError();
+}
+
+hipError_t tree_sort_range(uint64_t *A, int L, int R, hipStream_t st) {
+    const int n = R - L;
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int size = 2; size <= m; size <<= 1) {
+        hipLaunchKernelGGL(gsort_stage_kernel, dim3(nblk(m / 2, 256)), dim3(256), 0, st, A + L, n, m, size, 0);
+        for (int stride = size >> 2; stride > 0; stride >>= 1)
+            hipLaunchKernelGGL(gsort_stage_kernel, dim3(nblk(m / 2, 256)), dim3(256), 0, st, A + L, n, m, size,
+                               stride);
+    }
+    return hipGetLastError();
+}
+
+hipError_t tree_identity_keys(int n, uint64_t *A, hipStream_t st) {
+    hipLaunchKernelGGL(identity_keys_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, n, A);
+    return hipGetLastError();
+}
+
+hipError_t tree_finalize(const TreeBuildArgs &a, int n, int *perm, int *posof, float *xp, hipStream_t st) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, n, a.d, a.X, a.A, perm, posof, xp);
+    return hipGetLastError();
+}
+
+}  // namespace fh

@@ -196,9 +196,11 @@ def parallel_tree_query(data, queries, k=10, output_rdist=False, device: int = 0
     return np.sqrt(dist), idx
 
 
-def build_kdtree(data, device: int = 0):
+def build_kdtree(data, on_device: bool = False, device: int = 0):
     """numba_kdtree.build_kdtree(data, leaf_size=40) (numba_kdtree.py:478-521)
-    as a dict of arrays with the reference's dtypes."""
+    as a dict of arrays with the reference's dtypes.  ``on_device`` builds
+    it with the gfx950 kernels (the MST path's builder); otherwise with the
+    library's host builder."""
     X = _as_points(data)
     n, d = X.shape
     L = _lib.load()
@@ -212,11 +214,13 @@ def build_kdtree(data, device: int = 0):
         "is_leaf": np.empty(N, np.uint8),
         "node_bounds": np.empty((2, N, d), np.float32),
     }
-    ctx = None  # host-only entry point: no device context needed
+    ctx = _lib.context(device) if on_device else None
     code = L.fh_build_kdtree(ctx, _lib.ptr(X), n, d, _lib.ptr(out["idx_array"]), _lib.ptr(out["idx_start"]),
                              _lib.ptr(out["idx_end"]), _lib.ptr(out["radius"]), _lib.ptr(out["is_leaf"]),
                              _lib.ptr(out["node_bounds"]))
-    if code != _lib.FH_OK:
+    if ctx is not None:
+        _lib.check(ctx, code)
+    elif code != _lib.FH_OK:
         raise ValueError(f"fh_build_kdtree failed with code {code}")
     out["is_leaf"] = out["is_leaf"].astype(bool)
     return out

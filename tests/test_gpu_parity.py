@@ -42,6 +42,54 @@ def _check(X, ms):
     return stats
 
 
+def _tree_case(name):
+    rng = np.random.default_rng(0)
+    if name == "blobs":
+        return workloads.blobs_small(20_000, seed=0)
+    if name == "scale2d_2M":
+        return workloads.scale_2d(2_000_000, seed=1)
+    if name == "scale3d_300k":
+        return workloads.anisotropic_mixture(300_000, 3, 20, 0.05, seed=2)
+    if name == "d8":
+        return workloads.anisotropic_mixture(50_000, 8, 10, 0.05, seed=4)
+    if name == "dups":
+        X = workloads.blobs_small(30_000, seed=3)
+        X[15000:] = X[:15000]
+        return X
+    if name == "grid":
+        g = np.stack(np.meshgrid(np.arange(300), np.arange(200)), -1).reshape(-1, 2).astype(np.float32)
+        return g[rng.permutation(g.shape[0])]
+    if name == "sorted":
+        return np.sort(rng.standard_normal((100_000, 2)).astype(np.float32), axis=0)
+    if name == "reversed":
+        return np.sort(rng.standard_normal((100_000, 2)).astype(np.float32), axis=0)[::-1].copy()
+    if name == "organ_pipe":
+        v = np.concatenate([np.arange(40_000), np.arange(40_000)[::-1]]).astype(np.float32)
+        return np.stack([v, np.zeros_like(v)], 1)
+    if name == "constant_axis":
+        X = rng.standard_normal((60_000, 2)).astype(np.float32)
+        X[:, 1] = 3.0
+        return X
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["blobs", "scale2d_2M", "scale3d_300k", "d8", "dups", "grid", "sorted",
+                                  "reversed", "organ_pipe", "constant_axis"])
+def test_device_tree_matches_oracle(name):
+    from fast_hdbscan_amd.hdbscan import build_kdtree
+    from oracle import oracle as O
+
+    X = _tree_case(name)
+    print(f"tree case {name} n={X.shape[0]} d={X.shape[1]}", flush=True)
+    t = build_kdtree(X, on_device=True)
+    o = O.build_kdtree(X)
+    np.testing.assert_array_equal(t["idx_array"], o.idx_array)
+    np.testing.assert_array_equal(t["idx_start"], o.idx_start)
+    np.testing.assert_array_equal(t["idx_end"], o.idx_end)
+    np.testing.assert_array_equal(t["is_leaf"], o.is_leaf)
+    np.testing.assert_array_equal(t["node_bounds"], o.node_bounds)
+
+
 def test_config1_blobs_10k():
     _check(workloads.blobs_small(10_000, seed=0), 10)
 
