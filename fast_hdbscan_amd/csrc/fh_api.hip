@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -31,6 +32,8 @@ using fh::HostTree;
 namespace {
 
 constexpr int64_t kLeafSize = 40;  // numba_kdtree.py:478 default, used by hdbscan.py:466
+constexpr int kCertDepth = 4;      // certification nodes sit this many levels above the leaves
+constexpr int kSampleMask = 63;    // one sampled search per 64 tree positions
 
 template <class T>
 struct DevBuf {
@@ -64,6 +67,11 @@ struct fh_ctx {
     DevBuf<uint32_t> obox;
     DevBuf<int> taxis, tcnt, gtpos, ltpos, tactive;
     DevBuf<fh::SegState> tseg;
+    // pruned nearest-edge search
+    DevBuf<int> nb_pos, cert_of_pos;
+    DevBuf<float> nb_d, mincore;
+    DevBuf<unsigned char> cert;
+    int search_mode = 1;  // 0: one plain pass; 1: seeded + sampled + certified
     int64_t geom_n = -1;            // n the uploaded node ranges belong to
     std::vector<int2> geom_range;   // host copy of the node ranges
     std::vector<int> geom_maxsize;  // largest node per level
@@ -245,8 +253,11 @@ int knn_block(int k) {
 // kNN of `nq` query rows against the uploaded tree.  Self-query when
 // query_perm != nullptr (queries = tree-order rows).
 int run_knn(fh_ctx *ctx, int d, int levels, const float *queries, const int *query_perm, int64_t nq, int k,
-            float *out_d, int *out_i, float *core_o, float *core_p, int core_from_knn) {
+            float *out_d, int *out_i, float *core_o, float *core_p, int core_from_knn, int *nb_pos = nullptr,
+            float *nb_d = nullptr) {
     fh::KnnArgs a{};
+    a.nb_pos = nb_pos;
+    a.nb_d = nb_d;
     a.xp = ctx->xp.p;
     a.range = ctx->range.p;
     a.box = ctx->box.p;
@@ -330,8 +341,21 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
     FH_TRY(ensure(ctx, ctx->small, 8));
 
     // -- core distances
+    const bool pruned = ctx->search_mode != 0;
+    if (pruned) {
+        FH_TRY(ensure(ctx, ctx->nb_pos, (size_t)n * (kq - 1)));
+        FH_TRY(ensure(ctx, ctx->nb_d, (size_t)n * (kq - 1)));
+    }
     FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p, ctx->core_p.p,
-                   ms > 1 ? 1 : 0));
+                   ms > 1 ? 1 : 0, pruned ? ctx->nb_pos.p : nullptr, pruned ? ctx->nb_d.p : nullptr));
+    // certification nodes: kCertDepth levels above the leaves
+    const int cert_level = std::max(0, levels - 1 - kCertDepth);
+    if (pruned) {
+        FH_TRY(ensure(ctx, ctx->cert_of_pos, n));
+        FH_TRY(ensure(ctx, ctx->mincore, int64_t(1) << cert_level));
+        FH_TRY(ensure(ctx, ctx->cert, int64_t(1) << cert_level));
+        FH_HIP(fh::launch_cert_setup(cert_level, ctx->range.p, ctx->core_p.p, ctx->cert_of_pos.p, ctx->mincore.p, st));
+    }
     FH_HIP(hipEventRecord(ctx->ev[2], st));
 
     // -- kNN-seeded forest
@@ -366,13 +390,43 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
     qa.n = n;
     qa.d = d;
     qa.levels = levels;
+    qa.pass = 0;
+    qa.sample_mask = kSampleMask;
+    qa.cert_of_pos = pruned ? ctx->cert_of_pos.p : nullptr;
+    qa.cert = pruned ? ctx->cert.p : nullptr;
+    fh::CertArgs ca{};
+    ca.xp = ctx->xp.p;
+    ca.range = ctx->range.p;
+    ca.box = ctx->box.p;
+    ca.core_p = ctx->core_p.p;
+    ca.comp_p = ctx->comp_p.p;
+    ca.node_comp = ctx->node_comp.p;
+    ca.bound = ctx->bound.p;
+    ca.mincore = pruned ? ctx->mincore.p : nullptr;
+    ca.cert = pruned ? ctx->cert.p : nullptr;
+    ca.level = cert_level;
+    ca.levels = levels;
+    ca.d = d;
     double query_ms = 0.0;
     int rounds = 0;
     while (C > 1) {
         FH_HIP(fh::launch_round_reset(C, ctx->roots.p, ctx->bound.p, ctx->best.p, st));
         FH_HIP(fh::launch_purity(levels, ctx->range.p, ctx->comp_p.p, ctx->node_comp.p, st));
         FH_HIP(hipEventRecord(ctx->ev[5], st));
-        FH_HIP(fh::launch_query(qa, st));
+        if (pruned) {
+            // bounds from kNN cross edges, then a sampled search, then
+            // certification of whole subtrees, then everyone else
+            FH_HIP(fh::launch_seed_bounds(n, kq - 1, ctx->nb_pos.p, ctx->nb_d.p, ctx->core_p.p, ctx->comp_p.p,
+                                          ctx->bound.p, st));
+            qa.pass = 1;
+            FH_HIP(fh::launch_query(qa, st));
+            FH_HIP(fh::launch_cert(ca, st));
+            qa.pass = 2;
+            FH_HIP(fh::launch_query(qa, st));
+        } else {
+            qa.pass = 0;
+            FH_HIP(fh::launch_query(qa, st));
+        }
         FH_HIP(hipEventRecord(ctx->ev[6], st));
         FH_HIP(fh::launch_select(C, ni, ctx->roots.p, ctx->best.p, ctx->partner.p, ctx->posof.p, ctx->perm.p,
                                  ctx->comp_p.p, ctx->rootslot.p, ctx->succ.p, ctx->ei.p, ctx->ej.p, ctx->ew.p, st));
@@ -441,6 +495,7 @@ fh_ctx *fh_create(int device) {
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return nullptr;
     fh_ctx *c = new fh_ctx();
     c->device = device;
+    if (const char *m = std::getenv("FH_SEARCH_MODE")) c->search_mode = std::atoi(m);
     if (hipSetDevice(device) != hipSuccess) {
         delete c;
         return nullptr;
@@ -479,6 +534,7 @@ void fh_destroy(fh_ctx *c) {
     release(c->out_nbr); release(c->range); release(c->best);
     release(c->xdev); release(c->tkeys); release(c->obox); release(c->taxis); release(c->tcnt);
     release(c->gtpos); release(c->ltpos); release(c->tactive); release(c->tseg);
+    release(c->nb_pos); release(c->cert_of_pos); release(c->nb_d); release(c->mincore); release(c->cert);
     if (c->h_small) e = hipHostFree(c->h_small);
     for (auto &ev : c->ev)
         if (ev) e = hipEventDestroy(ev);

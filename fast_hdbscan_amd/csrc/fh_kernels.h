@@ -26,6 +26,8 @@ struct KnnArgs {
     float *heap_p;          // global heap slices for large k (nullptr: LDS)
     int *heap_i;
     int64_t heap_stride;
+    int *nb_pos;            // optional: [nq, k-1] neighbour tree positions (columns 1..k-1)
+    float *nb_d;            // optional: [nq, k-1] their squared distances
 };
 
 struct QueryArgs {
@@ -42,6 +44,27 @@ struct QueryArgs {
     int64_t n;
     int d;
     int levels;
+    int pass;                    // 0: every point; 1: sampled points; 2: the rest, minus certified subtrees
+    int sample_mask;             // 2^s - 1: position p is sampled iff (p & mask) == mask >> 1
+    const int *cert_of_pos;      // certification node (in-level index) of each position
+    const unsigned char *cert;   // 1: the node's points cannot beat their component's bound
+};
+
+// Subtree certification: nodes of one level whose points provably have no
+// out-of-component candidate at or below the component bound.
+struct CertArgs {
+    const float *xp;
+    const int2 *range;
+    const float *box;
+    const float *core_p;
+    const int *comp_p;
+    const int *node_comp;
+    const float *bound;
+    const float *mincore;        // per certification node
+    unsigned char *cert;
+    int level;                   // tree level of the certification nodes
+    int levels;
+    int d;
 };
 
 struct CycleBufs {
@@ -49,6 +72,11 @@ struct CycleBufs {
 };
 
 hipError_t launch_knn(const KnnArgs &a, hipStream_t st);
+hipError_t launch_cert(const CertArgs &a, hipStream_t st);
+hipError_t launch_cert_setup(int level, const int2 *range, const float *core_p, int *cert_of_pos, float *mincore,
+                             hipStream_t st);
+hipError_t launch_seed_bounds(int64_t n, int kn, const int *nb_pos, const float *nb_d, const float *core_p,
+                              const int *comp_p, float *bound, hipStream_t st);
 hipError_t launch_query(const QueryArgs &a, hipStream_t st);
 hipError_t launch_posof(int n, const int *perm, int *posof, hipStream_t st);
 hipError_t launch_neighbors(int64_t n, int k, const int *knn_i, int *nbr, hipStream_t st);

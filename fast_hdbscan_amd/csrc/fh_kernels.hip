@@ -160,6 +160,12 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
         a.out_d[row * k + j] = h.P(j);
         a.out_i[row * k + j] = pj >= 0 ? a.perm[pj] : -1;
     }
+    if (a.nb_pos != nullptr) {
+        for (int j = 1; j < k; ++j) {
+            a.nb_pos[q * (k - 1) + j - 1] = h.I(j);
+            a.nb_d[q * (k - 1) + j - 1] = h.P(j);
+        }
+    }
     if (a.core_o != nullptr) {
         const float c = a.core_from_knn ? h.P(k - 1) : 0.0f;
         a.core_o[row] = c;
@@ -225,6 +231,13 @@ template <int D>
 __global__ __launch_bounds__(256) void boruvka_query_kernel(QueryArgs a) {
     const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= a.n) return;
+    if (a.pass != 0) {
+        // pass 1: one position per group of sample_mask+1; pass 2: the
+        // others, except those inside certified subtrees
+        const bool sampled = (pos & a.sample_mask) == (a.sample_mask >> 1);
+        if ((a.pass == 1) != sampled) return;
+        if (a.pass == 2 && a.cert[a.cert_of_pos[pos]]) return;
+    }
     const int dd = D > 0 ? D : a.d;
     const int comp = a.comp_p[pos];
     BoruvkaVisitor<D> v;
@@ -252,6 +265,138 @@ __global__ __launch_bounds__(256) void boruvka_query_kernel(QueryArgs a) {
         atomicMin(a.best + comp, key);
         a.partner[pos] = v.best_j;
     }
+}
+
+// ------------------------------------------------ bound tightening ----
+//
+// The per-component bound only has to stay >= the component's minimum
+// outgoing weight for the searches to remain exact (every prune compares
+// strictly against it), so any real cross-component edge may lower it.
+
+// Cross-component edges among the kNN lists.  The distances are the kNN
+// kernel's rdist of the same two rows, bit-identical to what the search
+// computes for that pair.
+__global__ void seed_bounds_kernel(int64_t n, int kn, const int *__restrict__ nb_pos, const float *__restrict__ nb_d,
+                                   const float *__restrict__ core_p, const int *__restrict__ comp_p, float *bound) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int c = comp_p[p];
+    const float ci = core_p[p];
+    float w = kInf;
+    for (int j = 0; j < kn; ++j) {
+        const int q = nb_pos[p * kn + j];
+        if (q < 0 || comp_p[q] == c) continue;
+        float e = nb_d[p * kn + j];
+        e = e > ci ? e : ci;
+        const float cq = core_p[q];
+        e = e > cq ? e : cq;
+        w = e < w ? e : w;
+    }
+    if (w < kInf) atomicMin(reinterpret_cast<unsigned int *>(bound + c), __float_as_uint(w));
+}
+
+// Which certification node (in-level index at `level`) holds each position,
+// and the smallest core distance inside each such node.
+__global__ void cert_setup_kernel(int level, const int2 *__restrict__ range, const float *__restrict__ core_p,
+                                  int *cert_of_pos, float *mincore) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (1 << level)) return;
+    const int2 r = range[(1 << level) - 1 + k];
+    float m = kInf;
+    for (int p = r.x; p < r.y; ++p) {
+        cert_of_pos[p] = k;
+        m = core_p[p] < m ? core_p[p] : m;
+    }
+    mincore[k] = m;
+}
+
+// Lower bound of the squared distance between two boxes, in the same
+// sequential fused form as rdist so it never exceeds rdist of any pair of
+// points inside them.
+template <int D>
+__device__ __forceinline__ float box_box_lb(const float *__restrict__ a, const float *__restrict__ b, int d) {
+    const int dd = D > 0 ? D : d;
+    float r = 0.0f;
+#pragma unroll
+    for (int j = 0; j < dd; ++j) {
+        const float alo = a[j], ahi = a[dd + j], blo = b[j], bhi = b[dd + j];
+        const float g1 = blo > ahi ? blo - ahi : 0.0f;
+        const float g2 = alo > bhi ? alo - bhi : 0.0f;
+        const float g = g1 + g2;
+        r = __builtin_fmaf(g, g, r);
+    }
+    return r;
+}
+
+// One lane per certification node P of a single component C: walk the tree
+// with the box-to-box bound and fail as soon as some point j outside C could
+// give a weight max(d(box P, x_j), core_j, min core of P) <= B_C.  A
+// certified node's points cannot reach B_C (nor, therefore, the component
+// minimum), so the main search skips them.
+template <int D>
+__global__ __launch_bounds__(256) void cert_kernel(CertArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (1 << a.level)) return;
+    const int dd = D > 0 ? D : a.d;
+    const int stride = 2 * dd;
+    const int P = (1 << a.level) - 1 + k;
+    const int C = a.node_comp[P];
+    unsigned char ok = 0;
+    if (C >= 0) {
+        const float B = ld_relaxed(a.bound + C);
+        const float mc = a.mincore[k];
+        if (mc > B) {
+            ok = 1;
+        } else if (B < kInf) {
+            float pb[2 * (D > 0 ? D : 1)];
+            const float *boxP = a.box + (size_t)P * stride;
+            if constexpr (D > 0) {
+#pragma unroll
+                for (int j = 0; j < 2 * D; ++j) pb[j] = boxP[j];
+            }
+            const float *bp = D > 0 ? pb : boxP;
+            bool fail = false;
+            int node = 0, depth = 0;
+            bool entering = true;
+            for (;;) {
+                if (entering && !fail) {
+                    const float lb = box_box_lb<D>(bp, a.box + (size_t)node * stride, a.d);
+                    if (!(lb > B) && a.node_comp[node] != C) {
+                        if (depth == a.levels - 1) {
+                            const int2 r = a.range[node];
+                            for (int j = r.x; j < r.y; ++j) {
+                                if (a.comp_p[j] == C) continue;
+                                const float cj = a.core_p[j];
+                                if (!(cj <= B)) continue;
+                                float w = box_lb<D>(a.xp + (size_t)j * dd, bp, a.d);
+                                w = w > cj ? w : cj;
+                                w = w > mc ? w : mc;
+                                if (!(w > B)) {
+                                    fail = true;
+                                    break;
+                                }
+                            }
+                        } else {
+                            node = 2 * node + 1;
+                            ++depth;
+                            continue;
+                        }
+                    }
+                }
+                if (fail || depth == 0) break;
+                if (node & 1) {  // left child done: go to its sibling
+                    node += 1;
+                    entering = true;
+                } else {
+                    node = (node - 1) >> 1;
+                    --depth;
+                    entering = false;
+                }
+            }
+            ok = fail ? 0 : 1;
+        }
+    }
+    a.cert[k] = ok;
 }
 
 // ------------------------------------------------- kNN-seeded forest ----
@@ -669,6 +814,29 @@ static hipError_t launch_query_t(const QueryArgs &a, hipStream_t st) {
     }
 
 hipError_t launch_knn(const KnnArgs &a, hipStream_t st) { FH_DISPATCH_D(a.d, launch_knn_t, a, st) }
+
+template <int D>
+static hipError_t launch_cert_t(const CertArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(cert_kernel<D>, dim3(blocks(int64_t(1) << a.level, 64)), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cert(const CertArgs &a, hipStream_t st) { FH_DISPATCH_D(a.d, launch_cert_t, a, st) }
+
+hipError_t launch_cert_setup(int level, const int2 *range, const float *core_p, int *cert_of_pos, float *mincore,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(cert_setup_kernel, dim3(blocks(int64_t(1) << level, 64)), dim3(64), 0, st, level, range,
+                       core_p, cert_of_pos, mincore);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_bounds(int64_t n, int kn, const int *nb_pos, const float *nb_d, const float *core_p,
+                              const int *comp_p, float *bound, hipStream_t st) {
+    if (kn <= 0) return hipSuccess;
+    hipLaunchKernelGGL(seed_bounds_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, kn, nb_pos, nb_d, core_p,
+                       comp_p, bound);
+    return hipGetLastError();
+}
 
 hipError_t launch_query(const QueryArgs &a, hipStream_t st) { FH_DISPATCH_D(a.d, launch_query_t, a, st) }
 
