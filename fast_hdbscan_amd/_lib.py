@@ -32,6 +32,8 @@ EXPORTED_SYMBOLS = (
     "fh_knn",
     "fh_tree_nnodes",
     "fh_build_kdtree",
+    "fh_labels_from_sorted_mst",
+    "fh_sort_mst_device",
 )
 
 
@@ -47,11 +49,33 @@ class FHStats(ctypes.Structure):
         ("t_init_ms", ctypes.c_double),
         ("t_rounds_ms", ctypes.c_double),
         ("t_query_ms", ctypes.c_double),
+        ("t_query_main_ms", ctypes.c_double),
         ("t_total_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class FHLabelParams(ctypes.Structure):
+    _fields_ = [
+        ("min_cluster_size", ctypes.c_int64),
+        ("max_cluster_size", ctypes.c_double),
+        ("cluster_selection_epsilon", ctypes.c_double),
+        ("selection_method", ctypes.c_int32),
+        ("allow_single_cluster", ctypes.c_int32),
+    ]
+
+
+class FHCondensedTree(ctypes.Structure):
+    _fields_ = [
+        ("parent", ctypes.c_void_p),
+        ("child", ctypes.c_void_p),
+        ("lambda_val", ctypes.c_void_p),
+        ("child_size", ctypes.c_void_p),
+        ("capacity", ctypes.c_int64),
+        ("rows", ctypes.c_int64),
+    ]
 
 
 class FHError(RuntimeError):
@@ -100,6 +124,11 @@ def load() -> ctypes.CDLL:
         L.fh_tree_nnodes.argtypes = [_i64]
         L.fh_build_kdtree.restype = ctypes.c_int
         L.fh_build_kdtree.argtypes = [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.fh_labels_from_sorted_mst.restype = ctypes.c_int
+        L.fh_labels_from_sorted_mst.argtypes = [_i64, _vp, _vp, _vp, ctypes.POINTER(FHLabelParams), _vp, _vp, _vp,
+                                                ctypes.POINTER(FHCondensedTree)]
+        L.fh_sort_mst_device.restype = ctypes.c_int
+        L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 

@@ -24,6 +24,7 @@
 
 #include "../../include/fh.h"
 #include "fh_kernels.h"
+#include "fh_sort.h"
 #include "fh_tree.h"
 #include "kdtree_build.h"
 
@@ -72,12 +73,17 @@ struct fh_ctx {
     DevBuf<float> nb_d, mincore;
     DevBuf<unsigned char> cert;
     int search_mode = 1;  // 0: one plain pass; 1: seeded + sampled + certified
+    // edge sort
+    DevBuf<uint32_t> sk1, sk1s;
+    DevBuf<uint64_t> sk2, sk2s;
+    DevBuf<int> sv1, sv1s;
+    DevBuf<unsigned char> stemp;
     int64_t geom_n = -1;            // n the uploaded node ranges belong to
     std::vector<int2> geom_range;   // host copy of the node ranges
     std::vector<int> geom_maxsize;  // largest node per level
     int geom_levels = 0;
     int *h_small = nullptr;  // pinned readback slots
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
 };
 
 namespace {
@@ -407,7 +413,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
     ca.level = cert_level;
     ca.levels = levels;
     ca.d = d;
-    double query_ms = 0.0;
+    double query_ms = 0.0, query_main_ms = 0.0;
     int rounds = 0;
     while (C > 1) {
         FH_HIP(fh::launch_round_reset(C, ctx->roots.p, ctx->bound.p, ctx->best.p, st));
@@ -422,10 +428,14 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
             FH_HIP(fh::launch_query(qa, st));
             FH_HIP(fh::launch_cert(ca, st));
             qa.pass = 2;
+            FH_HIP(hipEventRecord(ctx->ev[8], st));
             FH_HIP(fh::launch_query(qa, st));
+            FH_HIP(hipEventRecord(ctx->ev[9], st));
         } else {
             qa.pass = 0;
+            FH_HIP(hipEventRecord(ctx->ev[8], st));
             FH_HIP(fh::launch_query(qa, st));
+            FH_HIP(hipEventRecord(ctx->ev[9], st));
         }
         FH_HIP(hipEventRecord(ctx->ev[6], st));
         FH_HIP(fh::launch_select(C, ni, ctx->roots.p, ctx->best.p, ctx->partner.p, ctx->posof.p, ctx->perm.p,
@@ -440,6 +450,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
                                         st));
         FH_TRY(read_small(ctx, 2));
         query_ms += event_ms(ctx->ev[5], ctx->ev[6]);
+        query_main_ms += event_ms(ctx->ev[8], ctx->ev[9]);
         const int emitted = ctx->h_small[0];
         const int C2 = ctx->h_small[1];
         if (emitted != C - C2 || C2 >= C)
@@ -466,6 +477,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
     S.t_init_ms = event_ms(ctx->ev[2], ctx->ev[3]);
     S.t_rounds_ms = event_ms(ctx->ev[3], ctx->ev[4]);
     S.t_query_ms = query_ms;
+    S.t_query_main_ms = query_main_ms;
     S.t_total_ms = total.ms();
     if (base != n - 1)
         return fail(ctx, FH_EDEVICE, "spanning tree has " + std::to_string(base) + " edges for " +
@@ -535,6 +547,8 @@ void fh_destroy(fh_ctx *c) {
     release(c->xdev); release(c->tkeys); release(c->obox); release(c->taxis); release(c->tcnt);
     release(c->gtpos); release(c->ltpos); release(c->tactive); release(c->tseg);
     release(c->nb_pos); release(c->cert_of_pos); release(c->nb_d); release(c->mincore); release(c->cert);
+    release(c->sk1); release(c->sk1s); release(c->sk2); release(c->sk2s); release(c->sv1); release(c->sv1s);
+    release(c->stemp);
     if (c->h_small) e = hipHostFree(c->h_small);
     for (auto &ev : c->ev)
         if (ev) e = hipEventDestroy(ev);
@@ -659,6 +673,43 @@ int fh_build_kdtree(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int64_t *
         }
         radius[i] = std::sqrt(r);
     }
+    return FH_OK;
+}
+
+int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t *dst, const float *w_sq,
+                       int32_t *src_out, int32_t *dst_out, float *w_out) {
+    if (!ctx) return FH_EINVAL;
+    if (m < 0 || m > (int64_t(1) << 30)) return fail(ctx, FH_EINVAL, "bad row count");
+    if (m == 0) return FH_OK;
+    if (!src || !dst || !w_sq || !src_out || !dst_out || !w_out) return fail(ctx, FH_EINVAL, "null pointer");
+    FH_HIP(hipSetDevice(ctx->device));
+    const int mi = static_cast<int>(m);
+    FH_TRY(ensure(ctx, ctx->sk1, m));
+    FH_TRY(ensure(ctx, ctx->sk1s, m));
+    FH_TRY(ensure(ctx, ctx->sk2, m));
+    FH_TRY(ensure(ctx, ctx->sk2s, m));
+    FH_TRY(ensure(ctx, ctx->sv1, m));
+    FH_TRY(ensure(ctx, ctx->sv1s, m));
+    const size_t tb = fh::sort_mst_temp_bytes(mi);
+    FH_TRY(ensure(ctx, ctx->stemp, tb));
+    fh::SortMstArgs a{};
+    a.m = mi;
+    a.src = src;
+    a.dst = dst;
+    a.w = w_sq;
+    a.src_out = src_out;
+    a.dst_out = dst_out;
+    a.w_out = w_out;
+    a.k1 = ctx->sk1.p;
+    a.k1s = ctx->sk1s.p;
+    a.k2 = ctx->sk2.p;
+    a.k2s = ctx->sk2s.p;
+    a.v1 = ctx->sv1.p;
+    a.v1s = ctx->sv1s.p;
+    a.temp = ctx->stemp.p;
+    a.temp_bytes = ctx->stemp.cap;
+    FH_HIP(fh::sort_mst(a, ctx->stream));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
     return FH_OK;
 }
 

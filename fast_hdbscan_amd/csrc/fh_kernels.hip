@@ -227,16 +227,16 @@ struct BoruvkaVisitor {
     }
 };
 
-template <int D>
+// PASS 0: every point; 1: one position per group of sample_mask+1;
+// 2: the others, except those inside certified subtrees.
+template <int D, int PASS>
 __global__ __launch_bounds__(256) void boruvka_query_kernel(QueryArgs a) {
     const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= a.n) return;
-    if (a.pass != 0) {
-        // pass 1: one position per group of sample_mask+1; pass 2: the
-        // others, except those inside certified subtrees
+    if constexpr (PASS != 0) {
         const bool sampled = (pos & a.sample_mask) == (a.sample_mask >> 1);
-        if ((a.pass == 1) != sampled) return;
-        if (a.pass == 2 && a.cert[a.cert_of_pos[pos]]) return;
+        if ((PASS == 1) != sampled) return;
+        if (PASS == 2 && a.cert[a.cert_of_pos[pos]]) return;
     }
     const int dd = D > 0 ? D : a.d;
     const int comp = a.comp_p[pos];
@@ -796,7 +796,12 @@ static hipError_t launch_knn_t(const KnnArgs &a, hipStream_t st) {
 
 template <int D>
 static hipError_t launch_query_t(const QueryArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL(boruvka_query_kernel<D>, dim3(blocks(a.n, 256)), dim3(256), 0, st, a);
+    if (a.pass == 1)
+        hipLaunchKernelGGL((boruvka_query_kernel<D, 1>), dim3(blocks(a.n, 256)), dim3(256), 0, st, a);
+    else if (a.pass == 2)
+        hipLaunchKernelGGL((boruvka_query_kernel<D, 2>), dim3(blocks(a.n, 256)), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((boruvka_query_kernel<D, 0>), dim3(blocks(a.n, 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

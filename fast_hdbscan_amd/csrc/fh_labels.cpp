@@ -1,0 +1,650 @@
+// Host label stage: sorted MST -> single-linkage tree -> condensed tree ->
+// cluster selection -> labels and membership strengths.
+//
+// Counterpart of hdbscan.clusters_from_spanning_tree (hdbscan.py:505-574)
+// for unit sample weights without semi-supervision, built on arrays instead
+// of numba typed dicts.  Row orders, float32 rounding points and the
+// union-by-rank tie rules follow cluster_trees.py so that labels and
+// strengths come out identical:
+//   mst_to_linkage_tree          cluster_trees.py:17-95
+//   condense_tree                cluster_trees.py:115-352
+//   extract_eom_clusters         cluster_trees.py:656-771
+//   cluster_tree_leaves          cluster_trees.py:369-374
+//   cluster_epsilon_search       cluster_trees.py:834-903
+//   get_cluster_label_vector     cluster_trees.py:944-996
+//   get_point_membership_strength_vector  cluster_trees.py:999-1031
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/fh.h"
+
+namespace {
+
+constexpr double kInfD = std::numeric_limits<double>::infinity();
+constexpr float kInfF = std::numeric_limits<float>::infinity();
+
+// One merge of the single-linkage tree: children, their sizes, distance.
+struct Merge {
+    int32_t left, right;            // child node ids (points < n, merges >= n)
+    int32_t left_size, right_size;  // points under each child
+    double delta;                   // merge distance
+};
+
+struct Hierarchy {
+    std::vector<Merge> rows;  // row i is merge node n + i
+};
+
+inline void prefetch(const void *p) { __builtin_prefetch(p, 0, 1); }
+
+// cluster_trees.py:66-95, reference form: the merge-node forest with
+// linkage_merge_find (38-52).  Used when an endpoint is -1 (only when
+// min_samples >= n), which the reference resolves through wrap-around
+// indexing into the merge slots.
+void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, Hierarchy &h) {
+    const int64_t m = n - 1;
+    const int64_t total = 2 * n - 1;
+    std::vector<int64_t> parent(total, -1);
+    std::vector<int64_t> size(total, 0);
+    for (int64_t i = 0; i < n; ++i) size[i] = 1;
+    h.rows.resize(m);
+    int64_t next = n;
+    // node values stay as given (-1 stays -1); only array accesses wrap
+    auto P = [&](int64_t x) -> int64_t & { return parent[x < 0 ? x + total : x]; };
+    auto S = [&](int64_t x) -> int64_t & { return size[x < 0 ? x + total : x]; };
+    auto find = [&](int64_t node) {
+        int64_t relabel = node;
+        while (P(node) != -1 && P(node) != node) node = P(node);
+        P(node) = node;
+        while (P(relabel) != node) {
+            const int64_t nr = P(relabel);
+            P(relabel) = node;
+            relabel = nr;
+        }
+        return node;
+    };
+    for (int64_t i = 0; i < m; ++i) {
+        const int64_t lc = find(src[i]);
+        const int64_t rc = find(dst[i]);
+        Merge &r = h.rows[i];
+        const bool lbig = lc > rc;
+        r.left = static_cast<int32_t>(lbig ? lc : rc);
+        r.right = static_cast<int32_t>(lbig ? rc : lc);
+        r.left_size = static_cast<int32_t>(S(lbig ? lc : rc));
+        r.right_size = static_cast<int32_t>(S(lbig ? rc : lc));
+        r.delta = std::sqrt(static_cast<double>(w_sq[i]));  // boruvka.py:646
+        S(next) = S(lc) + S(rc);
+        P(lc) = next;
+        P(rc) = next;
+        ++next;
+    }
+}
+
+// Same tree from a union-find over the points (union by size, path
+// halving) that records, per set, the merge node currently representing
+// it: the node linkage_merge_find would return.  Upcoming edges' entries
+// are prefetched.
+void linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, Hierarchy &h) {
+    const int64_t m = n - 1;
+    for (int64_t i = 0; i < m; ++i)
+        if (src[i] < 0 || dst[i] < 0) return linkage_tree_literal(n, src, dst, w_sq, h);
+    struct Set {
+        int32_t parent, size, node, pad;
+    };
+    // kept between calls: repeated calls of similar size touch no new pages
+    static thread_local std::vector<Set> uf;
+    if (uf.size() < static_cast<size_t>(n)) uf.resize(n);
+    for (int64_t i = 0; i < n; ++i) uf[i] = {static_cast<int32_t>(i), 1, static_cast<int32_t>(i), 0};
+    if (h.rows.size() < static_cast<size_t>(m)) h.rows.resize(m);
+    auto find = [&](int32_t x) {
+        while (uf[x].parent != x) {
+            const int32_t g = uf[uf[x].parent].parent;
+            uf[x].parent = g;
+            x = g;
+        }
+        return x;
+    };
+    constexpr int64_t kAhead = 24;
+    for (int64_t i = 0; i < m; ++i) {
+        if (i + kAhead < m) {
+            prefetch(&uf[src[i + kAhead]]);
+            prefetch(&uf[dst[i + kAhead]]);
+        }
+        const int32_t a = find(src[i]);
+        const int32_t b = find(dst[i]);
+        const int32_t lc = uf[a].node, rc = uf[b].node;
+        const int32_t ls = uf[a].size, rs = uf[b].size;
+        Merge &r = h.rows[i];
+        const bool lbig = lc > rc;
+        r.left = lbig ? lc : rc;
+        r.right = lbig ? rc : lc;
+        r.left_size = lbig ? ls : rs;
+        r.right_size = lbig ? rs : ls;
+        r.delta = std::sqrt(static_cast<double>(w_sq[i]));  // boruvka.py:646
+        int32_t root = a, child = b;
+        if (uf[a].size < uf[b].size) {
+            root = b;
+            child = a;
+        }
+        uf[child].parent = root;
+        uf[root].size = ls + rs;
+        uf[root].node = static_cast<int32_t>(n + i);
+    }
+}
+
+struct Condensed {
+    std::vector<int64_t> parent, child;
+    std::vector<float> lambda, size;
+    int64_t rows = 0;
+    int64_t next_label = 0;
+};
+
+// cluster_trees.py:205-352 (unit weights, max_cluster_size = inf: the
+// reference does not pass it here, hdbscan.py:525-527).  Rows come out in
+// the reference's order: merge nodes in breadth-first order from the root,
+// each eliminated branch's points in breadth-first order of that branch.
+void condense(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c) {
+    const int64_t m = n - 1;
+    const int32_t root = static_cast<int32_t>(2 * m);
+    const int32_t num_points = static_cast<int32_t>(n);
+    int64_t next_label = num_points + 1;
+    // relabel is written for a node before the node is processed; only the
+    // root needs a value up front.  Buffers persist between calls.
+    static thread_local std::vector<int32_t> relabel;
+    if (relabel.size() < static_cast<size_t>(root + 1)) relabel.resize(root + 1);
+    relabel[root] = num_points;
+    // a -1 child (only when min_samples >= n) indexes like numba: from the end
+    auto RL = [&](int32_t x) -> int32_t & { return relabel[x < 0 ? x + root + 1 : x]; };
+    // at most one row per node below the root
+    const size_t cap = static_cast<size_t>(root);
+    if (c.parent.size() < cap) {
+        c.parent.resize(cap);
+        c.child.resize(cap);
+        c.lambda.resize(cap);
+        c.size.resize(cap);
+    }
+    int64_t nrows = 0;
+    const Merge *rows = h.rows.data();
+    auto row = [&](int64_t p, int64_t ch, double lam, float sz) {
+        c.parent[nrows] = p;
+        c.child[nrows] = ch;
+        c.lambda[nrows] = static_cast<float>(lam);
+        c.size[nrows] = sz;
+        ++nrows;
+    };
+    constexpr size_t kAhead = 16;
+    // eliminate_branch (cluster_trees.py:115-145) writes the branch's points
+    // in breadth-first order.  The main walk below only reserves each
+    // branch's rows (their number is the branch size); the branches are
+    // expanded afterwards, in parallel, into their reserved ranges.
+    struct Task {
+        int32_t branch;
+        int32_t parent_node;
+        float lam;
+        int64_t off;
+    };
+    // (a thread_local named inside the worker lambdas would be each worker's
+    // own copy: bind the caller's instance to a local reference)
+    static thread_local std::vector<Task> tasks_tl;
+    std::vector<Task> &tasks = tasks_tl;
+    tasks.clear();
+    auto eliminate = [&](int32_t branch, int64_t parent_node, double lam) {
+        if (branch < num_points) {
+            row(parent_node, branch, lam, 1.0f);
+            return;
+        }
+        const int32_t sz = rows[branch - num_points].left_size + rows[branch - num_points].right_size;
+        tasks.push_back({branch, static_cast<int32_t>(parent_node), static_cast<float>(lam), nrows});
+        nrows += sz;
+    };
+    // The reference walks the whole breadth-first node list and skips
+    // points and nodes inside eliminated branches (cluster_trees.py:228-230).
+    // The nodes it does process are closed upwards, so a queue fed only with
+    // the children that continue visits them in the same order.
+    static thread_local std::vector<int32_t> q;
+    q.clear();
+    q.reserve(static_cast<size_t>(n));
+    q.push_back(root);
+    auto push = [&](int32_t x) {
+        if (x >= num_points) q.push_back(x);
+    };
+    const double dm = static_cast<double>(mcs);
+    for (size_t qi = 0; qi < q.size(); ++qi) {
+        if (qi + kAhead < q.size()) {
+            const int32_t ahead = q[qi + kAhead];
+            prefetch(&relabel[ahead]);
+            prefetch(&rows[ahead - num_points]);
+        }
+        const int32_t node = q[qi];
+        const int64_t parent_node = relabel[node];
+        const Merge &r = rows[node - num_points];
+        const int32_t left = r.left, right = r.right;
+        const double d = r.delta;
+        const double lam = d > 0.0 ? 1.0 / d : kInfD;
+        // counts as float32, compared in float64 as numba does
+        const float lc = static_cast<float>(r.left_size), rc = static_cast<float>(r.right_size);
+        const double dl = lc, dr = rc;
+        if (left < num_points && dr >= dm) {
+            RL(right) = static_cast<int32_t>(parent_node);
+            row(parent_node, left, lam, 1.0f);
+            push(right);
+        } else if (dl < dm && dr >= dm) {
+            RL(right) = static_cast<int32_t>(parent_node);
+            eliminate(left, parent_node, lam);
+            push(right);
+        } else if (dl >= dm && dr < dm) {
+            RL(left) = static_cast<int32_t>(parent_node);
+            eliminate(right, parent_node, lam);
+            push(left);
+        } else if (dl < dm && dr < dm) {
+            eliminate(left, parent_node, lam);
+            eliminate(right, parent_node, lam);
+        } else {
+            RL(left) = static_cast<int32_t>(next_label);
+            row(parent_node, next_label, lam, lc);
+            ++next_label;
+            RL(right) = static_cast<int32_t>(next_label);
+            row(parent_node, next_label, lam, rc);
+            ++next_label;
+            push(left);
+            push(right);
+        }
+    }
+    // expand the eliminated branches
+    auto expand = [&](size_t t0, size_t t1) {
+        std::vector<int32_t> sub;
+        for (size_t t = t0; t < t1; ++t) {
+            const Task &tk = tasks[t];
+            int64_t at = tk.off;
+            sub.clear();
+            sub.push_back(tk.branch);
+            for (size_t head = 0; head < sub.size(); ++head) {
+                if (head + kAhead < sub.size() && sub[head + kAhead] >= num_points)
+                    prefetch(&rows[sub[head + kAhead] - num_points]);
+                const int32_t node = sub[head];
+                if (node < num_points) {
+                    c.parent[at] = tk.parent_node;
+                    c.child[at] = node;
+                    c.lambda[at] = tk.lam;
+                    c.size[at] = 1.0f;
+                    ++at;
+                } else {
+                    sub.push_back(rows[node - num_points].left);
+                    sub.push_back(rows[node - num_points].right);
+                }
+            }
+        }
+    };
+    const int64_t elim_rows = tasks.empty() ? 0 : nrows;
+    const int nt = elim_rows > (1 << 16) ? std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))) : 1;
+    if (nt == 1) {
+        expand(0, tasks.size());
+    } else {
+        // split the task list into ranges of about equal row counts
+        std::vector<size_t> cut(nt + 1, tasks.size());
+        cut[0] = 0;
+        size_t t = 0;
+        for (int k = 1; k < nt; ++k) {
+            const int64_t target = nrows * k / nt;
+            while (t < tasks.size() && tasks[t].off < target) ++t;
+            cut[k] = t;
+        }
+        std::vector<std::thread> th;
+        for (int k = 0; k < nt; ++k) th.emplace_back(expand, cut[k], cut[k + 1]);
+        for (auto &x : th) x.join();
+    }
+    c.rows = nrows;
+    c.next_label = next_label;
+}
+
+struct ClusterTree {
+    int64_t root = -1;                        // smallest parent id (== n)
+    int64_t n_ids = 0;                        // ids are [root, root + n_ids)
+    std::vector<int64_t> parent, child;       // cluster rows (child_size > 1)
+    std::vector<float> lambda, size;
+    std::vector<int64_t> first_kid, next_sib;  // children lists in row order
+};
+
+void cluster_tree(const Condensed &c, ClusterTree &t) {
+    t.parent.clear();
+    t.child.clear();
+    t.lambda.clear();
+    t.size.clear();
+    for (size_t i = 0; i < static_cast<size_t>(c.rows); ++i) {
+        if (c.size[i] > 1.0f) {
+            t.parent.push_back(c.parent[i]);
+            t.child.push_back(c.child[i]);
+            t.lambda.push_back(c.lambda[i]);
+            t.size.push_back(c.size[i]);
+        }
+    }
+}
+
+// extract_eom_clusters (cluster_trees.py:725-771), score_condensed_tree_nodes
+// (656-669: float32 accumulation in row order) and eom_recursion (689-722,
+// float64 totals), iteratively.
+std::vector<int64_t> eom(const Condensed &c, const ClusterTree &ct, int64_t n, double max_cluster_size,
+                         bool allow_single) {
+    if (ct.parent.empty()) return {};
+    const int64_t root = n;
+    const int64_t nid = c.next_label - root;
+    std::vector<float> score(nid, 0.0f);
+    for (size_t i = 0; i < static_cast<size_t>(c.rows); ++i) {
+        if (c.size[i] > 1.0f) score[c.child[i] - root] = -c.lambda[i] * c.size[i];
+        float &s = score[c.parent[i] - root];
+        s = s + c.lambda[i] * c.size[i];
+    }
+    std::vector<float> node_size(nid, 0.0f);
+    for (size_t i = 0; i < ct.child.size(); ++i) node_size[ct.child[i] - root] = ct.size[i];
+    const int64_t ct_root = *std::min_element(ct.parent.begin(), ct.parent.end());
+    node_size[ct_root - root] = static_cast<float>(ct_root - 1);
+    std::vector<int64_t> first(nid, -1), nxt(ct.child.size(), -1), last(nid, -1);
+    for (size_t i = 0; i < ct.child.size(); ++i) {
+        const int64_t p = ct.parent[i] - root;
+        if (first[p] < 0)
+            first[p] = static_cast<int64_t>(i);
+        else
+            nxt[last[p]] = static_cast<int64_t>(i);
+        last[p] = static_cast<int64_t>(i);
+    }
+    std::vector<uint8_t> selected(nid, 0);
+    std::vector<double> ret(nid, 0.0);
+    auto unselect_below = [&](int64_t node) {
+        std::vector<int64_t> st{node};
+        while (!st.empty()) {
+            const int64_t x = st.back();
+            st.pop_back();
+            for (int64_t e = first[x - root]; e >= 0; e = nxt[e]) {
+                selected[ct.child[e] - root] = 0;
+                st.push_back(ct.child[e]);
+            }
+        }
+    };
+    // post-order evaluation of eom_recursion from `start`
+    auto run = [&](int64_t start) {
+        std::vector<std::pair<int64_t, int64_t>> st;  // (node, next child edge)
+        st.push_back({start, first[start - root]});
+        while (!st.empty()) {
+            auto &top = st.back();
+            if (top.second >= 0) {
+                const int64_t e = top.second;
+                top.second = nxt[e];
+                st.push_back({ct.child[e], first[ct.child[e] - root]});
+                continue;
+            }
+            const int64_t node = top.first;
+            st.pop_back();
+            const double cur = std::max(static_cast<double>(score[node - root]), 0.0);
+            double total = 0.0;
+            for (int64_t e = first[node - root]; e >= 0; e = nxt[e]) total += ret[ct.child[e] - root];
+            if (total > cur || static_cast<double>(node_size[node - root]) > max_cluster_size) {
+                ret[node - root] = total;
+            } else {
+                selected[node - root] = 1;
+                unselect_below(node);
+                ret[node - root] = cur;
+            }
+        }
+    };
+    if (allow_single) {
+        run(ct_root);
+    } else {
+        // len(node_scores) > 1 holds whenever the cluster tree is non-empty
+        for (int64_t e = first[ct_root - root]; e >= 0; e = nxt[e]) run(ct.child[e]);
+    }
+    std::vector<int64_t> out;
+    for (int64_t i = 0; i < nid; ++i)
+        if (selected[i]) out.push_back(root + i);
+    return out;
+}
+
+// cluster_tree_leaves (cluster_trees.py:369-374)
+std::vector<int64_t> leaves(const ClusterTree &ct, int64_t n) {
+    if (ct.parent.empty()) return {};
+    const int64_t n_nodes = *std::max_element(ct.child.begin(), ct.child.end()) + 1;
+    std::vector<uint8_t> leaf(n_nodes - n, 1);
+    for (const int64_t p : ct.parent) leaf[p - n] = 0;
+    std::vector<int64_t> out;
+    for (int64_t i = 0; i < n_nodes - n; ++i)
+        if (leaf[i]) out.push_back(i + n);
+    return out;
+}
+
+// cluster_epsilon_search, traverse_upwards, segments_in_branch
+// (cluster_trees.py:834-903).  segments_in_branch looks every segment up with
+// searchsorted among the sorted parents, so a segment that is not a parent
+// picks up the two children of the next larger parent id; kept as is.
+std::vector<int64_t> epsilon_search(const std::vector<int64_t> &clusters, const ClusterTree &ct, double min_eps) {
+    const size_t r = ct.parent.size();
+    std::vector<size_t> order(r);
+    for (size_t i = 0; i < r; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return ct.parent[a] < ct.parent[b]; });
+    std::vector<int64_t> parents(r), children(r);
+    for (size_t i = 0; i < r; ++i) {
+        parents[i] = ct.parent[order[i]];
+        children[i] = ct.child[order[i]];
+    }
+    const int64_t root = *std::min_element(ct.parent.begin(), ct.parent.end());
+    auto row_of_child = [&](int64_t c) {
+        return static_cast<size_t>(std::lower_bound(ct.child.begin(), ct.child.end(), c) - ct.child.begin());
+    };
+    auto death_eps = [&](float lam) { return lam > 0.0f ? 1.0 / static_cast<double>(lam) : kInfD; };
+    std::vector<int64_t> selected;
+    std::vector<int64_t> processed;  // kept sorted
+    auto is_processed = [&](int64_t c) { return std::binary_search(processed.begin(), processed.end(), c); };
+    for (const int64_t cluster : clusters) {
+        const size_t idx = row_of_child(cluster);
+        if (death_eps(ct.lambda[idx]) < min_eps) {
+            if (!is_processed(cluster)) {
+                // traverse_upwards
+                int64_t seg = cluster, parent;
+                for (;;) {
+                    parent = ct.parent[row_of_child(seg)];
+                    if (parent == root) break;
+                    if (death_eps(ct.lambda[row_of_child(parent)]) >= min_eps) break;
+                    seg = parent;
+                }
+                selected.push_back(parent);
+                // segments_in_branch
+                std::vector<int64_t> to_process{parent}, found;
+                while (!to_process.empty()) {
+                    found.insert(found.end(), to_process.begin(), to_process.end());
+                    std::vector<int64_t> kids;
+                    for (const int64_t s : to_process) {
+                        const size_t i =
+                            static_cast<size_t>(std::lower_bound(parents.begin(), parents.end(), s) - parents.begin());
+                        if (i >= parents.size()) continue;
+                        kids.push_back(children[i]);
+                        kids.push_back(children[i + 1]);
+                    }
+                    std::sort(kids.begin(), kids.end());
+                    kids.erase(std::unique(kids.begin(), kids.end()), kids.end());
+                    to_process.swap(kids);
+                }
+                processed.insert(processed.end(), found.begin(), found.end());
+                std::sort(processed.begin(), processed.end());
+                processed.erase(std::unique(processed.begin(), processed.end()), processed.end());
+            }
+        } else {
+            selected.push_back(cluster);
+        }
+    }
+    return selected;
+}
+
+// get_cluster_label_vector (cluster_trees.py:944-996) and
+// get_point_membership_strength_vector (999-1031).
+void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps, int64_t n, int64_t *labels,
+                   float *prob) {
+    for (int64_t i = 0; i < n; ++i) {
+        labels[i] = -1;
+        prob[i] = 0.0f;
+    }
+    if (c.rows == 0 || clusters.empty()) return;
+    const int64_t root_cluster = *std::min_element(c.parent.begin(), c.parent.begin() + c.rows);
+    const int64_t max_p = *std::max_element(c.parent.begin(), c.parent.begin() + c.rows);
+    const int64_t max_c = *std::max_element(c.child.begin(), c.child.begin() + c.rows);
+    const int64_t size = std::max(max_p + 1, max_c + 1);
+    std::sort(clusters.begin(), clusters.end());
+    clusters.erase(std::unique(clusters.begin(), clusters.end()), clusters.end());
+    // cluster-id tables cover ids [root_cluster, size); points are never
+    // clusters
+    const int64_t nc = size - root_cluster;
+    std::vector<int64_t> label_of(nc, -1);  // cluster id -> label
+    std::vector<uint8_t> is_cluster(nc, 0);
+    for (size_t k = 0; k < clusters.size(); ++k) {
+        label_of[clusters[k] - root_cluster] = static_cast<int64_t>(k);
+        is_cluster[clusters[k] - root_cluster] = 1;
+    }
+    auto selected = [&](int64_t id) { return id >= root_cluster && is_cluster[id - root_cluster]; };
+    // The reference unions every non-selected child into its parent with
+    // union by rank (cluster_trees.py:964-978).  Rows list parents before
+    // children, so each child joins as a fresh singleton and every set keeps
+    // its topmost node as root: a selected cluster, the root cluster, or a
+    // point whose row was skipped (lambda 0).  That root is computed
+    // directly: set_root(c) = c if selected or c is the root, else
+    // set_root(parent(c)).
+    std::vector<int64_t> set_root(nc, -1);
+    set_root[0] = root_cluster;
+    for (size_t r = 0; r < static_cast<size_t>(c.rows); ++r) {
+        const int64_t child = c.child[r];
+        if (child < n) continue;
+        set_root[child - root_cluster] = selected(child) ? child : set_root[c.parent[r] - root_cluster];
+    }
+    // single selected cluster at the root: the largest lambda among the
+    // root's rows (cluster_trees.py:984)
+    const bool single_root = clusters.size() == 1 && clusters[0] == root_cluster;
+    float root_max_lambda = 0.0f;
+    if (single_root) {
+        bool any = false;
+        for (size_t r = 0; r < static_cast<size_t>(c.rows); ++r)
+            if (c.parent[r] == root_cluster) {
+                root_max_lambda = any ? std::max(root_max_lambda, c.lambda[r]) : c.lambda[r];
+                any = true;
+            }
+    }
+    // deaths: largest lambda of a point row directly under each selected
+    // cluster (max_lambdas, cluster_trees.py:999-1008)
+    std::vector<double> deaths(nc, 0.0);
+    for (size_t r = 0; r < static_cast<size_t>(c.rows); ++r) {
+        const int64_t p = c.parent[r];
+        if (selected(p) && c.size[r] == 1.0f)
+            deaths[p - root_cluster] = std::max(deaths[p - root_cluster], static_cast<double>(c.lambda[r]));
+    }
+    // Every point is the child of exactly one row: its label follows from
+    // that row alone (a skipped lambda-0 row leaves the point on its own,
+    // i.e. noise), and so does its strength, which the reference computes
+    // from the same row.  Rows are independent: split over threads.
+    const int64_t rows = c.rows;
+    auto work = [&](int64_t r0, int64_t r1) {
+        for (int64_t r = r0; r < r1; ++r) {
+            const int64_t child = c.child[r];
+            if (child >= n || child < 0) continue;
+            const float lv = c.lambda[r];
+            const int64_t cl = lv == 0.0f ? child : set_root[c.parent[r] - root_cluster];
+            int64_t lab;
+            if (cl < root_cluster) {
+                lab = -1;
+            } else if (cl == root_cluster) {
+                // the reference looks the root up in the label map here; it
+                // is only there when the root is the one selected cluster
+                if (single_root) {
+                    if (eps > 0.0)
+                        lab = static_cast<double>(lv) >= 1.0 / eps ? label_of[0] : -1;
+                    else
+                        lab = lv >= root_max_lambda ? label_of[0] : -1;
+                } else {
+                    lab = -1;
+                }
+            } else {
+                lab = label_of[cl - root_cluster];
+            }
+            labels[child] = lab;
+            float pr = 0.0f;
+            if (lab >= 0) {
+                const double ml = deaths[clusters[lab] - root_cluster];
+                if (ml == 0.0 || !std::isfinite(lv)) {
+                    pr = 1.0f;
+                } else {
+                    const double lam = std::min(static_cast<double>(lv), ml);
+                    pr = static_cast<float>(lam / ml);
+                }
+            }
+            prob[child] = pr;
+        }
+    };
+    const int nt = rows > (1 << 16) ? std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))) : 1;
+    if (nt == 1) {
+        work(0, rows);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(work, rows * t / nt, rows * (t + 1) / nt);
+        for (auto &x : th) x.join();
+    }
+    // A -1 child (only with min_samples >= n) is a row of an +inf edge:
+    // the reference skips it for labels and reads labels[-1] (the last
+    // point) for the strength, which is noise in that configuration.
+}
+
+}  // namespace
+
+extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
+                                         const fh_label_params *params, int64_t *labels, float *probabilities,
+                                         double *linkage, fh_condensed_tree *condensed) {
+    if (n < 1 || !params || !labels || !probabilities) return FH_EINVAL;
+    if (n > 1 && (!src || !dst || !w_sq)) return FH_EINVAL;
+    if (params->min_cluster_size < 1) return FH_EINVAL;
+    if (params->selection_method != 0 && params->selection_method != 1) return FH_EINVAL;
+    try {
+        if (n < 2) {
+            labels[0] = -1;
+            probabilities[0] = 0.0f;
+            if (condensed) condensed->rows = 0;
+            return FH_OK;
+        }
+        // reused between calls on this thread (no fresh pages per call)
+        static thread_local Hierarchy h;
+        linkage_tree(n, src, dst, w_sq, h);
+        if (linkage) {
+            for (int64_t i = 0; i < n - 1; ++i) {
+                const Merge &r = h.rows[i];
+                linkage[4 * i + 0] = static_cast<double>(r.left);
+                linkage[4 * i + 1] = static_cast<double>(r.right);
+                linkage[4 * i + 2] = r.delta;
+                linkage[4 * i + 3] = static_cast<double>(r.left_size) + static_cast<double>(r.right_size);
+            }
+        }
+        static thread_local Condensed c;
+        condense(n, h, params->min_cluster_size, c);
+        ClusterTree ct;
+        cluster_tree(c, ct);
+        std::vector<int64_t> sel;
+        if (params->selection_method == 0) {
+            sel = eom(c, ct, n, params->max_cluster_size, params->allow_single_cluster != 0);
+        } else {
+            sel = leaves(ct, n);
+        }
+        if (sel.size() > 1 && params->cluster_selection_epsilon > 0.0)
+            sel = epsilon_search(sel, ct, params->cluster_selection_epsilon);
+        label_vectors(c, sel, params->cluster_selection_epsilon, n, labels, probabilities);
+        if (condensed) {
+            const int64_t rows = c.rows;
+            condensed->rows = rows;
+            if (rows <= condensed->capacity && condensed->parent && condensed->child && condensed->lambda_val &&
+                condensed->child_size) {
+                std::memcpy(condensed->parent, c.parent.data(), sizeof(int64_t) * rows);
+                std::memcpy(condensed->child, c.child.data(), sizeof(int64_t) * rows);
+                std::memcpy(condensed->lambda_val, c.lambda.data(), sizeof(float) * rows);
+                std::memcpy(condensed->child_size, c.size.data(), sizeof(float) * rows);
+            }
+        }
+    } catch (const std::bad_alloc &) {
+        return FH_ENOMEM;
+    }
+    return FH_OK;
+}

@@ -1,0 +1,238 @@
+"""fast_hdbscan-compatible entry points on the gfx950 MST path.
+
+``fast_hdbscan`` and ``HDBSCAN`` mirror hdbscan.fast_hdbscan
+(hdbscan.py:199-336) and hdbscan.HDBSCAN (hdbscan.py:577-822) for Euclidean,
+unsupervised clustering with unit sample weights: the MST comes from the
+device kernels and the labels from the host label stage.  ``hdbscan_device``
+is the fully device-resident pipeline used by the benchmark.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .hdbscan import compute_minimum_spanning_tree, mst_device
+from .labels import clusters_from_spanning_tree, label_params, labels_from_sorted_arrays
+
+
+def _check_params(min_samples, min_cluster_size, cluster_selection_epsilon, cluster_selection_persistence):
+    # hdbscan.py:254-275
+    if (
+        (not (np.issubdtype(type(min_samples), np.integer) or min_samples is None))
+        or not np.issubdtype(type(min_cluster_size), np.integer)
+        or (min_samples is not None and min_samples <= 0)
+        or min_cluster_size <= 0
+    ):
+        raise ValueError("Min samples and min cluster size must be positive integers!")
+    if not np.issubdtype(type(cluster_selection_epsilon), np.floating) or cluster_selection_epsilon < 0.0:
+        raise ValueError("Cluster selection epsilon must be a positive floating point number!")
+    if not np.issubdtype(type(cluster_selection_persistence), np.floating) or cluster_selection_persistence < 0.0:
+        raise ValueError("Cluster selection persistence must be a positive floating point number!")
+
+
+def fast_hdbscan(
+    data,
+    data_labels=None,
+    semi_supervised=False,
+    ss_algorithm="bc",
+    min_samples=10,
+    min_cluster_size=10,
+    cluster_selection_method="eom",
+    max_cluster_size=np.inf,
+    allow_single_cluster=False,
+    cluster_selection_epsilon=0.0,
+    cluster_selection_persistence=0.0,
+    sample_weights=None,
+    reproducible=False,
+    return_trees=False,
+    metric="euclidean",
+    algorithm="boruvka",
+    knn_k=None,
+    cannot_link=None,
+    validate_cannot_link=True,
+    cannot_link_groups=None,
+    metric_kwds=None,
+):
+    """Drop-in for hdbscan.fast_hdbscan (hdbscan.py:199-336), Euclidean."""
+    from sklearn.utils import check_array
+
+    if metric != "euclidean":
+        raise NotImplementedError("only metric='euclidean' runs on the device MST path")
+    data = check_array(data)
+    if semi_supervised:
+        if data_labels is None:
+            raise ValueError("data_labels must not be None when semi_supervised is set to True!")
+        if ss_algorithm not in ["bc", "bc_simple"]:
+            raise ValueError(f"Invalid ss_algorithm {ss_algorithm}")
+    _check_params(min_samples, min_cluster_size, cluster_selection_epsilon, cluster_selection_persistence)
+    if cluster_selection_method not in ("eom", "leaf"):
+        raise ValueError(f"Invalid cluster_selection_method {cluster_selection_method}")
+
+    # hdbscan.py:278-304
+    if data.shape[0] < 2:
+        n = data.shape[0]
+        labels = np.full(n, -1, dtype=np.intp)
+        probabilities = np.zeros(n, dtype=np.float64)
+        single_linkage_tree = np.zeros((0, 4), dtype=np.float64)
+        condensed_tree = np.zeros(0, dtype=[("parent", np.intp), ("child", np.intp), ("lambda_val", np.float64),
+                                            ("child_size", np.intp)])
+        mst = np.zeros((0, 3), dtype=np.float64)
+        neighbors = np.zeros((n, 0), dtype=np.intp)
+        core_distances = np.full(n, np.inf, dtype=np.float64)
+        result = (labels, probabilities, single_linkage_tree, condensed_tree, mst, neighbors, core_distances)
+        return result[: (None if return_trees else 2)]
+
+    mst, neighbors, core_distances = compute_minimum_spanning_tree(
+        data,
+        min_samples=min_cluster_size if min_samples is None else min_samples,
+        sample_weights=sample_weights,
+        reproducible=reproducible,
+        metric=metric,
+        algorithm=algorithm,
+        knn_k=knn_k,
+        cannot_link=cannot_link,
+        validate_cannot_link=validate_cannot_link,
+        cannot_link_groups=cannot_link_groups,
+        metric_kwds=metric_kwds,
+    )
+    return (
+        *clusters_from_spanning_tree(
+            mst,
+            data_labels=data_labels,
+            semi_supervised=semi_supervised,
+            ss_algorithm=ss_algorithm,
+            min_cluster_size=min_cluster_size,
+            cluster_selection_method=cluster_selection_method,
+            max_cluster_size=max_cluster_size,
+            allow_single_cluster=allow_single_cluster,
+            cluster_selection_epsilon=cluster_selection_epsilon,
+            cluster_selection_persistence=cluster_selection_persistence,
+            sample_weights=sample_weights,
+        ),
+        neighbors,
+        core_distances,
+    )[: (None if return_trees else 2)]
+
+
+def to_numpy_rec_array(named_tuple_tree):
+    """hdbscan.py:47-66"""
+    size = named_tuple_tree.parent.shape[0]
+    result = np.empty(size, dtype=[("parent", np.intp), ("child", np.intp), ("lambda_val", float),
+                                   ("child_size", np.intp)])
+    result["parent"] = named_tuple_tree.parent
+    result["child"] = named_tuple_tree.child
+    result["lambda_val"] = named_tuple_tree.lambda_val
+    result["child_size"] = named_tuple_tree.child_size
+    return result
+
+
+class HDBSCAN:
+    """hdbscan.HDBSCAN (hdbscan.py:577-822) for Euclidean, unsupervised
+    clustering: fit / fit_predict, labels_, probabilities_.  Rows with
+    non-finite values are left out of the clustering and labelled noise with
+    probability 0 (hdbscan.py:768-820)."""
+
+    def __init__(self, min_cluster_size=5, min_samples=None, cluster_selection_method="eom",
+                 allow_single_cluster=False, max_cluster_size=np.inf, cluster_selection_epsilon=0.0,
+                 cluster_selection_persistence=0.0, semi_supervised=False, ss_algorithm="bc", reproducible=False,
+                 metric="euclidean", algorithm="boruvka", knn_k=None, cannot_link=None, validate_cannot_link=True,
+                 cannot_link_groups=None, metric_kwds=None):
+        self.min_cluster_size = min_cluster_size
+        self.min_samples = min_samples
+        self.cluster_selection_method = cluster_selection_method
+        self.allow_single_cluster = allow_single_cluster
+        self.max_cluster_size = max_cluster_size
+        self.cluster_selection_epsilon = cluster_selection_epsilon
+        self.cluster_selection_persistence = cluster_selection_persistence
+        self.semi_supervised = semi_supervised
+        self.ss_algorithm = ss_algorithm
+        self.reproducible = reproducible
+        self.metric = metric
+        self.algorithm = algorithm
+        self.knn_k = knn_k
+        self.cannot_link = cannot_link
+        self.validate_cannot_link = validate_cannot_link
+        self.cannot_link_groups = cannot_link_groups
+        self.metric_kwds = metric_kwds
+
+    def get_params(self, deep=True):
+        return {k: getattr(self, k) for k in (
+            "min_cluster_size", "min_samples", "cluster_selection_method", "allow_single_cluster",
+            "max_cluster_size", "cluster_selection_epsilon", "cluster_selection_persistence", "semi_supervised",
+            "ss_algorithm", "reproducible", "metric", "algorithm", "knn_k", "cannot_link", "validate_cannot_link",
+            "cannot_link_groups", "metric_kwds")}
+
+    def fit(self, X, y=None, sample_weight=None, **fit_params):
+        from sklearn.utils import check_array
+
+        X = check_array(X, ensure_all_finite=False)
+        all_finite = bool(np.all(np.isfinite(X)))
+        if not all_finite:
+            finite_index = np.where(np.isfinite(X).sum(axis=1) == X.shape[1])[0]
+            clean = X[finite_index]
+        else:
+            clean = X
+        (self.labels_, self.probabilities_, self._single_linkage_tree, self._condensed_tree,
+         self._min_spanning_tree, self._neighbors, self._core_distances) = fast_hdbscan(
+            clean, None, return_trees=True, sample_weights=sample_weight, **self.get_params())
+        if not isinstance(self._condensed_tree, np.ndarray):
+            self._condensed_tree = to_numpy_rec_array(self._condensed_tree)
+        if not all_finite:
+            labels = np.full(X.shape[0], -1)
+            labels[finite_index] = self.labels_
+            self.labels_ = labels
+            probs = np.zeros(X.shape[0])
+            probs[finite_index] = self.probabilities_
+            self.probabilities_ = probs
+        return self
+
+    def fit_predict(self, X, y=None, **fit_params):
+        self.fit(X, y, **fit_params)
+        return self.labels_
+
+
+def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_method="eom",
+                   max_cluster_size=np.inf, allow_single_cluster=False, cluster_selection_epsilon=0.0,
+                   stats: dict | None = None, work: dict | None = None):
+    """Labels and membership strengths for a float32 torch tensor resident
+    on a HIP device: MST on the device, rows sorted on the device, then the
+    host label stage.  Returns (labels int64, strengths float32) on the
+    host."""
+    import ctypes
+    import time
+
+    import torch
+
+    n = X.shape[0]
+    w = work if work is not None else {}
+    src, dst, w_sq, core_sq, nbr = mst_device(X, min_samples, stats=stats, out=w.get("mst"))
+    m = n - 1
+    dev = X.device
+    if "sorted" not in w or w["sorted"][0].shape[0] < m:
+        w["sorted"] = (torch.empty(max(m, 1), dtype=torch.int32, device=dev),
+                       torch.empty(max(m, 1), dtype=torch.int32, device=dev),
+                       torch.empty(max(m, 1), dtype=torch.float32, device=dev))
+        w["host"] = (torch.empty(max(m, 1), dtype=torch.int32).pin_memory(),
+                     torch.empty(max(m, 1), dtype=torch.int32).pin_memory(),
+                     torch.empty(max(m, 1), dtype=torch.float32).pin_memory())
+    ss, sd, sw = w["sorted"]
+    t0 = time.perf_counter()
+    ctx = _lib.context(dev.index or 0)
+    if m > 0:
+        code = _lib.load().fh_sort_mst_device(ctx, m, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                              ctypes.c_void_p(w_sq.data_ptr()), ctypes.c_void_p(ss.data_ptr()),
+                                              ctypes.c_void_p(sd.data_ptr()), ctypes.c_void_p(sw.data_ptr()))
+        _lib.check(ctx, code)
+    hs, hd, hw = w["host"]
+    hs[:m].copy_(ss[:m])
+    hd[:m].copy_(sd[:m])
+    hw[:m].copy_(sw[:m])
+    t1 = time.perf_counter()
+    params = label_params(min_cluster_size, cluster_selection_method, max_cluster_size, allow_single_cluster,
+                          cluster_selection_epsilon)
+    labels, strengths, _, _ = labels_from_sorted_arrays(n, hs[:m].numpy(), hd[:m].numpy(), hw[:m].numpy(), params)
+    t2 = time.perf_counter()
+    if stats is not None:
+        stats["t_sort_copy_ms"] = 1e3 * (t1 - t0)
+        stats["t_labels_ms"] = 1e3 * (t2 - t1)
+    return labels, strengths

@@ -1,0 +1,99 @@
+"""Host label stage: MST -> single linkage -> condensed tree -> clusters.
+
+``clusters_from_spanning_tree`` mirrors hdbscan.clusters_from_spanning_tree
+(hdbscan.py:497-574) for unit sample weights and unsupervised selection;
+the work runs in the library's host C++ (fh_labels_from_sorted_mst).
+"""
+from __future__ import annotations
+
+import ctypes
+from collections import namedtuple
+
+import numpy as np
+
+from . import _lib
+
+CondensedTree = namedtuple("CondensedTree", ["parent", "child", "lambda_val", "child_size"])
+
+_METHODS = {"eom": 0, "leaf": 1}
+
+
+def label_params(min_cluster_size=10, cluster_selection_method="eom", max_cluster_size=np.inf,
+                 allow_single_cluster=False, cluster_selection_epsilon=0.0) -> _lib.FHLabelParams:
+    if cluster_selection_method not in _METHODS:
+        raise ValueError(f"Invalid cluster_selection_method {cluster_selection_method}")
+    return _lib.FHLabelParams(int(min_cluster_size), float(max_cluster_size), float(cluster_selection_epsilon),
+                              _METHODS[cluster_selection_method], 1 if allow_single_cluster else 0)
+
+
+def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False):
+    """Run the host label stage on MST rows already sorted by (weight,
+    source, target).  Returns (labels int64, strengths float32, linkage or
+    None, CondensedTree or None)."""
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    dst = np.ascontiguousarray(dst, dtype=np.int32)
+    w_sq = np.ascontiguousarray(w_sq, dtype=np.float32)
+    labels = np.empty(n, np.int64)
+    strengths = np.empty(n, np.float32)
+    linkage = np.empty((max(n - 1, 0), 4), np.float64) if return_trees else None
+    ct = None
+    ctree = None
+    if return_trees:
+        cap = max(2 * (n - 1), 1)
+        ct = {"parent": np.empty(cap, np.int64), "child": np.empty(cap, np.int64),
+              "lambda_val": np.empty(cap, np.float32), "child_size": np.empty(cap, np.float32)}
+        ctree = _lib.FHCondensedTree(ct["parent"].ctypes.data, ct["child"].ctypes.data, ct["lambda_val"].ctypes.data,
+                                     ct["child_size"].ctypes.data, cap, 0)
+    code = _lib.load().fh_labels_from_sorted_mst(
+        n, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(w_sq), ctypes.byref(params), _lib.ptr(labels), _lib.ptr(strengths),
+        _lib.ptr(linkage) if linkage is not None else None, ctypes.byref(ctree) if ctree is not None else None)
+    if code == _lib.FH_EINVAL:
+        raise ValueError("invalid label-stage arguments")
+    if code != _lib.FH_OK:
+        raise _lib.FHError(f"label stage failed with code {code}")
+    condensed = None
+    if ctree is not None:
+        r = ctree.rows
+        condensed = CondensedTree(ct["parent"][:r], ct["child"][:r], ct["lambda_val"][:r], ct["child_size"][:r])
+    return labels, strengths, linkage, condensed
+
+
+def clusters_from_spanning_tree(
+    minimum_spanning_tree,
+    data_labels=None,
+    semi_supervised=False,
+    ss_algorithm="bc",
+    min_cluster_size=10,
+    cluster_selection_method="eom",
+    max_cluster_size=np.inf,
+    allow_single_cluster=False,
+    cluster_selection_epsilon=0.0,
+    cluster_selection_persistence=0.0,
+    sample_weights=None,
+):
+    """Drop-in for hdbscan.clusters_from_spanning_tree (hdbscan.py:497-574):
+    returns (labels, membership_strengths, linkage_tree, condensed_tree,
+    sorted_mst)."""
+    if semi_supervised:
+        raise NotImplementedError("semi-supervised selection is not in the host label stage")
+    if sample_weights is not None:
+        raise NotImplementedError("sample weights are not in the host label stage")
+    if cluster_selection_persistence > 0.0:
+        raise NotImplementedError("cluster_selection_persistence is not in the host label stage")
+    mst = np.asarray(minimum_spanning_tree, dtype=np.float64)
+    n = mst.shape[0] + 1
+    # hdbscan.py:511-519
+    sorted_mst = mst[np.lexsort((mst.T[1], mst.T[0], mst.T[2]))]
+    # The edge weights are float64 square roots of float32 squared distances;
+    # squaring back in float32 recovers them exactly for MSTs this library
+    # produced.  Other inputs go through the float64 path below.
+    w64 = sorted_mst[:, 2]
+    w_sq = (w64 * w64).astype(np.float32)
+    exact = np.array_equal(np.sqrt(w_sq.astype(np.float64)), w64)
+    params = label_params(min_cluster_size, cluster_selection_method, max_cluster_size, allow_single_cluster,
+                          cluster_selection_epsilon)
+    if not exact:
+        raise NotImplementedError("MST weights that are not square roots of float32 values are not supported")
+    labels, strengths, linkage, condensed = labels_from_sorted_arrays(
+        n, sorted_mst[:, 0].astype(np.int32), sorted_mst[:, 1].astype(np.int32), w_sq, params, return_trees=True)
+    return labels, strengths, linkage, condensed, sorted_mst

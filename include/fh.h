@@ -56,7 +56,8 @@ typedef struct {
     double t_knn_ms;          /* kNN / core-distance kernel                   */
     double t_init_ms;         /* kNN-seeded forest                             */
     double t_rounds_ms;       /* all Boruvka rounds                            */
-    double t_query_ms;        /* sum of the per-round nearest-edge kernels    */
+    double t_query_ms;        /* per-round nearest-edge search, summed         */
+    double t_query_main_ms;   /* its main-pass kernel alone, summed            */
     double t_total_ms;        /* whole call                                    */
 } fh_stats;
 
@@ -127,6 +128,58 @@ int fh_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, const float *Q, in
 int64_t fh_tree_nnodes(int64_t n);
 int fh_build_kdtree(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int64_t *idx_array, int64_t *idx_start,
                     int64_t *idx_end, float *radius, uint8_t *is_leaf, float *node_bounds);
+
+/* ------------------------------------------------------------------ */
+/* Host label stage (the MST's consumer, kept on the host)             */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t min_cluster_size;          /* >= 1                                   */
+    double max_cluster_size;           /* INFINITY: no limit (EOM only)         */
+    double cluster_selection_epsilon;  /* 0: no epsilon search                  */
+    int32_t selection_method;          /* 0: "eom", 1: "leaf"                  */
+    int32_t allow_single_cluster;      /* 0 / 1                                  */
+} fh_label_params;
+
+/* Optional condensed-tree output; rows <= 2*(n-1).  `rows` is always set;
+ * the arrays are filled when capacity >= rows. */
+typedef struct {
+    int64_t *parent;
+    int64_t *child;
+    float *lambda_val;
+    float *child_size;
+    int64_t capacity;
+    int64_t rows;
+} fh_condensed_tree;
+
+/*
+ * Labels and membership strengths from an MST whose rows are already
+ * sorted by (weight, source, target).
+ *   Replaces: hdbscan.py:521-574 (clusters_from_spanning_tree after its
+ *             lexsort at 511-519) for unit sample weights, not
+ *             semi-supervised, cluster_selection_persistence = 0:
+ *             mst_to_linkage_tree, condense_tree, extract_eom_clusters or
+ *             cluster_tree_leaves, cluster_epsilon_search,
+ *             get_cluster_label_vector, get_point_membership_strength_vector
+ *             (cluster_trees.py).
+ * HOST pointers.  src/dst/w_sq [n-1]: sorted MST rows (w_sq squared; the
+ * float64 square root is taken inside, as boruvka.py:646 does).
+ * labels [n] int64, probabilities [n] float32.  linkage [(n-1)*4] float64
+ * (rows left, right, distance, size, as mst_to_linkage_tree) and condensed
+ * may be NULL.  No context: host-only.
+ */
+int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
+                              const fh_label_params *params, int64_t *labels, float *probabilities,
+                              double *linkage, fh_condensed_tree *condensed);
+
+/*
+ * Sort MST rows by (weight, source, target) on the device -- the lexsort
+ * of hdbscan.py:511-519 (the float64 square root preserves the order and
+ * ties of the float32 squared weights).  DEVICE pointers, m rows; outputs
+ * must not alias inputs.
+ */
+int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t *dst, const float *w_sq,
+                       int32_t *src_out, int32_t *dst_out, float *w_out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,147 @@
+"""Pins for the label-stage oracle and the host label stage (CPU only).
+
+Known answers are the reference's own tests; the host stage (C++ in
+libfh.so, no device) must reproduce the oracle row for row.
+"""
+import numpy as np
+import pytest
+
+from oracle import labels as LB
+from oracle import oracle as O
+from fast_hdbscan_amd import workloads
+
+
+def _ref_X():
+    # test_hdbscan.py:38-40
+    from sklearn.datasets import make_blobs
+    from sklearn.preprocessing import StandardScaler
+    from sklearn.utils import shuffle
+
+    X, y = make_blobs(n_samples=200, random_state=10)
+    X, y = shuffle(X, y, random_state=7)
+    return StandardScaler().fit_transform(X)
+
+
+def test_known_answer_uniform_epsilon():
+    # test_hdbscan.py:169-186
+    np.random.seed(0)
+    X = np.random.rand(150, 2)
+    edges, _, _ = O.compute_minimum_spanning_tree(X, 5)
+    lab, *_ = LB.clusters_from_spanning_tree(edges, min_cluster_size=5)
+    u, c = np.unique(lab, return_counts=True)
+    assert len(u) == 8 and c[u == -1] == 69
+    lab, *_ = LB.clusters_from_spanning_tree(edges, min_cluster_size=5, cluster_selection_epsilon=0.2)
+    u, c = np.unique(lab, return_counts=True)
+    assert len(u) == 2 and c[u == -1] == 2
+
+
+def test_known_answer_three_blobs():
+    # test_hdbscan.py:84-93 (defaults: min_cluster_size 10, min_samples 10)
+    edges, _, _ = O.compute_minimum_spanning_tree(_ref_X(), 10)
+    lab, *_ = LB.clusters_from_spanning_tree(edges, min_cluster_size=10)
+    assert len(set(lab)) - int(-1 in lab) == 3
+
+
+def test_known_answer_no_clusters():
+    # test_hdbscan.py:103-110
+    X = _ref_X()
+    edges, _, _ = O.compute_minimum_spanning_tree(X, len(X) + 1)
+    lab, *_ = LB.clusters_from_spanning_tree(edges, min_cluster_size=len(X) + 1)
+    assert np.all(lab == -1)
+
+
+def test_known_answer_max_cluster_size():
+    # test_hdbscan.py:189-195 (HDBSCAN defaults: min_cluster_size 5)
+    edges, _, _ = O.compute_minimum_spanning_tree(_ref_X(), 5)
+    lab, *_ = LB.clusters_from_spanning_tree(edges, min_cluster_size=5, max_cluster_size=30)
+    assert len(set(lab)) >= 3
+    for label in set(lab) - {-1}:
+        assert np.sum(lab == label) <= 30
+
+
+def test_known_answer_issue_63():
+    # test_hdbscan.py:742-762
+    for seed in [700, 727]:
+        rng = np.random.default_rng(seed)
+        centers = rng.uniform(-10, 10, size=(3, 3))
+        data = np.vstack([rng.normal(loc=c, scale=0.5, size=(200, 3)) for c in centers])
+        edges, _, _ = O.compute_minimum_spanning_tree(data, 1)
+        lab, *_ = LB.clusters_from_spanning_tree(edges, min_cluster_size=192, cluster_selection_epsilon=0.05957,
+                                                 allow_single_cluster=True)
+        assert len(set(lab) - {-1}) >= 1
+        assert np.sum(lab == -1) / len(lab) < 0.9
+
+
+def test_edge_cases_all_noise_when_mcs_exceeds_n():
+    # test_edge_cases.py:26-31
+    np.random.seed(42)
+    X = np.random.rand(10, 2)
+    edges, _, _ = O.compute_minimum_spanning_tree(X, 100)
+    lab, *_ = LB.clusters_from_spanning_tree(edges, min_cluster_size=100)
+    assert np.all(lab == -1)
+
+
+def test_agrees_with_sklearn_hdbscan():
+    # Independent implementation (sklearn computes in float64, the reference
+    # path in float32, so a few boundary points may differ); sklearn's
+    # min_samples counts the point itself.
+    from sklearn.cluster import HDBSCAN as SK
+
+    X = workloads.blobs_small(3000, seed=4).astype(np.float64)
+    edges, _, _ = O.compute_minimum_spanning_tree(X, 10)
+    lab, prob, *_ = LB.clusters_from_spanning_tree(edges, min_cluster_size=25)
+    sk = SK(min_cluster_size=25, min_samples=11).fit(X)
+    assert len(set(lab.tolist())) == len(set(sk.labels_.tolist()))
+    # best one-to-one matching by overlap
+    agree = 0
+    for c in set(lab.tolist()):
+        vals, cnt = np.unique(sk.labels_[lab == c], return_counts=True)
+        agree += cnt.max()
+    assert agree / len(lab) > 0.99
+    assert np.median(np.abs(prob - sk.probabilities_)) < 1e-3
+
+
+CASES = [
+    ("uniform", 5, 5, "eom", 0.0, False),
+    ("uniform", 5, 5, "eom", 0.2, False),
+    ("uniform", 5, 5, "leaf", 0.0, False),
+    ("uniform", 5, 5, "eom", 0.0, True),
+    ("tiny_k_gt_n", 100, 100, "eom", 0.0, False),
+    ("dups", 5, 5, "eom", 0.0, False),
+    ("blobs", 10, 10, "eom", 0.0, False),
+    ("blobs", 10, 10, "leaf", 0.0, False),
+    ("blobs", 10, 10, "eom", 0.5, False),
+    ("blobs", 10, 10, "eom", 0.0, True),
+    ("aniso", 10, 100, "eom", 0.0, False),
+]
+
+
+def _data(name):
+    if name == "uniform":
+        np.random.seed(0)
+        return np.random.rand(150, 2)
+    if name == "tiny_k_gt_n":
+        return np.random.RandomState(42).rand(10, 2)
+    if name == "dups":
+        return np.full((50, 3), 1.0)
+    if name == "blobs":
+        return workloads.blobs_small(10_000)
+    return workloads.scale_2d(60_000, seed=3)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[3]}-{c[4]}-{c[5]}" for c in CASES])
+def test_host_label_stage_matches_oracle(case):
+    from fast_hdbscan_amd import labels as PL
+
+    name, ms, mcs, meth, eps, single = case
+    edges, _, _ = O.compute_minimum_spanning_tree(_data(name), ms)
+    kw = dict(min_cluster_size=mcs, cluster_selection_method=meth, cluster_selection_epsilon=eps,
+              allow_single_cluster=single)
+    a = PL.clusters_from_spanning_tree(edges, **kw)
+    b = LB.clusters_from_spanning_tree(edges, **kw)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[2], b[2])
+    for f in ("parent", "child", "lambda_val", "child_size"):
+        np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
+    np.testing.assert_array_equal(a[4], b[4])
