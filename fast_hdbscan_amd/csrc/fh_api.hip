@@ -354,6 +354,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
     }
     FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p, ctx->core_p.p,
                    ms > 1 ? 1 : 0, pruned ? ctx->nb_pos.p : nullptr, pruned ? ctx->nb_d.p : nullptr));
+    FH_HIP(hipEventRecord(ctx->ev[2], st));
     // certification nodes: kCertDepth levels above the leaves
     const int cert_level = std::max(0, levels - 1 - kCertDepth);
     if (pruned) {
@@ -362,7 +363,6 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
         FH_TRY(ensure(ctx, ctx->cert, int64_t(1) << cert_level));
         FH_HIP(fh::launch_cert_setup(cert_level, ctx->range.p, ctx->core_p.p, ctx->cert_of_pos.p, ctx->mincore.p, st));
     }
-    FH_HIP(hipEventRecord(ctx->ev[2], st));
 
     // -- kNN-seeded forest
     fh::CycleBufs cb{ctx->S0.p, ctx->M0.p, ctx->S1.p, ctx->M1.p};
