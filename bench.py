@@ -29,6 +29,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per dispatch from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0, factors from a plain 256 MiB copy) on this workload;
+# command lines and raw counts in profiles/pmc_2Mx2D_traffic.md.
+PMC_TRAFFIC_BYTES = {
+    "boruvka_query_kernel<2, 2>": 68.1e6 + 11.3e6,
+    "knn_kernel<2>": 191.0e6 + 522.5e6,
+}
+
 
 def _parse():
     p = argparse.ArgumentParser()
@@ -41,11 +49,30 @@ def _parse():
     p.add_argument("--cpu-sample", type=int, default=100_000,
                    help="points of the same workload timed on the oracle (0: skip)")
     p.add_argument("--cpu-threads", type=int, default=1)
+    p.add_argument("--copy-probe", action="store_true",
+                   help="only run a plain 256 MiB device copy (calibrates HBM byte counters) and exit")
     return p.parse_args()
+
+
+def copy_probe():
+    """A plain device-to-device copy of known size: 256 MiB read, 256 MiB
+    written per dispatch, repeated 5 times."""
+    import torch
+
+    src = torch.empty(64 << 20, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": src.numel() * 4, "dispatches": 5}))
 
 
 def main():
     args = _parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -161,7 +188,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": PMC_TRAFFIC_BYTES.get(dom) if (n == 2_000_000 and ms == 10) else None,
             "avg_launch_ms": dur_ms,
             "alg_bytes_per_launch": alg,
         },
@@ -169,7 +196,8 @@ def main():
         "phases_ms": {key: float(np.mean([s[key] for s in per_step]))
                       for key in ("t_tree_ms", "t_knn_ms", "t_init_ms", "t_rounds_ms", "t_query_ms", "t_query_main_ms",
                                   "t_total_ms",
-                                  "t_sort_copy_ms", "t_labels_ms")},
+                                  "t_sort_copy_ms", "t_labels_ms", "t_linkage_ms", "t_condense_ms", "t_select_ms",
+                                  "t_label_pass_ms")},
         "mst_points_per_s": n / (1e-3 * float(np.mean([s["t_total_ms"] for s in per_step]))),
         "rounds": rounds,
         "clusters": n_clusters,

@@ -126,7 +126,7 @@ def load() -> ctypes.CDLL:
         L.fh_build_kdtree.argtypes = [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fh_labels_from_sorted_mst.restype = ctypes.c_int
         L.fh_labels_from_sorted_mst.argtypes = [_i64, _vp, _vp, _vp, ctypes.POINTER(FHLabelParams), _vp, _vp, _vp,
-                                                ctypes.POINTER(FHCondensedTree)]
+                                                ctypes.POINTER(FHCondensedTree), _vp]
         L.fh_sort_mst_device.restype = ctypes.c_int
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L

@@ -14,6 +14,7 @@
 //   get_cluster_label_vector     cluster_trees.py:944-996
 //   get_point_membership_strength_vector  cluster_trees.py:999-1031
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -595,7 +596,11 @@ void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps
 
 extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
                                          const fh_label_params *params, int64_t *labels, float *probabilities,
-                                         double *linkage, fh_condensed_tree *condensed) {
+                                         double *linkage, fh_condensed_tree *condensed, double *phase_ms) {
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point a) {
+        return std::chrono::duration<double, std::milli>(clk::now() - a).count();
+    };
     if (n < 1 || !params || !labels || !probabilities) return FH_EINVAL;
     if (n > 1 && (!src || !dst || !w_sq)) return FH_EINVAL;
     if (params->min_cluster_size < 1) return FH_EINVAL;
@@ -609,7 +614,9 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         }
         // reused between calls on this thread (no fresh pages per call)
         static thread_local Hierarchy h;
+        clk::time_point t = clk::now();
         linkage_tree(n, src, dst, w_sq, h);
+        if (phase_ms) phase_ms[0] = ms_since(t);
         if (linkage) {
             for (int64_t i = 0; i < n - 1; ++i) {
                 const Merge &r = h.rows[i];
@@ -620,7 +627,10 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
             }
         }
         static thread_local Condensed c;
+        t = clk::now();
         condense(n, h, params->min_cluster_size, c);
+        if (phase_ms) phase_ms[1] = ms_since(t);
+        t = clk::now();
         ClusterTree ct;
         cluster_tree(c, ct);
         std::vector<int64_t> sel;
@@ -631,7 +641,10 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         }
         if (sel.size() > 1 && params->cluster_selection_epsilon > 0.0)
             sel = epsilon_search(sel, ct, params->cluster_selection_epsilon);
+        if (phase_ms) phase_ms[2] = ms_since(t);
+        t = clk::now();
         label_vectors(c, sel, params->cluster_selection_epsilon, n, labels, probabilities);
+        if (phase_ms) phase_ms[3] = ms_since(t);
         if (condensed) {
             const int64_t rows = c.rows;
             condensed->rows = rows;

@@ -230,9 +230,12 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
     t1 = time.perf_counter()
     params = label_params(min_cluster_size, cluster_selection_method, max_cluster_size, allow_single_cluster,
                           cluster_selection_epsilon)
-    labels, strengths, _, _ = labels_from_sorted_arrays(n, hs[:m].numpy(), hd[:m].numpy(), hw[:m].numpy(), params)
+    phases = {}
+    labels, strengths, _, _ = labels_from_sorted_arrays(n, hs[:m].numpy(), hd[:m].numpy(), hw[:m].numpy(), params,
+                                                        phases=phases)
     t2 = time.perf_counter()
     if stats is not None:
         stats["t_sort_copy_ms"] = 1e3 * (t1 - t0)
         stats["t_labels_ms"] = 1e3 * (t2 - t1)
+        stats.update(phases)
     return labels, strengths

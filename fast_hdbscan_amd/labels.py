@@ -26,7 +26,7 @@ def label_params(min_cluster_size=10, cluster_selection_method="eom", max_cluste
                               _METHODS[cluster_selection_method], 1 if allow_single_cluster else 0)
 
 
-def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False):
+def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, phases: dict | None = None):
     """Run the host label stage on MST rows already sorted by (weight,
     source, target).  Returns (labels int64, strengths float32, linkage or
     None, CondensedTree or None)."""
@@ -44,13 +44,17 @@ def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False):
               "lambda_val": np.empty(cap, np.float32), "child_size": np.empty(cap, np.float32)}
         ctree = _lib.FHCondensedTree(ct["parent"].ctypes.data, ct["child"].ctypes.data, ct["lambda_val"].ctypes.data,
                                      ct["child_size"].ctypes.data, cap, 0)
+    ph = np.zeros(4, np.float64)
     code = _lib.load().fh_labels_from_sorted_mst(
         n, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(w_sq), ctypes.byref(params), _lib.ptr(labels), _lib.ptr(strengths),
-        _lib.ptr(linkage) if linkage is not None else None, ctypes.byref(ctree) if ctree is not None else None)
+        _lib.ptr(linkage) if linkage is not None else None, ctypes.byref(ctree) if ctree is not None else None,
+        _lib.ptr(ph))
     if code == _lib.FH_EINVAL:
         raise ValueError("invalid label-stage arguments")
     if code != _lib.FH_OK:
         raise _lib.FHError(f"label stage failed with code {code}")
+    if phases is not None:
+        phases.update(dict(zip(("t_linkage_ms", "t_condense_ms", "t_select_ms", "t_label_pass_ms"), ph.tolist())))
     condensed = None
     if ctree is not None:
         r = ctree.rows

@@ -165,12 +165,13 @@ typedef struct {
  * HOST pointers.  src/dst/w_sq [n-1]: sorted MST rows (w_sq squared; the
  * float64 square root is taken inside, as boruvka.py:646 does).
  * labels [n] int64, probabilities [n] float32.  linkage [(n-1)*4] float64
- * (rows left, right, distance, size, as mst_to_linkage_tree) and condensed
+ * (rows left, right, distance, size, as mst_to_linkage_tree), condensed
+ * and phase_ms [4] (linkage, condense, selection, labels; milliseconds)
  * may be NULL.  No context: host-only.
  */
 int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
                               const fh_label_params *params, int64_t *labels, float *probabilities,
-                              double *linkage, fh_condensed_tree *condensed);
+                              double *linkage, fh_condensed_tree *condensed, double *phase_ms);
 
 /*
  * Sort MST rows by (weight, source, target) on the device -- the lexsort
