@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "fh_build_kdtree",
     "fh_labels_from_sorted_mst",
     "fh_sort_mst_device",
+    "fh_neighbor_columns",
 )
 
 
@@ -113,7 +114,7 @@ def load() -> ctypes.CDLL:
         L.fh_last_error.argtypes = [_vp]
         L.fh_version.restype = ctypes.c_char_p
         L.fh_version.argtypes = []
-        mst_args = [_vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(FHStats)]
+        mst_args = [_vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(FHStats)]
         L.fh_mst_euclidean_device.restype = ctypes.c_int
         L.fh_mst_euclidean_device.argtypes = mst_args
         L.fh_mst_euclidean.restype = ctypes.c_int
@@ -127,6 +128,8 @@ def load() -> ctypes.CDLL:
         L.fh_labels_from_sorted_mst.restype = ctypes.c_int
         L.fh_labels_from_sorted_mst.argtypes = [_i64, _vp, _vp, _vp, ctypes.POINTER(FHLabelParams), _vp, _vp, _vp,
                                                 ctypes.POINTER(FHCondensedTree), _vp]
+        L.fh_neighbor_columns.restype = _i64
+        L.fh_neighbor_columns.argtypes = [_i64, _i32, _vp]
         L.fh_sort_mst_device.restype = ctypes.c_int
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L

@@ -63,7 +63,7 @@ struct fh_ctx {
     DevBuf<int2> range;
     DevBuf<unsigned long long> best;
     // device tree build
-    DevBuf<float> xdev;
+    DevBuf<float> xdev, wdev;
     DevBuf<uint64_t> tkeys;
     DevBuf<uint32_t> obox;
     DevBuf<int> taxis, tcnt, gtpos, ltpos, tactive;
@@ -260,8 +260,9 @@ int knn_block(int k) {
 // query_perm != nullptr (queries = tree-order rows).
 int run_knn(fh_ctx *ctx, int d, int levels, const float *queries, const int *query_perm, int64_t nq, int k,
             float *out_d, int *out_i, float *core_o, float *core_p, int core_from_knn, int *nb_pos = nullptr,
-            float *nb_d = nullptr) {
+            float *nb_d = nullptr, int sqrt_dist = 0) {
     fh::KnnArgs a{};
+    a.sqrt_dist = sqrt_dist;
     a.nb_pos = nb_pos;
     a.nb_d = nb_d;
     a.xp = ctx->xp.p;
@@ -299,10 +300,20 @@ int read_small(fh_ctx *ctx, int count) {
     return FH_OK;
 }
 
-// The MST proper.  Xd: the points in HBM (original order).  Outputs are
-// device pointers.
-int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_out, int *nbr_out, int *src_out,
-            int *dst_out, float *w_out, fh_stats *stats) {
+// boruvka.py:564-568: k = int(2 * min_samples / mean(w)), numba's float32
+// mean (index-order float32 sum, float64 division).
+int64_t weighted_k(const float *w, int64_t n, int ms) {
+    float acc = 0.0f;
+    for (int64_t i = 0; i < n; ++i) acc += w[i];
+    const double mean = static_cast<double>(acc) / static_cast<double>(n);
+    return static_cast<int64_t>(2.0 * (static_cast<double>(ms) / mean));
+}
+
+// The MST proper.  Xd: the points in HBM (original order).  weights_d:
+// per-point sample weights in HBM or nullptr; kq_w: the kNN width for the
+// weighted branch.  Outputs are device pointers.
+int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float *weights_d, int kq_w, float *core_out,
+            int *nbr_out, int *src_out, int *dst_out, float *w_out, fh_stats *stats) {
     Timer total;
     hipStream_t st = ctx->stream;
     fh_stats S{};
@@ -316,7 +327,9 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
     S.n_nodes = n_nodes;
 
     const int ni = (int)n;
-    const int kq = ms > 1 ? ms + 1 : 2;
+    const bool weighted = weights_d != nullptr;
+    // boruvka.py:567-569 (weighted), 581-583, 593-595
+    const int kq = weighted ? kq_w : (ms > 1 ? ms + 1 : 2);
     FH_TRY(ensure(ctx, ctx->knn_d, (size_t)n * kq));
     FH_TRY(ensure(ctx, ctx->knn_i, (size_t)n * kq));
     FH_TRY(ensure(ctx, ctx->core_o, n));
@@ -348,12 +361,20 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
 
     // -- core distances
     const bool pruned = ctx->search_mode != 0;
-    if (pruned) {
+    // Bound seeding reuses the kNN distances as candidate weights; in the
+    // weighted branch those are square roots while the search compares
+    // squared distances, so seeding is left out there.
+    const bool seed = pruned && !weighted && kq > 1;
+    if (seed) {
         FH_TRY(ensure(ctx, ctx->nb_pos, (size_t)n * (kq - 1)));
         FH_TRY(ensure(ctx, ctx->nb_d, (size_t)n * (kq - 1)));
     }
     FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p, ctx->core_p.p,
-                   ms > 1 ? 1 : 0, pruned ? ctx->nb_pos.p : nullptr, pruned ? ctx->nb_d.p : nullptr));
+                   (ms > 1 && !weighted) ? 1 : 0, seed ? ctx->nb_pos.p : nullptr, seed ? ctx->nb_d.p : nullptr,
+                   weighted ? 1 : 0));
+    if (weighted)
+        FH_HIP(fh::launch_weighted_core(ni, kq, ctx->knn_i.p, ctx->knn_d.p, weights_d, ms, ctx->posof.p, ctx->core_o.p,
+                                        ctx->core_p.p, st));
     FH_HIP(hipEventRecord(ctx->ev[2], st));
     // certification nodes: kCertDepth levels above the leaves
     const int cert_level = std::max(0, levels - 1 - kCertDepth);
@@ -422,8 +443,9 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, float *core_
         if (pruned) {
             // bounds from kNN cross edges, then a sampled search, then
             // certification of whole subtrees, then everyone else
-            FH_HIP(fh::launch_seed_bounds(n, kq - 1, ctx->nb_pos.p, ctx->nb_d.p, ctx->core_p.p, ctx->comp_p.p,
-                                          ctx->bound.p, st));
+            if (seed)
+                FH_HIP(fh::launch_seed_bounds(n, kq - 1, ctx->nb_pos.p, ctx->nb_d.p, ctx->core_p.p, ctx->comp_p.p,
+                                              ctx->bound.p, st));
             qa.pass = 1;
             FH_HIP(fh::launch_query(qa, st));
             FH_HIP(fh::launch_cert(ca, st));
@@ -544,7 +566,7 @@ void fh_destroy(fh_ctx *c) {
     release(c->rootslot); release(c->partner); release(c->ei); release(c->ej); release(c->tiles);
     release(c->small); release(c->heap_i); release(c->flag); release(c->out_src); release(c->out_dst);
     release(c->out_nbr); release(c->range); release(c->best);
-    release(c->xdev); release(c->tkeys); release(c->obox); release(c->taxis); release(c->tcnt);
+    release(c->xdev); release(c->wdev); release(c->tkeys); release(c->obox); release(c->taxis); release(c->tcnt);
     release(c->gtpos); release(c->ltpos); release(c->tactive); release(c->tseg);
     release(c->nb_pos); release(c->cert_of_pos); release(c->nb_d); release(c->mincore); release(c->cert);
     release(c->sk1); release(c->sk1s); release(c->sk2); release(c->sk2s); release(c->sv1); release(c->sv1s);
@@ -563,33 +585,60 @@ int64_t fh_tree_nnodes(int64_t n) {
     return (int64_t(1) << fh::tree_levels(n, kLeafSize)) - 1;
 }
 
-int fh_mst_euclidean_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, float *core_sq,
-                            int32_t *nbr, int32_t *src, int32_t *dst, float *w_sq, fh_stats *stats) {
-    FH_TRY(check_args(ctx, X, n, d, min_samples));
-    if (n > 1 && (!src || !dst || !w_sq)) return fail(ctx, FH_EINVAL, "edge outputs are null");
-    FH_HIP(hipSetDevice(ctx->device));
-    return run_mst(ctx, X, n, d, min_samples, core_sq, nbr, src, dst, w_sq, stats);
+int64_t fh_neighbor_columns(int64_t n, int32_t min_samples, const float *sample_weights) {
+    if (n < 1 || min_samples < 1) return -1;
+    if (!sample_weights) return min_samples > 1 ? min_samples : 1;
+    return weighted_k(sample_weights, n, min_samples) - 1;
 }
 
-int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, float *core_sq,
-                     int32_t *nbr, int32_t *src, int32_t *dst, float *w_sq, fh_stats *stats) {
+int fh_mst_euclidean_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples,
+                            const float *sample_weights, float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst,
+                            float *w_sq, fh_stats *stats) {
     FH_TRY(check_args(ctx, X, n, d, min_samples));
     if (n > 1 && (!src || !dst || !w_sq)) return fail(ctx, FH_EINVAL, "edge outputs are null");
     FH_HIP(hipSetDevice(ctx->device));
-    const int kn = min_samples > 1 ? min_samples : 1;
+    int kq_w = 0;
+    if (sample_weights) {
+        std::vector<float> hw(n);
+        FH_HIP(hipMemcpyAsync(hw.data(), sample_weights, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+        FH_HIP(hipStreamSynchronize(ctx->stream));
+        const int64_t k = weighted_k(hw.data(), n, min_samples);
+        if (k < 1 || k > 4096) return fail(ctx, FH_EUNSUPPORTED, "sample weights give a kNN width outside [1, 4096]");
+        kq_w = static_cast<int>(k);
+    }
+    return run_mst(ctx, X, n, d, min_samples, sample_weights, kq_w, core_sq, nbr, src, dst, w_sq, stats);
+}
+
+int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples,
+                     const float *sample_weights, float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst,
+                     float *w_sq, fh_stats *stats) {
+    FH_TRY(check_args(ctx, X, n, d, min_samples));
+    if (n > 1 && (!src || !dst || !w_sq)) return fail(ctx, FH_EINVAL, "edge outputs are null");
+    FH_HIP(hipSetDevice(ctx->device));
+    int kq_w = 0;
+    if (sample_weights) {
+        const int64_t k = weighted_k(sample_weights, n, min_samples);
+        if (k < 1 || k > 4096) return fail(ctx, FH_EUNSUPPORTED, "sample weights give a kNN width outside [1, 4096]");
+        kq_w = static_cast<int>(k);
+    }
+    const int kn = sample_weights ? kq_w - 1 : (min_samples > 1 ? min_samples : 1);
     const int64_t m = n - 1;
     FH_TRY(ensure(ctx, ctx->out_core, n));
-    FH_TRY(ensure(ctx, ctx->out_nbr, (size_t)n * kn));
+    FH_TRY(ensure(ctx, ctx->out_nbr, (size_t)n * (kn > 0 ? kn : 1)));
     FH_TRY(ensure(ctx, ctx->out_src, m));
     FH_TRY(ensure(ctx, ctx->out_dst, m));
     FH_TRY(ensure(ctx, ctx->out_w, m));
     FH_TRY(ensure(ctx, ctx->xdev, (size_t)n * d));
-    FH_HIP(hipMemcpyAsync(ctx->xdev.p, X, sizeof(float) * n * d, hipMemcpyHostToDevice, ctx->stream));
-    FH_TRY(run_mst(ctx, ctx->xdev.p, n, d, min_samples, ctx->out_core.p, ctx->out_nbr.p, ctx->out_src.p, ctx->out_dst.p,
-                   ctx->out_w.p, stats));
     hipStream_t st = ctx->stream;
+    FH_HIP(hipMemcpyAsync(ctx->xdev.p, X, sizeof(float) * n * d, hipMemcpyHostToDevice, st));
+    if (sample_weights) {
+        FH_TRY(ensure(ctx, ctx->wdev, n));
+        FH_HIP(hipMemcpyAsync(ctx->wdev.p, sample_weights, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    }
+    FH_TRY(run_mst(ctx, ctx->xdev.p, n, d, min_samples, sample_weights ? ctx->wdev.p : nullptr, kq_w,
+                   ctx->out_core.p, ctx->out_nbr.p, ctx->out_src.p, ctx->out_dst.p, ctx->out_w.p, stats));
     if (core_sq) FH_HIP(hipMemcpyAsync(core_sq, ctx->out_core.p, sizeof(float) * n, hipMemcpyDeviceToHost, st));
-    if (nbr) FH_HIP(hipMemcpyAsync(nbr, ctx->out_nbr.p, sizeof(int) * n * kn, hipMemcpyDeviceToHost, st));
+    if (nbr && kn > 0) FH_HIP(hipMemcpyAsync(nbr, ctx->out_nbr.p, sizeof(int) * n * kn, hipMemcpyDeviceToHost, st));
     if (m > 0) {
         FH_HIP(hipMemcpyAsync(src, ctx->out_src.p, sizeof(int) * m, hipMemcpyDeviceToHost, st));
         FH_HIP(hipMemcpyAsync(dst, ctx->out_dst.p, sizeof(int) * m, hipMemcpyDeviceToHost, st));

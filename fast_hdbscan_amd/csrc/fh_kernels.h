@@ -28,6 +28,7 @@ struct KnnArgs {
     int64_t heap_stride;
     int *nb_pos;            // optional: [nq, k-1] neighbour tree positions (columns 1..k-1)
     float *nb_d;            // optional: [nq, k-1] their squared distances
+    int sqrt_dist;          // 1: float32 square roots before the sort (output_rdist=False)
 };
 
 struct QueryArgs {
@@ -79,6 +80,8 @@ hipError_t launch_seed_bounds(int64_t n, int kn, const int *nb_pos, const float 
                               const int *comp_p, float *bound, hipStream_t st);
 hipError_t launch_query(const QueryArgs &a, hipStream_t st);
 hipError_t launch_posof(int n, const int *perm, int *posof, hipStream_t st);
+hipError_t launch_weighted_core(int n, int k, const int *knn_i, const float *knn_d, const float *weights, int ms,
+                                const int *posof, float *core_o, float *core_p, hipStream_t st);
 hipError_t launch_neighbors(int64_t n, int k, const int *knn_i, int *nbr, hipStream_t st);
 
 int scan_tiles(int m);

@@ -150,6 +150,10 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     v.hmax = kInf;
     v.d = a.d;
     walk<D>(a.box, a.levels, a.d, v);
+    if (a.sqrt_dist) {
+        // numba_kdtree.py:792: np.sqrt of the rows, then deheap_sort
+        for (int j = 0; j < k; ++j) h.P(j) = sqrtf(h.P(j));
+    }
     h.sort();
 
     // Output row: the original index of the query for the self-query, the
@@ -397,6 +401,26 @@ __global__ __launch_bounds__(256) void cert_kernel(CertArgs a) {
         }
     }
     a.cert[k] = ok;
+}
+
+// sample_weight_core_distance (boruvka.py:527-539): the first sorted
+// neighbour at which the float64 running weight reaches min_samples.  A -1
+// neighbour (k > n) reads the last weight, as numba's indexing does.
+__global__ void weighted_core_kernel(int n, int k, const int *__restrict__ knn_i, const float *__restrict__ knn_d,
+                                     const float *__restrict__ w, int ms, const int *__restrict__ posof,
+                                     float *core_o, float *core_p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double total = 0.0;
+    int j = 0;
+    while (total < (double)ms && j < k) {
+        const int nb = knn_i[(size_t)i * k + j];
+        total += (double)w[nb < 0 ? nb + n : nb];
+        ++j;
+    }
+    const float c = knn_d[(size_t)i * k + (j - 1)];
+    core_o[i] = c;
+    core_p[posof[i]] = c;
 }
 
 // ------------------------------------------------- kNN-seeded forest ----
@@ -844,6 +868,13 @@ hipError_t launch_seed_bounds(int64_t n, int kn, const int *nb_pos, const float 
 }
 
 hipError_t launch_query(const QueryArgs &a, hipStream_t st) { FH_DISPATCH_D(a.d, launch_query_t, a, st) }
+
+hipError_t launch_weighted_core(int n, int k, const int *knn_i, const float *knn_d, const float *weights, int ms,
+                                const int *posof, float *core_o, float *core_p, hipStream_t st) {
+    hipLaunchKernelGGL(weighted_core_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, k, knn_i, knn_d, weights, ms,
+                       posof, core_o, core_p);
+    return hipGetLastError();
+}
 
 hipError_t launch_posof(int n, const int *perm, int *posof, hipStream_t st) {
     hipLaunchKernelGGL(posof_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, perm, posof);

@@ -31,39 +31,53 @@ def _as_points(data) -> np.ndarray:
     return X
 
 
-def mst_arrays(X: np.ndarray, min_samples: int = 10, device: int = 0, stats: dict | None = None):
+def _neighbor_columns(n, ms, weights_host):
+    k = _lib.load().fh_neighbor_columns(n, ms, _lib.ptr(weights_host) if weights_host is not None else None)
+    if k < 0:
+        raise ValueError("invalid min_samples / sample weights")
+    return int(k)
+
+
+def mst_arrays(X: np.ndarray, min_samples: int = 10, device: int = 0, stats: dict | None = None,
+               sample_weights=None):
     """Raw library outputs for host float32 points.
 
     Returns (src int32[n-1], dst int32[n-1], w_sq float32[n-1],
-    core_sq float32[n], neighbors int32[n, max(min_samples, 1)]).
+    core_sq float32[n], neighbors int32[n, k]).
     """
     X = np.ascontiguousarray(X, dtype=np.float32)
     n, d = X.shape
     ms = int(min_samples)
-    kn = ms if ms > 1 else 1
+    w = None if sample_weights is None else np.ascontiguousarray(sample_weights, dtype=np.float32)
+    if w is not None and w.shape != (n,):
+        raise ValueError("sample_weights must have one value per point")
+    kn = _neighbor_columns(n, ms, w)
     m = max(n - 1, 0)
     src = np.empty(max(m, 1), np.int32)
     dst = np.empty(max(m, 1), np.int32)
-    w = np.empty(max(m, 1), np.float32)
+    wsq = np.empty(max(m, 1), np.float32)
     core = np.empty(n, np.float32)
-    nbr = np.empty((n, kn), np.int32)
+    nbr = np.empty((n, max(kn, 0)), np.int32)
     st = _lib.FHStats()
     ctx = _lib.context(device)
     code = _lib.load().fh_mst_euclidean(
-        ctx, _lib.ptr(X), n, d, ms, _lib.ptr(core), _lib.ptr(nbr), _lib.ptr(src), _lib.ptr(dst), _lib.ptr(w),
-        ctypes.byref(st),
+        ctx, _lib.ptr(X), n, d, ms, _lib.ptr(w) if w is not None else None, _lib.ptr(core),
+        _lib.ptr(nbr) if kn > 0 else None, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(wsq), ctypes.byref(st),
     )
     _lib.check(ctx, code)
     if stats is not None:
         stats.update(st.as_dict())
-    return src[:m], dst[:m], w[:m], core, nbr
+    return src[:m], dst[:m], wsq[:m], core, nbr
 
 
-def mst_device(X, min_samples: int = 10, stats: dict | None = None, out: dict | None = None):
+def mst_device(X, min_samples: int = 10, stats: dict | None = None, out: dict | None = None,
+               sample_weights=None):
     """MST of a float32 torch tensor resident on a HIP device.
 
     Returns torch tensors on the same device: (src, dst, w_sq, core_sq,
     neighbors).  ``out`` may hold preallocated tensors under those names.
+    ``sample_weights`` (float32 tensor on the same device) selects the
+    weighted-core branch.
     """
     import torch
 
@@ -74,7 +88,14 @@ def mst_device(X, min_samples: int = 10, stats: dict | None = None, out: dict | 
     X = X.contiguous()
     n, d = X.shape
     ms = int(min_samples)
-    kn = ms if ms > 1 else 1
+    wt = None
+    if sample_weights is not None:
+        wt = torch.as_tensor(sample_weights, dtype=torch.float32, device=X.device).contiguous()
+        if wt.shape != (n,):
+            raise ValueError("sample_weights must have one value per point")
+        kn = _neighbor_columns(n, ms, wt.cpu().numpy())
+    else:
+        kn = ms if ms > 1 else 1
     m = max(n - 1, 0)
     dev = X.device
     o = out if out is not None else {}
@@ -89,7 +110,8 @@ def mst_device(X, min_samples: int = 10, stats: dict | None = None, out: dict | 
     st = _lib.FHStats()
     ctx = _lib.context(dev.index or 0)
     code = _lib.load().fh_mst_euclidean_device(
-        ctx, ctypes.c_void_p(X.data_ptr()), n, d, ms, ctypes.c_void_p(core.data_ptr()),
+        ctx, ctypes.c_void_p(X.data_ptr()), n, d, ms, ctypes.c_void_p(wt.data_ptr()) if wt is not None else None,
+        ctypes.c_void_p(core.data_ptr()),
         ctypes.c_void_p(nbr.data_ptr()), ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
         ctypes.c_void_p(w.data_ptr()), ctypes.byref(st),
     )
@@ -128,7 +150,9 @@ def compute_minimum_spanning_tree(
     Returns ``(mst_edges f64[n-1, 3], neighbors int32[n, k], core_distances
     float32[n])`` exactly as ``parallel_boruvka`` does (boruvka.py:646-647).
     ``reproducible`` is accepted and has no effect: the device path is
-    deterministic and equals the reference's single-threaded run.  Data with
+    deterministic and equals the reference's single-threaded run.
+    ``sample_weights`` selects the reference's weighted-core branch
+    (boruvka.py:564-578), quirks included.  Data with
     more than 30 columns stays on the exact KD-tree path (the reference sends
     it to approximate pynndescent, hdbscan.py:447-463).
     """
@@ -154,14 +178,11 @@ def compute_minimum_spanning_tree(
         raise NotImplementedError(f"metric={metric!r} is not on the device MST path (euclidean only)")
     if algorithm == "kruskal":
         raise NotImplementedError("algorithm='kruskal' is not on the device MST path")
-    if sample_weights is not None:
-        raise NotImplementedError("sample_weights are not on the device MST path yet")
-
     try:
         import torch
 
         if isinstance(data, torch.Tensor) and data.is_cuda:
-            src, dst, w, core_sq, nbr = mst_device(data.float(), min_samples)
+            src, dst, w, core_sq, nbr = mst_device(data.float(), min_samples, sample_weights=sample_weights)
             return (
                 _edges_f64(src.cpu().numpy(), dst.cpu().numpy(), w.cpu().numpy()),
                 nbr.cpu().numpy(),
@@ -171,8 +192,10 @@ def compute_minimum_spanning_tree(
         pass
 
     X = _as_points(data)
-    src, dst, w, core_sq, nbr = mst_arrays(X, min_samples)
-    return _edges_f64(src, dst, w), nbr, np.sqrt(core_sq)
+    # boruvka.py:564-578: weights switch to the weighted-core branch
+    w = None if sample_weights is None else np.asarray(sample_weights, dtype=np.float32)
+    src, dst, wsq, core_sq, nbr = mst_arrays(X, min_samples, sample_weights=w)
+    return _edges_f64(src, dst, wsq), nbr, np.sqrt(core_sq)
 
 
 def parallel_tree_query(data, queries, k=10, output_rdist=False, device: int = 0):

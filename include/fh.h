@@ -79,17 +79,24 @@ const char *fh_version(void);
  * Mutual-reachability minimum spanning tree, inputs resident in HBM.
  *   Replaces: hdbscan.py:466 build_kdtree(data) +
  *             hdbscan.py:482-494 parallel_boruvka(tree, n_threads,
- *             min_samples, sample_weights=empty(1), reproducible)
- *             (boruvka.py:552-647, the sample-weight-free branches).
+ *             min_samples, sample_weights, reproducible)
+ *             (boruvka.py:552-647, all three branches).
  *
  * All pointers are DEVICE pointers on the context's device:
  *   X        [n*d]   float32 points, row-major (boruvka casts with
  *                    astype(float32), numba_kdtree.py:481; pass float32)
+ *   sample_weights [n] float32 or NULL.  With weights the kNN width is
+ *                    k = int(2 * min_samples / mean(w)) (boruvka.py:564-569),
+ *                    distances are square-rooted before sorting and the
+ *                    core is sample_weight_core_distance (527-539); the
+ *                    returned core_sq and weights then hold the reference's
+ *                    unsquared values, to which the caller still applies
+ *                    its square root (boruvka.py:646-647).
  *   core_sq  [n]     squared core distance of each point: the
  *                    (min_samples+1)-th smallest rdist including the point
  *                    itself (boruvka.py:581-584); zeros when min_samples==1
  *                    (boruvka.py:592)
- *   nbr      [n*kn]  kn = max(min_samples, 1); kNN columns 1..kn of the
+ *   nbr      [n*kn]  kn = fh_neighbor_columns(); kNN columns 1..kn of the
  *                    sorted query (boruvka.py:647 neighbors[:, 1:]), -1
  *                    where fewer than kn+1 points exist
  *   src,dst  [n-1]   edge endpoints (original point indices; dst may be -1
@@ -98,12 +105,18 @@ const char *fh_version(void);
  * stats may be NULL.
  */
 int fh_mst_euclidean_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples,
-                            float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst, float *w_sq,
-                            fh_stats *stats);
+                            const float *sample_weights, float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst,
+                            float *w_sq, fh_stats *stats);
 
 /* Same contract with HOST pointers; the library copies in and out. */
-int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, float *core_sq,
-                     int32_t *nbr, int32_t *src, int32_t *dst, float *w_sq, fh_stats *stats);
+int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples,
+                     const float *sample_weights, float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst,
+                     float *w_sq, fh_stats *stats);
+
+/* Width of the neighbour output: min_samples (1 when min_samples == 1), or
+ * int(2 * min_samples / mean(w)) - 1 with HOST sample weights
+ * (boruvka.py:564-569, 647).  -1 for invalid arguments. */
+int64_t fh_neighbor_columns(int64_t n, int32_t min_samples, const float *sample_weights);
 
 /*
  * Exact k nearest neighbours of Q's rows among X's rows.

@@ -406,9 +406,29 @@ static void knn_rec(const tree_view *t, int64_t node, const float *pt, float *hp
 
 /* numba_kdtree.py:774-792 with output_rdist=True: squared distances,
  * ascending per row after deheap_sort.  Q holds nq query rows. */
+static int knn_impl(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
+                    const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
+                    const float *Q, int64_t nq, int k, float *out_d, int32_t *out_i, int nthreads,
+                    int output_rdist);
+
 int orc_knn(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
             const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
             const float *Q, int64_t nq, int k, float *out_d, int32_t *out_i, int nthreads) {
+    return knn_impl(X, n, d, idx_array, idx_start, idx_end, is_leaf, bounds, N, Q, nq, k, out_d, out_i, nthreads, 1);
+}
+
+/* numba_kdtree.py:774-792 with output_rdist=False: the float32 square root
+ * is applied to the heap rows before deheap_sort (792). */
+int orc_knn_sqrt(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
+                 const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
+                 const float *Q, int64_t nq, int k, float *out_d, int32_t *out_i, int nthreads) {
+    return knn_impl(X, n, d, idx_array, idx_start, idx_end, is_leaf, bounds, N, Q, nq, k, out_d, out_i, nthreads, 0);
+}
+
+static int knn_impl(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
+                    const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
+                    const float *Q, int64_t nq, int k, float *out_d, int32_t *out_i, int nthreads,
+                    int output_rdist) {
     (void)n;
     tree_view t = {X, d, idx_array, idx_start, idx_end, is_leaf, bounds, N};
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
@@ -422,6 +442,8 @@ int orc_knn(const float *X, int64_t n, int d, const int64_t *idx_array, const in
         const float *pt = Q + q * d;
         float lb = box_lb(node_lo(&t, 0), node_hi(&t, 0), pt, d);
         knn_rec(&t, 0, pt, hp, hi, k, lb);
+        if (!output_rdist)
+            for (int j = 0; j < k; ++j) hp[j] = sqrtf(hp[j]);
         deheap_row(hp, hi, k);
     }
     return 0;
@@ -654,11 +676,58 @@ static int64_t merge_components(int32_t *parent, int32_t *rank, int64_t n, const
  *   knn_i, knn_d   : n x kq raw sorted kNN rows, kq = max(min_samples+1, 2)
  *   rounds         : number of Boruvka rounds executed
  * Returns the number of edges written. */
+static int64_t boruvka_rounds(tree_view t, int64_t n, const int32_t *knn_i, const float *knn_d, int kq,
+                              const float *core_sq, int64_t *src, int64_t *dst, float *w_sq, int32_t *rounds);
+
 int64_t orc_boruvka(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
                     const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
                     int min_samples, int64_t *src, int64_t *dst, float *w_sq, float *core_sq, int32_t *knn_i,
                     float *knn_d, int32_t *rounds, int nthreads) {
     tree_view t = {X, d, idx_array, idx_start, idx_end, is_leaf, bounds, N};
+    int kq = min_samples > 1 ? min_samples + 1 : 2;
+    orc_knn(X, n, d, idx_array, idx_start, idx_end, is_leaf, bounds, N, X, n, kq, knn_d, knn_i, nthreads);
+    for (int64_t i = 0; i < n; ++i) core_sq[i] = min_samples > 1 ? knn_d[i * kq + kq - 1] : 0.0f;
+    return boruvka_rounds(t, n, knn_i, knn_d, kq, core_sq, src, dst, w_sq, rounds);
+}
+
+/* boruvka.py:564-566: k = int(2 * min_samples / mean(w)).  numba's mean of
+ * a float32 array accumulates in float32 in index order and divides by the
+ * size in float64. */
+int64_t orc_weighted_k(const float *w, int64_t n, int min_samples) {
+    float acc = 0.0f;
+    for (int64_t i = 0; i < n; ++i) acc += w[i];
+    double mean = (double)acc / (double)n;
+    double expected = (double)min_samples / mean;
+    return (int64_t)(2.0 * expected);
+}
+
+/* boruvka.py:564-578 + sample_weight_core_distance (527-539): the kNN uses
+ * square-rooted distances (output_rdist=False), the core distance is the
+ * first neighbour at which the cumulative weight reaches min_samples, and
+ * those unsquared values then feed the same forest and rounds. */
+int64_t orc_boruvka_weighted(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
+                             const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
+                             int min_samples, const float *sample_weights, int kq, int64_t *src, int64_t *dst,
+                             float *w_sq, float *core_sq, int32_t *knn_i, float *knn_d, int32_t *rounds,
+                             int nthreads) {
+    tree_view t = {X, d, idx_array, idx_start, idx_end, is_leaf, bounds, N};
+    orc_knn_sqrt(X, n, d, idx_array, idx_start, idx_end, is_leaf, bounds, N, X, n, kq, knn_d, knn_i, nthreads);
+    for (int64_t i = 0; i < n; ++i) {
+        double total = 0.0;
+        int j = 0;
+        while (total < (double)min_samples && j < kq) {
+            int32_t nb = knn_i[i * kq + j];
+            total += (double)sample_weights[nb < 0 ? nb + n : nb];
+            ++j;
+        }
+        core_sq[i] = knn_d[i * kq + (j - 1)];
+    }
+    return boruvka_rounds(t, n, knn_i, knn_d, kq, core_sq, src, dst, w_sq, rounds);
+}
+
+static int64_t boruvka_rounds(tree_view t, int64_t n, const int32_t *knn_i, const float *knn_d, int kq,
+                              const float *core_sq, int64_t *src, int64_t *dst, float *w_sq, int32_t *rounds) {
+    const int64_t N = t.N;
     int32_t *parent = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
     int32_t *rank = (int32_t *)calloc((size_t)n, sizeof(int32_t));
     int64_t *pcomp = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
@@ -668,10 +737,6 @@ int64_t orc_boruvka(const float *X, int64_t n, int d, const int64_t *idx_array, 
         pcomp[i] = i;
     }
     for (int64_t i = 0; i < N; ++i) ncomp[i] = -1;
-
-    int kq = min_samples > 1 ? min_samples + 1 : 2;
-    orc_knn(X, n, d, idx_array, idx_start, idx_end, is_leaf, bounds, N, X, n, kq, knn_d, knn_i, nthreads);
-    for (int64_t i = 0; i < n; ++i) core_sq[i] = min_samples > 1 ? knn_d[i * kq + kq - 1] : 0.0f;
 
     edge_list E = {src, dst, w_sq, 0};
     init_from_knn(knn_i, knn_d, n, kq, core_sq, parent, rank, &E);
@@ -700,8 +765,8 @@ int64_t orc_boruvka(const float *X, int64_t n, int d, const int64_t *idx_array, 
             B[i] = INFINITY;
         }
         for (int64_t i = 0; i < n; ++i) {
-            const float *pt = X + i * d;
-            float lb = box_lb(node_lo(&t, 0), node_hi(&t, 0), pt, d);
+            const float *pt = t.X + i * t.d;
+            float lb = box_lb(node_lo(&t, 0), node_hi(&t, 0), pt, t.d);
             bq_rec(&c, 0, pt, cand_d + i, cand_i + i, core_sq[i], pcomp[i], lb, B + pcomp[i]);
         }
         int64_t added = merge_components(parent, rank, n, cand_i, cand_d, pcomp, &E, slot, order, ei, ej, ew);

@@ -94,6 +94,13 @@ def lib():
         L.orc_boruvka.restype = ctypes.c_int64
         L.orc_boruvka.argtypes = tree_args + [ctypes.c_int, _i64p, _i64p, _f32p, _f32p, _i32p, _f32p,
                                               ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+        L.orc_weighted_k.restype = ctypes.c_int64
+        L.orc_weighted_k.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int]
+        L.orc_boruvka_weighted.restype = ctypes.c_int64
+        L.orc_boruvka_weighted.argtypes = tree_args + [ctypes.c_int, _f32p, ctypes.c_int, _i64p, _i64p, _f32p, _f32p,
+                                                       _i32p, _f32p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+        L.orc_knn_sqrt.restype = ctypes.c_int
+        L.orc_knn_sqrt.argtypes = tree_args + [_f32p, ctypes.c_int64, ctypes.c_int, _f32p, _i32p, ctypes.c_int]
         L.orc_rdist.restype = ctypes.c_float
         L.orc_rdist.argtypes = [_f32p, _f32p, ctypes.c_int]
         L.orc_box_lb.restype = ctypes.c_float
@@ -216,11 +223,35 @@ def boruvka(tree: Tree, min_samples: int = 10, nthreads=None) -> BoruvkaResult:
     return BoruvkaResult(src[:cnt], dst[:cnt], w[:cnt], core, ki, kd, int(rounds.value))
 
 
-def compute_minimum_spanning_tree(data, min_samples: int = 10, nthreads=None):
+def boruvka_weighted(tree: Tree, min_samples: int, sample_weights, nthreads=None) -> BoruvkaResult:
+    """parallel_boruvka's sample-weight branch (boruvka.py:564-578, 527-539)
+    with one-thread semantics."""
+    n = tree.data.shape[0]
+    w = np.ascontiguousarray(np.asarray(sample_weights, dtype=np.float32))
+    kq = int(lib().orc_weighted_k(w, n, int(min_samples)))
+    if kq < 1:
+        raise ValueError("sample weights give a kNN width below 1")
+    m = max(n - 1, 0) + 1
+    src = np.empty(m, np.int64)
+    dst = np.empty(m, np.int64)
+    ww = np.empty(m, np.float32)
+    core = np.empty(n, np.float32)
+    ki = np.empty((n, kq), np.int32)
+    kd = np.empty((n, kq), np.float32)
+    rounds = ctypes.c_int32(0)
+    cnt = lib().orc_boruvka_weighted(*tree._args(), int(min_samples), w, kq, src, dst, ww, core, ki, kd,
+                                     ctypes.byref(rounds), _threads(nthreads))
+    return BoruvkaResult(src[:cnt], dst[:cnt], ww[:cnt], core, ki, kd, int(rounds.value))
+
+
+def compute_minimum_spanning_tree(data, min_samples: int = 10, nthreads=None, sample_weights=None):
     """The Euclidean Boruvka branch of hdbscan.compute_minimum_spanning_tree
     (hdbscan.py:466,482-494): returns (edges f64[n-1,3], neighbors i32[n,ms'],
     core_distances f32[n])."""
     tree = build_kdtree(data)
-    r = boruvka(tree, min_samples, nthreads)
+    if sample_weights is None:
+        r = boruvka(tree, min_samples, nthreads)
+    else:
+        r = boruvka_weighted(tree, min_samples, sample_weights, nthreads)
     core = np.sqrt(r.core_sq)
     return r.edges(), r.knn_i[:, 1:], core

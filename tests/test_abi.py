@@ -40,7 +40,7 @@ def test_null_context_is_rejected():
     L = _lib.load()
     X = np.zeros((4, 2), np.float32)
     out = np.zeros(4, np.float32)
-    code = L.fh_mst_euclidean(None, _lib.ptr(X), 4, 2, 2, _lib.ptr(out), None, None, None, None, None)
+    code = L.fh_mst_euclidean(None, _lib.ptr(X), 4, 2, 2, None, _lib.ptr(out), None, None, None, None, None)
     assert code == _lib.FH_EINVAL
 
 

@@ -183,3 +183,42 @@ def test_repeatable():
     b = _device(X, 10)[0]
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+@pytest.mark.parametrize("kind", ["unit", "uniform", "labels", "sparse_zeros"])
+def test_sample_weight_branch(kind):
+    from oracle import oracle as O
+    from fast_hdbscan_amd.hdbscan import mst_arrays
+
+    rng = np.random.default_rng(11)
+    X = workloads.blobs_small(6000, seed=12)
+    if kind == "unit":
+        w = np.ones(6000, np.float32)
+    elif kind == "uniform":
+        w = rng.uniform(0.5, 2.0, 6000).astype(np.float32)
+    elif kind == "labels":
+        w = rng.integers(0, 3, 6000).astype(np.float32)
+    else:
+        w = (rng.uniform(0, 1, 6000) > 0.7).astype(np.float32) * 3.0
+    print(f"weighted case {kind}", flush=True)
+    src, dst, wsq, core, nbr = mst_arrays(X, 10, sample_weights=w)
+    ref = O.boruvka_weighted(O.build_kdtree(X), 10, w)
+    np.testing.assert_array_equal(core, ref.core_sq)
+    np.testing.assert_array_equal(nbr, ref.knn_i[:, 1:])
+    np.testing.assert_array_equal(src, ref.src)
+    np.testing.assert_array_equal(dst, ref.dst)
+    np.testing.assert_array_equal(wsq, ref.w_sq)
+
+
+def test_sample_weight_seam_contract():
+    from oracle import oracle as O
+    from fast_hdbscan_amd import compute_minimum_spanning_tree
+
+    X = workloads.blobs_small(2000, seed=2).astype(np.float64)
+    w = np.random.default_rng(1).uniform(0.5, 1.5, 2000).astype(np.float32)
+    e, nb, core = compute_minimum_spanning_tree(X, min_samples=5, sample_weights=w)
+    re, rnb, rcore = O.compute_minimum_spanning_tree(X, 5, sample_weights=w)
+    np.testing.assert_array_equal(e, re)
+    np.testing.assert_array_equal(nb, rnb)
+    np.testing.assert_array_equal(core, rcore)
+    assert e.dtype == np.float64 and nb.dtype == np.int32 and core.dtype == np.float32

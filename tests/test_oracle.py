@@ -301,3 +301,44 @@ def test_boruvka_rounds_rederived_without_pruning(case):
     np.testing.assert_array_equal(r.src, [a for a, _, _ in rows])
     np.testing.assert_array_equal(r.dst, [b for _, b, _ in rows])
     np.testing.assert_array_equal(r.w_sq, np.array([w for _, _, w in rows], np.float32))
+
+
+# ------------------------------------------------- sample-weight branch ----
+
+def test_weighted_branch_unit_weights_core():
+    # boruvka.py:564-578 with unit weights: k = int(2 * ms / 1.0); the core is
+    # the square root of the ms-th smallest rdist (cumulative weight reaches
+    # ms at column ms-1, self included).
+    X = _blobs(3000, seed=8)
+    ms = 7
+    T = O.build_kdtree(X)
+    r = O.boruvka_weighted(T, ms, np.ones(3000, np.float32))
+    assert r.knn_i.shape[1] == 2 * ms
+    rd, _ = O.tree_query(T, X, 2 * ms)
+    np.testing.assert_array_equal(r.core_sq, np.sqrt(rd[:, ms - 1]))
+    assert r.src.shape[0] == X.shape[0] - 1
+    assert _spanning(X.shape[0], r.src, r.dst)
+
+
+def test_weighted_k_float32_mean():
+    # numba's np.mean on float32 accumulates in float32 in index order
+    w = np.full(1_000_001, 0.1, np.float32)
+    acc = np.float32(0.0)
+    for v in w[:1000]:
+        acc = np.float32(acc + v)
+    k = O.lib().orc_weighted_k(w[:1000], 1000, 5)
+    assert k == int(2.0 * (5 / (float(acc) / 1000)))
+
+
+def test_weighted_branch_label_weights_spanning():
+    # test_hdbscan.py:113-117 uses the blob labels (0, 1, 2) as weights
+    from sklearn.datasets import make_blobs
+    from sklearn.preprocessing import StandardScaler
+    from sklearn.utils import shuffle
+
+    X, y = make_blobs(n_samples=200, random_state=10)
+    X, y = shuffle(X, y, random_state=7)
+    X = StandardScaler().fit_transform(X)
+    r = O.boruvka_weighted(O.build_kdtree(X), 10, y.astype(np.float32))
+    assert r.src.shape[0] == 199
+    assert _spanning(200, r.src, r.dst)
