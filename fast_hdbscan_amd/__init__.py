@@ -13,3 +13,16 @@ from .hdbscan import (  # noqa: F401
 )
 
 __version__ = "0.1.0"
+
+
+def __getattr__(name):
+    # the estimators pull in scikit-learn; load them on first use
+    if name in ("HDBSCAN", "PLSCAN", "fast_hdbscan"):
+        from . import estimator
+
+        return getattr(estimator, name)
+    if name in ("build_cluster_layers", "build_layer_cluster_tree"):
+        from . import layers
+
+        return getattr(layers, name)
+    raise AttributeError(name)

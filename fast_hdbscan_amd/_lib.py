@@ -35,6 +35,8 @@ EXPORTED_SYMBOLS = (
     "fh_labels_from_sorted_mst",
     "fh_sort_mst_device",
     "fh_neighbor_columns",
+    "fh_cluster_layers",
+    "fh_layer_cluster_tree",
 )
 
 
@@ -76,6 +78,29 @@ class FHCondensedTree(ctypes.Structure):
         ("child_size", ctypes.c_void_p),
         ("capacity", ctypes.c_int64),
         ("rows", ctypes.c_int64),
+    ]
+
+
+class FHLayerParams(ctypes.Structure):
+    _fields_ = [
+        ("base_min_cluster_size", ctypes.c_int64),
+        ("base_n_clusters", ctypes.c_int64),
+        ("layer_similarity_threshold", ctypes.c_double),
+        ("max_layers", ctypes.c_int32),
+    ]
+
+
+class FHLayers(ctypes.Structure):
+    _fields_ = [
+        ("labels", ctypes.c_void_p),
+        ("strengths", ctypes.c_void_p),
+        ("persistence", ctypes.c_void_p),
+        ("sizes", ctypes.c_void_p),
+        ("total_persistence", ctypes.c_void_p),
+        ("capacity", ctypes.c_int64),
+        ("n_layers", ctypes.c_int32),
+        ("n_sizes", ctypes.c_int64),
+        ("min_cluster_size", ctypes.c_int64),
     ]
 
 
@@ -130,6 +155,10 @@ def load() -> ctypes.CDLL:
                                                 ctypes.POINTER(FHCondensedTree), _vp]
         L.fh_neighbor_columns.restype = _i64
         L.fh_neighbor_columns.argtypes = [_i64, _i32, _vp]
+        L.fh_cluster_layers.restype = ctypes.c_int
+        L.fh_cluster_layers.argtypes = [_i64, _vp, _vp, _vp, ctypes.POINTER(FHLayerParams), ctypes.POINTER(FHLayers)]
+        L.fh_layer_cluster_tree.restype = _i64
+        L.fh_layer_cluster_tree.argtypes = [_i32, _i64, _vp, _vp, _i64]
         L.fh_sort_mst_device.restype = ctypes.c_int
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L

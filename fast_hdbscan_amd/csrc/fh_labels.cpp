@@ -21,6 +21,8 @@
 #include <limits>
 #include <new>
 #include <thread>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/fh.h"
@@ -47,7 +49,14 @@ inline void prefetch(const void *p) { __builtin_prefetch(p, 0, 1); }
 // linkage_merge_find (38-52).  Used when an endpoint is -1 (only when
 // min_samples >= n), which the reference resolves through wrap-around
 // indexing into the merge slots.
-void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, Hierarchy &h) {
+// delta(i): the merge distance of sorted row i -- the float64 square root of
+// a squared float32 weight (boruvka.py:646), or a given float64 distance.
+inline double row_delta(const float *w_sq, const double *dist, int64_t i) {
+    return dist ? dist[i] : std::sqrt(static_cast<double>(w_sq[i]));
+}
+
+void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, const double *dist,
+                          Hierarchy &h) {
     const int64_t m = n - 1;
     const int64_t total = 2 * n - 1;
     std::vector<int64_t> parent(total, -1);
@@ -78,7 +87,7 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
         r.right = static_cast<int32_t>(lbig ? rc : lc);
         r.left_size = static_cast<int32_t>(S(lbig ? lc : rc));
         r.right_size = static_cast<int32_t>(S(lbig ? rc : lc));
-        r.delta = std::sqrt(static_cast<double>(w_sq[i]));  // boruvka.py:646
+        r.delta = row_delta(w_sq, dist, i);
         S(next) = S(lc) + S(rc);
         P(lc) = next;
         P(rc) = next;
@@ -90,10 +99,11 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
 // halving) that records, per set, the merge node currently representing
 // it: the node linkage_merge_find would return.  Upcoming edges' entries
 // are prefetched.
-void linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, Hierarchy &h) {
+void linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, const double *dist,
+                  Hierarchy &h) {
     const int64_t m = n - 1;
     for (int64_t i = 0; i < m; ++i)
-        if (src[i] < 0 || dst[i] < 0) return linkage_tree_literal(n, src, dst, w_sq, h);
+        if (src[i] < 0 || dst[i] < 0) return linkage_tree_literal(n, src, dst, w_sq, dist, h);
     struct Set {
         int32_t parent, size, node, pad;
     };
@@ -126,7 +136,7 @@ void linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         r.right = lbig ? rc : lc;
         r.left_size = lbig ? ls : rs;
         r.right_size = lbig ? rs : ls;
-        r.delta = std::sqrt(static_cast<double>(w_sq[i]));  // boruvka.py:646
+        r.delta = row_delta(w_sq, dist, i);
         int32_t root = a, child = b;
         if (uf[a].size < uf[b].size) {
             root = b;
@@ -146,10 +156,14 @@ struct Condensed {
 };
 
 // cluster_trees.py:205-352 (unit weights, max_cluster_size = inf: the
-// reference does not pass it here, hdbscan.py:525-527).  Rows come out in
-// the reference's order: merge nodes in breadth-first order from the root,
-// each eliminated branch's points in breadth-first order of that branch.
-void condense(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c) {
+// reference does not pass it here, hdbscan.py:525-527, layer_clusters.py:
+// 416-424).  Rows come out in the reference's order: merge nodes in
+// breadth-first order from the root, each eliminated branch's points in
+// breadth-first order of that branch.  lambda: 1/d (hdbscan_lambda_value,
+// cluster_trees.py:192-197) or exp(-d) (plscan_lambda_value, 200-202).
+enum LambdaMethod { kLambdaHdbscan = 0, kLambdaPlscan = 1 };
+
+void condense(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, int lambda_method = kLambdaHdbscan) {
     const int64_t m = n - 1;
     const int32_t root = static_cast<int32_t>(2 * m);
     const int32_t num_points = static_cast<int32_t>(n);
@@ -226,7 +240,7 @@ void condense(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c) {
         const Merge &r = rows[node - num_points];
         const int32_t left = r.left, right = r.right;
         const double d = r.delta;
-        const double lam = d > 0.0 ? 1.0 / d : kInfD;
+        const double lam = lambda_method == kLambdaHdbscan ? (d > 0.0 ? 1.0 / d : kInfD) : std::exp(-d);
         // counts as float32, compared in float64 as numba does
         const float lc = static_cast<float>(r.left_size), rc = static_cast<float>(r.right_size);
         const double dl = lc, dr = rc;
@@ -592,6 +606,264 @@ void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps
     // point) for the strength, which is noise in that configuration.
 }
 
+// ---------------------------------------------------------------- PLSCAN ---
+
+// numba's np.argsort(kind="quicksort") (numba/misc/quicksort.py:
+// run_quicksort1, partition, insertion_sort): median-of-three pivot parked
+// at the end, Hoare-style scan, larger side pushed, insertion sort below 15
+// elements.  Equal keys keep the order this procedure leaves them in, which
+// is what the reference's numba-compiled sorts return.  Float keys order NaN
+// last, as numba's float comparison for sorting does.
+template <class T>
+std::vector<int64_t> numba_argsort(const T *a, int64_t n) {
+    auto lt = [](T x, T y) {
+        if constexpr (std::is_floating_point<T>::value)
+            return x < y || (std::isnan(y) && !std::isnan(x));
+        else
+            return x < y;
+    };
+    std::vector<int64_t> R(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) R[i] = i;
+    if (n < 2) return R;
+    auto A = [&](int64_t r) { return a[R[r]]; };
+    auto insertion = [&](int64_t low, int64_t high) {
+        if (high <= low) return;
+        for (int64_t i = low + 1; i <= high; ++i) {
+            const int64_t k = R[i];
+            const T v = a[k];
+            int64_t j = i;
+            while (j > low && lt(v, a[R[j - 1]])) {
+                R[j] = R[j - 1];
+                --j;
+            }
+            R[j] = k;
+        }
+    };
+    auto partition = [&](int64_t low, int64_t high) {
+        const int64_t mid = (low + high) >> 1;
+        if (lt(A(mid), A(low))) std::swap(R[low], R[mid]);
+        if (lt(A(high), A(mid))) std::swap(R[high], R[mid]);
+        if (lt(A(mid), A(low))) std::swap(R[low], R[mid]);
+        const T pivot = A(mid);
+        std::swap(R[high], R[mid]);
+        int64_t i = low, j = high - 1;
+        for (;;) {
+            while (i < high && lt(A(i), pivot)) ++i;
+            while (j >= low && lt(pivot, A(j))) --j;
+            if (i >= j) break;
+            std::swap(R[i], R[j]);
+            ++i;
+            --j;
+        }
+        std::swap(R[i], R[high]);
+        return i;
+    };
+    constexpr int64_t kSmall = 15;
+    std::vector<std::pair<int64_t, int64_t>> stack;
+    stack.emplace_back(0, n - 1);
+    while (!stack.empty()) {
+        int64_t low = stack.back().first, high = stack.back().second;
+        stack.pop_back();
+        while (high - low >= kSmall) {
+            const int64_t i = partition(low, high);
+            if (high - i > i - low) {
+                if (high > i) stack.emplace_back(i + 1, high);
+                high = i - 1;
+            } else {
+                if (i > low) stack.emplace_back(low, i - 1);
+                low = i + 1;
+            }
+        }
+        insertion(low, high);
+    }
+    return R;
+}
+
+// extract_leaves (cluster_trees.py:356-366): ids in [min parent, max parent]
+// that have no child row of size > 1.
+std::vector<int64_t> extract_leaves(const Condensed &c) {
+    if (c.rows == 0) return {};
+    const int64_t lo = *std::min_element(c.parent.begin(), c.parent.begin() + c.rows);
+    const int64_t hi = *std::max_element(c.parent.begin(), c.parent.begin() + c.rows);
+    std::vector<uint8_t> leaf(static_cast<size_t>(hi - lo + 1), 1);
+    for (int64_t r = 0; r < c.rows; ++r)
+        if (c.size[r] > 1.0f) leaf[c.parent[r] - lo] = 0;
+    std::vector<int64_t> out;
+    for (int64_t i = 0; i <= hi - lo; ++i)
+        if (leaf[i]) out.push_back(lo + i);
+    return out;
+}
+
+// find_peaks (layer_clusters.py:26-68)
+std::vector<int64_t> find_peaks(const std::vector<float> &x) {
+    std::vector<int64_t> mids;
+    const int64_t i_max = static_cast<int64_t>(x.size()) - 1;
+    int64_t i = 1;
+    while (i < i_max) {
+        if (x[i - 1] < x[i]) {
+            int64_t ia = i + 1;
+            while (ia < i_max && x[ia] == x[i]) ++ia;
+            if (x[ia] < x[i]) {
+                mids.push_back((i + ia - 1) / 2);
+                i = ia;
+            }
+        }
+        ++i;
+    }
+    return mids;
+}
+
+struct Barcode {
+    std::vector<float> births, deaths, lambda_deaths;
+};
+
+// min_cluster_size_barcode (layer_clusters.py:183-212) over the rows whose
+// child is a cluster; those come in sibling pairs with consecutive ids.
+Barcode barcode(const std::vector<int64_t> &parent, const std::vector<int64_t> &child, const std::vector<float> &lam,
+                const std::vector<float> &size, int64_t n, int64_t min_size) {
+    const int64_t rows = static_cast<int64_t>(child.size());
+    const int64_t n_nodes = child.back() - n + 1;
+    Barcode b;
+    b.births.assign(n_nodes, static_cast<float>(min_size));
+    b.deaths.assign(n_nodes, 0.0f);
+    b.lambda_deaths.assign(n_nodes, 0.0f);
+    b.deaths[0] = static_cast<float>(n);
+    for (int64_t idx = rows - 1; idx > 0; idx -= 2) {
+        const int64_t o = child[idx] - n;
+        const float lv = lam[idx];
+        const float ld = lv > 0.0f ? static_cast<float>(std::exp(-1.0 / static_cast<double>(lv))) : 0.0f;
+        b.lambda_deaths[o - 1] = b.lambda_deaths[o] = ld;
+        const float ds = std::min(size[idx - 1], size[idx]);
+        b.deaths[o - 1] = b.deaths[o] = ds;
+        b.births[parent[idx] - n] = std::max(std::max(b.births[o - 1], b.births[o]), ds);
+    }
+    return b;
+}
+
+// compute_total_persistence (layer_clusters.py:215-248): float32 sums in
+// node order over the sizes in [birth, death).
+void total_persistence(const Barcode &b, std::vector<float> &sizes, std::vector<float> &tp) {
+    sizes = b.births;
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    tp.assign(sizes.size(), 0.0f);
+    for (size_t i = 1; i < b.births.size(); ++i) {
+        const float birth = b.births[i], death = b.deaths[i];
+        if (death <= birth) continue;
+        const size_t bi = static_cast<size_t>(std::lower_bound(sizes.begin(), sizes.end(), birth) - sizes.begin());
+        const size_t di = static_cast<size_t>(std::lower_bound(sizes.begin(), sizes.end(), death) - sizes.begin());
+        // (a birth above every size leaves birth_idx at 0 in the reference)
+        const size_t b0 = bi == sizes.size() ? 0 : bi;
+        const float prod = birth * b.lambda_deaths[i];
+        const float v = death - prod;
+        for (size_t k = b0; k < di; ++k) tp[k] += v;
+    }
+}
+
+// clusters alive at size s: birth <= s < death (estimate_cluster_similarity,
+// layer_clusters.py:265-289)
+template <class S>
+std::vector<int64_t> active_at(const Barcode &b, S s) {
+    std::vector<int64_t> out;
+    for (size_t i = 0; i < b.births.size(); ++i)
+        if (static_cast<S>(b.births[i]) <= s && static_cast<S>(b.deaths[i]) > s) out.push_back(static_cast<int64_t>(i));
+    return out;
+}
+
+// jaccard_similarity (layer_clusters.py:251-262) of two ascending id lists
+double jaccard(const std::vector<int64_t> &a, const std::vector<int64_t> &b) {
+    size_t i = 0, j = 0, inter = 0;
+    while (i < a.size() && j < b.size()) {
+        if (a[i] == b[j]) {
+            ++inter;
+            ++i;
+            ++j;
+        } else if (a[i] < b[j]) {
+            ++i;
+        } else {
+            ++j;
+        }
+    }
+    const size_t uni = a.size() + b.size() - inter;
+    return uni > 0 ? static_cast<double>(inter) / static_cast<double>(uni) : 0.0;
+}
+
+// select_diverse_peaks (layer_clusters.py:309-350)
+std::vector<int64_t> select_diverse_peaks(const std::vector<int64_t> &peaks, const std::vector<float> &tp,
+                                          const std::vector<float> &sizes, const Barcode &b, double threshold,
+                                          int64_t max_layers) {
+    if (peaks.empty()) return {};
+    std::vector<float> pp(peaks.size());
+    for (size_t i = 0; i < peaks.size(); ++i) pp[i] = tp[peaks[i]];
+    const std::vector<int64_t> order = numba_argsort(pp.data(), static_cast<int64_t>(pp.size()));
+    std::vector<int64_t> sel;
+    std::vector<double> sel_births;
+    for (size_t r = order.size(); r-- > 0;) {
+        if (static_cast<int64_t>(sel.size()) >= max_layers) break;
+        const int64_t peak = peaks[order[r]];
+        const float birth = sizes[peak];
+        const std::vector<int64_t> a = active_at<float>(b, birth);
+        bool diverse = true;
+        for (const double sb : sel_births) {
+            if (jaccard(a, active_at<double>(b, sb)) > threshold) {
+                diverse = false;
+                break;
+            }
+        }
+        if (diverse) {
+            sel.push_back(peak);
+            sel_births.push_back(static_cast<double>(birth));
+        }
+    }
+    return sel;
+}
+
+// _binary_search_for_n_clusters (layer_clusters.py:71-153), hdbscan
+// lambdas; round() is round-half-even as in Python and numba.
+void binary_search_for_n_clusters(const Hierarchy &h, int64_t n, int64_t approx, int64_t *labels, float *prob) {
+    int64_t lo = 2, hi = n / 2;
+    Condensed t;
+    auto count = [&](int64_t mcs) {
+        condense(n, h, mcs, t);
+        return static_cast<int64_t>(extract_leaves(t).size());
+    };
+    int64_t hi_n = count(hi);
+    int64_t lo_n = count(lo);
+    while (hi - lo > 1) {
+        const int64_t mid = static_cast<int64_t>(std::nearbyint((lo + hi) / 2.0));
+        if (mid == lo || mid == hi) break;
+        const int64_t mid_n = count(mid);
+        if (mid_n < approx) {
+            hi = mid;
+            hi_n = mid_n;
+        } else {
+            lo = mid;
+            lo_n = mid_n;
+        }
+    }
+    const int64_t dl = std::llabs(lo_n - approx), du = std::llabs(hi_n - approx);
+    if (dl != du) {
+        condense(n, h, dl < du ? lo : hi, t);
+        label_vectors(t, extract_leaves(t), 0.0, n, labels, prob);
+        return;
+    }
+    std::vector<int64_t> lab_lo(n), lab_hi(n);
+    std::vector<float> pr_lo(n), pr_hi(n);
+    Condensed tl;
+    condense(n, h, lo, tl);
+    label_vectors(tl, extract_leaves(tl), 0.0, n, lab_lo.data(), pr_lo.data());
+    condense(n, h, hi, t);
+    label_vectors(t, extract_leaves(t), 0.0, n, lab_hi.data(), pr_hi.data());
+    int64_t cl = 0, cu = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        cl += lab_lo[i] >= 0;
+        cu += lab_hi[i] >= 0;
+    }
+    const bool use_lo = cl > cu;
+    std::memcpy(labels, (use_lo ? lab_lo : lab_hi).data(), sizeof(int64_t) * n);
+    std::memcpy(prob, (use_lo ? pr_lo : pr_hi).data(), sizeof(float) * n);
+}
+
 }  // namespace
 
 extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
@@ -615,7 +887,7 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         // reused between calls on this thread (no fresh pages per call)
         static thread_local Hierarchy h;
         clk::time_point t = clk::now();
-        linkage_tree(n, src, dst, w_sq, h);
+        linkage_tree(n, src, dst, w_sq, nullptr, h);
         if (phase_ms) phase_ms[0] = ms_since(t);
         if (linkage) {
             for (int64_t i = 0; i < n - 1; ++i) {
@@ -660,4 +932,145 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         return FH_ENOMEM;
     }
     return FH_OK;
+}
+
+// layer_clusters.build_cluster_layers (layer_clusters.py:386-481) after the
+// MST: linkage tree of the normalised, sorted edges, base layer, barcode,
+// persistence peaks and one layer per selected peak.  Layers come out
+// unsorted (base layer first, then peaks in selection order).
+extern "C" int fh_cluster_layers(int64_t n, const int32_t *src, const int32_t *dst, const double *dist,
+                                 const fh_layer_params *p, fh_layers *out) {
+    if (n < 1 || !p || !out || !out->labels || !out->strengths || !out->persistence) return FH_EINVAL;
+    if (n > 1 && (!src || !dst || !dist)) return FH_EINVAL;
+    if (p->base_min_cluster_size < 1 || p->max_layers < 1 || p->base_n_clusters < 0) return FH_EINVAL;
+    try {
+        out->n_layers = 1;
+        out->n_sizes = 0;
+        out->min_cluster_size = p->base_min_cluster_size;
+        out->persistence[0] = 0.0f;
+        if (n < 2) {
+            out->labels[0] = -1;
+            out->strengths[0] = 0.0f;
+            return FH_OK;
+        }
+        Hierarchy h;
+        linkage_tree(n, src, dst, nullptr, dist, h);
+        int64_t mcs = p->base_min_cluster_size;
+        Condensed c;
+        if (p->base_n_clusters > 0) {
+            binary_search_for_n_clusters(h, n, p->base_n_clusters, out->labels, out->strengths);
+            // min(bincount(labels[labels >= 0])), zero counts included
+            int64_t top = -1;
+            for (int64_t i = 0; i < n; ++i) top = std::max(top, out->labels[i]);
+            if (top >= 0) {
+                std::vector<int64_t> cnt(static_cast<size_t>(top + 1), 0);
+                for (int64_t i = 0; i < n; ++i)
+                    if (out->labels[i] >= 0) ++cnt[out->labels[i]];
+                mcs = std::max<int64_t>(1, *std::min_element(cnt.begin(), cnt.end()));
+            }
+            condense(n, h, mcs, c, kLambdaPlscan);
+        } else {
+            condense(n, h, mcs, c, kLambdaPlscan);
+            label_vectors(c, extract_leaves(c), 0.0, n, out->labels, out->strengths);
+        }
+        out->min_cluster_size = mcs;
+        // rows whose child is a cluster (mask_condensed_tree, child >= n)
+        std::vector<int64_t> cp, cc;
+        std::vector<float> cl, cs;
+        for (int64_t r = 0; r < c.rows; ++r) {
+            if (c.child[r] >= n) {
+                cp.push_back(c.parent[r]);
+                cc.push_back(c.child[r]);
+                cl.push_back(c.lambda[r]);
+                cs.push_back(c.size[r]);
+            }
+        }
+        if (cc.empty()) return FH_OK;
+        const Barcode b = barcode(cp, cc, cl, cs, n, mcs);
+        std::vector<float> sizes, tp;
+        total_persistence(b, sizes, tp);
+        out->n_sizes = static_cast<int64_t>(sizes.size());
+        if (out->sizes && out->total_persistence && out->n_sizes <= out->capacity) {
+            std::memcpy(out->sizes, sizes.data(), sizeof(float) * sizes.size());
+            std::memcpy(out->total_persistence, tp.data(), sizeof(float) * tp.size());
+        }
+        const std::vector<int64_t> peaks = find_peaks(tp);
+        const std::vector<int64_t> sel =
+            select_diverse_peaks(peaks, tp, sizes, b, p->layer_similarity_threshold, p->max_layers - 1);
+        const int64_t n_samples = c.parent[0];
+        for (size_t k = 0; k < sel.size(); ++k) {
+            const float best = sizes[sel[k]];
+            std::vector<int64_t> ids;
+            for (size_t i = 0; i < b.births.size(); ++i)
+                if (b.births[i] <= best && b.deaths[i] > best) ids.push_back(static_cast<int64_t>(i) + n);
+            const int64_t layer = static_cast<int64_t>(k) + 1;
+            label_vectors(c, ids, 0.0, n_samples, out->labels + layer * n, out->strengths + layer * n);
+            out->persistence[layer] = tp[sel[k]];
+        }
+        out->n_layers = static_cast<int32_t>(1 + sel.size());
+    } catch (const std::bad_alloc &) {
+        return FH_ENOMEM;
+    }
+    return FH_OK;
+}
+
+// layer_clusters._build_cluster_tree (layer_clusters.py:498-525): the
+// (parent layer, parent cluster, child layer, child cluster) rows in the
+// reference's order; the points of each upper layer are grouped by numba's
+// argsort of its labels.
+extern "C" int64_t fh_layer_cluster_tree(int32_t n_layers, int64_t n, const int64_t *labels, int64_t *mapping,
+                                         int64_t capacity) {
+    if (n_layers < 1 || n < 1 || !labels) return FH_EINVAL;
+    try {
+        std::vector<std::vector<uint8_t>> found(n_layers);
+        std::vector<int64_t> top(n_layers, -1);
+        for (int32_t l = 0; l < n_layers; ++l) {
+            for (int64_t i = 0; i < n; ++i) top[l] = std::max(top[l], labels[l * n + i]);
+            found[l].assign(static_cast<size_t>(top[l] + 1), 0);
+        }
+        int64_t count = 0;
+        auto emit = [&](int64_t a, int64_t b, int64_t c, int64_t d) {
+            if (mapping && count < capacity) {
+                mapping[4 * count + 0] = a;
+                mapping[4 * count + 1] = b;
+                mapping[4 * count + 2] = c;
+                mapping[4 * count + 3] = d;
+            }
+            ++count;
+        };
+        for (int32_t up = 1; up < n_layers; ++up) {
+            const int64_t *U = labels + up * n;
+            std::vector<int64_t> uniq(U, U + n);
+            std::sort(uniq.begin(), uniq.end());
+            uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+            const std::vector<int64_t> order = numba_argsort(U, n);
+            // np.split at cumsum(bincount(U + 1))[:-1]: group g holds label g - 1
+            const int64_t ng = top[up] + 2;
+            std::vector<int64_t> start(static_cast<size_t>(ng + 1), 0);
+            for (int64_t i = 0; i < n; ++i) ++start[U[i] + 2];
+            for (int64_t g = 0; g < ng; ++g) start[g + 1] += start[g];
+            const int64_t drop = ng > static_cast<int64_t>(uniq.size()) ? 1 : 0;
+            for (int32_t lo = up - 1; lo >= 0; --lo) {
+                const int64_t *Lw = labels + lo * n;
+                for (size_t i = 0; i < uniq.size(); ++i) {
+                    const int64_t label = uniq[i];
+                    const int64_t g = static_cast<int64_t>(i) + drop;
+                    if (label < 0 || g >= ng) continue;
+                    for (int64_t r = start[g]; r < start[g + 1]; ++r) {
+                        const int64_t child = Lw[order[r]];
+                        if (child >= 0 && !found[lo][child]) {
+                            emit(up, label, lo, child);
+                            found[lo][child] = 1;
+                        }
+                    }
+                }
+            }
+        }
+        for (int32_t lo = n_layers - 1; lo >= 0; --lo)
+            for (int64_t child = 0; child <= top[lo]; ++child)
+                if (!found[lo][child]) emit(n_layers, 0, lo, child);
+        return count;
+    } catch (const std::bad_alloc &) {
+        return FH_ENOMEM;
+    }
 }

@@ -239,3 +239,116 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
         stats["t_labels_ms"] = 1e3 * (t2 - t1)
         stats.update(phases)
     return labels, strengths
+
+
+def _check_sample_weight(sample_weight, X, dtype):
+    """hdbscan.py:44-90 (scalar or 1-D array, one per row, not all zero)."""
+    import numbers
+
+    from sklearn.utils import check_array
+
+    n_samples = X.shape[0]
+    if isinstance(sample_weight, numbers.Number):
+        sample_weight = np.full(n_samples, sample_weight, dtype=dtype)
+    else:
+        sample_weight = check_array(sample_weight, accept_sparse=False, ensure_2d=False, dtype=dtype, order="C",
+                                    input_name="sample_weight")
+        if sample_weight.ndim != 1:
+            raise ValueError(
+                f"Sample weights must be 1D array or scalar, got {sample_weight.ndim}D array. Expected either a "
+                f"scalar value or a 1D array of length {n_samples}."
+            )
+        if sample_weight.shape != (n_samples,):
+            raise ValueError("sample_weight.shape == {}, expected {}!".format(sample_weight.shape, (n_samples,)))
+    if np.all(sample_weight == 0):
+        raise ValueError("Sample weights must contain at least one non-zero number.")
+    return sample_weight
+
+
+try:
+    from sklearn.base import BaseEstimator, ClusterMixin
+except ImportError:  # pragma: no cover - scikit-learn is a dependency of the reference
+    BaseEstimator = object
+    ClusterMixin = object
+
+
+class PLSCAN(ClusterMixin, BaseEstimator):
+    """hdbscan.PLSCAN (hdbscan.py:890-1187) for Euclidean data: the MST on
+    the device, the layer stage on the host.  Attributes as the reference:
+    cluster_layers_, membership_strength_layers_, layer_persistence_scores_,
+    min_cluster_sizes_, total_persistence_, cluster_tree_, labels_,
+    membership_strengths_ (the latter only with more than one layer)."""
+
+    def __init__(self, *, min_samples=5, max_layers=10, base_min_cluster_size=5, base_n_clusters=None,
+                 layer_similarity_threshold=0.2, reproducible=False, metric="euclidean", algorithm="boruvka",
+                 knn_k=None, cannot_link=None, validate_cannot_link=True, metric_kwds=None, verbose=False):
+        self.min_samples = min_samples
+        self.max_layers = max_layers
+        self.base_min_cluster_size = base_min_cluster_size
+        self.base_n_clusters = base_n_clusters
+        self.layer_similarity_threshold = layer_similarity_threshold
+        self.reproducible = reproducible
+        self.metric = metric
+        self.algorithm = algorithm
+        self.knn_k = knn_k
+        self.cannot_link = cannot_link
+        self.validate_cannot_link = validate_cannot_link
+        self.metric_kwds = metric_kwds
+        self.verbose = verbose
+        self._validate_params()
+
+    def _validate_params(self):
+        # hdbscan.py:1042-1073
+        if (not np.issubdtype(type(self.min_samples), np.integer)) or self.min_samples <= 0:
+            raise ValueError("Min samples must be a positive integer!")
+        if (not np.issubdtype(type(self.max_layers), np.integer)) or self.max_layers <= 0:
+            raise ValueError("Max layers must be a positive integer!")
+        if (not np.issubdtype(type(self.base_min_cluster_size), np.integer)) or self.base_min_cluster_size <= 0:
+            raise ValueError("Base min cluster size must be a positive integer!")
+        if self.base_n_clusters is not None:
+            if (not np.issubdtype(type(self.base_n_clusters), np.integer)) or self.base_n_clusters <= 0:
+                raise ValueError("Base n clusters must be a positive integer!")
+        if self.algorithm not in ("boruvka", "kruskal"):
+            raise ValueError("algorithm must be 'boruvka' or 'kruskal'. Got: %s" % self.algorithm)
+        if self.cannot_link is not None and self.algorithm != "kruskal":
+            raise ValueError(
+                "cannot_link constraints are only supported with algorithm='kruskal'. Got algorithm=%r."
+                % self.algorithm
+            )
+
+    def fit_predict(self, X, y=None, sample_weight=None, **fit_params):
+        """hdbscan.py:1075-1160, metric='euclidean'."""
+        import scipy.sparse
+        from sklearn.utils.validation import validate_data
+
+        from .layers import build_cluster_layers, build_layer_cluster_tree
+
+        self._validate_params()
+        if self.metric != "euclidean":
+            raise NotImplementedError(f"metric={self.metric!r} is not on the device MST path (euclidean only)")
+        if scipy.sparse.issparse(X):
+            raise ValueError(
+                "PLSCAN requires a dense feature matrix for metric='euclidean'. Sparse matrices are only "
+                "supported with metric='precomputed'."
+            )
+        X = validate_data(self, X, accept_sparse=False, ensure_all_finite=False)
+        self._raw_data = X
+        if sample_weight is not None:
+            sample_weight = _check_sample_weight(sample_weight, X, dtype=np.float32)
+        kwargs = self.get_params()
+        (self.cluster_layers_, self.membership_strength_layers_, self.layer_persistence_scores_,
+         self.min_cluster_sizes_, self.total_persistence_) = build_cluster_layers(X, sample_weights=sample_weight,
+                                                                                  **kwargs)
+        self.cluster_tree_ = build_layer_cluster_tree(self.cluster_layers_)
+        if len(self.cluster_layers_) == 1:
+            self.labels_ = self.cluster_layers_[0]
+            self.membership_strengths_ = self.membership_strength_layers_[0]
+        else:
+            best_layer = np.argmax(self.layer_persistence_scores_)
+            self.labels_ = self.cluster_layers_[best_layer]
+            self.membership_strengths_ = self.membership_strength_layers_[best_layer]
+        return self.labels_
+
+    def fit(self, X, y=None, sample_weight=None, **fit_params):
+        _ = self.fit_predict(X, y=y, sample_weight=sample_weight, **fit_params)
+        return self

@@ -1,0 +1,133 @@
+"""PLSCAN cluster layers on top of the device MST.
+
+``build_cluster_layers`` and ``build_layer_cluster_tree`` mirror
+layer_clusters.build_cluster_layers (layer_clusters.py:353-495) and
+build_layer_cluster_tree (528-554): the MST comes from the device path, the
+NumPy steps the reference runs in plain Python (core-distance
+normalisation, ``np.argsort`` of the weights, the final layer ordering) are
+the same NumPy calls, and the numba-compiled stages run in the host C++
+library (``fh_cluster_layers``, ``fh_layer_cluster_tree``).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .hdbscan import compute_minimum_spanning_tree
+
+
+def _normalize(edges, core_distances):
+    # layer_clusters.py:387-397
+    finite_core = core_distances[np.isfinite(core_distances)]
+    if len(finite_core) > 0:
+        mean_core_distance = np.mean(finite_core)
+        min_core_distance = np.min(finite_core)
+    else:
+        mean_core_distance = 0.0
+        min_core_distance = 0.0
+    if mean_core_distance > min_core_distance and np.isfinite(mean_core_distance):
+        edges[:, 2] -= min_core_distance
+        edges[:, 2] /= mean_core_distance - min_core_distance
+    return edges
+
+
+def layers_from_mst(edges, core_distances, n_samples, base_min_cluster_size=10, base_n_clusters=None,
+                    layer_similarity_threshold=0.2, max_layers=10):
+    """layer_clusters.py:386-495 for a given MST (edges float64[n-1, 3] and
+    core distances as compute_minimum_spanning_tree returns them)."""
+    edges = _normalize(np.array(edges, dtype=np.float64, copy=True), core_distances)
+    sorted_mst = edges[np.argsort(edges.T[2])]  # layer_clusters.py:398
+    n = int(n_samples)
+    src = np.ascontiguousarray(sorted_mst[:, 0].astype(np.int32))
+    dst = np.ascontiguousarray(sorted_mst[:, 1].astype(np.int32))
+    dist = np.ascontiguousarray(sorted_mst[:, 2])
+    ml = int(max_layers)
+    labels = np.empty((ml, n), dtype=np.int64)
+    strengths = np.empty((ml, n), dtype=np.float32)
+    persistence = np.zeros(ml, dtype=np.float32)
+    sizes = np.empty(max(n, 1), dtype=np.float32)
+    total = np.empty(max(n, 1), dtype=np.float32)
+    params = _lib.FHLayerParams(int(base_min_cluster_size), 0 if base_n_clusters is None else int(base_n_clusters),
+                                float(layer_similarity_threshold), ml)
+    out = _lib.FHLayers(_lib.ptr(labels), _lib.ptr(strengths), _lib.ptr(persistence), _lib.ptr(sizes),
+                        _lib.ptr(total), sizes.shape[0], 0, 0, 0)
+    code = _lib.load().fh_cluster_layers(n, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(dist), ctypes.byref(params),
+                                         ctypes.byref(out))
+    if code != 0:
+        raise _lib.FHError(f"fh_cluster_layers failed with code {code}")
+    k = int(out.n_layers)
+    cluster_layers = [labels[i].astype(np.intp) for i in range(k)]
+    strength_layers = [strengths[i].copy() for i in range(k)]
+    scores = [0.0] + [persistence[i] for i in range(1, k)]
+    if out.n_sizes > 0:
+        sizes_out = sizes[: out.n_sizes].copy()
+        total_out = total[: out.n_sizes].copy()
+    else:
+        sizes_out = np.array([])
+        total_out = np.array([])
+    # layer_clusters.py:484-490: most clusters first
+    n_clusters_per_layer = [layer.max() + 1 for layer in cluster_layers]
+    order = np.argsort(n_clusters_per_layer)[::-1]
+    return ([cluster_layers[i] for i in order], [strength_layers[i] for i in order], [scores[i] for i in order],
+            sizes_out, total_out)
+
+
+def build_cluster_layers(
+    data,
+    *,
+    min_samples=5,
+    base_min_cluster_size=10,
+    base_n_clusters=None,
+    reproducible=False,
+    layer_similarity_threshold=0.2,
+    max_layers=10,
+    sample_weights=None,
+    metric="euclidean",
+    algorithm="boruvka",
+    knn_k=None,
+    cannot_link=None,
+    validate_cannot_link=True,
+    metric_kwds=None,
+    verbose=False,
+):
+    """Drop-in for layer_clusters.build_cluster_layers (layer_clusters.py:
+    353-495).  Returns (cluster_layers, membership_strength_layers,
+    persistence_scores, sizes, total_persistence)."""
+    n_samples = data.shape[0]
+    if verbose:
+        print("Computing MST ...")
+    edges, _neighbors, core_distances = compute_minimum_spanning_tree(
+        data,
+        min_samples=base_min_cluster_size if min_samples is None else min_samples,
+        sample_weights=sample_weights,
+        reproducible=reproducible,
+        metric=metric,
+        algorithm=algorithm,
+        knn_k=knn_k,
+        cannot_link=cannot_link,
+        validate_cannot_link=validate_cannot_link,
+        metric_kwds=metric_kwds,
+    )
+    if verbose:
+        print("Finding optimal resolution layers ...")
+    return layers_from_mst(edges, np.asarray(core_distances), n_samples, base_min_cluster_size, base_n_clusters,
+                           layer_similarity_threshold, max_layers)
+
+
+def build_layer_cluster_tree(labels):
+    """Drop-in for layer_clusters.build_layer_cluster_tree (layer_clusters.
+    py:528-554): {(parent layer, parent cluster): [(child layer, child
+    cluster), ...]}."""
+    stacked = np.ascontiguousarray(np.vstack(labels), dtype=np.int64)
+    n_layers, n = stacked.shape
+    cap = int(sum(int(row.max()) + 1 for row in stacked))
+    mapping = np.empty((max(cap, 1), 4), dtype=np.int64)
+    count = _lib.load().fh_layer_cluster_tree(n_layers, n, _lib.ptr(stacked), _lib.ptr(mapping), cap)
+    if count < 0 or count > cap:
+        raise _lib.FHError(f"fh_layer_cluster_tree failed with code {count}")
+    result = {}
+    for parent_layer, parent_cluster, child_layer, child_cluster in mapping[:count].tolist():
+        result.setdefault((parent_layer, parent_cluster), []).append((child_layer, child_cluster))
+    return result

@@ -195,6 +195,59 @@ int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst,
 int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t *dst, const float *w_sq,
                        int32_t *src_out, int32_t *dst_out, float *w_out);
 
+/* ------------------------------------------------------------------ */
+/* PLSCAN layer stage (host)                                           */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t base_min_cluster_size;      /* >= 1                                  */
+    int64_t base_n_clusters;            /* 0: not given                          */
+    double layer_similarity_threshold;  /* jaccard limit between layers         */
+    int32_t max_layers;                 /* >= 1                                  */
+} fh_layer_params;
+
+/* Caller-allocated outputs of fh_cluster_layers. */
+typedef struct {
+    int64_t *labels;            /* [max_layers * n], layer-major              */
+    float *strengths;           /* [max_layers * n]                           */
+    float *persistence;         /* [max_layers]; 0 for the base layer         */
+    float *sizes;               /* [capacity] min-cluster-size axis (may be NULL) */
+    float *total_persistence;   /* [capacity] (may be NULL)                   */
+    int64_t capacity;           /* entries of sizes / total_persistence      */
+    int32_t n_layers;           /* out: base layer + selected peaks          */
+    int64_t n_sizes;            /* out: 0 when there is no cluster split     */
+    int64_t min_cluster_size;   /* out: the base layer's min cluster size    */
+} fh_layers;
+
+/*
+ * Cluster layers from the MST rows already normalised and sorted the way
+ * build_cluster_layers does before its linkage tree.
+ *   Replaces: layer_clusters.py:399-481 (build_cluster_layers after
+ *             np.argsort of the normalised weights): mst_to_linkage_tree,
+ *             _binary_search_for_n_clusters, condense_tree(lambda "plscan"),
+ *             extract_leaves, min_cluster_size_barcode,
+ *             compute_total_persistence, find_peaks, select_diverse_peaks,
+ *             extract_clusters_by_id.
+ * HOST pointers.  src/dst [n-1] int32, dist [n-1] float64 (ascending).
+ * Layers are returned unsorted: the base layer first, then one per
+ * selected peak in selection order (the caller sorts them by cluster
+ * count, layer_clusters.py:484-490).  No context: host-only.
+ */
+int fh_cluster_layers(int64_t n, const int32_t *src, const int32_t *dst, const double *dist,
+                      const fh_layer_params *params, fh_layers *out);
+
+/*
+ * Parent/child links between clusters of stacked layers.
+ *   Replaces: layer_clusters.py:498-525 (_build_cluster_tree).
+ * HOST pointers.  labels [n_layers * n] int64, layer-major.  mapping
+ * [capacity * 4] int64 rows (parent layer, parent cluster, child layer,
+ * child cluster) in the reference's order; capacity = sum over layers of
+ * (max label + 1) always suffices.  Returns the row count (rows beyond
+ * capacity are counted, not written) or a negative FH_E* code.
+ */
+int64_t fh_layer_cluster_tree(int32_t n_layers, int64_t n, const int64_t *labels, int64_t *mapping,
+                              int64_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

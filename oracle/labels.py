@@ -102,9 +102,10 @@ class CondensedTree:
         self.child_size = child_size
 
 
-def condense_tree(hierarchy, min_cluster_size=10, max_cluster_size=np.inf):
-    """cluster_trees.py:205-352 with eliminate_branch (115-145), hdbscan
-    lambda 1/d (177-182), unit sample weights."""
+def condense_tree(hierarchy, min_cluster_size=10, max_cluster_size=np.inf, lambda_method="hdbscan"):
+    """cluster_trees.py:205-352 with eliminate_branch (115-145), unit sample
+    weights; lambda 1/d for "hdbscan" (192-197), exp(-d) for "plscan"
+    (200-202), stored as float32."""
     root = 2 * hierarchy.shape[0]
     num_points = hierarchy.shape[0] + 1
     next_label = num_points + 1
@@ -142,7 +143,10 @@ def condense_tree(hierarchy, min_cluster_size=10, max_cluster_size=np.inf):
         l, r, d, _ = hierarchy[node - num_points]
         left = np.int64(l)
         right = np.int64(r)
-        lambda_value = (1.0 / d) if d > 0.0 else np.inf
+        if lambda_method == "hdbscan":
+            lambda_value = (1.0 / d) if d > 0.0 else np.inf
+        else:
+            lambda_value = np.exp(-d)
         left_count = np.float32(hierarchy[left - num_points, 3]) if left >= num_points else sample_weights[left]
         right_count = (np.float32(hierarchy[right - num_points, 3]) if right >= num_points
                        else sample_weights[right])
