@@ -54,6 +54,10 @@ class FHStats(ctypes.Structure):
         ("t_query_ms", ctypes.c_double),
         ("t_query_main_ms", ctypes.c_double),
         ("t_total_ms", ctypes.c_double),
+        ("high_dim", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("knn_tree_rows", ctypes.c_int64),
+        ("exact_rows", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/fh.h"
+#include "fh_brute.h"
 #include "fh_kernels.h"
 #include "fh_sort.h"
 #include "fh_tree.h"
@@ -73,6 +74,13 @@ struct fh_ctx {
     DevBuf<float> nb_d, mincore;
     DevBuf<unsigned char> cert;
     int search_mode = 1;  // 0: one plain pass; 1: seeded + sampled + certified
+    // high-dimensional MFMA candidate scans
+    DevBuf<float> bx_xc, bx_nrm, bx_thr, bx_rowmin;
+    DevBuf<int> bx_cand, bx_cnt, bx_flags, bx_rowarg, bx_rows, bx_small;
+    DevBuf<unsigned int> bx_nmax;
+    int brute_min_dim = 17;  // d from which the MFMA scans replace the tree searches
+    int brute_window = 2048; // tree-order window that bounds each row's k-th distance
+    int brute_cap = 512;     // kNN candidates kept per row
     // edge sort
     DevBuf<uint32_t> sk1, sk1s;
     DevBuf<uint64_t> sk2, sk2s;
@@ -260,8 +268,9 @@ int knn_block(int k) {
 // query_perm != nullptr (queries = tree-order rows).
 int run_knn(fh_ctx *ctx, int d, int levels, const float *queries, const int *query_perm, int64_t nq, int k,
             float *out_d, int *out_i, float *core_o, float *core_p, int core_from_knn, int *nb_pos = nullptr,
-            float *nb_d = nullptr, int sqrt_dist = 0) {
+            float *nb_d = nullptr, int sqrt_dist = 0, const int *query_pos = nullptr) {
     fh::KnnArgs a{};
+    a.query_pos = query_pos;
     a.sqrt_dist = sqrt_dist;
     a.nb_pos = nb_pos;
     a.nb_d = nb_d;
@@ -364,21 +373,89 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     // Bound seeding reuses the kNN distances as candidate weights; in the
     // weighted branch those are square roots while the search compares
     // squared distances, so seeding is left out there.
-    const bool seed = pruned && !weighted && kq > 1;
+    // high dimension: MFMA candidate scans instead of the tree searches
+    // (FH_HIGH_DIM_FROM, read per call, moves the switch-over dimension)
+    int brute_from = ctx->brute_min_dim;
+    if (const char *m = std::getenv("FH_HIGH_DIM_FROM")) brute_from = std::atoi(m);
+    const int dp = fh::brute_padded_dim(d);
+    const bool brute = !weighted && dp > 0 && d >= brute_from && kq <= 64 && n >= 256;
+    const bool seed = pruned && !weighted && kq > 1 && !brute;
     if (seed) {
         FH_TRY(ensure(ctx, ctx->nb_pos, (size_t)n * (kq - 1)));
         FH_TRY(ensure(ctx, ctx->nb_d, (size_t)n * (kq - 1)));
     }
-    FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p, ctx->core_p.p,
-                   (ms > 1 && !weighted) ? 1 : 0, seed ? ctx->nb_pos.p : nullptr, seed ? ctx->nb_d.p : nullptr,
-                   weighted ? 1 : 0));
+    const float margin_c = 8.0f * dp + 64.0f;
+    const int64_t npad = (n + 127) / 128 * 128;
+    fh::BruteScanArgs bsa{};
+    if (brute) {
+        S.high_dim = 1;
+        FH_TRY(ensure(ctx, ctx->bx_xc, (size_t)npad * dp));
+        FH_TRY(ensure(ctx, ctx->bx_nrm, npad));
+        FH_TRY(ensure(ctx, ctx->bx_nmax, 1));
+        FH_TRY(ensure(ctx, ctx->bx_thr, n));
+        FH_TRY(ensure(ctx, ctx->bx_cand, (size_t)n * ctx->brute_cap));
+        FH_TRY(ensure(ctx, ctx->bx_cnt, n));
+        FH_TRY(ensure(ctx, ctx->bx_flags, n));
+        FH_TRY(ensure(ctx, ctx->bx_rowmin, n));
+        FH_TRY(ensure(ctx, ctx->bx_rowarg, n));
+        FH_TRY(ensure(ctx, ctx->bx_rows, n));
+        FH_TRY(ensure(ctx, ctx->bx_small, 4));
+        FH_HIP(hipMemsetAsync(ctx->bx_nmax.p, 0, sizeof(unsigned int), st));
+        FH_HIP(hipMemsetAsync(ctx->bx_small.p, 0, 4 * sizeof(int), st));
+        FH_HIP(fh::launch_center_pack(n, d, dp, npad, ctx->xp.p, ctx->box.p, ctx->bx_xc.p, ctx->bx_nrm.p,
+                                      ctx->bx_nmax.p, st));
+        FH_HIP(fh::launch_local_kth(n, d, kq, ctx->brute_window, ctx->xp.p, ctx->bx_nrm.p, ctx->bx_nmax.p, margin_c,
+                                    ctx->bx_thr.p, st));
+        bsa.xc = ctx->bx_xc.p;
+        bsa.nrm = ctx->bx_nrm.p;
+        bsa.n = n;
+        bsa.npad = npad;
+        bsa.dp = dp;
+        bsa.thr = ctx->bx_thr.p;
+        bsa.cand = ctx->bx_cand.p;
+        bsa.cand_cnt = ctx->bx_cnt.p;
+        bsa.cap = ctx->brute_cap;
+        bsa.core_p = ctx->core_p.p;
+        bsa.comp_p = ctx->comp_p.p;
+        bsa.row_min = ctx->bx_rowmin.p;
+        bsa.row_arg = ctx->bx_rowarg.p;
+        FH_HIP(fh::launch_scan_knn(bsa, st));
+        fh::KnnRescoreArgs ra{};
+        ra.xp = ctx->xp.p;
+        ra.perm = ctx->perm.p;
+        ra.cand = ctx->bx_cand.p;
+        ra.cand_cnt = ctx->bx_cnt.p;
+        ra.cap = ctx->brute_cap;
+        ra.n = n;
+        ra.d = d;
+        ra.k = kq;
+        ra.core_from_knn = ms > 1 ? 1 : 0;
+        ra.out_d = ctx->knn_d.p;
+        ra.out_i = ctx->knn_i.p;
+        ra.core_o = ctx->core_o.p;
+        ra.core_p = ctx->core_p.p;
+        ra.flag_list = ctx->bx_flags.p;
+        ra.flag_count = ctx->bx_small.p;
+        FH_HIP(fh::launch_knn_rescore(ra, st));
+        FH_HIP(hipMemcpyAsync(ctx->h_small, ctx->bx_small.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        FH_HIP(hipStreamSynchronize(st));
+        const int nflag = ctx->h_small[0];
+        S.knn_tree_rows = nflag;
+        if (nflag > 0)
+            FH_TRY(run_knn(ctx, d, levels, nullptr, nullptr, nflag, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p,
+                           ctx->core_p.p, ms > 1 ? 1 : 0, nullptr, nullptr, 0, ctx->bx_flags.p));
+    } else {
+        FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p,
+                       ctx->core_p.p, (ms > 1 && !weighted) ? 1 : 0, seed ? ctx->nb_pos.p : nullptr,
+                       seed ? ctx->nb_d.p : nullptr, weighted ? 1 : 0));
+    }
     if (weighted)
         FH_HIP(fh::launch_weighted_core(ni, kq, ctx->knn_i.p, ctx->knn_d.p, weights_d, ms, ctx->posof.p, ctx->core_o.p,
                                         ctx->core_p.p, st));
     FH_HIP(hipEventRecord(ctx->ev[2], st));
     // certification nodes: kCertDepth levels above the leaves
     const int cert_level = std::max(0, levels - 1 - kCertDepth);
-    if (pruned) {
+    if (pruned && !brute) {
         FH_TRY(ensure(ctx, ctx->cert_of_pos, n));
         FH_TRY(ensure(ctx, ctx->mincore, int64_t(1) << cert_level));
         FH_TRY(ensure(ctx, ctx->cert, int64_t(1) << cert_level));
@@ -436,11 +513,42 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     ca.d = d;
     double query_ms = 0.0, query_main_ms = 0.0;
     int rounds = 0;
+    fh::BoruvkaExactArgs be{};
+    be.xp = ctx->xp.p;
+    be.range = ctx->range.p;
+    be.box = ctx->box.p;
+    be.perm = ctx->perm.p;
+    be.core_p = ctx->core_p.p;
+    be.comp_p = ctx->comp_p.p;
+    be.nrm = ctx->bx_nrm.p;
+    be.nrm_max_bits = ctx->bx_nmax.p;
+    be.margin_c = margin_c;
+    be.row_min = ctx->bx_rowmin.p;
+    be.row_arg = ctx->bx_rowarg.p;
+    be.bound = ctx->bound.p;
+    be.rows = ctx->bx_rows.p;
+    be.row_count = ctx->bx_small.p + 1;
+    be.best = ctx->best.p;
+    be.partner = ctx->partner.p;
+    be.n = n;
+    be.d = d;
+    be.levels = levels;
+    int64_t exact_rows = 0;
     while (C > 1) {
         FH_HIP(fh::launch_round_reset(C, ctx->roots.p, ctx->bound.p, ctx->best.p, st));
-        FH_HIP(fh::launch_purity(levels, ctx->range.p, ctx->comp_p.p, ctx->node_comp.p, st));
+        if (!brute) FH_HIP(fh::launch_purity(levels, ctx->range.p, ctx->comp_p.p, ctx->node_comp.p, st));
         FH_HIP(hipEventRecord(ctx->ev[5], st));
-        if (pruned) {
+        if (brute) {
+            // approximate row minima on MFMA, exact upper bound per
+            // component, exact re-scan of the rows that can reach it
+            FH_HIP(hipEventRecord(ctx->ev[8], st));
+            FH_HIP(fh::launch_scan_boruvka(bsa, st));
+            FH_HIP(hipEventRecord(ctx->ev[9], st));
+            FH_HIP(fh::launch_boruvka_upper(be, st));
+            FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));
+            FH_HIP(fh::launch_boruvka_rows(be, st));
+            FH_HIP(fh::launch_boruvka_row_scan(be, st));
+        } else if (pruned) {
             // bounds from kNN cross edges, then a sampled search, then
             // certification of whole subtrees, then everyone else
             if (seed)
@@ -471,6 +579,11 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
         FH_HIP(fh::launch_roots_compact(C, ctx->roots.p, ctx->flag.p, ctx->offs.p, ctx->roots2.p, ctx->rootslot.p,
                                         st));
         FH_TRY(read_small(ctx, 2));
+        if (brute) {
+            int cnt = 0;
+            FH_HIP(hipMemcpy(&cnt, be.row_count, sizeof(int), hipMemcpyDeviceToHost));
+            exact_rows += cnt;
+        }
         query_ms += event_ms(ctx->ev[5], ctx->ev[6]);
         query_main_ms += event_ms(ctx->ev[8], ctx->ev[9]);
         const int emitted = ctx->h_small[0];
@@ -493,6 +606,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     FH_HIP(hipEventSynchronize(ctx->ev[7]));
 
     S.rounds = rounds;
+    S.exact_rows = exact_rows;
     S.n_edges = base;
     S.t_tree_ms = event_ms(ctx->ev[0], ctx->ev[1]);
     S.t_knn_ms = event_ms(ctx->ev[1], ctx->ev[2]);
@@ -530,6 +644,8 @@ fh_ctx *fh_create(int device) {
     fh_ctx *c = new fh_ctx();
     c->device = device;
     if (const char *m = std::getenv("FH_SEARCH_MODE")) c->search_mode = std::atoi(m);
+    if (const char *m = std::getenv("FH_HIGH_DIM_WINDOW")) c->brute_window = std::max(64, std::atoi(m) / 64 * 64);
+    if (const char *m = std::getenv("FH_HIGH_DIM_CAP")) c->brute_cap = std::max(16, std::atoi(m));
     if (hipSetDevice(device) != hipSuccess) {
         delete c;
         return nullptr;
@@ -571,6 +687,9 @@ void fh_destroy(fh_ctx *c) {
     release(c->nb_pos); release(c->cert_of_pos); release(c->nb_d); release(c->mincore); release(c->cert);
     release(c->sk1); release(c->sk1s); release(c->sk2); release(c->sk2s); release(c->sv1); release(c->sv1s);
     release(c->stemp);
+    release(c->bx_xc); release(c->bx_nrm); release(c->bx_thr); release(c->bx_rowmin); release(c->bx_cand);
+    release(c->bx_cnt); release(c->bx_flags); release(c->bx_rowarg); release(c->bx_rows); release(c->bx_small);
+    release(c->bx_nmax);
     if (c->h_small) e = hipHostFree(c->h_small);
     for (auto &ev : c->ev)
         if (ev) e = hipEventDestroy(ev);

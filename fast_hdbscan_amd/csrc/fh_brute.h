@@ -1,0 +1,82 @@
+// High-dimensional candidate search: MFMA (FP32 matrix core) distance tiles
+// that find candidates under certified error bounds, followed by exact
+// re-scoring in the reference's float32 arithmetic.  Used for the kNN core
+// distances and the Boruvka rounds when the KD-tree stops pruning (d large).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fh {
+
+// Padded dimension for the MFMA tiles (multiple of 4, one of the compiled
+// sizes); 0 if d is above the largest.
+int brute_padded_dim(int d);
+
+struct BruteScanArgs {
+    const float *xc;      // [npad * dp] centred points, tree order, dims grouped by (k mod 4)
+    const float *nrm;     // [npad] squared norms of the centred rows (approximate)
+    int64_t n, npad;
+    int dp;
+    // kNN candidate collection
+    const float *thr;     // [n] row thresholds (approximate squared distance)
+    int *cand;            // [n * cap] candidate positions
+    int *cand_cnt;        // [n] candidates found (may exceed cap)
+    int cap;
+    // Boruvka row minimum
+    const float *core_p;  // [n]
+    const int *comp_p;    // [n]
+    float *row_min;       // [n] smallest approximate mutual-reachability value
+    int *row_arg;         // [n] position that gave it
+};
+
+hipError_t launch_center_pack(int64_t n, int d, int dp, int64_t npad, const float *xp, const float *root_box,
+                              float *xc, float *nrm, unsigned int *nrm_max_bits, hipStream_t st);
+hipError_t launch_local_kth(int64_t n, int d, int k, int window, const float *xp, const float *nrm,
+                            const unsigned int *nrm_max_bits, float margin_c, float *thr, hipStream_t st);
+hipError_t launch_scan_knn(const BruteScanArgs &a, hipStream_t st);
+hipError_t launch_scan_boruvka(const BruteScanArgs &a, hipStream_t st);
+
+struct KnnRescoreArgs {
+    const float *xp;
+    const int *perm;
+    const int *cand;
+    const int *cand_cnt;
+    int cap;
+    int64_t n;
+    int d, k;
+    int core_from_knn;
+    float *out_d;
+    int *out_i;
+    float *core_o;
+    float *core_p;
+    int *flag_list;   // positions whose result must come from the tree search
+    int *flag_count;
+};
+hipError_t launch_knn_rescore(const KnnRescoreArgs &a, hipStream_t st);
+
+struct BoruvkaExactArgs {
+    const float *xp;
+    const int2 *range;
+    const float *box;
+    const int *perm;
+    const float *core_p;
+    const int *comp_p;
+    const float *nrm;
+    const unsigned int *nrm_max_bits;
+    float margin_c;
+    const float *row_min;
+    const int *row_arg;
+    float *bound;                 // [components] exact upper bound, float bits min
+    int *rows;                    // candidate rows
+    int *row_count;
+    unsigned long long *best;     // [components] packed (weight bits, original index)
+    int *partner;                 // [n] partner position of each scanned row
+    int64_t n;
+    int d, levels;
+};
+hipError_t launch_boruvka_upper(const BoruvkaExactArgs &a, hipStream_t st);
+hipError_t launch_boruvka_rows(const BoruvkaExactArgs &a, hipStream_t st);
+hipError_t launch_boruvka_row_scan(const BoruvkaExactArgs &a, hipStream_t st);
+
+}  // namespace fh

@@ -1,0 +1,548 @@
+// High-dimensional candidate search on the FP32 matrix cores.
+//
+// In high dimension the KD-tree's box bounds stop pruning and every query
+// degenerates into a scan of all leaves, so the kNN and the Boruvka rounds
+// are done as blocked all-pairs scans instead.  The scans run on MFMA
+// (v_mfma_f32_16x16x4f32) with the Gram expansion
+//     ||x - y||^2 ~= ||x'||^2 + ||y'||^2 - 2 x'.y'     (x' = x - mu)
+// which only *selects candidates*: every pair that could matter under a
+// certified error bound is re-scored with the reference's own float32
+// difference-sum (rdist, sequential FMA), so all outputs stay exact.
+//
+// Error bound.  With u = 2^-24, |approx - rdist| <= (4d + 6) u (|x'|^2 +
+// |y'|^2) covers the dot-product and norm sums (gamma_d each), the final
+// combination, the centring roundings and rdist's own rounding; the code
+// uses margin_c * u * (|x'|^2 + max|y'|^2) with margin_c = 8 dp + 64, twice
+// that.
+//
+// kNN (numba_kdtree.py:765-792): per row, T = the k-th smallest exact rdist
+// among a window of tree-order neighbours (an upper bound of the true k-th
+// distance); the MFMA scan collects every column with approx <= T + margin,
+// a superset of the true k nearest; exact re-scoring sorts them.  When two
+// of the k+1 smallest exact values tie, the reference's heap order decides
+// the output, so such rows (and rows whose candidate list overflowed) are
+// answered by the exact tree search instead.
+//
+// Boruvka round (boruvka.py:290-331 + 34-78): the MFMA scan gives each row
+// the smallest approximate mutual-reachability value m_i to another
+// component and the column j0 that gave it.  The exact weight of (i, j0) is
+// a real candidate, so U_c = min_i exact(i, j0_i) bounds the component's
+// minimum w*_c from above; any row with m_i - margin_i > U_c cannot reach
+// w*_c.  The remaining rows (a few per component) are re-scanned exactly
+// against every column, and ties in weight are broken by the reference's
+// visiting order (first in the depth-first order of that row's KD-tree
+// query: closer child first, left on equal bounds), which is the partner the
+// tree search would report.  The packed atomicMin per component then picks
+// the lowest original index at the minimum, as merge_components does.
+#include "fh_brute.h"
+#include "fh_device.h"
+
+namespace fh {
+
+namespace {
+
+constexpr float kInfB = __builtin_huge_valf();
+constexpr float kU = 5.9604644775390625e-08f;  // 2^-24
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+inline int64_t nblocks(int64_t n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+int brute_padded_dim(int d) {
+    static const int sizes[] = {16, 24, 32, 48, 64};
+    for (int s : sizes)
+        if (d <= s) return s;
+    return 0;
+}
+
+// ------------------------------------------------------------- packing ----
+
+// Centre on the root box's midpoint, zero-pad to dp dimensions and group
+// the dimensions by (k mod 4) so that each MFMA lane reads a contiguous run.
+__global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const float *__restrict__ xp,
+                                   const float *__restrict__ root_box, float *__restrict__ xc, float *__restrict__ nrm,
+                                   unsigned int *nrm_max_bits) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npad) return;
+    const int q4 = dp / 4;
+    float s = 0.0f;
+    for (int k = 0; k < dp; ++k) {
+        float v = 0.0f;
+        if (p < n && k < d) v = xp[p * d + k] - 0.5f * (root_box[k] + root_box[d + k]);
+        xc[p * dp + (k & 3) * q4 + (k >> 2)] = v;
+        s = __builtin_fmaf(v, v, s);
+    }
+    nrm[p] = s;
+    if (p < n) atomicMax(nrm_max_bits, __float_as_uint(s));
+}
+
+// k-th smallest exact rdist among the window of tree-order positions
+// around each row; thr = that + margin.  64 rows per block share the window,
+// staged through LDS 64 candidates at a time.
+template <int DP>
+__global__ __launch_bounds__(64) void local_kth_kernel(int64_t n, int d, int k, int window,
+                                                       const float *__restrict__ xp, const float *__restrict__ nrm,
+                                                       const unsigned int *__restrict__ nrm_max_bits, float margin_c,
+                                                       float *__restrict__ thr) {
+    __shared__ float heap[64 * 64];
+    __shared__ float cs[64 * DP];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * 64;
+    const int64_t p = base + tid;
+    int64_t w0 = (base / window) * window;
+    if (w0 + window > n) w0 = n > window ? n - window : 0;
+    const int64_t w1 = w0 + window < n ? w0 + window : n;
+    float x[DP];
+#pragma unroll
+    for (int j = 0; j < DP; ++j) x[j] = (p < n && j < d) ? xp[p * d + j] : 0.0f;
+    float *h = heap + tid;  // slot s at h[s * 64]
+    for (int s = 0; s < k; ++s) h[s * 64] = kInfB;
+    for (int64_t c0 = w0; c0 < w1; c0 += 64) {
+        const int cnt = (int)((w1 - c0) < 64 ? (w1 - c0) : 64);
+        __syncthreads();
+        for (int e = tid; e < 64 * DP; e += 64) {
+            const int c = e / DP, j = e % DP;
+            cs[e] = (c < cnt && j < d) ? xp[(c0 + c) * d + j] : 0.0f;
+        }
+        __syncthreads();
+        if (p < n) {
+            for (int c = 0; c < cnt; ++c) {
+                const float *y = cs + c * DP;
+                float r = 0.0f;
+#pragma unroll
+                for (int j = 0; j < DP; ++j) {
+                    const float df = x[j] - y[j];
+                    r = __builtin_fmaf(df, df, r);
+                }
+                if (r < h[0]) {
+                    // replace the root of the max-heap and sift down
+                    int at = 0;
+                    for (;;) {
+                        const int c1 = 2 * at + 1;
+                        if (c1 >= k) break;
+                        int ch = c1;
+                        if (c1 + 1 < k && h[(c1 + 1) * 64] > h[c1 * 64]) ch = c1 + 1;
+                        if (h[ch * 64] > r) {
+                            h[at * 64] = h[ch * 64];
+                            at = ch;
+                        } else {
+                            break;
+                        }
+                    }
+                    h[at * 64] = r;
+                }
+            }
+        }
+    }
+    if (p < n) {
+        const float nm = __uint_as_float(*nrm_max_bits);
+        thr[p] = h[0] + margin_c * kU * (nrm[p] + nm);
+    }
+}
+
+// --------------------------------------------------------- MFMA scans ----
+
+// One block owns 128 rows (4 waves x 32) and streams every 128-column tile
+// of the centred points through LDS.  Per tile each wave computes a 32 x 128
+// block of dot products: 2 row blocks x 8 column blocks of 16 x 16, K = 4 per
+// instruction.  LDS layout: row (s * 4 + g) holds dimension 4 s + g of the
+// tile's 128 columns (+16 floats of padding against bank conflicts).
+// MODE 0: collect kNN candidates under the row thresholds.
+// MODE 1: smallest approximate mutual-reachability value to another
+//         component per row.
+template <int DP4, int MODE>
+__global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
+    constexpr int DP = 4 * DP4;
+    constexpr int LROW = 144;
+    constexpr int NV = DP / 8;  // float4 loads per thread per tile
+    __shared__ float bs[DP * LROW];
+    __shared__ float col_nrm[128];
+    __shared__ float col_core[128];
+    __shared__ int col_comp[128];
+    __shared__ int cnt_l[128];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, c16 = lane & 15;
+    const int64_t blk_row = (int64_t)blockIdx.x * 128;
+    const int64_t rbase = blk_row + wave * 32;
+    const int64_t n = a.n;
+
+    float A[2][DP4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+        const float *src = a.xc + (rbase + rb * 16 + c16) * DP + g * DP4;
+#pragma unroll
+        for (int s = 0; s < DP4; ++s) A[rb][s] = src[s];
+    }
+    // the 8 output rows this lane holds: rbase + rb*16 + 4g + r
+    float rn[2][4], rv[2][4], rcore[2][4];
+    int ri[2][4], rcomp[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t row = rbase + rb * 16 + 4 * g + r;
+            const bool ok = row < n;
+            rn[rb][r] = a.nrm[row];
+            ri[rb][r] = -1;
+            if constexpr (MODE == 0) {
+                rv[rb][r] = ok ? a.thr[row] : -kInfB;
+                rcore[rb][r] = 0.0f;
+                rcomp[rb][r] = 0;
+            } else {
+                rv[rb][r] = kInfB;
+                rcore[rb][r] = ok ? a.core_p[row] : 0.0f;
+                rcomp[rb][r] = ok ? a.comp_p[row] : -2;
+            }
+        }
+    if (MODE == 0 && tid < 128) cnt_l[tid] = 0;
+
+    const int ntiles = (int)(a.npad / 128);
+    floatx4 pre[NV];
+    float pn = 0.0f, pc = 0.0f;
+    int pk = -1;
+    auto gload = [&](int t) {
+        const floatx4 *src = reinterpret_cast<const floatx4 *>(a.xc + (int64_t)t * 128 * DP);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) pre[v] = src[tid + v * 256];
+        if (tid < 128) {
+            const int64_t col = (int64_t)t * 128 + tid;
+            pn = a.nrm[col];
+            if constexpr (MODE == 1) {
+                pc = col < n ? a.core_p[col] : 0.0f;
+                pk = col < n ? a.comp_p[col] : -3;
+            }
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int f = tid + v * 256;
+            const int col = f / (DP / 4);
+            const int e0 = (f % (DP / 4)) * 4;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int e = e0 + i;
+                const int gg = e / DP4, s = e % DP4;
+                bs[(s * 4 + gg) * LROW + col] = pre[v][i];
+            }
+        }
+        if (tid < 128) {
+            col_nrm[tid] = pn;
+            col_core[tid] = pc;
+            col_comp[tid] = pk;
+        }
+    };
+
+    gload(0);
+    for (int t = 0; t < ntiles; ++t) {
+        __syncthreads();
+        lstore();
+        __syncthreads();
+        if (t + 1 < ntiles) gload(t + 1);
+        floatx4 acc[2][8];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < DP4; ++s) {
+            const float *brow = bs + (s * 4 + g) * LROW + c16;
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const float b = brow[cb * 16];
+                acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[0][s], b, acc[0][cb], 0, 0, 0);
+                acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[1][s], b, acc[1][cb], 0, 0, 0);
+            }
+        }
+        const int64_t col0 = (int64_t)t * 128;
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) {
+            const int cl = cb * 16 + c16;
+            const int64_t col = col0 + cl;
+            const bool valid = col < n;
+            const float cn = col_nrm[cl];
+            if constexpr (MODE == 0) {
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float approx = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
+                        if (valid && approx <= rv[rb][r]) {
+                            const int rl = wave * 32 + rb * 16 + 4 * g + r;
+                            const int slot = atomicAdd(&cnt_l[rl], 1);
+                            if (slot < a.cap) a.cand[(blk_row + rl) * a.cap + slot] = (int)col;
+                        }
+                    }
+            } else {
+                const float cc = col_core[cl];
+                const int ck = col_comp[cl];
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float approx = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
+                        float m = rcore[rb][r] > cc ? rcore[rb][r] : cc;
+                        m = approx > m ? approx : m;
+                        if (valid && ck != rcomp[rb][r] && m < rv[rb][r]) {
+                            rv[rb][r] = m;
+                            ri[rb][r] = (int)col;
+                        }
+                    }
+            }
+        }
+    }
+
+    if constexpr (MODE == 0) {
+        __syncthreads();
+        if (tid < 128 && blk_row + tid < n) a.cand_cnt[blk_row + tid] = cnt_l[tid];
+    } else {
+        // minimum over the 16 lanes that hold the same rows
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float v = rv[rb][r];
+                int i = ri[rb][r];
+#pragma unroll
+                for (int m = 1; m < 16; m <<= 1) {
+                    const float ov = __shfl_xor(v, m, 16);
+                    const int oi = __shfl_xor(i, m, 16);
+                    if (ov < v || (ov == v && (unsigned)oi < (unsigned)i)) {
+                        v = ov;
+                        i = oi;
+                    }
+                }
+                const int64_t row = rbase + rb * 16 + 4 * g + r;
+                if (c16 == 0 && row < n) {
+                    a.row_min[row] = v;
+                    a.row_arg[row] = i;
+                }
+            }
+    }
+}
+
+// ------------------------------------------------- exact kNN re-score ----
+
+// Exact rdist of every collected candidate, the k+1 smallest kept sorted in
+// LDS; rows with a tie among them, an overflowed list or fewer than k
+// candidates go to the tree search.
+__global__ __launch_bounds__(64) void knn_rescore_kernel(KnnRescoreArgs a) {
+    __shared__ float sv[65 * 64];
+    __shared__ int si[65 * 64];
+    const int tid = threadIdx.x;
+    const int64_t p = (int64_t)blockIdx.x * 64 + tid;
+    if (p >= a.n) return;
+    const int K1 = a.k + 1;
+    float *v = sv + tid;
+    int *id = si + tid;
+    for (int t = 0; t < K1; ++t) {
+        v[t * 64] = kInfB;
+        id[t * 64] = -1;
+    }
+    const int cnt = a.cand_cnt[p];
+    bool flag = cnt > a.cap;
+    if (!flag) {
+        const float *x = a.xp + p * a.d;
+        for (int t = 0; t < cnt; ++t) {
+            const int j = a.cand[p * a.cap + t];
+            const float r = rdist<0>(x, a.xp + (int64_t)j * a.d, a.d);
+            if (r < v[(K1 - 1) * 64]) {
+                int i = K1 - 1;
+                while (i > 0 && v[(i - 1) * 64] > r) {
+                    v[i * 64] = v[(i - 1) * 64];
+                    id[i * 64] = id[(i - 1) * 64];
+                    --i;
+                }
+                v[i * 64] = r;
+                id[i * 64] = j;
+            }
+        }
+        for (int t = 0; t < a.k; ++t)
+            if (!(v[t * 64] < v[(t + 1) * 64])) flag = true;
+    }
+    if (flag) {
+        const int slot = atomicAdd(a.flag_count, 1);
+        a.flag_list[slot] = (int)p;
+        return;
+    }
+    const int64_t row = a.perm[p];
+    for (int t = 0; t < a.k; ++t) {
+        a.out_d[row * a.k + t] = v[t * 64];
+        a.out_i[row * a.k + t] = a.perm[id[t * 64]];
+    }
+    const float c = a.core_from_knn ? v[(a.k - 1) * 64] : 0.0f;
+    a.core_o[row] = c;
+    a.core_p[p] = c;
+}
+
+// ------------------------------------------------ exact Boruvka steps ----
+
+__global__ void boruvka_upper_kernel(BoruvkaExactArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const int j = a.row_arg[p];
+    if (j < 0) return;
+    float w = rdist<0>(a.xp + p * a.d, a.xp + (int64_t)j * a.d, a.d);
+    const float ci = a.core_p[p], cj = a.core_p[j];
+    w = w > ci ? w : ci;
+    w = w > cj ? w : cj;
+    atomicMin(reinterpret_cast<unsigned int *>(a.bound + a.comp_p[p]), __float_as_uint(w));
+}
+
+__global__ void boruvka_rows_kernel(BoruvkaExactArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const float nm = __uint_as_float(*a.nrm_max_bits);
+    const float lo = a.row_min[p] - a.margin_c * kU * (a.nrm[p] + nm);
+    if (lo <= a.bound[a.comp_p[p]]) {
+        const int slot = atomicAdd(a.row_count, 1);
+        a.rows[slot] = (int)p;
+    }
+}
+
+// true when position j1 comes before j2 in the depth-first visiting order
+// of the KD-tree query for point x (closer child first, left on equal
+// bounds; within a leaf, position order)
+__device__ bool dfs_before(const float *x, int j1, int j2, const int2 *__restrict__ range,
+                           const float *__restrict__ box, int levels, int d) {
+    int node = 0;
+    for (int depth = 0; depth < levels - 1; ++depth) {
+        const int l = 2 * node + 1;
+        const int split = range[l].y;
+        const bool a1 = j1 < split, a2 = j2 < split;
+        if (a1 == a2) {
+            node = a1 ? l : l + 1;
+            continue;
+        }
+        const float lbl = box_lb<0>(x, box + (size_t)l * 2 * d, d);
+        const float lbr = box_lb<0>(x, box + (size_t)(l + 1) * 2 * d, d);
+        const bool left_first = lbl <= lbr;
+        return a1 == left_first;
+    }
+    return j1 < j2;
+}
+
+__global__ __launch_bounds__(256) void boruvka_row_scan_kernel(BoruvkaExactArgs a) {
+    __shared__ float xs[64];
+    __shared__ float sw[256];
+    __shared__ int sj[256];
+    const int tid = threadIdx.x;
+    const int count = *a.row_count;
+    for (int it = blockIdx.x; it < count; it += gridDim.x) {
+        const int p = a.rows[it];
+        const int comp = a.comp_p[p];
+        const float ci = a.core_p[p];
+        const float B = a.bound[comp];
+        if (tid < a.d) xs[tid] = a.xp[(int64_t)p * a.d + tid];
+        __syncthreads();
+        float bw = kInfB;
+        int bj = -1;
+        for (int64_t j = tid; j < a.n; j += 256) {
+            if (a.comp_p[j] == comp) continue;
+            const float cj = a.core_p[j];
+            if (cj > B) continue;
+            float w = rdist<0>(xs, a.xp + j * a.d, a.d);
+            w = w > ci ? w : ci;
+            w = w > cj ? w : cj;
+            if (w > B) continue;
+            if (w < bw || (w == bw && dfs_before(xs, (int)j, bj, a.range, a.box, a.levels, a.d))) {
+                bw = w;
+                bj = (int)j;
+            }
+        }
+        sw[tid] = bw;
+        sj[tid] = bj;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (tid < off) {
+                const float ow = sw[tid + off];
+                const int oj = sj[tid + off];
+                const float mw = sw[tid];
+                const int mj = sj[tid];
+                bool take = false;
+                if (oj >= 0) {
+                    if (mj < 0 || ow < mw) take = true;
+                    else if (ow == mw) take = dfs_before(xs, oj, mj, a.range, a.box, a.levels, a.d);
+                }
+                if (take) {
+                    sw[tid] = ow;
+                    sj[tid] = oj;
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0 && sj[0] >= 0) {
+            const unsigned long long key =
+                ((unsigned long long)__float_as_uint(sw[0]) << 32) | (unsigned int)a.perm[p];
+            atomicMin(a.best + comp, key);
+            a.partner[p] = sj[0];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ launches ----
+
+hipError_t launch_center_pack(int64_t n, int d, int dp, int64_t npad, const float *xp, const float *root_box,
+                              float *xc, float *nrm, unsigned int *nrm_max_bits, hipStream_t st) {
+    hipLaunchKernelGGL(center_pack_kernel, dim3(nblocks(npad, 256)), dim3(256), 0, st, n, d, dp, npad, xp, root_box,
+                       xc, nrm, nrm_max_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_local_kth(int64_t n, int d, int k, int window, const float *xp, const float *nrm,
+                            const unsigned int *nrm_max_bits, float margin_c, float *thr, hipStream_t st) {
+    const dim3 g(nblocks(n, 64)), b(64);
+    switch (brute_padded_dim(d)) {
+        case 16: hipLaunchKernelGGL(local_kth_kernel<16>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
+        case 24: hipLaunchKernelGGL(local_kth_kernel<24>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
+        case 32: hipLaunchKernelGGL(local_kth_kernel<32>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
+        case 48: hipLaunchKernelGGL(local_kth_kernel<48>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
+        case 64: hipLaunchKernelGGL(local_kth_kernel<64>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_scan(const BruteScanArgs &a, hipStream_t st) {
+    const dim3 g(a.npad / 128), b(256);
+    switch (a.dp) {
+        case 16: hipLaunchKernelGGL((brute_scan_kernel<4, MODE>), g, b, 0, st, a); break;
+        case 24: hipLaunchKernelGGL((brute_scan_kernel<6, MODE>), g, b, 0, st, a); break;
+        case 32: hipLaunchKernelGGL((brute_scan_kernel<8, MODE>), g, b, 0, st, a); break;
+        case 48: hipLaunchKernelGGL((brute_scan_kernel<12, MODE>), g, b, 0, st, a); break;
+        case 64: hipLaunchKernelGGL((brute_scan_kernel<16, MODE>), g, b, 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_knn(const BruteScanArgs &a, hipStream_t st) { return launch_scan<0>(a, st); }
+hipError_t launch_scan_boruvka(const BruteScanArgs &a, hipStream_t st) { return launch_scan<1>(a, st); }
+
+hipError_t launch_knn_rescore(const KnnRescoreArgs &a, hipStream_t st) {
+    if (a.k + 1 > 65) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(knn_rescore_kernel, dim3(nblocks(a.n, 64)), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_boruvka_upper(const BoruvkaExactArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(boruvka_upper_kernel, dim3(nblocks(a.n, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_boruvka_rows(const BoruvkaExactArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(boruvka_rows_kernel, dim3(nblocks(a.n, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_boruvka_row_scan(const BoruvkaExactArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(boruvka_row_scan_kernel, dim3(2048), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace fh

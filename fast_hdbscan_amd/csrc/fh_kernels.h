@@ -29,6 +29,7 @@ struct KnnArgs {
     int *nb_pos;            // optional: [nq, k-1] neighbour tree positions (columns 1..k-1)
     float *nb_d;            // optional: [nq, k-1] their squared distances
     int sqrt_dist;          // 1: float32 square roots before the sort (output_rdist=False)
+    const int *query_pos;   // optional: the nq queries are these tree positions of xp
 };
 
 struct QueryArgs {

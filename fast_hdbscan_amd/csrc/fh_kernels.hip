@@ -142,8 +142,10 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     h.init();
 
     const int dd = D > 0 ? D : a.d;
+    // a listed tree position, or query row q
+    const int64_t qp = a.query_pos ? a.query_pos[q] : q;
     KnnVisitor<D> v;
-    v.x.load(a.queries + (size_t)q * dd, a.d);
+    v.x.load(a.query_pos ? a.xp + (size_t)qp * dd : a.queries + (size_t)q * dd, a.d);
     v.xp = a.xp;
     v.range = a.range;
     v.h = h;
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
 
     // Output row: the original index of the query for the self-query, the
     // query's own row otherwise.
-    const int64_t row = a.query_perm ? a.query_perm[q] : q;
+    const int64_t row = a.query_pos ? a.perm[qp] : (a.query_perm ? a.query_perm[q] : q);
     for (int j = 0; j < k; ++j) {
         const int pj = h.I(j);
         a.out_d[row * k + j] = h.P(j);
@@ -166,14 +168,14 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     }
     if (a.nb_pos != nullptr) {
         for (int j = 1; j < k; ++j) {
-            a.nb_pos[q * (k - 1) + j - 1] = h.I(j);
-            a.nb_d[q * (k - 1) + j - 1] = h.P(j);
+            a.nb_pos[qp * (k - 1) + j - 1] = h.I(j);
+            a.nb_d[qp * (k - 1) + j - 1] = h.P(j);
         }
     }
     if (a.core_o != nullptr) {
         const float c = a.core_from_knn ? h.P(k - 1) : 0.0f;
         a.core_o[row] = c;
-        a.core_p[q] = c;
+        a.core_p[qp] = c;
     }
 }
 

@@ -59,6 +59,10 @@ typedef struct {
     double t_query_ms;        /* per-round nearest-edge search, summed         */
     double t_query_main_ms;   /* its main-pass kernel alone, summed            */
     double t_total_ms;        /* whole call                                    */
+    int32_t high_dim;         /* 1: MFMA candidate scans + exact re-scoring   */
+    int32_t pad0;
+    int64_t knn_tree_rows;    /* high_dim: kNN rows answered by the tree search */
+    int64_t exact_rows;       /* high_dim: rows re-scanned exactly, summed     */
 } fh_stats;
 
 /* Create a context on HIP device `device`.  Returns NULL on failure (no

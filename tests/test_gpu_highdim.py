@@ -1,0 +1,83 @@
+"""High-dimensional path: MFMA candidate scans + exact re-scoring.
+
+Every case is checked bit for bit against the oracle (the reference's
+KD-tree semantics) or against the device's own exact KD-tree path, which
+the oracle pins at smaller sizes.  FH_HIGH_DIM_FROM moves the dimension
+from which the MFMA path is used, so low-dimensional inputs with many ties
+exercise its tie handling too."""
+import numpy as np
+import pytest
+
+from fast_hdbscan_amd import workloads
+from fast_hdbscan_amd.hdbscan import mst_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(X, ms):
+    from oracle import oracle as O
+
+    return O.boruvka(O.build_kdtree(X), ms)
+
+
+def _check(X, ms, expect_high_dim=True):
+    st = {}
+    src, dst, wsq, core, nbr = mst_arrays(X, ms, stats=st)
+    assert st["high_dim"] == (1 if expect_high_dim else 0)
+    ref = _oracle(X, ms)
+    np.testing.assert_array_equal(core, ref.core_sq)
+    np.testing.assert_array_equal(nbr, ref.knn_i[:, 1:])
+    np.testing.assert_array_equal(src, ref.src)
+    np.testing.assert_array_equal(dst, ref.dst)
+    np.testing.assert_array_equal(wsq, ref.w_sq)
+    return st
+
+
+@pytest.mark.parametrize("d,n,ms", [(64, 6000, 10), (48, 5000, 5), (32, 6000, 10), (24, 4000, 1), (17, 5000, 30),
+                                    (64, 3000, 63)])
+def test_highdim_matches_oracle(d, n, ms):
+    print(f"highdim d={d} n={n} ms={ms}", flush=True)
+    X = workloads.gaussian_highdim(n, d, n_clusters=20, seed=d + ms)
+    st = _check(X, ms)
+    print(st["knn_tree_rows"], st["exact_rows"], st["rounds"], flush=True)
+
+
+def test_highdim_duplicates():
+    # exact ties: duplicated rows send their kNN rows to the tree search and
+    # make mutual-reachability ties that the depth-first order resolves
+    rng = np.random.default_rng(4)
+    X = workloads.gaussian_highdim(3000, 32, n_clusters=10, seed=4)
+    X = np.vstack([X, X[rng.choice(3000, 600, replace=False)]])
+    X = X[rng.permutation(X.shape[0])]
+    st = _check(X, 8)
+    assert st["knn_tree_rows"] > 0
+
+
+def test_highdim_integer_lattice_ties():
+    rng = np.random.default_rng(5)
+    X = rng.integers(0, 4, size=(4000, 20)).astype(np.float32)
+    _check(X, 5)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_mfma_path_forced_low_dim(monkeypatch, d):
+    # the MFMA path on a low-dimensional integer grid: ties everywhere
+    monkeypatch.setenv("FH_HIGH_DIM_FROM", "1")
+    rng = np.random.default_rng(d)
+    X = rng.integers(0, 40, size=(5000, d)).astype(np.float32)
+    _check(X, 6)
+    Y = workloads.blobs_small(6000, seed=d).astype(np.float32)[:, :d] if d <= 2 else rng.normal(size=(6000, d))
+    _check(np.ascontiguousarray(Y, dtype=np.float32), 10)
+
+
+def test_highdim_matches_tree_path_50k(monkeypatch):
+    # same exact answer from both device paths at a size the oracle would
+    # take minutes on
+    X = workloads.gaussian_highdim(50000, 64, n_clusters=100, seed=3)
+    a = mst_arrays(X, 10)
+    monkeypatch.setenv("FH_HIGH_DIM_FROM", "1000")
+    st = {}
+    b = mst_arrays(X, 10, stats=st)
+    assert st["high_dim"] == 0
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
