@@ -79,7 +79,6 @@ struct fh_ctx {
     DevBuf<int> bx_cand, bx_cnt, bx_flags, bx_rowarg, bx_rows, bx_small;
     DevBuf<unsigned int> bx_nmax;
     int brute_min_dim = 17;  // d from which the MFMA scans replace the tree searches
-    int brute_window = 2048; // tree-order window that bounds each row's k-th distance
     int brute_cap = 512;     // kNN candidates kept per row
     // edge sort
     DevBuf<uint32_t> sk1, sk1s;
@@ -318,6 +317,93 @@ int64_t weighted_k(const float *w, int64_t n, int ms) {
     return static_cast<int64_t>(2.0 * (static_cast<double>(ms) / mean));
 }
 
+// kNN of every point by the MFMA candidate scan (fh_brute.hip) with exact
+// re-scoring; rows that need the reference's heap order for ties go to the
+// tree search.  Leaves the packed points and norms in ctx for the rounds.
+int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_knn, fh::BruteScanArgs &bsa,
+          fh_stats &S) {
+    hipStream_t st = ctx->stream;
+    const int dp = fh::brute_padded_dim(d);
+    const float margin_c = 8.0f * dp + 64.0f;
+    const int64_t npad = (n + 127) / 128 * 128;
+    S.high_dim = 1;
+    FH_TRY(ensure(ctx, ctx->bx_xc, (size_t)npad * dp));
+    FH_TRY(ensure(ctx, ctx->bx_nrm, npad));
+    FH_TRY(ensure(ctx, ctx->bx_nmax, 1));
+    FH_TRY(ensure(ctx, ctx->bx_thr, n));
+    FH_TRY(ensure(ctx, ctx->bx_cand, (size_t)n * ctx->brute_cap));
+    FH_TRY(ensure(ctx, ctx->bx_cnt, n));
+    FH_TRY(ensure(ctx, ctx->bx_flags, n));
+    FH_TRY(ensure(ctx, ctx->bx_rowmin, n));
+    FH_TRY(ensure(ctx, ctx->bx_rowarg, n));
+    FH_TRY(ensure(ctx, ctx->bx_rows, n));
+    FH_TRY(ensure(ctx, ctx->bx_small, 4));
+    FH_HIP(hipMemsetAsync(ctx->bx_nmax.p, 0, sizeof(unsigned int), st));
+    FH_HIP(hipMemsetAsync(ctx->bx_small.p, 0, 4 * sizeof(int), st));
+    FH_HIP(fh::launch_center_pack(n, d, dp, npad, ctx->xp.p, ctx->box.p, ctx->bx_xc.p, ctx->bx_nrm.p,
+                                  ctx->bx_nmax.p, st));
+    bsa.xc = ctx->bx_xc.p;
+    bsa.nrm = ctx->bx_nrm.p;
+    bsa.n = n;
+    bsa.npad = npad;
+    bsa.dp = dp;
+    bsa.thr = ctx->bx_thr.p;
+    bsa.cand = ctx->bx_cand.p;
+    bsa.cand_cnt = ctx->bx_cnt.p;
+    bsa.cap = ctx->brute_cap;
+    bsa.core_p = ctx->core_p.p;
+    bsa.comp_p = ctx->comp_p.p;
+    bsa.row_min = ctx->bx_rowmin.p;
+    bsa.row_arg = ctx->bx_rowarg.p;
+    bsa.nrm_max_bits = ctx->bx_nmax.p;
+    bsa.margin_c = margin_c;
+    // thresholds: MODE 2 writes them where MODE 0 reads them
+    fh::BruteScanArgs kth = bsa;
+    kth.row_min = ctx->bx_thr.p;
+    kth.cap = kq;
+    FH_HIP(fh::launch_scan_kth(kth, st));
+    FH_HIP(fh::launch_scan_knn(bsa, st));
+    fh::KnnRescoreArgs ra{};
+    ra.xp = ctx->xp.p;
+    ra.perm = ctx->perm.p;
+    ra.cand = ctx->bx_cand.p;
+    ra.cand_cnt = ctx->bx_cnt.p;
+    ra.cap = ctx->brute_cap;
+    ra.n = n;
+    ra.d = d;
+    ra.k = kq;
+    ra.core_from_knn = core_from_knn;
+    ra.out_d = ctx->knn_d.p;
+    ra.out_i = ctx->knn_i.p;
+    ra.core_o = ctx->core_o.p;
+    ra.core_p = ctx->core_p.p;
+    ra.flag_list = ctx->bx_flags.p;
+    ra.flag_count = ctx->bx_small.p;
+    FH_HIP(fh::launch_knn_rescore(ra, st));
+    FH_HIP(hipMemcpyAsync(ctx->h_small, ctx->bx_small.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    const int nflag = ctx->h_small[0];
+    S.knn_tree_rows = nflag;
+    if (nflag > 0) {
+        fh::KnnWaveArgs wa{};
+        wa.xp = ctx->xp.p;
+        wa.range = ctx->range.p;
+        wa.box = ctx->box.p;
+        wa.perm = ctx->perm.p;
+        wa.query_pos = ctx->bx_flags.p;
+        wa.d = d;
+        wa.k = kq;
+        wa.levels = levels;
+        wa.core_from_knn = core_from_knn;
+        wa.out_d = ctx->knn_d.p;
+        wa.out_i = ctx->knn_i.p;
+        wa.core_o = ctx->core_o.p;
+        wa.core_p = ctx->core_p.p;
+        FH_HIP(fh::launch_knn_wave(wa, nflag, st));
+    }
+    return FH_OK;
+}
+
 // The MST proper.  Xd: the points in HBM (original order).  weights_d:
 // per-point sample weights in HBM or nullptr; kq_w: the kNN width for the
 // weighted branch.  Outputs are device pointers.
@@ -385,65 +471,9 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
         FH_TRY(ensure(ctx, ctx->nb_d, (size_t)n * (kq - 1)));
     }
     const float margin_c = 8.0f * dp + 64.0f;
-    const int64_t npad = (n + 127) / 128 * 128;
     fh::BruteScanArgs bsa{};
     if (brute) {
-        S.high_dim = 1;
-        FH_TRY(ensure(ctx, ctx->bx_xc, (size_t)npad * dp));
-        FH_TRY(ensure(ctx, ctx->bx_nrm, npad));
-        FH_TRY(ensure(ctx, ctx->bx_nmax, 1));
-        FH_TRY(ensure(ctx, ctx->bx_thr, n));
-        FH_TRY(ensure(ctx, ctx->bx_cand, (size_t)n * ctx->brute_cap));
-        FH_TRY(ensure(ctx, ctx->bx_cnt, n));
-        FH_TRY(ensure(ctx, ctx->bx_flags, n));
-        FH_TRY(ensure(ctx, ctx->bx_rowmin, n));
-        FH_TRY(ensure(ctx, ctx->bx_rowarg, n));
-        FH_TRY(ensure(ctx, ctx->bx_rows, n));
-        FH_TRY(ensure(ctx, ctx->bx_small, 4));
-        FH_HIP(hipMemsetAsync(ctx->bx_nmax.p, 0, sizeof(unsigned int), st));
-        FH_HIP(hipMemsetAsync(ctx->bx_small.p, 0, 4 * sizeof(int), st));
-        FH_HIP(fh::launch_center_pack(n, d, dp, npad, ctx->xp.p, ctx->box.p, ctx->bx_xc.p, ctx->bx_nrm.p,
-                                      ctx->bx_nmax.p, st));
-        FH_HIP(fh::launch_local_kth(n, d, kq, ctx->brute_window, ctx->xp.p, ctx->bx_nrm.p, ctx->bx_nmax.p, margin_c,
-                                    ctx->bx_thr.p, st));
-        bsa.xc = ctx->bx_xc.p;
-        bsa.nrm = ctx->bx_nrm.p;
-        bsa.n = n;
-        bsa.npad = npad;
-        bsa.dp = dp;
-        bsa.thr = ctx->bx_thr.p;
-        bsa.cand = ctx->bx_cand.p;
-        bsa.cand_cnt = ctx->bx_cnt.p;
-        bsa.cap = ctx->brute_cap;
-        bsa.core_p = ctx->core_p.p;
-        bsa.comp_p = ctx->comp_p.p;
-        bsa.row_min = ctx->bx_rowmin.p;
-        bsa.row_arg = ctx->bx_rowarg.p;
-        FH_HIP(fh::launch_scan_knn(bsa, st));
-        fh::KnnRescoreArgs ra{};
-        ra.xp = ctx->xp.p;
-        ra.perm = ctx->perm.p;
-        ra.cand = ctx->bx_cand.p;
-        ra.cand_cnt = ctx->bx_cnt.p;
-        ra.cap = ctx->brute_cap;
-        ra.n = n;
-        ra.d = d;
-        ra.k = kq;
-        ra.core_from_knn = ms > 1 ? 1 : 0;
-        ra.out_d = ctx->knn_d.p;
-        ra.out_i = ctx->knn_i.p;
-        ra.core_o = ctx->core_o.p;
-        ra.core_p = ctx->core_p.p;
-        ra.flag_list = ctx->bx_flags.p;
-        ra.flag_count = ctx->bx_small.p;
-        FH_HIP(fh::launch_knn_rescore(ra, st));
-        FH_HIP(hipMemcpyAsync(ctx->h_small, ctx->bx_small.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        FH_HIP(hipStreamSynchronize(st));
-        const int nflag = ctx->h_small[0];
-        S.knn_tree_rows = nflag;
-        if (nflag > 0)
-            FH_TRY(run_knn(ctx, d, levels, nullptr, nullptr, nflag, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p,
-                           ctx->core_p.p, ms > 1 ? 1 : 0, nullptr, nullptr, 0, ctx->bx_flags.p));
+        FH_TRY(brute_knn(ctx, n, d, levels, kq, ms > 1 ? 1 : 0, bsa, S));
     } else {
         FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p,
                        ctx->core_p.p, (ms > 1 && !weighted) ? 1 : 0, seed ? ctx->nb_pos.p : nullptr,
@@ -644,7 +674,6 @@ fh_ctx *fh_create(int device) {
     fh_ctx *c = new fh_ctx();
     c->device = device;
     if (const char *m = std::getenv("FH_SEARCH_MODE")) c->search_mode = std::atoi(m);
-    if (const char *m = std::getenv("FH_HIGH_DIM_WINDOW")) c->brute_window = std::max(64, std::atoi(m) / 64 * 64);
     if (const char *m = std::getenv("FH_HIGH_DIM_CAP")) c->brute_cap = std::max(16, std::atoi(m));
     if (hipSetDevice(device) != hipSuccess) {
         delete c;

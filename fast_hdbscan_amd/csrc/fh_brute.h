@@ -23,6 +23,9 @@ struct BruteScanArgs {
     int *cand;            // [n * cap] candidate positions
     int *cand_cnt;        // [n] candidates found (may exceed cap)
     int cap;
+    // k-th distance bounds (MODE 2): the k-th smallest is taken with k = cap
+    const unsigned int *nrm_max_bits;
+    float margin_c;
     // Boruvka row minimum
     const float *core_p;  // [n]
     const int *comp_p;    // [n]
@@ -32,8 +35,8 @@ struct BruteScanArgs {
 
 hipError_t launch_center_pack(int64_t n, int d, int dp, int64_t npad, const float *xp, const float *root_box,
                               float *xc, float *nrm, unsigned int *nrm_max_bits, hipStream_t st);
-hipError_t launch_local_kth(int64_t n, int d, int k, int window, const float *xp, const float *nrm,
-                            const unsigned int *nrm_max_bits, float margin_c, float *thr, hipStream_t st);
+// row thresholds into row_min (k = cap), then candidate collection
+hipError_t launch_scan_kth(const BruteScanArgs &a, hipStream_t st);
 hipError_t launch_scan_knn(const BruteScanArgs &a, hipStream_t st);
 hipError_t launch_scan_boruvka(const BruteScanArgs &a, hipStream_t st);
 
@@ -54,6 +57,22 @@ struct KnnRescoreArgs {
     int *flag_count;
 };
 hipError_t launch_knn_rescore(const KnnRescoreArgs &a, hipStream_t st);
+
+// tree kNN (the reference's heap order) for listed positions, a wave each
+struct KnnWaveArgs {
+    const float *xp;
+    const int2 *range;
+    const float *box;
+    const int *perm;
+    const int *query_pos;
+    int d, k, levels;
+    int core_from_knn;
+    float *out_d;
+    int *out_i;
+    float *core_o;
+    float *core_p;
+};
+hipError_t launch_knn_wave(const KnnWaveArgs &a, int count, hipStream_t st);
 
 struct BoruvkaExactArgs {
     const float *xp;

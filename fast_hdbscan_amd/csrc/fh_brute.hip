@@ -15,10 +15,11 @@
 // uses margin_c * u * (|x'|^2 + max|y'|^2) with margin_c = 8 dp + 64, twice
 // that.
 //
-// kNN (numba_kdtree.py:765-792): per row, T = the k-th smallest exact rdist
-// among a window of tree-order neighbours (an upper bound of the true k-th
-// distance); the MFMA scan collects every column with approx <= T + margin,
-// a superset of the true k nearest; exact re-scoring sorts them.  When two
+// kNN (numba_kdtree.py:765-792): a first MFMA scan bounds each row's k-th
+// distance from above (the k-th smallest of per-lane top-4 lists: k real
+// columns); a second collects every column whose approximate distance is
+// within two margins of that bound, a superset of the true k nearest;
+// exact re-scoring sorts them.  When two
 // of the k+1 smallest exact values tie, the reference's heap order decides
 // the output, so such rows (and rows whose candidate list overflowed) are
 // answered by the exact tree search instead.
@@ -78,70 +79,6 @@ __global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const
     if (p < n) atomicMax(nrm_max_bits, __float_as_uint(s));
 }
 
-// k-th smallest exact rdist among the window of tree-order positions
-// around each row; thr = that + margin.  64 rows per block share the window,
-// staged through LDS 64 candidates at a time.
-template <int DP>
-__global__ __launch_bounds__(64) void local_kth_kernel(int64_t n, int d, int k, int window,
-                                                       const float *__restrict__ xp, const float *__restrict__ nrm,
-                                                       const unsigned int *__restrict__ nrm_max_bits, float margin_c,
-                                                       float *__restrict__ thr) {
-    __shared__ float heap[64 * 64];
-    __shared__ float cs[64 * DP];
-    const int tid = threadIdx.x;
-    const int64_t base = (int64_t)blockIdx.x * 64;
-    const int64_t p = base + tid;
-    int64_t w0 = (base / window) * window;
-    if (w0 + window > n) w0 = n > window ? n - window : 0;
-    const int64_t w1 = w0 + window < n ? w0 + window : n;
-    float x[DP];
-#pragma unroll
-    for (int j = 0; j < DP; ++j) x[j] = (p < n && j < d) ? xp[p * d + j] : 0.0f;
-    float *h = heap + tid;  // slot s at h[s * 64]
-    for (int s = 0; s < k; ++s) h[s * 64] = kInfB;
-    for (int64_t c0 = w0; c0 < w1; c0 += 64) {
-        const int cnt = (int)((w1 - c0) < 64 ? (w1 - c0) : 64);
-        __syncthreads();
-        for (int e = tid; e < 64 * DP; e += 64) {
-            const int c = e / DP, j = e % DP;
-            cs[e] = (c < cnt && j < d) ? xp[(c0 + c) * d + j] : 0.0f;
-        }
-        __syncthreads();
-        if (p < n) {
-            for (int c = 0; c < cnt; ++c) {
-                const float *y = cs + c * DP;
-                float r = 0.0f;
-#pragma unroll
-                for (int j = 0; j < DP; ++j) {
-                    const float df = x[j] - y[j];
-                    r = __builtin_fmaf(df, df, r);
-                }
-                if (r < h[0]) {
-                    // replace the root of the max-heap and sift down
-                    int at = 0;
-                    for (;;) {
-                        const int c1 = 2 * at + 1;
-                        if (c1 >= k) break;
-                        int ch = c1;
-                        if (c1 + 1 < k && h[(c1 + 1) * 64] > h[c1 * 64]) ch = c1 + 1;
-                        if (h[ch * 64] > r) {
-                            h[at * 64] = h[ch * 64];
-                            at = ch;
-                        } else {
-                            break;
-                        }
-                    }
-                    h[at * 64] = r;
-                }
-            }
-        }
-    }
-    if (p < n) {
-        const float nm = __uint_as_float(*nrm_max_bits);
-        thr[p] = h[0] + margin_c * kU * (nrm[p] + nm);
-    }
-}
-
 // --------------------------------------------------------- MFMA scans ----
 
 // One block owns 128 rows (4 waves x 32) and streams every 128-column tile
@@ -152,12 +89,18 @@ __global__ __launch_bounds__(64) void local_kth_kernel(int64_t n, int d, int k, 
 // MODE 0: collect kNN candidates under the row thresholds.
 // MODE 1: smallest approximate mutual-reachability value to another
 //         component per row.
+// MODE 2: row thresholds for MODE 0.  Each lane keeps its 4 smallest
+//         approximate distances per row; the k-th smallest of the union
+//         (64 values from 16 lanes, all real columns) plus one margin
+//         bounds the true k-th exact distance, plus a second margin bounds
+//         the approximate value of every true k nearest neighbour.
 template <int DP4, int MODE>
 __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
     constexpr int DP = 4 * DP4;
     constexpr int LROW = 144;
     constexpr int NV = DP / 8;  // float4 loads per thread per tile
-    __shared__ float bs[DP * LROW];
+    // (MODE 2 reuses this space for the per-row unions: 128 x 64 floats)
+    __shared__ float bs[(DP * LROW) > (MODE == 2 ? 128 * 64 : 0) ? DP * LROW : 128 * 64];
     __shared__ float col_nrm[128];
     __shared__ float col_core[128];
     __shared__ int col_comp[128];
@@ -179,6 +122,14 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
     // the 8 output rows this lane holds: rbase + rb*16 + 4g + r
     float rn[2][4], rv[2][4], rcore[2][4];
     int ri[2][4], rcomp[2][4];
+    // MODE 2: per row the lane's 4 smallest values, t[..][0] <= ... <= t[..][3]
+    float t4[2][4][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) t4[rb][r][q] = kInfB;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -189,6 +140,10 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
             ri[rb][r] = -1;
             if constexpr (MODE == 0) {
                 rv[rb][r] = ok ? a.thr[row] : -kInfB;
+                rcore[rb][r] = 0.0f;
+                rcomp[rb][r] = 0;
+            } else if constexpr (MODE == 2) {
+                rv[rb][r] = kInfB;
                 rcore[rb][r] = 0.0f;
                 rcomp[rb][r] = 0;
             } else {
@@ -264,7 +219,29 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
             const int64_t col = col0 + cl;
             const bool valid = col < n;
             const float cn = col_nrm[cl];
-            if constexpr (MODE == 0) {
+            if constexpr (MODE == 2) {
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float approx = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
+                        if (valid && approx < t4[rb][r][3]) {
+                            float v = approx;
+                            float *t = t4[rb][r];
+                            // insert into the sorted 4-list
+                            const float a3 = t[2] > v ? t[2] : v;
+                            v = t[2] > v ? v : t[2];
+                            const float a2 = t[1] > v ? t[1] : v;
+                            v = t[1] > v ? v : t[1];
+                            const float a1 = t[0] > v ? t[0] : v;
+                            v = t[0] > v ? v : t[0];
+                            t[3] = a3;
+                            t[2] = a2;
+                            t[1] = a1;
+                            t[0] = v;
+                        }
+                    }
+            } else if constexpr (MODE == 0) {
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -298,6 +275,38 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
     if constexpr (MODE == 0) {
         __syncthreads();
         if (tid < 128 && blk_row + tid < n) a.cand_cnt[blk_row + tid] = cnt_l[tid];
+    } else if constexpr (MODE == 2) {
+        // gather the 16 lanes' lists per row, then one thread per row picks
+        // the k-th smallest of its 64 values
+        __syncthreads();
+        float *un = bs;  // [128 rows][64]
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rl = wave * 32 + rb * 16 + 4 * g + r;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) un[rl * 64 + c16 * 4 + q] = t4[rb][r][q];
+            }
+        __syncthreads();
+        if (tid < 128 && blk_row + tid < n) {
+            const int64_t row = blk_row + tid;
+            const float *u = un + tid * 64;
+            // k-th smallest (1-based k) by counting: the value v with
+            // #(< v) < k <= #(<= v)
+            float kth = kInfB;
+            for (int i = 0; i < 64; ++i) {
+                const float v = u[i];
+                int lt = 0, le = 0;
+                for (int j = 0; j < 64; ++j) {
+                    lt += u[j] < v;
+                    le += u[j] <= v;
+                }
+                if (lt < a.cap && a.cap <= le) kth = v;
+            }
+            const float nm = __uint_as_float(*a.nrm_max_bits);
+            a.row_min[row] = kth + 2.0f * a.margin_c * kU * (a.nrm[row] + nm);
+        }
     } else {
         // minimum over the 16 lanes that hold the same rows
 #pragma unroll
@@ -376,6 +385,94 @@ __global__ __launch_bounds__(64) void knn_rescore_kernel(KnnRescoreArgs a) {
     const float c = a.core_from_knn ? v[(a.k - 1) * 64] : 0.0f;
     a.core_o[row] = c;
     a.core_p[p] = c;
+}
+
+// ---------------------------------------------- tree kNN, one wave/row ----
+
+// The reference's kNN query for the rows the MFMA path hands back (ties):
+// the same depth-first walk, pruning and heap pushes as knn_kernel, with a
+// whole wave per row.  The walk is uniform across the wave; a leaf's rdist
+// values are computed one point per lane and pushed by lane 0 in leaf
+// order, child bounds by lanes 0 and 1.  In high dimension the walk visits
+// almost every leaf, so this is what makes those rows affordable.
+__global__ __launch_bounds__(64) void knn_wave_kernel(KnnWaveArgs a) {
+    __shared__ float xs[64];
+    __shared__ float hp[64];
+    __shared__ int hi[64];
+    __shared__ float pv[64];
+    const int lane = threadIdx.x;
+    const int pos = a.query_pos[blockIdx.x];
+    const int d = a.d, k = a.k;
+    if (lane < d) xs[lane] = a.xp[(int64_t)pos * d + lane];
+    if (lane < k) {
+        hp[lane] = kInfB;
+        hi[lane] = -1;
+    }
+    __syncthreads();
+    LdsHeap h{hp, hi, 1, k};
+    const size_t stride = (size_t)2 * d;
+    float v0 = 0.0f;
+    if (lane == 0) v0 = box_lb<0>(xs, a.box, d);
+    float lb = __shfl(v0, 0);
+    int node = 0, depth = 0;
+    uint32_t first_right = 0;
+    bool entering = true;
+    for (;;) {
+        if (entering && !(lb > hp[0])) {
+            if (depth == a.levels - 1) {
+                const int2 r = a.range[node];
+                for (int c0 = r.x; c0 < r.y; c0 += 64) {
+                    const int i = c0 + lane;
+                    if (i < r.y) pv[lane] = rdist<0>(xs, a.xp + (int64_t)i * d, d);
+                    __syncthreads();
+                    if (lane == 0) {
+                        const int cnt = r.y - c0 < 64 ? r.y - c0 : 64;
+                        for (int t = 0; t < cnt; ++t) {
+                            const float dist = pv[t];
+                            if (dist < hp[0]) h.push(dist, c0 + t);
+                        }
+                    }
+                    __syncthreads();
+                }
+            } else {
+                const int l = 2 * node + 1;
+                float v = 0.0f;
+                if (lane < 2) v = box_lb<0>(xs, a.box + (size_t)(l + lane) * stride, d);
+                const float lbl = __shfl(v, 0), lbr = __shfl(v, 1);
+                const bool rf = !(lbl <= lbr);
+                ++depth;
+                first_right = rf ? (first_right | (1u << depth)) : (first_right & ~(1u << depth));
+                node = rf ? l + 1 : l;
+                lb = rf ? lbr : lbl;
+                continue;
+            }
+        }
+        if (depth == 0) break;
+        const bool rf = (first_right >> depth) & 1u;
+        const bool is_right = (node & 1) == 0;
+        if (is_right == rf) {
+            node = is_right ? node - 1 : node + 1;
+            float v = 0.0f;
+            if (lane == 0) v = box_lb<0>(xs, a.box + (size_t)node * stride, d);
+            lb = __shfl(v, 0);
+            entering = true;
+        } else {
+            node = (node - 1) >> 1;
+            --depth;
+            entering = false;
+        }
+    }
+    if (lane == 0) {
+        h.sort();
+        const int64_t row = a.perm[pos];
+        for (int j = 0; j < k; ++j) {
+            a.out_d[row * k + j] = hp[j];
+            a.out_i[row * k + j] = hi[j] >= 0 ? a.perm[hi[j]] : -1;
+        }
+        const float c = a.core_from_knn ? hp[k - 1] : 0.0f;
+        a.core_o[row] = c;
+        a.core_p[pos] = c;
+    }
 }
 
 // ------------------------------------------------ exact Boruvka steps ----
@@ -493,20 +590,6 @@ hipError_t launch_center_pack(int64_t n, int d, int dp, int64_t npad, const floa
     return hipGetLastError();
 }
 
-hipError_t launch_local_kth(int64_t n, int d, int k, int window, const float *xp, const float *nrm,
-                            const unsigned int *nrm_max_bits, float margin_c, float *thr, hipStream_t st) {
-    const dim3 g(nblocks(n, 64)), b(64);
-    switch (brute_padded_dim(d)) {
-        case 16: hipLaunchKernelGGL(local_kth_kernel<16>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
-        case 24: hipLaunchKernelGGL(local_kth_kernel<24>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
-        case 32: hipLaunchKernelGGL(local_kth_kernel<32>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
-        case 48: hipLaunchKernelGGL(local_kth_kernel<48>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
-        case 64: hipLaunchKernelGGL(local_kth_kernel<64>, g, b, 0, st, n, d, k, window, xp, nrm, nrm_max_bits, margin_c, thr); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 template <int MODE>
 static hipError_t launch_scan(const BruteScanArgs &a, hipStream_t st) {
     const dim3 g(a.npad / 128), b(256);
@@ -522,11 +605,19 @@ static hipError_t launch_scan(const BruteScanArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_scan_knn(const BruteScanArgs &a, hipStream_t st) { return launch_scan<0>(a, st); }
+hipError_t launch_scan_kth(const BruteScanArgs &a, hipStream_t st) { return launch_scan<2>(a, st); }
 hipError_t launch_scan_boruvka(const BruteScanArgs &a, hipStream_t st) { return launch_scan<1>(a, st); }
 
 hipError_t launch_knn_rescore(const KnnRescoreArgs &a, hipStream_t st) {
     if (a.k + 1 > 65) return hipErrorInvalidValue;
     hipLaunchKernelGGL(knn_rescore_kernel, dim3(nblocks(a.n, 64)), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_wave(const KnnWaveArgs &a, int count, hipStream_t st) {
+    if (a.k > 64 || a.d > 64) return hipErrorInvalidValue;
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(knn_wave_kernel, dim3(count), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 
