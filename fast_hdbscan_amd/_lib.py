@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = (
     "fh_neighbor_columns",
     "fh_cluster_layers",
     "fh_layer_cluster_tree",
+    "fh_mst_kruskal_knn",
 )
 
 
@@ -163,6 +164,9 @@ def load() -> ctypes.CDLL:
         L.fh_cluster_layers.argtypes = [_i64, _vp, _vp, _vp, ctypes.POINTER(FHLayerParams), ctypes.POINTER(FHLayers)]
         L.fh_layer_cluster_tree.restype = _i64
         L.fh_layer_cluster_tree.argtypes = [_i32, _i64, _vp, _vp, _i64]
+        L.fh_mst_kruskal_knn.restype = ctypes.c_int
+        L.fh_mst_kruskal_knn.argtypes = [_vp, _vp, _i64, _i32, _i32, _i32, _vp, _i32, _vp, _vp, _i32, _vp,
+                                         ctypes.POINTER(FHStats)]
         L.fh_sort_mst_device.restype = ctypes.c_int
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L

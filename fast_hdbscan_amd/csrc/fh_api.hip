@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,10 @@ struct fh_ctx {
     DevBuf<int> bx_cand, bx_cnt, bx_flags, bx_rowarg, bx_rows, bx_small;
     DevBuf<unsigned int> bx_nmax;
     int brute_min_dim = 17;  // d from which the MFMA scans replace the tree searches
+    // kNN-graph Kruskal
+    DevBuf<uint64_t> kr_keys, kr_keys_s;
+    DevBuf<int> kr_vals, kr_vals_s, kr_src, kr_dst;
+    DevBuf<double> kr_w, kr_core;
     int brute_cap = 512;     // kNN candidates kept per row
     // edge sort
     DevBuf<uint32_t> sk1, sk1s;
@@ -719,6 +724,8 @@ void fh_destroy(fh_ctx *c) {
     release(c->bx_xc); release(c->bx_nrm); release(c->bx_thr); release(c->bx_rowmin); release(c->bx_cand);
     release(c->bx_cnt); release(c->bx_flags); release(c->bx_rowarg); release(c->bx_rows); release(c->bx_small);
     release(c->bx_nmax);
+    release(c->kr_keys); release(c->kr_keys_s); release(c->kr_vals); release(c->kr_vals_s); release(c->kr_src);
+    release(c->kr_dst); release(c->kr_w); release(c->kr_core);
     if (c->h_small) e = hipHostFree(c->h_small);
     for (auto &ev : c->ev)
         if (ev) e = hipEventDestroy(ev);
@@ -907,6 +914,185 @@ int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t
     a.temp_bytes = ctx->stemp.cap;
     FH_HIP(fh::sort_mst(a, ctx->stream));
     FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
+// kruskal.py:950-1034 (_kruskal_mst_knn) without cannot-link constraints.
+// Device: tree, kNN (k_query columns), weighted core, mutual-reachability
+// edges in float64 and their stable global sort.  Host: Kruskal's
+// union-find (_kruskal_core, kruskal.py:31-108) and the +inf bridges of a
+// disconnected forest (_finish_mst, precomputed.bridge_forest_with_inf).
+int fh_mst_kruskal_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, int32_t knn_k,
+                       const float *sample_weights, int32_t k_for_core, double *mst, int32_t *nbr, int32_t nbr_cols,
+                       double *core, fh_stats *stats) {
+    FH_TRY(check_args(ctx, X, n, d, min_samples));
+    if (knn_k < 1) return fail(ctx, FH_EINVAL, "knn_k must be >= 1");
+    if (n > 1 && !mst) return fail(ctx, FH_EINVAL, "mst output is null");
+    const bool weighted = sample_weights != nullptr;
+    // kruskal.py:987-1011
+    int64_t kq64;
+    int want_cols;
+    if (weighted) {
+        kq64 = (int64_t)std::max(knn_k, k_for_core) + 1;
+        want_cols = (int)kq64 - 1;
+    } else if (min_samples > 1) {
+        kq64 = (int64_t)std::max(knn_k, min_samples) + 1;
+        want_cols = min_samples;
+    } else {
+        kq64 = (int64_t)knn_k + 1;
+        want_cols = 1;
+    }
+    if (kq64 > 4096) return fail(ctx, FH_EUNSUPPORTED, "k_query above 4096");
+    if (nbr && nbr_cols != want_cols) return fail(ctx, FH_EINVAL, "nbr_cols must be " + std::to_string(want_cols));
+    const int kq = (int)kq64;
+    const int64_t m = n * (kq - 1);
+    if (m >= (int64_t(1) << 31)) return fail(ctx, FH_EUNSUPPORTED, "n * (k_query - 1) must stay below 2^31");
+    FH_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    Timer total;
+    fh_stats S{};
+    FH_TRY(ensure(ctx, ctx->xdev, (size_t)n * d));
+    FH_HIP(hipMemcpyAsync(ctx->xdev.p, X, sizeof(float) * n * d, hipMemcpyHostToDevice, st));
+    if (weighted) {
+        FH_TRY(ensure(ctx, ctx->wdev, n));
+        FH_HIP(hipMemcpyAsync(ctx->wdev.p, sample_weights, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    }
+    int levels = 0;
+    int64_t n_nodes = 0;
+    FH_TRY(build_tree_device(ctx, ctx->xdev.p, n, d, &levels, &n_nodes));
+    S.n_levels = levels;
+    S.n_nodes = n_nodes;
+    FH_TRY(ensure(ctx, ctx->knn_d, (size_t)n * kq));
+    FH_TRY(ensure(ctx, ctx->knn_i, (size_t)n * kq));
+    FH_TRY(ensure(ctx, ctx->core_o, n));
+    FH_TRY(ensure(ctx, ctx->core_p, n));
+    FH_TRY(ensure(ctx, ctx->comp_p, n));
+    int brute_from = ctx->brute_min_dim;
+    if (const char *e = std::getenv("FH_HIGH_DIM_FROM")) brute_from = std::atoi(e);
+    const bool brute = !weighted && fh::brute_padded_dim(d) > 0 && d >= brute_from && kq <= 64 && n >= 256;
+    if (brute) {
+        fh::BruteScanArgs bsa{};
+        FH_TRY(brute_knn(ctx, n, d, levels, kq, 0, bsa, S));
+    } else {
+        FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p,
+                       ctx->core_p.p, 0, nullptr, nullptr, weighted ? 1 : 0));
+    }
+    if (weighted)
+        FH_HIP(fh::launch_weighted_core((int)n, kq, ctx->knn_i.p, ctx->knn_d.p, ctx->wdev.p, min_samples, ctx->posof.p,
+                                        ctx->core_o.p, ctx->core_p.p, st));
+    // edges
+    FH_TRY(ensure(ctx, ctx->kr_keys, m));
+    FH_TRY(ensure(ctx, ctx->kr_keys_s, m));
+    FH_TRY(ensure(ctx, ctx->kr_vals, m));
+    FH_TRY(ensure(ctx, ctx->kr_vals_s, m));
+    FH_TRY(ensure(ctx, ctx->kr_src, m));
+    FH_TRY(ensure(ctx, ctx->kr_dst, m));
+    FH_TRY(ensure(ctx, ctx->kr_w, m));
+    FH_TRY(ensure(ctx, ctx->kr_core, n));
+    FH_TRY(ensure(ctx, ctx->small, 8));
+    const size_t tb = fh::knn_edges_temp_bytes(m);
+    FH_TRY(ensure(ctx, ctx->stemp, tb));
+    FH_HIP(hipMemsetAsync(ctx->small.p, 0, sizeof(int), st));
+    fh::KnnEdgesArgs ka{};
+    ka.n = n;
+    ka.kq = kq;
+    ka.mode = weighted ? 2 : (min_samples > 1 ? 0 : 1);
+    ka.ms = min_samples;
+    ka.knn_d = ctx->knn_d.p;
+    ka.knn_i = ctx->knn_i.p;
+    ka.core32 = ctx->core_o.p;
+    ka.core64 = ctx->kr_core.p;
+    ka.keys = ctx->kr_keys.p;
+    ka.keys_s = ctx->kr_keys_s.p;
+    ka.vals = ctx->kr_vals.p;
+    ka.vals_s = ctx->kr_vals_s.p;
+    ka.valid_count = ctx->small.p;
+    ka.src_out = ctx->kr_src.p;
+    ka.dst_out = ctx->kr_dst.p;
+    ka.w_out = ctx->kr_w.p;
+    ka.temp = ctx->stemp.p;
+    ka.temp_bytes = ctx->stemp.cap;
+    FH_HIP(fh::knn_edges_sorted(ka, st));
+    int nvalid = 0;
+    FH_HIP(hipMemcpyAsync(ctx->h_small, ctx->small.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    nvalid = ctx->h_small[0];
+    std::vector<int> hs(nvalid), hd(nvalid);
+    std::vector<double> hw(nvalid);
+    if (nvalid > 0) {
+        FH_HIP(hipMemcpyAsync(hs.data(), ctx->kr_src.p, sizeof(int) * nvalid, hipMemcpyDeviceToHost, st));
+        FH_HIP(hipMemcpyAsync(hd.data(), ctx->kr_dst.p, sizeof(int) * nvalid, hipMemcpyDeviceToHost, st));
+        FH_HIP(hipMemcpyAsync(hw.data(), ctx->kr_w.p, sizeof(double) * nvalid, hipMemcpyDeviceToHost, st));
+    }
+    if (core) FH_HIP(hipMemcpyAsync(core, ctx->kr_core.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    std::vector<int> hk;
+    if (nbr) {
+        hk.resize((size_t)n * kq);
+        FH_HIP(hipMemcpyAsync(hk.data(), ctx->knn_i.p, sizeof(int) * n * kq, hipMemcpyDeviceToHost, st));
+    }
+    FH_HIP(hipStreamSynchronize(st));
+    if (nbr)
+        for (int64_t i = 0; i < n; ++i)
+            for (int c = 0; c < want_cols; ++c) nbr[i * want_cols + c] = hk[(size_t)i * kq + 1 + c];
+    // _kruskal_core: union by rank, full path compression, stop at n-1
+    std::vector<int> pred(n), rank(n, 0);
+    for (int64_t i = 0; i < n; ++i) pred[i] = (int)i;
+    auto root_of = [&](int x) {
+        int r = x;
+        while (pred[r] != r) r = pred[r];
+        while (x != r) {
+            const int nx = pred[x];
+            pred[x] = r;
+            x = nx;
+        }
+        return r;
+    };
+    int64_t n_mst = 0;
+    for (int64_t r = 0; r < nvalid && n_mst < n - 1; ++r) {
+        const int u = hs[r], v = hd[r];
+        if (u == v) continue;
+        const int ru = root_of(u), rv = root_of(v);
+        if (ru == rv) continue;
+        mst[3 * n_mst + 0] = (double)u;
+        mst[3 * n_mst + 1] = (double)v;
+        mst[3 * n_mst + 2] = hw[r];
+        ++n_mst;
+        if (rank[ru] > rank[rv]) {
+            pred[rv] = ru;
+        } else if (rank[ru] < rank[rv]) {
+            pred[ru] = rv;
+        } else {
+            pred[rv] = ru;
+            rank[ru] += 1;
+        }
+    }
+    // _finish_mst: one representative (smallest id) per component, bridged
+    // in ascending order with +inf edges
+    if (n_mst < n - 1) {
+        std::vector<char> seen(n, 0);
+        std::vector<int> reps;
+        for (int64_t i = 0; i < n; ++i) {
+            int r = (int)i;
+            while (pred[r] != r) r = pred[r];
+            if (!seen[r]) {
+                seen[r] = 1;
+                reps.push_back((int)i);  // first (smallest) member met
+            }
+        }
+        std::sort(reps.begin(), reps.end());
+        for (size_t q = 0; q + 1 < reps.size(); ++q) {
+            mst[3 * n_mst + 0] = (double)reps[q];
+            mst[3 * n_mst + 1] = (double)reps[q + 1];
+            mst[3 * n_mst + 2] = std::numeric_limits<double>::infinity();
+            ++n_mst;
+        }
+    }
+    if (n_mst != n - 1)
+        return fail(ctx, FH_EDEVICE, "Kruskal produced " + std::to_string(n_mst) + " rows for " + std::to_string(n) +
+                                         " points");
+    S.n_edges = n_mst;
+    S.t_total_ms = total.ms();
+    if (stats) *stats = S;
     return FH_OK;
 }
 

@@ -26,4 +26,31 @@ struct SortMstArgs {
 size_t sort_mst_temp_bytes(int m);
 hipError_t sort_mst(const SortMstArgs &a, hipStream_t st);
 
+// Mutual-reachability edge list of a kNN graph, globally sorted by weight
+// (kruskal.py:455-512 + the stable argsort at 1023): kNN rows by original
+// index, columns 1..kq-1.  mode 0: core = sqrt(float64(rdist column ms));
+// 1: core = 0; 2: core = float64(core32), distances already square roots.
+struct KnnEdgesArgs {
+    int64_t n;
+    int kq;
+    int mode;
+    int ms;
+    const float *knn_d;
+    const int *knn_i;
+    const float *core32;
+    double *core64;     // [n] out
+    uint64_t *keys;     // [m] scratch, m = n * (kq - 1)
+    uint64_t *keys_s;   // [m] sorted keys
+    int *vals;          // [m] scratch
+    int *vals_s;        // [m] sorted edge ids
+    int *valid_count;   // [1] out
+    int *src_out;       // [m] sorted rows (first valid_count meaningful)
+    int *dst_out;
+    double *w_out;
+    void *temp;
+    size_t temp_bytes;
+};
+size_t knn_edges_temp_bytes(int64_t m);
+hipError_t knn_edges_sorted(const KnnEdgesArgs &a, hipStream_t st);
+
 }  // namespace fh

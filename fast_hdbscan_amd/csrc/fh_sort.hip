@@ -40,7 +40,84 @@ __global__ void sort_gather_kernel(int m, const int *__restrict__ perm, const in
 
 inline unsigned nb(int m) { return (unsigned)((m + 255) / 256); }
 
+__global__ void knn_core64_kernel(int64_t n, int kq, int mode, int ms, const float *__restrict__ knn_d,
+                                  const float *__restrict__ core32, double *core64) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double c = 0.0;
+    if (mode == 0) c = sqrt(static_cast<double>(knn_d[i * kq + ms]));
+    else if (mode == 2) c = static_cast<double>(core32[i]);
+    core64[i] = c;
+}
+
+// kruskal.py:483-510: w = max(cd_i, cd_j, d) by the same comparisons; a -1
+// neighbour becomes an infinite self-loop, which the caller's filter drops
+// (kruskal.py:1017-1021).  Dropped edges get the largest key so the sorted
+// valid edges come first, in stable order.
+__global__ void knn_edge_keys_kernel(int64_t n, int kq, int mode, const float *__restrict__ knn_d,
+                                     const int *__restrict__ knn_i, const double *__restrict__ core64,
+                                     uint64_t *keys, int *vals, int *valid_count) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int kc = kq - 1;
+    if (e >= n * kc) return;
+    const int64_t i = e / kc;
+    const int t = (int)(e % kc) + 1;
+    const int j = knn_i[i * kq + t];
+    uint64_t key = ~0ull;
+    if (j >= 0 && j != i) {
+        const float dv = knn_d[i * kq + t];
+        const double d = mode == 2 ? static_cast<double>(dv) : sqrt(static_cast<double>(dv));
+        double w = core64[i];
+        if (core64[j] > w) w = core64[j];
+        if (d > w) w = d;
+        if (isfinite(w)) {
+            key = static_cast<uint64_t>(__double_as_longlong(w));  // w >= 0: bits order like values
+            atomicAdd(valid_count, 1);
+        }
+    }
+    keys[e] = key;
+    vals[e] = (int)e;
+}
+
+__global__ void knn_edge_gather_kernel(int64_t m, int kq, const int *__restrict__ knn_i,
+                                       const uint64_t *__restrict__ keys_s, const int *__restrict__ vals_s, int *so,
+                                       int *dso, double *wo) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const int64_t e = vals_s[r];
+    const int kc = kq - 1;
+    const int64_t i = e / kc;
+    const int t = (int)(e % kc) + 1;
+    so[r] = (int)i;
+    dso[r] = knn_i[i * kq + t];
+    wo[r] = __longlong_as_double(static_cast<long long>(keys_s[r]));
+}
+
 }  // namespace
+
+size_t knn_edges_temp_bytes(int64_t m) {
+    size_t b = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<uint64_t *>(nullptr),
+                                                      static_cast<uint64_t *>(nullptr), static_cast<int *>(nullptr),
+                                                      static_cast<int *>(nullptr), (int)m);
+    return e == hipSuccess ? b : 0;
+}
+
+hipError_t knn_edges_sorted(const KnnEdgesArgs &a, hipStream_t st) {
+    const int64_t m = a.n * (a.kq - 1);
+    hipLaunchKernelGGL(knn_core64_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a.n, a.kq, a.mode,
+                       a.ms, a.knn_d, a.core32, a.core64);
+    if (m == 0) return hipGetLastError();
+    const unsigned g = (unsigned)((m + 255) / 256);
+    hipLaunchKernelGGL(knn_edge_keys_kernel, dim3(g), dim3(256), 0, st, a.n, a.kq, a.mode, a.knn_d, a.knn_i,
+                       a.core64, a.keys, a.vals, a.valid_count);
+    size_t tb = a.temp_bytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(a.temp, tb, a.keys, a.keys_s, a.vals, a.vals_s, (int)m, 0, 64, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(knn_edge_gather_kernel, dim3(g), dim3(256), 0, st, m, a.kq, a.knn_i, a.keys_s, a.vals_s,
+                       a.src_out, a.dst_out, a.w_out);
+    return hipGetLastError();
+}
 
 size_t sort_mst_temp_bytes(int m) {
     size_t a = 0, b = 0;

@@ -20,6 +20,7 @@ __all__ = [
     "mst_device",
     "parallel_tree_query",
     "build_kdtree",
+    "kruskal_mst_knn",
 ]
 
 
@@ -177,7 +178,18 @@ def compute_minimum_spanning_tree(
     if metric != "euclidean":
         raise NotImplementedError(f"metric={metric!r} is not on the device MST path (euclidean only)")
     if algorithm == "kruskal":
-        raise NotImplementedError("algorithm='kruskal' is not on the device MST path")
+        # hdbscan.py:469-478 -> kruskal.kruskal_mst_from_feature_matrix
+        if _has_cl:
+            raise NotImplementedError("cannot-link constraints are not on the device MST path")
+        if knn_k is None:
+            if sample_weights is not None:
+                raise NotImplementedError(
+                    "sample_weights with knn_k=None (full pairwise) is not supported. "
+                    "Use knn_k=<int> or algorithm='boruvka'."
+                )
+            raise NotImplementedError("algorithm='kruskal' with knn_k=None (dense float64 pairwise) is not on the "
+                                      "device MST path; pass knn_k")
+        return kruskal_mst_knn(_as_points(data), min_samples, knn_k, sample_weights)
     try:
         import torch
 
@@ -196,6 +208,41 @@ def compute_minimum_spanning_tree(
     w = None if sample_weights is None else np.asarray(sample_weights, dtype=np.float32)
     src, dst, wsq, core_sq, nbr = mst_arrays(X, min_samples, sample_weights=w)
     return _edges_f64(src, dst, wsq), nbr, np.sqrt(core_sq)
+
+
+def kruskal_mst_knn(X, min_samples=10, knn_k=10, sample_weights=None, device: int = 0, stats: dict | None = None):
+    """kruskal._kruskal_mst_knn (kruskal.py:950-1034) on the device: returns
+    (mst_edges float64[n-1, 3], neighbors int32[n, c], core_distances
+    float64[n]) with the reference's row order and +inf bridges."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, d = X.shape
+    ms = int(min_samples)
+    kk = int(knn_k)
+    w = None
+    kfc = 0
+    if sample_weights is not None:
+        mean_sw = float(np.mean(sample_weights))  # kruskal.py:987-988
+        kfc = int(2 * ms / mean_sw)
+        w = np.ascontiguousarray(sample_weights, dtype=np.float32)
+        if w.shape != (n,):
+            raise ValueError("sample_weights must have one value per point")
+        cols = max(kk, kfc)
+    elif ms > 1:
+        cols = ms
+    else:
+        cols = 1
+    mst = np.empty((max(n - 1, 0), 3), np.float64)
+    nbr = np.empty((n, cols), np.int32)
+    core = np.empty(n, np.float64)
+    st = _lib.FHStats()
+    ctx = _lib.context(device)
+    code = _lib.load().fh_mst_kruskal_knn(ctx, _lib.ptr(X), n, d, ms, kk, _lib.ptr(w) if w is not None else None,
+                                          kfc, _lib.ptr(mst) if n > 1 else None, _lib.ptr(nbr), cols, _lib.ptr(core),
+                                          ctypes.byref(st))
+    _lib.check(ctx, code)
+    if stats is not None:
+        stats.update(st.as_dict())
+    return mst, nbr, core
 
 
 def parallel_tree_query(data, queries, k=10, output_rdist=False, device: int = 0):

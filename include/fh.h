@@ -146,6 +146,25 @@ int64_t fh_tree_nnodes(int64_t n);
 int fh_build_kdtree(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int64_t *idx_array, int64_t *idx_start,
                     int64_t *idx_end, float *radius, uint8_t *is_leaf, float *node_bounds);
 
+/*
+ * Kruskal MST of the mutual-reachability kNN graph.
+ *   Replaces: kruskal.py:950-1034 (_kruskal_mst_knn, reached through
+ *             hdbscan.py:466-478 with algorithm="kruskal", knn_k=int) without
+ *             cannot-link constraints: tree, kNN of k_query columns, float64
+ *             weights max(core_i, core_j, d_ij), stable global sort, Kruskal
+ *             with union by rank and full path compression, +inf bridges
+ *             between the smallest ids of disconnected components.
+ * HOST pointers.  k_query = max(knn_k, min_samples) + 1 (min_samples > 1),
+ * knn_k + 1 (min_samples == 1) or max(knn_k, k_for_core) + 1 with sample
+ * weights, where k_for_core = int(2 * min_samples / mean(w)) is computed by
+ * the caller as the reference does (float(np.mean(w)), kruskal.py:987-989).
+ *   mst [(n-1)*3] float64 rows (u, v, w); nbr [n*nbr_cols] int32 with
+ *   nbr_cols = min_samples, 1, or k_query-1 (weighted); core [n] float64.
+ */
+int fh_mst_kruskal_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, int32_t knn_k,
+                       const float *sample_weights, int32_t k_for_core, double *mst, int32_t *nbr, int32_t nbr_cols,
+                       double *core, fh_stats *stats);
+
 /* ------------------------------------------------------------------ */
 /* Host label stage (the MST's consumer, kept on the host)             */
 /* ------------------------------------------------------------------ */

@@ -175,6 +175,32 @@ def tree_query(tree: Tree, queries, k: int, nthreads=None):
     return out_d, out_i
 
 
+def tree_query_sqrt(tree: Tree, queries, k: int, nthreads=None):
+    """parallel_tree_query(..., output_rdist=False) (numba_kdtree.py:
+    788-792): float32 square roots taken before deheap_sort."""
+    Q = np.ascontiguousarray(np.asarray(queries, dtype=np.float32))
+    nq = Q.shape[0]
+    out_d = np.empty((nq, k), np.float32)
+    out_i = np.empty((nq, k), np.int32)
+    lib().orc_knn_sqrt(*tree._args(), Q, nq, k, out_d, out_i, _threads(nthreads))
+    return out_d, out_i
+
+
+def sample_weight_core_distance(distances, neighbors, sample_weights, min_samples):
+    """boruvka.py:527-539: float64 running weight, first column reaching
+    min_samples; a -1 neighbour reads the last weight (numba wraps)."""
+    n, k = distances.shape
+    out = np.zeros(n, np.float32)
+    for i in range(n):
+        total = 0.0
+        j = 0
+        while total < min_samples and j < k:
+            total += float(sample_weights[neighbors[i, j]])
+            j += 1
+        out[i] = distances[i, j - 1]
+    return out
+
+
 def knn_brute(data, k: int, nthreads=None):
     """Brute-force kNN, same float32 rdist, rows sorted by (distance, index)."""
     X = np.ascontiguousarray(np.asarray(data, dtype=np.float32))
