@@ -38,6 +38,8 @@ EXPORTED_SYMBOLS = (
     "fh_cluster_layers",
     "fh_layer_cluster_tree",
     "fh_mst_kruskal_knn",
+    "fh_set_learned_radius",
+    "fh_mlp_step",
 )
 
 
@@ -59,6 +61,11 @@ class FHStats(ctypes.Structure):
         ("pad0", ctypes.c_int32),
         ("knn_tree_rows", ctypes.c_int64),
         ("exact_rows", ctypes.c_int64),
+        ("learned", ctypes.c_int32),
+        ("pad1", ctypes.c_int32),
+        ("ball_misses", ctypes.c_int64),
+        ("t_learn_ms", ctypes.c_double),
+        ("t_predict_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -167,6 +174,10 @@ def load() -> ctypes.CDLL:
         L.fh_mst_kruskal_knn.restype = ctypes.c_int
         L.fh_mst_kruskal_knn.argtypes = [_vp, _vp, _i64, _i32, _i32, _i32, _vp, _i32, _vp, _vp, _i32, _vp,
                                          ctypes.POINTER(FHStats)]
+        L.fh_set_learned_radius.restype = ctypes.c_int
+        L.fh_set_learned_radius.argtypes = [_vp, _i32]
+        L.fh_mlp_step.restype = ctypes.c_int
+        L.fh_mlp_step.argtypes = [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_float]
         L.fh_sort_mst_device.restype = ctypes.c_int
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L

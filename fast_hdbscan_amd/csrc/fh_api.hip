@@ -26,6 +26,7 @@
 #include "../../include/fh.h"
 #include "fh_brute.h"
 #include "fh_kernels.h"
+#include "fh_mlp.h"
 #include "fh_sort.h"
 #include "fh_tree.h"
 #include "kdtree_build.h"
@@ -80,6 +81,11 @@ struct fh_ctx {
     DevBuf<int> bx_cand, bx_cnt, bx_flags, bx_rowarg, bx_rows, bx_small;
     DevBuf<unsigned int> bx_nmax;
     int brute_min_dim = 17;  // d from which the MFMA scans replace the tree searches
+    // learned search radii
+    DevBuf<float> ml_feat, ml_w, ml_g, ml_m, ml_v, ml_mu, ml_sd, ml_scratch, ml_tgt, ml_pred, ml_bound, ml_stats;
+    DevBuf<int> ml_rows, ml_miss;
+    int learned_mode = 0;  // 0 off; 1 train on every call; 2 train once, reuse
+    int ml_F = 0;          // feature count of the trained model (0: none)
     // kNN-graph Kruskal
     DevBuf<uint64_t> kr_keys, kr_keys_s;
     DevBuf<int> kr_vals, kr_vals_s, kr_src, kr_dst;
@@ -409,6 +415,115 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
     return FH_OK;
 }
 
+// Learned-radius kNN.  A 1% subsample of tree positions gets the plain
+// exact query; its log core distances train the MLP (fh_mlp.hip) on KD-leaf
+// features; every position then runs the reference query inside the ball
+// the network predicts (plus a residual margin), and positions where the
+// ball could change the answer (not filled, or an equality the heap layout
+// decides) are queried again without it.  Output identical to the plain
+// kNN by construction.
+constexpr int kLearnStride = 100;  // training sample: one position in 100
+constexpr int kLearnSteps = 120;   // Adam steps per training
+constexpr int kLearnBatch = 1024;
+constexpr float kLearnRate = 3e-3f;
+constexpr float kLearnZ = 2.0f;    // margin in residual standard deviations
+
+int learned_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int *nb_pos, float *nb_d, fh_stats &S) {
+    hipStream_t st = ctx->stream;
+    const int F = 2 + 2 * d;
+    const int P = fh::mlp_param_count(F);
+    hipEvent_t e0, e1, e2;
+    FH_HIP(hipEventCreate(&e0));
+    FH_HIP(hipEventCreate(&e1));
+    FH_HIP(hipEventCreate(&e2));
+    FH_HIP(hipEventRecord(e0, st));
+    FH_TRY(ensure(ctx, ctx->ml_feat, (size_t)n * F));
+    FH_TRY(ensure(ctx, ctx->ml_w, P));
+    FH_TRY(ensure(ctx, ctx->ml_g, P));
+    FH_TRY(ensure(ctx, ctx->ml_m, P));
+    FH_TRY(ensure(ctx, ctx->ml_v, P));
+    FH_TRY(ensure(ctx, ctx->ml_mu, F));
+    FH_TRY(ensure(ctx, ctx->ml_sd, F));
+    FH_TRY(ensure(ctx, ctx->ml_pred, n));
+    FH_TRY(ensure(ctx, ctx->ml_bound, n));
+    FH_TRY(ensure(ctx, ctx->ml_stats, 2));
+    FH_TRY(ensure(ctx, ctx->ml_miss, n));
+    FH_TRY(ensure(ctx, ctx->bx_small, 4));
+    fh::MlpParams mp{ctx->ml_w.p, ctx->ml_g.p, ctx->ml_m.p, ctx->ml_v.p, ctx->ml_mu.p, ctx->ml_sd.p, F};
+    FH_HIP(fh::launch_mlp_features(n, d, levels, ctx->xp.p, ctx->range.p, ctx->box.p, ctx->ml_feat.p, st));
+    const bool train = ctx->learned_mode == 1 || ctx->ml_F != F;
+    // training sample: exact answers for one position in kLearnStride
+    const int ntrain = (int)std::max<int64_t>(1, n / kLearnStride);
+    std::vector<int> rows(ntrain);
+    for (int i = 0; i < ntrain; ++i) rows[i] = (int)std::min<int64_t>(n - 1, (int64_t)i * kLearnStride + kLearnStride / 2);
+    FH_TRY(ensure(ctx, ctx->ml_rows, ntrain));
+    FH_TRY(ensure(ctx, ctx->ml_tgt, ntrain));
+    FH_HIP(hipMemcpyAsync(ctx->ml_rows.p, rows.data(), sizeof(int) * ntrain, hipMemcpyHostToDevice, st));
+    FH_TRY(run_knn(ctx, d, levels, nullptr, nullptr, ntrain, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p,
+                   ctx->core_p.p, 1, nb_pos, nb_d, 0, ctx->ml_rows.p));
+    if (train) {
+        const int B = std::min(ntrain, kLearnBatch);
+        FH_TRY(ensure(ctx, ctx->ml_scratch, (size_t)B * (F + 4 * fh::kMlpHidden + 2)));
+        FH_HIP(fh::launch_mlp_log_targets(ntrain, ctx->ml_rows.p, ctx->core_p.p, ctx->ml_tgt.p, st));
+        FH_HIP(fh::launch_mlp_standardise(ntrain, F, ctx->ml_feat.p, ctx->ml_rows.p, ctx->ml_mu.p, ctx->ml_sd.p, st));
+        FH_HIP(fh::launch_mlp_init(mp, 12345u, st));
+        for (int step = 1; step <= kLearnSteps; ++step)
+            FH_HIP(fh::launch_mlp_train_step(mp, ntrain, B, (int)(((int64_t)(step - 1) * B) % ntrain),
+                                             ctx->ml_feat.p, ctx->ml_rows.p, ctx->ml_tgt.p, ctx->ml_scratch.p,
+                                             kLearnRate, step, st));
+        // residual margin from the training rows
+        FH_HIP(fh::launch_mlp_predict(mp, ntrain, ctx->ml_feat.p, ctx->ml_rows.p, ctx->ml_pred.p, st));
+        FH_HIP(fh::launch_mlp_residual(ntrain, ctx->ml_pred.p, ctx->ml_tgt.p, ctx->ml_stats.p, st));
+        ctx->ml_F = F;
+    }
+    FH_HIP(hipEventRecord(e1, st));
+    FH_HIP(fh::launch_mlp_predict(mp, n, ctx->ml_feat.p, nullptr, ctx->ml_pred.p, st));
+    FH_HIP(fh::launch_mlp_bounds(n, ctx->ml_pred.p, ctx->ml_stats.p, kLearnZ, ctx->ml_bound.p, st));
+    FH_HIP(hipEventRecord(e2, st));
+    // ball-bounded queries, then the plain query where the ball may matter
+    FH_HIP(hipMemsetAsync(ctx->bx_small.p, 0, sizeof(int), st));
+    {
+        fh::KnnArgs a{};
+        a.xp = ctx->xp.p;
+        a.range = ctx->range.p;
+        a.box = ctx->box.p;
+        a.perm = ctx->perm.p;
+        a.queries = ctx->xp.p;
+        a.query_perm = ctx->perm.p;
+        a.nq = n;
+        a.d = d;
+        a.k = kq;
+        a.levels = levels;
+        a.out_d = ctx->knn_d.p;
+        a.out_i = ctx->knn_i.p;
+        a.core_o = ctx->core_o.p;
+        a.core_p = ctx->core_p.p;
+        a.core_from_knn = 1;
+        a.block = knn_block(kq);
+        a.nb_pos = nb_pos;
+        a.nb_d = nb_d;
+        a.bound_sq = ctx->ml_bound.p;
+        a.miss_list = ctx->ml_miss.p;
+        a.miss_count = ctx->bx_small.p;
+        if (kq > 128) return fail(ctx, FH_EUNSUPPORTED, "learned radii need k_query <= 128");
+        FH_HIP(fh::launch_knn(a, st));
+    }
+    FH_HIP(hipMemcpyAsync(ctx->h_small, ctx->bx_small.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    const int misses = ctx->h_small[0];
+    if (misses > 0)
+        FH_TRY(run_knn(ctx, d, levels, nullptr, nullptr, misses, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p,
+                       ctx->core_p.p, 1, nb_pos, nb_d, 0, ctx->ml_miss.p));
+    S.learned = ctx->learned_mode;
+    S.ball_misses = misses;
+    S.t_learn_ms = event_ms(e0, e1);
+    S.t_predict_ms = event_ms(e1, e2);
+    FH_HIP(hipEventDestroy(e0));
+    FH_HIP(hipEventDestroy(e1));
+    FH_HIP(hipEventDestroy(e2));
+    return FH_OK;
+}
+
 // The MST proper.  Xd: the points in HBM (original order).  weights_d:
 // per-point sample weights in HBM or nullptr; kq_w: the kNN width for the
 // weighted branch.  Outputs are device pointers.
@@ -479,6 +594,8 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     fh::BruteScanArgs bsa{};
     if (brute) {
         FH_TRY(brute_knn(ctx, n, d, levels, kq, ms > 1 ? 1 : 0, bsa, S));
+    } else if (ctx->learned_mode != 0 && !weighted && ms > 1 && d <= 8 && kq <= 128 && n >= 1000) {
+        FH_TRY(learned_knn(ctx, n, d, levels, kq, seed ? ctx->nb_pos.p : nullptr, seed ? ctx->nb_d.p : nullptr, S));
     } else {
         FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p,
                        ctx->core_p.p, (ms > 1 && !weighted) ? 1 : 0, seed ? ctx->nb_pos.p : nullptr,
@@ -680,6 +797,7 @@ fh_ctx *fh_create(int device) {
     c->device = device;
     if (const char *m = std::getenv("FH_SEARCH_MODE")) c->search_mode = std::atoi(m);
     if (const char *m = std::getenv("FH_HIGH_DIM_CAP")) c->brute_cap = std::max(16, std::atoi(m));
+    if (const char *m = std::getenv("FH_LEARNED_RADIUS")) c->learned_mode = std::atoi(m);
     if (hipSetDevice(device) != hipSuccess) {
         delete c;
         return nullptr;
@@ -724,6 +842,9 @@ void fh_destroy(fh_ctx *c) {
     release(c->bx_xc); release(c->bx_nrm); release(c->bx_thr); release(c->bx_rowmin); release(c->bx_cand);
     release(c->bx_cnt); release(c->bx_flags); release(c->bx_rowarg); release(c->bx_rows); release(c->bx_small);
     release(c->bx_nmax);
+    release(c->ml_feat); release(c->ml_w); release(c->ml_g); release(c->ml_m); release(c->ml_v); release(c->ml_mu);
+    release(c->ml_sd); release(c->ml_scratch); release(c->ml_tgt); release(c->ml_pred); release(c->ml_bound);
+    release(c->ml_stats); release(c->ml_rows); release(c->ml_miss);
     release(c->kr_keys); release(c->kr_keys_s); release(c->kr_vals); release(c->kr_vals_s); release(c->kr_src);
     release(c->kr_dst); release(c->kr_w); release(c->kr_core);
     if (c->h_small) e = hipHostFree(c->h_small);
@@ -914,6 +1035,52 @@ int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t
     a.temp_bytes = ctx->stemp.cap;
     FH_HIP(fh::sort_mst(a, ctx->stream));
     FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
+int fh_set_learned_radius(fh_ctx *ctx, int32_t mode) {
+    if (!ctx) return FH_EINVAL;
+    if (mode < 0 || mode > 2) return fail(ctx, FH_EINVAL, "learned-radius mode must be 0, 1 or 2");
+    ctx->learned_mode = mode;
+    if (mode != 2) ctx->ml_F = 0;
+    return FH_OK;
+}
+
+int fh_mlp_step(fh_ctx *ctx, int32_t F, int32_t B, const float *x, const float *t, float *params, float *grad,
+                float *pred, float lr) {
+    if (!ctx) return FH_EINVAL;
+    if (F < 1 || F > fh::kMlpMaxF || B < 1 || !x || !t || !params) return fail(ctx, FH_EINVAL, "bad MLP step arguments");
+    FH_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int P = fh::mlp_param_count(F);
+    FH_TRY(ensure(ctx, ctx->ml_feat, (size_t)B * F));
+    FH_TRY(ensure(ctx, ctx->ml_tgt, B));
+    FH_TRY(ensure(ctx, ctx->ml_w, P));
+    FH_TRY(ensure(ctx, ctx->ml_g, P));
+    FH_TRY(ensure(ctx, ctx->ml_m, P));
+    FH_TRY(ensure(ctx, ctx->ml_v, P));
+    FH_TRY(ensure(ctx, ctx->ml_mu, F));
+    FH_TRY(ensure(ctx, ctx->ml_sd, F));
+    FH_TRY(ensure(ctx, ctx->ml_scratch, (size_t)B * (F + 4 * fh::kMlpHidden + 2)));
+    std::vector<float> zero(F, 0.0f), one(F, 1.0f), zp(P, 0.0f);
+    FH_HIP(hipMemcpyAsync(ctx->ml_feat.p, x, sizeof(float) * B * F, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->ml_tgt.p, t, sizeof(float) * B, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->ml_w.p, params, sizeof(float) * P, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->ml_m.p, zp.data(), sizeof(float) * P, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->ml_v.p, zp.data(), sizeof(float) * P, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->ml_mu.p, zero.data(), sizeof(float) * F, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->ml_sd.p, one.data(), sizeof(float) * F, hipMemcpyHostToDevice, st));
+    fh::MlpParams mp{ctx->ml_w.p, ctx->ml_g.p, ctx->ml_m.p, ctx->ml_v.p, ctx->ml_mu.p, ctx->ml_sd.p, F};
+    FH_HIP(fh::launch_mlp_train_step(mp, B, B, 0, ctx->ml_feat.p, nullptr, ctx->ml_tgt.p, ctx->ml_scratch.p, lr, 1, st));
+    if (grad) FH_HIP(hipMemcpyAsync(grad, ctx->ml_g.p, sizeof(float) * P, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(params, ctx->ml_w.p, sizeof(float) * P, hipMemcpyDeviceToHost, st));
+    if (pred) {
+        // Y follows X, H1, H2, D1, D2, D3 in the scratch block
+        const size_t yoff = (size_t)B * (F + 4 * fh::kMlpHidden + 1);
+        FH_HIP(hipMemcpyAsync(pred, ctx->ml_scratch.p + yoff, sizeof(float) * B, hipMemcpyDeviceToHost, st));
+    }
+    FH_HIP(hipStreamSynchronize(st));
+    ctx->ml_F = 0;  // the learned-radius model, if any, was overwritten
     return FH_OK;
 }
 

@@ -30,6 +30,12 @@ struct KnnArgs {
     float *nb_d;            // optional: [nq, k-1] their squared distances
     int sqrt_dist;          // 1: float32 square roots before the sort (output_rdist=False)
     const int *query_pos;   // optional: the nq queries are these tree positions of xp
+    // optional ball bound per query (learned radius, squared): rows whose
+    // result could depend on it (heap not filled inside the ball, or a tie
+    // at an eviction, a rejection or in the output) are listed instead
+    const float *bound_sq;
+    int *miss_list;
+    int *miss_count;
 };
 
 struct QueryArgs {

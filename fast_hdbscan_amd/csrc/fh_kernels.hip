@@ -100,6 +100,8 @@ struct KnnVisitor {
     const int2 *__restrict__ range;
     LdsHeap h;
     float hmax;
+    float bound;  // +inf: the reference's unbounded query
+    int tie;      // bounded query only: an equality that the heap layout decides
     int d;
 
     __device__ __forceinline__ bool enter(int, float lb) const { return !(lb > hmax); }
@@ -107,11 +109,19 @@ struct KnnVisitor {
     __device__ __forceinline__ void leaf(int node) {
         const int2 r = range[node];
         const int dd = D > 0 ? D : d;
+        const bool bounded = bound < kInf;
         for (int i = r.x; i < r.y; ++i) {
             const float dist = rdist<D>(x.p(), xp + (size_t)i * dd, d);
             if (dist < hmax) {
+                if (bounded && h.P(0) < kInf && h.k > 1) {
+                    float c = h.P(1);
+                    if (h.k > 2) c = h.P(2) > c ? h.P(2) : c;
+                    if (c == h.P(0)) tie = 1;  // which equal element leaves depends on the layout
+                }
                 h.push(dist, i);
-                hmax = h.P(0);
+                hmax = h.P(0) < bound ? h.P(0) : bound;
+            } else if (bounded && dist == hmax && hmax == h.P(0)) {
+                tie = 1;
             }
         }
     }
@@ -149,14 +159,28 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     v.xp = a.xp;
     v.range = a.range;
     v.h = h;
-    v.hmax = kInf;
+    v.bound = a.bound_sq ? a.bound_sq[qp] : kInf;
+    v.hmax = v.bound;
+    v.tie = 0;
     v.d = a.d;
     walk<D>(a.box, a.levels, a.d, v);
-    if (a.sqrt_dist) {
+    if (a.bound_sq) {
+        bool miss = !(h.P(0) < kInf) || v.tie;
+        if (!miss) {
+            h.sort();
+            for (int j = 0; j + 1 < k; ++j)
+                if (h.P(j) == h.P(j + 1)) miss = true;
+        }
+        if (miss) {
+            const int slot = atomicAdd(a.miss_count, 1);
+            a.miss_list[slot] = (int)qp;
+            return;
+        }
+    } else if (a.sqrt_dist) {
         // numba_kdtree.py:792: np.sqrt of the rows, then deheap_sort
         for (int j = 0; j < k; ++j) h.P(j) = sqrtf(h.P(j));
     }
-    h.sort();
+    if (!a.bound_sq) h.sort();
 
     // Output row: the original index of the query for the self-query, the
     // query's own row otherwise.

@@ -63,6 +63,11 @@ typedef struct {
     int32_t pad0;
     int64_t knn_tree_rows;    /* high_dim: kNN rows answered by the tree search */
     int64_t exact_rows;       /* high_dim: rows re-scanned exactly, summed     */
+    int32_t learned;          /* learned-radius mode used for the kNN (0: off) */
+    int32_t pad1;
+    int64_t ball_misses;      /* learned: rows queried again without the ball  */
+    double t_learn_ms;        /* learned: features, sample kNN, training       */
+    double t_predict_ms;      /* learned: inference + radii                    */
 } fh_stats;
 
 /* Create a context on HIP device `device`.  Returns NULL on failure (no
@@ -164,6 +169,34 @@ int fh_build_kdtree(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int64_t *
 int fh_mst_kruskal_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples, int32_t knn_k,
                        const float *sample_weights, int32_t k_for_core, double *mst, int32_t *nbr, int32_t nbr_cols,
                        double *core, fh_stats *stats);
+
+/* ------------------------------------------------------------------ */
+/* Learned search radii (no counterpart in the reference)              */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Core distances through learned ball bounds: a 3-layer FP32 MLP
+ * (F = 2 + 2d inputs from KD-leaf statistics, 64, 64, 1) trained on the
+ * device on a 1% sample's exact log core distances predicts a radius per
+ * point; the reference's kNN query runs inside that ball and is repeated
+ * without it wherever the ball could change the answer, so the outputs of
+ * fh_mst_euclidean* are unchanged.  mode 0: off (default); 1: train on
+ * every call; 2: train once per context and reuse.  Applies to d <= 8,
+ * min_samples > 1, no sample weights.  FH_LEARNED_RADIUS sets the default.
+ */
+int fh_set_learned_radius(fh_ctx *ctx, int32_t mode);
+
+/*
+ * One training step of that network on a given batch, for verification:
+ * forward, backward (loss = mean squared error), gradient summed over the
+ * batch and one Adam step (fresh moments, step 1, learning rate lr; lr = 0
+ * skips the update).  HOST pointers: x [B*F] (used as given), t [B],
+ * params [P] in/out with P = 64F + 64 + 64*64 + 64 + 64 + 1 laid out
+ * W1[64][F], b1, W2[64][64], b2, W3[64], b3; grad [P] and pred [B] may be
+ * NULL.
+ */
+int fh_mlp_step(fh_ctx *ctx, int32_t F, int32_t B, const float *x, const float *t, float *params, float *grad,
+                float *pred, float lr);
 
 /* ------------------------------------------------------------------ */
 /* Host label stage (the MST's consumer, kept on the host)             */
