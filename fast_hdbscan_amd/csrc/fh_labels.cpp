@@ -37,6 +37,8 @@ struct Merge {
     int32_t left, right;            // child node ids (points < n, merges >= n)
     int32_t left_size, right_size;  // points under each child
     double delta;                   // merge distance
+    int32_t relabel;                // condense_tree's relabel of this node (same cache line)
+    int32_t pad;
 };
 
 struct Hierarchy {
@@ -163,18 +165,23 @@ struct Condensed {
 // cluster_trees.py:192-197) or exp(-d) (plscan_lambda_value, 200-202).
 enum LambdaMethod { kLambdaHdbscan = 0, kLambdaPlscan = 1 };
 
-void condense(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, int lambda_method = kLambdaHdbscan) {
+void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_method = kLambdaHdbscan) {
     const int64_t m = n - 1;
     const int32_t root = static_cast<int32_t>(2 * m);
     const int32_t num_points = static_cast<int32_t>(n);
     int64_t next_label = num_points + 1;
     // relabel is written for a node before the node is processed; only the
-    // root needs a value up front.  Buffers persist between calls.
-    static thread_local std::vector<int32_t> relabel;
-    if (relabel.size() < static_cast<size_t>(root + 1)) relabel.resize(root + 1);
-    relabel[root] = num_points;
-    // a -1 child (only when min_samples >= n) indexes like numba: from the end
-    auto RL = [&](int32_t x) -> int32_t & { return relabel[x < 0 ? x + root + 1 : x]; };
+    // root needs a value up front.  It lives in the node's merge row, which
+    // the walk reads anyway.  Points are never processed: their relabel is
+    // not kept.
+    Merge *mrows = h.rows.data();
+    mrows[root - num_points].relabel = num_points;
+    // a -1 child (only when min_samples >= n) indexes like numba: from the
+    // end, i.e. the root's entry
+    auto RL = [&](int32_t x, int32_t v) {
+        if (x >= num_points) mrows[x - num_points].relabel = v;
+        else if (x < 0) mrows[x + root + 1 - num_points].relabel = v;
+    };
     // at most one row per node below the root
     const size_t cap = static_cast<size_t>(root);
     if (c.parent.size() < cap) {
@@ -232,12 +239,11 @@ void condense(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, int lamb
     for (size_t qi = 0; qi < q.size(); ++qi) {
         if (qi + kAhead < q.size()) {
             const int32_t ahead = q[qi + kAhead];
-            prefetch(&relabel[ahead]);
             prefetch(&rows[ahead - num_points]);
         }
         const int32_t node = q[qi];
-        const int64_t parent_node = relabel[node];
         const Merge &r = rows[node - num_points];
+        const int64_t parent_node = r.relabel;
         const int32_t left = r.left, right = r.right;
         const double d = r.delta;
         const double lam = lambda_method == kLambdaHdbscan ? (d > 0.0 ? 1.0 / d : kInfD) : std::exp(-d);
@@ -245,25 +251,25 @@ void condense(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, int lamb
         const float lc = static_cast<float>(r.left_size), rc = static_cast<float>(r.right_size);
         const double dl = lc, dr = rc;
         if (left < num_points && dr >= dm) {
-            RL(right) = static_cast<int32_t>(parent_node);
+            RL(right, static_cast<int32_t>(parent_node));
             row(parent_node, left, lam, 1.0f);
             push(right);
         } else if (dl < dm && dr >= dm) {
-            RL(right) = static_cast<int32_t>(parent_node);
+            RL(right, static_cast<int32_t>(parent_node));
             eliminate(left, parent_node, lam);
             push(right);
         } else if (dl >= dm && dr < dm) {
-            RL(left) = static_cast<int32_t>(parent_node);
+            RL(left, static_cast<int32_t>(parent_node));
             eliminate(right, parent_node, lam);
             push(left);
         } else if (dl < dm && dr < dm) {
             eliminate(left, parent_node, lam);
             eliminate(right, parent_node, lam);
         } else {
-            RL(left) = static_cast<int32_t>(next_label);
+            RL(left, static_cast<int32_t>(next_label));
             row(parent_node, next_label, lam, lc);
             ++next_label;
-            RL(right) = static_cast<int32_t>(next_label);
+            RL(right, static_cast<int32_t>(next_label));
             row(parent_node, next_label, lam, rc);
             ++next_label;
             push(left);
@@ -820,7 +826,7 @@ std::vector<int64_t> select_diverse_peaks(const std::vector<int64_t> &peaks, con
 
 // _binary_search_for_n_clusters (layer_clusters.py:71-153), hdbscan
 // lambdas; round() is round-half-even as in Python and numba.
-void binary_search_for_n_clusters(const Hierarchy &h, int64_t n, int64_t approx, int64_t *labels, float *prob) {
+void binary_search_for_n_clusters(Hierarchy &h, int64_t n, int64_t approx, int64_t *labels, float *prob) {
     int64_t lo = 2, hi = n / 2;
     Condensed t;
     auto count = [&](int64_t mcs) {
