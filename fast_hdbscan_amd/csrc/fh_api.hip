@@ -37,7 +37,7 @@ namespace {
 
 constexpr int64_t kLeafSize = 40;  // numba_kdtree.py:478 default, used by hdbscan.py:466
 constexpr int kCertDepth = 4;      // certification nodes sit this many levels above the leaves
-constexpr int kSampleMask = 63;    // one sampled search per 64 tree positions
+constexpr int kSampleMask = 255;   // one sampled search per 256 tree positions
 
 template <class T>
 struct DevBuf {
@@ -648,6 +648,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     qa.levels = levels;
     qa.pass = 0;
     qa.sample_mask = kSampleMask;
+    if (const char *m = std::getenv("FH_SAMPLE_MASK")) qa.sample_mask = std::atoi(m);
     qa.cert_of_pos = pruned ? ctx->cert_of_pos.p : nullptr;
     qa.cert = pruned ? ctx->cert.p : nullptr;
     fh::CertArgs ca{};

@@ -327,17 +327,24 @@ __global__ void seed_bounds_kernel(int64_t n, int kn, const int *__restrict__ nb
 
 // Which certification node (in-level index at `level`) holds each position,
 // and the smallest core distance inside each such node.
-__global__ void cert_setup_kernel(int level, const int2 *__restrict__ range, const float *__restrict__ core_p,
-                                  int *cert_of_pos, float *mincore) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (1 << level)) return;
+__global__ __launch_bounds__(64) void cert_setup_kernel(int level, const int2 *__restrict__ range,
+                                                        const float *__restrict__ core_p, int *cert_of_pos,
+                                                        float *mincore) {
+    __shared__ float sm[64];
+    const int k = blockIdx.x;
     const int2 r = range[(1 << level) - 1 + k];
     float m = kInf;
-    for (int p = r.x; p < r.y; ++p) {
+    for (int p = r.x + threadIdx.x; p < r.y; p += 64) {
         cert_of_pos[p] = k;
         m = core_p[p] < m ? core_p[p] : m;
     }
-    mincore[k] = m;
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) {
+        if (threadIdx.x < o && sm[threadIdx.x + o] < sm[threadIdx.x]) sm[threadIdx.x] = sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mincore[k] = sm[0];
 }
 
 // Lower bound of the squared distance between two boxes, in the same
@@ -358,75 +365,87 @@ __device__ __forceinline__ float box_box_lb(const float *__restrict__ a, const f
     return r;
 }
 
-// One lane per certification node P of a single component C: walk the tree
-// with the box-to-box bound and fail as soon as some point j outside C could
-// give a weight max(d(box P, x_j), core_j, min core of P) <= B_C.  A
-// certified node's points cannot reach B_C (nor, therefore, the component
-// minimum), so the main search skips them.
+// One wave per certification node P of a single component C: the tree is
+// split into the subtrees 6 levels below the root, one per lane, and each
+// lane walks its subtree with the box-to-box bound; the node fails as soon
+// as some point j outside C could give a weight max(d(box P, x_j), core_j,
+// min core of P) <= B_C (an existence test, so the split does not change
+// the answer).  A certified node's points cannot reach B_C (nor, therefore,
+// the component minimum), so the main search skips them.
 template <int D>
-__global__ __launch_bounds__(256) void cert_kernel(CertArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (1 << a.level)) return;
+__global__ __launch_bounds__(64) void cert_kernel(CertArgs a) {
+    __shared__ int fail_s;
+    const int k = blockIdx.x;
+    const int lane = threadIdx.x;
     const int dd = D > 0 ? D : a.d;
     const int stride = 2 * dd;
     const int P = (1 << a.level) - 1 + k;
     const int C = a.node_comp[P];
-    unsigned char ok = 0;
-    if (C >= 0) {
-        const float B = ld_relaxed(a.bound + C);
-        const float mc = a.mincore[k];
-        if (mc > B) {
-            ok = 1;
-        } else if (B < kInf) {
-            float pb[2 * (D > 0 ? D : 1)];
-            const float *boxP = a.box + (size_t)P * stride;
-            if constexpr (D > 0) {
+    if (C < 0) {
+        if (lane == 0) a.cert[k] = 0;
+        return;
+    }
+    const float B = ld_relaxed(a.bound + C);
+    const float mc = a.mincore[k];
+    if (mc > B || !(B < kInf)) {
+        if (lane == 0) a.cert[k] = mc > B ? 1 : 0;
+        return;
+    }
+    if (lane == 0) fail_s = 0;
+    __syncthreads();
+    float pb[2 * (D > 0 ? D : 1)];
+    const float *boxP = a.box + (size_t)P * stride;
+    if constexpr (D > 0) {
 #pragma unroll
-                for (int j = 0; j < 2 * D; ++j) pb[j] = boxP[j];
-            }
-            const float *bp = D > 0 ? pb : boxP;
-            bool fail = false;
-            int node = 0, depth = 0;
-            bool entering = true;
-            for (;;) {
-                if (entering && !fail) {
-                    const float lb = box_box_lb<D>(bp, a.box + (size_t)node * stride, a.d);
-                    if (!(lb > B) && a.node_comp[node] != C) {
-                        if (depth == a.levels - 1) {
-                            const int2 r = a.range[node];
-                            for (int j = r.x; j < r.y; ++j) {
-                                if (a.comp_p[j] == C) continue;
-                                const float cj = a.core_p[j];
-                                if (!(cj <= B)) continue;
-                                float w = box_lb<D>(a.xp + (size_t)j * dd, bp, a.d);
-                                w = w > cj ? w : cj;
-                                w = w > mc ? w : mc;
-                                if (!(w > B)) {
-                                    fail = true;
-                                    break;
-                                }
+        for (int j = 0; j < 2 * D; ++j) pb[j] = boxP[j];
+    }
+    const float *bp = D > 0 ? pb : boxP;
+    const int sd = a.levels - 1 < 6 ? a.levels - 1 : 6;
+    const int nsub = 1 << sd;
+    if (lane < nsub) {
+        const int root = nsub - 1 + lane;
+        int node = root, depth = sd;
+        bool entering = true, fail = false;
+        for (;;) {
+            if (entering) {
+                if (*(volatile int *)&fail_s) break;
+                const float lb = box_box_lb<D>(bp, a.box + (size_t)node * stride, a.d);
+                if (!(lb > B) && a.node_comp[node] != C) {
+                    if (depth == a.levels - 1) {
+                        const int2 r = a.range[node];
+                        for (int j = r.x; j < r.y; ++j) {
+                            if (a.comp_p[j] == C) continue;
+                            const float cj = a.core_p[j];
+                            if (!(cj <= B)) continue;
+                            float w = box_lb<D>(a.xp + (size_t)j * dd, bp, a.d);
+                            w = w > cj ? w : cj;
+                            w = w > mc ? w : mc;
+                            if (!(w > B)) {
+                                fail = true;
+                                break;
                             }
-                        } else {
-                            node = 2 * node + 1;
-                            ++depth;
-                            continue;
                         }
+                    } else {
+                        node = 2 * node + 1;
+                        ++depth;
+                        continue;
                     }
                 }
-                if (fail || depth == 0) break;
-                if (node & 1) {  // left child done: go to its sibling
-                    node += 1;
-                    entering = true;
-                } else {
-                    node = (node - 1) >> 1;
-                    --depth;
-                    entering = false;
-                }
             }
-            ok = fail ? 0 : 1;
+            if (fail || node == root) break;
+            if (node & 1) {  // left child done: go to its sibling
+                node += 1;
+                entering = true;
+            } else {
+                node = (node - 1) >> 1;
+                --depth;
+                entering = false;
+            }
         }
+        if (fail) fail_s = 1;
     }
-    a.cert[k] = ok;
+    __syncthreads();
+    if (lane == 0) a.cert[k] = fail_s ? 0 : 1;
 }
 
 // sample_weight_core_distance (boruvka.py:527-539): the first sorted
@@ -872,7 +891,7 @@ hipError_t launch_knn(const KnnArgs &a, hipStream_t st) { FH_DISPATCH_D(a.d, lau
 
 template <int D>
 static hipError_t launch_cert_t(const CertArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL(cert_kernel<D>, dim3(blocks(int64_t(1) << a.level, 64)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(cert_kernel<D>, dim3((unsigned)(1u << a.level)), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 
@@ -880,8 +899,8 @@ hipError_t launch_cert(const CertArgs &a, hipStream_t st) { FH_DISPATCH_D(a.d, l
 
 hipError_t launch_cert_setup(int level, const int2 *range, const float *core_p, int *cert_of_pos, float *mincore,
                              hipStream_t st) {
-    hipLaunchKernelGGL(cert_setup_kernel, dim3(blocks(int64_t(1) << level, 64)), dim3(64), 0, st, level, range,
-                       core_p, cert_of_pos, mincore);
+    hipLaunchKernelGGL(cert_setup_kernel, dim3((unsigned)(1u << level)), dim3(64), 0, st, level, range, core_p,
+                       cert_of_pos, mincore);
     return hipGetLastError();
 }
 
