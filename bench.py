@@ -51,6 +51,8 @@ def _parse():
     p.add_argument("--cpu-threads", type=int, default=1)
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 256 MiB device copy (calibrates HBM byte counters) and exit")
+    p.add_argument("--no-forms", action="store_true",
+                   help="skip the learned / high-dimensional / 10M x 3-D / PLSCAN-sweep side measurements")
     return p.parse_args()
 
 
@@ -160,6 +162,8 @@ def main():
                       f"in Python ({t3 - t2:.2f} s)",
         }
 
+    forms = None if args.no_forms else side_forms(Xd, ms)
+
     line = {
         "metric": "points/s to exact MR-MST+labels at 2Mx2D",
         "value": value,
@@ -202,8 +206,101 @@ def main():
         "rounds": rounds,
         "clusters": n_clusters,
         "noise_points": noise,
+        "forms": forms,
     }
     print(json.dumps(line))
+
+
+def side_forms(Xd, ms):
+    """The other forms of the path, each run once after the timed steps
+    (not part of `value`): learned search radii on the same 2M x 2-D batch,
+    the MFMA high-dimensional path at 1M x 64, the 10M x 3-D mixture and a
+    PLSCAN min_samples sweep on 500k x 2-D."""
+    import numpy as np
+    import torch
+
+    from fast_hdbscan_amd import _lib, workloads
+    from fast_hdbscan_amd.hdbscan import mst_device
+
+    out = {}
+    ctx = _lib.context(0)
+    L = _lib.load()
+
+    def mst_ms(X, m, reps=3, **kw):
+        best, st_best = None, None
+        for _ in range(reps):
+            st = {}
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            mst_device(X, m, stats=st, **kw)
+            torch.cuda.synchronize()
+            el = 1e3 * (time.perf_counter() - t)
+            if best is None or el < best:
+                best, st_best = el, st
+        return best, st_best
+
+    # learned search radii: plain, train-every-call, train-once-reuse
+    plain_ms, pst = mst_ms(Xd, ms)
+    res = {"plain": {"mst_ms": plain_ms, "knn_ms": pst["t_knn_ms"]}}
+    for mode, name in ((1, "train_each_call"), (2, "trained_once")):
+        _lib.check(ctx, L.fh_set_learned_radius(ctx, mode))
+        if mode == 2:
+            mst_ms(Xd, ms, reps=1)  # the one training
+        t, st = mst_ms(Xd, ms)
+        res[name] = {"mst_ms": t, "knn_ms": st["t_knn_ms"], "train_ms": st["t_learn_ms"],
+                     "predict_ms": st["t_predict_ms"],
+                     "ball_hit_rate": 1.0 - st["ball_misses"] / Xd.shape[0],
+                     "speedup_vs_plain": plain_ms / t}
+    _lib.check(ctx, L.fh_set_learned_radius(ctx, 0))
+    res["note"] = ("exact by construction (ball misses and ties re-run the plain query); the plain kNN already "
+                   "prunes after its first leaf, so the ball bound does not pay for itself here")
+    out["learned"] = res
+
+    # high-dimensional MFMA path: 1M x 64, 100 Gaussians, min_samples 10
+    n5, d5 = 1_000_000, 64
+    X5 = torch.from_numpy(workloads.gaussian_highdim(n5, d5, 100, seed=3)).cuda()
+    t5, st5 = mst_ms(X5, 10, reps=1)
+    npad = (n5 + 127) // 128 * 128
+    scan_ms = st5["t_query_main_ms"] / max(st5["rounds"], 1)
+    flops = 2.0 * d5 * float(npad) * float(npad)
+    out["high_dim"] = {
+        "workload": "1M x 64, 100 Gaussians (means N(0,4I), within N(0,I)), seed 3, min_samples 10",
+        "mst_ms": t5, "points_per_s": n5 / (t5 * 1e-3), "rounds": st5["rounds"],
+        "knn_ms": st5["t_knn_ms"], "knn_tree_rows": st5["knn_tree_rows"], "exact_rows": st5["exact_rows"],
+        "roofline": {"bound": "mfma", "kernel": "brute_scan_kernel<16, 1>", "achieved": flops / (scan_ms * 1e-3) / 1e12,
+                     "peak": 157.3, "unit": "TFLOP/s", "frac": flops / (scan_ms * 1e-3) / 1e12 / 157.3,
+                     "avg_launch_ms": scan_ms, "alg_flops_per_launch": flops},
+    }
+    del X5
+
+    # 10M x 3-D mixture, min_samples 15
+    X3 = torch.from_numpy(workloads.scale_3d()).cuda()
+    t3, st3 = mst_ms(X3, 15, reps=2)
+    out["scale_3d"] = {"workload": "10M x 3-D, 200 anisotropic Gaussians + 5% noise, seed 2, min_samples 15",
+                       "mst_ms": t3, "points_per_s": X3.shape[0] / (t3 * 1e-3), "rounds": st3["rounds"]}
+    del X3
+
+    # PLSCAN sweep: min_samples 2..64 on 500k x 2-D, layers for each
+    from fast_hdbscan_amd.hdbscan import _edges_f64
+    from fast_hdbscan_amd.layers import layers_from_mst
+
+    Xs = torch.from_numpy(workloads.anisotropic_mixture(500_000, 2, 50, 0.1, seed=4)).cuda()
+    t0 = time.perf_counter()
+    t_mst = 0.0
+    n_layers = []
+    for m in range(2, 65):
+        a = time.perf_counter()
+        src, dst, w, core, _ = mst_device(Xs, m)
+        e = _edges_f64(src.cpu().numpy(), dst.cpu().numpy(), w.cpu().numpy())
+        c = np.sqrt(core.cpu().numpy())
+        t_mst += time.perf_counter() - a
+        n_layers.append(len(layers_from_mst(e, c, Xs.shape[0], base_min_cluster_size=5)[0]))
+    total = time.perf_counter() - t0
+    out["plscan_sweep"] = {"workload": "500k x 2-D, 50 anisotropic Gaussians + 10% noise, seed 4; min_samples 2..64, "
+                                       "base_min_cluster_size 5",
+                           "total_s": total, "mst_and_copy_s": t_mst, "layer_stage_s": total - t_mst,
+                           "layers_per_min_samples": n_layers}
+    return out
 
 
 if __name__ == "__main__":
