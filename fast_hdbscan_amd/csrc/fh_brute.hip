@@ -60,19 +60,22 @@ int brute_padded_dim(int d) {
 
 // ------------------------------------------------------------- packing ----
 
-// Centre on the root box's midpoint, zero-pad to dp dimensions and group
-// the dimensions by (k mod 4) so that each MFMA lane reads a contiguous run.
+// Centre on the root box's midpoint, zero-pad to dp dimensions and lay the
+// rows out in tiles of 128: inside a tile, dimension k of row r sits at
+// [k / 4][r][k % 4], which is both the order the LDS tile is read in (one
+// wave reads 64 consecutive floats per MFMA operand) and a straight copy.
 __global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const float *__restrict__ xp,
                                    const float *__restrict__ root_box, float *__restrict__ xc, float *__restrict__ nrm,
                                    unsigned int *nrm_max_bits) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npad) return;
-    const int q4 = dp / 4;
+    // tile layout: 128 rows per tile, inside a tile [k / 4][row % 128][k % 4]
+    float *tile = xc + (p / 128) * 128 * dp + (p % 128) * 4;
     float s = 0.0f;
     for (int k = 0; k < dp; ++k) {
         float v = 0.0f;
         if (p < n && k < d) v = xp[p * d + k] - 0.5f * (root_box[k] + root_box[d + k]);
-        xc[p * dp + (k & 3) * q4 + (k >> 2)] = v;
+        tile[(k >> 2) * 512 + (k & 3)] = v;
         s = __builtin_fmaf(v, v, s);
     }
     nrm[p] = s;
@@ -82,10 +85,9 @@ __global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const
 // --------------------------------------------------------- MFMA scans ----
 
 // One block owns 128 rows (4 waves x 32) and streams every 128-column tile
-// of the centred points through LDS.  Per tile each wave computes a 32 x 128
-// block of dot products: 2 row blocks x 8 column blocks of 16 x 16, K = 4 per
-// instruction.  LDS layout: row (s * 4 + g) holds dimension 4 s + g of the
-// tile's 128 columns (+16 floats of padding against bank conflicts).
+// of the centred points through LDS, double-buffered (one barrier per
+// tile).  Per tile each wave computes a 32 x 128 block of dot products:
+// 2 row blocks x 8 column blocks of 16 x 16, K = 4 per instruction.
 // MODE 0: collect kNN candidates under the row thresholds.
 // MODE 1: smallest approximate mutual-reachability value to another
 //         component per row.
@@ -95,15 +97,14 @@ __global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const
 //         bounds the true k-th exact distance, plus a second margin bounds
 //         the approximate value of every true k nearest neighbour.
 template <int DP4, int MODE>
-__global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
+__global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     constexpr int DP = 4 * DP4;
-    constexpr int LROW = 144;
-    constexpr int NV = DP / 8;  // float4 loads per thread per tile
-    // (MODE 2 reuses this space for the per-row unions: 128 x 64 floats)
-    __shared__ float bs[(DP * LROW) > (MODE == 2 ? 128 * 64 : 0) ? DP * LROW : 128 * 64];
-    __shared__ float col_nrm[128];
-    __shared__ float col_core[128];
-    __shared__ int col_comp[128];
+    constexpr int TILE = 128 * DP;  // floats per tile
+    constexpr int NV = DP / 8;      // float4 loads per thread per tile
+    __shared__ float bs[2 * TILE > 128 * 64 ? 2 * TILE : 128 * 64];
+    __shared__ float col_nrm[2][128];
+    __shared__ float col_core[2][128];
+    __shared__ int col_comp[2][128];
     __shared__ int cnt_l[128];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -115,9 +116,9 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
     float A[2][DP4];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
-        const float *src = a.xc + (rbase + rb * 16 + c16) * DP + g * DP4;
+        const float *src = a.xc + blk_row * DP + (wave * 32 + rb * 16 + c16) * 4 + g;
 #pragma unroll
-        for (int s = 0; s < DP4; ++s) A[rb][s] = src[s];
+        for (int s = 0; s < DP4; ++s) A[rb][s] = src[s * 512];
     }
     // the 8 output rows this lane holds: rbase + rb*16 + 4g + r
     float rn[2][4], rv[2][4], rcore[2][4];
@@ -171,32 +172,24 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
             }
         }
     };
-    auto lstore = [&]() {
+    auto lstore = [&](int buf) {
+        floatx4 *dst = reinterpret_cast<floatx4 *>(bs + buf * TILE);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const int f = tid + v * 256;
-            const int col = f / (DP / 4);
-            const int e0 = (f % (DP / 4)) * 4;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int e = e0 + i;
-                const int gg = e / DP4, s = e % DP4;
-                bs[(s * 4 + gg) * LROW + col] = pre[v][i];
-            }
-        }
+        for (int v = 0; v < NV; ++v) dst[tid + v * 256] = pre[v];
         if (tid < 128) {
-            col_nrm[tid] = pn;
-            col_core[tid] = pc;
-            col_comp[tid] = pk;
+            col_nrm[buf][tid] = pn;
+            col_core[buf][tid] = pc;
+            col_comp[buf][tid] = pk;
         }
     };
 
     gload(0);
+    lstore(0);
+    __syncthreads();
+    if (ntiles > 1) gload(1);
     for (int t = 0; t < ntiles; ++t) {
-        __syncthreads();
-        lstore();
-        __syncthreads();
-        if (t + 1 < ntiles) gload(t + 1);
+        const int cur = t & 1;
+        const float *bt = bs + cur * TILE;
         floatx4 acc[2][8];
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
@@ -204,10 +197,10 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
             for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int s = 0; s < DP4; ++s) {
-            const float *brow = bs + (s * 4 + g) * LROW + c16;
+            const float *brow = bt + s * 512 + c16 * 4 + g;
 #pragma unroll
             for (int cb = 0; cb < 8; ++cb) {
-                const float b = brow[cb * 16];
+                const float b = brow[cb * 64];
                 acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[0][s], b, acc[0][cb], 0, 0, 0);
                 acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[1][s], b, acc[1][cb], 0, 0, 0);
             }
@@ -218,7 +211,7 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
             const int cl = cb * 16 + c16;
             const int64_t col = col0 + cl;
             const bool valid = col < n;
-            const float cn = col_nrm[cl];
+            const float cn = col_nrm[cur][cl];
             if constexpr (MODE == 2) {
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
@@ -254,8 +247,8 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
                         }
                     }
             } else {
-                const float cc = col_core[cl];
-                const int ck = col_comp[cl];
+                const float cc = col_core[cur][cl];
+                const int ck = col_comp[cur][cl];
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -270,6 +263,13 @@ __global__ __launch_bounds__(256) void brute_scan_kernel(BruteScanArgs a) {
                     }
             }
         }
+        // tile t+1 into the other buffer (last read in iteration t-1, before
+        // the previous barrier), prefetch t+2, then one barrier
+        if (t + 1 < ntiles) {
+            lstore(cur ^ 1);
+            if (t + 2 < ntiles) gload(t + 2);
+        }
+        __syncthreads();
     }
 
     if constexpr (MODE == 0) {
