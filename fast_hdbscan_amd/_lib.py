@@ -163,8 +163,8 @@ def load() -> ctypes.CDLL:
         L.fh_build_kdtree.restype = ctypes.c_int
         L.fh_build_kdtree.argtypes = [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fh_labels_from_sorted_mst.restype = ctypes.c_int
-        L.fh_labels_from_sorted_mst.argtypes = [_i64, _vp, _vp, _vp, ctypes.POINTER(FHLabelParams), _vp, _vp, _vp,
-                                                ctypes.POINTER(FHCondensedTree), _vp]
+        L.fh_labels_from_sorted_mst.argtypes = [_i64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(FHLabelParams), _vp,
+                                                _vp, _vp, _vp, _vp]
         L.fh_neighbor_columns.restype = _i64
         L.fh_neighbor_columns.argtypes = [_i64, _i32, _vp]
         L.fh_cluster_layers.restype = ctypes.c_int

@@ -43,6 +43,10 @@ struct Merge {
 
 struct Hierarchy {
     std::vector<Merge> rows;  // row i is merge node n + i
+    // with sample weights: float32 weighted sizes of each row's children
+    // (create_linkage_merge_data_w_sample_weights, cluster_trees.py:29-35)
+    bool weighted = false;
+    std::vector<float> lw, rw;
 };
 
 inline void prefetch(const void *p) { __builtin_prefetch(p, 0, 1); }
@@ -101,11 +105,25 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
 // halving) that records, per set, the merge node currently representing
 // it: the node linkage_merge_find would return.  Upcoming edges' entries
 // are prefetched.
-void linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, const double *dist,
-                  Hierarchy &h) {
+// With weights (cluster_trees.py:98-127) each set also carries its float32
+// weighted size, summed in the reference's order (size[l] + size[r]).
+// Returns false for a -1 endpoint with weights (not supported).
+bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, const double *dist,
+                  Hierarchy &h, const float *weights = nullptr) {
     const int64_t m = n - 1;
+    h.weighted = weights != nullptr;
     for (int64_t i = 0; i < m; ++i)
-        if (src[i] < 0 || dst[i] < 0) return linkage_tree_literal(n, src, dst, w_sq, dist, h);
+        if (src[i] < 0 || dst[i] < 0) {
+            if (weights) return false;
+            linkage_tree_literal(n, src, dst, w_sq, dist, h);
+            return true;
+        }
+    static thread_local std::vector<float> wsz;
+    if (weights) {
+        wsz.assign(weights, weights + n);
+        h.lw.resize(m);
+        h.rw.resize(m);
+    }
     struct Set {
         int32_t parent, size, node, pad;
     };
@@ -147,7 +165,14 @@ void linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         uf[child].parent = root;
         uf[root].size = ls + rs;
         uf[root].node = static_cast<int32_t>(n + i);
+        if (weights) {
+            const float wa = wsz[a], wb = wsz[b];
+            h.lw[i] = lbig ? wa : wb;
+            h.rw[i] = lbig ? wb : wa;
+            wsz[root] = wa + wb;
+        }
     }
+    return true;
 }
 
 struct Condensed {
@@ -247,8 +272,10 @@ void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_met
         const int32_t left = r.left, right = r.right;
         const double d = r.delta;
         const double lam = lambda_method == kLambdaHdbscan ? (d > 0.0 ? 1.0 / d : kInfD) : std::exp(-d);
-        // counts as float32, compared in float64 as numba does
-        const float lc = static_cast<float>(r.left_size), rc = static_cast<float>(r.right_size);
+        // counts as float32 (weighted sizes with weights), compared in
+        // float64 as numba does
+        const float lc = h.weighted ? h.lw[node - num_points] : static_cast<float>(r.left_size);
+        const float rc = h.weighted ? h.rw[node - num_points] : static_cast<float>(r.right_size);
         const double dl = lc, dr = rc;
         if (left < num_points && dr >= dm) {
             RL(right, static_cast<int32_t>(parent_node));
@@ -873,6 +900,7 @@ void binary_search_for_n_clusters(Hierarchy &h, int64_t n, int64_t approx, int64
 }  // namespace
 
 extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
+                                         const double *dist, const float *sample_weights,
                                          const fh_label_params *params, int64_t *labels, float *probabilities,
                                          double *linkage, fh_condensed_tree *condensed, double *phase_ms) {
     using clk = std::chrono::steady_clock;
@@ -880,7 +908,7 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         return std::chrono::duration<double, std::milli>(clk::now() - a).count();
     };
     if (n < 1 || !params || !labels || !probabilities) return FH_EINVAL;
-    if (n > 1 && (!src || !dst || !w_sq)) return FH_EINVAL;
+    if (n > 1 && (!src || !dst || (!w_sq && !dist))) return FH_EINVAL;
     if (params->min_cluster_size < 1) return FH_EINVAL;
     if (params->selection_method != 0 && params->selection_method != 1) return FH_EINVAL;
     try {
@@ -893,7 +921,7 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         // reused between calls on this thread (no fresh pages per call)
         static thread_local Hierarchy h;
         clk::time_point t = clk::now();
-        linkage_tree(n, src, dst, w_sq, nullptr, h);
+        if (!linkage_tree(n, src, dst, dist ? nullptr : w_sq, dist, h, sample_weights)) return FH_EUNSUPPORTED;
         if (phase_ms) phase_ms[0] = ms_since(t);
         if (linkage) {
             for (int64_t i = 0; i < n - 1; ++i) {
@@ -901,7 +929,8 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
                 linkage[4 * i + 0] = static_cast<double>(r.left);
                 linkage[4 * i + 1] = static_cast<double>(r.right);
                 linkage[4 * i + 2] = r.delta;
-                linkage[4 * i + 3] = static_cast<double>(r.left_size) + static_cast<double>(r.right_size);
+                linkage[4 * i + 3] = h.weighted ? static_cast<double>(h.lw[i] + h.rw[i])
+                                                : static_cast<double>(r.left_size) + static_cast<double>(r.right_size);
             }
         }
         static thread_local Condensed c;
@@ -960,7 +989,7 @@ extern "C" int fh_cluster_layers(int64_t n, const int32_t *src, const int32_t *d
             return FH_OK;
         }
         Hierarchy h;
-        linkage_tree(n, src, dst, nullptr, dist, h);
+        if (!linkage_tree(n, src, dst, nullptr, dist, h)) return FH_EUNSUPPORTED;
         int64_t mcs = p->base_min_cluster_size;
         Condensed c;
         if (p->base_n_clusters > 0) {

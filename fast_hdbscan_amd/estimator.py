@@ -2,7 +2,7 @@
 
 ``fast_hdbscan`` and ``HDBSCAN`` mirror hdbscan.fast_hdbscan
 (hdbscan.py:199-336) and hdbscan.HDBSCAN (hdbscan.py:577-822) for Euclidean,
-unsupervised clustering with unit sample weights: the MST comes from the
+unsupervised clustering (sample weights included): the MST comes from the
 device kernels and the labels from the host label stage.  ``hdbscan_device``
 is the fully device-resident pipeline used by the benchmark.
 """
@@ -128,7 +128,8 @@ def to_numpy_rec_array(named_tuple_tree):
 
 class HDBSCAN:
     """hdbscan.HDBSCAN (hdbscan.py:577-822) for Euclidean, unsupervised
-    clustering: fit / fit_predict, labels_, probabilities_.  Rows with
+    clustering, with or without sample weights: fit / fit_predict, labels_,
+    probabilities_.  Rows with
     non-finite values are left out of the clustering and labelled noise with
     probability 0 (hdbscan.py:768-820)."""
 
@@ -166,10 +167,14 @@ class HDBSCAN:
         from sklearn.utils import check_array
 
         X = check_array(X, ensure_all_finite=False)
+        if sample_weight is not None:
+            sample_weight = _check_sample_weight(sample_weight, X, dtype=np.float32)
         all_finite = bool(np.all(np.isfinite(X)))
         if not all_finite:
             finite_index = np.where(np.isfinite(X).sum(axis=1) == X.shape[1])[0]
             clean = X[finite_index]
+            if sample_weight is not None:
+                sample_weight = sample_weight[finite_index]
         else:
             clean = X
         (self.labels_, self.probabilities_, self._single_linkage_tree, self._condensed_tree,

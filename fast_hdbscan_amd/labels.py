@@ -1,8 +1,8 @@
 """Host label stage: MST -> single linkage -> condensed tree -> clusters.
 
 ``clusters_from_spanning_tree`` mirrors hdbscan.clusters_from_spanning_tree
-(hdbscan.py:497-574) for unit sample weights and unsupervised selection;
-the work runs in the library's host C++ (fh_labels_from_sorted_mst).
+(hdbscan.py:497-574) for unsupervised selection, with or without sample
+weights; the work runs in the library's host C++ (fh_labels_from_sorted_mst).
 """
 from __future__ import annotations
 
@@ -26,13 +26,18 @@ def label_params(min_cluster_size=10, cluster_selection_method="eom", max_cluste
                               _METHODS[cluster_selection_method], 1 if allow_single_cluster else 0)
 
 
-def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, phases: dict | None = None):
+def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, phases: dict | None = None,
+                              dist=None, sample_weights=None):
     """Run the host label stage on MST rows already sorted by (weight,
-    source, target).  Returns (labels int64, strengths float32, linkage or
-    None, CondensedTree or None)."""
+    source, target): squared float32 weights w_sq, or float64 distances
+    dist (then w_sq may be None); float32 sample weights optional.  Returns
+    (labels int64, strengths float32, linkage or None, CondensedTree or
+    None)."""
     src = np.ascontiguousarray(src, dtype=np.int32)
     dst = np.ascontiguousarray(dst, dtype=np.int32)
-    w_sq = np.ascontiguousarray(w_sq, dtype=np.float32)
+    w_sq = None if w_sq is None else np.ascontiguousarray(w_sq, dtype=np.float32)
+    dist = None if dist is None else np.ascontiguousarray(dist, dtype=np.float64)
+    sw = None if sample_weights is None else np.ascontiguousarray(sample_weights, dtype=np.float32)
     labels = np.empty(n, np.int64)
     strengths = np.empty(n, np.float32)
     linkage = np.empty((max(n - 1, 0), 4), np.float64) if return_trees else None
@@ -46,11 +51,15 @@ def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, pha
                                      ct["child_size"].ctypes.data, cap, 0)
     ph = np.zeros(4, np.float64)
     code = _lib.load().fh_labels_from_sorted_mst(
-        n, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(w_sq), ctypes.byref(params), _lib.ptr(labels), _lib.ptr(strengths),
+        n, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(w_sq) if w_sq is not None else None,
+        _lib.ptr(dist) if dist is not None else None, _lib.ptr(sw) if sw is not None else None,
+        ctypes.byref(params), _lib.ptr(labels), _lib.ptr(strengths),
         _lib.ptr(linkage) if linkage is not None else None, ctypes.byref(ctree) if ctree is not None else None,
         _lib.ptr(ph))
     if code == _lib.FH_EINVAL:
         raise ValueError("invalid label-stage arguments")
+    if code == _lib.FH_EUNSUPPORTED:
+        raise NotImplementedError("sample weights with a -1 MST endpoint are not supported")
     if code != _lib.FH_OK:
         raise _lib.FHError(f"label stage failed with code {code}")
     if phases is not None:
@@ -80,24 +89,22 @@ def clusters_from_spanning_tree(
     sorted_mst)."""
     if semi_supervised:
         raise NotImplementedError("semi-supervised selection is not in the host label stage")
-    if sample_weights is not None:
-        raise NotImplementedError("sample weights are not in the host label stage")
     if cluster_selection_persistence > 0.0:
         raise NotImplementedError("cluster_selection_persistence is not in the host label stage")
     mst = np.asarray(minimum_spanning_tree, dtype=np.float64)
     n = mst.shape[0] + 1
     # hdbscan.py:511-519
     sorted_mst = mst[np.lexsort((mst.T[1], mst.T[0], mst.T[2]))]
-    # The edge weights are float64 square roots of float32 squared distances;
-    # squaring back in float32 recovers them exactly for MSTs this library
-    # produced.  Other inputs go through the float64 path below.
-    w64 = sorted_mst[:, 2]
-    w_sq = (w64 * w64).astype(np.float32)
-    exact = np.array_equal(np.sqrt(w_sq.astype(np.float64)), w64)
     params = label_params(min_cluster_size, cluster_selection_method, max_cluster_size, allow_single_cluster,
                           cluster_selection_epsilon)
-    if not exact:
-        raise NotImplementedError("MST weights that are not square roots of float32 values are not supported")
+    sw = None
+    if sample_weights is not None:
+        sw = np.asarray(sample_weights)
+        if sw.shape != (n,):
+            raise ValueError("sample_weights must have one value per point")
+        if sw.dtype != np.float32 and not np.array_equal(sw.astype(np.float32).astype(sw.dtype), sw):
+            raise NotImplementedError("sample weights must be float32-representable")
     labels, strengths, linkage, condensed = labels_from_sorted_arrays(
-        n, sorted_mst[:, 0].astype(np.int32), sorted_mst[:, 1].astype(np.int32), w_sq, params, return_trees=True)
+        n, sorted_mst[:, 0].astype(np.int32), sorted_mst[:, 1].astype(np.int32), None, params, return_trees=True,
+        dist=np.ascontiguousarray(sorted_mst[:, 2]), sample_weights=sw)
     return labels, strengths, linkage, condensed, sorted_mst

@@ -225,22 +225,26 @@ typedef struct {
  * Labels and membership strengths from an MST whose rows are already
  * sorted by (weight, source, target).
  *   Replaces: hdbscan.py:521-574 (clusters_from_spanning_tree after its
- *             lexsort at 511-519) for unit sample weights, not
- *             semi-supervised, cluster_selection_persistence = 0:
- *             mst_to_linkage_tree, condense_tree, extract_eom_clusters or
- *             cluster_tree_leaves, cluster_epsilon_search,
- *             get_cluster_label_vector, get_point_membership_strength_vector
- *             (cluster_trees.py).
- * HOST pointers.  src/dst/w_sq [n-1]: sorted MST rows (w_sq squared; the
- * float64 square root is taken inside, as boruvka.py:646 does).
+ *             lexsort at 511-519), not semi-supervised,
+ *             cluster_selection_persistence = 0: mst_to_linkage_tree (or
+ *             mst_to_linkage_tree_w_sample_weights), condense_tree,
+ *             extract_eom_clusters or cluster_tree_leaves,
+ *             cluster_epsilon_search, get_cluster_label_vector,
+ *             get_point_membership_strength_vector (cluster_trees.py).
+ * HOST pointers.  src/dst [n-1]: sorted MST rows; their distances either
+ * as w_sq [n-1] float32 squared (the float64 square root is taken inside,
+ * as boruvka.py:646 does) or, when dist is not NULL, as dist [n-1] float64.
+ * sample_weights [n] float32 or NULL (float32 weighted sizes in the
+ * linkage and condensed trees, cluster_trees.py:29-35, 98-127, 240-250).
  * labels [n] int64, probabilities [n] float32.  linkage [(n-1)*4] float64
  * (rows left, right, distance, size, as mst_to_linkage_tree), condensed
  * and phase_ms [4] (linkage, condense, selection, labels; milliseconds)
  * may be NULL.  No context: host-only.
  */
 int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
-                              const fh_label_params *params, int64_t *labels, float *probabilities,
-                              double *linkage, fh_condensed_tree *condensed, double *phase_ms);
+                              const double *dist, const float *sample_weights, const fh_label_params *params,
+                              int64_t *labels, float *probabilities, double *linkage, fh_condensed_tree *condensed,
+                              double *phase_ms);
 
 /*
  * Sort MST rows by (weight, source, target) on the device -- the lexsort

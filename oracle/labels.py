@@ -35,14 +35,19 @@ def sort_mst(mst: np.ndarray) -> np.ndarray:
     return mst[np.lexsort((mst.T[1], mst.T[0], mst.T[2]))]
 
 
-def mst_to_linkage_tree(sorted_mst: np.ndarray) -> np.ndarray:
+def mst_to_linkage_tree(sorted_mst: np.ndarray, sample_weights=None) -> np.ndarray:
     """cluster_trees.py:66-95 with create_linkage_merge_data (17-25),
-    linkage_merge_find (38-52) and linkage_merge_join (55-62).  Python's
-    negative indexing matches numba's for a -1 endpoint."""
+    linkage_merge_find (38-52) and linkage_merge_join (55-62); with
+    sample_weights, mst_to_linkage_tree_w_sample_weights (98-127): sizes
+    are the weights concatenated with float32 zeros, summed in that dtype.
+    Python's negative indexing matches numba's for a -1 endpoint."""
     m = sorted_mst.shape[0]
     n = m + 1
     parent = np.full(2 * n - 1, -1, dtype=np.intp)
-    size = np.concatenate((np.ones(n, dtype=np.intp), np.zeros(n - 1, dtype=np.intp)))
+    if sample_weights is None:
+        size = np.concatenate((np.ones(n, dtype=np.intp), np.zeros(n - 1, dtype=np.intp)))
+    else:
+        size = np.concatenate((np.asarray(sample_weights), np.zeros(n - 1, dtype=np.float32)))
     nxt = n
     result = np.empty((m, 4))
 
@@ -102,7 +107,8 @@ class CondensedTree:
         self.child_size = child_size
 
 
-def condense_tree(hierarchy, min_cluster_size=10, max_cluster_size=np.inf, lambda_method="hdbscan"):
+def condense_tree(hierarchy, min_cluster_size=10, max_cluster_size=np.inf, lambda_method="hdbscan",
+                  sample_weights=None):
     """cluster_trees.py:205-352 with eliminate_branch (115-145), unit sample
     weights; lambda 1/d for "hdbscan" (192-197), exp(-d) for "plscan"
     (200-202), stored as float32."""
@@ -117,7 +123,8 @@ def condense_tree(hierarchy, min_cluster_size=10, max_cluster_size=np.inf, lambd
     lambdas = np.empty(root, dtype=np.float32)
     sizes = np.ones(root, dtype=np.float32)
     ignore = np.zeros(root + 1, dtype=np.bool_)
-    sample_weights = np.ones(num_points, dtype=np.float32)
+    if sample_weights is None:
+        sample_weights = np.ones(num_points, dtype=np.float32)
     idx = 0
 
     def eliminate(branch_node, parent_node, lambda_value, idx):
@@ -392,14 +399,14 @@ def get_point_membership_strength_vector(t, clusters, labels):
 
 def clusters_from_spanning_tree(mst, min_cluster_size=10, cluster_selection_method="eom",
                                 max_cluster_size=np.inf, allow_single_cluster=False,
-                                cluster_selection_epsilon=0.0):
-    """hdbscan.py:497-574 for unit weights, not semi-supervised, without
-    persistence simplification.  Returns (labels, membership_strengths, linkage_tree,
+                                cluster_selection_epsilon=0.0, sample_weights=None):
+    """hdbscan.py:497-574, not semi-supervised, without persistence
+    simplification.  Returns (labels, membership_strengths, linkage_tree,
     condensed_tree, sorted_mst)."""
     n_points = mst.shape[0] + 1
     sorted_mst = sort_mst(mst)
-    linkage = mst_to_linkage_tree(sorted_mst)
-    ct_full = condense_tree(linkage, min_cluster_size=min_cluster_size)
+    linkage = mst_to_linkage_tree(sorted_mst, sample_weights)
+    ct_full = condense_tree(linkage, min_cluster_size=min_cluster_size, sample_weights=sample_weights)
     ct = cluster_tree_from_condensed_tree(ct_full)
     if cluster_selection_method == "eom":
         selected = extract_eom_clusters(ct_full, ct, max_cluster_size=max_cluster_size,

@@ -112,3 +112,26 @@ def test_estimator_missing_data():
     clean = list(range(1, 5)) + list(range(6, 200))
     clean_model = HDBSCAN().fit(Xm[clean])
     assert np.allclose(clean_model.labels_, model.labels_[clean])
+
+
+def test_reference_sample_weight_estimator():
+    # test_hdbscan.py:113-117 end to end: device weighted MST + weighted labels
+    from sklearn.datasets import make_blobs
+    from sklearn.preprocessing import StandardScaler
+    from sklearn.utils import shuffle
+
+    from fast_hdbscan_amd.estimator import HDBSCAN, fast_hdbscan
+    from oracle import labels as OL
+    from oracle import oracle as O
+
+    X, y = make_blobs(n_samples=200, random_state=10)
+    X, y = shuffle(X, y, random_state=7)
+    X = StandardScaler().fit_transform(X)
+    sw = y.astype(np.float32)
+    labels, p = fast_hdbscan(X, sample_weights=sw)
+    assert len(set(labels)) - int(-1 in labels) == 2
+    e, _, _ = O.compute_minimum_spanning_tree(X, 10, sample_weights=sw)
+    ref = OL.clusters_from_spanning_tree(e, min_cluster_size=10, sample_weights=sw)
+    np.testing.assert_array_equal(labels, ref[0])
+    np.testing.assert_array_equal(p, ref[1])
+    np.testing.assert_array_equal(HDBSCAN(min_cluster_size=10).fit(X, sample_weight=sw).labels_, labels)

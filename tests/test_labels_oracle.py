@@ -145,3 +145,47 @@ def test_host_label_stage_matches_oracle(case):
     for f in ("parent", "child", "lambda_val", "child_size"):
         np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
     np.testing.assert_array_equal(a[4], b[4])
+
+
+# ------------------------------------------------------ sample weights ----
+
+def _weighted_case(seed, n=1500, ms=6):
+    from oracle import oracle as O
+    from sklearn.datasets import make_blobs
+
+    X, _ = make_blobs(n, centers=5, cluster_std=1.2, random_state=seed)
+    w = np.random.default_rng(seed).uniform(0.2, 3.0, n).astype(np.float32)
+    e, _, _ = O.compute_minimum_spanning_tree(X, ms, sample_weights=w)
+    return e, w
+
+
+@pytest.mark.parametrize("seed,mcs,method", [(0, 10, "eom"), (1, 25, "eom"), (2, 15, "leaf"), (3, 40, "eom")])
+def test_weighted_label_stage_matches_oracle(seed, mcs, method):
+    from fast_hdbscan_amd.labels import clusters_from_spanning_tree
+    from oracle import labels as OL
+
+    e, w = _weighted_case(seed)
+    ref = OL.clusters_from_spanning_tree(e, min_cluster_size=mcs, cluster_selection_method=method, sample_weights=w)
+    got = clusters_from_spanning_tree(e, min_cluster_size=mcs, cluster_selection_method=method, sample_weights=w)
+    np.testing.assert_array_equal(got[0], ref[0])
+    np.testing.assert_array_equal(got[1], ref[1])
+    np.testing.assert_array_equal(got[2], ref[2])
+    for a, b in zip(got[3], (ref[3].parent, ref[3].child, ref[3].lambda_val, ref[3].child_size)):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_reference_sample_weight_known_answer():
+    # test_hdbscan.py:113-117: the blob labels as weights -> 2 clusters
+    from fast_hdbscan_amd.labels import clusters_from_spanning_tree
+    from oracle import oracle as O
+    from sklearn.datasets import make_blobs
+    from sklearn.preprocessing import StandardScaler
+    from sklearn.utils import shuffle
+
+    X, y = make_blobs(n_samples=200, random_state=10)
+    X, y = shuffle(X, y, random_state=7)
+    X = StandardScaler().fit_transform(X)
+    sw = y.astype(np.float32)
+    e, _, _ = O.compute_minimum_spanning_tree(X, 10, sample_weights=sw)
+    labels = clusters_from_spanning_tree(e, min_cluster_size=10, sample_weights=sw)[0]
+    assert len(set(labels)) - int(-1 in labels) == 2
