@@ -40,6 +40,8 @@ EXPORTED_SYMBOLS = (
     "fh_mst_kruskal_knn",
     "fh_set_learned_radius",
     "fh_mlp_step",
+    "fh_core_graph",
+    "fh_csr_boruvka",
 )
 
 
@@ -178,6 +180,10 @@ def load() -> ctypes.CDLL:
         L.fh_set_learned_radius.argtypes = [_vp, _i32]
         L.fh_mlp_step.restype = ctypes.c_int
         L.fh_mlp_step.argtypes = [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_float]
+        L.fh_core_graph.restype = _i64
+        L.fh_core_graph.argtypes = [_i64, _i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64]
+        L.fh_csr_boruvka.restype = ctypes.c_int
+        L.fh_csr_boruvka.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fh_sort_mst_device.restype = ctypes.c_int
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L

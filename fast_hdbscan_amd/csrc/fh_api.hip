@@ -25,6 +25,7 @@
 
 #include "../../include/fh.h"
 #include "fh_brute.h"
+#include "fh_csr.h"
 #include "fh_kernels.h"
 #include "fh_mlp.h"
 #include "fh_sort.h"
@@ -1036,6 +1037,17 @@ int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t
     a.temp_bytes = ctx->stemp.cap;
     FH_HIP(fh::sort_mst(a, ctx->stream));
     FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
+int fh_csr_boruvka(fh_ctx *ctx, int64_t n, const int32_t *indptr, const int32_t *indices, const float *weights,
+                   double *edges, int64_t *n_edges, int64_t *point_components, int64_t *n_components) {
+    if (!ctx) return FH_EINVAL;
+    FH_HIP(hipSetDevice(ctx->device));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    const int code = fh::csr_boruvka(ctx->device, n, indptr, indices, weights, edges, n_edges, point_components,
+                                     n_components);
+    if (code != FH_OK) return fail(ctx, code, "CSR Boruvka failed");
     return FH_OK;
 }
 

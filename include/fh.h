@@ -171,6 +171,43 @@ int fh_mst_kruskal_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_
                        double *core, fh_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Core graphs (lens-weighted kNN + MST graphs)                        */
+/* ------------------------------------------------------------------ */
+
+/*
+ * The core graph as CSR.
+ *   Replaces: core_graph.py:13-104 (knn_mst_union, flatten_to_csr,
+ *             sort_by_lens): each point's children in insertion order (its
+ *             kNN row skipping -1, then MST rows whose parent it is and
+ *             whose child is new), weight = max(lens_p, lens_c), distance =
+ *             max(core_p, core_c) (the MST distance for MST-only children),
+ *             stored float32, each row ordered by numba's argsort of its
+ *             weights.
+ * HOST pointers.  neighbors [n*k] int32, core_distances [n] float32, mst
+ * [m*3] float64 rows (parent, child, distance), lens [n] float64; outputs
+ * indptr [n+1], indices/weights/distances [capacity] (n*k + m suffices).
+ * Returns the entry count, or a negative FH_E* code.  Host-only.
+ */
+int64_t fh_core_graph(int64_t n, int32_t k, const int32_t *neighbors, const float *core_distances, int64_t m,
+                      const double *mst, const double *lens, int32_t *indptr, int32_t *indices, float *weights,
+                      float *distances, int64_t capacity);
+
+/*
+ * Boruvka on a CSR graph whose rows are sorted by weight.
+ *   Replaces: core_graph.py:111-208 (boruvka_mst with select_components,
+ *             merge_components, update_point_components,
+ *             update_graph_components): the same edge rows in the same
+ *             order, the reference's union-by-rank roots as component
+ *             labels, stopping when a round adds no edge.
+ * HOST pointers.  indptr [n+1], indices / weights [indptr[n]] (weights are
+ * what the reference calls the graph's weights, e.g. lens values);
+ * edges [(n-1)*3] float64 rows (point, child, weight); n_edges,
+ * point_components [n] int64, n_components out.
+ */
+int fh_csr_boruvka(fh_ctx *ctx, int64_t n, const int32_t *indptr, const int32_t *indices, const float *weights,
+                   double *edges, int64_t *n_edges, int64_t *point_components, int64_t *n_components);
+
+/* ------------------------------------------------------------------ */
 /* Learned search radii (no counterpart in the reference)              */
 /* ------------------------------------------------------------------ */
 
