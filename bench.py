@@ -215,7 +215,8 @@ def side_forms(Xd, ms):
     """The other forms of the path, each run once after the timed steps
     (not part of `value`): learned search radii on the same 2M x 2-D batch,
     the MFMA high-dimensional path at 1M x 64, the 10M x 3-D mixture and a
-    PLSCAN min_samples sweep on 500k x 2-D."""
+    PLSCAN min_samples sweep on 500k x 2-D and a batch of 512 small
+    independent datasets."""
     import numpy as np
     import torch
 
@@ -300,6 +301,20 @@ def side_forms(Xd, ms):
                                        "base_min_cluster_size 5",
                            "total_s": total, "mst_and_copy_s": t_mst, "layer_stage_s": total - t_mst,
                            "layers_per_min_samples": n_layers}
+
+    # many independent small datasets through the native batch scheduler
+    from fast_hdbscan_amd.batch import hdbscan_batch
+
+    data = workloads.batch_datasets(512)
+    pts = sum(x.shape[0] for x in data)
+    res = {"workload": "512 datasets, n log-uniform 500..50000, d in {2,3,5,8}, 3-20 Gaussians + 5% noise, "
+                       "seeds 1000..1511; min_samples 5, min_cluster_size 20", "points": pts}
+    for w in (1, 8, 16):
+        t0 = time.perf_counter()
+        hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w)
+        el = time.perf_counter() - t0
+        res[f"workers_{w}"] = {"s": el, "datasets_per_s": len(data) / el, "points_per_s": pts / el}
+    out["batched"] = res
     return out
 
 

@@ -42,6 +42,7 @@ EXPORTED_SYMBOLS = (
     "fh_mlp_step",
     "fh_core_graph",
     "fh_csr_boruvka",
+    "fh_hdbscan_batch",
 )
 
 
@@ -184,6 +185,8 @@ def load() -> ctypes.CDLL:
         L.fh_core_graph.argtypes = [_i64, _i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64]
         L.fh_csr_boruvka.restype = ctypes.c_int
         L.fh_csr_boruvka.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.fh_hdbscan_batch.restype = ctypes.c_int
+        L.fh_hdbscan_batch.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _i64, _i32, _vp, _vp, _vp]
         L.fh_sort_mst_device.restype = ctypes.c_int
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L

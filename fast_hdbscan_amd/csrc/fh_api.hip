@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1038,6 +1040,86 @@ int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t
     FH_HIP(fh::sort_mst(a, ctx->stream));
     FH_HIP(hipStreamSynchronize(ctx->stream));
     return FH_OK;
+}
+
+// Many independent requests: a pool of worker threads, each with its own
+// context (own HIP stream and buffers), takes datasets from a shared
+// counter and runs the MST on the device, sorts the rows on the host and
+// runs the host label stage, so different datasets' kernels overlap on the
+// GPU and their label stages on the host cores.
+int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
+                     int32_t min_samples, int64_t min_cluster_size, int32_t workers, int64_t *const *labels,
+                     float *const *probabilities, double *ms_out) {
+    if (count < 0 || !X || !n || !d || !labels || !probabilities || min_samples < 1 || min_cluster_size < 1)
+        return FH_EINVAL;
+    const int W = std::max(1, std::min<int>(workers, 16));
+    std::atomic<int> next{0};
+    std::atomic<int> err{0};
+    auto work = [&]() {
+        fh_ctx *ctx = fh_create(device);
+        if (!ctx) {
+            err = FH_EDEVICE;
+            return;
+        }
+        std::vector<int32_t> src, dst, s2, d2;
+        std::vector<float> w, w2;
+        std::vector<int32_t> order;
+        fh_label_params prm{min_cluster_size, std::numeric_limits<double>::infinity(), 0.0, 0, 0};
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= count || err.load() != 0) break;
+            const auto t0 = std::chrono::steady_clock::now();
+            const int64_t ni = n[i];
+            if (ni < 1 || !X[i] || !labels[i] || !probabilities[i]) {
+                err = FH_EINVAL;
+                break;
+            }
+            if (ni < 2) {
+                labels[i][0] = -1;
+                probabilities[i][0] = 0.0f;
+            } else {
+                const int64_t m = ni - 1;
+                src.resize(m);
+                dst.resize(m);
+                w.resize(m);
+                int rc = fh_mst_euclidean(ctx, X[i], ni, d[i], min_samples, nullptr, nullptr, nullptr, src.data(),
+                                          dst.data(), w.data(), nullptr);
+                if (rc != FH_OK) {
+                    err = rc;
+                    break;
+                }
+                // hdbscan.py:511-519: rows by (weight, source, target)
+                order.resize(m);
+                for (int64_t r = 0; r < m; ++r) order[r] = (int32_t)r;
+                std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+                    if (w[a] != w[b]) return w[a] < w[b];
+                    if (src[a] != src[b]) return src[a] < src[b];
+                    return dst[a] < dst[b];
+                });
+                s2.resize(m);
+                d2.resize(m);
+                w2.resize(m);
+                for (int64_t r = 0; r < m; ++r) {
+                    s2[r] = src[order[r]];
+                    d2[r] = dst[order[r]];
+                    w2[r] = w[order[r]];
+                }
+                rc = fh_labels_from_sorted_mst(ni, s2.data(), d2.data(), w2.data(), nullptr, nullptr, &prm, labels[i],
+                                               probabilities[i], nullptr, nullptr, nullptr);
+                if (rc != FH_OK) {
+                    err = rc;
+                    break;
+                }
+            }
+            if (ms_out)
+                ms_out[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        fh_destroy(ctx);
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < W; ++t) th.emplace_back(work);
+    for (auto &x : th) x.join();
+    return err.load();
 }
 
 int fh_csr_boruvka(fh_ctx *ctx, int64_t n, const int32_t *indptr, const int32_t *indices, const float *weights,

@@ -72,3 +72,27 @@ def gaussian_highdim(n: int = 1_000_000, d: int = 64, n_clusters: int = 100, see
     X = means[lab] + rng.standard_normal((n, d))
     X = X[rng.permutation(n)]
     return np.ascontiguousarray(X.astype(np.float32))
+
+
+def batch_datasets(count: int = 4096, seed_base: int = 1000, n_lo: int = 500, n_hi: int = 50_000):
+    """Independent small datasets: per dataset i (rng seed seed_base + i)
+    n log-uniform in [n_lo, n_hi], d in {2, 3, 5, 8}, a 3-20 centre Gaussian
+    mixture (centres uniform in [-10, 10]^d, std uniform 0.5-2.0) plus 5%
+    uniform noise over its bounding box."""
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng(seed_base + i)
+        n = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        d = int(rng.choice([2, 3, 5, 8]))
+        k = int(rng.integers(3, 21))
+        std = rng.uniform(0.5, 2.0)
+        n_noise = int(0.05 * n)
+        n_clu = n - n_noise
+        centres = rng.uniform(-10.0, 10.0, size=(k, d))
+        lab = rng.integers(0, k, n_clu)
+        pts = centres[lab] + rng.normal(0.0, std, size=(n_clu, d))
+        lo, hi = pts.min(0), pts.max(0)
+        noise = rng.uniform(lo, hi, size=(n_noise, d))
+        X = np.vstack([pts, noise])[rng.permutation(n)]
+        out.append(np.ascontiguousarray(X, dtype=np.float32))
+    return out

@@ -171,6 +171,24 @@ int fh_mst_kruskal_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_
                        double *core, fh_stats *stats);
 
 /* ------------------------------------------------------------------ */
+/* Many independent requests                                           */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Labels and membership strengths for count independent datasets (the
+ * reference's fast_hdbscan(X_i, min_samples, min_cluster_size) per
+ * dataset: Euclidean Boruvka MST, lexsort of the rows, EOM labels;
+ * hdbscan.py:199-336).  workers (1..16) threads, each with its own
+ * context on `device`, take datasets in turn.  HOST pointers:
+ * X[i] [n[i]*d[i]], labels[i] [n[i]] int64, probabilities[i] [n[i]]
+ * float32; ms_out [count] per-dataset wall times or NULL.  No context
+ * argument: the call creates and releases its workers' contexts.
+ */
+int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
+                     int32_t min_samples, int64_t min_cluster_size, int32_t workers, int64_t *const *labels,
+                     float *const *probabilities, double *ms_out);
+
+/* ------------------------------------------------------------------ */
 /* Core graphs (lens-weighted kNN + MST graphs)                        */
 /* ------------------------------------------------------------------ */
 
