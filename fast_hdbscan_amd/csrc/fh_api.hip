@@ -334,6 +334,23 @@ int64_t weighted_k(const float *w, int64_t n, int ms) {
 // kNN of every point by the MFMA candidate scan (fh_brute.hip) with exact
 // re-scoring; rows that need the reference's heap order for ties go to the
 // tree search.  Leaves the packed points and norms in ctx for the rounds.
+// Whether the MFMA candidate scans replace the tree searches: from
+// brute_min_dim dimensions on, and for small inputs from 5 dimensions on,
+// where a tree search visits a large share of the leaves anyway and the
+// per-thread traversal of a few thousand points leaves most of the chip idle
+// (measured crossover n ~ 3000 d).  FH_HIGH_DIM_FROM, read per call,
+// replaces both rules with one switch-over dimension.  Both paths give the
+// same result.
+bool use_brute(const fh_ctx *ctx, int64_t n, int d, int kq, bool weighted) {
+    int from = ctx->brute_min_dim;
+    bool small = d >= 5 && n <= 3000 * (int64_t)d;
+    if (const char *m = std::getenv("FH_HIGH_DIM_FROM")) {
+        from = std::atoi(m);
+        small = false;
+    }
+    return !weighted && fh::brute_padded_dim(d) > 0 && (d >= from || small) && kq <= 64 && n >= 256;
+}
+
 int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_knn, fh::BruteScanArgs &bsa,
           fh_stats &S) {
     hipStream_t st = ctx->stream;
@@ -584,10 +601,8 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     // squared distances, so seeding is left out there.
     // high dimension: MFMA candidate scans instead of the tree searches
     // (FH_HIGH_DIM_FROM, read per call, moves the switch-over dimension)
-    int brute_from = ctx->brute_min_dim;
-    if (const char *m = std::getenv("FH_HIGH_DIM_FROM")) brute_from = std::atoi(m);
     const int dp = fh::brute_padded_dim(d);
-    const bool brute = !weighted && dp > 0 && d >= brute_from && kq <= 64 && n >= 256;
+    const bool brute = use_brute(ctx, n, d, kq, weighted);
     const bool seed = pruned && !weighted && kq > 1 && !brute;
     if (seed) {
         FH_TRY(ensure(ctx, ctx->nb_pos, (size_t)n * (kq - 1)));
@@ -1053,6 +1068,14 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
     if (count < 0 || !X || !n || !d || !labels || !probabilities || min_samples < 1 || min_cluster_size < 1)
         return FH_EINVAL;
     const int W = std::max(1, std::min<int>(workers, 16));
+    // Largest first: a worker's buffers reach their final size on its first
+    // dataset instead of growing (hipFree waits for the whole device) as
+    // larger ones arrive, and the long jobs do not trail at the end.
+    std::vector<int32_t> queue(std::max(count, 0));
+    for (int32_t i = 0; i < count; ++i) queue[i] = i;
+    std::stable_sort(queue.begin(), queue.end(), [&](int32_t a, int32_t b) {
+        return n[a] * std::max(d[a], 1) > n[b] * std::max(d[b], 1);
+    });
     std::atomic<int> next{0};
     std::atomic<int> err{0};
     auto work = [&]() {
@@ -1066,8 +1089,9 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
         std::vector<int32_t> order;
         fh_label_params prm{min_cluster_size, std::numeric_limits<double>::infinity(), 0.0, 0, 0};
         for (;;) {
-            const int i = next.fetch_add(1);
-            if (i >= count || err.load() != 0) break;
+            const int qi = next.fetch_add(1);
+            if (qi >= count || err.load() != 0) break;
+            const int i = queue[qi];
             const auto t0 = std::chrono::steady_clock::now();
             const int64_t ni = n[i];
             if (ni < 1 || !X[i] || !labels[i] || !probabilities[i]) {
@@ -1229,9 +1253,7 @@ int fh_mst_kruskal_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_
     FH_TRY(ensure(ctx, ctx->core_o, n));
     FH_TRY(ensure(ctx, ctx->core_p, n));
     FH_TRY(ensure(ctx, ctx->comp_p, n));
-    int brute_from = ctx->brute_min_dim;
-    if (const char *e = std::getenv("FH_HIGH_DIM_FROM")) brute_from = std::atoi(e);
-    const bool brute = !weighted && fh::brute_padded_dim(d) > 0 && d >= brute_from && kq <= 64 && n >= 256;
+    const bool brute = use_brute(ctx, n, d, kq, weighted);
     if (brute) {
         fh::BruteScanArgs bsa{};
         FH_TRY(brute_knn(ctx, n, d, levels, kq, 0, bsa, S));

@@ -100,11 +100,17 @@ def test_blobs_min_samples(ms):
 
 
 @pytest.mark.parametrize("d", [1, 3, 4, 5, 8, 10])
-def test_dimensions(d):
+@pytest.mark.parametrize("path", ["auto", "tree"])
+def test_dimensions(d, path, monkeypatch):
+    # small inputs from 5 dimensions on take the MFMA scans unless the
+    # switch-over dimension is moved past d
     from sklearn.datasets import make_blobs
 
+    if path == "tree":
+        monkeypatch.setenv("FH_HIGH_DIM_FROM", "1000")
     X = make_blobs(n_samples=3000, n_features=d, centers=6, random_state=d)[0]
-    _check(X, 5)
+    st = _check(X, 5)
+    assert st["high_dim"] == (1 if path == "auto" and d >= 5 else 0)
 
 
 def test_uniform_reference_fixture():
