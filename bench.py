@@ -281,9 +281,11 @@ def side_forms(Xd, ms):
                        "mst_ms": t3, "points_per_s": X3.shape[0] / (t3 * 1e-3), "rounds": st3["rounds"]}
     del X3
 
-    # PLSCAN sweep: min_samples 2..64 on 500k x 2-D, layers for each
+    # PLSCAN sweep: min_samples 2..64 on 500k x 2-D, layers for each; first
+    # one at a time, then pipelined (device MSTs back to back, layer stages
+    # in 8 host threads)
     from fast_hdbscan_amd.hdbscan import _edges_f64
-    from fast_hdbscan_amd.layers import layers_from_mst
+    from fast_hdbscan_amd.layers import layers_from_mst, plscan_sweep
 
     Xs = torch.from_numpy(workloads.anisotropic_mixture(500_000, 2, 50, 0.1, seed=4)).cuda()
     t0 = time.perf_counter()
@@ -297,9 +299,15 @@ def side_forms(Xd, ms):
         t_mst += time.perf_counter() - a
         n_layers.append(len(layers_from_mst(e, c, Xs.shape[0], base_min_cluster_size=5)[0]))
     total = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sw = plscan_sweep(Xs, range(2, 65), base_min_cluster_size=5, workers=8)
+    total_p = time.perf_counter() - t0
+    same = [len(sw[m][0]) for m in range(2, 65)] == n_layers
     out["plscan_sweep"] = {"workload": "500k x 2-D, 50 anisotropic Gaussians + 10% noise, seed 4; min_samples 2..64, "
                                        "base_min_cluster_size 5",
-                           "total_s": total, "mst_and_copy_s": t_mst, "layer_stage_s": total - t_mst,
+                           "serial_total_s": total, "serial_mst_and_copy_s": t_mst,
+                           "serial_layer_stage_s": total - t_mst, "pipelined_total_s": total_p,
+                           "pipelined_workers": 8, "pipelined_same_layer_counts": same,
                            "layers_per_min_samples": n_layers}
 
     # many independent small datasets through the native batch scheduler

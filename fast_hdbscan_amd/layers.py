@@ -74,6 +74,44 @@ def layers_from_mst(edges, core_distances, n_samples, base_min_cluster_size=10, 
             sizes_out, total_out)
 
 
+def plscan_sweep(data, min_samples_values, base_min_cluster_size=10, base_n_clusters=None,
+                 layer_similarity_threshold=0.2, max_layers=10, workers=8, device=0):
+    """build_cluster_layers (layer_clusters.py:353-495) of one dataset for
+    each value in ``min_samples_values``, as a pipeline: the points go to the
+    device once, the MSTs run one after another on the device, and each
+    MST's layer stage (host NumPy + C++, which release the GIL) runs in one
+    of ``workers`` host threads while the device computes the next MST.
+
+    Returns {min_samples: (cluster_layers, membership_strength_layers,
+    persistence_scores, sizes, total_persistence)}, each entry what
+    ``layers_from_mst`` returns for that MST.
+    """
+    import concurrent.futures as cf
+
+    import torch
+
+    from .hdbscan import _as_points, _edges_f64, mst_device
+
+    if isinstance(data, torch.Tensor) and data.is_cuda:
+        Xd = data.contiguous().float()
+    else:
+        Xd = torch.from_numpy(_as_points(data)).to(f"cuda:{device}")
+    n = int(Xd.shape[0])
+    values = [int(m) for m in min_samples_values]
+    out = {}
+    with cf.ThreadPoolExecutor(max_workers=max(1, int(workers))) as pool:
+        futures = {}
+        for m in values:
+            src, dst, w_sq, core_sq, _ = mst_device(Xd, m)
+            edges = _edges_f64(src.cpu().numpy(), dst.cpu().numpy(), w_sq.cpu().numpy())
+            core = np.sqrt(core_sq.cpu().numpy())
+            futures[m] = pool.submit(layers_from_mst, edges, core, n, base_min_cluster_size, base_n_clusters,
+                                     layer_similarity_threshold, max_layers)
+        for m in values:
+            out[m] = futures[m].result()
+    return out
+
+
 def build_cluster_layers(
     data,
     *,

@@ -36,3 +36,24 @@ def test_layers_device_mst_vs_oracle(ms, mcs):
         np.testing.assert_array_equal(a, b)
     assert [float(v) for v in ro[2]] == [float(v) for v in rf[2]]
     np.testing.assert_array_equal(ro[4], rf[4])
+
+
+def test_plscan_sweep_matches_one_at_a_time():
+    from fast_hdbscan_amd import compute_minimum_spanning_tree
+    from fast_hdbscan_amd import layers as FL
+
+    X = workloads.anisotropic_mixture(30000, 2, 15, 0.1, seed=4).astype(np.float32)
+    values = [2, 3, 7, 16, 33]
+    print("plscan sweep", values, flush=True)
+    sweep = FL.plscan_sweep(X, values, base_min_cluster_size=5, workers=4)
+    assert sorted(sweep) == values
+    for m in values:
+        e, _, core = compute_minimum_spanning_tree(X, min_samples=m)
+        one = FL.layers_from_mst(e, core, X.shape[0], base_min_cluster_size=5)
+        got = sweep[m]
+        assert len(one[0]) == len(got[0])
+        for a, b in zip(one[0] + one[1], got[0] + got[1]):
+            np.testing.assert_array_equal(a, b)
+        assert [float(v) for v in one[2]] == [float(v) for v in got[2]]
+        np.testing.assert_array_equal(one[3], got[3])
+        np.testing.assert_array_equal(one[4], got[4])
