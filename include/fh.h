@@ -179,7 +179,7 @@ int fh_mst_kruskal_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_
  * reference's fast_hdbscan(X_i, min_samples, min_cluster_size) per
  * dataset: Euclidean Boruvka MST, lexsort of the rows, EOM labels;
  * hdbscan.py:199-336).  workers (1..16) threads, each with its own
- * context on `device`, take datasets in turn.  HOST pointers:
+ * context on `device`, take datasets largest first.  HOST pointers:
  * X[i] [n[i]*d[i]], labels[i] [n[i]] int64, probabilities[i] [n[i]]
  * float32; ms_out [count] per-dataset wall times or NULL.  No context
  * argument: the call creates and releases its workers' contexts.
