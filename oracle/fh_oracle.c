@@ -558,6 +558,54 @@ static void bq_rec(const bq_ctx *c, int64_t node, const float *pt, float *hp, in
     }
 }
 
+/* The same search as the reference runs it under prange (boruvka.py:300-
+ * 331): the per-component bound B is shared by all threads without
+ * synchronisation.  Loads and stores of B are relaxed atomics here (plain
+ * word accesses in the reference), so which thread's bound a search sees
+ * depends on timing; the component minimum is still found, ties may go
+ * either way.  For timing the CPU path only: never a parity reference. */
+static inline float ld_relaxed(const float *p) {
+    float v;
+    __atomic_load(p, &v, __ATOMIC_RELAXED);
+    return v;
+}
+static inline void st_relaxed(float *p, float v) { __atomic_store(p, &v, __ATOMIC_RELAXED); }
+
+static void bq_rec_racy(const bq_ctx *c, int64_t node, const float *pt, float *hp, int32_t *hi, float core_i,
+                        int64_t comp_i, float lb, float *B) {
+    const tree_view *t = &c->t;
+    if (lb > hp[0]) return;
+    const float b0 = ld_relaxed(B);
+    if (lb > b0 || core_i > b0) return;
+    if (c->ncomp[node] == comp_i) return;
+    if (t->is_leaf[node]) {
+        for (int64_t i = t->idx_start[node]; i < t->idx_end[node]; ++i) {
+            int64_t idx = t->idx[i];
+            if (c->pcomp[idx] != comp_i && c->core[idx] < ld_relaxed(B)) {
+                float dd = rdist(pt, t->X + idx * t->d, t->d);
+                if (core_i > dd) dd = core_i;
+                if (c->core[idx] > dd) dd = c->core[idx];
+                if (dd < hp[0]) {
+                    hp[0] = dd;
+                    hi[0] = (int32_t)idx;
+                    if (dd < ld_relaxed(B)) st_relaxed(B, dd);
+                }
+            }
+        }
+    } else {
+        int64_t l = 2 * node + 1, r = l + 1;
+        float lbl = box_lb(node_lo(t, l), node_hi(t, l), pt, t->d);
+        float lbr = box_lb(node_lo(t, r), node_hi(t, r), pt, t->d);
+        if (lbl <= lbr) {
+            bq_rec_racy(c, l, pt, hp, hi, core_i, comp_i, lbl, B);
+            bq_rec_racy(c, r, pt, hp, hi, core_i, comp_i, lbr, B);
+        } else {
+            bq_rec_racy(c, r, pt, hp, hi, core_i, comp_i, lbr, B);
+            bq_rec_racy(c, l, pt, hp, hi, core_i, comp_i, lbl, B);
+        }
+    }
+}
+
 /* boruvka.py:94-124 */
 static void update_component_vectors(const tree_view *t, const int32_t *parent, int64_t n, int64_t *ncomp,
                                      int64_t *pcomp) {
@@ -677,7 +725,8 @@ static int64_t merge_components(int32_t *parent, int32_t *rank, int64_t n, const
  *   rounds         : number of Boruvka rounds executed
  * Returns the number of edges written. */
 static int64_t boruvka_rounds(tree_view t, int64_t n, const int32_t *knn_i, const float *knn_d, int kq,
-                              const float *core_sq, int64_t *src, int64_t *dst, float *w_sq, int32_t *rounds);
+                              const float *core_sq, int64_t *src, int64_t *dst, float *w_sq, int32_t *rounds,
+                              int racy_threads);
 
 int64_t orc_boruvka(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
                     const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
@@ -687,7 +736,22 @@ int64_t orc_boruvka(const float *X, int64_t n, int d, const int64_t *idx_array, 
     int kq = min_samples > 1 ? min_samples + 1 : 2;
     orc_knn(X, n, d, idx_array, idx_start, idx_end, is_leaf, bounds, N, X, n, kq, knn_d, knn_i, nthreads);
     for (int64_t i = 0; i < n; ++i) core_sq[i] = min_samples > 1 ? knn_d[i * kq + kq - 1] : 0.0f;
-    return boruvka_rounds(t, n, knn_i, knn_d, kq, core_sq, src, dst, w_sq, rounds);
+    return boruvka_rounds(t, n, knn_i, knn_d, kq, core_sq, src, dst, w_sq, rounds, 0);
+}
+
+/* orc_boruvka with the round searches spread over nthreads as the
+ * reference's prange spreads them (shared bounds, see bq_rec_racy).  Edge
+ * choice among exact ties and row order may differ from orc_boruvka; the
+ * weights' multiset does not.  For timing the CPU path. */
+int64_t orc_boruvka_racy(const float *X, int64_t n, int d, const int64_t *idx_array, const int64_t *idx_start,
+                         const int64_t *idx_end, const uint8_t *is_leaf, const float *bounds, int64_t N,
+                         int min_samples, int64_t *src, int64_t *dst, float *w_sq, float *core_sq, int32_t *knn_i,
+                         float *knn_d, int32_t *rounds, int nthreads) {
+    tree_view t = {X, d, idx_array, idx_start, idx_end, is_leaf, bounds, N};
+    int kq = min_samples > 1 ? min_samples + 1 : 2;
+    orc_knn(X, n, d, idx_array, idx_start, idx_end, is_leaf, bounds, N, X, n, kq, knn_d, knn_i, nthreads);
+    for (int64_t i = 0; i < n; ++i) core_sq[i] = min_samples > 1 ? knn_d[i * kq + kq - 1] : 0.0f;
+    return boruvka_rounds(t, n, knn_i, knn_d, kq, core_sq, src, dst, w_sq, rounds, nthreads > 1 ? nthreads : 1);
 }
 
 /* boruvka.py:564-566: k = int(2 * min_samples / mean(w)).  numba's mean of
@@ -722,11 +786,12 @@ int64_t orc_boruvka_weighted(const float *X, int64_t n, int d, const int64_t *id
         }
         core_sq[i] = knn_d[i * kq + (j - 1)];
     }
-    return boruvka_rounds(t, n, knn_i, knn_d, kq, core_sq, src, dst, w_sq, rounds);
+    return boruvka_rounds(t, n, knn_i, knn_d, kq, core_sq, src, dst, w_sq, rounds, 0);
 }
 
 static int64_t boruvka_rounds(tree_view t, int64_t n, const int32_t *knn_i, const float *knn_d, int kq,
-                              const float *core_sq, int64_t *src, int64_t *dst, float *w_sq, int32_t *rounds) {
+                              const float *core_sq, int64_t *src, int64_t *dst, float *w_sq, int32_t *rounds,
+                              int racy_threads) {
     const int64_t N = t.N;
     int32_t *parent = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
     int32_t *rank = (int32_t *)calloc((size_t)n, sizeof(int32_t));
@@ -764,10 +829,19 @@ static int64_t boruvka_rounds(tree_view t, int64_t n, const int32_t *knn_i, cons
             cand_i[i] = -1;
             B[i] = INFINITY;
         }
-        for (int64_t i = 0; i < n; ++i) {
-            const float *pt = t.X + i * t.d;
-            float lb = box_lb(node_lo(&t, 0), node_hi(&t, 0), pt, t.d);
-            bq_rec(&c, 0, pt, cand_d + i, cand_i + i, core_sq[i], pcomp[i], lb, B + pcomp[i]);
+        if (racy_threads > 0) {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(racy_threads)
+            for (int64_t i = 0; i < n; ++i) {
+                const float *pt = t.X + i * t.d;
+                float lb = box_lb(node_lo(&t, 0), node_hi(&t, 0), pt, t.d);
+                bq_rec_racy(&c, 0, pt, cand_d + i, cand_i + i, core_sq[i], pcomp[i], lb, B + pcomp[i]);
+            }
+        } else {
+            for (int64_t i = 0; i < n; ++i) {
+                const float *pt = t.X + i * t.d;
+                float lb = box_lb(node_lo(&t, 0), node_hi(&t, 0), pt, t.d);
+                bq_rec(&c, 0, pt, cand_d + i, cand_i + i, core_sq[i], pcomp[i], lb, B + pcomp[i]);
+            }
         }
         int64_t added = merge_components(parent, rank, n, cand_i, cand_d, pcomp, &E, slot, order, ei, ej, ew);
         n_comp -= added;

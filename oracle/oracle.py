@@ -94,6 +94,8 @@ def lib():
         L.orc_boruvka.restype = ctypes.c_int64
         L.orc_boruvka.argtypes = tree_args + [ctypes.c_int, _i64p, _i64p, _f32p, _f32p, _i32p, _f32p,
                                               ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+        L.orc_boruvka_racy.restype = ctypes.c_int64
+        L.orc_boruvka_racy.argtypes = L.orc_boruvka.argtypes
         L.orc_weighted_k.restype = ctypes.c_int64
         L.orc_weighted_k.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int]
         L.orc_boruvka_weighted.restype = ctypes.c_int64
@@ -246,6 +248,27 @@ def boruvka(tree: Tree, min_samples: int = 10, nthreads=None) -> BoruvkaResult:
     rounds = ctypes.c_int32(0)
     cnt = lib().orc_boruvka(*tree._args(), int(min_samples), src, dst, w, core, ki, kd,
                             ctypes.byref(rounds), _threads(nthreads))
+    return BoruvkaResult(src[:cnt], dst[:cnt], w[:cnt], core, ki, kd, int(rounds.value))
+
+
+def boruvka_racy(tree: Tree, min_samples: int = 10, nthreads=None) -> BoruvkaResult:
+    """parallel_boruvka as the reference runs it on several cores: kNN rows
+    and each round's per-point searches spread over nthreads, the
+    per-component bounds shared without synchronisation (boruvka.py:300-331
+    under prange).  Tie choices and row order can differ from ``boruvka``;
+    for timing the CPU path, never as a parity reference."""
+    n = tree.data.shape[0]
+    kq = min_samples + 1 if min_samples > 1 else 2
+    m = max(n - 1, 0) + 1
+    src = np.empty(m, np.int64)
+    dst = np.empty(m, np.int64)
+    w = np.empty(m, np.float32)
+    core = np.empty(n, np.float32)
+    ki = np.empty((n, kq), np.int32)
+    kd = np.empty((n, kq), np.float32)
+    rounds = ctypes.c_int32(0)
+    cnt = lib().orc_boruvka_racy(*tree._args(), int(min_samples), src, dst, w, core, ki, kd,
+                                 ctypes.byref(rounds), _threads(nthreads))
     return BoruvkaResult(src[:cnt], dst[:cnt], w[:cnt], core, ki, kd, int(rounds.value))
 
 

@@ -342,3 +342,15 @@ def test_weighted_branch_label_weights_spanning():
     r = O.boruvka_weighted(O.build_kdtree(X), 10, y.astype(np.float32))
     assert r.src.shape[0] == 199
     assert _spanning(200, r.src, r.dst)
+
+
+def test_racy_boruvka_keeps_the_weights():
+    # the multi-threaded timing variant may pick other edges among exact
+    # ties, but core distances and the MST weight multiset are fixed
+    X = workloads.blobs_small(20_000, seed=5)
+    t = O.build_kdtree(X)
+    a = O.boruvka(t, 10, nthreads=1)
+    b = O.boruvka_racy(t, 10, nthreads=4)
+    np.testing.assert_array_equal(a.core_sq, b.core_sq)
+    np.testing.assert_array_equal(np.sort(a.w_sq), np.sort(b.w_sq))
+    assert b.src.shape[0] == X.shape[0] - 1
