@@ -243,7 +243,10 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
         FH_HIP(fh::tree_level_axis(a, l, internal ? 1 : 0, st));
         if (!internal) break;
         FH_HIP(fh::tree_level_keys(a, l, maxs, 0, st));
-        if (maxs > small) {
+        if (maxs > small && maxs <= fh::tree_medium_limit()) {
+            // one work-group per node runs the steps down to an LDS-sized range
+            FH_HIP(fh::tree_medium_select(a, count, st));
+        } else if (maxs > small) {
             // multi-block steps until every node's active range fits in LDS
             for (;;) {
                 FH_HIP(hipMemsetAsync(a.active, 0, sizeof(int) * kBatch, st));

@@ -35,12 +35,16 @@ struct TreeBuildArgs {
 };
 
 int tree_small_limit();
+int tree_medium_limit();
 int tree_chunk();
 hipError_t tree_identity_keys(int n, uint64_t *A, hipStream_t st);
 hipError_t tree_level_bounds(const TreeBuildArgs &a, int level, int max_size, int identity, hipStream_t st);
 hipError_t tree_level_axis(const TreeBuildArgs &a, int level, int internal, hipStream_t st);
 hipError_t tree_level_keys(const TreeBuildArgs &a, int level, int max_size, int identity, hipStream_t st);
 hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStream_t st);
+// steps of every SEG_LARGE node by one work-group each; leaves SEG_SMALL or
+// SEG_DONE (for levels whose nodes are at most tree_medium_limit() long)
+hipError_t tree_medium_select(const TreeBuildArgs &a, int count, hipStream_t st);
 hipError_t tree_small_select(const TreeBuildArgs &a, int count, hipStream_t st);
 hipError_t tree_sort_range(uint64_t *A, int L, int R, hipStream_t st);
 hipError_t tree_finalize(const TreeBuildArgs &a, int n, int *perm, int *posof, float *xp, hipStream_t st);

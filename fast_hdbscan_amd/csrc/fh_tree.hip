@@ -24,8 +24,11 @@
 // integer comparison is the reference's (value, index) comparison
 // (numba_kdtree.py:164-179) and the low word doubles as the permutation.
 //
-// Nodes larger than kSmall elements run their steps as multi-block kernels
-// over global memory; smaller ones finish inside one work-group in LDS.
+// Nodes larger than kMedium elements run their steps as multi-block kernels
+// over global memory; nodes of a level whose largest node is at most kMedium
+// run them in one work-group per node over global memory (no launches or
+// host round trips per step); either way a range of at most kSmall elements
+// finishes inside one work-group in LDS.
 #include <hip/hip_runtime.h>
 
 #include "fh_tree.h"
@@ -35,6 +38,7 @@ namespace fh {
 namespace {
 
 constexpr int kSmall = 4096;    // largest range finished in LDS
+constexpr int kMedium = 1 << 16;  // largest node whose steps one work-group runs
 constexpr int kChunk = 4096;    // elements per block in the multi-block steps
 constexpr int kTB = 256;        // threads per block
 constexpr int kPer = kChunk / kTB;
@@ -357,7 +361,7 @@ __device__ void lds_sort(uint64_t *sk, int n) {
     int m = 1;
     while (m < n) m <<= 1;
     for (int size = 2; size <= m; size <<= 1) {
-        for (int t = threadIdx.x; t < m / 2; t += kTB) {
+        for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
             const int half = size >> 1;
             const int blk = t / half, off = t % half;
             const int lo = blk * size + off;
@@ -370,7 +374,7 @@ __device__ void lds_sort(uint64_t *sk, int n) {
         }
         __syncthreads();
         for (int stride = size >> 2; stride > 0; stride >>= 1) {
-            for (int t = threadIdx.x; t < m / 2; t += kTB) {
+            for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
                 const int lo = 2 * stride * (t / stride) + (t % stride);
                 const int hi = lo + stride;
                 if (hi < n && sk[lo] > sk[hi]) {
@@ -471,6 +475,130 @@ __global__ __launch_bounds__(kTB) void small_select_kernel(int count, SegState *
     seg[k].flag = SEG_DONE;
 }
 
+// ------------------------------------- one work-group per node ----
+
+constexpr int kMTB = 1024;  // threads of the one-work-group-per-node steps
+constexpr int kMB = 8;      // keys each thread loads at once
+
+// Block-wide exclusive scan for NT threads; total returned by reference.
+template <int NT>
+__device__ __forceinline__ int block_excl_nt(int v, int &total, int *ws) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = wave_incl(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+        int s = ws[i];
+        off += i < w ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - v;
+}
+
+// One partition step of introselect on keys[L, R) (pivot of numba_kdtree.py:
+// 280-299, the two-pointer exchanges of 313-350 as one parallel exchange,
+// the pivot move of 349) by one work-group of kMTB threads, keys in global
+// memory, each thread over a contiguous slice read kMB keys at a time;
+// gtpos / ltpos hold the exchange partners (indices L .. R-1 of the scratch
+// arrays the multi-block steps use).  Returns the pivot's final position P.
+__device__ int wg_select_step(uint64_t *keys, int *gtpos, int *ltpos, int L, int R, int *ws, int *sh) {
+    if (threadIdx.x == 0) median3_and_move(keys, L, R);
+    __syncthreads();
+    const uint64_t pivot = keys[R - 1];
+    const int len = R - 1 - L;
+    const int per = (len + kMTB - 1) / kMTB;
+    const int b = L + threadIdx.x * per;
+    const int e = min(b + per, R - 1);
+    int v = 0;
+    for (int p0 = b; p0 < e; p0 += kMB) {
+        uint64_t kv[kMB];
+#pragma unroll
+        for (int j = 0; j < kMB; ++j) kv[j] = p0 + j < e ? keys[p0 + j] : ~0ull;  // ~0 exceeds every key
+#pragma unroll
+        for (int j = 0; j < kMB; ++j) v += kv[j] < pivot ? 1 : 0;
+    }
+    int S;
+    int before = block_excl_nt<kMTB>(v, S, ws);
+    const int P = L + S;
+    if (threadIdx.x == 0) sh[0] = 0;
+    __syncthreads();
+    for (int p0 = b; p0 < e; p0 += kMB) {
+        uint64_t kv[kMB];
+#pragma unroll
+        for (int j = 0; j < kMB; ++j) kv[j] = p0 + j < e ? keys[p0 + j] : ~0ull;
+#pragma unroll
+        for (int j = 0; j < kMB; ++j) {
+            const int p = p0 + j;
+            if (p < e) {
+                const bool lt = kv[j] < pivot;
+                if (p == P) sh[0] = (P - L) - before;
+                if (p < P) {
+                    if (!lt) gtpos[L + (p - L) - before] = p;
+                } else if (lt) {
+                    ltpos[L + (S - 1 - before)] = p;
+                }
+                before += lt ? 1 : 0;
+            }
+        }
+    }
+    __syncthreads();
+    const int M = sh[0];
+    for (int m = threadIdx.x; m < M; m += kMTB) swap64(keys, gtpos[L + m], ltpos[L + m]);
+    __syncthreads();
+    if (threadIdx.x == 0) swap64(keys, P, R - 1);
+    __syncthreads();
+    return P;
+}
+
+// One block per node of a level whose nodes are at most kMedium long: the
+// steps over global memory while the active range is longer than kSmall
+// (a range reaching the depth limit first is sorted in place, as the
+// reference heap-sorts it); small_select_kernel then finishes in LDS.
+__global__ __launch_bounds__(kMTB) void medium_select_kernel(int count, SegState *seg, uint64_t *A, int *gtpos,
+                                                             int *ltpos) {
+    __shared__ int ws[kMTB / 64];
+    __shared__ int sh[4];
+    const int k = blockIdx.x;
+    if (k >= count) return;
+    SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    int L = st.L, R = st.R, depth = st.depth;
+    const int nth = st.nth;
+    bool done = false;
+    while (R - L > kSmall) {
+        if (depth == 0) {
+            lds_sort(A + L, R - L);  // the same network, here over global memory
+            done = true;
+            break;
+        }
+        --depth;
+        const int P = wg_select_step(A, gtpos, ltpos, L, R, ws, sh);
+        if (nth < P) {
+            R = P;
+        } else if (nth > P) {
+            L = P + 1;
+        } else {
+            done = true;
+            break;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (done) {
+            seg[k].flag = SEG_DONE;
+        } else {
+            seg[k].L = L;
+            seg[k].R = R;
+            seg[k].depth = depth;
+            seg[k].flag = SEG_SMALL;
+        }
+    }
+}
+
 // ----------------------------------------------------------- output ----
 
 __global__ void finalize_kernel(int n, int d, const float *__restrict__ X, const uint64_t *__restrict__ A, int *perm,
@@ -516,6 +644,7 @@ inline unsigned nblk(int64_t m, int b) { return (unsigned)((m + b - 1) / b); }
 }  // namespace
 
 int tree_small_limit() { return kSmall; }
+int tree_medium_limit() { return kMedium; }
 int tree_chunk() { return kChunk; }
 
 hipError_t tree_level_bounds(const TreeBuildArgs &a, int level, int max_size, int identity, hipStream_t st) {
@@ -554,6 +683,12 @@ hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStr
     hipLaunchKernelGGL(big_swap_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
                        a.gtpos, a.ltpos);
     hipLaunchKernelGGL(big_finish_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A);
+    return hipGetLastError();
+}
+
+hipError_t tree_medium_select(const TreeBuildArgs &a, int count, hipStream_t st) {
+    hipLaunchKernelGGL(medium_select_kernel, dim3((unsigned)count), dim3(kMTB), 0, st, count, a.seg, a.A, a.gtpos,
+                       a.ltpos);
     return hipGetLastError();
 }
 
