@@ -105,7 +105,7 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
 // Same tree from a union-find over the points (union by size, path
 // halving) that records, per set, the merge node currently representing
 // it: the node linkage_merge_find would return.  Upcoming edges' entries
-// are prefetched.
+// and their parents' entries are prefetched.
 // With weights (cluster_trees.py:98-127) each set also carries its float32
 // weighted size, summed in the reference's order (size[l] + size[r]).
 // Returns false for a -1 endpoint with weights (not supported).
@@ -141,11 +141,17 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         }
         return x;
     };
-    constexpr int64_t kAhead = 24;
+    // two prefetch stages: the endpoints' entries kAhead edges ahead, then
+    // (those entries having arrived) their parents' entries kAhead2 ahead
+    constexpr int64_t kAhead = 32, kAhead2 = 16;
     for (int64_t i = 0; i < m; ++i) {
         if (i + kAhead < m) {
             prefetch(&uf[src[i + kAhead]]);
             prefetch(&uf[dst[i + kAhead]]);
+        }
+        if (i + kAhead2 < m) {
+            prefetch(&uf[uf[src[i + kAhead2]].parent]);
+            prefetch(&uf[uf[dst[i + kAhead2]].parent]);
         }
         const int32_t a = find(src[i]);
         const int32_t b = find(dst[i]);
@@ -241,12 +247,12 @@ void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_met
     static thread_local std::vector<Task> tasks_tl;
     std::vector<Task> &tasks = tasks_tl;
     tasks.clear();
-    auto eliminate = [&](int32_t branch, int64_t parent_node, double lam) {
+    // sz: the branch's point count (its parent's row holds it)
+    auto eliminate = [&](int32_t branch, int32_t sz, int64_t parent_node, double lam) {
         if (branch < num_points) {
             row(parent_node, branch, lam, 1.0f);
             return;
         }
-        const int32_t sz = rows[branch - num_points].left_size + rows[branch - num_points].right_size;
         tasks.push_back({branch, static_cast<int32_t>(parent_node), static_cast<float>(lam), nrows});
         nrows += sz;
     };
@@ -284,15 +290,15 @@ void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_met
             push(right);
         } else if (dl < dm && dr >= dm) {
             RL(right, static_cast<int32_t>(parent_node));
-            eliminate(left, parent_node, lam);
+            eliminate(left, r.left_size, parent_node, lam);
             push(right);
         } else if (dl >= dm && dr < dm) {
             RL(left, static_cast<int32_t>(parent_node));
-            eliminate(right, parent_node, lam);
+            eliminate(right, r.right_size, parent_node, lam);
             push(left);
         } else if (dl < dm && dr < dm) {
-            eliminate(left, parent_node, lam);
-            eliminate(right, parent_node, lam);
+            eliminate(left, r.left_size, parent_node, lam);
+            eliminate(right, r.right_size, parent_node, lam);
         } else {
             RL(left, static_cast<int32_t>(next_label));
             row(parent_node, next_label, lam, lc);
