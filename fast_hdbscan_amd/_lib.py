@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "fh_build_kdtree",
     "fh_labels_from_sorted_mst",
     "fh_sort_mst_device",
+    "fh_edges_to_tree_order",
     "fh_neighbor_columns",
     "fh_cluster_layers",
     "fh_layer_cluster_tree",
@@ -167,7 +168,7 @@ def load() -> ctypes.CDLL:
         L.fh_build_kdtree.argtypes = [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fh_labels_from_sorted_mst.restype = ctypes.c_int
         L.fh_labels_from_sorted_mst.argtypes = [_i64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(FHLabelParams), _vp,
-                                                _vp, _vp, _vp, _vp]
+                                                _vp, _vp, _vp, _vp, _vp]
         L.fh_neighbor_columns.restype = _i64
         L.fh_neighbor_columns.argtypes = [_i64, _i32, _vp]
         L.fh_cluster_layers.restype = ctypes.c_int
@@ -189,6 +190,8 @@ def load() -> ctypes.CDLL:
         L.fh_hdbscan_batch.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _i64, _i32, _vp, _vp, _vp]
         L.fh_sort_mst_device.restype = ctypes.c_int
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.fh_edges_to_tree_order.restype = ctypes.c_int
+        L.fh_edges_to_tree_order.argtypes = [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 

@@ -100,6 +100,7 @@ struct fh_ctx {
     DevBuf<int> sv1, sv1s;
     DevBuf<unsigned char> stemp;
     int64_t geom_n = -1;            // n the uploaded node ranges belong to
+    int64_t tree_n = -1;            // n of the last tree built (perm, posof)
     std::vector<int2> geom_range;   // host copy of the node ranges
     std::vector<int> geom_maxsize;  // largest node per level
     int geom_levels = 0;
@@ -269,6 +270,7 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
         FH_HIP(fh::tree_small_select(a, count, st));
     }
     FH_HIP(fh::tree_finalize(a, ni, ctx->perm.p, ctx->posof.p, ctx->xp.p, st));
+    ctx->tree_n = n;
     if (levels_out) *levels_out = L;
     if (nodes_out) *nodes_out = N;
     return FH_OK;
@@ -1023,6 +1025,20 @@ int fh_build_kdtree(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int64_t *
     return FH_OK;
 }
 
+int fh_edges_to_tree_order(fh_ctx *ctx, int64_t n, int64_t m, const int32_t *src, const int32_t *dst,
+                           int32_t *pos_src, int32_t *pos_dst, int32_t *order) {
+    if (!ctx) return FH_EINVAL;
+    if (n < 1 || n != ctx->tree_n) return fail(ctx, FH_EINVAL, "n is not the point count of the context's last tree");
+    if (m < 0 || m > n) return fail(ctx, FH_EINVAL, "bad row count");
+    if (m > 0 && (!src || !dst || !pos_src || !pos_dst)) return fail(ctx, FH_EINVAL, "null pointer");
+    FH_HIP(hipSetDevice(ctx->device));
+    FH_HIP(fh::tree_positions(static_cast<int>(m), src, dst, ctx->posof.p, pos_src, pos_dst, ctx->stream));
+    if (order)
+        FH_HIP(hipMemcpyAsync(order, ctx->perm.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
 int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t *dst, const float *w_sq,
                        int32_t *src_out, int32_t *dst_out, float *w_out) {
     if (!ctx) return FH_EINVAL;
@@ -1132,7 +1148,7 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
                     w2[r] = w[order[r]];
                 }
                 rc = fh_labels_from_sorted_mst(ni, s2.data(), d2.data(), w2.data(), nullptr, nullptr, &prm, labels[i],
-                                               probabilities[i], nullptr, nullptr, nullptr);
+                                               probabilities[i], nullptr, nullptr, nullptr, nullptr);
                 if (rc != FH_OK) {
                     err = rc;
                     break;

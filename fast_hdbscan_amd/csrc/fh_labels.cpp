@@ -108,20 +108,34 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
 // and their parents' entries are prefetched.
 // With weights (cluster_trees.py:98-127) each set also carries its float32
 // weighted size, summed in the reference's order (size[l] + size[r]).
+// order (or nullptr): the endpoints are positions p of some point order and
+// order[p] is the point's id.  The union-find then runs over positions (a
+// spatial order makes an edge's two endpoints, and consecutive finds,
+// share cache lines); which set becomes the root is invisible in the rows.
 // Returns false for a -1 endpoint with weights (not supported).
 bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, const double *dist,
-                  Hierarchy &h, const float *weights = nullptr) {
+                  Hierarchy &h, const float *weights = nullptr, const int32_t *order = nullptr) {
     const int64_t m = n - 1;
     h.weighted = weights != nullptr;
     for (int64_t i = 0; i < m; ++i)
         if (src[i] < 0 || dst[i] < 0) {
             if (weights) return false;
-            linkage_tree_literal(n, src, dst, w_sq, dist, h);
+            if (!order) {
+                linkage_tree_literal(n, src, dst, w_sq, dist, h);
+                return true;
+            }
+            std::vector<int32_t> s2(m), d2(m);
+            for (int64_t j = 0; j < m; ++j) {
+                s2[j] = src[j] < 0 ? src[j] : order[src[j]];
+                d2[j] = dst[j] < 0 ? dst[j] : order[dst[j]];
+            }
+            linkage_tree_literal(n, s2.data(), d2.data(), w_sq, dist, h);
             return true;
         }
     static thread_local std::vector<float> wsz;
     if (weights) {
-        wsz.assign(weights, weights + n);
+        wsz.resize(n);
+        for (int64_t i = 0; i < n; ++i) wsz[i] = weights[order ? order[i] : i];
         h.lw.resize(m);
         h.rw.resize(m);
     }
@@ -131,7 +145,7 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
     // kept between calls: repeated calls of similar size touch no new pages
     static thread_local std::vector<Set> uf;
     if (uf.size() < static_cast<size_t>(n)) uf.resize(n);
-    for (int64_t i = 0; i < n; ++i) uf[i] = {static_cast<int32_t>(i), 1, static_cast<int32_t>(i), 0};
+    for (int64_t i = 0; i < n; ++i) uf[i] = {static_cast<int32_t>(i), 1, order ? order[i] : static_cast<int32_t>(i), 0};
     if (h.rows.size() < static_cast<size_t>(m)) h.rows.resize(m);
     auto find = [&](int32_t x) {
         while (uf[x].parent != x) {
@@ -838,7 +852,8 @@ void binary_search_for_n_clusters(Hierarchy &h, int64_t n, int64_t approx, int64
 extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
                                          const double *dist, const float *sample_weights,
                                          const fh_label_params *params, int64_t *labels, float *probabilities,
-                                         double *linkage, fh_condensed_tree *condensed, double *phase_ms) {
+                                         double *linkage, fh_condensed_tree *condensed, double *phase_ms,
+                                         const int32_t *point_order) {
     using clk = std::chrono::steady_clock;
     auto ms_since = [](clk::time_point a) {
         return std::chrono::duration<double, std::milli>(clk::now() - a).count();
@@ -857,7 +872,8 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         // reused between calls on this thread (no fresh pages per call)
         static thread_local Hierarchy h;
         clk::time_point t = clk::now();
-        if (!linkage_tree(n, src, dst, dist ? nullptr : w_sq, dist, h, sample_weights)) return FH_EUNSUPPORTED;
+        if (!linkage_tree(n, src, dst, dist ? nullptr : w_sq, dist, h, sample_weights, point_order))
+            return FH_EUNSUPPORTED;
         if (phase_ms) phase_ms[0] = ms_since(t);
         if (linkage) {
             for (int64_t i = 0; i < n - 1; ++i) {

@@ -51,6 +51,10 @@ struct KnnEdgesArgs {
     size_t temp_bytes;
 };
 size_t knn_edges_temp_bytes(int64_t m);
+
+// pos_src[i] = posof[src[i]], pos_dst[i] = posof[dst[i]] (negative ids kept)
+hipError_t tree_positions(int m, const int *src, const int *dst, const int *posof, int *pos_src, int *pos_dst,
+                          hipStream_t st);
 hipError_t knn_edges_sorted(const KnnEdgesArgs &a, hipStream_t st);
 
 }  // namespace fh

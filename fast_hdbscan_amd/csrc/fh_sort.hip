@@ -145,4 +145,23 @@ hipError_t sort_mst(const SortMstArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
+namespace {
+__global__ void tree_positions_kernel(int m, const int *__restrict__ src, const int *__restrict__ dst,
+                                      const int *__restrict__ posof, int *pos_src, int *pos_dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int s = src[i], t = dst[i];
+    pos_src[i] = s < 0 ? s : posof[s];
+    pos_dst[i] = t < 0 ? t : posof[t];
+}
+}  // namespace
+
+hipError_t tree_positions(int m, const int *src, const int *dst, const int *posof, int *pos_src, int *pos_dst,
+                          hipStream_t st) {
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tree_positions_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m, src, dst, posof,
+                       pos_src, pos_dst);
+    return hipGetLastError();
+}
+
 }  // namespace fh

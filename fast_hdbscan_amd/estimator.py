@@ -213,31 +213,36 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
     src, dst, w_sq, core_sq, nbr = mst_device(X, min_samples, stats=stats, out=w.get("mst"))
     m = n - 1
     dev = X.device
-    if "sorted" not in w or w["sorted"][0].shape[0] < m:
-        w["sorted"] = (torch.empty(max(m, 1), dtype=torch.int32, device=dev),
-                       torch.empty(max(m, 1), dtype=torch.int32, device=dev),
-                       torch.empty(max(m, 1), dtype=torch.float32, device=dev))
-        w["host"] = (torch.empty(max(m, 1), dtype=torch.int32).pin_memory(),
-                     torch.empty(max(m, 1), dtype=torch.int32).pin_memory(),
-                     torch.empty(max(m, 1), dtype=torch.float32).pin_memory())
-    ss, sd, sw = w["sorted"]
+    if "sorted" not in w or w["sorted"][0].shape[0] < m or w["order"][0].shape[0] < n:
+        w["sorted"] = tuple(torch.empty(max(m, 1), dtype=t, device=dev)
+                            for t in (torch.int32, torch.int32, torch.float32, torch.int32, torch.int32))
+        w["host"] = tuple(torch.empty(max(m, 1), dtype=t).pin_memory() for t in (torch.int32, torch.int32, torch.float32))
+        w["order"] = (torch.empty(max(n, 1), dtype=torch.int32, device=dev),
+                      torch.empty(max(n, 1), dtype=torch.int32).pin_memory())
+    ss, sd, sw, ps, pd = w["sorted"]
+    od, oh = w["order"]
     t0 = time.perf_counter()
     ctx = _lib.context(dev.index or 0)
+    L = _lib.load()
+    vp = ctypes.c_void_p
     if m > 0:
-        code = _lib.load().fh_sort_mst_device(ctx, m, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
-                                              ctypes.c_void_p(w_sq.data_ptr()), ctypes.c_void_p(ss.data_ptr()),
-                                              ctypes.c_void_p(sd.data_ptr()), ctypes.c_void_p(sw.data_ptr()))
-        _lib.check(ctx, code)
+        # rows sorted on the device, then their endpoints as KD-tree
+        # positions (the host union-find runs faster in that order)
+        _lib.check(ctx, L.fh_sort_mst_device(ctx, m, vp(src.data_ptr()), vp(dst.data_ptr()), vp(w_sq.data_ptr()),
+                                             vp(ss.data_ptr()), vp(sd.data_ptr()), vp(sw.data_ptr())))
+        _lib.check(ctx, L.fh_edges_to_tree_order(ctx, n, m, vp(ss.data_ptr()), vp(sd.data_ptr()), vp(ps.data_ptr()),
+                                                 vp(pd.data_ptr()), vp(od.data_ptr())))
     hs, hd, hw = w["host"]
-    hs[:m].copy_(ss[:m])
-    hd[:m].copy_(sd[:m])
+    hs[:m].copy_(ps[:m])
+    hd[:m].copy_(pd[:m])
     hw[:m].copy_(sw[:m])
+    oh[:n].copy_(od[:n])
     t1 = time.perf_counter()
     params = label_params(min_cluster_size, cluster_selection_method, max_cluster_size, allow_single_cluster,
                           cluster_selection_epsilon)
     phases = {}
     labels, strengths, _, _ = labels_from_sorted_arrays(n, hs[:m].numpy(), hd[:m].numpy(), hw[:m].numpy(), params,
-                                                        phases=phases)
+                                                        phases=phases, point_order=oh[:n].numpy())
     t2 = time.perf_counter()
     if stats is not None:
         stats["t_sort_copy_ms"] = 1e3 * (t1 - t0)

@@ -27,17 +27,19 @@ def label_params(min_cluster_size=10, cluster_selection_method="eom", max_cluste
 
 
 def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, phases: dict | None = None,
-                              dist=None, sample_weights=None):
+                              dist=None, sample_weights=None, point_order=None):
     """Run the host label stage on MST rows already sorted by (weight,
     source, target): squared float32 weights w_sq, or float64 distances
-    dist (then w_sq may be None); float32 sample weights optional.  Returns
-    (labels int64, strengths float32, linkage or None, CondensedTree or
-    None)."""
+    dist (then w_sq may be None); float32 sample weights optional.  With
+    ``point_order`` (int32 [n]) src / dst are positions p in that order and
+    point_order[p] is the point id (same outputs).  Returns (labels int64,
+    strengths float32, linkage or None, CondensedTree or None)."""
     src = np.ascontiguousarray(src, dtype=np.int32)
     dst = np.ascontiguousarray(dst, dtype=np.int32)
     w_sq = None if w_sq is None else np.ascontiguousarray(w_sq, dtype=np.float32)
     dist = None if dist is None else np.ascontiguousarray(dist, dtype=np.float64)
     sw = None if sample_weights is None else np.ascontiguousarray(sample_weights, dtype=np.float32)
+    po = None if point_order is None else np.ascontiguousarray(point_order, dtype=np.int32)
     labels = np.empty(n, np.int64)
     strengths = np.empty(n, np.float32)
     linkage = np.empty((max(n - 1, 0), 4), np.float64) if return_trees else None
@@ -55,7 +57,7 @@ def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, pha
         _lib.ptr(dist) if dist is not None else None, _lib.ptr(sw) if sw is not None else None,
         ctypes.byref(params), _lib.ptr(labels), _lib.ptr(strengths),
         _lib.ptr(linkage) if linkage is not None else None, ctypes.byref(ctree) if ctree is not None else None,
-        _lib.ptr(ph))
+        _lib.ptr(ph), _lib.ptr(po) if po is not None else None)
     if code == _lib.FH_EINVAL:
         raise ValueError("invalid label-stage arguments")
     if code == _lib.FH_EUNSUPPORTED:

@@ -294,12 +294,15 @@ typedef struct {
  * labels [n] int64, probabilities [n] float32.  linkage [(n-1)*4] float64
  * (rows left, right, distance, size, as mst_to_linkage_tree), condensed
  * and phase_ms [4] (linkage, condense, selection, labels; milliseconds)
- * may be NULL.  No context: host-only.
+ * may be NULL.  point_order [n] or NULL: when given, src/dst hold positions
+ * p of a point order and point_order[p] is the point id (as
+ * fh_edges_to_tree_order returns them); outputs are unchanged, the
+ * union-find just runs over positions.  No context: host-only.
  */
 int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
                               const double *dist, const float *sample_weights, const fh_label_params *params,
                               int64_t *labels, float *probabilities, double *linkage, fh_condensed_tree *condensed,
-                              double *phase_ms);
+                              double *phase_ms, const int32_t *point_order);
 
 /*
  * Sort MST rows by (weight, source, target) on the device -- the lexsort
@@ -309,6 +312,17 @@ int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst,
  */
 int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t *dst, const float *w_sq,
                        int32_t *src_out, int32_t *dst_out, float *w_out);
+
+/*
+ * MST endpoints as positions in the KD-tree order of the context's last
+ * MST call (the tree's idx_array, numba_kdtree.py:478-521): pos_src[i] =
+ * position of src[i], likewise dst; order [n] = idx_array (point id at
+ * each position).  Feeds fh_labels_from_sorted_mst's point_order, whose
+ * union-find is faster over spatially ordered positions.  DEVICE pointers;
+ * n must be the point count of that call.
+ */
+int fh_edges_to_tree_order(fh_ctx *ctx, int64_t n, int64_t m, const int32_t *src, const int32_t *dst,
+                           int32_t *pos_src, int32_t *pos_dst, int32_t *order);
 
 /* ------------------------------------------------------------------ */
 /* PLSCAN layer stage (host)                                           */

@@ -189,3 +189,36 @@ def test_reference_sample_weight_known_answer():
     e, _, _ = O.compute_minimum_spanning_tree(X, 10, sample_weights=sw)
     labels = clusters_from_spanning_tree(e, min_cluster_size=10, sample_weights=sw)[0]
     assert len(set(labels)) - int(-1 in labels) == 2
+
+
+@pytest.mark.parametrize("case", ["plain", "weighted", "k_exceeds_n"])
+def test_point_order_changes_nothing(case):
+    # endpoints given as positions of a permutation (point_order[p] = id)
+    # must give the same labels, strengths, linkage and condensed tree
+    from fast_hdbscan_amd import labels as PL
+
+    rng = np.random.default_rng(11)
+    w = None
+    if case == "weighted":
+        e, w = _weighted_case(4, n=800)
+    elif case == "k_exceeds_n":
+        e, _, _ = O.compute_minimum_spanning_tree(rng.normal(size=(12, 2)), 15)  # -1 endpoints
+    else:
+        e, _, _ = O.compute_minimum_spanning_tree(workloads.blobs_small(3000, seed=2), 5)
+    n = e.shape[0] + 1
+    o = np.lexsort((e[:, 1], e[:, 0], e[:, 2]))
+    src, dst = e[o, 0].astype(np.int32), e[o, 1].astype(np.int32)
+    dist = e[o, 2].copy()
+    order = rng.permutation(n).astype(np.int32)
+    pos = np.empty(n, np.int32)
+    pos[order] = np.arange(n, dtype=np.int32)
+    ps = np.where(src < 0, src, pos[np.maximum(src, 0)]).astype(np.int32)
+    pd = np.where(dst < 0, dst, pos[np.maximum(dst, 0)]).astype(np.int32)
+    params = PL.label_params(5, "eom", np.inf, False, 0.0)
+    a = PL.labels_from_sorted_arrays(n, src, dst, None, params, return_trees=True, dist=dist, sample_weights=w)
+    b = PL.labels_from_sorted_arrays(n, ps, pd, None, params, return_trees=True, dist=dist, sample_weights=w,
+                                     point_order=order)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x, y)
+    for f in ("parent", "child", "lambda_val", "child_size"):
+        np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
