@@ -35,6 +35,7 @@ EXPORTED_SYMBOLS = (
     "fh_labels_from_sorted_mst",
     "fh_sort_mst_device",
     "fh_edges_to_tree_order",
+    "fh_linkage_plan",
     "fh_neighbor_columns",
     "fh_cluster_layers",
     "fh_layer_cluster_tree",
@@ -83,6 +84,18 @@ class FHLabelParams(ctypes.Structure):
         ("cluster_selection_epsilon", ctypes.c_double),
         ("selection_method", ctypes.c_int32),
         ("allow_single_cluster", ctypes.c_int32),
+    ]
+
+
+class FHLabelOrder(ctypes.Structure):
+    _fields_ = [
+        ("point_order", ctypes.c_void_p),
+        ("prefix_rows", ctypes.c_int64),
+        ("n_groups", ctypes.c_int64),
+        ("row_order", ctypes.c_void_p),
+        ("group_sizes", ctypes.c_void_p),
+        ("threads", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 
@@ -192,6 +205,8 @@ def load() -> ctypes.CDLL:
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fh_edges_to_tree_order.restype = ctypes.c_int
         L.fh_edges_to_tree_order.argtypes = [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]
+        L.fh_linkage_plan.restype = ctypes.c_int
+        L.fh_linkage_plan.argtypes = [_vp, _i64, _i64, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int64)]
         _lib = L
     return _lib
 

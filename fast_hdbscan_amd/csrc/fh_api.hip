@@ -98,6 +98,10 @@ struct fh_ctx {
     DevBuf<uint32_t> sk1, sk1s;
     DevBuf<uint64_t> sk2, sk2s;
     DevBuf<int> sv1, sv1s;
+    // row groups for the host linkage
+    DevBuf<uint32_t> pc_keys, pc_keys_s;
+    DevBuf<int> pc_parent, pc_vals, pc_vals_s, pc_counts, pc_runs;
+    DevBuf<unsigned char> pc_temp;
     DevBuf<unsigned char> stemp;
     int64_t geom_n = -1;            // n the uploaded node ranges belong to
     int64_t tree_n = -1;            // n of the last tree built (perm, posof)
@@ -863,6 +867,8 @@ void fh_destroy(fh_ctx *c) {
     release(c->nb_pos); release(c->cert_of_pos); release(c->nb_d); release(c->mincore); release(c->cert);
     release(c->sk1); release(c->sk1s); release(c->sk2); release(c->sk2s); release(c->sv1); release(c->sv1s);
     release(c->stemp);
+    release(c->pc_keys); release(c->pc_keys_s); release(c->pc_parent); release(c->pc_vals); release(c->pc_vals_s);
+    release(c->pc_counts); release(c->pc_runs); release(c->pc_temp);
     release(c->bx_xc); release(c->bx_nrm); release(c->bx_thr); release(c->bx_rowmin); release(c->bx_cand);
     release(c->bx_cnt); release(c->bx_flags); release(c->bx_rowarg); release(c->bx_rows); release(c->bx_small);
     release(c->bx_nmax);
@@ -1036,6 +1042,54 @@ int fh_edges_to_tree_order(fh_ctx *ctx, int64_t n, int64_t m, const int32_t *src
     if (order)
         FH_HIP(hipMemcpyAsync(order, ctx->perm.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
     FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
+int fh_linkage_plan(fh_ctx *ctx, int64_t n, int64_t k, const int32_t *src, const int32_t *dst, int32_t *row_order,
+                    int32_t *group_sizes, int64_t *n_groups) {
+    if (!ctx) return FH_EINVAL;
+    if (n < 2 || k < 0 || k > n - 1 || n > (int64_t(1) << 31) - 1) return fail(ctx, FH_EINVAL, "bad sizes");
+    if (!n_groups || (k > 0 && (!src || !dst || !row_order || !group_sizes)))
+        return fail(ctx, FH_EINVAL, "null pointer");
+    *n_groups = 0;
+    if (k == 0) return FH_OK;
+    FH_HIP(hipSetDevice(ctx->device));
+    const int ki = static_cast<int>(k);
+    FH_TRY(ensure(ctx, ctx->pc_parent, n));
+    FH_TRY(ensure(ctx, ctx->pc_keys, k));
+    FH_TRY(ensure(ctx, ctx->pc_keys_s, k));
+    FH_TRY(ensure(ctx, ctx->pc_vals, k));
+    FH_TRY(ensure(ctx, ctx->pc_vals_s, k));
+    FH_TRY(ensure(ctx, ctx->pc_counts, k));
+    FH_TRY(ensure(ctx, ctx->pc_runs, 1));
+    const size_t tb = fh::prefix_components_temp_bytes(ki);
+    if (tb == 0) return fail(ctx, FH_EDEVICE, "hipCUB temp size query failed");
+    FH_TRY(ensure(ctx, ctx->pc_temp, tb));
+    fh::PrefixComponentsArgs a{};
+    a.n = static_cast<int>(n);
+    a.k = ki;
+    a.src = src;
+    a.dst = dst;
+    a.parent = ctx->pc_parent.p;
+    a.keys = ctx->pc_keys.p;
+    a.keys_s = ctx->pc_keys_s.p;
+    a.vals = ctx->pc_vals.p;
+    a.vals_s = ctx->pc_vals_s.p;
+    a.counts = ctx->pc_counts.p;
+    a.n_runs = ctx->pc_runs.p;
+    a.temp = ctx->pc_temp.p;
+    a.temp_bytes = ctx->pc_temp.cap;
+    FH_HIP(fh::prefix_components(a, ctx->stream));
+    int runs = 0;
+    FH_HIP(hipMemcpyAsync(&runs, ctx->pc_runs.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FH_HIP(hipMemcpyAsync(row_order, ctx->pc_vals_s.p, sizeof(int32_t) * (size_t)k, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    if (runs < 1 || runs > ki) return fail(ctx, FH_EDEVICE, "bad group count");
+    FH_HIP(hipMemcpyAsync(group_sizes, ctx->pc_counts.p, sizeof(int32_t) * (size_t)runs, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    *n_groups = runs;
     return FH_OK;
 }
 

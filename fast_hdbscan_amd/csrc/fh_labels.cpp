@@ -14,6 +14,7 @@
 //   get_cluster_label_vector     cluster_trees.py:944-996
 //   get_point_membership_strength_vector  cluster_trees.py:999-1031
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -108,14 +109,22 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
 // and their parents' entries are prefetched.
 // With weights (cluster_trees.py:98-127) each set also carries its float32
 // weighted size, summed in the reference's order (size[l] + size[r]).
-// order (or nullptr): the endpoints are positions p of some point order and
-// order[p] is the point's id.  The union-find then runs over positions (a
-// spatial order makes an edge's two endpoints, and consecutive finds,
-// share cache lines); which set becomes the root is invisible in the rows.
+// ord (or nullptr) changes only how the rows are walked:
+//   point_order: the endpoints are positions p of some point order and
+//     point_order[p] is the point's id.  The union-find then runs over
+//     positions (a spatial order makes an edge's two endpoints, and
+//     consecutive finds, share cache lines); which set becomes the root is
+//     invisible in the rows.
+//   prefix_rows k: rows [0, k) grouped by the connected components of
+//     their forest.  Row i's entry depends only on the sets of its
+//     endpoints at step i, i.e. on the earlier rows of its component, and
+//     components touch disjoint union-find entries, so the groups run on
+//     separate threads (each in row order) before rows [k, m) run in order.
 // Returns false for a -1 endpoint with weights (not supported).
 bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, const double *dist,
-                  Hierarchy &h, const float *weights = nullptr, const int32_t *order = nullptr) {
+                  Hierarchy &h, const float *weights = nullptr, const fh_label_order *ord = nullptr) {
     const int64_t m = n - 1;
+    const int32_t *order = ord ? ord->point_order : nullptr;
     h.weighted = weights != nullptr;
     for (int64_t i = 0; i < m; ++i)
         if (src[i] < 0 || dst[i] < 0) {
@@ -132,10 +141,10 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
             linkage_tree_literal(n, s2.data(), d2.data(), w_sq, dist, h);
             return true;
         }
-    static thread_local std::vector<float> wsz;
+    static thread_local std::vector<float> wsz_tl;
     if (weights) {
-        wsz.resize(n);
-        for (int64_t i = 0; i < n; ++i) wsz[i] = weights[order ? order[i] : i];
+        wsz_tl.resize(n);
+        for (int64_t i = 0; i < n; ++i) wsz_tl[i] = weights[order ? order[i] : i];
         h.lw.resize(m);
         h.rw.resize(m);
     }
@@ -143,11 +152,17 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         int32_t parent, size, node, pad;
     };
     // kept between calls: repeated calls of similar size touch no new pages
-    static thread_local std::vector<Set> uf;
-    if (uf.size() < static_cast<size_t>(n)) uf.resize(n);
+    static thread_local std::vector<Set> uf_tl;
+    if (uf_tl.size() < static_cast<size_t>(n)) uf_tl.resize(n);
+    // (worker threads below must use the caller's instances)
+    Set *uf = uf_tl.data();
+    float *wsz = wsz_tl.data();
     for (int64_t i = 0; i < n; ++i) uf[i] = {static_cast<int32_t>(i), 1, order ? order[i] : static_cast<int32_t>(i), 0};
     if (h.rows.size() < static_cast<size_t>(m)) h.rows.resize(m);
-    auto find = [&](int32_t x) {
+    Merge *rows = h.rows.data();
+    float *lw = weights ? h.lw.data() : nullptr;
+    float *rw = weights ? h.rw.data() : nullptr;
+    auto find = [uf](int32_t x) {
         while (uf[x].parent != x) {
             const int32_t g = uf[uf[x].parent].parent;
             uf[x].parent = g;
@@ -155,23 +170,12 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         }
         return x;
     };
-    // two prefetch stages: the endpoints' entries kAhead edges ahead, then
-    // (those entries having arrived) their parents' entries kAhead2 ahead
-    constexpr int64_t kAhead = 32, kAhead2 = 16;
-    for (int64_t i = 0; i < m; ++i) {
-        if (i + kAhead < m) {
-            prefetch(&uf[src[i + kAhead]]);
-            prefetch(&uf[dst[i + kAhead]]);
-        }
-        if (i + kAhead2 < m) {
-            prefetch(&uf[uf[src[i + kAhead2]].parent]);
-            prefetch(&uf[uf[dst[i + kAhead2]].parent]);
-        }
+    auto step = [&](int64_t i) {
         const int32_t a = find(src[i]);
         const int32_t b = find(dst[i]);
         const int32_t lc = uf[a].node, rc = uf[b].node;
         const int32_t ls = uf[a].size, rs = uf[b].size;
-        Merge &r = h.rows[i];
+        Merge &r = rows[i];
         const bool lbig = lc > rc;
         r.left = lbig ? lc : rc;
         r.right = lbig ? rc : lc;
@@ -188,10 +192,83 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         uf[root].node = static_cast<int32_t>(n + i);
         if (weights) {
             const float wa = wsz[a], wb = wsz[b];
-            h.lw[i] = lbig ? wa : wb;
-            h.rw[i] = lbig ? wb : wa;
+            lw[i] = lbig ? wa : wb;
+            rw[i] = lbig ? wb : wa;
             wsz[root] = wa + wb;
         }
+    };
+    // two prefetch stages: the endpoints' entries kAhead rows ahead, then
+    // (those entries having arrived) their parents' entries kAhead2 ahead
+    constexpr int64_t kAhead = 32, kAhead2 = 16;
+    auto prefetch_row = [&](int64_t i, int stage) {
+        if (stage == 0) {
+            prefetch(&uf[src[i]]);
+            prefetch(&uf[dst[i]]);
+        } else {
+            prefetch(&uf[uf[src[i]].parent]);
+            prefetch(&uf[uf[dst[i]].parent]);
+        }
+    };
+    int64_t first_seq = 0;
+    if (ord && ord->prefix_rows > 0 && ord->row_order && ord->group_sizes && ord->n_groups > 0 &&
+        ord->prefix_rows <= m) {
+        const int64_t k = ord->prefix_rows, G = ord->n_groups;
+        const int32_t *ro = ord->row_order;
+        std::vector<int64_t> start(G + 1, 0);
+        int64_t big = 0;
+        for (int64_t g = 0; g < G; ++g) {
+            start[g + 1] = start[g] + ord->group_sizes[g];
+            if (ord->group_sizes[g] > ord->group_sizes[big]) big = g;
+        }
+        if (start[G] != k) return false;
+        auto run_group = [&](int64_t g) {
+            const int64_t e = start[g + 1];
+            for (int64_t j = start[g]; j < e; ++j) {
+                if (j + kAhead < e) prefetch_row(ro[j + kAhead], 0);
+                if (j + kAhead2 < e) prefetch_row(ro[j + kAhead2], 1);
+                step(ro[j]);
+            }
+        };
+        int nt = ord->threads > 0 ? ord->threads : static_cast<int>(std::thread::hardware_concurrency());
+        nt = std::max(1, std::min(nt, 16));
+        if (nt == 1 || k < (1 << 15)) {
+            for (int64_t g = 0; g < G; ++g) run_group(g);
+        } else {
+            // the largest group on the calling thread; the others in chunks
+            // of about k / (16 nt) rows taken from a shared counter
+            std::vector<int64_t> cut{0};
+            const int64_t target = std::max<int64_t>(1, k / (16 * static_cast<int64_t>(nt)));
+            int64_t acc = 0;
+            for (int64_t g = 0; g < G; ++g) {
+                if (g != big) acc += ord->group_sizes[g];
+                if (acc >= target) {
+                    cut.push_back(g + 1);
+                    acc = 0;
+                }
+            }
+            if (cut.back() != G) cut.push_back(G);
+            const int64_t nchunks = static_cast<int64_t>(cut.size()) - 1;
+            std::atomic<int64_t> next{0};
+            auto worker = [&](bool take_big) {
+                if (take_big) run_group(big);
+                for (;;) {
+                    const int64_t c = next.fetch_add(1);
+                    if (c >= nchunks) break;
+                    for (int64_t g = cut[c]; g < cut[c + 1]; ++g)
+                        if (g != big) run_group(g);
+                }
+            };
+            std::vector<std::thread> th;
+            for (int t = 1; t < nt; ++t) th.emplace_back(worker, false);
+            worker(true);
+            for (auto &x : th) x.join();
+        }
+        first_seq = k;
+    }
+    for (int64_t i = first_seq; i < m; ++i) {
+        if (i + kAhead < m) prefetch_row(i + kAhead, 0);
+        if (i + kAhead2 < m) prefetch_row(i + kAhead2, 1);
+        step(i);
     }
     return true;
 }
@@ -853,7 +930,7 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
                                          const double *dist, const float *sample_weights,
                                          const fh_label_params *params, int64_t *labels, float *probabilities,
                                          double *linkage, fh_condensed_tree *condensed, double *phase_ms,
-                                         const int32_t *point_order) {
+                                         const fh_label_order *order) {
     using clk = std::chrono::steady_clock;
     auto ms_since = [](clk::time_point a) {
         return std::chrono::duration<double, std::milli>(clk::now() - a).count();
@@ -869,10 +946,20 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
             if (condensed) condensed->rows = 0;
             return FH_OK;
         }
+        if (order && order->prefix_rows > 0) {
+            // the groups must cover rows [0, k) with in-range indices
+            const int64_t k = order->prefix_rows;
+            if (k > n - 1 || order->n_groups < 1 || !order->row_order || !order->group_sizes) return FH_EINVAL;
+            int64_t tot = 0;
+            for (int64_t g = 0; g < order->n_groups; ++g) tot += order->group_sizes[g];
+            if (tot != k) return FH_EINVAL;
+            for (int64_t j = 0; j < k; ++j)
+                if (static_cast<uint64_t>(order->row_order[j]) >= static_cast<uint64_t>(k)) return FH_EINVAL;
+        }
         // reused between calls on this thread (no fresh pages per call)
         static thread_local Hierarchy h;
         clk::time_point t = clk::now();
-        if (!linkage_tree(n, src, dst, dist ? nullptr : w_sq, dist, h, sample_weights, point_order))
+        if (!linkage_tree(n, src, dst, dist ? nullptr : w_sq, dist, h, sample_weights, order))
             return FH_EUNSUPPORTED;
         if (phase_ms) phase_ms[0] = ms_since(t);
         if (linkage) {

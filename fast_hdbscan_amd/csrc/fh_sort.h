@@ -52,6 +52,27 @@ struct KnnEdgesArgs {
 };
 size_t knn_edges_temp_bytes(int64_t m);
 
+// Connected components of the forest of rows [0, k) (non-negative
+// endpoints < n): vals_s = row indices grouped by component, increasing
+// within a group (stable radix sort on the component's root); counts [runs]
+// = group sizes in the same order, n_runs [1].  keys is reused as the run
+// keys' output.
+struct PrefixComponentsArgs {
+    int n, k;
+    const int *src, *dst;
+    int *parent;        // [n] scratch
+    uint32_t *keys;     // [k] scratch
+    uint32_t *keys_s;   // [k] scratch
+    int *vals;          // [k] scratch
+    int *vals_s;        // [k] out
+    int *counts;        // [k] out
+    int *n_runs;        // [1] out
+    void *temp;
+    size_t temp_bytes;
+};
+size_t prefix_components_temp_bytes(int k);
+hipError_t prefix_components(const PrefixComponentsArgs &a, hipStream_t st);
+
 // pos_src[i] = posof[src[i]], pos_dst[i] = posof[dst[i]] (negative ids kept)
 hipError_t tree_positions(int m, const int *src, const int *dst, const int *posof, int *pos_src, int *pos_dst,
                           hipStream_t st);

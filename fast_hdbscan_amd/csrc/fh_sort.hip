@@ -164,4 +164,73 @@ hipError_t tree_positions(int m, const int *src, const int *dst, const int *poso
     return hipGetLastError();
 }
 
+namespace {
+__device__ __forceinline__ int pc_find(int *parent, int x) {
+    while (parent[x] != x) {
+        const int g = parent[parent[x]];
+        parent[x] = g;  // halving; races only shorten paths to the same root
+        x = g;
+    }
+    return x;
+}
+
+__global__ void pc_init_kernel(int n, int *parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) parent[i] = i;
+}
+
+// lock-free union to the smaller label
+__global__ void pc_unite_kernel(int k, const int *__restrict__ src, const int *__restrict__ dst, int *parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    int a = src[i], b = dst[i];
+    for (;;) {
+        a = pc_find(parent, a);
+        b = pc_find(parent, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(parent + a, a, b) == a) return;
+    }
+}
+
+__global__ void pc_keys_kernel(int k, const int *__restrict__ src, int *parent, uint32_t *keys, int *vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    keys[i] = static_cast<uint32_t>(pc_find(parent, src[i]));
+    vals[i] = i;
+}
+}  // namespace
+
+size_t prefix_components_temp_bytes(int k) {
+    size_t a = 0, b = 0;
+    hipError_t e1 = hipcub::DeviceRadixSort::SortPairs(nullptr, a, static_cast<uint32_t *>(nullptr),
+                                                       static_cast<uint32_t *>(nullptr), static_cast<int *>(nullptr),
+                                                       static_cast<int *>(nullptr), k, 0, 32);
+    hipError_t e2 = hipcub::DeviceRunLengthEncode::Encode(nullptr, b, static_cast<uint32_t *>(nullptr),
+                                                          static_cast<uint32_t *>(nullptr), static_cast<int *>(nullptr),
+                                                          static_cast<int *>(nullptr), k);
+    if (e1 != hipSuccess || e2 != hipSuccess) return 0;
+    return a > b ? a : b;
+}
+
+hipError_t prefix_components(const PrefixComponentsArgs &a, hipStream_t st) {
+    if (a.k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pc_init_kernel, dim3(nb(a.n)), dim3(256), 0, st, a.n, a.parent);
+    hipLaunchKernelGGL(pc_unite_kernel, dim3(nb(a.k)), dim3(256), 0, st, a.k, a.src, a.dst, a.parent);
+    hipLaunchKernelGGL(pc_keys_kernel, dim3(nb(a.k)), dim3(256), 0, st, a.k, a.src, a.parent, a.keys, a.vals);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int bits = 1;
+    while (bits < 32 && (1u << bits) < static_cast<uint32_t>(a.n)) ++bits;
+    size_t tb = a.temp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(a.temp, tb, a.keys, a.keys_s, a.vals, a.vals_s, a.k, 0, bits, st);
+    if (e != hipSuccess) return e;
+    tb = a.temp_bytes;
+    return hipcub::DeviceRunLengthEncode::Encode(a.temp, tb, a.keys_s, a.keys, a.counts, a.n_runs, a.k, st);
+}
+
 }  // namespace fh

@@ -232,6 +232,18 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
                                              vp(ss.data_ptr()), vp(sd.data_ptr()), vp(sw.data_ptr())))
         _lib.check(ctx, L.fh_edges_to_tree_order(ctx, n, m, vp(ss.data_ptr()), vp(sd.data_ptr()), vp(ps.data_ptr()),
                                                  vp(pd.data_ptr()), vp(od.data_ptr())))
+    # the lightest 90% of the rows, grouped by the components they form:
+    # the host linkage runs those groups on parallel threads
+    k = (9 * m) // 10 if m >= (1 << 16) else 0
+    groups = None
+    if k > 0:
+        if w.get("plan") is None or w["plan"][0].shape[0] < k:
+            w["plan"] = (np.empty(k, np.int32), np.empty(k, np.int32))
+        ro, gs = w["plan"]
+        ng = ctypes.c_int64(0)
+        _lib.check(ctx, L.fh_linkage_plan(ctx, n, k, vp(ps.data_ptr()), vp(pd.data_ptr()), _lib.ptr(ro), _lib.ptr(gs),
+                                          ctypes.byref(ng)))
+        groups = (k, ro[:k], gs[:ng.value])
     hs, hd, hw = w["host"]
     hs[:m].copy_(ps[:m])
     hd[:m].copy_(pd[:m])
@@ -242,7 +254,7 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
                           cluster_selection_epsilon)
     phases = {}
     labels, strengths, _, _ = labels_from_sorted_arrays(n, hs[:m].numpy(), hd[:m].numpy(), hw[:m].numpy(), params,
-                                                        phases=phases, point_order=oh[:n].numpy())
+                                                        phases=phases, point_order=oh[:n].numpy(), groups=groups)
     t2 = time.perf_counter()
     if stats is not None:
         stats["t_sort_copy_ms"] = 1e3 * (t1 - t0)

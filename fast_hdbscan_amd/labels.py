@@ -27,19 +27,31 @@ def label_params(min_cluster_size=10, cluster_selection_method="eom", max_cluste
 
 
 def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, phases: dict | None = None,
-                              dist=None, sample_weights=None, point_order=None):
+                              dist=None, sample_weights=None, point_order=None, groups=None, threads=0):
     """Run the host label stage on MST rows already sorted by (weight,
     source, target): squared float32 weights w_sq, or float64 distances
     dist (then w_sq may be None); float32 sample weights optional.  With
     ``point_order`` (int32 [n]) src / dst are positions p in that order and
-    point_order[p] is the point id (same outputs).  Returns (labels int64,
-    strengths float32, linkage or None, CondensedTree or None)."""
+    point_order[p] is the point id.  ``groups`` = (k, row_order int32 [k],
+    group_sizes int32 [G]) as fh_linkage_plan gives them lets the linkage
+    of rows [0, k) run on ``threads`` host threads.  Neither changes the
+    outputs.  Returns (labels int64, strengths float32, linkage or None,
+    CondensedTree or None)."""
     src = np.ascontiguousarray(src, dtype=np.int32)
     dst = np.ascontiguousarray(dst, dtype=np.int32)
     w_sq = None if w_sq is None else np.ascontiguousarray(w_sq, dtype=np.float32)
     dist = None if dist is None else np.ascontiguousarray(dist, dtype=np.float64)
     sw = None if sample_weights is None else np.ascontiguousarray(sample_weights, dtype=np.float32)
     po = None if point_order is None else np.ascontiguousarray(point_order, dtype=np.int32)
+    order = None
+    if po is not None or groups is not None:
+        order = _lib.FHLabelOrder(_lib.ptr(po) if po is not None else None, 0, 0, None, None, int(threads), 0)
+        if groups is not None:
+            k, ro, gs = groups
+            ro = np.ascontiguousarray(ro, dtype=np.int32)
+            gs = np.ascontiguousarray(gs, dtype=np.int32)
+            order.prefix_rows, order.n_groups = int(k), int(gs.shape[0])
+            order.row_order, order.group_sizes = _lib.ptr(ro), _lib.ptr(gs)
     labels = np.empty(n, np.int64)
     strengths = np.empty(n, np.float32)
     linkage = np.empty((max(n - 1, 0), 4), np.float64) if return_trees else None
@@ -57,7 +69,7 @@ def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, pha
         _lib.ptr(dist) if dist is not None else None, _lib.ptr(sw) if sw is not None else None,
         ctypes.byref(params), _lib.ptr(labels), _lib.ptr(strengths),
         _lib.ptr(linkage) if linkage is not None else None, ctypes.byref(ctree) if ctree is not None else None,
-        _lib.ptr(ph), _lib.ptr(po) if po is not None else None)
+        _lib.ptr(ph), ctypes.byref(order) if order is not None else None)
     if code == _lib.FH_EINVAL:
         raise ValueError("invalid label-stage arguments")
     if code == _lib.FH_EUNSUPPORTED:

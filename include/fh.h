@@ -277,6 +277,28 @@ typedef struct {
 } fh_condensed_tree;
 
 /*
+ * How the label stage may walk the rows; outputs never depend on it.
+ *   point_order [n] or NULL: src/dst hold positions p of a point order and
+ *     point_order[p] is the point id (fh_edges_to_tree_order); the
+ *     union-find then runs over positions.
+ *   prefix_rows k > 0: rows [0, k) form a forest whose connected
+ *     components are given by row_order [k] (row indices grouped by
+ *     component, increasing within a group) and group_sizes [n_groups]
+ *     (fh_linkage_plan).  A row's linkage entry depends only on the earlier
+ *     rows of its component, so the groups run on host threads (threads,
+ *     0 = all cores) and rows [k, n-1) follow in order.
+ */
+typedef struct {
+    const int32_t *point_order;
+    int64_t prefix_rows;
+    int64_t n_groups;
+    const int32_t *row_order;
+    const int32_t *group_sizes;
+    int32_t threads;
+    int32_t pad;
+} fh_label_order;
+
+/*
  * Labels and membership strengths from an MST whose rows are already
  * sorted by (weight, source, target).
  *   Replaces: hdbscan.py:521-574 (clusters_from_spanning_tree after its
@@ -294,15 +316,13 @@ typedef struct {
  * labels [n] int64, probabilities [n] float32.  linkage [(n-1)*4] float64
  * (rows left, right, distance, size, as mst_to_linkage_tree), condensed
  * and phase_ms [4] (linkage, condense, selection, labels; milliseconds)
- * may be NULL.  point_order [n] or NULL: when given, src/dst hold positions
- * p of a point order and point_order[p] is the point id (as
- * fh_edges_to_tree_order returns them); outputs are unchanged, the
- * union-find just runs over positions.  No context: host-only.
+ * may be NULL.  order may be NULL (see fh_label_order).  No context:
+ * host-only.
  */
 int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
                               const double *dist, const float *sample_weights, const fh_label_params *params,
                               int64_t *labels, float *probabilities, double *linkage, fh_condensed_tree *condensed,
-                              double *phase_ms, const int32_t *point_order);
+                              double *phase_ms, const fh_label_order *order);
 
 /*
  * Sort MST rows by (weight, source, target) on the device -- the lexsort
@@ -323,6 +343,16 @@ int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t
  */
 int fh_edges_to_tree_order(fh_ctx *ctx, int64_t n, int64_t m, const int32_t *src, const int32_t *dst,
                            int32_t *pos_src, int32_t *pos_dst, int32_t *order);
+
+/*
+ * Connected components of the forest of sorted MST rows [0, k) -- the
+ * groups of an fh_label_order; no reference counterpart: it partitions
+ * mst_to_linkage_tree's union-find, cluster_trees.py:66-95, into
+ * independent parts.  DEVICE src/dst (point ids or positions, >= 0),
+ * HOST outputs: row_order [k], group_sizes [k] (first *n_groups used).
+ */
+int fh_linkage_plan(fh_ctx *ctx, int64_t n, int64_t k, const int32_t *src, const int32_t *dst, int32_t *row_order,
+                    int32_t *group_sizes, int64_t *n_groups);
 
 /* ------------------------------------------------------------------ */
 /* PLSCAN layer stage (host)                                           */

@@ -135,3 +135,45 @@ def test_reference_sample_weight_estimator():
     np.testing.assert_array_equal(labels, ref[0])
     np.testing.assert_array_equal(p, ref[1])
     np.testing.assert_array_equal(HDBSCAN(min_cluster_size=10).fit(X, sample_weight=sw).labels_, labels)
+
+
+def test_linkage_plan_matches_scipy_components():
+    import ctypes
+
+    import torch
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    from fast_hdbscan_amd import _lib
+
+    rng = np.random.default_rng(5)
+    n = 150_000
+    # a random spanning tree in random row order
+    parent = np.array([rng.integers(0, i) for i in range(1, n)], np.int32)
+    src = np.arange(1, n, dtype=np.int32)
+    dst = parent
+    p = rng.permutation(n - 1)
+    src, dst = src[p].copy(), dst[p].copy()
+    k = 100_000
+    print(f"linkage plan n={n} k={k}", flush=True)
+    ctx = _lib.context(0)
+    ds, dd = torch.from_numpy(src).cuda(), torch.from_numpy(dst).cuda()
+    ro = np.empty(k, np.int32)
+    gs = np.empty(k, np.int32)
+    ng = ctypes.c_int64(0)
+    _lib.check(ctx, _lib.load().fh_linkage_plan(ctx, n, k, ctypes.c_void_p(ds.data_ptr()), ctypes.c_void_p(dd.data_ptr()),
+                                                _lib.ptr(ro), _lib.ptr(gs), ctypes.byref(ng)))
+    gs = gs[: ng.value]
+    _, lab = connected_components(coo_matrix((np.ones(k), (src[:k], dst[:k])), shape=(n, n)), directed=False)
+    comp = lab[src[:k]]
+    assert np.array_equal(np.sort(ro), np.arange(k))
+    starts = np.concatenate([[0], np.cumsum(gs)])
+    assert starts[-1] == k
+    seen = set()
+    for g in range(gs.shape[0]):
+        rows = ro[starts[g]:starts[g + 1]]
+        assert np.all(np.diff(rows) > 0)
+        c = np.unique(comp[rows])
+        assert c.shape[0] == 1 and int(c[0]) not in seen
+        seen.add(int(c[0]))
+    assert len(seen) == np.unique(comp).shape[0]

@@ -222,3 +222,51 @@ def test_point_order_changes_nothing(case):
         np.testing.assert_array_equal(x, y)
     for f in ("parent", "child", "lambda_val", "child_size"):
         np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
+
+
+@pytest.mark.parametrize("frac,threads", [(0.9, 4), (0.5, 16), (1.0, 3)])
+def test_component_groups_change_nothing(frac, threads):
+    # rows [0, k) grouped by the components of their forest (computed here
+    # with scipy) and linked on several threads: identical outputs
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    from fast_hdbscan_amd import labels as PL
+
+    X = workloads.anisotropic_mixture(120_000, 2, 12, 0.1, seed=9)
+    e, _, _ = O.compute_minimum_spanning_tree(X, 5)
+    n = e.shape[0] + 1
+    o = np.lexsort((e[:, 1], e[:, 0], e[:, 2]))
+    src, dst = e[o, 0].astype(np.int32), e[o, 1].astype(np.int32)
+    dist = e[o, 2].copy()
+    k = int((n - 1) * frac)
+    g = coo_matrix((np.ones(k), (src[:k], dst[:k])), shape=(n, n))
+    _, lab = connected_components(g, directed=False)
+    comp = lab[src[:k]]
+    ro = np.argsort(comp, kind="stable").astype(np.int32)
+    gs = np.bincount(comp)
+    gs = gs[gs > 0].astype(np.int32)  # sorted component ids: groups in the same order as ro
+    params = PL.label_params(20, "eom", np.inf, False, 0.0)
+    a = PL.labels_from_sorted_arrays(n, src, dst, None, params, return_trees=True, dist=dist)
+    b = PL.labels_from_sorted_arrays(n, src, dst, None, params, return_trees=True, dist=dist, groups=(k, ro, gs),
+                                     threads=threads)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x, y)
+    for f in ("parent", "child", "lambda_val", "child_size"):
+        np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
+
+
+def test_component_groups_are_checked():
+    from fast_hdbscan_amd import labels as PL
+
+    e, _, _ = O.compute_minimum_spanning_tree(workloads.blobs_small(500, seed=1), 5)
+    n = e.shape[0] + 1
+    o = np.lexsort((e[:, 1], e[:, 0], e[:, 2]))
+    src, dst = e[o, 0].astype(np.int32), e[o, 1].astype(np.int32)
+    params = PL.label_params(5, "eom", np.inf, False, 0.0)
+    with pytest.raises(ValueError):
+        PL.labels_from_sorted_arrays(n, src, dst, None, params, dist=e[o, 2].copy(),
+                                     groups=(10, np.arange(10, dtype=np.int32), np.array([4, 4], np.int32)))
+    with pytest.raises(ValueError):
+        PL.labels_from_sorted_arrays(n, src, dst, None, params, dist=e[o, 2].copy(),
+                                     groups=(3, np.array([0, 1, 7], np.int32), np.array([3], np.int32)))
