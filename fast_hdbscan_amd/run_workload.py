@@ -5,7 +5,9 @@ statistics -- the driver used for kernel profiles:
     rocprofv3 --kernel-trace --stats -d out -- python3 -m fast_hdbscan_amd.run_workload --workload scale2d
 
 Workloads (fast_hdbscan_amd.workloads): scale2d (2-D mixture, seed 1),
-scale3d (3-D mixture, seed 2), highdim (d = 64 Gaussians, seed 3), blobs.
+scale3d (3-D mixture, seed 2), highdim (d = 64 Gaussians, seed 3), blobs,
+batch (--n independent small datasets, default 512, through the batch
+scheduler with --workers threads; min_samples 5, min_cluster_size 20).
 """
 from __future__ import annotations
 
@@ -16,7 +18,8 @@ import time
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="scale2d", choices=["scale2d", "scale3d", "highdim", "blobs"])
+    ap.add_argument("--workload", default="scale2d", choices=["scale2d", "scale3d", "highdim", "blobs", "batch"])
+    ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--min-samples", type=int, default=None)
     ap.add_argument("--repeat", type=int, default=1)
@@ -27,6 +30,19 @@ def main(argv=None):
 
     from . import _lib, workloads
     from .hdbscan import mst_device
+
+    if args.workload == "batch":
+        from .batch import hdbscan_batch
+
+        data = workloads.batch_datasets(args.n or 512)
+        pts = sum(x.shape[0] for x in data)
+        for r in range(args.repeat):
+            t = time.perf_counter()
+            hdbscan_batch(data, min_samples=args.min_samples or 5, min_cluster_size=20, workers=args.workers)
+            el = time.perf_counter() - t
+            print(json.dumps({"workload": "batch", "datasets": len(data), "points": pts, "workers": args.workers,
+                              "wall_ms": 1e3 * el, "datasets_per_s": len(data) / el}), flush=True)
+        return
 
     if args.workload == "scale2d":
         X, ms = workloads.scale_2d(args.n or 2_000_000, seed=1), 10
