@@ -49,6 +49,7 @@ struct Hierarchy {
     // (create_linkage_merge_data_w_sample_weights, cluster_trees.py:29-35)
     bool weighted = false;
     std::vector<float> lw, rw;
+    bool literal = false;  // built by linkage_tree_literal (may hold -1 children)
 };
 
 inline void prefetch(const void *p) { __builtin_prefetch(p, 0, 1); }
@@ -71,6 +72,7 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
     std::vector<int64_t> size(total, 0);
     for (int64_t i = 0; i < n; ++i) size[i] = 1;
     h.rows.resize(m);
+    h.literal = true;
     int64_t next = n;
     // node values stay as given (-1 stays -1); only array accesses wrap
     auto P = [&](int64_t x) -> int64_t & { return parent[x < 0 ? x + total : x]; };
@@ -126,6 +128,7 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
     const int64_t m = n - 1;
     const int32_t *order = ord ? ord->point_order : nullptr;
     h.weighted = weights != nullptr;
+    h.literal = false;
     for (int64_t i = 0; i < m; ++i)
         if (src[i] < 0 || dst[i] < 0) {
             if (weights) return false;
@@ -273,11 +276,18 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
     return true;
 }
 
+// ids are below 2n; the label stage takes n < 2^30, so they fit int32
 struct Condensed {
-    std::vector<int64_t> parent, child;
+    std::vector<int32_t> parent, child;
     std::vector<float> lambda, size;
     int64_t rows = 0;
     int64_t next_label = 0;
+    int64_t root = 0;                   // the root cluster's id (n)
+    std::vector<int64_t> cluster_rows;  // rows whose child is a cluster id, ascending
+    // score_condensed_tree_nodes (cluster_trees.py:656-669) per id - root,
+    // filled by the chain form when asked for
+    bool has_scores = false;
+    std::vector<float> score;
 };
 
 // cluster_trees.py:205-352 (unit weights, max_cluster_size = inf: the
@@ -288,7 +298,7 @@ struct Condensed {
 // cluster_trees.py:192-197) or exp(-d) (plscan_lambda_value, 200-202).
 enum LambdaMethod { kLambdaHdbscan = 0, kLambdaPlscan = 1 };
 
-void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_method = kLambdaHdbscan) {
+void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_method) {
     const int64_t m = n - 1;
     const int32_t root = static_cast<int32_t>(2 * m);
     const int32_t num_points = static_cast<int32_t>(n);
@@ -316,8 +326,8 @@ void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_met
     int64_t nrows = 0;
     const Merge *rows = h.rows.data();
     auto row = [&](int64_t p, int64_t ch, double lam, float sz) {
-        c.parent[nrows] = p;
-        c.child[nrows] = ch;
+        c.parent[nrows] = static_cast<int32_t>(p);
+        c.child[nrows] = static_cast<int32_t>(ch);
         c.lambda[nrows] = static_cast<float>(lam);
         c.size[nrows] = sz;
         ++nrows;
@@ -351,6 +361,8 @@ void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_met
     // points and nodes inside eliminated branches (cluster_trees.py:228-230).
     // The nodes it does process are closed upwards, so a queue fed only with
     // the children that continue visits them in the same order.
+    c.cluster_rows.clear();
+    c.has_scores = false;
     static thread_local std::vector<int32_t> q;
     q.clear();
     q.reserve(static_cast<size_t>(n));
@@ -391,6 +403,8 @@ void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_met
             eliminate(left, r.left_size, parent_node, lam);
             eliminate(right, r.right_size, parent_node, lam);
         } else {
+            c.cluster_rows.push_back(nrows);
+            c.cluster_rows.push_back(nrows + 1);
             RL(left, static_cast<int32_t>(next_label));
             row(parent_node, next_label, lam, lc);
             ++next_label;
@@ -446,6 +460,422 @@ void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_met
     }
     c.rows = nrows;
     c.next_label = next_label;
+    c.root = num_points;
+}
+
+// The same rows as condense_walk, computed without walking the tree one
+// level at a time (the walk waits on memory once per level; a 2M-point
+// tree has ~300k levels of ~4 processed nodes).  Processed merge nodes form
+// a tree in which most nodes continue into one child only; a chain is a
+// maximal path of processed nodes from the root or a split's child down to
+// a split or to a node with no continuing child.  The walk's order is
+// breadth-first: by depth, then left to right.  Two chains present at one
+// depth lie in disjoint subtrees, so their left-to-right order is the
+// preorder (left child first) of the chains' tree.  Passes:
+//   1. merge rows in decreasing index order (a parent row has a larger
+//      index than its children): chain of each processed node, its row
+//      count and float32 lambda;
+//   2. tree of chains: preorder and the depth of each chain's head;
+//   3. counting sorts: nodes grouped by chain (top-down within a chain),
+//      then placed by depth, chains in preorder: the walk's order;
+//   4. in that order: each node's first row and the cluster ids of the
+//      splits (next_label counts splits in the walk's order);
+//   5. rows written in parallel from those offsets, eliminated branches
+//      expanded in place (breadth-first, as eliminate_branch does).
+// With want_scores, score_condensed_tree_nodes (cluster_trees.py:656-669)
+// is summed per chain: a cluster's rows are its chain's rows, top-down,
+// i.e. in row order, so each float32 sum runs in the reference's order.
+// phase timing hooks: empty unless a profiling build defines them
+#ifndef FH_TMARK
+#define FH_TMARK(x)
+#define FH_TSTART()
+#endif
+void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, int lambda_method, bool want_scores) {
+    FH_TSTART();
+    const int64_t m = n - 1;
+    const int32_t N = static_cast<int32_t>(n);
+    const Merge *rows = h.rows.data();
+    const double dm = static_cast<double>(mcs);
+    struct Node {
+        int32_t row;    // merge row (node N + row)
+        int32_t chain;
+        int32_t nr;     // rows this node writes; -2 for a split (two cluster rows)
+        float lam;      // float32 lambda of its rows
+    };
+    struct Chain {
+        int32_t parent;     // chain ending in the split above, -1 for the root's
+        int32_t len;
+        int32_t kid[2];     // chains headed by the split's merge children, -1: none
+        int32_t split_row;  // merge row of the split ending this chain, -1: none
+        int32_t pad;
+        int64_t start;        // depth of its head
+        int64_t label;        // the cluster id its rows hang from
+        int64_t split_label;  // the first cluster id its split creates
+        int64_t first;        // its nodes' offset in `grouped`
+    };
+    static thread_local std::vector<int32_t> st_tl;
+    static thread_local std::vector<Node> grouped_tl, bfs_tl;
+    static thread_local std::vector<Chain> chains_tl;
+    static thread_local std::vector<int64_t> roff_tl;
+    std::vector<int32_t> &st = st_tl;
+    std::vector<Node> &grouped = grouped_tl, &bfs = bfs_tl;
+    std::vector<Chain> &chains = chains_tl;
+    std::vector<int64_t> &roff = roff_tl;
+    auto side_size = [&](int64_t i, int s) -> float {
+        if (h.weighted) return s == 0 ? h.lw[i] : h.rw[i];
+        return static_cast<float>(s == 0 ? rows[i].left_size : rows[i].right_size);
+    };
+
+    // 1a. what each merge row does if processed (independent rows:
+    //     threads).  Each merge child learns its parent row and its role
+    //     there: dropped (eliminated), continuing the parent's chain, or
+    //     heading a new chain (left or right child of a split).
+    enum : int32_t { kDrop = 0, kSame = 1, kHeadL = 2, kHeadR = 3 };
+    struct Up {
+        int32_t parent;
+        int32_t role;
+    };
+    static thread_local std::vector<Up> up_tl;
+    static thread_local std::vector<int32_t> nr_tl;
+    static thread_local std::vector<float> lamv_tl;
+    std::vector<Up> &up = up_tl;
+    std::vector<int32_t> &nrv = nr_tl;
+    std::vector<float> &lamv = lamv_tl;
+    if (up.size() < static_cast<size_t>(m)) {
+        up.resize(m);
+        nrv.resize(m);
+        lamv.resize(m);
+    }
+    if (st.size() < static_cast<size_t>(m)) st.resize(m);
+    const int nt = m > (1 << 16) ? std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))) : 1;
+    auto classify = [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            const Merge &r = rows[i];
+            const double dl = side_size(i, 0), dr = side_size(i, 1);
+            const double d = r.delta;
+            const double lam = lambda_method == kLambdaHdbscan ? (d > 0.0 ? 1.0 / d : kInfD) : std::exp(-d);
+            int32_t nr, lrole = kDrop, rrole = kDrop;
+            if (r.left < N && dr >= dm) {
+                nr = 1;
+                rrole = kSame;
+            } else if (dl < dm && dr >= dm) {
+                nr = r.left_size;
+                rrole = kSame;
+            } else if (dl >= dm && dr < dm) {
+                nr = r.right_size;
+                lrole = kSame;
+            } else if (dl < dm && dr < dm) {
+                nr = r.left_size + r.right_size;
+            } else {
+                nr = -2;
+                lrole = kHeadL;
+                rrole = kHeadR;
+            }
+            if (r.left >= N) up[r.left - N] = {static_cast<int32_t>(i), lrole};
+            if (r.right >= N) up[r.right - N] = {static_cast<int32_t>(i), rrole};
+            nrv[i] = nr;
+            lamv[i] = static_cast<float>(lam);
+        }
+    };
+    if (nt == 1) {
+        classify(0, m);
+    } else {
+        std::vector<std::thread> th;
+        for (int k = 0; k < nt; ++k) th.emplace_back(classify, m * k / nt, m * (k + 1) / nt);
+        for (auto &x : th) x.join();
+    }
+    FH_TMARK("1a");
+
+    // 1b. chain of every row, top-down (a row's parent has a larger index);
+    //     -1: not processed.  Threads own row ranges.  In its range a thread
+    //     resolves dropped rows (-1) and rows continuing a parent of the same
+    //     range (the parent's value); the rest -- heads of new chains and
+    //     rows whose parent lies in another range -- are pending, held as
+    //     -2 - row, and resolved in decreasing row order on one thread (a
+    //     few thousand), after which the threads replace every reference.
+    chains.clear();
+    chains.push_back({-1, 0, {-1, -1}, -1, 0, 0, N, 0, 0});
+    std::vector<std::vector<int32_t>> pend(nt);
+    auto lo_of = [&](int k) { return m * k / nt; };
+    auto local = [&](int k) {
+        int32_t *stp = st.data();
+        const Up *upp = up.data();
+        const int64_t lo = lo_of(k), hi = lo_of(k + 1);
+        std::vector<int32_t> &pk = pend[k];
+        pk.clear();
+        for (int64_t i = hi - 1; i >= lo; --i) {
+            if (i == m - 1) {
+                stp[i] = 0;
+                continue;
+            }
+            const Up u = upp[i];
+            if (u.role == kDrop) {
+                stp[i] = -1;
+            } else if (u.role == kSame && u.parent < hi) {
+                stp[i] = stp[u.parent];
+            } else {
+                stp[i] = -2 - static_cast<int32_t>(i);
+                pk.push_back(static_cast<int32_t>(i));
+            }
+        }
+    };
+    auto resolve = [&](int k) {
+        int32_t *stp = st.data();
+        for (int64_t i = lo_of(k), e = lo_of(k + 1); i < e; ++i)
+            if (stp[i] <= -2) stp[i] = stp[-2 - stp[i]];
+    };
+    auto on_threads = [&](auto &&fn) {
+        if (nt == 1) {
+            fn(0);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (int k = 0; k < nt; ++k) th.emplace_back(fn, k);
+        for (auto &x : th) x.join();
+    };
+    on_threads(local);
+    {
+        int32_t *stp = st.data();
+        for (int k = nt - 1; k >= 0; --k)
+            for (const int32_t i : pend[k]) {
+                const Up u = up[i];
+                int32_t cp = stp[u.parent];
+                if (cp <= -2) cp = stp[-2 - cp];
+                int32_t ch = u.role == kSame ? cp : -1;
+                if (cp >= 0 && u.role >= kHeadL) {
+                    ch = static_cast<int32_t>(chains.size());
+                    chains[cp].split_row = u.parent;
+                    chains[cp].kid[u.role - kHeadL] = ch;
+                    chains.push_back({cp, 0, {-1, -1}, -1, 0, 0, 0, 0, 0});
+                }
+                stp[i] = ch;
+            }
+    }
+    on_threads(resolve);
+    FH_TMARK("1b");
+
+    // 2. nodes grouped by chain, top-down within each.  Chain sizes are
+    //    counted per range; a range's nodes of a chain follow those of the
+    //    ranges above it.
+    const size_t C = chains.size();
+    std::vector<std::vector<int64_t>> part(nt, std::vector<int64_t>(C + 1, 0));
+    on_threads([&](int k) {
+        int64_t *pc = part[k].data();
+        const int32_t *stp = st.data();
+        for (int64_t i = lo_of(k), e = lo_of(k + 1); i < e; ++i) ++pc[stp[i] + 1];
+    });
+    int64_t P = 0;
+    for (size_t x = 0; x < C; ++x) {
+        chains[x].first = P;
+        int64_t acc = P;
+        for (int k = nt - 1; k >= 0; --k) {
+            const int64_t c_k = part[k][x + 1];
+            part[k][x + 1] = acc;  // this range's first slot for chain x
+            acc += c_k;
+        }
+        chains[x].len = static_cast<int32_t>(acc - P);
+        P = acc;
+    }
+    if (grouped.size() < static_cast<size_t>(P)) grouped.resize(P);
+    on_threads([&](int k) {
+        int64_t *cur = part[k].data() + 1;
+        const int32_t *stp = st.data();
+        Node *g = grouped.data();
+        for (int64_t i = lo_of(k + 1) - 1, lo = lo_of(k); i >= lo; --i) {
+            const int32_t ch = stp[i];
+            if (ch < 0) continue;
+            g[cur[ch]++] = {static_cast<int32_t>(i), ch, nrv[i], lamv[i]};
+        }
+    });
+    FH_TMARK("2");
+
+    // 4. the walk's order: depth by depth, the chains present in preorder
+    //    (a chain ending in a split is followed by its kids, left first);
+    //    each node's first row and the cluster ids of the splits.  The list
+    //    of chains changes only where one ends: it is rebuilt there only.
+    for (size_t x = 1; x < C; ++x) chains[x].start = chains[chains[x].parent].start + chains[chains[x].parent].len;
+    int64_t nrows = 0, next_label = N + 1;
+    c.cluster_rows.clear();
+    bfs.resize(P);
+    roff.resize(P);
+    {
+        std::vector<int64_t> pos(C);
+        for (size_t x = 0; x < C; ++x) pos[x] = chains[x].first;
+        std::vector<int32_t> cur{0}, nxt;
+        Node *bp = bfs.data();
+        int64_t *rp = roff.data();
+        int64_t k = 0, out = 0;
+        while (!cur.empty()) {
+            int64_t change = std::numeric_limits<int64_t>::max();
+            for (const int32_t x : cur) change = std::min(change, chains[x].start + chains[x].len);
+            for (; k < change; ++k) {
+                for (const int32_t x : cur) {
+                    const Node v = grouped[pos[x]++];
+                    bp[out] = v;
+                    rp[out++] = nrows;
+                    if (v.nr == -2) {
+                        Chain &cx = chains[x];
+                        cx.split_label = next_label;
+                        for (int s2 = 0; s2 < 2; ++s2)
+                            if (cx.kid[s2] >= 0) chains[cx.kid[s2]].label = next_label + s2;
+                        c.cluster_rows.push_back(nrows);
+                        c.cluster_rows.push_back(nrows + 1);
+                        next_label += 2;
+                        nrows += 2;
+                    } else {
+                        nrows += v.nr;
+                    }
+                }
+            }
+            nxt.clear();
+            for (const int32_t x : cur) {
+                const Chain &cx = chains[x];
+                if (cx.start + cx.len > change) {
+                    nxt.push_back(x);
+                } else {
+                    for (int s2 = 0; s2 < 2; ++s2)
+                        if (cx.kid[s2] >= 0) nxt.push_back(cx.kid[s2]);
+                }
+            }
+            cur.swap(nxt);
+        }
+    }
+    FH_TMARK("4");
+    // 5. rows
+    const size_t cap = static_cast<size_t>(2 * m);
+    if (c.parent.size() < cap) {
+        c.parent.resize(cap);
+        c.child.resize(cap);
+        c.lambda.resize(cap);
+        c.size.resize(cap);
+    }
+    int32_t *cpar = c.parent.data();
+    int32_t *cchild = c.child.data();
+    float *clam = c.lambda.data();
+    float *csize = c.size.data();
+    constexpr int64_t kAhead = 16;
+    auto emit = [&](int64_t p0, int64_t p1) {
+        std::vector<int32_t> sub;
+        for (int64_t p = p0; p < p1; ++p) {
+            if (p + kAhead < p1) prefetch(&rows[bfs[p + kAhead].row]);
+            const Node &v = bfs[p];
+            const Merge &r = rows[v.row];
+            const Chain &cv = chains[v.chain];
+            const int64_t label = cv.label;
+            const float lam = v.lam;
+            int64_t at = roff[p];
+            auto put = [&](int64_t child, float sz) {
+                cpar[at] = static_cast<int32_t>(label);
+                cchild[at] = static_cast<int32_t>(child);
+                clam[at] = lam;
+                csize[at] = sz;
+                ++at;
+            };
+            auto eliminate = [&](int32_t b) {
+                if (b < N) {
+                    put(b, 1.0f);
+                    return;
+                }
+                sub.clear();
+                sub.push_back(b);
+                for (size_t head = 0; head < sub.size(); ++head) {
+                    if (head + kAhead < sub.size() && sub[head + kAhead] >= N) prefetch(&rows[sub[head + kAhead] - N]);
+                    const int32_t x = sub[head];
+                    if (x < N) {
+                        put(x, 1.0f);
+                    } else {
+                        sub.push_back(rows[x - N].left);
+                        sub.push_back(rows[x - N].right);
+                    }
+                }
+            };
+            if (v.nr == -2) {
+                put(cv.split_label, side_size(v.row, 0));
+                put(cv.split_label + 1, side_size(v.row, 1));
+                continue;
+            }
+            const double dl = side_size(v.row, 0), dr = side_size(v.row, 1);
+            if (r.left < N && dr >= dm) {
+                put(r.left, 1.0f);
+            } else if (dl < dm && dr >= dm) {
+                eliminate(r.left);
+            } else if (dl >= dm && dr < dm) {
+                eliminate(r.right);
+            } else {
+                eliminate(r.left);
+                eliminate(r.right);
+            }
+        }
+    };
+    if (nt == 1) {
+        emit(0, P);
+    } else {
+        // pieces of about equal row counts, taken from a shared counter
+        const int pieces = 8 * nt;
+        std::vector<int64_t> cut(pieces + 1, P);
+        cut[0] = 0;
+        for (int k = 1; k < pieces; ++k)
+            cut[k] = std::lower_bound(roff.begin(), roff.begin() + P, nrows * k / pieces) - roff.begin();
+        std::atomic<int> next{0};
+        on_threads([&](int) {
+            for (int k; (k = next.fetch_add(1)) < pieces;) emit(cut[k], cut[k + 1]);
+        });
+    }
+    FH_TMARK("5");
+    c.rows = nrows;
+    c.next_label = next_label;
+    c.root = N;
+
+    // scores per cluster id, each chain's rows in order
+    c.has_scores = want_scores;
+    if (!want_scores) return;
+    c.score.assign(static_cast<size_t>(next_label - N), 0.0f);
+    float *score = c.score.data();
+    auto chain_scores = [&](size_t x0, size_t x1) {
+        for (size_t x = x0; x < x1; ++x) {
+            const Chain &cx = chains[x];
+            const Node *g = grouped.data() + cx.first;
+            float s = score[cx.label - N];  // the split row's initial value (or 0)
+            for (int32_t j = 0; j < cx.len; ++j) {
+                const float lam = g[j].lam;
+                if (g[j].nr == -2) {
+                    s = s + lam * side_size(g[j].row, 0);
+                    s = s + lam * side_size(g[j].row, 1);
+                } else {
+                    for (int32_t k = 0; k < g[j].nr; ++k) s = s + lam;
+                }
+            }
+            score[cx.label - N] = s;
+        }
+    };
+    // split rows set their children's initial scores (-lambda * size when
+    // size > 1), before any of the children's own rows
+    for (const Chain &cx : chains) {
+        if (cx.split_row < 0) continue;
+        const Node &sv = grouped[cx.first + cx.len - 1];
+        for (int s = 0; s < 2; ++s) {
+            const float sz = side_size(cx.split_row, s);
+            if (sz > 1.0f) score[cx.split_label + s - N] = -sv.lam * sz;
+        }
+    }
+    if (nt == 1 || C < 64) {
+        chain_scores(0, C);
+    } else {
+        std::vector<std::thread> th;
+        for (int k = 0; k < nt; ++k) th.emplace_back(chain_scores, C * k / nt, C * (k + 1) / nt);
+        for (auto &x : th) x.join();
+    }
+}
+
+// condense_tree (cluster_trees.py:205-352): the chain form, or the
+// level-by-level walk when a merge has a -1 child (min_samples >= n,
+// whose wrap-around indexing the walk reproduces).
+void condense(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambda_method = kLambdaHdbscan,
+              bool want_scores = false) {
+    if (h.literal || n < 3)
+        condense_walk(n, h, mcs, c, lambda_method);
+    else
+        condense_chains(n, h, mcs, c, lambda_method, want_scores);
 }
 
 struct ClusterTree {
@@ -461,7 +891,7 @@ void cluster_tree(const Condensed &c, ClusterTree &t) {
     t.child.clear();
     t.lambda.clear();
     t.size.clear();
-    for (size_t i = 0; i < static_cast<size_t>(c.rows); ++i) {
+    for (const int64_t i : c.cluster_rows) {
         if (c.size[i] > 1.0f) {
             t.parent.push_back(c.parent[i]);
             t.child.push_back(c.child[i]);
@@ -479,12 +909,16 @@ std::vector<int64_t> eom(const Condensed &c, const ClusterTree &ct, int64_t n, d
     if (ct.parent.empty()) return {};
     const int64_t root = n;
     const int64_t nid = c.next_label - root;
-    std::vector<float> score(nid, 0.0f);
-    for (size_t i = 0; i < static_cast<size_t>(c.rows); ++i) {
-        if (c.size[i] > 1.0f) score[c.child[i] - root] = -c.lambda[i] * c.size[i];
-        float &s = score[c.parent[i] - root];
-        s = s + c.lambda[i] * c.size[i];
+    std::vector<float> scanned;
+    if (!c.has_scores) {
+        scanned.assign(nid, 0.0f);
+        for (size_t i = 0; i < static_cast<size_t>(c.rows); ++i) {
+            if (c.size[i] > 1.0f) scanned[c.child[i] - root] = -c.lambda[i] * c.size[i];
+            float &s = scanned[c.parent[i] - root];
+            s = s + c.lambda[i] * c.size[i];
+        }
     }
+    const float *score = c.has_scores ? c.score.data() : scanned.data();
     std::vector<float> node_size(nid, 0.0f);
     for (size_t i = 0; i < ct.child.size(); ++i) node_size[ct.child[i] - root] = ct.size[i];
     const int64_t ct_root = *std::min_element(ct.parent.begin(), ct.parent.end());
@@ -627,15 +1061,19 @@ std::vector<int64_t> epsilon_search(const std::vector<int64_t> &clusters, const 
 // get_point_membership_strength_vector (999-1031).
 void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps, int64_t n, int64_t *labels,
                    float *prob) {
-    for (int64_t i = 0; i < n; ++i) {
-        labels[i] = -1;
-        prob[i] = 0.0f;
+    // Every point is the child of exactly one row unless the tree came
+    // from the -1 wrap-around linkage: then points without a row stay noise.
+    if (c.rows == 0 || clusters.empty() || c.rows - static_cast<int64_t>(c.cluster_rows.size()) != n) {
+        for (int64_t i = 0; i < n; ++i) {
+            labels[i] = -1;
+            prob[i] = 0.0f;
+        }
     }
     if (c.rows == 0 || clusters.empty()) return;
-    const int64_t root_cluster = *std::min_element(c.parent.begin(), c.parent.begin() + c.rows);
-    const int64_t max_p = *std::max_element(c.parent.begin(), c.parent.begin() + c.rows);
-    const int64_t max_c = *std::max_element(c.child.begin(), c.child.begin() + c.rows);
-    const int64_t size = std::max(max_p + 1, max_c + 1);
+    // ids: the root cluster (smallest parent) up to the last cluster id
+    // (largest parent or child; point ids are smaller)
+    const int64_t root_cluster = c.root;
+    const int64_t size = c.next_label;
     std::sort(clusters.begin(), clusters.end());
     clusters.erase(std::unique(clusters.begin(), clusters.end()), clusters.end());
     // cluster-id tables cover ids [root_cluster, size); points are never
@@ -657,9 +1095,8 @@ void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps
     // set_root(parent(c)).
     std::vector<int64_t> set_root(nc, -1);
     set_root[0] = root_cluster;
-    for (size_t r = 0; r < static_cast<size_t>(c.rows); ++r) {
+    for (const int64_t r : c.cluster_rows) {
         const int64_t child = c.child[r];
-        if (child < n) continue;
         set_root[child - root_cluster] = selected(child) ? child : set_root[c.parent[r] - root_cluster];
     }
     // single selected cluster at the root: the largest lambda among the
@@ -676,17 +1113,35 @@ void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps
     }
     // deaths: largest lambda of a point row directly under each selected
     // cluster (max_lambdas, cluster_trees.py:999-1008)
+    // (a maximum: any split of the rows over threads gives the same values)
+    const int64_t rows = c.rows;
+    const int nt = rows > (1 << 16) ? std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))) : 1;
     std::vector<double> deaths(nc, 0.0);
-    for (size_t r = 0; r < static_cast<size_t>(c.rows); ++r) {
-        const int64_t p = c.parent[r];
-        if (selected(p) && c.size[r] == 1.0f)
-            deaths[p - root_cluster] = std::max(deaths[p - root_cluster], static_cast<double>(c.lambda[r]));
+    {
+        std::vector<std::vector<double>> part(nt, std::vector<double>(nc, 0.0));
+        auto scan = [&](int t) {
+            std::vector<double> &dt = part[t];
+            const int64_t r0 = rows * t / nt, r1 = rows * (t + 1) / nt;
+            for (int64_t r = r0; r < r1; ++r) {
+                const int64_t p = c.parent[r];
+                if (selected(p) && c.size[r] == 1.0f)
+                    dt[p - root_cluster] = std::max(dt[p - root_cluster], static_cast<double>(c.lambda[r]));
+            }
+        };
+        if (nt == 1) {
+            scan(0);
+        } else {
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; ++t) th.emplace_back(scan, t);
+            for (auto &x : th) x.join();
+        }
+        for (int t = 0; t < nt; ++t)
+            for (int64_t k = 0; k < nc; ++k) deaths[k] = std::max(deaths[k], part[t][k]);
     }
     // Every point is the child of exactly one row: its label follows from
     // that row alone (a skipped lambda-0 row leaves the point on its own,
     // i.e. noise), and so does its strength, which the reference computes
     // from the same row.  Rows are independent: split over threads.
-    const int64_t rows = c.rows;
     auto work = [&](int64_t r0, int64_t r1) {
         for (int64_t r = r0; r < r1; ++r) {
             const int64_t child = c.child[r];
@@ -724,7 +1179,6 @@ void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps
             prob[child] = pr;
         }
     };
-    const int nt = rows > (1 << 16) ? std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))) : 1;
     if (nt == 1) {
         work(0, rows);
     } else {
@@ -936,6 +1390,7 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         return std::chrono::duration<double, std::milli>(clk::now() - a).count();
     };
     if (n < 1 || !params || !labels || !probabilities) return FH_EINVAL;
+    if (n >= (int64_t{1} << 30)) return FH_EUNSUPPORTED;
     if (n > 1 && (!src || !dst || (!w_sq && !dist))) return FH_EINVAL;
     if (params->min_cluster_size < 1) return FH_EINVAL;
     if (params->selection_method != 0 && params->selection_method != 1) return FH_EINVAL;
@@ -974,7 +1429,7 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         }
         static thread_local Condensed c;
         t = clk::now();
-        condense(n, h, params->min_cluster_size, c);
+        condense(n, h, params->min_cluster_size, c, kLambdaHdbscan, params->selection_method == 0);
         if (phase_ms) phase_ms[1] = ms_since(t);
         t = clk::now();
         ClusterTree ct;
@@ -996,8 +1451,10 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
             condensed->rows = rows;
             if (rows <= condensed->capacity && condensed->parent && condensed->child && condensed->lambda_val &&
                 condensed->child_size) {
-                std::memcpy(condensed->parent, c.parent.data(), sizeof(int64_t) * rows);
-                std::memcpy(condensed->child, c.child.data(), sizeof(int64_t) * rows);
+                for (int64_t r = 0; r < rows; ++r) {
+                    condensed->parent[r] = c.parent[r];
+                    condensed->child[r] = c.child[r];
+                }
                 std::memcpy(condensed->lambda_val, c.lambda.data(), sizeof(float) * rows);
                 std::memcpy(condensed->child_size, c.size.data(), sizeof(float) * rows);
             }
@@ -1015,6 +1472,7 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
 extern "C" int fh_cluster_layers(int64_t n, const int32_t *src, const int32_t *dst, const double *dist,
                                  const fh_layer_params *p, fh_layers *out) {
     if (n < 1 || !p || !out || !out->labels || !out->strengths || !out->persistence) return FH_EINVAL;
+    if (n >= (int64_t{1} << 30)) return FH_EUNSUPPORTED;
     if (n > 1 && (!src || !dst || !dist)) return FH_EINVAL;
     if (p->base_min_cluster_size < 1 || p->max_layers < 1 || p->base_n_clusters < 0) return FH_EINVAL;
     try {

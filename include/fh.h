@@ -317,7 +317,8 @@ typedef struct {
  * (rows left, right, distance, size, as mst_to_linkage_tree), condensed
  * and phase_ms [4] (linkage, condense, selection, labels; milliseconds)
  * may be NULL.  order may be NULL (see fh_label_order).  No context:
- * host-only.
+ * host-only.  n < 2^30 (cluster ids below 2n are held as int32), else
+ * FH_EUNSUPPORTED.
  */
 int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq,
                               const double *dist, const float *sample_weights, const fh_label_params *params,
@@ -390,7 +391,8 @@ typedef struct {
  * HOST pointers.  src/dst [n-1] int32, dist [n-1] float64 (ascending).
  * Layers are returned unsorted: the base layer first, then one per
  * selected peak in selection order (the caller sorts them by cluster
- * count, layer_clusters.py:484-490).  No context: host-only.
+ * count, layer_clusters.py:484-490).  No context: host-only.  n < 2^30,
+ * else FH_EUNSUPPORTED.
  */
 int fh_cluster_layers(int64_t n, const int32_t *src, const int32_t *dst, const double *dist,
                       const fh_layer_params *params, fh_layers *out);

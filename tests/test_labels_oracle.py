@@ -3,6 +3,8 @@
 Known answers are the reference's own tests; the host stage (C++ in
 libfh.so, no device) must reproduce the oracle row for row.
 """
+import sys
+
 import numpy as np
 import pytest
 
@@ -113,6 +115,10 @@ CASES = [
     ("blobs", 10, 10, "eom", 0.5, False),
     ("blobs", 10, 10, "eom", 0.0, True),
     ("aniso", 10, 100, "eom", 0.0, False),
+    # small cluster sizes: many splits, points as split children (mcs 1)
+    ("blobs", 10, 1, "eom", 0.0, False),
+    ("blobs", 5, 2, "leaf", 0.0, False),
+    ("uniform", 3, 2, "eom", 0.0, True),
 ]
 
 
@@ -134,6 +140,10 @@ def test_host_label_stage_matches_oracle(case):
     from fast_hdbscan_amd import labels as PL
 
     name, ms, mcs, meth, eps, single = case
+    if mcs <= 2:
+        # the oracle's eom_recursion follows the reference's recursion
+        # (cluster_trees.py:689-722); small cluster sizes make it deep
+        sys.setrecursionlimit(max(sys.getrecursionlimit(), 20000))
     edges, _, _ = O.compute_minimum_spanning_tree(_data(name), ms)
     kw = dict(min_cluster_size=mcs, cluster_selection_method=meth, cluster_selection_epsilon=eps,
               allow_single_cluster=single)
