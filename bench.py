@@ -281,11 +281,21 @@ def side_forms(Xd, ms):
     }
     del X5
 
-    # 10M x 3-D mixture, min_samples 15
+    # 10M x 3-D mixture, min_samples 15: plain, then the learned-radius form
+    # (trained once on a 1% sample, then reused)
     X3 = torch.from_numpy(workloads.scale_3d()).cuda()
     t3, st3 = mst_ms(X3, 15, reps=2)
     out["scale_3d"] = {"workload": "10M x 3-D, 200 anisotropic Gaussians + 5% noise, seed 2, min_samples 15",
-                       "mst_ms": t3, "points_per_s": X3.shape[0] / (t3 * 1e-3), "rounds": st3["rounds"]}
+                       "mst_ms": t3, "points_per_s": X3.shape[0] / (t3 * 1e-3), "rounds": st3["rounds"],
+                       "knn_ms": st3["t_knn_ms"], "rounds_ms": st3["t_rounds_ms"], "tree_ms": st3["t_tree_ms"]}
+    _lib.check(ctx, L.fh_set_learned_radius(ctx, 2))
+    mst_ms(X3, 15, reps=1)  # the one training
+    t3l, st3l = mst_ms(X3, 15, reps=2)
+    _lib.check(ctx, L.fh_set_learned_radius(ctx, 0))
+    out["scale_3d"]["learned_trained_once"] = {
+        "mst_ms": t3l, "knn_ms": st3l["t_knn_ms"], "predict_ms": st3l["t_predict_ms"],
+        "sample_and_features_ms": st3l["t_learn_ms"],
+        "ball_hit_rate": 1.0 - st3l["ball_misses"] / X3.shape[0], "speedup_vs_plain": t3 / t3l}
     del X3
 
     # PLSCAN sweep: min_samples 2..64 on 500k x 2-D, layers for each; first

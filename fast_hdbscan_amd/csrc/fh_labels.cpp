@@ -28,6 +28,7 @@
 
 #include "../../include/fh.h"
 #include "fh_numba_sort.h"
+#include "fh_pool.h"
 
 namespace {
 
@@ -261,10 +262,7 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
                         if (g != big) run_group(g);
                 }
             };
-            std::vector<std::thread> th;
-            for (int t = 1; t < nt; ++t) th.emplace_back(worker, false);
-            worker(true);
-            for (auto &x : th) x.join();
+            fh::HostPool::run(nt, [&](int t) { worker(t == 0); });
         }
         first_seq = k;
     }
@@ -441,7 +439,7 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
         }
     };
     const int64_t elim_rows = tasks.empty() ? 0 : nrows;
-    const int nt = elim_rows > (1 << 16) ? std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))) : 1;
+    const int nt = elim_rows > (1 << 16) ? fh::HostPool::width() : 1;
     if (nt == 1) {
         expand(0, tasks.size());
     } else {
@@ -454,9 +452,7 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
             while (t < tasks.size() && tasks[t].off < target) ++t;
             cut[k] = t;
         }
-        std::vector<std::thread> th;
-        for (int k = 0; k < nt; ++k) th.emplace_back(expand, cut[k], cut[k + 1]);
-        for (auto &x : th) x.join();
+        fh::HostPool::run(nt, [&](int k) { expand(cut[k], cut[k + 1]); });
     }
     c.rows = nrows;
     c.next_label = next_label;
@@ -547,7 +543,7 @@ void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, i
         lamv.resize(m);
     }
     if (st.size() < static_cast<size_t>(m)) st.resize(m);
-    const int nt = m > (1 << 16) ? std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))) : 1;
+    const int nt = m > (1 << 16) ? fh::HostPool::width() : 1;
     auto classify = [&](int64_t i0, int64_t i1) {
         for (int64_t i = i0; i < i1; ++i) {
             const Merge &r = rows[i];
@@ -577,13 +573,7 @@ void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, i
             lamv[i] = static_cast<float>(lam);
         }
     };
-    if (nt == 1) {
-        classify(0, m);
-    } else {
-        std::vector<std::thread> th;
-        for (int k = 0; k < nt; ++k) th.emplace_back(classify, m * k / nt, m * (k + 1) / nt);
-        for (auto &x : th) x.join();
-    }
+    fh::HostPool::run(nt, [&](int k) { classify(m * k / nt, m * (k + 1) / nt); });
     FH_TMARK("1a");
 
     // 1b. chain of every row, top-down (a row's parent has a larger index);
@@ -624,15 +614,7 @@ void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, i
         for (int64_t i = lo_of(k), e = lo_of(k + 1); i < e; ++i)
             if (stp[i] <= -2) stp[i] = stp[-2 - stp[i]];
     };
-    auto on_threads = [&](auto &&fn) {
-        if (nt == 1) {
-            fn(0);
-            return;
-        }
-        std::vector<std::thread> th;
-        for (int k = 0; k < nt; ++k) th.emplace_back(fn, k);
-        for (auto &x : th) x.join();
-    };
+    auto on_threads = [&](const std::function<void(int)> &fn) { fh::HostPool::run(nt, fn); };
     on_threads(local);
     {
         int32_t *stp = st.data();
@@ -816,10 +798,7 @@ void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, i
         cut[0] = 0;
         for (int k = 1; k < pieces; ++k)
             cut[k] = std::lower_bound(roff.begin(), roff.begin() + P, nrows * k / pieces) - roff.begin();
-        std::atomic<int> next{0};
-        on_threads([&](int) {
-            for (int k; (k = next.fetch_add(1)) < pieces;) emit(cut[k], cut[k + 1]);
-        });
+        fh::HostPool::run(pieces, [&](int k) { emit(cut[k], cut[k + 1]); });
     }
     FH_TMARK("5");
     c.rows = nrows;
@@ -861,9 +840,7 @@ void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, i
     if (nt == 1 || C < 64) {
         chain_scores(0, C);
     } else {
-        std::vector<std::thread> th;
-        for (int k = 0; k < nt; ++k) th.emplace_back(chain_scores, C * k / nt, C * (k + 1) / nt);
-        for (auto &x : th) x.join();
+        fh::HostPool::run(nt, [&](int k) { chain_scores(C * k / nt, C * (k + 1) / nt); });
     }
 }
 
@@ -1115,7 +1092,7 @@ void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps
     // cluster (max_lambdas, cluster_trees.py:999-1008)
     // (a maximum: any split of the rows over threads gives the same values)
     const int64_t rows = c.rows;
-    const int nt = rows > (1 << 16) ? std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))) : 1;
+    const int nt = rows > (1 << 16) ? fh::HostPool::width() : 1;
     std::vector<double> deaths(nc, 0.0);
     {
         std::vector<std::vector<double>> part(nt, std::vector<double>(nc, 0.0));
@@ -1128,13 +1105,7 @@ void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps
                     dt[p - root_cluster] = std::max(dt[p - root_cluster], static_cast<double>(c.lambda[r]));
             }
         };
-        if (nt == 1) {
-            scan(0);
-        } else {
-            std::vector<std::thread> th;
-            for (int t = 0; t < nt; ++t) th.emplace_back(scan, t);
-            for (auto &x : th) x.join();
-        }
+        fh::HostPool::run(nt, scan);
         for (int t = 0; t < nt; ++t)
             for (int64_t k = 0; k < nc; ++k) deaths[k] = std::max(deaths[k], part[t][k]);
     }
@@ -1179,13 +1150,7 @@ void label_vectors(const Condensed &c, std::vector<int64_t> clusters, double eps
             prob[child] = pr;
         }
     };
-    if (nt == 1) {
-        work(0, rows);
-    } else {
-        std::vector<std::thread> th;
-        for (int t = 0; t < nt; ++t) th.emplace_back(work, rows * t / nt, rows * (t + 1) / nt);
-        for (auto &x : th) x.join();
-    }
+    fh::HostPool::run(nt, [&](int t) { work(rows * t / nt, rows * (t + 1) / nt); });
     // A -1 child (only with min_samples >= n) is a row of an +inf edge:
     // the reference skips it for labels and reads labels[-1] (the last
     // point) for the strength, which is noise in that configuration.
