@@ -159,6 +159,14 @@ def load() -> ctypes.CDLL:
                 f"{LIB_PATH} is missing: build it with `make -C fast_hdbscan_amd/csrc` "
                 "(or __graft_entry__.build()); there is no CPU fallback"
             )
+        # torch (when installed) ships its own HIP runtime under the same
+        # soname: load it first so that libfh.so binds to that one and the
+        # process holds a single runtime (torch cannot see the GPU once
+        # another HIP runtime has opened it)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         L.fh_create.restype = _vp
         L.fh_create.argtypes = [ctypes.c_int]

@@ -100,10 +100,12 @@ __device__ __forceinline__ int block_excl(int v, int &total, int *ws) {
 
 // Blocks over (node, chunk): per-dimension min / max of the node's points,
 // merged with order-preserving integer atomics (numba_kdtree.py:96-105).
+// Up to kRegDims dimensions come from one gather of each point's row.
+constexpr int kRegDims = 8;
 __global__ __launch_bounds__(kTB) void bounds_kernel(LevelRef lv, const float *__restrict__ X, int d,
                                                      const uint64_t *__restrict__ A, int identity,
                                                      uint32_t *obox) {
-    __shared__ uint32_t smin[kTB / 64], smax[kTB / 64];
+    __shared__ uint32_t smin[kRegDims * (kTB / 64)], smax[kRegDims * (kTB / 64)];
     const int k = blockIdx.x / lv.cps, c = blockIdx.x % lv.cps;
     if (k >= lv.count) return;
     const int node = lv.first + k;
@@ -111,6 +113,53 @@ __global__ __launch_bounds__(kTB) void bounds_kernel(LevelRef lv, const float *_
     const int s = r.x + c * kChunk;
     if (s >= r.y) return;
     const int e = min(r.y, s + kChunk);
+    if (d <= kRegDims) {
+        // every dimension from one gather of the point's row
+        uint32_t mn[kRegDims], mx[kRegDims];
+#pragma unroll
+        for (int j = 0; j < kRegDims; ++j) {
+            mn[j] = 0xffffffffu;
+            mx[j] = 0u;
+        }
+        for (int p = s + threadIdx.x; p < e; p += kTB) {
+            const uint32_t idx = identity ? (uint32_t)p : key_idx(A[p]);
+            const float *row = X + (size_t)idx * d;
+#pragma unroll
+            for (int j = 0; j < kRegDims; ++j) {
+                if (j < d) {
+                    const uint32_t o = f2o(row[j]);
+                    mn[j] = min(mn[j], o);
+                    mx[j] = max(mx[j], o);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kRegDims; ++j) {
+            if (j >= d) break;
+            uint32_t a = mn[j], b = mx[j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                a = min(a, (uint32_t)__shfl_xor((int)a, o, 64));
+                b = max(b, (uint32_t)__shfl_xor((int)b, o, 64));
+            }
+            if ((threadIdx.x & 63) == 0) {
+                smin[j * (kTB / 64) + (threadIdx.x >> 6)] = a;
+                smax[j * (kTB / 64) + (threadIdx.x >> 6)] = b;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < d) {
+            const int j = threadIdx.x;
+            uint32_t a = smin[j * (kTB / 64)], b = smax[j * (kTB / 64)];
+            for (int w = 1; w < kTB / 64; ++w) {
+                a = min(a, smin[j * (kTB / 64) + w]);
+                b = max(b, smax[j * (kTB / 64) + w]);
+            }
+            atomicMin(obox + (size_t)node * 2 * d + j, a);
+            atomicMax(obox + (size_t)node * 2 * d + d + j, b);
+        }
+        return;
+    }
     for (int j = 0; j < d; ++j) {
         uint32_t mn = 0xffffffffu, mx = 0u;
         for (int p = s + threadIdx.x; p < e; p += kTB) {
