@@ -736,40 +736,89 @@ void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, i
     float *clam = c.lambda.data();
     float *csize = c.size.data();
     constexpr int64_t kAhead = 16;
+    // An eliminated branch's points in breadth-first order (eliminate_branch,
+    // cluster_trees.py:115-145).  Each walk waits on memory once per level,
+    // so kWalks branches are walked side by side, one queue entry each in
+    // turn, and a merge node's row is prefetched when it is queued.
+    struct Branch {
+        int32_t root;
+        int32_t label;
+        float lam;
+        int64_t at;
+    };
+    constexpr int kWalks = 16;
+    auto expand = [&](const std::vector<Branch> &todo) {
+        struct Walk {
+            std::vector<int32_t> q;
+            size_t head = 0;
+            int64_t at = 0;
+            int32_t label = 0;
+            float lam = 0.0f;
+            bool live = false;
+        };
+        Walk w[kWalks];
+        size_t next = 0;
+        auto start = [&](Walk &x) {
+            x.live = next < todo.size();
+            if (!x.live) return;
+            const Branch &b = todo[next++];
+            x.q.clear();
+            x.q.push_back(b.root);
+            x.head = 0;
+            x.at = b.at;
+            x.label = b.label;
+            x.lam = b.lam;
+        };
+        for (auto &x : w) start(x);
+        for (int live = kWalks; live > 0;) {
+            live = 0;
+            for (auto &x : w) {
+                if (!x.live) continue;
+                ++live;
+                const int32_t y = x.q[x.head++];
+                if (y < N) {
+                    cpar[x.at] = x.label;
+                    cchild[x.at] = y;
+                    clam[x.at] = x.lam;
+                    csize[x.at] = 1.0f;
+                    ++x.at;
+                } else {
+                    const int32_t l = rows[y - N].left, r = rows[y - N].right;
+                    if (l >= N) prefetch(&rows[l - N]);
+                    if (r >= N) prefetch(&rows[r - N]);
+                    x.q.push_back(l);
+                    x.q.push_back(r);
+                }
+                if (x.head == x.q.size()) start(x);
+            }
+        }
+    };
     auto emit = [&](int64_t p0, int64_t p1) {
-        std::vector<int32_t> sub;
+        std::vector<Branch> todo;
         for (int64_t p = p0; p < p1; ++p) {
             if (p + kAhead < p1) prefetch(&rows[bfs[p + kAhead].row]);
             const Node &v = bfs[p];
             const Merge &r = rows[v.row];
             const Chain &cv = chains[v.chain];
-            const int64_t label = cv.label;
+            const int32_t label = static_cast<int32_t>(cv.label);
             const float lam = v.lam;
             int64_t at = roff[p];
             auto put = [&](int64_t child, float sz) {
-                cpar[at] = static_cast<int32_t>(label);
+                cpar[at] = label;
                 cchild[at] = static_cast<int32_t>(child);
                 clam[at] = lam;
                 csize[at] = sz;
                 ++at;
             };
+            // a point's row now; a merge node's rows reserved for expand()
             auto eliminate = [&](int32_t b) {
                 if (b < N) {
                     put(b, 1.0f);
                     return;
                 }
-                sub.clear();
-                sub.push_back(b);
-                for (size_t head = 0; head < sub.size(); ++head) {
-                    if (head + kAhead < sub.size() && sub[head + kAhead] >= N) prefetch(&rows[sub[head + kAhead] - N]);
-                    const int32_t x = sub[head];
-                    if (x < N) {
-                        put(x, 1.0f);
-                    } else {
-                        sub.push_back(rows[x - N].left);
-                        sub.push_back(rows[x - N].right);
-                    }
-                }
+                if (b < N + m) prefetch(&rows[b - N]);
+                todo.push_back({b, label, lam, at});
+                at += rows[b - N].left_size + rows[b - N].right_size;
             };
             if (v.nr == -2) {
                 put(cv.split_label, side_size(v.row, 0));
@@ -788,6 +837,7 @@ void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, i
                 eliminate(r.right);
             }
         }
+        expand(todo);
     };
     if (nt == 1) {
         emit(0, P);
