@@ -675,53 +675,131 @@ void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, i
     //    (a chain ending in a split is followed by its kids, left first);
     //    each node's first row and the cluster ids of the splits.  The list
     //    of chains changes only where one ends: it is rebuilt there only.
+    //    Threads take ranges of depths: per-depth node, row and split counts
+    //    give each range its first output slot, row and cluster id, and the
+    //    chains present at its first depth, in preorder, its starting list.
     for (size_t x = 1; x < C; ++x) chains[x].start = chains[chains[x].parent].start + chains[chains[x].parent].len;
-    int64_t nrows = 0, next_label = N + 1;
-    c.cluster_rows.clear();
+    int64_t depth_end = 0;
+    for (const Chain &cx : chains) depth_end = std::max(depth_end, cx.start + cx.len);
+    std::vector<int32_t> rank(C);
+    {
+        std::vector<int32_t> stack{0};
+        int32_t r = 0;
+        while (!stack.empty()) {
+            const int32_t x = stack.back();
+            stack.pop_back();
+            rank[x] = r++;
+            for (int s2 = 1; s2 >= 0; --s2)
+                if (chains[x].kid[s2] >= 0) stack.push_back(chains[x].kid[s2]);
+        }
+    }
+    // nodes per depth (chain spans), as first output slot per depth
+    static thread_local std::vector<int64_t> dn_tl;
+    std::vector<int64_t> &dn = dn_tl;
+    dn.assign(depth_end + 1, 0);
+    for (const Chain &cx : chains) {
+        ++dn[cx.start];
+        --dn[cx.start + cx.len];
+    }
+    {
+        int64_t active = 0, on = 0;
+        for (int64_t k = 0; k <= depth_end; ++k) {
+            active += dn[k];
+            dn[k] = on;
+            on += active;
+        }
+    }
     bfs.resize(P);
     roff.resize(P);
-    {
-        std::vector<int64_t> pos(C);
-        for (size_t x = 0; x < C; ++x) pos[x] = chains[x].first;
-        std::vector<int32_t> cur{0}, nxt;
+    // depth ranges of about equal node counts; each range's rows and splits
+    // summed by its thread, then scanned: its first row and cluster id
+    const int T = P > (1 << 16) ? nt : 1;
+    std::vector<int64_t> kcut(T + 1, depth_end);
+    kcut[0] = 0;
+    for (int t = 1; t < T; ++t)
+        kcut[t] = std::upper_bound(dn.begin(), dn.begin() + depth_end, P * t / T) - dn.begin() - 1;
+    for (int t = 1; t <= T; ++t) kcut[t] = std::max(kcut[t], kcut[t - 1]);
+    std::vector<int64_t> row0(T + 1, 0), split0(T + 1, 0);
+    fh::HostPool::run(T, [&](int t) {
+        const int64_t k0 = kcut[t], k1 = kcut[t + 1];
+        int64_t rws = 0, sps = 0;
+        for (size_t x = 0; x < C; ++x) {
+            const Chain &cx = chains[x];
+            const int64_t lo = std::max(cx.start, k0), hi = std::min(cx.start + cx.len, k1);
+            if (lo >= hi) continue;
+            const Node *g = grouped.data() + cx.first - cx.start;
+            for (int64_t k = lo; k < hi; ++k) {
+                const int32_t nr = g[k].nr;
+                rws += nr == -2 ? 2 : nr;
+                sps += nr == -2;
+            }
+        }
+        row0[t + 1] = rws;
+        split0[t + 1] = sps;
+    });
+    for (int t = 0; t < T; ++t) {
+        row0[t + 1] += row0[t];
+        split0[t + 1] += split0[t];
+    }
+    const int64_t nrows = row0[T];
+    const int64_t next_label = N + 1 + 2 * split0[T];
+    std::vector<std::vector<int64_t>> crows(T);
+    fh::HostPool::run(T, [&](int t) {
+        const int64_t k0 = kcut[t], k1 = kcut[t + 1];
+        if (k0 >= k1) return;
+        struct Live {
+            int32_t x;
+            int64_t pos;
+        };
+        std::vector<Live> cur, nxt;
+        for (size_t x = 0; x < C; ++x)
+            if (chains[x].start <= k0 && k0 < chains[x].start + chains[x].len)
+                cur.push_back({static_cast<int32_t>(x), chains[x].first + (k0 - chains[x].start)});
+        std::sort(cur.begin(), cur.end(), [&](const Live &a, const Live &b) { return rank[a.x] < rank[b.x]; });
         Node *bp = bfs.data();
         int64_t *rp = roff.data();
-        int64_t k = 0, out = 0;
-        while (!cur.empty()) {
+        int64_t out = dn[k0], rw = row0[t], lab = N + 1 + 2 * split0[t];
+        std::vector<int64_t> &cr = crows[t];
+        int64_t k = k0;
+        while (!cur.empty() && k < k1) {
             int64_t change = std::numeric_limits<int64_t>::max();
-            for (const int32_t x : cur) change = std::min(change, chains[x].start + chains[x].len);
-            for (; k < change; ++k) {
-                for (const int32_t x : cur) {
-                    const Node v = grouped[pos[x]++];
+            for (const Live &l : cur) change = std::min(change, chains[l.x].start + chains[l.x].len);
+            const int64_t stop = std::min(change, k1);
+            for (; k < stop; ++k) {
+                for (Live &l : cur) {
+                    const Node v = grouped[l.pos++];
                     bp[out] = v;
-                    rp[out++] = nrows;
+                    rp[out++] = rw;
                     if (v.nr == -2) {
-                        Chain &cx = chains[x];
-                        cx.split_label = next_label;
+                        Chain &cx = chains[l.x];
+                        cx.split_label = lab;
                         for (int s2 = 0; s2 < 2; ++s2)
-                            if (cx.kid[s2] >= 0) chains[cx.kid[s2]].label = next_label + s2;
-                        c.cluster_rows.push_back(nrows);
-                        c.cluster_rows.push_back(nrows + 1);
-                        next_label += 2;
-                        nrows += 2;
+                            if (cx.kid[s2] >= 0) chains[cx.kid[s2]].label = lab + s2;
+                        cr.push_back(rw);
+                        cr.push_back(rw + 1);
+                        lab += 2;
+                        rw += 2;
                     } else {
-                        nrows += v.nr;
+                        rw += v.nr;
                     }
                 }
             }
+            if (k == k1) break;
             nxt.clear();
-            for (const int32_t x : cur) {
-                const Chain &cx = chains[x];
+            for (const Live &l : cur) {
+                const Chain &cx = chains[l.x];
                 if (cx.start + cx.len > change) {
-                    nxt.push_back(x);
+                    nxt.push_back(l);
                 } else {
                     for (int s2 = 0; s2 < 2; ++s2)
-                        if (cx.kid[s2] >= 0) nxt.push_back(cx.kid[s2]);
+                        if (cx.kid[s2] >= 0) nxt.push_back({cx.kid[s2], chains[cx.kid[s2]].first});
                 }
             }
             cur.swap(nxt);
         }
-    }
+    });
+    c.cluster_rows.clear();
+    for (const auto &cr : crows) c.cluster_rows.insert(c.cluster_rows.end(), cr.begin(), cr.end());
     FH_TMARK("4");
     // 5. rows
     const size_t cap = static_cast<size_t>(2 * m);
