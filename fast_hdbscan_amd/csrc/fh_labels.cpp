@@ -30,6 +30,12 @@
 #include "fh_numba_sort.h"
 #include "fh_pool.h"
 
+// phase timing hooks: empty unless a profiling build defines them
+#ifndef FH_TMARK
+#define FH_TMARK(x)
+#define FH_TSTART()
+#endif
+
 namespace {
 
 constexpr double kInfD = std::numeric_limits<double>::infinity();
@@ -130,21 +136,28 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
     const int32_t *order = ord ? ord->point_order : nullptr;
     h.weighted = weights != nullptr;
     h.literal = false;
-    for (int64_t i = 0; i < m; ++i)
-        if (src[i] < 0 || dst[i] < 0) {
-            if (weights) return false;
-            if (!order) {
-                linkage_tree_literal(n, src, dst, w_sq, dist, h);
-                return true;
-            }
-            std::vector<int32_t> s2(m), d2(m);
-            for (int64_t j = 0; j < m; ++j) {
-                s2[j] = src[j] < 0 ? src[j] : order[src[j]];
-                d2[j] = dst[j] < 0 ? dst[j] : order[dst[j]];
-            }
-            linkage_tree_literal(n, s2.data(), d2.data(), w_sq, dist, h);
+    const int nt = m > (1 << 16) ? fh::HostPool::width() : 1;
+    // any -1 endpoint? (row ranges on threads)
+    std::vector<uint8_t> neg(nt, 0);
+    fh::HostPool::run(nt, [&](int t) {
+        uint8_t any = 0;
+        for (int64_t i = m * t / nt, e = m * (t + 1) / nt; i < e; ++i) any |= (src[i] < 0) | (dst[i] < 0);
+        neg[t] = any;
+    });
+    if (std::find(neg.begin(), neg.end(), 1) != neg.end()) {
+        if (weights) return false;
+        if (!order) {
+            linkage_tree_literal(n, src, dst, w_sq, dist, h);
             return true;
         }
+        std::vector<int32_t> s2(m), d2(m);
+        for (int64_t j = 0; j < m; ++j) {
+            s2[j] = src[j] < 0 ? src[j] : order[src[j]];
+            d2[j] = dst[j] < 0 ? dst[j] : order[dst[j]];
+        }
+        linkage_tree_literal(n, s2.data(), d2.data(), w_sq, dist, h);
+        return true;
+    }
     static thread_local std::vector<float> wsz_tl;
     if (weights) {
         wsz_tl.resize(n);
@@ -161,7 +174,10 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
     // (worker threads below must use the caller's instances)
     Set *uf = uf_tl.data();
     float *wsz = wsz_tl.data();
-    for (int64_t i = 0; i < n; ++i) uf[i] = {static_cast<int32_t>(i), 1, order ? order[i] : static_cast<int32_t>(i), 0};
+    fh::HostPool::run(nt, [&](int t) {
+        for (int64_t i = n * t / nt, e = n * (t + 1) / nt; i < e; ++i)
+            uf[i] = {static_cast<int32_t>(i), 1, order ? order[i] : static_cast<int32_t>(i), 0};
+    });
     if (h.rows.size() < static_cast<size_t>(m)) h.rows.resize(m);
     Merge *rows = h.rows.data();
     float *lw = weights ? h.lw.data() : nullptr;
@@ -213,6 +229,7 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
             prefetch(&uf[uf[dst[i]].parent]);
         }
     };
+    FH_TSTART();
     int64_t first_seq = 0;
     if (ord && ord->prefix_rows > 0 && ord->row_order && ord->group_sizes && ord->n_groups > 0 &&
         ord->prefix_rows <= m) {
@@ -266,11 +283,13 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         }
         first_seq = k;
     }
+    FH_TMARK("link-groups");
     for (int64_t i = first_seq; i < m; ++i) {
         if (i + kAhead < m) prefetch_row(i + kAhead, 0);
         if (i + kAhead2 < m) prefetch_row(i + kAhead2, 1);
         step(i);
     }
+    FH_TMARK("link-tail");
     return true;
 }
 
@@ -481,11 +500,6 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
 // With want_scores, score_condensed_tree_nodes (cluster_trees.py:656-669)
 // is summed per chain: a cluster's rows are its chain's rows, top-down,
 // i.e. in row order, so each float32 sum runs in the reference's order.
-// phase timing hooks: empty unless a profiling build defines them
-#ifndef FH_TMARK
-#define FH_TMARK(x)
-#define FH_TSTART()
-#endif
 void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, int lambda_method, bool want_scores) {
     FH_TSTART();
     const int64_t m = n - 1;
