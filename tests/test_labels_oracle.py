@@ -280,3 +280,43 @@ def test_component_groups_are_checked():
     with pytest.raises(ValueError):
         PL.labels_from_sorted_arrays(n, src, dst, None, params, dist=e[o, 2].copy(),
                                      groups=(3, np.array([0, 1, 7], np.int32), np.array([3], np.int32)))
+
+
+def test_concurrent_label_stages_match():
+    # several host threads in the label stage at once (the batch scheduler
+    # and the PLSCAN sweep do this): the worker pool serves one, the others
+    # run their parallel passes on threads of their own; results unchanged
+    import threading
+
+    from fast_hdbscan_amd import labels as PL
+
+    X = workloads.anisotropic_mixture(90_000, 2, 12, 0.1, seed=5)
+    e, _, _ = O.compute_minimum_spanning_tree(X, 5)
+    n = e.shape[0] + 1
+    o = np.lexsort((e[:, 1], e[:, 0], e[:, 2]))
+    src, dst = e[o, 0].astype(np.int32), e[o, 1].astype(np.int32)
+    dist = e[o, 2].copy()
+    cases = [PL.label_params(mcs, "eom", np.inf, False, 0.0) for mcs in (5, 20, 50, 200)]
+    ref = [PL.labels_from_sorted_arrays(n, src, dst, None, p, return_trees=True, dist=dist) for p in cases]
+    out = [None] * (2 * len(cases))
+    errs = []
+
+    def run(i):
+        try:
+            out[i] = PL.labels_from_sorted_arrays(n, src, dst, None, cases[i % len(cases)], return_trees=True,
+                                                  dist=dist)
+        except Exception as ex:  # pragma: no cover - reported below
+            errs.append(ex)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(out))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs
+    for i, got in enumerate(out):
+        want = ref[i % len(cases)]
+        for x, y in zip(got[:3], want[:3]):
+            np.testing.assert_array_equal(x, y)
+        for f in ("parent", "child", "lambda_val", "child_size"):
+            np.testing.assert_array_equal(getattr(got[3], f), getattr(want[3], f))
