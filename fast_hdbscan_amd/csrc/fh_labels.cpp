@@ -41,17 +41,26 @@ namespace {
 constexpr double kInfD = std::numeric_limits<double>::infinity();
 constexpr float kInfF = std::numeric_limits<float>::infinity();
 
-// One merge of the single-linkage tree: children, their sizes, distance.
+// One merge of the single-linkage tree: children and their sizes (16 bytes:
+// the linkage writes and condensing reads them at random; the distance is
+// the sorted row's own, Hierarchy::delta).
 struct Merge {
     int32_t left, right;            // child node ids (points < n, merges >= n)
     int32_t left_size, right_size;  // points under each child
-    double delta;                   // merge distance
-    int32_t relabel;                // condense_tree's relabel of this node (same cache line)
-    int32_t pad;
 };
+
+// delta(i): the merge distance of sorted row i -- the float64 square root of
+// a squared float32 weight (boruvka.py:646), or a given float64 distance.
+inline double row_delta(const float *w_sq, const double *dist, int64_t i) {
+    return dist ? dist[i] : std::sqrt(static_cast<double>(w_sq[i]));
+}
 
 struct Hierarchy {
     std::vector<Merge> rows;  // row i is merge node n + i
+    // the caller's sorted distances (valid for the call): row i's delta
+    const float *w_sq = nullptr;
+    const double *dist = nullptr;
+    double delta(int64_t i) const { return row_delta(w_sq, dist, i); }
     // with sample weights: float32 weighted sizes of each row's children
     // (create_linkage_merge_data_w_sample_weights, cluster_trees.py:29-35)
     bool weighted = false;
@@ -65,12 +74,6 @@ inline void prefetch(const void *p) { __builtin_prefetch(p, 0, 1); }
 // linkage_merge_find (38-52).  Used when an endpoint is -1 (only when
 // min_samples >= n), which the reference resolves through wrap-around
 // indexing into the merge slots.
-// delta(i): the merge distance of sorted row i -- the float64 square root of
-// a squared float32 weight (boruvka.py:646), or a given float64 distance.
-inline double row_delta(const float *w_sq, const double *dist, int64_t i) {
-    return dist ? dist[i] : std::sqrt(static_cast<double>(w_sq[i]));
-}
-
 void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, const double *dist,
                           Hierarchy &h) {
     const int64_t m = n - 1;
@@ -80,6 +83,8 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
     for (int64_t i = 0; i < n; ++i) size[i] = 1;
     h.rows.resize(m);
     h.literal = true;
+    h.w_sq = w_sq;
+    h.dist = dist;
     int64_t next = n;
     // node values stay as given (-1 stays -1); only array accesses wrap
     auto P = [&](int64_t x) -> int64_t & { return parent[x < 0 ? x + total : x]; };
@@ -104,7 +109,6 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
         r.right = static_cast<int32_t>(lbig ? rc : lc);
         r.left_size = static_cast<int32_t>(S(lbig ? lc : rc));
         r.right_size = static_cast<int32_t>(S(lbig ? rc : lc));
-        r.delta = row_delta(w_sq, dist, i);
         S(next) = S(lc) + S(rc);
         P(lc) = next;
         P(rc) = next;
@@ -136,6 +140,8 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
     const int32_t *order = ord ? ord->point_order : nullptr;
     h.weighted = weights != nullptr;
     h.literal = false;
+    h.w_sq = w_sq;
+    h.dist = dist;
     const int nt = m > (1 << 16) ? fh::HostPool::width() : 1;
     // any -1 endpoint? (row ranges on threads)
     std::vector<uint8_t> neg(nt, 0);
@@ -201,7 +207,6 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         r.right = lbig ? rc : lc;
         r.left_size = lbig ? ls : rs;
         r.right_size = lbig ? rs : ls;
-        r.delta = row_delta(w_sq, dist, i);
         int32_t root = a, child = b;
         if (uf[a].size < uf[b].size) {
             root = b;
@@ -321,16 +326,17 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
     const int32_t num_points = static_cast<int32_t>(n);
     int64_t next_label = num_points + 1;
     // relabel is written for a node before the node is processed; only the
-    // root needs a value up front.  It lives in the node's merge row, which
-    // the walk reads anyway.  Points are never processed: their relabel is
-    // not kept.
-    Merge *mrows = h.rows.data();
-    mrows[root - num_points].relabel = num_points;
+    // root needs a value up front.  Points are never processed: their
+    // relabel is not kept.
+    static thread_local std::vector<int32_t> relabel_tl;
+    std::vector<int32_t> &relabel = relabel_tl;
+    if (relabel.size() < static_cast<size_t>(m)) relabel.resize(m);
+    relabel[root - num_points] = num_points;
     // a -1 child (only when min_samples >= n) indexes like numba: from the
     // end, i.e. the root's entry
     auto RL = [&](int32_t x, int32_t v) {
-        if (x >= num_points) mrows[x - num_points].relabel = v;
-        else if (x < 0) mrows[x + root + 1 - num_points].relabel = v;
+        if (x >= num_points) relabel[x - num_points] = v;
+        else if (x < 0) relabel[x + root + 1 - num_points] = v;
     };
     // at most one row per node below the root
     const size_t cap = static_cast<size_t>(root);
@@ -395,9 +401,9 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
         }
         const int32_t node = q[qi];
         const Merge &r = rows[node - num_points];
-        const int64_t parent_node = r.relabel;
+        const int64_t parent_node = relabel[node - num_points];
         const int32_t left = r.left, right = r.right;
-        const double d = r.delta;
+        const double d = h.delta(node - num_points);
         const double lam = lambda_method == kLambdaHdbscan ? (d > 0.0 ? 1.0 / d : kInfD) : std::exp(-d);
         // counts as float32 (weighted sizes with weights), compared in
         // float64 as numba does
@@ -562,7 +568,7 @@ void condense_chains(int64_t n, const Hierarchy &h, int64_t mcs, Condensed &c, i
         for (int64_t i = i0; i < i1; ++i) {
             const Merge &r = rows[i];
             const double dl = side_size(i, 0), dr = side_size(i, 1);
-            const double d = r.delta;
+            const double d = h.delta(i);
             const double lam = lambda_method == kLambdaHdbscan ? (d > 0.0 ? 1.0 / d : kInfD) : std::exp(-d);
             int32_t nr, lrole = kDrop, rrole = kDrop;
             if (r.left < N && dr >= dm) {
@@ -1529,7 +1535,7 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
                 const Merge &r = h.rows[i];
                 linkage[4 * i + 0] = static_cast<double>(r.left);
                 linkage[4 * i + 1] = static_cast<double>(r.right);
-                linkage[4 * i + 2] = r.delta;
+                linkage[4 * i + 2] = h.delta(i);
                 linkage[4 * i + 3] = h.weighted ? static_cast<double>(h.lw[i] + h.rw[i])
                                                 : static_cast<double>(r.left_size) + static_cast<double>(r.right_size);
             }
