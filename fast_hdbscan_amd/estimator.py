@@ -232,23 +232,29 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
                                              vp(ss.data_ptr()), vp(sd.data_ptr()), vp(sw.data_ptr())))
         _lib.check(ctx, L.fh_edges_to_tree_order(ctx, n, m, vp(ss.data_ptr()), vp(sd.data_ptr()), vp(ps.data_ptr()),
                                                  vp(pd.data_ptr()), vp(od.data_ptr())))
+    # the sorted rows (ready: the library calls above end synchronised) go
+    # to pinned host memory on torch's stream while the library's stream
+    # computes the row groups
+    hs, hd, hw = w["host"]
+    hs[:m].copy_(ps[:m], non_blocking=True)
+    hd[:m].copy_(pd[:m], non_blocking=True)
+    hw[:m].copy_(sw[:m], non_blocking=True)
+    oh[:n].copy_(od[:n], non_blocking=True)
     # the lightest 90% of the rows, grouped by the components they form:
     # the host linkage runs those groups on parallel threads
     k = (9 * m) // 10 if m >= (1 << 16) else 0
     groups = None
     if k > 0:
         if w.get("plan") is None or w["plan"][0].shape[0] < k:
-            w["plan"] = (np.empty(k, np.int32), np.empty(k, np.int32))
+            # pinned host buffers (kept in `w`), used through numpy views
+            w["plan_pinned"] = tuple(torch.empty(k, dtype=torch.int32).pin_memory() for _ in range(2))
+            w["plan"] = tuple(t.numpy() for t in w["plan_pinned"])
         ro, gs = w["plan"]
         ng = ctypes.c_int64(0)
         _lib.check(ctx, L.fh_linkage_plan(ctx, n, k, vp(ps.data_ptr()), vp(pd.data_ptr()), _lib.ptr(ro), _lib.ptr(gs),
                                           ctypes.byref(ng)))
         groups = (k, ro[:k], gs[:ng.value])
-    hs, hd, hw = w["host"]
-    hs[:m].copy_(ps[:m])
-    hd[:m].copy_(pd[:m])
-    hw[:m].copy_(sw[:m])
-    oh[:n].copy_(od[:n])
+    torch.cuda.current_stream(dev).synchronize()
     t1 = time.perf_counter()
     params = label_params(min_cluster_size, cluster_selection_method, max_cluster_size, allow_single_cluster,
                           cluster_selection_epsilon)
