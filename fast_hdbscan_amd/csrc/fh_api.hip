@@ -271,7 +271,7 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
             for (int k = 0; k < count; ++k)
                 if (segs[k].flag == fh::SEG_SORT) FH_HIP(fh::tree_sort_range(a.A, segs[k].L, segs[k].R, st));
         }
-        FH_HIP(fh::tree_small_select(a, count, st));
+        FH_HIP(fh::tree_small_select(a, count, maxs, st));
     }
     FH_HIP(fh::tree_finalize(a, ni, ctx->perm.p, ctx->posof.p, ctx->xp.p, st));
     ctx->tree_n = n;

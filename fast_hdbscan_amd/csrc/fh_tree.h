@@ -45,7 +45,8 @@ hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStr
 // steps of every SEG_LARGE node by one work-group each; leaves SEG_SMALL or
 // SEG_DONE (for levels whose nodes are at most tree_medium_limit() long)
 hipError_t tree_medium_select(const TreeBuildArgs &a, int count, hipStream_t st);
-hipError_t tree_small_select(const TreeBuildArgs &a, int count, hipStream_t st);
+// max_size: the level's largest node (the segments left are no longer)
+hipError_t tree_small_select(const TreeBuildArgs &a, int count, int max_size, hipStream_t st);
 hipError_t tree_sort_range(uint64_t *A, int L, int R, hipStream_t st);
 hipError_t tree_finalize(const TreeBuildArgs &a, int n, int *perm, int *posof, float *xp, hipStream_t st);
 

@@ -96,6 +96,25 @@ __device__ __forceinline__ int block_excl(int v, int &total, int *ws) {
     return off + inc - v;
 }
 
+// Block-wide exclusive scan for NT threads; total returned by reference.
+template <int NT>
+__device__ __forceinline__ int block_excl_nt(int v, int &total, int *ws) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = wave_incl(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+        int s = ws[i];
+        off += i < w ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - v;
+}
+
 // ---------------------------------------------------------- bounds ----
 
 // Blocks over (node, chunk): per-dimension min / max of the node's points,
@@ -437,12 +456,15 @@ __device__ void lds_sort(uint64_t *sk, int n) {
     }
 }
 
-// One block per node of the level: finish the introselect of segments that
-// are (or became) at most kSmall long.
-__global__ __launch_bounds__(kTB) void small_select_kernel(int count, SegState *seg, uint64_t *A) {
-    __shared__ uint64_t sk[kSmall];
-    __shared__ int gt_s[kSmall / 2], lt_s[kSmall / 2];
-    __shared__ int ws[kTB / 64];
+// One block of NT threads per node of the level: finish the introselect of
+// segments that are (or became) at most CAP <= kSmall long.  Deep levels,
+// whose nodes hold a few dozen keys, use a small CAP and NT so that many
+// blocks share a CU.
+template <int CAP, int NT>
+__global__ __launch_bounds__(NT) void small_select_kernel(int count, SegState *seg, uint64_t *A) {
+    __shared__ uint64_t sk[CAP];
+    __shared__ int gt_s[CAP / 2], lt_s[CAP / 2];
+    __shared__ int ws[NT / 64];
     __shared__ int sh[4];
     const int k = blockIdx.x;
     if (k >= count) return;
@@ -450,7 +472,7 @@ __global__ __launch_bounds__(kTB) void small_select_kernel(int count, SegState *
     if (st.flag != SEG_SMALL) return;
     const int L0 = st.L, R0 = st.R;
     const int n0 = R0 - L0;
-    for (int p = threadIdx.x; p < n0; p += kTB) sk[p] = A[L0 + p];
+    for (int p = threadIdx.x; p < n0; p += NT) sk[p] = A[L0 + p];
     __syncthreads();
     // local coordinates
     int L = 0, R = n0, nth = st.nth - L0, depth = st.depth;
@@ -469,7 +491,7 @@ __global__ __launch_bounds__(kTB) void small_select_kernel(int count, SegState *
         __syncthreads();
         const uint64_t pivot = sk[R - 1];
         const int len = R - 1 - L;
-        const int per = (len + kTB - 1) / kTB;
+        const int per = (len + NT - 1) / NT;
         const int b = L + threadIdx.x * per;
         int v = 0;
         for (int j = 0; j < per; ++j) {
@@ -477,7 +499,7 @@ __global__ __launch_bounds__(kTB) void small_select_kernel(int count, SegState *
             v += (p < R - 1 && sk[p] < pivot) ? 1 : 0;
         }
         int S;
-        int before = block_excl(v, S, ws);
+        int before = block_excl_nt<NT>(v, S, ws);
         const int P = L + S;
         if (threadIdx.x == 0) sh[0] = 0;
         __syncthreads();
@@ -495,7 +517,7 @@ __global__ __launch_bounds__(kTB) void small_select_kernel(int count, SegState *
         }
         __syncthreads();
         const int M = sh[0];
-        for (int m = threadIdx.x; m < M; m += kTB) {
+        for (int m = threadIdx.x; m < M; m += NT) {
             const int a = gt_s[m], c = lt_s[m];
             const uint64_t t = sk[a];
             sk[a] = sk[c];
@@ -520,7 +542,7 @@ __global__ __launch_bounds__(kTB) void small_select_kernel(int count, SegState *
         // [L, R) ascending
         lds_sort(sk + L, R - L);
     }
-    for (int p = threadIdx.x; p < n0; p += kTB) A[L0 + p] = sk[p];
+    for (int p = threadIdx.x; p < n0; p += NT) A[L0 + p] = sk[p];
     seg[k].flag = SEG_DONE;
 }
 
@@ -528,25 +550,6 @@ __global__ __launch_bounds__(kTB) void small_select_kernel(int count, SegState *
 
 constexpr int kMTB = 1024;  // threads of the one-work-group-per-node steps
 constexpr int kMB = 8;      // keys each thread loads at once
-
-// Block-wide exclusive scan for NT threads; total returned by reference.
-template <int NT>
-__device__ __forceinline__ int block_excl_nt(int v, int &total, int *ws) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int inc = wave_incl(v);
-    if (lane == 63) ws[w] = inc;
-    __syncthreads();
-    int off = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < NT / 64; ++i) {
-        int s = ws[i];
-        off += i < w ? s : 0;
-        tot += s;
-    }
-    __syncthreads();
-    total = tot;
-    return off + inc - v;
-}
 
 // One partition step of introselect on keys[L, R) (pivot of numba_kdtree.py:
 // 280-299, the two-pointer exchanges of 313-350 as one parallel exchange,
@@ -741,8 +744,15 @@ hipError_t tree_medium_select(const TreeBuildArgs &a, int count, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t tree_small_select(const TreeBuildArgs &a, int count, hipStream_t st) {
-    hipLaunchKernelGGL(small_select_kernel, dim3((unsigned)count), dim3(kTB), 0, st, count, a.seg, a.A);
+hipError_t tree_small_select(const TreeBuildArgs &a, int count, int max_size, hipStream_t st) {
+    if (max_size <= 128)
+        hipLaunchKernelGGL((small_select_kernel<128, 64>), dim3((unsigned)count), dim3(64), 0, st, count, a.seg, a.A);
+    else if (max_size <= 512)
+        hipLaunchKernelGGL((small_select_kernel<512, 128>), dim3((unsigned)count), dim3(128), 0, st, count, a.seg,
+                           a.A);
+    else
+        hipLaunchKernelGGL((small_select_kernel<kSmall, kTB>), dim3((unsigned)count), dim3(kTB), 0, st, count, a.seg,
+                           a.A);
     return hipGetLastError();
 }
 
