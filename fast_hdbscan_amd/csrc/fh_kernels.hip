@@ -223,9 +223,11 @@ struct BoruvkaVisitor {
 
     // boruvka.py:176-192: prune on the point's own best, on the component
     // bound (strict comparisons) and on single-component nodes.
+    // B is the component bound as last read (at the start and on entering
+    // each leaf): it only ever decreases, so a stale value prunes less but
+    // never wrongly.
     __device__ __forceinline__ bool enter(int node, float lb) {
         if (lb > best) return false;
-        B = ld_relaxed(bound_c);
         if (lb > B || core_i > B) return false;
         if (node_comp[node] == comp) return false;
         return true;
@@ -238,6 +240,7 @@ struct BoruvkaVisitor {
     __device__ __forceinline__ void leaf(int node) {
         const int2 r = range[node];
         const int dd = D > 0 ? D : d;
+        B = ld_relaxed(bound_c);
         for (int i = r.x; i < r.y; ++i) {
             if (comp_p[i] == comp) continue;
             const float cj = core_p[i];
@@ -283,7 +286,7 @@ __global__ __launch_bounds__(256) void boruvka_query_kernel(QueryArgs a) {
     v.d = a.d;
     v.best = kInf;
     v.best_j = -1;
-    v.B = kInf;
+    v.B = ld_relaxed(v.bound_c);
     walk<D>(a.box, a.levels, a.d, v);
     if (v.best_j >= 0) {
         // Positive floats order like their bit patterns: the packed key
