@@ -339,6 +339,13 @@ def side_forms(Xd, ms):
         hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w)
         el = time.perf_counter() - t0
         res[f"workers_{w}"] = {"s": el, "datasets_per_s": len(data) / el, "points_per_s": pts / el}
+    # the whole 4,096-dataset workload once, 8 workers
+    data = workloads.batch_datasets(4096)
+    pts = sum(x.shape[0] for x in data)
+    t0 = time.perf_counter()
+    hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=8)
+    el = time.perf_counter() - t0
+    res["all_4096_workers_8"] = {"points": pts, "s": el, "datasets_per_s": len(data) / el, "points_per_s": pts / el}
     out["batched"] = res
     return out
 
