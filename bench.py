@@ -34,8 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE x 1.0, factors from a plain 256 MiB copy) on this workload;
 # command lines and raw counts in profiles/pmc_2Mx2D_traffic.md.
 PMC_TRAFFIC_BYTES = {
-    "boruvka_query_kernel<2, 2>": 67.8e6 + 11.4e6,
-    "knn_kernel<2>": 190.7e6 + 522.5e6,
+    "boruvka_query_kernel<2, 2>": 68.0e6 + 11.4e6,
+    "knn_kernel<2>": 190.9e6 + 522.5e6,
 }
 
 
