@@ -356,29 +356,27 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
         ++nrows;
     };
     constexpr size_t kAhead = 16;
-    // eliminate_branch (cluster_trees.py:115-145) writes the branch's points
-    // in breadth-first order.  The main walk below only reserves each
-    // branch's rows (their number is the branch size); the branches are
-    // expanded afterwards, in parallel, into their reserved ranges.
-    struct Task {
-        int32_t branch;
-        int32_t parent_node;
-        float lam;
-        int64_t off;
-    };
-    // (a thread_local named inside the worker lambdas would be each worker's
-    // own copy: bind the caller's instance to a local reference)
-    static thread_local std::vector<Task> tasks_tl;
-    std::vector<Task> &tasks = tasks_tl;
-    tasks.clear();
-    // sz: the branch's point count (its parent's row holds it)
-    auto eliminate = [&](int32_t branch, int32_t sz, int64_t parent_node, double lam) {
+    // eliminate_branch (cluster_trees.py:115-145): the branch's points in
+    // breadth-first order, written where the walk stands.  (A -1 leaf adds
+    // a row but no size, so rows cannot be reserved by size here.)
+    static thread_local std::vector<int32_t> sub_tl;
+    std::vector<int32_t> &sub = sub_tl;
+    auto eliminate = [&](int32_t branch, int64_t parent_node, double lam) {
         if (branch < num_points) {
             row(parent_node, branch, lam, 1.0f);
             return;
         }
-        tasks.push_back({branch, static_cast<int32_t>(parent_node), static_cast<float>(lam), nrows});
-        nrows += sz;
+        sub.clear();
+        sub.push_back(branch);
+        for (size_t head = 0; head < sub.size(); ++head) {
+            const int32_t x = sub[head];
+            if (x < num_points) {
+                row(parent_node, x, lam, 1.0f);
+            } else {
+                sub.push_back(rows[x - num_points].left);
+                sub.push_back(rows[x - num_points].right);
+            }
+        }
     };
     // The reference walks the whole breadth-first node list and skips
     // points and nodes inside eliminated branches (cluster_trees.py:228-230).
@@ -406,9 +404,13 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
         const double d = h.delta(node - num_points);
         const double lam = lambda_method == kLambdaHdbscan ? (d > 0.0 ? 1.0 / d : kInfD) : std::exp(-d);
         // counts as float32 (weighted sizes with weights), compared in
-        // float64 as numba does
-        const float lc = h.weighted ? h.lw[node - num_points] : static_cast<float>(r.left_size);
-        const float rc = h.weighted ? h.rw[node - num_points] : static_cast<float>(r.right_size);
+        // float64 as numba does.  A point child counts its sample weight,
+        // 1 without weights (cluster_trees.py:252-260), also a -1 child,
+        // whose recorded size is the wrap-around slot's
+        const float lc = h.weighted ? h.lw[node - num_points]
+                                    : (left < num_points ? 1.0f : static_cast<float>(r.left_size));
+        const float rc = h.weighted ? h.rw[node - num_points]
+                                    : (right < num_points ? 1.0f : static_cast<float>(r.right_size));
         const double dl = lc, dr = rc;
         if (left < num_points && dr >= dm) {
             RL(right, static_cast<int32_t>(parent_node));
@@ -416,15 +418,15 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
             push(right);
         } else if (dl < dm && dr >= dm) {
             RL(right, static_cast<int32_t>(parent_node));
-            eliminate(left, r.left_size, parent_node, lam);
+            eliminate(left, parent_node, lam);
             push(right);
         } else if (dl >= dm && dr < dm) {
             RL(left, static_cast<int32_t>(parent_node));
-            eliminate(right, r.right_size, parent_node, lam);
+            eliminate(right, parent_node, lam);
             push(left);
         } else if (dl < dm && dr < dm) {
-            eliminate(left, r.left_size, parent_node, lam);
-            eliminate(right, r.right_size, parent_node, lam);
+            eliminate(left, parent_node, lam);
+            eliminate(right, parent_node, lam);
         } else {
             c.cluster_rows.push_back(nrows);
             c.cluster_rows.push_back(nrows + 1);
@@ -437,47 +439,6 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
             push(left);
             push(right);
         }
-    }
-    // expand the eliminated branches
-    auto expand = [&](size_t t0, size_t t1) {
-        std::vector<int32_t> sub;
-        for (size_t t = t0; t < t1; ++t) {
-            const Task &tk = tasks[t];
-            int64_t at = tk.off;
-            sub.clear();
-            sub.push_back(tk.branch);
-            for (size_t head = 0; head < sub.size(); ++head) {
-                if (head + kAhead < sub.size() && sub[head + kAhead] >= num_points)
-                    prefetch(&rows[sub[head + kAhead] - num_points]);
-                const int32_t node = sub[head];
-                if (node < num_points) {
-                    c.parent[at] = tk.parent_node;
-                    c.child[at] = node;
-                    c.lambda[at] = tk.lam;
-                    c.size[at] = 1.0f;
-                    ++at;
-                } else {
-                    sub.push_back(rows[node - num_points].left);
-                    sub.push_back(rows[node - num_points].right);
-                }
-            }
-        }
-    };
-    const int64_t elim_rows = tasks.empty() ? 0 : nrows;
-    const int nt = elim_rows > (1 << 16) ? fh::HostPool::width() : 1;
-    if (nt == 1) {
-        expand(0, tasks.size());
-    } else {
-        // split the task list into ranges of about equal row counts
-        std::vector<size_t> cut(nt + 1, tasks.size());
-        cut[0] = 0;
-        size_t t = 0;
-        for (int k = 1; k < nt; ++k) {
-            const int64_t target = nrows * k / nt;
-            while (t < tasks.size() && tasks[t].off < target) ++t;
-            cut[k] = t;
-        }
-        fh::HostPool::run(nt, [&](int k) { expand(cut[k], cut[k + 1]); });
     }
     c.rows = nrows;
     c.next_label = next_label;

@@ -320,3 +320,22 @@ def test_concurrent_label_stages_match():
             np.testing.assert_array_equal(x, y)
         for f in ("parent", "child", "lambda_val", "child_size"):
             np.testing.assert_array_equal(getattr(got[3], f), getattr(want[3], f))
+
+
+def test_k_exceeds_n_condensed_tree_matches_oracle():
+    # min_samples > n: +inf rows with -1 endpoints, whose leaf adds a
+    # condensed row but no size; an eliminated branch holding it must list
+    # all its leaves (regression: a size-based reservation dropped a row)
+    from fast_hdbscan_amd import labels as PL
+
+    X = np.array([[3, 0], [0, 4], [4, 3], [0, 4], [3, 0], [2, 1], [0, 1], [5, 5]], dtype=np.float64)
+    e, _, _ = O.compute_minimum_spanning_tree(X, 11)
+    assert (e[:, :2] < 0).any()
+    for mcs, single in ((6, True), (6, False), (3, True), (2, False)):
+        kw = dict(min_cluster_size=mcs, allow_single_cluster=single)
+        a = PL.clusters_from_spanning_tree(e, **kw)
+        b = LB.clusters_from_spanning_tree(e, **kw)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        for f in ("parent", "child", "lambda_val", "child_size"):
+            np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
