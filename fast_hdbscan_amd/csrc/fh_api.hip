@@ -619,13 +619,18 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     }
     const float margin_c = 8.0f * dp + 64.0f;
     fh::BruteScanArgs bsa{};
+    bool knn_d_rows = true;  // the kNN distance rows were written
     if (brute) {
         FH_TRY(brute_knn(ctx, n, d, levels, kq, ms > 1 ? 1 : 0, bsa, S));
     } else if (ctx->learned_mode != 0 && !weighted && ms > 1 && d <= 8 && kq <= 128 && n >= 1000) {
         FH_TRY(learned_knn(ctx, n, d, levels, kq, seed ? ctx->nb_pos.p : nullptr, seed ? ctx->nb_d.p : nullptr, S));
     } else {
-        FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, ctx->knn_d.p, ctx->knn_i.p, ctx->core_o.p,
-                       ctx->core_p.p, (ms > 1 && !weighted) ? 1 : 0, seed ? ctx->nb_pos.p : nullptr,
+        // the distance rows are read only by the weighted core and, when
+        // the core is not a kNN column (min_samples == 1), by the forest
+        // (with the core the k-th distance, max(core_i, d_ij) == core_i)
+        knn_d_rows = weighted || ms <= 1;
+        FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, knn_d_rows ? ctx->knn_d.p : nullptr, ctx->knn_i.p,
+                       ctx->core_o.p, ctx->core_p.p, (ms > 1 && !weighted) ? 1 : 0, seed ? ctx->nb_pos.p : nullptr,
                        seed ? ctx->nb_d.p : nullptr, weighted ? 1 : 0));
     }
     if (weighted)
@@ -643,8 +648,8 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
 
     // -- kNN-seeded forest
     fh::CycleBufs cb{ctx->S0.p, ctx->M0.p, ctx->S1.p, ctx->M1.p};
-    FH_HIP(fh::launch_init_edges(ni, kq, ctx->knn_i.p, ctx->knn_d.p, ctx->core_o.p, ctx->succ.p, ctx->dst_raw.p,
-                                 ctx->w_init.p, st));
+    FH_HIP(fh::launch_init_edges(ni, kq, ctx->knn_i.p, knn_d_rows ? ctx->knn_d.p : nullptr, ctx->core_o.p,
+                                 ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p, st));
     FH_HIP(fh::launch_cycle_keep(ni, ctx->succ.p, cb, ctx->keep.p, st));
     FH_HIP(fh::launch_excl_scan(ni, ctx->keep.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 0, st));
     FH_HIP(fh::launch_init_emit_union(ni, ctx->keep.p, ctx->offs.p, ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p,

@@ -18,7 +18,7 @@ struct KnnArgs {
     int k;
     int levels;
     int block;
-    float *out_d;           // [rows, k] squared distances, ascending
+    float *out_d;           // [rows, k] squared distances, ascending (nullptr: not written)
     int *out_i;             // [rows, k] original indices
     float *core_o;          // optional: core by original index
     float *core_p;          // optional: core by query (tree) position

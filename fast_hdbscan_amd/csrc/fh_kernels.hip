@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     const int64_t row = a.query_pos ? a.perm[qp] : (a.query_perm ? a.query_perm[q] : q);
     for (int j = 0; j < k; ++j) {
         const int pj = h.I(j);
-        a.out_d[row * k + j] = h.P(j);
+        if (a.out_d != nullptr) a.out_d[row * k + j] = h.P(j);
         a.out_i[row * k + j] = pj >= 0 ? a.perm[pj] : -1;
     }
     if (a.nb_pos != nullptr) {
@@ -488,7 +488,8 @@ __global__ void init_edges_kernel(int n, int k, const int *__restrict__ knn_i, c
         const int kk = knn_i[(size_t)i * k + j];
         const int kw = kk < 0 ? kk + n : kk;
         if (ci >= core_o[kw]) {
-            const float dj = knn_d[(size_t)i * k + j];
+            // without distance rows the core is the k-th distance: d_ij <= ci
+            const float dj = knn_d != nullptr ? knn_d[(size_t)i * k + j] : ci;
             s = kw;
             draw = kk;
             wi = ci > dj ? ci : dj;
