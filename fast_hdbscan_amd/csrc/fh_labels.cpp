@@ -453,17 +453,22 @@ void condense_walk(int64_t n, Hierarchy &h, int64_t mcs, Condensed &c, int lambd
 // a split or to a node with no continuing child.  The walk's order is
 // breadth-first: by depth, then left to right.  Two chains present at one
 // depth lie in disjoint subtrees, so their left-to-right order is the
-// preorder (left child first) of the chains' tree.  Passes:
-//   1. merge rows in decreasing index order (a parent row has a larger
-//      index than its children): chain of each processed node, its row
-//      count and float32 lambda;
-//   2. tree of chains: preorder and the depth of each chain's head;
-//   3. counting sorts: nodes grouped by chain (top-down within a chain),
-//      then placed by depth, chains in preorder: the walk's order;
-//   4. in that order: each node's first row and the cluster ids of the
-//      splits (next_label counts splits in the walk's order);
-//   5. rows written in parallel from those offsets, eliminated branches
-//      expanded in place (breadth-first, as eliminate_branch does).
+// preorder (left child first) of the chains' tree.  Passes (all but the
+// small per-chain steps on the worker pool):
+//   1a. classify every merge row: rows it writes, float32 lambda, and for
+//       each merge child its parent row and role (dropped, same chain, new
+//       chain's head);
+//   1b. chain of every row, top-down (a parent row has a larger index than
+//       its children): row ranges resolved locally, the few rows whose
+//       parent lies in another range resolved in order, then substituted;
+//   2.  nodes grouped by chain, top-down within each;
+//   4.  the walk's order over depth ranges: each range starts from its
+//       chains in preorder and its first slot, row and cluster id (from
+//       per-depth node counts and per-range row and split totals): each
+//       node's first row, the cluster ids of the splits (next_label counts
+//       splits in the walk's order);
+//   5.  rows written from those offsets; eliminated branches expanded
+//       afterwards, breadth-first as eliminate_branch does.
 // With want_scores, score_condensed_tree_nodes (cluster_trees.py:656-669)
 // is summed per chain: a cluster's rows are its chain's rows, top-down,
 // i.e. in row order, so each float32 sum runs in the reference's order.
