@@ -202,7 +202,9 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
     """Labels and membership strengths for a float32 torch tensor resident
     on a HIP device: MST on the device, rows sorted on the device, then the
     host label stage.  Returns (labels int64, strengths float32) on the
-    host."""
+    host.  ``work`` (a dict, kept by the caller between calls) holds the
+    device, pinned and output buffers; with it the returned arrays are
+    reused by the next call."""
     import ctypes
     import time
 
@@ -259,8 +261,13 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
     params = label_params(min_cluster_size, cluster_selection_method, max_cluster_size, allow_single_cluster,
                           cluster_selection_epsilon)
     phases = {}
+    # label outputs kept in `w` too: large fresh arrays would cost their page
+    # faults on every call (the results alias them until the next call)
+    if w.get("labels") is None or w["labels"][0].shape[0] < n:
+        w["labels"] = (np.empty(n, np.int64), np.empty(n, np.float32))
     labels, strengths, _, _ = labels_from_sorted_arrays(n, hs[:m].numpy(), hd[:m].numpy(), hw[:m].numpy(), params,
-                                                        phases=phases, point_order=oh[:n].numpy(), groups=groups)
+                                                        phases=phases, point_order=oh[:n].numpy(), groups=groups,
+                                                        out=w["labels"] if work is not None else None)
     t2 = time.perf_counter()
     if stats is not None:
         stats["t_sort_copy_ms"] = 1e3 * (t1 - t0)

@@ -27,7 +27,7 @@ def label_params(min_cluster_size=10, cluster_selection_method="eom", max_cluste
 
 
 def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, phases: dict | None = None,
-                              dist=None, sample_weights=None, point_order=None, groups=None, threads=0):
+                              dist=None, sample_weights=None, point_order=None, groups=None, threads=0, out=None):
     """Run the host label stage on MST rows already sorted by (weight,
     source, target): squared float32 weights w_sq, or float64 distances
     dist (then w_sq may be None); float32 sample weights optional.  With
@@ -35,8 +35,9 @@ def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, pha
     point_order[p] is the point id.  ``groups`` = (k, row_order int32 [k],
     group_sizes int32 [G]) as fh_linkage_plan gives them lets the linkage
     of rows [0, k) run on ``threads`` host threads.  Neither changes the
-    outputs.  Returns (labels int64, strengths float32, linkage or None,
-    CondensedTree or None)."""
+    outputs.  ``out`` = (labels int64 [>= n], strengths float32 [>= n]) to
+    fill instead of new arrays.  Returns (labels int64, strengths float32,
+    linkage or None, CondensedTree or None)."""
     src = np.ascontiguousarray(src, dtype=np.int32)
     dst = np.ascontiguousarray(dst, dtype=np.int32)
     w_sq = None if w_sq is None else np.ascontiguousarray(w_sq, dtype=np.float32)
@@ -52,8 +53,14 @@ def labels_from_sorted_arrays(n, src, dst, w_sq, params, return_trees=False, pha
             gs = np.ascontiguousarray(gs, dtype=np.int32)
             order.prefix_rows, order.n_groups = int(k), int(gs.shape[0])
             order.row_order, order.group_sizes = _lib.ptr(ro), _lib.ptr(gs)
-    labels = np.empty(n, np.int64)
-    strengths = np.empty(n, np.float32)
+    if out is not None:
+        labels, strengths = out[0][:n], out[1][:n]
+        if labels.dtype != np.int64 or strengths.dtype != np.float32 or labels.shape[0] < n or \
+                strengths.shape[0] < n or not (labels.flags.c_contiguous and strengths.flags.c_contiguous):
+            raise ValueError("out must be contiguous (int64 [n], float32 [n]) arrays")
+    else:
+        labels = np.empty(n, np.int64)
+        strengths = np.empty(n, np.float32)
     linkage = np.empty((max(n - 1, 0), 4), np.float64) if return_trees else None
     ct = None
     ctree = None
