@@ -72,7 +72,7 @@ struct fh_ctx {
     DevBuf<float> xdev, wdev;
     DevBuf<uint64_t> tkeys;
     DevBuf<uint32_t> obox;
-    DevBuf<int> taxis, tcnt, gtpos, ltpos, tactive;
+    DevBuf<int> taxis, tcnt, gtpos, ltpos, tactive, taux;
     DevBuf<fh::SegState> tseg;
     // pruned nearest-edge search
     DevBuf<int> nb_pos, cert_of_pos;
@@ -222,7 +222,8 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
     FH_TRY(ensure(ctx, ctx->tcnt, (size_t)n / fh::tree_chunk() + 2 * last_count + 1));
     FH_TRY(ensure(ctx, ctx->gtpos, n));
     FH_TRY(ensure(ctx, ctx->ltpos, n));
-    FH_TRY(ensure(ctx, ctx->tactive, 8));
+    FH_TRY(ensure(ctx, ctx->tactive, 16));
+    FH_TRY(ensure(ctx, ctx->taux, (size_t)3 * last_count));
     fh::TreeBuildArgs a{};
     a.X = Xd;
     a.d = d;
@@ -236,9 +237,13 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
     a.gtpos = ctx->gtpos.p;
     a.ltpos = ctx->ltpos.p;
     a.active = ctx->tactive.p;
+    a.aux = ctx->taux.p;
     hipStream_t st = ctx->stream;
     const int small = fh::tree_small_limit();
-    constexpr int kBatch = 4;
+    // steps launched between host checks: a large node needs ~9-14 steps
+    // to come down to the LDS range, so the first batch covers most levels
+    // in one check; a step whose nodes are all finished launches no-ops
+    constexpr int kFirstBatch = 12, kBatch = 4;
     FH_HIP(fh::tree_identity_keys(ni, a.A, st));
     for (int l = 0; l < L; ++l) {
         const int count = 1 << l;
@@ -253,16 +258,17 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
             FH_HIP(fh::tree_medium_select(a, count, st));
         } else if (maxs > small) {
             // multi-block steps until every node's active range fits in LDS
-            for (;;) {
-                FH_HIP(hipMemsetAsync(a.active, 0, sizeof(int) * kBatch, st));
-                for (int b = 0; b < kBatch; ++b) {
+            FH_HIP(fh::tree_big_begin(a, count, st));
+            for (int batch = kFirstBatch;; batch = kBatch) {
+                FH_HIP(hipMemsetAsync(a.active, 0, sizeof(int) * batch, st));
+                for (int b = 0; b < batch; ++b) {
                     fh::TreeBuildArgs ab = a;
                     ab.active = a.active + b;
                     FH_HIP(fh::tree_big_step(ab, count, maxs, st));
                 }
-                FH_HIP(hipMemcpyAsync(ctx->h_small, a.active, sizeof(int) * kBatch, hipMemcpyDeviceToHost, st));
+                FH_HIP(hipMemcpyAsync(ctx->h_small, a.active + batch - 1, sizeof(int), hipMemcpyDeviceToHost, st));
                 FH_HIP(hipStreamSynchronize(st));
-                if (ctx->h_small[kBatch - 1] == 0) break;
+                if (ctx->h_small[0] == 0) break;
             }
             // ranges that hit the depth limit while still large: sort them
             std::vector<fh::SegState> segs(count);
@@ -868,7 +874,7 @@ void fh_destroy(fh_ctx *c) {
     release(c->small); release(c->heap_i); release(c->flag); release(c->out_src); release(c->out_dst);
     release(c->out_nbr); release(c->range); release(c->best);
     release(c->xdev); release(c->wdev); release(c->tkeys); release(c->obox); release(c->taxis); release(c->tcnt);
-    release(c->gtpos); release(c->ltpos); release(c->tactive); release(c->tseg);
+    release(c->gtpos); release(c->ltpos); release(c->tactive); release(c->taux); release(c->tseg);
     release(c->nb_pos); release(c->cert_of_pos); release(c->nb_d); release(c->mincore); release(c->cert);
     release(c->sk1); release(c->sk1s); release(c->sk2); release(c->sk2s); release(c->sv1); release(c->sv1s);
     release(c->stemp);

@@ -16,7 +16,6 @@ enum SegFlag : int {
 // Introselect state of one node (numba_kdtree.py:361-386 locals).
 struct SegState {
     int L, R, nth, depth, flag;
-    int P, S, M;
     uint64_t pivot;
 };
 
@@ -31,6 +30,7 @@ struct TreeBuildArgs {
     SegState *seg;       // per node of the current level
     int *cnt;            // per (node, chunk) counts
     int *gtpos, *ltpos;  // n each: exchange partners
+    int *aux;            // 3 per node of the level: (S, P, M) of the current step
     int *active;         // step counters
 };
 
@@ -41,6 +41,11 @@ hipError_t tree_identity_keys(int n, uint64_t *A, hipStream_t st);
 hipError_t tree_level_bounds(const TreeBuildArgs &a, int level, int max_size, int identity, hipStream_t st);
 hipError_t tree_level_axis(const TreeBuildArgs &a, int level, int internal, hipStream_t st);
 hipError_t tree_level_keys(const TreeBuildArgs &a, int level, int max_size, int identity, hipStream_t st);
+// multi-block introselect steps of the SEG_LARGE nodes of a level: begin
+// chooses each one's first pivot; every step partitions around the pending
+// pivot, narrows, and chooses the next pivot (counted in *active), so the
+// steps may stop once a step counts no pivot
+hipError_t tree_big_begin(const TreeBuildArgs &a, int count, hipStream_t st);
 hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStream_t st);
 // steps of every SEG_LARGE node by one work-group each; leaves SEG_SMALL or
 // SEG_DONE (for levels whose nodes are at most tree_medium_limit() long)

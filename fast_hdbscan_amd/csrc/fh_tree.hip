@@ -246,9 +246,6 @@ __global__ void axis_kernel(LevelRef lv, int d, const uint32_t *__restrict__ obo
     st.nth = r.x + size / 2;
     st.depth = size > 16 ? 2 * floor_log2(size) : 0;
     st.flag = size > kSmall ? SEG_LARGE : SEG_SMALL;
-    st.P = 0;
-    st.S = 0;
-    st.M = 0;
     st.pivot = 0;
     seg[k] = st;
 }
@@ -288,25 +285,34 @@ __device__ __forceinline__ void median3_and_move(uint64_t *A, int L, int R) {
     swap64(A, mid, R - 1);
 }
 
-__global__ void big_pivot_kernel(int count, SegState *seg, uint64_t *A, int *active) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    SegState st = seg[k];
-    if (st.flag != SEG_LARGE) return;
+// Pivot of the next step of a large segment (numba_kdtree.py:301-318):
+// ranges of at most kSmall go to the in-LDS finish, ranges at the depth
+// limit to the sort.  A segment left SEG_LARGE has its pivot moved to R-1
+// and counts itself in *active.
+__device__ __forceinline__ void pivot_step(SegState &st, uint64_t *A, int *active) {
     const int size = st.R - st.L;
     if (size <= kSmall) {
-        seg[k].flag = SEG_SMALL;
+        st.flag = SEG_SMALL;
         return;
     }
     if (st.depth == 0) {
-        seg[k].flag = SEG_SORT;
+        st.flag = SEG_SORT;
         return;
     }
     st.depth -= 1;
     median3_and_move(A, st.L, st.R);
     st.pivot = A[st.R - 1];
-    seg[k] = st;
     atomicAdd(active, 1);
+}
+
+// First pivot of every large segment of a level.
+__global__ void big_pivot_kernel(int count, SegState *seg, uint64_t *A, int *active) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    pivot_step(st, A, active);
+    seg[k] = st;
 }
 
 // Blocks over (segment, chunk of [L, R-1)): count of keys < pivot.
@@ -326,36 +332,48 @@ __global__ __launch_bounds__(kTB) void big_count_kernel(int count, int cps, cons
     if (threadIdx.x == 0) cnt[(size_t)k * cps + c] = s < e ? tot : 0;
 }
 
-// One block per segment: exclusive scan of its chunk counts; P and S.
-__global__ __launch_bounds__(kTB) void big_scan_kernel(int count, int cps, SegState *seg, int *cnt) {
-    __shared__ int ws[kTB / 64];
-    const int k = blockIdx.x;
-    if (k >= count) return;
-    SegState st = seg[k];
-    if (st.flag != SEG_LARGE) return;
-    const int nch = (st.R - 1 - st.L + kChunk - 1) / kChunk;
-    int carry = 0;
-    for (int b0 = 0; b0 < nch; b0 += kTB) {
-        const int t = b0 + threadIdx.x;
-        const int v = t < nch ? cnt[(size_t)k * cps + t] : 0;
-        int tot;
-        const int ex = block_excl(v, tot, ws);
-        if (t < nch) cnt[(size_t)k * cps + t] = carry + ex;
-        carry += tot;
+// Sums over the nch chunk counts of one segment: those of chunks before c
+// (returned) and all of them (total).
+__device__ __forceinline__ int chunk_prefix(const int *__restrict__ cnt, int nch, int c, int &total, int *ws) {
+    int pre = 0, all = 0;
+    for (int t = threadIdx.x; t < nch; t += kTB) {
+        const int v = cnt[t];
+        all += v;
+        pre += t < c ? v : 0;
     }
-    if (threadIdx.x == 0) {
-        seg[k].S = carry;
-        seg[k].P = st.L + carry;
-        seg[k].M = 0;  // written by the block holding position P when P < R-1
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        pre += __shfl_xor(pre, o, 64);
+        all += __shfl_xor(all, o, 64);
     }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        ws[2 * w] = pre;
+        ws[2 * w + 1] = all;
+    }
+    __syncthreads();
+    pre = 0;
+    all = 0;
+#pragma unroll
+    for (int i = 0; i < kTB / 64; ++i) {
+        pre += ws[2 * i];
+        all += ws[2 * i + 1];
+    }
+    __syncthreads();
+    total = all;
+    return pre;
 }
 
-// Blocks over (segment, chunk): ranks of misplaced keys (the partners of
-// the reference's two-pointer exchanges).
-__global__ __launch_bounds__(kTB) void big_rank_kernel(int count, int cps, SegState *seg,
+// Blocks over (segment, chunk): the pivot's final position P = L + S (S
+// keys < pivot) and the ranks of misplaced keys (the partners of the
+// reference's two-pointer exchanges, numba_kdtree.py:320-348).  aux holds
+// (S, P, M) per segment, M = number of exchanges: written by the block
+// holding position P, or by chunk 0 when P = R-1 (then M = 0).
+__global__ __launch_bounds__(kTB) void big_rank_kernel(int count, int cps, const SegState *__restrict__ seg,
                                                        const uint64_t *__restrict__ A, const int *__restrict__ cnt,
-                                                       int *gtpos, int *ltpos) {
+                                                       int *gtpos, int *ltpos, int *aux) {
     __shared__ int ws[kTB / 64];
+    __shared__ int ws2[2 * (kTB / 64)];
     const int k = blockIdx.x / cps, c = blockIdx.x % cps;
     if (k >= count) return;
     const SegState st = seg[k];
@@ -363,6 +381,15 @@ __global__ __launch_bounds__(kTB) void big_rank_kernel(int count, int cps, SegSt
     const int s = st.L + c * kChunk;
     const int e = min(st.R - 1, s + kChunk);
     if (s >= e) return;
+    const int nch = (st.R - 1 - st.L + kChunk - 1) / kChunk;
+    int S;
+    const int pre = chunk_prefix(cnt + (size_t)k * cps, nch, c, S, ws2);
+    const int P = st.L + S;
+    if (c == 0 && threadIdx.x == 0) {
+        aux[3 * k] = S;
+        aux[3 * k + 1] = P;
+        if (P == st.R - 1) aux[3 * k + 2] = 0;
+    }
     // each thread owns kPer consecutive elements
     const int base = s + threadIdx.x * kPer;
     uint32_t ltmask = 0;
@@ -375,17 +402,17 @@ __global__ __launch_bounds__(kTB) void big_rank_kernel(int count, int cps, SegSt
         v += lt ? 1 : 0;
     }
     int tot;
-    int before = cnt[(size_t)k * cps + c] + block_excl(v, tot, ws);
+    int before = pre + block_excl(v, tot, ws);
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int p = base + j;
         if (p >= e) break;
         const bool lt = (ltmask >> j) & 1u;
-        if (p == st.P) seg[k].M = (st.P - st.L) - before;
-        if (p < st.P) {
+        if (p == P) aux[3 * k + 2] = (P - st.L) - before;
+        if (p < P) {
             if (!lt) gtpos[st.L + (p - st.L) - before] = p;
         } else if (lt) {
-            ltpos[st.L + (st.S - 1 - before)] = p;
+            ltpos[st.L + (S - 1 - before)] = p;
         }
         before += lt ? 1 : 0;
     }
@@ -394,29 +421,35 @@ __global__ __launch_bounds__(kTB) void big_rank_kernel(int count, int cps, SegSt
 // Blocks over (segment, chunk of pairs): the exchanges.
 __global__ __launch_bounds__(kTB) void big_swap_kernel(int count, int cps, const SegState *__restrict__ seg,
                                                        uint64_t *A, const int *__restrict__ gtpos,
-                                                       const int *__restrict__ ltpos) {
+                                                       const int *__restrict__ ltpos, const int *__restrict__ aux) {
     const int k = blockIdx.x / cps, c = blockIdx.x % cps;
     if (k >= count) return;
     const SegState st = seg[k];
     if (st.flag != SEG_LARGE) return;
     const int m0 = c * kChunk;
-    const int m1 = min(st.M, m0 + kChunk);
+    const int m1 = min(aux[3 * k + 2], m0 + kChunk);
     for (int m = m0 + threadIdx.x; m < m1; m += kTB) swap64(A, gtpos[st.L + m], ltpos[st.L + m]);
 }
 
-// Pivot into place (numba_kdtree.py:349) and narrowing (376-383).
-__global__ void big_finish_kernel(int count, SegState *seg, uint64_t *A) {
+// Pivot into place (numba_kdtree.py:349), narrowing (376-383), then the
+// pivot of the segment's next step.
+__global__ void big_finish_kernel(int count, SegState *seg, uint64_t *A, const int *__restrict__ aux, int *active) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     SegState st = seg[k];
     if (st.flag != SEG_LARGE) return;
-    swap64(A, st.P, st.R - 1);
-    if (st.nth < st.P)
-        st.R = st.P;
-    else if (st.nth > st.P)
-        st.L = st.P + 1;
-    else
+    const int P = aux[3 * k + 1];
+    swap64(A, P, st.R - 1);
+    if (st.nth < P) {
+        st.R = P;
+    } else if (st.nth > P) {
+        st.L = P + 1;
+    } else {
         st.flag = SEG_DONE;
+        seg[k] = st;
+        return;
+    }
+    pivot_step(st, A, active);
     seg[k] = st;
 }
 
@@ -724,17 +757,21 @@ hipError_t tree_level_keys(const TreeBuildArgs &a, int level, int max_size, int 
     return hipGetLastError();
 }
 
+hipError_t tree_big_begin(const TreeBuildArgs &a, int count, hipStream_t st) {
+    hipLaunchKernelGGL(big_pivot_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.active);
+    return hipGetLastError();
+}
+
 hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStream_t st) {
     const int cps = (max_size + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(big_pivot_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.active);
     hipLaunchKernelGGL(big_count_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
                        a.cnt);
-    hipLaunchKernelGGL(big_scan_kernel, dim3((unsigned)count), dim3(kTB), 0, st, count, cps, a.seg, a.cnt);
     hipLaunchKernelGGL(big_rank_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
-                       a.cnt, a.gtpos, a.ltpos);
+                       a.cnt, a.gtpos, a.ltpos, a.aux);
     hipLaunchKernelGGL(big_swap_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
-                       a.gtpos, a.ltpos);
-    hipLaunchKernelGGL(big_finish_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A);
+                       a.gtpos, a.ltpos, a.aux);
+    hipLaunchKernelGGL(big_finish_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.aux,
+                       a.active);
     return hipGetLastError();
 }
 
