@@ -259,9 +259,22 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         nt = std::max(1, std::min(nt, 16));
         if (nt == 1 || k < (1 << 15)) {
             for (int64_t g = 0; g < G; ++g) run_group(g);
+            first_seq = k;
         } else {
-            // the largest group on the calling thread; the others in chunks
-            // of about k / (16 nt) rows taken from a shared counter
+            // Task 0 runs the largest group.  The other tasks first mark
+            // that group's points (uf[p].pad = 1, a field no step touches)
+            // in pieces, then run the other groups in chunks of about
+            // k / (16 nt) rows, all taken from a shared counter.  Task 1
+            // then waits for those items and walks rows [k, m) in order
+            // until task 0 is done: a row whose component at its step holds
+            // no point of the largest group depends on earlier rows of that
+            // component only, and those are all done, so it runs now; a row
+            // that joins the largest group's component is kept for later
+            // and flags the other side's root (pad = 1).  After every task
+            // has finished, the kept rows run in order, then the rows from
+            // where task 1 stopped: the rows run early touch sets that no
+            // later-run row touches before them, so every row sees the sets
+            // it sees in the sequential order.
             std::vector<int64_t> cut{0};
             const int64_t target = std::max<int64_t>(1, k / (16 * static_cast<int64_t>(nt)));
             int64_t acc = 0;
@@ -274,19 +287,74 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
             }
             if (cut.back() != G) cut.push_back(G);
             const int64_t nchunks = static_cast<int64_t>(cut.size()) - 1;
-            std::atomic<int64_t> next{0};
-            auto worker = [&](bool take_big) {
-                if (take_big) run_group(big);
+            const int64_t b0 = start[big], b1 = start[big + 1];
+            const int64_t npieces = std::min<int64_t>(4 * nt, (b1 - b0 + 32767) / 32768);
+            const int64_t items = npieces + nchunks;
+            std::atomic<int64_t> next{0}, done{0};
+            std::atomic<bool> big_done{false};
+            int64_t resume = k;  // first row left to the sequential pass
+            static thread_local std::vector<int64_t> kept_tl;
+            std::vector<int64_t> &kept = kept_tl;
+            kept.clear();
+            kept.reserve(static_cast<size_t>(m - k));
+            auto tail_early = [&] {
+                while (done.load(std::memory_order_acquire) < items) std::this_thread::yield();
+                int64_t i = k;
+                for (; i < m; ++i) {
+                    // past task 0's end this walk is the critical path: stop
+                    if ((i & 63) == 0 && big_done.load(std::memory_order_relaxed)) break;
+                    if (i + kAhead < m) {
+                        prefetch(&uf[src[i + kAhead]]);
+                        prefetch(&uf[dst[i + kAhead]]);
+                    }
+                    const int32_t a = src[i], b = dst[i];
+                    // points of the largest group are never followed: task
+                    // 0 may be changing their sets
+                    const int32_t ra = uf[a].pad ? -1 : find(a);
+                    const int32_t rb = uf[b].pad ? -1 : find(b);
+                    const bool ka = ra < 0 || uf[ra].pad, kb = rb < 0 || uf[rb].pad;
+                    if (ka || kb) {
+                        if (!ka) uf[ra].pad = 1;
+                        if (!kb) uf[rb].pad = 1;
+                        kept.push_back(i);
+                    } else {
+                        step(i);
+                    }
+                }
+                resume = i;
+            };
+            auto worker = [&](int t) {
+                if (t == 0) {
+                    run_group(big);
+                    big_done.store(true, std::memory_order_relaxed);
+                }
                 for (;;) {
                     const int64_t c = next.fetch_add(1);
-                    if (c >= nchunks) break;
-                    for (int64_t g = cut[c]; g < cut[c + 1]; ++g)
-                        if (g != big) run_group(g);
+                    if (c >= items) break;
+                    if (c < npieces) {
+                        const int64_t e = b0 + (b1 - b0) * (c + 1) / npieces;
+                        // a point ends several rows, possibly in two pieces
+                        for (int64_t j = b0 + (b1 - b0) * c / npieces; j < e; ++j) {
+                            __atomic_store_n(&uf[src[ro[j]]].pad, 1, __ATOMIC_RELAXED);
+                            __atomic_store_n(&uf[dst[ro[j]]].pad, 1, __ATOMIC_RELAXED);
+                        }
+                    } else {
+                        const int64_t q = c - npieces;
+                        for (int64_t g = cut[q]; g < cut[q + 1]; ++g)
+                            if (g != big) run_group(g);
+                    }
+                    done.fetch_add(1, std::memory_order_release);
                 }
+                if (t == 1) tail_early();
             };
-            fh::HostPool::run(nt, [&](int t) { worker(t == 0); });
+            fh::HostPool::run(nt, worker);
+            for (size_t j = 0; j < kept.size(); ++j) {
+                if (j + kAhead < kept.size()) prefetch_row(kept[j + kAhead], 0);
+                if (j + kAhead2 < kept.size()) prefetch_row(kept[j + kAhead2], 1);
+                step(kept[j]);
+            }
+            first_seq = resume;
         }
-        first_seq = k;
     }
     FH_TMARK("link-groups");
     for (int64_t i = first_seq; i < m; ++i) {

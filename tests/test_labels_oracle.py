@@ -234,10 +234,12 @@ def test_point_order_changes_nothing(case):
         np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
 
 
-@pytest.mark.parametrize("frac,threads", [(0.9, 4), (0.5, 16), (1.0, 3)])
-def test_component_groups_change_nothing(frac, threads):
+@pytest.mark.parametrize("frac,threads,weighted", [(0.9, 4, False), (0.5, 16, False), (1.0, 3, False),
+                                                   (0.9, 2, False), (0.7, 8, True)])
+def test_component_groups_change_nothing(frac, threads, weighted):
     # rows [0, k) grouped by the components of their forest (computed here
-    # with scipy) and linked on several threads: identical outputs
+    # with scipy) and linked on several threads, rows [k, m) that do not join
+    # the largest group run beside it: identical outputs
     from scipy.sparse import coo_matrix
     from scipy.sparse.csgraph import connected_components
 
@@ -257,9 +259,10 @@ def test_component_groups_change_nothing(frac, threads):
     gs = np.bincount(comp)
     gs = gs[gs > 0].astype(np.int32)  # sorted component ids: groups in the same order as ro
     params = PL.label_params(20, "eom", np.inf, False, 0.0)
-    a = PL.labels_from_sorted_arrays(n, src, dst, None, params, return_trees=True, dist=dist)
+    w = np.random.default_rng(3).uniform(0.5, 2.0, n).astype(np.float32) if weighted else None
+    a = PL.labels_from_sorted_arrays(n, src, dst, None, params, return_trees=True, dist=dist, sample_weights=w)
     b = PL.labels_from_sorted_arrays(n, src, dst, None, params, return_trees=True, dist=dist, groups=(k, ro, gs),
-                                     threads=threads)
+                                     threads=threads, sample_weights=w)
     for x, y in zip(a[:3], b[:3]):
         np.testing.assert_array_equal(x, y)
     for f in ("parent", "child", "lambda_val", "child_size"):
