@@ -223,7 +223,9 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         }
     };
     // two prefetch stages: the endpoints' entries kAhead rows ahead, then
-    // (those entries having arrived) their parents' entries kAhead2 ahead
+    // (those entries having arrived) their parents' entries kAhead2 ahead;
+    // walks over a row list (not in row order) first fetch the row's
+    // endpoints and output slot 2 kAhead ahead (prefetch_list)
     constexpr int64_t kAhead = 32, kAhead2 = 16;
     auto prefetch_row = [&](int64_t i, int stage) {
         if (stage == 0) {
@@ -233,6 +235,11 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
             prefetch(&uf[uf[src[i]].parent]);
             prefetch(&uf[uf[dst[i]].parent]);
         }
+    };
+    auto prefetch_list = [&](int64_t i) {
+        prefetch(&src[i]);
+        prefetch(&dst[i]);
+        __builtin_prefetch(&rows[i], 1, 1);
     };
     FH_TSTART();
     int64_t first_seq = 0;
@@ -250,6 +257,7 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         auto run_group = [&](int64_t g) {
             const int64_t e = start[g + 1];
             for (int64_t j = start[g]; j < e; ++j) {
+                if (j + 2 * kAhead < e) prefetch_list(ro[j + 2 * kAhead]);
                 if (j + kAhead < e) prefetch_row(ro[j + kAhead], 0);
                 if (j + kAhead2 < e) prefetch_row(ro[j + kAhead2], 1);
                 step(ro[j]);
@@ -349,6 +357,7 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
             };
             fh::HostPool::run(nt, worker);
             for (size_t j = 0; j < kept.size(); ++j) {
+                if (j + 2 * kAhead < kept.size()) prefetch_list(kept[j + 2 * kAhead]);
                 if (j + kAhead < kept.size()) prefetch_row(kept[j + kAhead], 0);
                 if (j + kAhead2 < kept.size()) prefetch_row(kept[j + kAhead2], 1);
                 step(kept[j]);
