@@ -167,7 +167,6 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
     static thread_local std::vector<float> wsz_tl;
     if (weights) {
         wsz_tl.resize(n);
-        for (int64_t i = 0; i < n; ++i) wsz_tl[i] = weights[order ? order[i] : i];
         h.lw.resize(m);
         h.rw.resize(m);
     }
@@ -180,10 +179,15 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
     // (worker threads below must use the caller's instances)
     Set *uf = uf_tl.data();
     float *wsz = wsz_tl.data();
-    fh::HostPool::run(nt, [&](int t) {
-        for (int64_t i = n * t / nt, e = n * (t + 1) / nt; i < e; ++i)
-            uf[i] = {static_cast<int32_t>(i), 1, order ? order[i] : static_cast<int32_t>(i), 0};
-    });
+    auto init_sets = [&] {
+        fh::HostPool::run(nt, [&](int t) {
+            for (int64_t i = n * t / nt, e = n * (t + 1) / nt; i < e; ++i) {
+                uf[i] = {static_cast<int32_t>(i), 1, order ? order[i] : static_cast<int32_t>(i), 0};
+                if (weights) wsz[i] = weights[order ? order[i] : i];
+            }
+        });
+    };
+    init_sets();
     if (h.rows.size() < static_cast<size_t>(m)) h.rows.resize(m);
     Merge *rows = h.rows.data();
     float *lw = weights ? h.lw.data() : nullptr;
@@ -196,31 +200,35 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         }
         return x;
     };
-    auto step = [&](int64_t i) {
-        const int32_t a = find(src[i]);
-        const int32_t b = find(dst[i]);
-        const int32_t lc = uf[a].node, rc = uf[b].node;
-        const int32_t ls = uf[a].size, rs = uf[b].size;
+    // row i merging sets (node lc, size ls, weight wa) and (rc, rs, wb):
+    // the larger node id goes left
+    auto emit = [&](int64_t i, int32_t lc, int32_t rc, int32_t ls, int32_t rs, float wa, float wb) {
         Merge &r = rows[i];
         const bool lbig = lc > rc;
         r.left = lbig ? lc : rc;
         r.right = lbig ? rc : lc;
         r.left_size = lbig ? ls : rs;
         r.right_size = lbig ? rs : ls;
+        if (weights) {
+            lw[i] = lbig ? wa : wb;
+            rw[i] = lbig ? wb : wa;
+        }
+    };
+    auto step = [&](int64_t i) {
+        const int32_t a = find(src[i]);
+        const int32_t b = find(dst[i]);
+        const int32_t ls = uf[a].size, rs = uf[b].size;
+        const float wa = weights ? wsz[a] : 0.0f, wb = weights ? wsz[b] : 0.0f;
+        emit(i, uf[a].node, uf[b].node, ls, rs, wa, wb);
         int32_t root = a, child = b;
-        if (uf[a].size < uf[b].size) {
+        if (ls < rs) {
             root = b;
             child = a;
         }
         uf[child].parent = root;
         uf[root].size = ls + rs;
         uf[root].node = static_cast<int32_t>(n + i);
-        if (weights) {
-            const float wa = wsz[a], wb = wsz[b];
-            lw[i] = lbig ? wa : wb;
-            rw[i] = lbig ? wb : wa;
-            wsz[root] = wa + wb;
-        }
+        if (weights) wsz[root] = wa + wb;
     };
     // two prefetch stages: the endpoints' entries kAhead rows ahead, then
     // (those entries having arrived) their parents' entries kAhead2 ahead;
@@ -274,15 +282,20 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
             // in pieces, then run the other groups in chunks of about
             // k / (16 nt) rows, all taken from a shared counter.  Task 1
             // then waits for those items and walks rows [k, m) in order
-            // until task 0 is done: a row whose component at its step holds
-            // no point of the largest group depends on earlier rows of that
-            // component only, and those are all done, so it runs now; a row
-            // that joins the largest group's component is kept for later
-            // and flags the other side's root (pad = 1).  After every task
-            // has finished, the kept rows run in order, then the rows from
-            // where task 1 stopped: the rows run early touch sets that no
-            // later-run row touches before them, so every row sees the sets
-            // it sees in the sequential order.
+            // while task 0 may still run.  A row whose component at its step
+            // holds no point of the largest group depends on earlier rows of
+            // that component only, and those are all done, so it runs now.
+            // A row that joins the largest group's component (the giant)
+            // adds one other set to it: that set's node, size and weight
+            // are final when the row is reached, so the row is kept with
+            // them and the set's root is flagged (pad = 1) as part of the
+            // giant.  Once task 0 has finished, the kept rows follow from
+            // the giant's state after its own rows: row j merges the giant
+            // (node n + previous kept row, size and weight the running sums)
+            // with its set.  Rows run early touch no set of the giant, so
+            // every row sees the sets it sees in the sequential order.  A
+            // row with both ends in the giant closes a cycle (the rows are
+            // no forest): then every row runs again in order.
             std::vector<int64_t> cut{0};
             const int64_t target = std::max<int64_t>(1, k / (16 * static_cast<int64_t>(nt)));
             int64_t acc = 0;
@@ -299,18 +312,19 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
             const int64_t npieces = std::min<int64_t>(4 * nt, (b1 - b0 + 32767) / 32768);
             const int64_t items = npieces + nchunks;
             std::atomic<int64_t> next{0}, done{0};
-            std::atomic<bool> big_done{false};
-            int64_t resume = k;  // first row left to the sequential pass
-            static thread_local std::vector<int64_t> kept_tl;
-            std::vector<int64_t> &kept = kept_tl;
+            struct Kept {
+                int64_t row;
+                int32_t node, size;
+                float w;
+            };
+            static thread_local std::vector<Kept> kept_tl;
+            std::vector<Kept> &kept = kept_tl;
             kept.clear();
             kept.reserve(static_cast<size_t>(m - k));
+            bool cycle = false;
             auto tail_early = [&] {
                 while (done.load(std::memory_order_acquire) < items) std::this_thread::yield();
-                int64_t i = k;
-                for (; i < m; ++i) {
-                    // past task 0's end this walk is the critical path: stop
-                    if ((i & 63) == 0 && big_done.load(std::memory_order_relaxed)) break;
+                for (int64_t i = k; i < m; ++i) {
                     if (i + kAhead < m) {
                         prefetch(&uf[src[i + kAhead]]);
                         prefetch(&uf[dst[i + kAhead]]);
@@ -320,22 +334,22 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
                     // 0 may be changing their sets
                     const int32_t ra = uf[a].pad ? -1 : find(a);
                     const int32_t rb = uf[b].pad ? -1 : find(b);
-                    const bool ka = ra < 0 || uf[ra].pad, kb = rb < 0 || uf[rb].pad;
-                    if (ka || kb) {
-                        if (!ka) uf[ra].pad = 1;
-                        if (!kb) uf[rb].pad = 1;
-                        kept.push_back(i);
+                    const bool ga = ra < 0 || uf[ra].pad, gb = rb < 0 || uf[rb].pad;
+                    if (ga && gb) {
+                        cycle = true;
+                        return;
+                    }
+                    if (ga || gb) {
+                        const int32_t r = ga ? rb : ra;
+                        kept.push_back({i, uf[r].node, uf[r].size, weights ? wsz[r] : 0.0f});
+                        uf[r].pad = 1;
                     } else {
                         step(i);
                     }
                 }
-                resume = i;
             };
             auto worker = [&](int t) {
-                if (t == 0) {
-                    run_group(big);
-                    big_done.store(true, std::memory_order_relaxed);
-                }
+                if (t == 0) run_group(big);
                 for (;;) {
                     const int64_t c = next.fetch_add(1);
                     if (c >= items) break;
@@ -356,13 +370,21 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
                 if (t == 1) tail_early();
             };
             fh::HostPool::run(nt, worker);
-            for (size_t j = 0; j < kept.size(); ++j) {
-                if (j + 2 * kAhead < kept.size()) prefetch_list(kept[j + 2 * kAhead]);
-                if (j + kAhead < kept.size()) prefetch_row(kept[j + kAhead], 0);
-                if (j + kAhead2 < kept.size()) prefetch_row(kept[j + kAhead2], 1);
-                step(kept[j]);
+            if (cycle) {
+                init_sets();
+                first_seq = 0;
+            } else {
+                const int32_t g = find(src[ro[b0]]);
+                int32_t gnode = uf[g].node, gsize = uf[g].size;
+                float gw = weights ? wsz[g] : 0.0f;
+                for (const Kept &e : kept) {
+                    emit(e.row, gnode, e.node, gsize, e.size, gw, e.w);
+                    gnode = static_cast<int32_t>(n + e.row);
+                    gsize += e.size;
+                    gw += e.w;
+                }
+                first_seq = m;
             }
-            first_seq = resume;
         }
     }
     FH_TMARK("link-groups");

@@ -269,6 +269,38 @@ def test_component_groups_change_nothing(frac, threads, weighted):
         np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
 
 
+def test_component_groups_cycle_runs_in_order():
+    # a row after the grouped prefix that closes a cycle inside the largest
+    # group (the rows are no forest): the grouped linkage gives up its
+    # parallel tail and reproduces the sequential result
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    from fast_hdbscan_amd import labels as PL
+
+    X = workloads.anisotropic_mixture(120_000, 2, 12, 0.1, seed=9)
+    e, _, _ = O.compute_minimum_spanning_tree(X, 5)
+    n = e.shape[0] + 1
+    o = np.lexsort((e[:, 1], e[:, 0], e[:, 2]))
+    src, dst = e[o, 0].astype(np.int32), e[o, 1].astype(np.int32)
+    dist = e[o, 2].copy()
+    k = int((n - 1) * 0.9)
+    _, lab = connected_components(coo_matrix((np.ones(k), (src[:k], dst[:k])), shape=(n, n)), directed=False)
+    comp = lab[src[:k]]
+    big = np.bincount(comp).argmax()
+    pts = np.unique(np.concatenate([src[:k][comp == big], dst[:k][comp == big]]))
+    src[k + 100], dst[k + 100] = pts[0], pts[1]
+    ro = np.argsort(comp, kind="stable").astype(np.int32)
+    gs = np.bincount(comp)
+    gs = gs[gs > 0].astype(np.int32)
+    params = PL.label_params(20, "eom", np.inf, False, 0.0)
+    a = PL.labels_from_sorted_arrays(n, src, dst, None, params, return_trees=True, dist=dist)
+    b = PL.labels_from_sorted_arrays(n, src, dst, None, params, return_trees=True, dist=dist, groups=(k, ro, gs),
+                                     threads=4)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_component_groups_are_checked():
     from fast_hdbscan_amd import labels as PL
 
