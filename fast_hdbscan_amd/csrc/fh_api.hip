@@ -222,7 +222,9 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
     FH_TRY(ensure(ctx, ctx->tcnt, (size_t)n / fh::tree_chunk() + 2 * last_count + 1));
     FH_TRY(ensure(ctx, ctx->gtpos, n));
     FH_TRY(ensure(ctx, ctx->ltpos, n));
-    FH_TRY(ensure(ctx, ctx->tactive, 16));
+    // step counters (a batch's steps use slots from 0) and the sort count
+    constexpr int kSortSlot = 15;
+    FH_TRY(ensure(ctx, ctx->tactive, kSortSlot + 1));
     FH_TRY(ensure(ctx, ctx->taux, (size_t)3 * last_count));
     fh::TreeBuildArgs a{};
     a.X = Xd;
@@ -237,6 +239,7 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
     a.gtpos = ctx->gtpos.p;
     a.ltpos = ctx->ltpos.p;
     a.active = ctx->tactive.p;
+    a.sorts = ctx->tactive.p + kSortSlot;
     a.aux = ctx->taux.p;
     hipStream_t st = ctx->stream;
     const int small = fh::tree_small_limit();
@@ -258,6 +261,7 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
             FH_HIP(fh::tree_medium_select(a, count, st));
         } else if (maxs > small) {
             // multi-block steps until every node's active range fits in LDS
+            FH_HIP(hipMemsetAsync(a.active, 0, sizeof(int) * (kSortSlot + 1), st));
             FH_HIP(fh::tree_big_begin(a, count, st));
             for (int batch = kFirstBatch;; batch = kBatch) {
                 FH_HIP(hipMemsetAsync(a.active, 0, sizeof(int) * batch, st));
@@ -266,16 +270,19 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
                     ab.active = a.active + b;
                     FH_HIP(fh::tree_big_step(ab, count, maxs, st));
                 }
-                FH_HIP(hipMemcpyAsync(ctx->h_small, a.active + batch - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+                FH_HIP(hipMemcpyAsync(ctx->h_small, a.active, sizeof(int) * (kSortSlot + 1), hipMemcpyDeviceToHost,
+                                      st));
                 FH_HIP(hipStreamSynchronize(st));
-                if (ctx->h_small[0] == 0) break;
+                if (ctx->h_small[batch - 1] == 0) break;
             }
-            // ranges that hit the depth limit while still large: sort them
-            std::vector<fh::SegState> segs(count);
-            FH_HIP(hipMemcpyAsync(segs.data(), a.seg, sizeof(fh::SegState) * count, hipMemcpyDeviceToHost, st));
-            FH_HIP(hipStreamSynchronize(st));
-            for (int k = 0; k < count; ++k)
-                if (segs[k].flag == fh::SEG_SORT) FH_HIP(fh::tree_sort_range(a.A, segs[k].L, segs[k].R, st));
+            // ranges that hit the depth limit while still large (rare): sort them
+            if (ctx->h_small[kSortSlot] > 0) {
+                std::vector<fh::SegState> segs(count);
+                FH_HIP(hipMemcpyAsync(segs.data(), a.seg, sizeof(fh::SegState) * count, hipMemcpyDeviceToHost, st));
+                FH_HIP(hipStreamSynchronize(st));
+                for (int k = 0; k < count; ++k)
+                    if (segs[k].flag == fh::SEG_SORT) FH_HIP(fh::tree_sort_range(a.A, segs[k].L, segs[k].R, st));
+            }
         }
         FH_HIP(fh::tree_small_select(a, count, maxs, st));
     }

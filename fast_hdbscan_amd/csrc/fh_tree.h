@@ -32,6 +32,7 @@ struct TreeBuildArgs {
     int *gtpos, *ltpos;  // n each: exchange partners
     int *aux;            // 3 per node of the level: (S, P, M) of the current step
     int *active;         // step counters
+    int *sorts;          // nodes of the level left to the sort (depth limit)
 };
 
 int tree_small_limit();

@@ -287,9 +287,9 @@ __device__ __forceinline__ void median3_and_move(uint64_t *A, int L, int R) {
 
 // Pivot of the next step of a large segment (numba_kdtree.py:301-318):
 // ranges of at most kSmall go to the in-LDS finish, ranges at the depth
-// limit to the sort.  A segment left SEG_LARGE has its pivot moved to R-1
-// and counts itself in *active.
-__device__ __forceinline__ void pivot_step(SegState &st, uint64_t *A, int *active) {
+// limit to the sort (counted in *sorts).  A segment left SEG_LARGE has its
+// pivot moved to R-1 and counts itself in *active.
+__device__ __forceinline__ void pivot_step(SegState &st, uint64_t *A, int *active, int *sorts) {
     const int size = st.R - st.L;
     if (size <= kSmall) {
         st.flag = SEG_SMALL;
@@ -297,6 +297,7 @@ __device__ __forceinline__ void pivot_step(SegState &st, uint64_t *A, int *activ
     }
     if (st.depth == 0) {
         st.flag = SEG_SORT;
+        atomicAdd(sorts, 1);
         return;
     }
     st.depth -= 1;
@@ -306,12 +307,12 @@ __device__ __forceinline__ void pivot_step(SegState &st, uint64_t *A, int *activ
 }
 
 // First pivot of every large segment of a level.
-__global__ void big_pivot_kernel(int count, SegState *seg, uint64_t *A, int *active) {
+__global__ void big_pivot_kernel(int count, SegState *seg, uint64_t *A, int *active, int *sorts) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     SegState st = seg[k];
     if (st.flag != SEG_LARGE) return;
-    pivot_step(st, A, active);
+    pivot_step(st, A, active, sorts);
     seg[k] = st;
 }
 
@@ -433,7 +434,8 @@ __global__ __launch_bounds__(kTB) void big_swap_kernel(int count, int cps, const
 
 // Pivot into place (numba_kdtree.py:349), narrowing (376-383), then the
 // pivot of the segment's next step.
-__global__ void big_finish_kernel(int count, SegState *seg, uint64_t *A, const int *__restrict__ aux, int *active) {
+__global__ void big_finish_kernel(int count, SegState *seg, uint64_t *A, const int *__restrict__ aux, int *active,
+                                  int *sorts) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     SegState st = seg[k];
@@ -449,7 +451,7 @@ __global__ void big_finish_kernel(int count, SegState *seg, uint64_t *A, const i
         seg[k] = st;
         return;
     }
-    pivot_step(st, A, active);
+    pivot_step(st, A, active, sorts);
     seg[k] = st;
 }
 
@@ -758,7 +760,8 @@ hipError_t tree_level_keys(const TreeBuildArgs &a, int level, int max_size, int 
 }
 
 hipError_t tree_big_begin(const TreeBuildArgs &a, int count, hipStream_t st) {
-    hipLaunchKernelGGL(big_pivot_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.active);
+    hipLaunchKernelGGL(big_pivot_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.active,
+                       a.sorts);
     return hipGetLastError();
 }
 
@@ -771,7 +774,7 @@ hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStr
     hipLaunchKernelGGL(big_swap_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
                        a.gtpos, a.ltpos, a.aux);
     hipLaunchKernelGGL(big_finish_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.aux,
-                       a.active);
+                       a.active, a.sorts);
     return hipGetLastError();
 }
 
