@@ -212,7 +212,8 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
 
     n = X.shape[0]
     w = work if work is not None else {}
-    src, dst, w_sq, core_sq, nbr = mst_device(X, min_samples, stats=stats, out=w.get("mst"))
+    src, dst, w_sq, _, _ = mst_device(X, min_samples, stats=stats, out=w.setdefault("mst", {}),
+                                      core_and_neighbors=False)
     m = n - 1
     dev = X.device
     if "sorted" not in w or w["sorted"][0].shape[0] < m or w["order"][0].shape[0] < n:

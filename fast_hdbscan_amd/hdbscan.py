@@ -72,13 +72,15 @@ def mst_arrays(X: np.ndarray, min_samples: int = 10, device: int = 0, stats: dic
 
 
 def mst_device(X, min_samples: int = 10, stats: dict | None = None, out: dict | None = None,
-               sample_weights=None):
+               sample_weights=None, core_and_neighbors: bool = True):
     """MST of a float32 torch tensor resident on a HIP device.
 
     Returns torch tensors on the same device: (src, dst, w_sq, core_sq,
-    neighbors).  ``out`` may hold preallocated tensors under those names.
-    ``sample_weights`` (float32 tensor on the same device) selects the
-    weighted-core branch.
+    neighbors).  ``out`` may hold preallocated tensors under those names;
+    the tensors allocated here are stored in it (and reused by a later call
+    when large enough).  ``sample_weights`` (float32 tensor on the same
+    device) selects the weighted-core branch.  With ``core_and_neighbors``
+    False the library writes neither and both come back as None.
     """
     import torch
 
@@ -100,20 +102,28 @@ def mst_device(X, min_samples: int = 10, stats: dict | None = None, out: dict | 
     m = max(n - 1, 0)
     dev = X.device
     o = out if out is not None else {}
-    src = o.get("src") if o.get("src") is not None else torch.empty(max(m, 1), dtype=torch.int32, device=dev)
-    dst = o.get("dst") if o.get("dst") is not None else torch.empty(max(m, 1), dtype=torch.int32, device=dev)
-    w = o.get("w_sq") if o.get("w_sq") is not None else torch.empty(max(m, 1), dtype=torch.float32, device=dev)
-    core = o.get("core_sq") if o.get("core_sq") is not None else torch.empty(n, dtype=torch.float32, device=dev)
-    nbr = o.get("neighbors") if o.get("neighbors") is not None else torch.empty(
-        (n, kn), dtype=torch.int32, device=dev)
+
+    def buf(name, shape, dtype):
+        t = o.get(name)
+        if t is None or t.dtype != dtype or t.device != dev or t.numel() < int(np.prod(shape)):
+            t = torch.empty(shape, dtype=dtype, device=dev)
+            o[name] = t
+        return t.view(-1)[:int(np.prod(shape))].view(shape)
+
+    src = buf("src", (max(m, 1),), torch.int32)
+    dst = buf("dst", (max(m, 1),), torch.int32)
+    w = buf("w_sq", (max(m, 1),), torch.float32)
+    core = buf("core_sq", (n,), torch.float32) if core_and_neighbors else None
+    nbr = buf("neighbors", (n, kn), torch.int32) if core_and_neighbors else None
     # The library works on its own stream: inputs must be complete.
     torch.cuda.current_stream(dev).synchronize()
     st = _lib.FHStats()
     ctx = _lib.context(dev.index or 0)
     code = _lib.load().fh_mst_euclidean_device(
         ctx, ctypes.c_void_p(X.data_ptr()), n, d, ms, ctypes.c_void_p(wt.data_ptr()) if wt is not None else None,
-        ctypes.c_void_p(core.data_ptr()),
-        ctypes.c_void_p(nbr.data_ptr()), ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+        ctypes.c_void_p(core.data_ptr()) if core is not None else None,
+        ctypes.c_void_p(nbr.data_ptr()) if nbr is not None else None,
+        ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
         ctypes.c_void_p(w.data_ptr()), ctypes.byref(st),
     )
     _lib.check(ctx, code)

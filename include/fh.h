@@ -111,7 +111,7 @@ const char *fh_version(void);
  *   src,dst  [n-1]   edge endpoints (original point indices; dst may be -1
  *                    only when min_samples >= n, as in the reference)
  *   w_sq     [n-1]   float32 squared mutual-reachability weight
- * stats may be NULL.
+ * core_sq, nbr and stats may be NULL (not written).
  */
 int fh_mst_euclidean_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples,
                             const float *sample_weights, float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst,

@@ -37,6 +37,26 @@ def test_device_pipeline_matches_oracle(name, ms, mcs):
     np.testing.assert_array_equal(prob, oprob)
 
 
+def test_device_pipeline_reuses_work_buffers():
+    # one `work` dict over calls whose sizes shrink and grow (buffers kept,
+    # regrown when too small, outputs reused): each result equals a call
+    # that keeps nothing
+    import torch
+
+    from fast_hdbscan_amd.estimator import hdbscan_device
+
+    work = {}
+    for n, seed in [(150_000, 11), (20_000, 12), (160_000, 13)]:
+        X = torch.from_numpy(workloads.scale_2d(n, seed=seed)).cuda()
+        print(f"work reuse n={n}", flush=True)
+        lab, prob = hdbscan_device(X, 10, 50, work=work)
+        lab, prob = lab.copy(), prob.copy()
+        flab, fprob = hdbscan_device(X, 10, 50)
+        np.testing.assert_array_equal(lab, flab)
+        np.testing.assert_array_equal(prob, fprob)
+        assert lab.shape == (n,)
+
+
 def test_device_sort_matches_lexsort():
     import ctypes
 
