@@ -1586,8 +1586,18 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
             int64_t tot = 0;
             for (int64_t g = 0; g < order->n_groups; ++g) tot += order->group_sizes[g];
             if (tot != k) return FH_EINVAL;
-            for (int64_t j = 0; j < k; ++j)
-                if (static_cast<uint64_t>(order->row_order[j]) >= static_cast<uint64_t>(k)) return FH_EINVAL;
+            // (a branch-free scan in parallel pieces)
+            const int nt = k > (1 << 18) ? fh::HostPool::width() : 1;
+            std::vector<uint32_t> bad(nt, 0);
+            fh::HostPool::run(nt, [&](int t) {
+                uint32_t any = 0;
+                for (int64_t j = k * t / nt, e = k * (t + 1) / nt; j < e; ++j)
+                    any |= static_cast<uint32_t>(static_cast<uint32_t>(order->row_order[j]) >=
+                                                 static_cast<uint32_t>(k));
+                bad[t] = any;
+            });
+            for (const uint32_t b : bad)
+                if (b) return FH_EINVAL;
         }
         // reused between calls on this thread (no fresh pages per call)
         static thread_local Hierarchy h;
