@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # command lines and raw counts in profiles/pmc_2Mx2D_traffic.md.
 PMC_TRAFFIC_BYTES = {
     "boruvka_query_kernel<2, 2>": 68.0e6 + 11.4e6,
-    "knn_kernel<2>": 190.9e6 + 522.5e6,
+    "knn_kernel<2>": 159.4e6 + 379.0e6,
 }
 
 
@@ -127,14 +127,17 @@ def main():
 
     # Dominant kernel: kNN vs the per-round main-pass nearest-edge search
     # (HIP events on the library stream).  Algorithmic bytes per launch:
-    #   knn   : n*(4d) read + n*k*8 written + 8n core     (k = min_samples+1)
+    #   knn   : n*(4d) read + n*k*4 indices + n*(k-1)*8 neighbour positions
+    #           and distances + 8n core                 (k = min_samples+1;
+    #           the distance rows are only written with sample weights or
+    #           min_samples <= 1)
     #   query : n*(4d + 4 core + 4 label + 8 best)        (main pass, per round)
     k = ms + 1 if ms > 1 else 2
     knn_ms = float(np.mean([s["t_knn_ms"] for s in per_step]))
     rounds = per_step[-1]["rounds"]
     q_ms_total = float(np.mean([s["t_query_main_ms"] for s in per_step]))
     q_ms = q_ms_total / max(rounds, 1)
-    knn_bytes = n * (4 * d) + n * k * 8 + 8 * n
+    knn_bytes = n * (4 * d) + n * k * 4 + n * (k - 1) * 8 + 8 * n
     q_bytes = n * (4 * d + 16)
     if knn_ms >= q_ms_total:
         dom, dur_ms, alg = "knn_kernel<2>", knn_ms, knn_bytes
