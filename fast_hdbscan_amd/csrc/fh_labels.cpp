@@ -133,8 +133,9 @@ void linkage_tree_literal(int64_t n, const int32_t *src, const int32_t *dst, con
 //     endpoints at step i, i.e. on the earlier rows of its component, and
 //     components touch disjoint union-find entries, so the groups run on
 //     separate threads (each in row order) before rows [k, m) run in order.
-// Returns false for a -1 endpoint with weights (not supported).
-bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, const double *dist,
+// Returns FH_OK, FH_EINVAL for an endpoint outside [-1, n) or a point order
+// value outside [0, n), or FH_EUNSUPPORTED for a -1 endpoint with weights.
+int linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float *w_sq, const double *dist,
                   Hierarchy &h, const float *weights = nullptr, const fh_label_order *ord = nullptr) {
     const int64_t m = n - 1;
     const int32_t *order = ord ? ord->point_order : nullptr;
@@ -143,18 +144,34 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
     h.w_sq = w_sq;
     h.dist = dist;
     const int nt = m > (1 << 16) ? fh::HostPool::width() : 1;
-    // any -1 endpoint? (row ranges on threads)
-    std::vector<uint8_t> neg(nt, 0);
+    // any -1 endpoint, any endpoint outside [-1, n)? (row ranges on threads)
+    std::vector<uint8_t> neg(nt, 0), bad(nt, 0);
     fh::HostPool::run(nt, [&](int t) {
-        uint8_t any = 0;
-        for (int64_t i = m * t / nt, e = m * (t + 1) / nt; i < e; ++i) any |= (src[i] < 0) | (dst[i] < 0);
+        uint8_t any = 0, out = 0;
+        for (int64_t i = m * t / nt, e = m * (t + 1) / nt; i < e; ++i) {
+            any |= (src[i] < 0) | (dst[i] < 0);
+            out |= (static_cast<uint64_t>(int64_t{src[i]} + 1) > static_cast<uint64_t>(n)) |
+                   (static_cast<uint64_t>(int64_t{dst[i]} + 1) > static_cast<uint64_t>(n));
+        }
         neg[t] = any;
+        bad[t] = out;
     });
+    if (std::find(bad.begin(), bad.end(), 1) != bad.end()) return FH_EINVAL;
+    if (order) {
+        std::vector<uint8_t> obad(nt, 0);
+        fh::HostPool::run(nt, [&](int t) {
+            uint8_t out = 0;
+            for (int64_t i = n * t / nt, e = n * (t + 1) / nt; i < e; ++i)
+                out |= static_cast<uint32_t>(order[i]) >= static_cast<uint32_t>(n);
+            obad[t] = out;
+        });
+        if (std::find(obad.begin(), obad.end(), 1) != obad.end()) return FH_EINVAL;
+    }
     if (std::find(neg.begin(), neg.end(), 1) != neg.end()) {
-        if (weights) return false;
+        if (weights) return FH_EUNSUPPORTED;
         if (!order) {
             linkage_tree_literal(n, src, dst, w_sq, dist, h);
-            return true;
+            return FH_OK;
         }
         std::vector<int32_t> s2(m), d2(m);
         for (int64_t j = 0; j < m; ++j) {
@@ -162,7 +179,7 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
             d2[j] = dst[j] < 0 ? dst[j] : order[dst[j]];
         }
         linkage_tree_literal(n, s2.data(), d2.data(), w_sq, dist, h);
-        return true;
+        return FH_OK;
     }
     static thread_local std::vector<float> wsz_tl;
     if (weights) {
@@ -261,7 +278,7 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
             start[g + 1] = start[g] + ord->group_sizes[g];
             if (ord->group_sizes[g] > ord->group_sizes[big]) big = g;
         }
-        if (start[G] != k) return false;
+        if (start[G] != k) return FH_EINVAL;
         auto run_group = [&](int64_t g) {
             const int64_t e = start[g + 1];
             for (int64_t j = start[g]; j < e; ++j) {
@@ -394,7 +411,7 @@ bool linkage_tree(int64_t n, const int32_t *src, const int32_t *dst, const float
         step(i);
     }
     FH_TMARK("link-tail");
-    return true;
+    return FH_OK;
 }
 
 // ids are below 2n; the label stage takes n < 2^30, so they fit int32
@@ -1602,8 +1619,8 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
         // reused between calls on this thread (no fresh pages per call)
         static thread_local Hierarchy h;
         clk::time_point t = clk::now();
-        if (!linkage_tree(n, src, dst, dist ? nullptr : w_sq, dist, h, sample_weights, order))
-            return FH_EUNSUPPORTED;
+        const int lc = linkage_tree(n, src, dst, dist ? nullptr : w_sq, dist, h, sample_weights, order);
+        if (lc != FH_OK) return lc;
         if (phase_ms) phase_ms[0] = ms_since(t);
         if (linkage) {
             for (int64_t i = 0; i < n - 1; ++i) {
@@ -1674,7 +1691,8 @@ extern "C" int fh_cluster_layers(int64_t n, const int32_t *src, const int32_t *d
             return FH_OK;
         }
         Hierarchy h;
-        if (!linkage_tree(n, src, dst, nullptr, dist, h)) return FH_EUNSUPPORTED;
+        const int lc = linkage_tree(n, src, dst, nullptr, dist, h);
+        if (lc != FH_OK) return lc;
         int64_t mcs = p->base_min_cluster_size;
         Condensed c;
         if (p->base_n_clusters > 0) {

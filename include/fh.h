@@ -279,14 +279,15 @@ typedef struct {
 /*
  * How the label stage may walk the rows; outputs never depend on it.
  *   point_order [n] or NULL: src/dst hold positions p of a point order and
- *     point_order[p] is the point id (fh_edges_to_tree_order); the
- *     union-find then runs over positions.
+ *     point_order[p] is the point id in [0, n) (fh_edges_to_tree_order;
+ *     else FH_EINVAL); the union-find then runs over positions.
  *   prefix_rows k > 0: rows [0, k) form a forest whose connected
  *     components are given by row_order [k] (row indices grouped by
  *     component, increasing within a group) and group_sizes [n_groups]
  *     (fh_linkage_plan).  A row's linkage entry depends only on the earlier
  *     rows of its component, so the groups run on host threads (threads,
- *     0 = all cores) and rows [k, n-1) follow in order.
+ *     0 = all cores); rows [k, n-1) that do not join the largest group run
+ *     beside it, and those that do follow from running sums.
  */
 typedef struct {
     const int32_t *point_order;
@@ -308,7 +309,9 @@ typedef struct {
  *             extract_eom_clusters or cluster_tree_leaves,
  *             cluster_epsilon_search, get_cluster_label_vector,
  *             get_point_membership_strength_vector (cluster_trees.py).
- * HOST pointers.  src/dst [n-1]: sorted MST rows; their distances either
+ * HOST pointers.  src/dst [n-1]: sorted MST rows, endpoints in [-1, n) (-1
+ * only as the MST gives it with min_samples >= n; with an order, positions),
+ * else FH_EINVAL; their distances either
  * as w_sq [n-1] float32 squared (the float64 square root is taken inside,
  * as boruvka.py:646 does) or, when dist is not NULL, as dist [n-1] float64.
  * sample_weights [n] float32 or NULL (float32 weighted sizes in the

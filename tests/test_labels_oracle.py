@@ -317,6 +317,31 @@ def test_component_groups_are_checked():
                                      groups=(3, np.array([0, 1, 7], np.int32), np.array([3], np.int32)))
 
 
+def test_endpoints_and_point_order_are_checked():
+    # endpoints outside [-1, n) and point ids outside [0, n) are errors, not
+    # reads or writes out of bounds
+    from fast_hdbscan_amd import labels as PL
+
+    e, _, _ = O.compute_minimum_spanning_tree(workloads.blobs_small(500, seed=1), 5)
+    n = e.shape[0] + 1
+    o = np.lexsort((e[:, 1], e[:, 0], e[:, 2]))
+    src, dst = e[o, 0].astype(np.int32), e[o, 1].astype(np.int32)
+    dist = e[o, 2].copy()
+    params = PL.label_params(5, "eom", np.inf, False, 0.0)
+    for bad in (n, -2, np.iinfo(np.int32).max, np.iinfo(np.int32).min):
+        s2 = src.copy()
+        s2[7] = bad
+        with pytest.raises(ValueError):
+            PL.labels_from_sorted_arrays(n, s2, dst, None, params, dist=dist)
+    order = np.arange(n, dtype=np.int32)
+    order[3] = n
+    with pytest.raises(ValueError):
+        PL.labels_from_sorted_arrays(n, src, dst, None, params, dist=dist, point_order=order)
+    lab, *_ = PL.labels_from_sorted_arrays(n, src, dst, None, params, dist=dist, point_order=np.arange(n, dtype=np.int32))
+    ref, *_ = PL.labels_from_sorted_arrays(n, src, dst, None, params, dist=dist)
+    np.testing.assert_array_equal(lab, ref)
+
+
 def test_concurrent_label_stages_match():
     # several host threads in the label stage at once (the batch scheduler
     # and the PLSCAN sweep do this): the worker pool serves one, the others
