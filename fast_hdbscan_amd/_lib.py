@@ -5,6 +5,7 @@ visible, every entry point raises -- there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import threading
@@ -139,7 +140,6 @@ class FHError(RuntimeError):
 
 _lib = None
 _lock = threading.Lock()
-_ctx = {}
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -182,7 +182,7 @@ def load() -> ctypes.CDLL:
         L.fh_mst_euclidean.restype = ctypes.c_int
         L.fh_mst_euclidean.argtypes = mst_args
         L.fh_knn.restype = ctypes.c_int
-        L.fh_knn.argtypes = [_vp, _vp, _i64, _i32, _vp, _i64, _i32, _vp, _vp]
+        L.fh_knn.argtypes = [_vp, _vp, _i64, _i32, _vp, _i64, _i32, _i32, _vp, _vp]
         L.fh_tree_nnodes.restype = _i64
         L.fh_tree_nnodes.argtypes = [_i64]
         L.fh_build_kdtree.restype = ctypes.c_int
@@ -219,17 +219,46 @@ def load() -> ctypes.CDLL:
     return _lib
 
 
+class _Context:
+    """One library context (a HIP stream and its cached device buffers),
+    owned by one Python thread.  A context is not thread-safe and ctypes
+    releases the GIL during each call, so threads never share one; a
+    thread's contexts are released when the thread ends."""
+
+    def __init__(self, L, device: int):
+        self._L = L
+        self.handle = L.fh_create(int(device))
+
+    def __del__(self):
+        if self.handle and not _exiting:
+            self._L.fh_destroy(self.handle)
+            self.handle = None
+
+
+_tls = threading.local()
+_exiting = False
+
+
+@atexit.register
+def _at_exit():
+    # the HIP runtime tears itself down at exit; contexts die with it
+    global _exiting
+    _exiting = True
+
+
 def context(device: int = 0):
-    """Per-device context (one HIP stream + cached device buffers)."""
+    """The calling thread's context on `device` (created on first use)."""
     L = load()
-    with _lock:
-        c = _ctx.get(device)
-        if c is None:
-            c = L.fh_create(int(device))
-            if not c:
-                raise FHError(f"fh_create({device}) failed: no usable HIP device")
-            _ctx[device] = c
-    return c
+    per_thread = getattr(_tls, "contexts", None)
+    if per_thread is None:
+        per_thread = _tls.contexts = {}
+    c = per_thread.get(device)
+    if c is None:
+        c = _Context(L, device)
+        if not c.handle:
+            raise FHError(f"fh_create({device}) failed: no usable HIP device")
+        per_thread[device] = c
+    return c.handle
 
 
 def check(ctx, code: int) -> None:

@@ -972,10 +972,10 @@ int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t 
     return FH_OK;
 }
 
-int fh_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, const float *Q, int64_t nq, int32_t k, float *dist_sq,
-           int32_t *idx) {
+int fh_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, const float *Q, int64_t nq, int32_t k,
+           int32_t output_rdist, float *dist, int32_t *idx) {
     FH_TRY(check_args(ctx, X, n, d, 1));
-    if (!Q || !dist_sq || !idx || nq < 0 || k < 1) return fail(ctx, FH_EINVAL, "bad query arguments");
+    if (!Q || !dist || !idx || nq < 0 || k < 1) return fail(ctx, FH_EINVAL, "bad query arguments");
     if (nq == 0) return FH_OK;
     FH_HIP(hipSetDevice(ctx->device));
     FH_TRY(ensure(ctx, ctx->xdev, (size_t)n * d));
@@ -987,8 +987,10 @@ int fh_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, const float *Q, in
     FH_TRY(ensure(ctx, ctx->knn_i, (size_t)nq * k));
     hipStream_t st = ctx->stream;
     FH_HIP(hipMemcpyAsync(ctx->queries.p, Q, sizeof(float) * nq * d, hipMemcpyHostToDevice, st));
-    FH_TRY(run_knn(ctx, d, levels, ctx->queries.p, nullptr, nq, k, ctx->knn_d.p, ctx->knn_i.p, nullptr, nullptr, 0));
-    FH_HIP(hipMemcpyAsync(dist_sq, ctx->knn_d.p, sizeof(float) * nq * k, hipMemcpyDeviceToHost, st));
+    // numba_kdtree.py:789-792: square roots (when asked) before deheap_sort
+    FH_TRY(run_knn(ctx, d, levels, ctx->queries.p, nullptr, nq, k, ctx->knn_d.p, ctx->knn_i.p, nullptr, nullptr, 0,
+                   nullptr, nullptr, output_rdist ? 0 : 1));
+    FH_HIP(hipMemcpyAsync(dist, ctx->knn_d.p, sizeof(float) * nq * k, hipMemcpyDeviceToHost, st));
     FH_HIP(hipMemcpyAsync(idx, ctx->knn_i.p, sizeof(int) * nq * k, hipMemcpyDeviceToHost, st));
     FH_HIP(hipStreamSynchronize(st));
     return FH_OK;

@@ -1597,20 +1597,31 @@ extern "C" int fh_labels_from_sorted_mst(int64_t n, const int32_t *src, const in
             return FH_OK;
         }
         if (order && order->prefix_rows > 0) {
-            // the groups must cover rows [0, k) with in-range indices
+            // the groups must be non-empty and cover rows [0, k), and
+            // row_order must be a permutation of [0, k): in range, each row
+            // once (a repeated row would leave another row's entry unset)
             const int64_t k = order->prefix_rows;
             if (k > n - 1 || order->n_groups < 1 || !order->row_order || !order->group_sizes) return FH_EINVAL;
             int64_t tot = 0;
-            for (int64_t g = 0; g < order->n_groups; ++g) tot += order->group_sizes[g];
+            for (int64_t g = 0; g < order->n_groups; ++g) {
+                if (order->group_sizes[g] < 1) return FH_EINVAL;
+                tot += order->group_sizes[g];
+            }
             if (tot != k) return FH_EINVAL;
-            // (a branch-free scan in parallel pieces)
+            // (parallel pieces; the second sight of a row reads its byte as 1)
+            std::vector<uint8_t> seen(static_cast<size_t>(k), 0);
             const int nt = k > (1 << 18) ? fh::HostPool::width() : 1;
             std::vector<uint32_t> bad(nt, 0);
             fh::HostPool::run(nt, [&](int t) {
                 uint32_t any = 0;
-                for (int64_t j = k * t / nt, e = k * (t + 1) / nt; j < e; ++j)
-                    any |= static_cast<uint32_t>(static_cast<uint32_t>(order->row_order[j]) >=
-                                                 static_cast<uint32_t>(k));
+                for (int64_t j = k * t / nt, e = k * (t + 1) / nt; j < e; ++j) {
+                    const uint32_t r = static_cast<uint32_t>(order->row_order[j]);
+                    if (r >= static_cast<uint32_t>(k)) {
+                        any = 1;
+                        continue;
+                    }
+                    any |= __atomic_exchange_n(&seen[r], static_cast<uint8_t>(1), __ATOMIC_RELAXED);
+                }
                 bad[t] = any;
             });
             for (const uint32_t b : bad)

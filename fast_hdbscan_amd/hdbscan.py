@@ -257,7 +257,9 @@ def kruskal_mst_knn(X, min_samples=10, knn_k=10, sample_weights=None, device: in
 
 def parallel_tree_query(data, queries, k=10, output_rdist=False, device: int = 0):
     """numba_kdtree.parallel_tree_query(build_kdtree(data), queries, k,
-    output_rdist) (numba_kdtree.py:774-792) on the device."""
+    output_rdist) (numba_kdtree.py:765-792) on the device: squared distances
+    with ``output_rdist``, else float32 square roots taken before the sort
+    (numba_kdtree.py:792)."""
     X = _as_points(data)
     Q = _as_points(queries)
     if Q.shape[1] != X.shape[1]:
@@ -266,14 +268,10 @@ def parallel_tree_query(data, queries, k=10, output_rdist=False, device: int = 0
     dist = np.empty((nq, int(k)), np.float32)
     idx = np.empty((nq, int(k)), np.int32)
     ctx = _lib.context(device)
-    code = _lib.load().fh_knn(ctx, _lib.ptr(X), X.shape[0], X.shape[1], _lib.ptr(Q), nq, int(k), _lib.ptr(dist),
-                              _lib.ptr(idx))
+    code = _lib.load().fh_knn(ctx, _lib.ptr(X), X.shape[0], X.shape[1], _lib.ptr(Q), nq, int(k),
+                              1 if output_rdist else 0, _lib.ptr(dist), _lib.ptr(idx))
     _lib.check(ctx, code)
-    if output_rdist:
-        return dist, idx
-    # numba_kdtree.py:792 takes the square root before sorting; for distinct
-    # squared distances that order is the one returned here.
-    return np.sqrt(dist), idx
+    return dist, idx
 
 
 def build_kdtree(data, on_device: bool = False, device: int = 0):

@@ -129,15 +129,18 @@ int64_t fh_neighbor_columns(int64_t n, int32_t min_samples, const float *sample_
 
 /*
  * Exact k nearest neighbours of Q's rows among X's rows.
- *   Replaces: numba_kdtree.py:774-792 parallel_tree_query(build_kdtree(X),
- *             Q, k, output_rdist=True): per row ascending squared float32
- *             distances and int32 indices, in the reference's order for
- *             ties (its bounded max-heap and deheap_sort); rows with fewer
- *             than k points are padded with (+inf, -1).
- * HOST pointers: X [n*d], Q [nq*d], dist_sq [nq*k], idx [nq*k].
+ *   Replaces: numba_kdtree.py:765-792 parallel_tree_query(build_kdtree(X),
+ *             Q, k, output_rdist): per row ascending float32 distances and
+ *             int32 indices, in the reference's order for ties (its bounded
+ *             max-heap and deheap_sort); rows with fewer than k points are
+ *             padded with (+inf, -1).  output_rdist != 0: squared distances
+ *             (rdist); 0: their float32 square roots, taken before the sort
+ *             as numba_kdtree.py:792 does (equal roots of distinct squares
+ *             keep the heap's order).
+ * HOST pointers: X [n*d], Q [nq*d], dist [nq*k], idx [nq*k].
  */
 int fh_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, const float *Q, int64_t nq, int32_t k,
-           float *dist_sq, int32_t *idx);
+           int32_t output_rdist, float *dist, int32_t *idx);
 
 /*
  * The KD-tree alone, built exactly as numba_kdtree.build_kdtree(X, 40)

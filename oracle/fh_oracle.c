@@ -76,6 +76,21 @@ static inline float box_lb(const float *lo, const float *hi, const float *pt, in
 }
 
 float orc_rdist(const float *x, const float *y, int d) { return rdist(x, y, d); }
+
+/* rdist from each of the m rows of Q to each of the n rows of X: out[m*n]
+ * (brute-force checks of single rows at full input sizes). */
+void orc_rdist_rows(const float *Q, int64_t m, const float *X, int64_t n, int d, float *out, int nthreads) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t t = 0; t < m * n; ++t) {
+        const int64_t q = t / n, j = t % n;
+        out[t] = rdist(Q + q * d, X + j * d, d);
+    }
+}
+
+/* rdist of the pairs (a[i], b[i]) of rows of X. */
+void orc_rdist_pairs(const float *X, int d, const int32_t *a, const int32_t *b, int64_t m, float *out) {
+    for (int64_t i = 0; i < m; ++i) out[i] = rdist(X + (int64_t)a[i] * d, X + (int64_t)b[i] * d, d);
+}
 float orc_box_lb(const float *lo, const float *hi, const float *pt, int d) {
     return box_lb(lo, hi, pt, d);
 }

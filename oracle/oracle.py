@@ -105,6 +105,10 @@ def lib():
         L.orc_knn_sqrt.argtypes = tree_args + [_f32p, ctypes.c_int64, ctypes.c_int, _f32p, _i32p, ctypes.c_int]
         L.orc_rdist.restype = ctypes.c_float
         L.orc_rdist.argtypes = [_f32p, _f32p, ctypes.c_int]
+        L.orc_rdist_rows.restype = None
+        L.orc_rdist_rows.argtypes = [_f32p, ctypes.c_int64, _f32p, ctypes.c_int64, ctypes.c_int, _f32p, ctypes.c_int]
+        L.orc_rdist_pairs.restype = None
+        L.orc_rdist_pairs.argtypes = [_f32p, ctypes.c_int, _i32p, _i32p, ctypes.c_int64, _f32p]
         L.orc_box_lb.restype = ctypes.c_float
         L.orc_box_lb.argtypes = [_f32p, _f32p, _f32p, ctypes.c_int]
         _lib = L
@@ -113,6 +117,26 @@ def lib():
 
 def set_rdist_mode(mode: int) -> None:
     lib().orc_set_rdist_mode(int(mode))
+
+
+def rdist_rows(Q, X, nthreads=None) -> np.ndarray:
+    """float32 rdist (numba_kdtree.py:539-546) from every row of Q to every
+    row of X: [m, n]."""
+    Q = np.ascontiguousarray(Q, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    out = np.empty((Q.shape[0], X.shape[0]), np.float32)
+    lib().orc_rdist_rows(Q, Q.shape[0], X, X.shape[0], X.shape[1], out, _threads(nthreads))
+    return out
+
+
+def rdist_pairs(X, a, b) -> np.ndarray:
+    """float32 rdist of the row pairs (X[a[i]], X[b[i]])."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    out = np.empty(a.shape[0], np.float32)
+    lib().orc_rdist_pairs(X, X.shape[1], a, b, a.shape[0], out)
+    return out
 
 
 def _threads(nthreads):

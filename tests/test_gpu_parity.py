@@ -183,12 +183,73 @@ def test_knn_external_queries():
     np.testing.assert_array_equal(i, ri)
 
 
+def test_knn_sqrt_mode_ties():
+    """output_rdist=False: numba_kdtree.py:792 takes the float32 square root
+    of the heap rows before deheap_sort.  Queries 20,000 units from an
+    integer grid give distinct squared distances whose roots collide, so the
+    sort then meets equal keys that the squared rows did not have."""
+    from fast_hdbscan_amd.hdbscan import parallel_tree_query
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(0)
+    g = np.stack(np.meshgrid(np.arange(60), np.arange(60)), -1).reshape(-1, 2).astype(np.float32)
+    X = g[rng.permutation(g.shape[0])]
+    Q = np.stack([np.full(2000, -20000.0), rng.integers(0, 60, 2000)], 1).astype(np.float32)
+    Q[:, 0] -= rng.integers(0, 50, 2000)
+    tree = O.build_kdtree(X)
+    for k in (12, 32):
+        print(f"case sqrt-mode k={k}", flush=True)
+        d, i = parallel_tree_query(X, Q, k=k, output_rdist=False)
+        rd, ri = O.tree_query_sqrt(tree, Q, k)
+        sq_d, sq_i = O.tree_query(tree, Q, k)
+        # the case really exercises the order: squaring first would differ
+        assert (sq_i != ri).any()
+        np.testing.assert_array_equal(d, rd)
+        np.testing.assert_array_equal(i, ri)
+
+
 def test_repeatable():
     X = workloads.scale_2d(100_000, seed=9)
     a = _device(X, 10)[0]
     b = _device(X, 10)[0]
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+def test_threads_get_their_own_contexts():
+    """Two Python threads calling the library at once (ctypes releases the
+    GIL) each use their own context and stream; results equal the
+    one-thread calls."""
+    import threading
+
+    from fast_hdbscan_amd import _lib
+
+    xs = [workloads.scale_2d(150_000, seed=21), workloads.blobs_small(120_000, seed=22)]
+    want = [_device(x, 10)[0] for x in xs]
+    got = [None, None]
+    handles = [None, None]
+    errors = []
+    both = threading.Barrier(2, timeout=60)
+
+    def run(j):
+        try:
+            handles[j] = _lib.context(0)
+            both.wait()  # both contexts exist at once
+            for _ in range(3):
+                got[j] = _device(xs[j], 10)[0]
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    ts = [threading.Thread(target=run, args=(j,)) for j in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    assert handles[0] != handles[1] and _lib.context(0) not in handles
+    for j in range(2):
+        for u, v in zip(want[j], got[j]):
+            np.testing.assert_array_equal(u, v)
 
 
 @pytest.mark.parametrize("kind", ["unit", "uniform", "labels", "sparse_zeros"])

@@ -315,6 +315,14 @@ def test_component_groups_are_checked():
     with pytest.raises(ValueError):
         PL.labels_from_sorted_arrays(n, src, dst, None, params, dist=e[o, 2].copy(),
                                      groups=(3, np.array([0, 1, 7], np.int32), np.array([3], np.int32)))
+    # a repeated row (row_order not a permutation of [0, k))
+    with pytest.raises(ValueError):
+        PL.labels_from_sorted_arrays(n, src, dst, None, params, dist=e[o, 2].copy(),
+                                     groups=(4, np.array([0, 1, 1, 3], np.int32), np.array([2, 2], np.int32)))
+    # a negative group size that the others make up for
+    with pytest.raises(ValueError):
+        PL.labels_from_sorted_arrays(n, src, dst, None, params, dist=e[o, 2].copy(),
+                                     groups=(4, np.arange(4, dtype=np.int32), np.array([5, -1], np.int32)))
 
 
 def test_endpoints_and_point_order_are_checked():

@@ -92,6 +92,28 @@ def test_numba_argsort_is_a_sort():
             np.testing.assert_array_equal(o, np.argsort(y, kind="stable"))
 
 
+def test_numba_argsort_ties_by_hand():
+    # numba's quicksort argsort (numba/misc/quicksort.py: run_quicksort,
+    # partition, insertion_sort; SMALL_QUICKSORT = 15) on 16 keys with ties.
+    A = [1, 0, 1, 2, 0, 1, 2, 0, 1, 1, 2, 0, 1, 2, 0, 1]
+    # run_quicksort: (0, 15) has high - low = 15 >= 15 -> partition(0, 15):
+    #   mid = 7; A[R7]=0 < A[R0]=1: swap R0,R7; A[R15]=1 < A[R7]=1 no;
+    #   A[R7]=1 < A[R0]=0 no; pivot = 1 (key 0, now at 7), parked at 15.
+    #   R = [7,1,2,3,4,5,6,15,8,...,14,0]; i/j scans (strict '<' both
+    #   ways, so keys equal to the pivot stop both scans) swap positions
+    #   (2,14) (3,12) (5,11) (6,9) (7,8), then i=8 > j=7: the pivot goes
+    #   to 8.  R = [7,1,14,12,4,11,9,8,0,6,10,5,3,13,2,15].
+    #   high - i = 7 is not > i - low = 8: push (0,7), low = 9;
+    #   15 - 9 < 15 -> insertion_sort(9,15) over keys 6,10,5,3,13,2,15
+    #   (values 2,2,1,2,2,1,1) -> 5,2,15,6,10,3,13 (stable within each value).
+    # pop (0,7): insertion_sort over 7,1,14,12,4,11,9,8 (values 0,0,0,1,0,0,
+    #   1,1) -> 7,1,14,4,11,12,9,8.
+    # (np.argsort(kind="stable") would give 1,4,7,11,14,0,2,5,...)
+    expect = [7, 1, 14, 4, 11, 12, 9, 8, 0, 5, 2, 15, 6, 10, 3, 13]
+    assert OL.numba_argsort(np.asarray(A, np.int64)).tolist() == expect
+    assert OL.numba_argsort(np.asarray(A, np.float32)).tolist() == expect
+
+
 def test_total_persistence_definition():
     rng = np.random.default_rng(7)
     births = np.concatenate([[5.0], rng.integers(2, 30, 40)]).astype(np.float32)

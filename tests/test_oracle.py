@@ -82,6 +82,55 @@ def test_tree_median_invariant():
         assert max(left) < min(right)
 
 
+def test_introselect_partition_then_insertion_by_hand():
+    # 20 points in 1-D, value(i) = 7*i mod 20, leaf_size 9: n_levels =
+    # int(log2(19/9) + 1) = 2 (numba_kdtree.py:489), so only the root is
+    # split, by _introselect(left=0, right=20, nth=10) (numba_kdtree.py:
+    # 393-407, 449): size 20 > 16, max_depth = 2*int(log2 20) = 8.
+    # Trace of _introselect_impl (366-390), idx_array starting as 0..19:
+    #  _median_of_three_pivot (280-299): mid = 19//2 = 9; idx 0 (value 0),
+    #    idx 9 (value 3), idx 19 (value 13) are already ordered -> pivot
+    #    at position 9 (value 3).
+    #  _partition_indices (313-350): swap positions 9 and 19, pivot idx 9;
+    #    i=0, j=18.  i stops at 1 (value 7), j walks down to 6 (value 2):
+    #    swap -> positions 1,6 hold idx 6,1; i=2, j=5.  i stops at 2
+    #    (value 14), j at 3 (value 1): swap -> positions 2,3 hold idx 3,2;
+    #    i=3, j=2: break; the pivot goes to position 3 (idx 2 to the end).
+    #    Returns 3: idx_array = [0,6,3,9,4,5,1,7,8,19,10,...,18,2].
+    #  nth=10 > 3 -> left = 4; right-left = 16 is not > 16, so positions
+    #    4..19 are insertion-sorted by value: idx 12,15,18,1,4,7,10,13,16,
+    #    19,2,5,8,11,14,17 (values 4..19).
+    # The leaves take positions 0..9 and 10..19 without reordering, so the
+    # left leaf keeps the partition's order 0,6,3,9 (values 0,2,1,3).
+    X = ((7 * np.arange(20)) % 20).astype(np.float32).reshape(-1, 1)
+    T = O.build_kdtree(X, leaf_size=9)
+    assert T.n_nodes == 3
+    assert list(T.idx_array) == [0, 6, 3, 9, 12, 15, 18, 1, 4, 7, 10, 13, 16, 19, 2, 5, 8, 11, 14, 17]
+    assert list(T.idx_start) == [0, 0, 10] and list(T.idx_end) == [20, 10, 20]
+
+
+def test_introselect_pivot_at_nth_exits_by_hand():
+    # 21 points in 1-D, leaf_size 10: n_levels = int(log2(20/10) + 1) = 2,
+    # root split by _introselect(0, 21, nth=10).  Values by index:
+    V = [20, 11, 1, 12, 2, 13, 3, 4, 14, 5, 0, 6, 15, 7, 16, 17, 8, 18, 9, 19, 10]
+    # _median_of_three_pivot: mid = 20//2 = 10; candidates idx 0 (20),
+    #   idx 10 (0), idx 20 (10).  20 > 0: swap positions 0,10; then
+    #   20 > 10: swap positions 10,20; then 0 < 10: no swap.  Pivot idx 20
+    #   (value 10, rank 10) at position 10.
+    # _partition_indices: positions 10 and 20 swap (idx 0, value 20, goes
+    #   to position 10); i/j scans then swap the pairs of positions
+    #   (1,18) (3,16) (5,13) (8,11): idx 18,16,13,11 come left and idx
+    #   1,3,5,8 go right.  Then i stops at 10 (value 20), j falls to 9,
+    #   break, and the pivot returns to position 10 (idx 0 to position 20).
+    # pivot_pos == nth: _introselect_impl returns at once (385-387), with
+    # no insertion sort, so both halves keep the partition's order.
+    X = np.asarray(V, np.float32).reshape(-1, 1)
+    T = O.build_kdtree(X, leaf_size=10)
+    assert T.n_nodes == 3
+    assert list(T.idx_array) == [10, 18, 2, 16, 4, 13, 6, 7, 11, 9, 20, 8, 12, 5, 14, 15, 3, 17, 1, 19, 0]
+    assert list(T.idx_start) == [0, 0, 10] and list(T.idx_end) == [21, 10, 21]
+
+
 def test_tree_permutation():
     X = _blobs(10000)
     T = O.build_kdtree(X)
@@ -156,6 +205,48 @@ def test_duplicate_points_heap_layout_by_hand():
     assert list(r.dst) == [4] * 49
     assert np.all(r.w_sq == 0)
     assert r.rounds == 0
+
+
+def test_heap_partial_ties_by_hand():
+    # One leaf (8 points < 41, n_levels = 1) scanned in index order; the
+    # query at 0 gives rdist 4,1,4,9,1,0,1,4 for points 0..7.  Tracing
+    # simple_heap_push (numba_kdtree.py:593-635) with k = 5 (priorities /
+    # indices in heap order):
+    #   push 0 (4): sinks 0 -> 1 -> 3          [inf,inf,inf,4,inf] [-,-,-,0,-]
+    #   push 1 (1): 0 -> 1 -> 4 (p[3]=4 < inf)  [inf,inf,inf,4,1]   [-,-,-,0,1]
+    #   push 2 (4): 0 -> 1, stops (4 < 4 false) [inf,4,inf,4,1]     [-,2,-,0,1]
+    #   push 3 (9): 0 -> 2 (p[1]=4 < inf)       [inf,4,9,4,1]       [-,2,3,0,1]
+    #   push 4 (1): 0 -> 2 (4 < 9)              [9,4,1,4,1]         [3,2,4,0,1]
+    #   push 5 (0): 0 -> 1 -> 3                 [4,4,1,0,1]         [2,0,4,5,1]
+    #   push 6 (1): 0 -> 1, stops (1 < 1 false) [4,1,1,0,1]         [0,6,4,5,1]
+    #     -- of the two 4s, point 2 leaves and point 0 stays
+    #   push 7 (4): rejected, 4 >= p[0] = 4.
+    # deheap_sort (667-677) with siftdown's strict '<' (647-664):
+    #   j=4: swap -> [1,1,1,0,4] [1,6,4,5,0], no sift (ties)
+    #   j=3: swap -> [0,1,1,1,4] [5,6,4,1,0], sift 0<->1 -> [6,5,4,1,0]
+    #   j=2: swap -> [1,0,1,...] [4,5,6,1,0], no sift
+    #   j=1: swap -> [0,1,1,1,4] [5,4,6,1,0]
+    # (a stable sort of the kept points would give 5,1,4,6,0).
+    X = np.asarray([2, 1, -2, 3, -1, 0, 1, -2], np.float32).reshape(-1, 1)
+    Q = np.zeros((1, 1), np.float32)
+    d, i = O.tree_query(O.build_kdtree(X), Q, 5)
+    assert list(i[0]) == [5, 4, 6, 1, 0]
+    assert list(d[0]) == [0, 1, 1, 1, 4]
+
+
+def test_sqrt_mode_rows():
+    # output_rdist=False rows are the square roots of the rdist rows (same
+    # multiset per row); the index sets agree wherever the k-th squared
+    # distance is below the (k+1)-th (numba_kdtree.py:789-792).
+    X = _blobs(3000, seed=8)
+    T = O.build_kdtree(X)
+    d1, i1 = O.tree_query(T, X, 9)
+    d2, i2 = O.tree_query_sqrt(T, X, 9)
+    np.testing.assert_array_equal(np.sqrt(d1), d2)
+    d10, _ = O.tree_query(T, X, 10)
+    clean = d10[:, 8] < d10[:, 9]
+    assert clean.mean() > 0.9
+    np.testing.assert_array_equal(np.sort(i1[clean], 1), np.sort(i2[clean], 1))
 
 
 # -------------------------------------------------------------- Boruvka ----
