@@ -30,6 +30,7 @@ EXPORTED_SYMBOLS = (
     "fh_version",
     "fh_mst_euclidean_device",
     "fh_mst_euclidean",
+    "fh_mst_euclidean_device_noalloc",
     "fh_knn",
     "fh_tree_nnodes",
     "fh_build_kdtree",
@@ -181,6 +182,9 @@ def load() -> ctypes.CDLL:
         L.fh_mst_euclidean_device.argtypes = mst_args
         L.fh_mst_euclidean.restype = ctypes.c_int
         L.fh_mst_euclidean.argtypes = mst_args
+        L.fh_mst_euclidean_device_noalloc.restype = ctypes.c_int
+        L.fh_mst_euclidean_device_noalloc.argtypes = [_i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp,
+                                                      ctypes.POINTER(FHStats), ctypes.c_char_p, _i32]
         L.fh_knn.restype = ctypes.c_int
         L.fh_knn.argtypes = [_vp, _vp, _i64, _i32, _vp, _i64, _i32, _i32, _vp, _vp]
         L.fh_tree_nnodes.restype = _i64

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -105,12 +106,41 @@ struct fh_ctx {
     DevBuf<unsigned char> stemp;
     int64_t geom_n = -1;            // n the uploaded node ranges belong to
     int64_t tree_n = -1;            // n of the last tree built (perm, posof)
-    std::vector<int2> geom_range;   // host copy of the node ranges
-    std::vector<int> geom_maxsize;  // largest node per level
+    int geom_maxsize[32] = {};      // largest node per level
     int geom_levels = 0;
-    int *h_small = nullptr;  // pinned readback slots
+    int *h_small = nullptr;  // readback slots: pinned, or a static host array in allocation-free mode
     hipEvent_t ev[10] = {};
+    bool timing = true;      // events recorded (allocation-free mode creates none)
+    // Allocation-free mode: buffers are carved from this device range in
+    // call order, from offset 0 on every call, instead of hipMalloc'd.
+    unsigned char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 };
+
+namespace {
+
+// Every device buffer of a context, for release and for the arena reset.
+template <class F>
+void for_each_buf(fh_ctx *c, F &&f) {
+    auto all = [&](auto &...b) { (f(b), ...); };
+    all(
+        c->xp, c->box, c->core_p, c->core_o, c->knn_d, c->w_init, c->bound, c->ew,
+        c->heap_p, c->queries, c->out_w, c->out_core, c->perm, c->posof, c->comp_p, c->node_comp,
+        c->knn_i, c->succ, c->dst_raw, c->S0, c->M0, c->S1, c->M1, c->keep,
+        c->offs, c->parent, c->roots, c->roots2, c->rootslot, c->partner, c->ei, c->ej,
+        c->tiles, c->small, c->heap_i, c->flag, c->out_src, c->out_dst, c->out_nbr, c->range,
+        c->best, c->xdev, c->wdev, c->tkeys, c->obox, c->taxis, c->tcnt, c->gtpos,
+        c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->nb_d, c->mincore,
+        c->cert, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
+        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
+        c->ml_v, c->ml_mu, c->ml_sd, c->ml_scratch, c->ml_tgt, c->ml_pred, c->ml_bound, c->ml_stats,
+        c->ml_rows, c->ml_miss, c->kr_keys, c->kr_keys_s, c->kr_vals, c->kr_vals_s, c->kr_src, c->kr_dst,
+        c->kr_w, c->kr_core, c->sk1, c->sk1s, c->sk2, c->sk2s, c->sv1, c->sv1s,
+        c->pc_keys, c->pc_keys_s, c->pc_parent, c->pc_vals, c->pc_vals_s, c->pc_counts, c->pc_runs, c->pc_temp,
+        c->stemp);
+}
+
+}  // namespace
 
 namespace {
 
@@ -130,6 +160,19 @@ template <class T>
 int ensure(fh_ctx *ctx, DevBuf<T> &b, size_t count) {
     if (count == 0) count = 1;
     if (b.cap >= count) return FH_OK;
+    if (ctx->arena) {
+        // allocation-free mode: the next 256-byte aligned piece of the arena
+        const size_t bytes = (count * sizeof(T) + 255) / 256 * 256;
+        if (ctx->arena_used + bytes > ctx->arena_cap)
+            return fail(ctx, FH_EINVAL,
+                        "input needs more than the " + std::to_string(ctx->arena_cap >> 20) +
+                            " MiB scratch of the allocation-free entry point (" +
+                            std::to_string((ctx->arena_used + bytes) >> 20) + " MiB reached)");
+        b.p = reinterpret_cast<T *>(ctx->arena + ctx->arena_used);
+        b.cap = count;
+        ctx->arena_used += bytes;
+        return FH_OK;
+    }
     if (b.p) {
         hipError_t e = hipFree(b.p);
         (void)e;
@@ -162,6 +205,13 @@ void release(DevBuf<T> &b) {
         if (r_ != FH_OK) return r_; \
     } while (0)
 
+// Phase marks of run_mst on the context's stream (none without events).
+hipError_t mark(fh_ctx *c, int i) { return c->timing ? hipEventRecord(c->ev[i], c->stream) : hipSuccess; }
+
+float event_ms(hipEvent_t a, hipEvent_t b);
+
+float span(fh_ctx *c, int i, int j) { return c->timing ? event_ms(c->ev[i], c->ev[j]) : 0.0f; }
+
 float event_ms(hipEvent_t a, hipEvent_t b) {
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
@@ -169,35 +219,19 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
 }
 
 // Node ranges of the implicit tree for n points (numba_kdtree.py:426-473:
-// the left child takes n//2 points), uploaded once per n.
+// the left child takes n//2 points), computed on the device once per n.  The
+// nodes of level l hold floor(n / 2^l) or ceil(n / 2^l) points.
 int tree_geometry(fh_ctx *ctx, int64_t n) {
     if (ctx->geom_n == n) return FH_OK;
     const int levels = fh::tree_levels(n, kLeafSize);
+    if (levels > 31) return fail(ctx, FH_EUNSUPPORTED, "tree deeper than 31 levels");
     const int64_t N = (int64_t(1) << levels) - 1;
-    ctx->geom_range.assign(N, make_int2(0, 0));
-    ctx->geom_maxsize.assign(levels, 0);
-    ctx->geom_range[0] = make_int2(0, (int)n);
-    for (int64_t i = 0; i < N; ++i) {
-        const int2 r = ctx->geom_range[i];
-        int lvl = 0;
-        while (((int64_t(1) << (lvl + 1)) - 1) <= i) ++lvl;
-        ctx->geom_maxsize[lvl] = std::max(ctx->geom_maxsize[lvl], r.y - r.x);
-        if (2 * i + 2 < N) {
-            const int mid = r.x + (r.y - r.x) / 2;
-            ctx->geom_range[2 * i + 1] = make_int2(r.x, mid);
-            ctx->geom_range[2 * i + 2] = make_int2(mid, r.y);
-        }
-    }
     // every node above the last level must hold >= 2 points so that all
     // leaves sit on the last level (numba_kdtree.py:442-445)
-    for (int64_t i = 0; i < (N - 1) / 2; ++i)
-        if (ctx->geom_range[i].y - ctx->geom_range[i].x < 2)
-            return fail(ctx, FH_EUNSUPPORTED, "tree has a leaf above the last level");
-    if (levels > 31) return fail(ctx, FH_EUNSUPPORTED, "tree deeper than 31 levels");
+    if (levels >= 2 && (n >> (levels - 2)) < 2) return fail(ctx, FH_EUNSUPPORTED, "tree has a leaf above the last level");
+    for (int l = 0; l < levels && l < 32; ++l) ctx->geom_maxsize[l] = (int)((n + (int64_t(1) << l) - 1) >> l);
     FH_TRY(ensure(ctx, ctx->range, N));
-    FH_HIP(hipMemcpyAsync(ctx->range.p, ctx->geom_range.data(), sizeof(int2) * N, hipMemcpyHostToDevice,
-                          ctx->stream));
-    FH_HIP(hipStreamSynchronize(ctx->stream));
+    FH_HIP(fh::launch_node_ranges(N, (int)n, ctx->range.p, ctx->stream));
     ctx->geom_levels = levels;
     ctx->geom_n = n;
     return FH_OK;
@@ -574,12 +608,12 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     Timer total;
     hipStream_t st = ctx->stream;
     fh_stats S{};
-    FH_HIP(hipEventRecord(ctx->ev[0], st));
+    FH_HIP(mark(ctx, 0));
 
     int levels = 0;
     int64_t n_nodes = 0;
     FH_TRY(build_tree_device(ctx, Xd, n, d, &levels, &n_nodes));
-    FH_HIP(hipEventRecord(ctx->ev[1], st));
+    FH_HIP(mark(ctx, 1));
     S.n_levels = levels;
     S.n_nodes = n_nodes;
 
@@ -649,7 +683,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     if (weighted)
         FH_HIP(fh::launch_weighted_core(ni, kq, ctx->knn_i.p, ctx->knn_d.p, weights_d, ms, ctx->posof.p, ctx->core_o.p,
                                         ctx->core_p.p, st));
-    FH_HIP(hipEventRecord(ctx->ev[2], st));
+    FH_HIP(mark(ctx, 2));
     // certification nodes: kCertDepth levels above the leaves
     const int cert_level = std::max(0, levels - 1 - kCertDepth);
     if (pruned && !brute) {
@@ -674,7 +708,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     int base = ctx->h_small[0];
     int C = ctx->h_small[1];
     S.initial_components = C;
-    FH_HIP(hipEventRecord(ctx->ev[3], st));
+    FH_HIP(mark(ctx, 3));
 
     // -- Boruvka rounds
     fh::QueryArgs qa{};
@@ -735,13 +769,13 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     while (C > 1) {
         FH_HIP(fh::launch_round_reset(C, ctx->roots.p, ctx->bound.p, ctx->best.p, st));
         if (!brute) FH_HIP(fh::launch_purity(levels, ctx->range.p, ctx->comp_p.p, ctx->node_comp.p, st));
-        FH_HIP(hipEventRecord(ctx->ev[5], st));
+        FH_HIP(mark(ctx, 5));
         if (brute) {
             // approximate row minima on MFMA, exact upper bound per
             // component, exact re-scan of the rows that can reach it
-            FH_HIP(hipEventRecord(ctx->ev[8], st));
+            FH_HIP(mark(ctx, 8));
             FH_HIP(fh::launch_scan_boruvka(bsa, st));
-            FH_HIP(hipEventRecord(ctx->ev[9], st));
+            FH_HIP(mark(ctx, 9));
             FH_HIP(fh::launch_boruvka_upper(be, st));
             FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));
             FH_HIP(fh::launch_boruvka_rows(be, st));
@@ -756,16 +790,16 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
             FH_HIP(fh::launch_query(qa, st));
             FH_HIP(fh::launch_cert(ca, st));
             qa.pass = 2;
-            FH_HIP(hipEventRecord(ctx->ev[8], st));
+            FH_HIP(mark(ctx, 8));
             FH_HIP(fh::launch_query(qa, st));
-            FH_HIP(hipEventRecord(ctx->ev[9], st));
+            FH_HIP(mark(ctx, 9));
         } else {
             qa.pass = 0;
-            FH_HIP(hipEventRecord(ctx->ev[8], st));
+            FH_HIP(mark(ctx, 8));
             FH_HIP(fh::launch_query(qa, st));
-            FH_HIP(hipEventRecord(ctx->ev[9], st));
+            FH_HIP(mark(ctx, 9));
         }
-        FH_HIP(hipEventRecord(ctx->ev[6], st));
+        FH_HIP(mark(ctx, 6));
         FH_HIP(fh::launch_select(C, ni, ctx->roots.p, ctx->best.p, ctx->partner.p, ctx->posof.p, ctx->perm.p,
                                  ctx->comp_p.p, ctx->rootslot.p, ctx->succ.p, ctx->ei.p, ctx->ej.p, ctx->ew.p, st));
         FH_HIP(fh::launch_cycle_keep(C, ctx->succ.p, cb, ctx->keep.p, st));
@@ -782,8 +816,8 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
             FH_HIP(hipMemcpy(&cnt, be.row_count, sizeof(int), hipMemcpyDeviceToHost));
             exact_rows += cnt;
         }
-        query_ms += event_ms(ctx->ev[5], ctx->ev[6]);
-        query_main_ms += event_ms(ctx->ev[8], ctx->ev[9]);
+        query_ms += span(ctx, 5, 6);
+        query_main_ms += span(ctx, 8, 9);
         const int emitted = ctx->h_small[0];
         const int C2 = ctx->h_small[1];
         if (emitted != C - C2 || C2 >= C)
@@ -795,21 +829,21 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
         std::swap(ctx->roots, ctx->roots2);
         ++rounds;
     }
-    FH_HIP(hipEventRecord(ctx->ev[4], st));
+    FH_HIP(mark(ctx, 4));
 
     // -- outputs
     if (core_out) FH_HIP(hipMemcpyAsync(core_out, ctx->core_o.p, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
     if (nbr_out) FH_HIP(fh::launch_neighbors(n, kq, ctx->knn_i.p, nbr_out, st));
-    FH_HIP(hipEventRecord(ctx->ev[7], st));
-    FH_HIP(hipEventSynchronize(ctx->ev[7]));
+    FH_HIP(mark(ctx, 7));
+    FH_HIP(ctx->timing ? hipEventSynchronize(ctx->ev[7]) : hipStreamSynchronize(st));
 
     S.rounds = rounds;
     S.exact_rows = exact_rows;
     S.n_edges = base;
-    S.t_tree_ms = event_ms(ctx->ev[0], ctx->ev[1]);
-    S.t_knn_ms = event_ms(ctx->ev[1], ctx->ev[2]);
-    S.t_init_ms = event_ms(ctx->ev[2], ctx->ev[3]);
-    S.t_rounds_ms = event_ms(ctx->ev[3], ctx->ev[4]);
+    S.t_tree_ms = span(ctx, 0, 1);
+    S.t_knn_ms = span(ctx, 1, 2);
+    S.t_init_ms = span(ctx, 2, 3);
+    S.t_rounds_ms = span(ctx, 3, 4);
     S.t_query_ms = query_ms;
     S.t_query_main_ms = query_main_ms;
     S.t_total_ms = total.ms();
@@ -828,6 +862,71 @@ int check_args(fh_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t ms) {
     if (d < 1 || d > FH_MAX_DIM) return fail(ctx, FH_EUNSUPPORTED, "d outside [1, FH_MAX_DIM]");
     if (ms < 1) return fail(ctx, FH_EINVAL, "min_samples must be >= 1");
     return FH_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------- allocation-free entry ----
+//
+// fh_mst_euclidean_device_noalloc keeps every device buffer of its calls in
+// this file-scope array: FH_NOALLOC_SCRATCH_MIB MiB that each GPU the
+// library runs on holds until the process ends.  Host code reaches it only
+// through the address hipGetSymbolAddress returns on that device (the
+// array's name is a host-side stand-in there).
+static __device__ __align__(256) unsigned char scratch[size_t(FH_NOALLOC_SCRATCH_MIB) << 20];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+
+struct NoAllocSlot {
+    std::mutex mu;                   // one call at a time per device
+    bool ready = false;
+    unsigned char *scratch_ptr = nullptr;  // device address of `scratch`
+    int small[64] = {};              // readback slots (pageable host memory)
+    fh_ctx ctx;                      // host-side state of the calls
+};
+
+NoAllocSlot g_noalloc[kNoAllocDevices];
+
+int noalloc_run(NoAllocSlot &sl, int device, void *stream, const float *X, int64_t n, int32_t d, int32_t ms,
+                float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst, float *w_sq, fh_stats *stats) {
+    fh_ctx *ctx = &sl.ctx;
+    FH_TRY(check_args(ctx, X, n, d, ms));
+    if (n > 1 && (!src || !dst || !w_sq)) return fail(ctx, FH_EINVAL, "edge outputs are null");
+    FH_HIP(hipSetDevice(device));
+    if (!sl.ready) {
+        // once per device, with that device current
+        void *p = nullptr;
+        FH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        sl.scratch_ptr = static_cast<unsigned char *>(p);
+        ctx->device = device;
+        ctx->timing = false;
+        ctx->h_small = sl.small;
+        ctx->arena = sl.scratch_ptr;
+        ctx->arena_cap = size_t(FH_NOALLOC_SCRATCH_MIB) << 20;
+        sl.ready = true;
+    }
+    ctx->stream = static_cast<hipStream_t>(stream);
+    // every call carves its buffers from offset 0: nothing survives a call
+    for_each_buf(ctx, [](auto &b) {
+        b.p = nullptr;
+        b.cap = 0;
+    });
+    ctx->arena_used = 0;
+    ctx->geom_n = -1;
+    ctx->tree_n = -1;
+    // a lower bound of what the buffers take (points in tree order, kNN
+    // rows, per-point round state): larger inputs are refused before any
+    // launch; the carving itself refuses any input that does not fit
+    const int64_t kq = ms > 1 ? ms + 1 : 2;
+    const double need = double(n) * double(4 * d + 8 * kq + 140);
+    if (need > double(ctx->arena_cap))
+        return fail(ctx, FH_EINVAL,
+                    "input needs more than the " + std::to_string(FH_NOALLOC_SCRATCH_MIB) +
+                        " MiB scratch of the allocation-free entry point (at least " +
+                        std::to_string(int64_t(need) >> 20) + " MiB)");
+    return run_mst(ctx, X, n, d, ms, nullptr, 0, core_sq, nbr, src, dst, w_sq, stats);
 }
 
 }  // namespace
@@ -871,30 +970,7 @@ void fh_destroy(fh_ctx *c) {
     hipError_t e = hipSetDevice(c->device);
     (void)e;
     if (c->stream) e = hipStreamSynchronize(c->stream);
-    release(c->xp); release(c->box); release(c->core_p); release(c->core_o); release(c->knn_d);
-    release(c->w_init); release(c->bound); release(c->ew); release(c->heap_p); release(c->queries);
-    release(c->out_w); release(c->out_core);
-    release(c->perm); release(c->posof); release(c->comp_p); release(c->node_comp); release(c->knn_i);
-    release(c->succ); release(c->dst_raw); release(c->S0); release(c->M0); release(c->S1); release(c->M1);
-    release(c->keep); release(c->offs); release(c->parent); release(c->roots); release(c->roots2);
-    release(c->rootslot); release(c->partner); release(c->ei); release(c->ej); release(c->tiles);
-    release(c->small); release(c->heap_i); release(c->flag); release(c->out_src); release(c->out_dst);
-    release(c->out_nbr); release(c->range); release(c->best);
-    release(c->xdev); release(c->wdev); release(c->tkeys); release(c->obox); release(c->taxis); release(c->tcnt);
-    release(c->gtpos); release(c->ltpos); release(c->tactive); release(c->taux); release(c->tseg);
-    release(c->nb_pos); release(c->cert_of_pos); release(c->nb_d); release(c->mincore); release(c->cert);
-    release(c->sk1); release(c->sk1s); release(c->sk2); release(c->sk2s); release(c->sv1); release(c->sv1s);
-    release(c->stemp);
-    release(c->pc_keys); release(c->pc_keys_s); release(c->pc_parent); release(c->pc_vals); release(c->pc_vals_s);
-    release(c->pc_counts); release(c->pc_runs); release(c->pc_temp);
-    release(c->bx_xc); release(c->bx_nrm); release(c->bx_thr); release(c->bx_rowmin); release(c->bx_cand);
-    release(c->bx_cnt); release(c->bx_flags); release(c->bx_rowarg); release(c->bx_rows); release(c->bx_small);
-    release(c->bx_nmax);
-    release(c->ml_feat); release(c->ml_w); release(c->ml_g); release(c->ml_m); release(c->ml_v); release(c->ml_mu);
-    release(c->ml_sd); release(c->ml_scratch); release(c->ml_tgt); release(c->ml_pred); release(c->ml_bound);
-    release(c->ml_stats); release(c->ml_rows); release(c->ml_miss);
-    release(c->kr_keys); release(c->kr_keys_s); release(c->kr_vals); release(c->kr_vals_s); release(c->kr_src);
-    release(c->kr_dst); release(c->kr_w); release(c->kr_core);
+    if (!c->arena) for_each_buf(c, [](auto &b) { release(b); });
     if (c->h_small) e = hipHostFree(c->h_small);
     for (auto &ev : c->ev)
         if (ev) e = hipEventDestroy(ev);
@@ -931,6 +1007,21 @@ int fh_mst_euclidean_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, i
         kq_w = static_cast<int>(k);
     }
     return run_mst(ctx, X, n, d, min_samples, sample_weights, kq_w, core_sq, nbr, src, dst, w_sq, stats);
+}
+
+int fh_mst_euclidean_device_noalloc(int32_t device, void *stream, const float *X, int64_t n, int32_t d,
+                                    int32_t min_samples, float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst,
+                                    float *w_sq, fh_stats *stats, char *err, int32_t err_cap) {
+    int count = 0;
+    if (device < 0 || device >= kNoAllocDevices || hipGetDeviceCount(&count) != hipSuccess || device >= count) {
+        if (err && err_cap > 0) std::snprintf(err, (size_t)err_cap, "no HIP device %d", (int)device);
+        return FH_EINVAL;
+    }
+    NoAllocSlot &sl = g_noalloc[device];
+    std::lock_guard<std::mutex> lock(sl.mu);
+    const int code = noalloc_run(sl, device, stream, X, n, d, min_samples, core_sq, nbr, src, dst, w_sq, stats);
+    if (err && err_cap > 0) std::snprintf(err, (size_t)err_cap, "%s", code == FH_OK ? "" : sl.ctx.err.c_str());
+    return code;
 }
 
 int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t min_samples,
@@ -1029,14 +1120,16 @@ int fh_build_kdtree(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int64_t *
     FH_TRY(build_tree_device(ctx, ctx->xdev.p, n, d, &levels, &N));
     std::vector<int> perm(n);
     std::vector<float> box((size_t)N * 2 * d);
+    std::vector<int2> range(N);
     FH_HIP(hipMemcpyAsync(perm.data(), ctx->perm.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
     FH_HIP(hipMemcpyAsync(box.data(), ctx->box.p, sizeof(float) * N * 2 * d, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(range.data(), ctx->range.p, sizeof(int2) * N, hipMemcpyDeviceToHost, st));
     FH_HIP(hipStreamSynchronize(st));
     for (int64_t i = 0; i < n; ++i) idx_array[i] = perm[i];
     const int64_t first_leaf = (N - 1) / 2;
     for (int64_t i = 0; i < N; ++i) {
-        idx_start[i] = ctx->geom_range[i].x;
-        idx_end[i] = ctx->geom_range[i].y;
+        idx_start[i] = range[i].x;
+        idx_end[i] = range[i].y;
         is_leaf[i] = i >= first_leaf ? 1 : 0;
         float r = 0.0f;
         for (int j = 0; j < d; ++j) {

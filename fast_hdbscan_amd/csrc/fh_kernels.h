@@ -91,6 +91,8 @@ hipError_t launch_weighted_core(int n, int k, const int *knn_i, const float *knn
                                 const int *posof, float *core_o, float *core_p, hipStream_t st);
 hipError_t launch_neighbors(int64_t n, int k, const int *knn_i, int *nbr, hipStream_t st);
 
+hipError_t launch_node_ranges(int64_t N, int n, int2 *range, hipStream_t st);
+
 int scan_tiles(int m);
 hipError_t launch_excl_scan(int m, const int *in, int *out, int *tile_sum, int *total, hipStream_t st);
 hipError_t launch_cycle_keep(int m, const int *succ, CycleBufs &b, int *keep, hipStream_t st);

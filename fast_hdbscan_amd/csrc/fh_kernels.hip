@@ -938,6 +938,29 @@ hipError_t launch_neighbors(int64_t n, int k, const int *knn_i, int *nbr, hipStr
 
 int scan_tiles(int m) { return (m + kScanTile - 1) / kScanTile; }
 
+// Node ranges of the implicit tree (numba_kdtree.py:426-473: a node's left
+// child takes n_points // 2 of its points): each node descends from the root
+// along the bits of its index.
+__global__ void node_range_kernel(int64_t N, int n, int2 *range) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int level = 0;
+    while (((int64_t(1) << (level + 1)) - 1) <= i) ++level;
+    const int64_t k = i - ((int64_t(1) << level) - 1);  // index within the level
+    int s = 0, e = n;
+    for (int b = level - 1; b >= 0; --b) {
+        const int m = s + (e - s) / 2;
+        if ((k >> b) & 1) s = m;
+        else e = m;
+    }
+    range[i] = make_int2(s, e);
+}
+
+hipError_t launch_node_ranges(int64_t N, int n, int2 *range, hipStream_t st) {
+    hipLaunchKernelGGL(node_range_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, n, range);
+    return hipGetLastError();
+}
+
 hipError_t launch_excl_scan(int m, const int *in, int *out, int *tile_sum, int *total, hipStream_t st) {
     const int tiles = scan_tiles(m);
     if (m == 0) {

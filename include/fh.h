@@ -122,6 +122,29 @@ int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t 
                      const float *sample_weights, float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst,
                      float *w_sq, fh_stats *stats);
 
+/*
+ * The same MST without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ *
+ * Same outputs as fh_mst_euclidean_device without sample weights (DEVICE
+ * pointers on `device`; core_sq, nbr and stats may be NULL; stats carry no
+ * times), with every work buffer carved from one static device array of
+ * FH_NOALLOC_SCRATCH_MIB MiB that each GPU the library runs on holds until
+ * the process ends.  The call makes no hipMalloc, hipHostMalloc or hipFree
+ * and creates no stream or event: its kernels run on `stream` (a
+ * hipStream_t; NULL for the null stream) and it returns once they are done.
+ * Inputs that need more scratch than that (about 400,000 points at d = 2,
+ * min_samples = 10; fewer in higher dimensions) return FH_EINVAL with a
+ * message in err (err_cap bytes, may be NULL).  Calls on one device share
+ * the array and run one at a time (a lock per device); the array keeps the
+ * last call's contents, which no call reads.
+ */
+#define FH_NOALLOC_SCRATCH_MIB 128
+int fh_mst_euclidean_device_noalloc(int32_t device, void *stream, const float *X, int64_t n, int32_t d,
+                                    int32_t min_samples, float *core_sq, int32_t *nbr, int32_t *src, int32_t *dst,
+                                    float *w_sq, fh_stats *stats, char *err, int32_t err_cap);
+
 /* Width of the neighbour output: min_samples (1 when min_samples == 1), or
  * int(2 * min_samples / mean(w)) - 1 with HOST sample weights
  * (boruvka.py:564-569, 647).  -1 for invalid arguments. */
