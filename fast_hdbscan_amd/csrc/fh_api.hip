@@ -621,9 +621,6 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     const bool weighted = weights_d != nullptr;
     // boruvka.py:567-569 (weighted), 581-583, 593-595
     const int kq = weighted ? kq_w : (ms > 1 ? ms + 1 : 2);
-    FH_TRY(ensure(ctx, ctx->knn_d, (size_t)n * kq));
-    FH_TRY(ensure(ctx, ctx->knn_i, (size_t)n * kq));
-    FH_TRY(ensure(ctx, ctx->core_o, n));
     FH_TRY(ensure(ctx, ctx->core_p, n));
     FH_TRY(ensure(ctx, ctx->comp_p, n));
     FH_TRY(ensure(ctx, ctx->node_comp, n_nodes));
@@ -667,9 +664,26 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     const float margin_c = 8.0f * dp + 64.0f;
     fh::BruteScanArgs bsa{};
     bool knn_d_rows = true;  // the kNN distance rows were written
-    if (brute) {
+    const bool learned = ctx->learned_mode != 0 && !weighted && ms > 1 && d <= 8 && kq <= 128 && n >= 1000;
+    // Tree-order kNN rows (the self-query writes each tree position's
+    // columns 1..k-1 as positions and its core by position, all coalesced
+    // across a wavefront): the forest and the outputs are then taken from
+    // those rows, and no point-order rows are written
+    const bool tree_rows = !brute && !learned && !weighted && std::getenv("FH_KNN_ROWS") == nullptr;
+    if (!tree_rows) {
+        // point-order kNN rows
+        FH_TRY(ensure(ctx, ctx->knn_d, (size_t)n * kq));
+        FH_TRY(ensure(ctx, ctx->knn_i, (size_t)n * kq));
+        FH_TRY(ensure(ctx, ctx->core_o, n));
+    }
+    if (tree_rows) {
+        FH_TRY(ensure(ctx, ctx->nb_pos, (size_t)n * (kq - 1)));
+        FH_TRY(ensure(ctx, ctx->nb_d, (size_t)n * (kq - 1)));
+        FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, nullptr, nullptr, nullptr, ctx->core_p.p,
+                       ms > 1 ? 1 : 0, ctx->nb_pos.p, ctx->nb_d.p, 0));
+    } else if (brute) {
         FH_TRY(brute_knn(ctx, n, d, levels, kq, ms > 1 ? 1 : 0, bsa, S));
-    } else if (ctx->learned_mode != 0 && !weighted && ms > 1 && d <= 8 && kq <= 128 && n >= 1000) {
+    } else if (learned) {
         FH_TRY(learned_knn(ctx, n, d, levels, kq, seed ? ctx->nb_pos.p : nullptr, seed ? ctx->nb_d.p : nullptr, S));
     } else {
         // the distance rows are read only by the weighted core and, when
@@ -695,8 +709,12 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
 
     // -- kNN-seeded forest
     fh::CycleBufs cb{ctx->S0.p, ctx->M0.p, ctx->S1.p, ctx->M1.p};
-    FH_HIP(fh::launch_init_edges(ni, kq, ctx->knn_i.p, knn_d_rows ? ctx->knn_d.p : nullptr, ctx->core_o.p,
-                                 ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p, st));
+    if (tree_rows)
+        FH_HIP(fh::launch_init_edges_pos(ni, kq, ctx->nb_pos.p, ms <= 1 ? ctx->nb_d.p : nullptr, ctx->core_p.p,
+                                         ctx->perm.p, ctx->posof.p, ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p, st));
+    else
+        FH_HIP(fh::launch_init_edges(ni, kq, ctx->knn_i.p, knn_d_rows ? ctx->knn_d.p : nullptr, ctx->core_o.p,
+                                     ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p, st));
     FH_HIP(fh::launch_cycle_keep(ni, ctx->succ.p, cb, ctx->keep.p, st));
     FH_HIP(fh::launch_excl_scan(ni, ctx->keep.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 0, st));
     FH_HIP(fh::launch_init_emit_union(ni, ctx->keep.p, ctx->offs.p, ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p,
@@ -727,6 +745,10 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     qa.levels = levels;
     qa.pass = 0;
     qa.sample_mask = kSampleMask;
+    // the wave-cooperative main search stages one leaf (< 2 * leaf_size
+    // points) per wave in LDS; FH_QUERY_KERNEL=lane keeps one lane per point
+    qa.wave = (d <= 8 && ctx->geom_maxsize[levels - 1] <= 128) ? 1 : 0;
+    if (const char *m = std::getenv("FH_QUERY_KERNEL")) qa.wave = std::strcmp(m, "lane") == 0 ? 0 : qa.wave;
     if (const char *m = std::getenv("FH_SAMPLE_MASK")) qa.sample_mask = std::atoi(m);
     qa.cert_of_pos = pruned ? ctx->cert_of_pos.p : nullptr;
     qa.cert = pruned ? ctx->cert.p : nullptr;
@@ -745,6 +767,20 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     ca.d = d;
     double query_ms = 0.0, query_main_ms = 0.0;
     int rounds = 0;
+    // Which bound-tightening passes run before the main search (each only
+    // prunes; outputs never change): FH_SEED_ROUNDS kNN seeding for the first
+    // rounds only, FH_SAMPLED=1 adds the sampled pass (one lane per sampled
+    // point; it paid before the main search was wave-cooperative, and costs
+    // more than it prunes since: 2-D 2M 14.8 -> 13.7 ms, 3-D 10M 127 -> 116
+    // ms without it), FH_CERT=0 drops subtree certification.
+    struct {
+        int seed_rounds = 1 << 30;
+        bool sampled = false, cert = true;
+    } tune;
+    tune.sampled = !qa.wave;
+    if (const char *m = std::getenv("FH_SEED_ROUNDS")) tune.seed_rounds = std::atoi(m);
+    if (const char *m = std::getenv("FH_SAMPLED")) tune.sampled = std::atoi(m) != 0;
+    if (const char *m = std::getenv("FH_CERT")) tune.cert = std::atoi(m) != 0;
     fh::BoruvkaExactArgs be{};
     be.xp = ctx->xp.p;
     be.range = ctx->range.p;
@@ -783,12 +819,17 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
         } else if (pruned) {
             // bounds from kNN cross edges, then a sampled search, then
             // certification of whole subtrees, then everyone else
-            if (seed)
+            if (seed && rounds < tune.seed_rounds)
                 FH_HIP(fh::launch_seed_bounds(n, kq - 1, ctx->nb_pos.p, ctx->nb_d.p, ctx->core_p.p, ctx->comp_p.p,
                                               ctx->bound.p, st));
-            qa.pass = 1;
-            FH_HIP(fh::launch_query(qa, st));
-            FH_HIP(fh::launch_cert(ca, st));
+            if (tune.sampled) {
+                qa.pass = 1;
+                FH_HIP(fh::launch_query(qa, st));
+            } else {
+                qa.sample_mask = -1;  // no position counts as sampled
+            }
+            qa.cert = tune.cert ? ctx->cert.p : nullptr;
+            if (tune.cert) FH_HIP(fh::launch_cert(ca, st));
             qa.pass = 2;
             FH_HIP(mark(ctx, 8));
             FH_HIP(fh::launch_query(qa, st));
@@ -832,8 +873,14 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     FH_HIP(mark(ctx, 4));
 
     // -- outputs
-    if (core_out) FH_HIP(hipMemcpyAsync(core_out, ctx->core_o.p, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-    if (nbr_out) FH_HIP(fh::launch_neighbors(n, kq, ctx->knn_i.p, nbr_out, st));
+    if (tree_rows) {
+        FH_HIP(fh::launch_outputs_by_point(n, kq, ctx->posof.p, ctx->perm.p, ctx->core_p.p, ctx->nb_pos.p, core_out,
+                                           nbr_out, st));
+    } else {
+        if (core_out)
+            FH_HIP(hipMemcpyAsync(core_out, ctx->core_o.p, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+        if (nbr_out) FH_HIP(fh::launch_neighbors(n, kq, ctx->knn_i.p, nbr_out, st));
+    }
     FH_HIP(mark(ctx, 7));
     FH_HIP(ctx->timing ? hipEventSynchronize(ctx->ev[7]) : hipStreamSynchronize(st));
 
@@ -1262,11 +1309,26 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
     std::stable_sort(queue.begin(), queue.end(), [&](int32_t a, int32_t b) {
         return n[a] * std::max(d[a], 1) > n[b] * std::max(d[b], 1);
     });
+    if (device < 0 || device >= kNoAllocDevices) return FH_EINVAL;
+    // The workers' contexts (stream, events, pinned slots, device buffers)
+    // are kept for the process's lifetime and reused by later calls: no
+    // stream or buffer is created or destroyed while other workers' kernels
+    // run, and a call starts with buffers already sized.  One batch call per
+    // device at a time.
+    static std::mutex batch_mu[kNoAllocDevices];
+    static std::vector<fh_ctx *> batch_ctx[kNoAllocDevices];
+    std::lock_guard<std::mutex> batch_lock(batch_mu[device]);
+    std::vector<fh_ctx *> &pool = batch_ctx[device];
+    while ((int)pool.size() < W) {
+        fh_ctx *c = fh_create(device);
+        if (!c) return FH_EDEVICE;
+        pool.push_back(c);
+    }
     std::atomic<int> next{0};
     std::atomic<int> err{0};
-    auto work = [&]() {
-        fh_ctx *ctx = fh_create(device);
-        if (!ctx) {
+    auto work = [&](int wi) {
+        fh_ctx *ctx = pool[wi];
+        if (hipSetDevice(device) != hipSuccess) {
             err = FH_EDEVICE;
             return;
         }
@@ -1324,10 +1386,9 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
             if (ms_out)
                 ms_out[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
-        fh_destroy(ctx);
     };
     std::vector<std::thread> th;
-    for (int t = 0; t < W; ++t) th.emplace_back(work);
+    for (int t = 0; t < W; ++t) th.emplace_back(work, t);
     for (auto &x : th) x.join();
     return err.load();
 }

@@ -56,6 +56,7 @@ struct QueryArgs {
     int sample_mask;             // 2^s - 1: position p is sampled iff (p & mask) == mask >> 1
     const int *cert_of_pos;      // certification node (in-level index) of each position
     const unsigned char *cert;   // 1: the node's points cannot beat their component's bound
+    int wave;                    // passes 0 and 2 for d <= 8: one wavefront per 64 positions walks the tree
 };
 
 // Subtree certification: nodes of one level whose points provably have no
@@ -80,6 +81,10 @@ struct CycleBufs {
 };
 
 hipError_t launch_knn(const KnnArgs &a, hipStream_t st);
+hipError_t launch_init_edges_pos(int n, int k, const int *nb_pos, const float *nb_d, const float *core_p,
+                                 const int *perm, const int *posof, int *succ, int *dst_raw, float *w, hipStream_t st);
+hipError_t launch_outputs_by_point(int64_t n, int k, const int *posof, const int *perm, const float *core_p,
+                                   const int *nb_pos, float *core_out, int *nbr, hipStream_t st);
 hipError_t launch_cert(const CertArgs &a, hipStream_t st);
 hipError_t launch_cert_setup(int level, const int2 *range, const float *core_p, int *cert_of_pos, float *mincore,
                              hipStream_t st);

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     for (int j = 0; j < k; ++j) {
         const int pj = h.I(j);
         if (a.out_d != nullptr) a.out_d[row * k + j] = h.P(j);
-        a.out_i[row * k + j] = pj >= 0 ? a.perm[pj] : -1;
+        if (a.out_i != nullptr) a.out_i[row * k + j] = pj >= 0 ? a.perm[pj] : -1;
     }
     if (a.nb_pos != nullptr) {
         for (int j = 1; j < k; ++j) {
@@ -196,10 +196,10 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
             a.nb_d[qp * (k - 1) + j - 1] = h.P(j);
         }
     }
-    if (a.core_o != nullptr) {
+    if (a.core_o != nullptr || a.core_p != nullptr) {
         const float c = a.core_from_knn ? h.P(k - 1) : 0.0f;
-        a.core_o[row] = c;
-        a.core_p[qp] = c;
+        if (a.core_o != nullptr) a.core_o[row] = c;
+        if (a.core_p != nullptr) a.core_p[qp] = c;
     }
 }
 
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) void boruvka_query_kernel(QueryArgs a) {
     if constexpr (PASS != 0) {
         const bool sampled = (pos & a.sample_mask) == (a.sample_mask >> 1);
         if ((PASS == 1) != sampled) return;
-        if (PASS == 2 && a.cert[a.cert_of_pos[pos]]) return;
+        if (PASS == 2 && a.cert && a.cert[a.cert_of_pos[pos]]) return;
     }
     const int dd = D > 0 ? D : a.d;
     const int comp = a.comp_p[pos];
@@ -451,6 +451,181 @@ __global__ __launch_bounds__(64) void cert_kernel(CertArgs a) {
     if (lane == 0) a.cert[k] = fail_s ? 0 : 1;
 }
 
+// ------------------------------------------ wave-cooperative search ----
+//
+// The main nearest-edge search with one wavefront per 64 consecutive tree
+// positions (spatially adjacent query points).  The wave walks the tree once
+// for all its lanes, depth first, the child nearer to the lanes' bounding
+// box first; it enters a node when at least one lane passes the per-point
+// tests of boruvka.py:176-192 there (lower bound against the lane's own best
+// and its component bound, core against the bound, single-component node),
+// stages each leaf it reaches in LDS with coalesced loads (points, core
+// distances, component labels), and every lane passing the tests at that
+// leaf scans the staged points (boruvka.py:196-212).
+//
+// Exactness (the selection rule of the one-thread reference run): only the
+// lanes whose point achieves its component's minimum w* matter, and for such
+// a lane every candidate j of weight w* lies in nodes whose lower bound is
+// <= w* <= best, <= w* <= B, with core_j <= w* <= B: no test prunes it, so
+// the lane scans every candidate of weight w*.  Equal weights are resolved
+// by the reference's depth-first order for that point (closer child first,
+// left on equal bounds, position order within a leaf), so the lane keeps the
+// first such candidate in that order, as the recursion records it.
+// Lanes of larger weight report values >= w* that cannot win the packed
+// atomicMin unless they also achieve w*.
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// True when position j1 comes before j2 in the depth-first visiting order of
+// the query for point x (the recursion of boruvka.py:228-285).
+template <int D>
+__device__ bool dfs_first(const float *x, int j1, int j2, const int2 *__restrict__ range,
+                          const float *__restrict__ box, int levels) {
+    int node = 0;
+    for (int depth = 0; depth < levels - 1; ++depth) {
+        const int l = 2 * node + 1;
+        const int split = range[l].y;
+        const bool a1 = j1 < split, a2 = j2 < split;
+        if (a1 == a2) {
+            node = a1 ? l : l + 1;
+            continue;
+        }
+        const float lbl = box_lb<D>(x, box + (size_t)l * 2 * D, D);
+        const float lbr = box_lb<D>(x, box + (size_t)(l + 1) * 2 * D, D);
+        return a1 == (lbl <= lbr);
+    }
+    return j1 < j2;
+}
+
+constexpr int kWaveStage = 128;  // staged points per leaf (leaves hold < 2 * leaf_size = 80)
+
+template <int D, int PASS>
+__global__ __launch_bounds__(256) void boruvka_wave_kernel(QueryArgs a) {
+    __shared__ float s_x[4][kWaveStage * D];
+    __shared__ float s_c[4][kWaveStage];
+    __shared__ int s_k[4][kWaveStage];
+    const int wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t pos = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    bool act = pos < a.n;
+    if constexpr (PASS == 2) {
+        if (act) {
+            const bool sampled = (pos & a.sample_mask) == (a.sample_mask >> 1);
+            act = !sampled && !(a.cert && a.cert[a.cert_of_pos[pos]]);
+        }
+    }
+    if (__ballot(act) == 0) return;  // the whole wave has nothing to search
+    float x[D];
+    float core_i = kInf, B = 0.0f;
+    int comp = -1;
+    if (act) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) x[j] = a.xp[pos * D + j];
+        comp = a.comp_p[pos];
+        core_i = a.core_p[pos];
+        B = ld_relaxed(a.bound + comp);
+    } else {
+#pragma unroll
+        for (int j = 0; j < D; ++j) x[j] = 0.0f;
+    }
+    // the active lanes' bounding box
+    float gb[2 * D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        float lo = act ? x[j] : kInf, hi = act ? x[j] : -kInf;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+            lo = l2 < lo ? l2 : lo;
+            hi = h2 > hi ? h2 : hi;
+        }
+        gb[j] = lo;
+        gb[D + j] = hi;
+    }
+    float best = kInf;
+    int best_j = -1;
+    float *sx = s_x[wv];
+    float *sc = s_c[wv];
+    int *sk = s_k[wv];
+    const int levels = a.levels;
+    int node = 0, depth = 0;
+    uint32_t first_right = 0;
+    bool entering = true;
+    for (;;) {
+        if (entering) {
+            const float *nb = a.box + (size_t)node * 2 * D;
+            const float lb = box_lb<D>(x, nb, D);
+            const bool t = act && !(core_i > B) && !(lb > best) && !(lb > B) && a.node_comp[node] != comp;
+            if (__ballot(t) != 0) {
+                if (depth == levels - 1) {
+                    const int2 r = a.range[node];
+                    const int cnt = r.y - r.x;
+                    for (int q = lane; q < cnt; q += 64) {
+                        const int64_t g = r.x + q;
+#pragma unroll
+                        for (int j = 0; j < D; ++j) sx[q * D + j] = a.xp[g * D + j];
+                        sc[q] = a.core_p[g];
+                        sk[q] = a.comp_p[g];
+                    }
+                    wave_sync();
+                    if (t) {
+                        B = ld_relaxed(a.bound + comp);
+                        for (int q = 0; q < cnt; ++q) {
+                            if (sk[q] == comp) continue;
+                            const float cj = sc[q];
+                            if (!(cj <= B)) continue;
+                            float dist = rdist<D>(x, sx + q * D, D);
+                            dist = dist > core_i ? dist : core_i;
+                            dist = dist > cj ? dist : cj;
+                            if (dist < best) {
+                                best = dist;
+                                best_j = r.x + q;
+                                if (dist < B) {
+                                    atomicMin(reinterpret_cast<unsigned int *>(a.bound + comp), __float_as_uint(dist));
+                                    B = dist;
+                                }
+                            } else if (dist == best && best_j >= 0 &&
+                                       dfs_first<D>(x, r.x + q, best_j, a.range, a.box, levels)) {
+                                best_j = r.x + q;
+                            }
+                        }
+                    }
+                    wave_sync();
+                } else {
+                    const int l = 2 * node + 1;
+                    const float lbl = box_box_lb<D>(gb, a.box + (size_t)l * 2 * D, D);
+                    const float lbr = box_box_lb<D>(gb, a.box + (size_t)(l + 1) * 2 * D, D);
+                    const bool rf = !(lbl <= lbr);
+                    ++depth;
+                    first_right = rf ? (first_right | (1u << depth)) : (first_right & ~(1u << depth));
+                    node = rf ? l + 1 : l;
+                    continue;
+                }
+            }
+        }
+        if (depth == 0) break;
+        const bool rf = (first_right >> depth) & 1u;
+        const bool is_right = (node & 1) == 0;
+        if (is_right == rf) {
+            node = is_right ? node - 1 : node + 1;
+            entering = true;
+        } else {
+            node = (node - 1) >> 1;
+            --depth;
+            entering = false;
+        }
+    }
+    if (act && best_j >= 0) {
+        const unsigned long long key = ((unsigned long long)__float_as_uint(best) << 32) | (unsigned int)a.perm[pos];
+        atomicMin(a.best + comp, key);
+        a.partner[pos] = best_j;
+    }
+}
+
 // sample_weight_core_distance (boruvka.py:527-539): the first sorted
 // neighbour at which the float64 running weight reaches min_samples.  A -1
 // neighbour (k > n) reads the last weight, as numba's indexing does.
@@ -499,6 +674,54 @@ __global__ void init_edges_kernel(int n, int k, const int *__restrict__ knn_i, c
     succ[i] = s;
     dst_raw[i] = draw;
     w[i] = wi;
+}
+
+// The same choice from tree-order kNN rows (KD-tree positions, -1 padding):
+// position p of point i = perm[p] scans its columns 1..k-1; the edge fields
+// are written by point index for the emission in point order.
+__global__ void init_edges_pos_kernel(int n, int k, const int *__restrict__ nb_pos, const float *__restrict__ nb_d,
+                                      const float *__restrict__ core_p, const int *__restrict__ perm,
+                                      const int *__restrict__ posof, int *succ, int *dst_raw, float *w) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float ci = core_p[p];
+    const int C = k - 1;
+    int s = -1, draw = -1;
+    float wi = 0.0f;
+    for (int j = 0; j < C; ++j) {
+        const int q = nb_pos[(int64_t)p * C + j];
+        const int kw = q < 0 ? n - 1 : perm[q];
+        const int qp = q < 0 ? posof[n - 1] : q;
+        if (ci >= core_p[qp]) {
+            const float dj = nb_d != nullptr ? nb_d[(int64_t)p * C + j] : ci;
+            s = kw;
+            draw = q < 0 ? -1 : kw;
+            wi = ci > dj ? ci : dj;
+            break;
+        }
+    }
+    const int i = perm[p];
+    succ[i] = s;
+    dst_raw[i] = draw;
+    w[i] = wi;
+}
+
+// Outputs by point index from tree-order rows: the core, and the neighbour
+// columns as point indices (-1 kept).
+__global__ void core_by_point_kernel(int n, const int *__restrict__ posof, const float *__restrict__ core_p,
+                                     float *core_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) core_out[i] = core_p[posof[i]];
+}
+
+__global__ void nbr_by_point_kernel(int64_t n, int C, const int *__restrict__ posof, const int *__restrict__ perm,
+                                    const int *__restrict__ nb_pos, int *nbr) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * C) return;
+    const int64_t i = e / C;
+    const int c = (int)(e - i * C);
+    const int q = nb_pos[(int64_t)posof[i] * C + c];
+    nbr[e] = q < 0 ? -1 : perm[q];
 }
 
 // ----------------------------------------------- cycles by doubling ----
@@ -869,6 +1092,15 @@ static hipError_t launch_knn_t(const KnnArgs &a, hipStream_t st) {
 
 template <int D>
 static hipError_t launch_query_t(const QueryArgs &a, hipStream_t st) {
+    if constexpr (D > 0) {
+        if (a.wave && a.pass != 1) {
+            if (a.pass == 2)
+                hipLaunchKernelGGL((boruvka_wave_kernel<D, 2>), dim3(blocks(a.n, 256)), dim3(256), 0, st, a);
+            else
+                hipLaunchKernelGGL((boruvka_wave_kernel<D, 0>), dim3(blocks(a.n, 256)), dim3(256), 0, st, a);
+            return hipGetLastError();
+        }
+    }
     if (a.pass == 1)
         hipLaunchKernelGGL((boruvka_query_kernel<D, 1>), dim3(blocks(a.n, 256)), dim3(256), 0, st, a);
     else if (a.pass == 2)
@@ -984,6 +1216,24 @@ hipError_t launch_cycle_keep(int m, const int *succ, CycleBufs &b, int *keep, hi
         std::swap(M, M2);
     }
     hipLaunchKernelGGL(dbl_keep_kernel, dim3(blocks(m, 256)), dim3(256), 0, st, m, succ, S, M, keep);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_edges_pos(int n, int k, const int *nb_pos, const float *nb_d, const float *core_p,
+                                 const int *perm, const int *posof, int *succ, int *dst_raw, float *w, hipStream_t st) {
+    hipLaunchKernelGGL(init_edges_pos_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, k, nb_pos, nb_d, core_p, perm,
+                       posof, succ, dst_raw, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_outputs_by_point(int64_t n, int k, const int *posof, const int *perm, const float *core_p,
+                                   const int *nb_pos, float *core_out, int *nbr, hipStream_t st) {
+    if (core_out)
+        hipLaunchKernelGGL(core_by_point_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, (int)n, posof, core_p,
+                           core_out);
+    if (nbr && k > 1)
+        hipLaunchKernelGGL(nbr_by_point_kernel, dim3(blocks(n * (k - 1), 256)), dim3(256), 0, st, n, k - 1, posof, perm,
+                           nb_pos, nbr);
     return hipGetLastError();
 }
 

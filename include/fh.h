@@ -208,7 +208,9 @@ int fh_mst_kruskal_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_
  * context on `device`, take datasets largest first.  HOST pointers:
  * X[i] [n[i]*d[i]], labels[i] [n[i]] int64, probabilities[i] [n[i]]
  * float32; ms_out [count] per-dataset wall times or NULL.  No context
- * argument: the call creates and releases its workers' contexts.
+ * argument: the workers' contexts are created on first use and kept for
+ * later calls (up to 16 per device, released at process exit); calls on one
+ * device run one at a time.
  */
 int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
                      int32_t min_samples, int64_t min_cluster_size, int32_t workers, int64_t *const *labels,

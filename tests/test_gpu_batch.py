@@ -30,3 +30,20 @@ def test_batch_matches_single_calls_and_oracle():
             ref = OL.clusters_from_spanning_tree(e, min_cluster_size=20)
             np.testing.assert_array_equal(lab, ref[0])
             np.testing.assert_array_equal(pr, ref[1])
+
+
+def test_consecutive_batch_calls_reuse_workers():
+    # the workers' contexts live across calls: later calls with fewer,
+    # more and larger datasets give what one-at-a-time calls give
+    from fast_hdbscan_amd.batch import hdbscan_batch
+    from fast_hdbscan_amd.estimator import fast_hdbscan
+
+    small = workloads.batch_datasets(10, n_hi=3000)
+    large = workloads.batch_datasets(6, n_hi=30000)[::-1]
+    for data, w in ((small, 4), (large, 8), (small[:3], 2), (large + small, 8)):
+        print(f"batch call: {len(data)} datasets, {w} workers", flush=True)
+        out = hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w)
+        for X, (lab, pr) in zip(data, out):
+            l1, p1 = fast_hdbscan(X, min_samples=5, min_cluster_size=20)
+            np.testing.assert_array_equal(lab, l1)
+            np.testing.assert_array_equal(pr, p1)

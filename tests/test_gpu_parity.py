@@ -289,3 +289,39 @@ def test_sample_weight_seam_contract():
     np.testing.assert_array_equal(nb, rnb)
     np.testing.assert_array_equal(core, rcore)
     assert e.dtype == np.float64 and nb.dtype == np.int32 and core.dtype == np.float32
+
+
+@pytest.mark.parametrize("form", ["lane", "unpruned", "knn_point_rows", "sampled"])
+def test_search_kernel_forms(form, monkeypatch):
+    """The main search exists as one lane per point (FH_QUERY_KERNEL=lane)
+    and as one wavefront per 64 positions (default for d <= 8), and the
+    unpruned single pass (FH_SEARCH_MODE=0, read when a context is made:
+    run in a thread of its own); the kNN rows kept in point order
+    (FH_KNN_ROWS=point) or in tree order (default); the sampled pre-pass on
+    or off (FH_SAMPLED).  Each equals the oracle."""
+    import threading
+
+    cases = [workloads.scale_2d(120_000, seed=41), workloads.anisotropic_mixture(60_000, 3, 20, 0.05, seed=42)]
+    g = np.stack(np.meshgrid(np.arange(90), np.arange(70)), -1).reshape(-1, 2).astype(np.float32)
+    cases.append(g[np.random.default_rng(43).permutation(g.shape[0])])
+    if form == "lane":
+        monkeypatch.setenv("FH_QUERY_KERNEL", "lane")
+    elif form == "knn_point_rows":
+        monkeypatch.setenv("FH_KNN_ROWS", "point")
+    elif form == "sampled":
+        monkeypatch.setenv("FH_SAMPLED", "1")
+    else:
+        monkeypatch.setenv("FH_SEARCH_MODE", "0")
+    errors = []
+
+    def run():
+        try:
+            for X in cases:
+                _check(X, 10 if X.shape[1] == 2 else 15)
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    t = threading.Thread(target=run)
+    t.start()
+    t.join()
+    assert not errors, errors
