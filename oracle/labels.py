@@ -27,6 +27,8 @@ fastmath changes a last bit there is not pinned by any reference test.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 
@@ -420,3 +422,27 @@ def clusters_from_spanning_tree(mst, min_cluster_size=10, cluster_selection_meth
     labels = get_cluster_label_vector(ct_full, selected, cluster_selection_epsilon, n_samples=n_points)
     strengths = get_point_membership_strength_vector(ct_full, selected, labels)
     return labels, strengths, linkage, ct_full, sorted_mst
+
+
+def clusters_from_spanning_tree_c(mst, min_cluster_size=10, cluster_selection_method="eom",
+                                  max_cluster_size=np.inf, allow_single_cluster=False):
+    """The same stage compiled (oracle/labels_c.c, one thread): unit sample
+    weights, cluster_selection_epsilon 0.  Returns (labels, strengths,
+    linkage, condensed tree)."""
+    from . import oracle as O
+
+    mst = np.ascontiguousarray(mst, dtype=np.float64)
+    n = mst.shape[0] + 1
+    labels = np.empty(n, np.int64)
+    strengths = np.empty(n, np.float32)
+    linkage = np.empty((max(n - 1, 1), 4), np.float64)
+    cap = 2 * n
+    ct = (np.empty(cap, np.int64), np.empty(cap, np.int64), np.empty(cap, np.float32), np.empty(cap, np.float32))
+    rows = ctypes.c_int64(0)
+    method = {"eom": 0, "leaf": 1}[cluster_selection_method]
+    r = O.lib().orc_labels(n, mst, int(min_cluster_size), float(max_cluster_size), method,
+                           1 if allow_single_cluster else 0, labels, strengths, linkage, *ct, ctypes.byref(rows))
+    if r < 0:
+        raise ValueError("invalid arguments")
+    k = rows.value
+    return labels, strengths, linkage[:n - 1], CondensedTree(ct[0][:k], ct[1][:k], ct[2][:k], ct[3][:k])

@@ -109,6 +109,10 @@ def lib():
         L.orc_rdist_rows.argtypes = [_f32p, ctypes.c_int64, _f32p, ctypes.c_int64, ctypes.c_int, _f32p, ctypes.c_int]
         L.orc_rdist_pairs.restype = None
         L.orc_rdist_pairs.argtypes = [_f32p, ctypes.c_int, _i32p, _i32p, ctypes.c_int64, _f32p]
+        _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        L.orc_labels.restype = ctypes.c_int64
+        L.orc_labels.argtypes = [ctypes.c_int64, _f64p, ctypes.c_int64, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                 _i64p, _f32p, _f64p, _i64p, _i64p, _f32p, _f32p, ctypes.POINTER(ctypes.c_int64)]
         L.orc_box_lb.restype = ctypes.c_float
         L.orc_box_lb.argtypes = [_f32p, _f32p, _f32p, ctypes.c_int]
         _lib = L

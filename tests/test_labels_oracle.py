@@ -407,3 +407,21 @@ def test_k_exceeds_n_condensed_tree_matches_oracle():
         np.testing.assert_array_equal(a[1], b[1])
         for f in ("parent", "child", "lambda_val", "child_size"):
             np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[4] == 0.0], ids=[f"{c[0]}-{c[3]}-{c[2]}-{c[5]}" for c in CASES
+                                                                     if c[4] == 0.0])
+def test_compiled_label_oracle_matches_python_oracle(case):
+    # the compiled restatement (the CPU baseline's label stage) against the
+    # Python one, row for row
+    name, ms, mcs, meth, eps, single = case
+    if mcs <= 2:
+        sys.setrecursionlimit(max(sys.getrecursionlimit(), 20000))
+    edges, _, _ = O.compute_minimum_spanning_tree(_data(name), ms)
+    kw = dict(min_cluster_size=mcs, cluster_selection_method=meth, allow_single_cluster=single)
+    a = LB.clusters_from_spanning_tree(edges, **kw)
+    b = LB.clusters_from_spanning_tree_c(edges, **kw)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x, y)
+    for f in ("parent", "child", "lambda_val", "child_size"):
+        np.testing.assert_array_equal(getattr(a[3], f), getattr(b[3], f))
