@@ -78,7 +78,7 @@ struct fh_ctx {
     // pruned nearest-edge search
     DevBuf<int> nb_pos, cert_of_pos;
     DevBuf<float> nb_d, mincore;
-    DevBuf<unsigned char> cert;
+    DevBuf<unsigned char> cert, seed_int;
     int search_mode = 1;  // 0: one plain pass; 1: seeded + sampled + certified
     // high-dimensional MFMA candidate scans
     DevBuf<float> bx_xc, bx_nrm, bx_thr, bx_rowmin;
@@ -131,7 +131,7 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->tiles, c->small, c->heap_i, c->flag, c->out_src, c->out_dst, c->out_nbr, c->range,
         c->best, c->xdev, c->wdev, c->tkeys, c->obox, c->taxis, c->tcnt, c->gtpos,
         c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->nb_d, c->mincore,
-        c->cert, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
+        c->cert, c->seed_int, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
         c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
         c->ml_v, c->ml_mu, c->ml_sd, c->ml_scratch, c->ml_tgt, c->ml_pred, c->ml_bound, c->ml_stats,
         c->ml_rows, c->ml_miss, c->kr_keys, c->kr_keys_s, c->kr_vals, c->kr_vals_s, c->kr_src, c->kr_dst,
@@ -781,6 +781,10 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     if (const char *m = std::getenv("FH_SEED_ROUNDS")) tune.seed_rounds = std::atoi(m);
     if (const char *m = std::getenv("FH_SAMPLED")) tune.sampled = std::atoi(m) != 0;
     if (const char *m = std::getenv("FH_CERT")) tune.cert = std::atoi(m) != 0;
+    if (seed) {
+        FH_TRY(ensure(ctx, ctx->seed_int, n));
+        FH_HIP(hipMemsetAsync(ctx->seed_int.p, 0, (size_t)n, st));
+    }
     fh::BoruvkaExactArgs be{};
     be.xp = ctx->xp.p;
     be.range = ctx->range.p;
@@ -821,7 +825,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
             // certification of whole subtrees, then everyone else
             if (seed && rounds < tune.seed_rounds)
                 FH_HIP(fh::launch_seed_bounds(n, kq - 1, ctx->nb_pos.p, ctx->nb_d.p, ctx->core_p.p, ctx->comp_p.p,
-                                              ctx->bound.p, st));
+                                              ctx->bound.p, ctx->seed_int.p, st));
             if (tune.sampled) {
                 qa.pass = 1;
                 FH_HIP(fh::launch_query(qa, st));

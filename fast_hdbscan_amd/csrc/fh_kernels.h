@@ -89,7 +89,7 @@ hipError_t launch_cert(const CertArgs &a, hipStream_t st);
 hipError_t launch_cert_setup(int level, const int2 *range, const float *core_p, int *cert_of_pos, float *mincore,
                              hipStream_t st);
 hipError_t launch_seed_bounds(int64_t n, int kn, const int *nb_pos, const float *nb_d, const float *core_p,
-                              const int *comp_p, float *bound, hipStream_t st);
+                              const int *comp_p, float *bound, unsigned char *interior, hipStream_t st);
 hipError_t launch_query(const QueryArgs &a, hipStream_t st);
 hipError_t launch_posof(int n, const int *perm, int *posof, hipStream_t st);
 hipError_t launch_weighted_core(int n, int k, const int *knn_i, const float *knn_d, const float *weights, int ms,

@@ -128,9 +128,10 @@ struct KnnVisitor {
 };
 
 // One lane per query row (queries in tree order for the self-query so a
-// wavefront's lanes walk nearby paths).  Heap in LDS (use_lds) or in a
-// global scratch slice for large k.
-template <int D>
+// wavefront's lanes walk nearby paths).  Heap in LDS, or (GLOBAL_HEAP) in a
+// global scratch slice for large k: two instantiations, so that the LDS
+// heap's accesses compile to LDS instructions rather than generic ones.
+template <int D, bool GLOBAL_HEAP>
 __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     extern __shared__ unsigned char smem_raw[];
     const int tid = threadIdx.x;
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     if (q >= a.nq) return;
     const int k = a.k;
     LdsHeap h;
-    if (a.heap_p == nullptr) {
+    if constexpr (!GLOBAL_HEAP) {
         float *sp = reinterpret_cast<float *>(smem_raw);
         h.p = sp + tid;
         h.i = reinterpret_cast<int *>(sp + (size_t)k * blockDim.x) + tid;
@@ -308,22 +309,32 @@ __global__ __launch_bounds__(256) void boruvka_query_kernel(QueryArgs a) {
 
 // Cross-component edges among the kNN lists.  The distances are the kNN
 // kernel's rdist of the same two rows, bit-identical to what the search
-// computes for that pair.
+// computes for that pair.  Components only merge, so a point whose
+// neighbours all share its component keeps that property for the rest of
+// the call: it is marked interior (interior[p] = 1) and skipped from then
+// on.
 __global__ void seed_bounds_kernel(int64_t n, int kn, const int *__restrict__ nb_pos, const float *__restrict__ nb_d,
-                                   const float *__restrict__ core_p, const int *__restrict__ comp_p, float *bound) {
+                                   const float *__restrict__ core_p, const int *__restrict__ comp_p, float *bound,
+                                   unsigned char *interior) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= n || interior[p]) return;
     const int c = comp_p[p];
     const float ci = core_p[p];
     float w = kInf;
+    bool cross = false;
     for (int j = 0; j < kn; ++j) {
         const int q = nb_pos[p * kn + j];
         if (q < 0 || comp_p[q] == c) continue;
+        cross = true;
         float e = nb_d[p * kn + j];
         e = e > ci ? e : ci;
         const float cq = core_p[q];
         e = e > cq ? e : cq;
         w = e < w ? e : w;
+    }
+    if (!cross) {
+        interior[p] = 1;
+        return;
     }
     if (w < kInf) atomicMin(reinterpret_cast<unsigned int *>(bound + c), __float_as_uint(w));
 }
@@ -556,6 +567,9 @@ __global__ __launch_bounds__(256) void boruvka_wave_kernel(QueryArgs a) {
     uint32_t first_right = 0;
     bool entering = true;
     for (;;) {
+        // the walk is the wave's: its node index lives in a scalar register,
+        // so the node's box, label and range come through scalar loads
+        node = __builtin_amdgcn_readfirstlane(node);
         if (entering) {
             const float *nb = a.box + (size_t)node * 2 * D;
             const float lb = box_lb<D>(x, nb, D);
@@ -1085,8 +1099,12 @@ inline unsigned blocks(int64_t m, int b) { return (unsigned)((m + b - 1) / b); }
 template <int D>
 static hipError_t launch_knn_t(const KnnArgs &a, hipStream_t st) {
     const int block = a.block;
-    const size_t shmem = a.heap_p ? 0 : (size_t)a.k * block * (sizeof(float) + sizeof(int));
-    hipLaunchKernelGGL(knn_kernel<D>, dim3(blocks(a.nq, block)), dim3(block), shmem, st, a);
+    if (a.heap_p) {
+        hipLaunchKernelGGL((knn_kernel<D, true>), dim3(blocks(a.nq, block)), dim3(block), 0, st, a);
+    } else {
+        const size_t shmem = (size_t)a.k * block * (sizeof(float) + sizeof(int));
+        hipLaunchKernelGGL((knn_kernel<D, false>), dim3(blocks(a.nq, block)), dim3(block), shmem, st, a);
+    }
     return hipGetLastError();
 }
 
@@ -1141,10 +1159,10 @@ hipError_t launch_cert_setup(int level, const int2 *range, const float *core_p, 
 }
 
 hipError_t launch_seed_bounds(int64_t n, int kn, const int *nb_pos, const float *nb_d, const float *core_p,
-                              const int *comp_p, float *bound, hipStream_t st) {
+                              const int *comp_p, float *bound, unsigned char *interior, hipStream_t st) {
     if (kn <= 0) return hipSuccess;
     hipLaunchKernelGGL(seed_bounds_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, kn, nb_pos, nb_d, core_p,
-                       comp_p, bound);
+                       comp_p, bound, interior);
     return hipGetLastError();
 }
 

@@ -41,9 +41,11 @@ def _compare_rows(out, ref):
 
 
 def _oracle_labels(ref, mcs):
+    # the compiled label oracle (pinned row for row against the Python one
+    # in tests/test_labels_oracle.py); the Python one takes ~20 s at 2M
     from oracle import labels as LB
 
-    lab, prob, *_ = LB.clusters_from_spanning_tree(ref.edges(), min_cluster_size=mcs)
+    lab, prob, *_ = LB.clusters_from_spanning_tree_c(ref.edges(), min_cluster_size=mcs)
     return lab, prob
 
 
