@@ -73,6 +73,7 @@ class FHStats(ctypes.Structure):
         ("ball_misses", ctypes.c_int64),
         ("t_learn_ms", ctypes.c_double),
         ("t_predict_ms", ctypes.c_double),
+        ("scored_tile_pairs", ctypes.c_int64),
     ]
 
     def as_dict(self):

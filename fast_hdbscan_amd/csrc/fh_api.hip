@@ -84,6 +84,8 @@ struct fh_ctx {
     DevBuf<float> bx_xc, bx_nrm, bx_thr, bx_rowmin;
     DevBuf<int> bx_cand, bx_cnt, bx_flags, bx_rowarg, bx_rows, bx_small;
     DevBuf<unsigned int> bx_nmax;
+    DevBuf<int> bx_tilecomp;
+    DevBuf<unsigned long long> bx_scored;
     int brute_min_dim = 17;  // d from which the MFMA scans replace the tree searches
     // learned search radii
     DevBuf<float> ml_feat, ml_w, ml_g, ml_m, ml_v, ml_mu, ml_sd, ml_scratch, ml_tgt, ml_pred, ml_bound, ml_stats;
@@ -132,7 +134,7 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->best, c->xdev, c->wdev, c->tkeys, c->obox, c->taxis, c->tcnt, c->gtpos,
         c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->nb_d, c->mincore,
         c->cert, c->seed_int, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
-        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
+        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
         c->ml_v, c->ml_mu, c->ml_sd, c->ml_scratch, c->ml_tgt, c->ml_pred, c->ml_bound, c->ml_stats,
         c->ml_rows, c->ml_miss, c->kr_keys, c->kr_keys_s, c->kr_vals, c->kr_vals_s, c->kr_src, c->kr_dst,
         c->kr_w, c->kr_core, c->sk1, c->sk1s, c->sk2, c->sk2s, c->sv1, c->sv1s,
@@ -444,6 +446,11 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
     bsa.row_arg = ctx->bx_rowarg.p;
     bsa.nrm_max_bits = ctx->bx_nmax.p;
     bsa.margin_c = margin_c;
+    FH_TRY(ensure(ctx, ctx->bx_tilecomp, npad / 128));
+    FH_TRY(ensure(ctx, ctx->bx_scored, 1));
+    FH_HIP(hipMemsetAsync(ctx->bx_scored.p, 0, sizeof(unsigned long long), st));
+    bsa.tile_comp = ctx->bx_tilecomp.p;
+    bsa.scored_tiles = ctx->bx_scored.p;
     // thresholds: MODE 2 writes them where MODE 0 reads them
     fh::BruteScanArgs kth = bsa;
     kth.row_min = ctx->bx_thr.p;
@@ -813,6 +820,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
         if (brute) {
             // approximate row minima on MFMA, exact upper bound per
             // component, exact re-scan of the rows that can reach it
+            FH_HIP(fh::launch_tile_comp(n, bsa.npad, ctx->comp_p.p, ctx->bx_tilecomp.p, st));
             FH_HIP(mark(ctx, 8));
             FH_HIP(fh::launch_scan_boruvka(bsa, st));
             FH_HIP(mark(ctx, 9));
@@ -857,9 +865,9 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
                                         st));
         FH_TRY(read_small(ctx, 2));
         if (brute) {
-            int cnt = 0;
-            FH_HIP(hipMemcpy(&cnt, be.row_count, sizeof(int), hipMemcpyDeviceToHost));
-            exact_rows += cnt;
+            FH_HIP(hipMemcpyAsync(ctx->h_small + 4, be.row_count, sizeof(int), hipMemcpyDeviceToHost, st));
+            FH_HIP(hipStreamSynchronize(st));
+            exact_rows += ctx->h_small[4];
         }
         query_ms += span(ctx, 5, 6);
         query_main_ms += span(ctx, 8, 9);
@@ -873,6 +881,14 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
         C = C2;
         std::swap(ctx->roots, ctx->roots2);
         ++rounds;
+    }
+    if (brute) {
+        FH_HIP(hipMemcpyAsync(ctx->h_small + 6, ctx->bx_scored.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              st));
+        FH_HIP(hipStreamSynchronize(st));
+        unsigned long long sc = 0;
+        std::memcpy(&sc, ctx->h_small + 6, sizeof(sc));
+        S.scored_tile_pairs = (int64_t)sc;
     }
     FH_HIP(mark(ctx, 4));
 

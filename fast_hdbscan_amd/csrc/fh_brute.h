@@ -31,7 +31,14 @@ struct BruteScanArgs {
     const int *comp_p;    // [n]
     float *row_min;       // [n] smallest approximate mutual-reachability value
     int *row_arg;         // [n] position that gave it
+    // Boruvka: per 128-row tile the component of all its rows, -1 when
+    // mixed (nullptr: none); tile pairs of one component are skipped and
+    // the tile pairs scored are added to *scored_tiles
+    const int *tile_comp;
+    unsigned long long *scored_tiles;
 };
+
+hipError_t launch_tile_comp(int64_t n, int64_t npad, const int *comp_p, int *tile_comp, hipStream_t st);
 
 hipError_t launch_center_pack(int64_t n, int d, int dp, int64_t npad, const float *xp, const float *root_box,
                               float *xc, float *nrm, unsigned int *nrm_max_bits, hipStream_t st);

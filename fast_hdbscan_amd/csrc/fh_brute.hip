@@ -156,6 +156,14 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     if (MODE == 0 && tid < 128) cnt_l[tid] = 0;
 
     const int ntiles = (int)(a.npad / 128);
+    // MODE 1: column tiles whose rows all share this tile's single
+    // component hold no candidate (boruvka.py:199-201) and are skipped
+    const int my_tc = (MODE == 1 && a.tile_comp) ? a.tile_comp[blockIdx.x] : -1;
+    auto next_tile = [&](int t) {
+        if (my_tc >= 0)
+            while (t < ntiles && a.tile_comp[t] == my_tc) ++t;
+        return t;
+    };
     floatx4 pre[NV];
     float pn = 0.0f, pc = 0.0f;
     int pk = -1;
@@ -183,12 +191,18 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
         }
     };
 
-    gload(0);
-    lstore(0);
+    // the tiles scanned, in order: t (in buffer cur), then t1, then t2
+    int t = next_tile(0);
+    if (t < ntiles) {
+        gload(t);
+        lstore(0);
+    }
     __syncthreads();
-    if (ntiles > 1) gload(1);
-    for (int t = 0; t < ntiles; ++t) {
-        const int cur = t & 1;
+    int t1 = t < ntiles ? next_tile(t + 1) : ntiles;
+    if (t1 < ntiles) gload(t1);
+    int scored = 0;
+    for (int cur = 0; t < ntiles; cur ^= 1) {
+        ++scored;
         const float *bt = bs + cur * TILE;
         floatx4 acc[2][8];
 #pragma unroll
@@ -263,14 +277,19 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                     }
             }
         }
-        // tile t+1 into the other buffer (last read in iteration t-1, before
-        // the previous barrier), prefetch t+2, then one barrier
-        if (t + 1 < ntiles) {
+        // tile t1 into the other buffer (last read in the previous
+        // iteration, before the previous barrier), prefetch the one after,
+        // then one barrier
+        const int t2 = t1 < ntiles ? next_tile(t1 + 1) : ntiles;
+        if (t1 < ntiles) {
             lstore(cur ^ 1);
-            if (t + 2 < ntiles) gload(t + 2);
+            if (t2 < ntiles) gload(t2);
         }
         __syncthreads();
+        t = t1;
+        t1 = t2;
     }
+    if (MODE == 1 && a.scored_tiles && tid == 0) atomicAdd(a.scored_tiles, (unsigned long long)scored);
 
     if constexpr (MODE == 0) {
         __syncthreads();
@@ -607,6 +626,24 @@ static hipError_t launch_scan(const BruteScanArgs &a, hipStream_t st) {
 hipError_t launch_scan_knn(const BruteScanArgs &a, hipStream_t st) { return launch_scan<0>(a, st); }
 hipError_t launch_scan_kth(const BruteScanArgs &a, hipStream_t st) { return launch_scan<2>(a, st); }
 hipError_t launch_scan_boruvka(const BruteScanArgs &a, hipStream_t st) { return launch_scan<1>(a, st); }
+
+// Component of every 128-row tile whose rows (below n) share one, else -1.
+__global__ __launch_bounds__(128) void tile_comp_kernel(int64_t n, const int *__restrict__ comp_p, int *tile_comp) {
+    __shared__ int mixed;
+    const int64_t row0 = (int64_t)blockIdx.x * 128;
+    const int c0 = comp_p[row0];
+    if (threadIdx.x == 0) mixed = 0;
+    __syncthreads();
+    const int64_t row = row0 + threadIdx.x;
+    if (row < n && comp_p[row] != c0) mixed = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_comp[blockIdx.x] = mixed ? -1 : c0;
+}
+
+hipError_t launch_tile_comp(int64_t n, int64_t npad, const int *comp_p, int *tile_comp, hipStream_t st) {
+    hipLaunchKernelGGL(tile_comp_kernel, dim3((unsigned)(npad / 128)), dim3(128), 0, st, n, comp_p, tile_comp);
+    return hipGetLastError();
+}
 
 hipError_t launch_knn_rescore(const KnnRescoreArgs &a, hipStream_t st) {
     if (a.k + 1 > 65) return hipErrorInvalidValue;

@@ -68,6 +68,7 @@ typedef struct {
     int64_t ball_misses;      /* learned: rows queried again without the ball  */
     double t_learn_ms;        /* learned: features, sample kNN, training       */
     double t_predict_ms;      /* learned: inference + radii                    */
+    int64_t scored_tile_pairs; /* high_dim: 128 x 128 tile pairs the Boruvka scans scored, summed */
 } fh_stats;
 
 /* Create a context on HIP device `device`.  Returns NULL on failure (no
