@@ -396,6 +396,13 @@ def side_forms(Xd, ms):
         t_mst += time.perf_counter() - a
         n_layers.append(len(layers_from_mst(e, c, Xs.shape[0], base_min_cluster_size=5)[0]))
     total = time.perf_counter() - t0
+    from fast_hdbscan_amd.layers import mst_sweep_device
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mst_sweep_device(Xs, list(range(2, 65)))
+    torch.cuda.synchronize()
+    sweep_mst_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     sw = plscan_sweep(Xs, range(2, 65), base_min_cluster_size=5, workers=8)
     total_p = time.perf_counter() - t0
@@ -403,7 +410,8 @@ def side_forms(Xd, ms):
     out["plscan_sweep"] = {"workload": "500k x 2-D, 50 anisotropic Gaussians + 10% noise, seed 4; min_samples 2..64, "
                                        "base_min_cluster_size 5",
                            "serial_total_s": total, "serial_mst_and_copy_s": t_mst,
-                           "serial_layer_stage_s": total - t_mst, "pipelined_total_s": total_p,
+                           "serial_layer_stage_s": total - t_mst,
+                           "shared_knn_all_63_msts_s": sweep_mst_s, "pipelined_total_s": total_p,
                            "pipelined_workers": 8, "pipelined_same_layer_counts": same,
                            "layers_per_min_samples": n_layers}
 

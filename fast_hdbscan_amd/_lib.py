@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "fh_mst_euclidean_device",
     "fh_mst_euclidean",
     "fh_mst_euclidean_device_noalloc",
+    "fh_mst_sweep_device",
     "fh_knn",
     "fh_tree_nnodes",
     "fh_build_kdtree",
@@ -186,6 +187,8 @@ def load() -> ctypes.CDLL:
         L.fh_mst_euclidean_device_noalloc.restype = ctypes.c_int
         L.fh_mst_euclidean_device_noalloc.argtypes = [_i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp,
                                                       ctypes.POINTER(FHStats), ctypes.c_char_p, _i32]
+        L.fh_mst_sweep_device.restype = ctypes.c_int
+        L.fh_mst_sweep_device.argtypes = [_vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(FHStats)]
         L.fh_knn.restype = ctypes.c_int
         L.fh_knn.argtypes = [_vp, _vp, _i64, _i32, _vp, _i64, _i32, _i32, _vp, _vp]
         L.fh_tree_nnodes.restype = _i64

@@ -76,7 +76,8 @@ struct fh_ctx {
     DevBuf<int> taxis, tcnt, gtpos, ltpos, tactive, taux;
     DevBuf<fh::SegState> tseg;
     // pruned nearest-edge search
-    DevBuf<int> nb_pos, cert_of_pos;
+    DevBuf<int> nb_pos, cert_of_pos, sw_pos, sw_ties;
+    DevBuf<float> sw_d;
     DevBuf<float> nb_d, mincore;
     DevBuf<unsigned char> cert, seed_int;
     int search_mode = 1;  // 0: one plain pass; 1: seeded + sampled + certified
@@ -132,7 +133,8 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->offs, c->parent, c->roots, c->roots2, c->rootslot, c->partner, c->ei, c->ej,
         c->tiles, c->small, c->heap_i, c->flag, c->out_src, c->out_dst, c->out_nbr, c->range,
         c->best, c->xdev, c->wdev, c->tkeys, c->obox, c->taxis, c->tcnt, c->gtpos,
-        c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->nb_d, c->mincore,
+        c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->sw_pos, c->sw_ties, c->sw_d, c->nb_d,
+        c->mincore,
         c->cert, c->seed_int, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
         c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
         c->ml_v, c->ml_mu, c->ml_sd, c->ml_scratch, c->ml_tgt, c->ml_pred, c->ml_bound, c->ml_stats,
@@ -610,8 +612,16 @@ int learned_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int *nb_pos, 
 // The MST proper.  Xd: the points in HBM (original order).  weights_d:
 // per-point sample weights in HBM or nullptr; kq_w: the kNN width for the
 // weighted branch.  Outputs are device pointers.
+// Shared kNN rows of a min_samples sweep: tree-order columns 1..K of the
+// (K+1)-nearest query of the context's current tree.
+struct SweepRows {
+    const int *pos;
+    const float *dist;
+    int K;
+};
+
 int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float *weights_d, int kq_w, float *core_out,
-            int *nbr_out, int *src_out, int *dst_out, float *w_out, fh_stats *stats) {
+            int *nbr_out, int *src_out, int *dst_out, float *w_out, fh_stats *stats, const SweepRows *sweep = nullptr) {
     Timer total;
     hipStream_t st = ctx->stream;
     fh_stats S{};
@@ -619,7 +629,14 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
 
     int levels = 0;
     int64_t n_nodes = 0;
-    FH_TRY(build_tree_device(ctx, Xd, n, d, &levels, &n_nodes));
+    if (sweep) {
+        // the tree of the sweep's points is already built
+        if (ctx->tree_n != n) return fail(ctx, FH_EINVAL, "sweep rows belong to another tree");
+        levels = ctx->geom_levels;
+        n_nodes = (int64_t(1) << levels) - 1;
+    } else {
+        FH_TRY(build_tree_device(ctx, Xd, n, d, &levels, &n_nodes));
+    }
     FH_HIP(mark(ctx, 1));
     S.n_levels = levels;
     S.n_nodes = n_nodes;
@@ -671,7 +688,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     const float margin_c = 8.0f * dp + 64.0f;
     fh::BruteScanArgs bsa{};
     bool knn_d_rows = true;  // the kNN distance rows were written
-    const bool learned = ctx->learned_mode != 0 && !weighted && ms > 1 && d <= 8 && kq <= 128 && n >= 1000;
+    const bool learned = !sweep && ctx->learned_mode != 0 && !weighted && ms > 1 && d <= 8 && kq <= 128 && n >= 1000;
     // Tree-order kNN rows (the self-query writes each tree position's
     // columns 1..k-1 as positions and its core by position, all coalesced
     // across a wavefront): the forest and the outputs are then taken from
@@ -683,7 +700,23 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
         FH_TRY(ensure(ctx, ctx->knn_i, (size_t)n * kq));
         FH_TRY(ensure(ctx, ctx->core_o, n));
     }
-    if (tree_rows) {
+    if (tree_rows && sweep) {
+        // this min_samples' columns of the shared rows; rows whose ms+2
+        // smallest values hold an equal pair get the exact (ms+1)-query
+        FH_TRY(ensure(ctx, ctx->nb_pos, (size_t)n * (kq - 1)));
+        FH_TRY(ensure(ctx, ctx->nb_d, (size_t)n * (kq - 1)));
+        FH_TRY(ensure(ctx, ctx->sw_ties, n));
+        FH_HIP(hipMemsetAsync(ctx->small.p + 2, 0, sizeof(int), st));
+        FH_HIP(fh::launch_sweep_slice(n, sweep->K, ms, sweep->pos, sweep->dist, ctx->nb_pos.p, ctx->nb_d.p,
+                                      ctx->core_p.p, ctx->sw_ties.p, ctx->small.p + 2, st));
+        FH_HIP(hipMemcpyAsync(ctx->h_small + 2, ctx->small.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+        FH_HIP(hipStreamSynchronize(st));
+        const int ties = ctx->h_small[2];
+        S.knn_tree_rows = ties;
+        if (ties > 0)
+            FH_TRY(run_knn(ctx, d, levels, nullptr, nullptr, ties, kq, nullptr, nullptr, nullptr, ctx->core_p.p, 1,
+                           ctx->nb_pos.p, ctx->nb_d.p, 0, ctx->sw_ties.p));
+    } else if (tree_rows) {
         FH_TRY(ensure(ctx, ctx->nb_pos, (size_t)n * (kq - 1)));
         FH_TRY(ensure(ctx, ctx->nb_d, (size_t)n * (kq - 1)));
         FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, kq, nullptr, nullptr, nullptr, ctx->core_p.p,
@@ -1074,6 +1107,42 @@ int fh_mst_euclidean_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, i
         kq_w = static_cast<int>(k);
     }
     return run_mst(ctx, X, n, d, min_samples, sample_weights, kq_w, core_sq, nbr, src, dst, w_sq, stats);
+}
+
+int fh_mst_sweep_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t count,
+                        const int32_t *min_samples, float *core_sq, int32_t *src, int32_t *dst, float *w_sq,
+                        fh_stats *stats) {
+    FH_TRY(check_args(ctx, X, n, d, 2));
+    if (count < 1 || !min_samples) return fail(ctx, FH_EINVAL, "empty min_samples list");
+    if (n > 1 && (!src || !dst || !w_sq)) return fail(ctx, FH_EINVAL, "edge outputs are null");
+    int mmax = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        if (min_samples[i] < 2 || min_samples[i] > 126)
+            return fail(ctx, FH_EUNSUPPORTED, "sweep min_samples outside [2, 126]");
+        mmax = std::max(mmax, (int)min_samples[i]);
+    }
+    if (use_brute(ctx, n, d, mmax + 2, false)) return fail(ctx, FH_EUNSUPPORTED, "sweep of the high-dimensional path");
+    FH_HIP(hipSetDevice(ctx->device));
+    Timer total;
+    int levels = 0;
+    FH_TRY(build_tree_device(ctx, X, n, d, &levels, nullptr));
+    // columns 1..K of the (K+1)-nearest query, K = largest min_samples + 1
+    SweepRows sw{nullptr, nullptr, mmax + 1};
+    FH_TRY(ensure(ctx, ctx->sw_pos, (size_t)n * sw.K));
+    FH_TRY(ensure(ctx, ctx->sw_d, (size_t)n * sw.K));
+    FH_TRY(ensure(ctx, ctx->core_p, n));
+    FH_TRY(run_knn(ctx, d, levels, ctx->xp.p, ctx->perm.p, n, sw.K + 1, nullptr, nullptr, nullptr, ctx->core_p.p, 1,
+                   ctx->sw_pos.p, ctx->sw_d.p, 0));
+    sw.pos = ctx->sw_pos.p;
+    sw.dist = ctx->sw_d.p;
+    fh_stats S{};
+    const int64_t m = n > 1 ? n - 1 : 0;
+    for (int32_t i = 0; i < count; ++i)
+        FH_TRY(run_mst(ctx, X, n, d, min_samples[i], nullptr, 0, core_sq ? core_sq + (size_t)i * n : nullptr, nullptr,
+                       src + (size_t)i * m, dst + (size_t)i * m, w_sq + (size_t)i * m, &S, &sw));
+    S.t_total_ms = total.ms();
+    if (stats) *stats = S;
+    return FH_OK;
 }
 
 int fh_mst_euclidean_device_noalloc(int32_t device, void *stream, const float *X, int64_t n, int32_t d,

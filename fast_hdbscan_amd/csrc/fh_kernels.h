@@ -83,6 +83,8 @@ struct CycleBufs {
 hipError_t launch_knn(const KnnArgs &a, hipStream_t st);
 hipError_t launch_init_edges_pos(int n, int k, const int *nb_pos, const float *nb_d, const float *core_p,
                                  const int *perm, const int *posof, int *succ, int *dst_raw, float *w, hipStream_t st);
+hipError_t launch_sweep_slice(int64_t n, int K, int ms, const int *sw_pos, const float *sw_d, int *nb_pos,
+                              float *nb_d, float *core_p, int *tie_list, int *tie_count, hipStream_t st);
 hipError_t launch_outputs_by_point(int64_t n, int k, const int *posof, const int *perm, const float *core_p,
                                    const int *nb_pos, float *core_out, int *nbr, hipStream_t st);
 hipError_t launch_cert(const CertArgs &a, hipStream_t st);

@@ -720,6 +720,30 @@ __global__ void init_edges_pos_kernel(int n, int k, const int *__restrict__ nb_p
     w[i] = wi;
 }
 
+// One min_samples value of a sweep from shared tree-order kNN rows of width
+// K (columns 1..K of a (K+1)-nearest query): the first ms columns and the
+// core (column ms).  The (ms+1)-nearest query's output equals these columns
+// whenever its ms+2 smallest values are distinct; positions with an equal
+// pair among them (a zero first column means a duplicate point, equal to the
+// query's own zero) are listed for the exact (ms+1)-nearest query, whose
+// heap order decides ties.
+__global__ void sweep_slice_kernel(int64_t n, int K, int ms, const int *__restrict__ sw_pos,
+                                   const float *__restrict__ sw_d, int *nb_pos, float *nb_d, float *core_p,
+                                   int *tie_list, int *tie_count) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int *rp = sw_pos + p * K;
+    const float *rd = sw_d + p * K;
+    bool tie = !(rd[0] > 0.0f);
+    for (int c = 0; c < ms; ++c) {
+        nb_pos[p * ms + c] = rp[c];
+        nb_d[p * ms + c] = rd[c];
+        tie |= rd[c] == rd[c + 1];
+    }
+    core_p[p] = rd[ms - 1];
+    if (tie) tie_list[atomicAdd(tie_count, 1)] = (int)p;
+}
+
 // Outputs by point index from tree-order rows: the core, and the neighbour
 // columns as point indices (-1 kept).
 __global__ void core_by_point_kernel(int n, const int *__restrict__ posof, const float *__restrict__ core_p,
@@ -1241,6 +1265,13 @@ hipError_t launch_init_edges_pos(int n, int k, const int *nb_pos, const float *n
                                  const int *perm, const int *posof, int *succ, int *dst_raw, float *w, hipStream_t st) {
     hipLaunchKernelGGL(init_edges_pos_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, k, nb_pos, nb_d, core_p, perm,
                        posof, succ, dst_raw, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_slice(int64_t n, int K, int ms, const int *sw_pos, const float *sw_d, int *nb_pos,
+                              float *nb_d, float *core_p, int *tie_list, int *tie_count, hipStream_t st) {
+    hipLaunchKernelGGL(sweep_slice_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, K, ms, sw_pos, sw_d, nb_pos,
+                       nb_d, core_p, tie_list, tie_count);
     return hipGetLastError();
 }
 

@@ -99,10 +99,16 @@ def plscan_sweep(data, min_samples_values, base_min_cluster_size=10, base_n_clus
     n = int(Xd.shape[0])
     values = [int(m) for m in min_samples_values]
     out = {}
+    shared = n > 1 and all(2 <= m <= 126 for m in values)
+    if shared:
+        msts = mst_sweep_device(Xd, values)
     with cf.ThreadPoolExecutor(max_workers=max(1, int(workers))) as pool:
         futures = {}
-        for m in values:
-            src, dst, w_sq, core_sq, _ = mst_device(Xd, m)
+        for i, m in enumerate(values):
+            if shared:
+                src, dst, w_sq, core_sq = (t[i] for t in msts)
+            else:
+                src, dst, w_sq, core_sq, _ = mst_device(Xd, m)
             edges = _edges_f64(src.cpu().numpy(), dst.cpu().numpy(), w_sq.cpu().numpy())
             core = np.sqrt(core_sq.cpu().numpy())
             futures[m] = pool.submit(layers_from_mst, edges, core, n, base_min_cluster_size, base_n_clusters,
@@ -110,6 +116,45 @@ def plscan_sweep(data, min_samples_values, base_min_cluster_size=10, base_n_clus
         for m in values:
             out[m] = futures[m].result()
     return out
+
+
+def mst_sweep_device(X, min_samples_values, stats: dict | None = None):
+    """MSTs of one device-resident float32 tensor for several min_samples
+    values (each in [2, 126]) from one KD-tree and one shared kNN query
+    (fh_mst_sweep_device); equal, value by value, to mst_device.  Returns
+    torch tensors on X's device: src, dst [count, n-1] int32, w_sq
+    [count, n-1] float32, core_sq [count, n] float32."""
+    import ctypes
+
+    import torch
+
+    from . import _lib
+
+    if not (isinstance(X, torch.Tensor) and X.is_cuda) or X.dtype != torch.float32 or X.dim() != 2:
+        raise TypeError("mst_sweep_device expects a 2-D float32 tensor on a HIP device")
+    X = X.contiguous()
+    n, d = X.shape
+    values = np.ascontiguousarray([int(m) for m in min_samples_values], dtype=np.int32)
+    c = values.shape[0]
+    m = max(n - 1, 0)
+    dev = X.device
+    src = torch.empty((c, max(m, 1)), dtype=torch.int32, device=dev)
+    dst = torch.empty((c, max(m, 1)), dtype=torch.int32, device=dev)
+    w = torch.empty((c, max(m, 1)), dtype=torch.float32, device=dev)
+    core = torch.empty((c, n), dtype=torch.float32, device=dev)
+    torch.cuda.current_stream(dev).synchronize()
+    st = _lib.FHStats()
+    ctx = _lib.context(dev.index or 0)
+    code = _lib.load().fh_mst_sweep_device(ctx, ctypes.c_void_p(X.data_ptr()), n, d, c, _lib.ptr(values),
+                                           ctypes.c_void_p(core.data_ptr()), ctypes.c_void_p(src.data_ptr()),
+                                           ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(w.data_ptr()),
+                                           ctypes.byref(st))
+    _lib.check(ctx, code)
+    if stats is not None:
+        stats.update(st.as_dict())
+    if m == 0:
+        src, dst, w = src[:, :0], dst[:, :0], w[:, :0]
+    return src, dst, w, core
 
 
 def build_cluster_layers(

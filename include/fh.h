@@ -124,6 +124,26 @@ int fh_mst_euclidean(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t 
                      float *w_sq, fh_stats *stats);
 
 /*
+ * Mutual-reachability MSTs of one dataset for several min_samples values
+ * (a PLSCAN min_samples sweep), inputs resident in HBM.
+ *   Replaces: for each value, hdbscan.py:466 build_kdtree(data) +
+ *             parallel_boruvka(tree, n_threads, min_samples)
+ *             (boruvka.py:552-647; layer_clusters.py:381-392 calls it once
+ *             per PLSCAN fit): the same outputs as count calls of
+ *             fh_mst_euclidean_device, computed from one tree and one kNN
+ *             query of width max(min_samples) + 2 (rows whose
+ *             min_samples + 2 smallest distances tie are queried again at
+ *             their own width, whose heap order decides ties).
+ * min_samples [count] HOST values in [2, 126]; DEVICE outputs laid out per
+ * value: core_sq [count * n] (may be NULL), src, dst, w_sq [count * (n-1)].
+ * Low-dimensional (KD-tree) path only, no sample weights: else
+ * FH_EUNSUPPORTED.  stats: the last value's, t_total_ms for the call.
+ */
+int fh_mst_sweep_device(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_t count,
+                        const int32_t *min_samples, float *core_sq, int32_t *src, int32_t *dst, float *w_sq,
+                        fh_stats *stats);
+
+/*
  * The same MST without any allocation, for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.

@@ -57,3 +57,43 @@ def test_plscan_sweep_matches_one_at_a_time():
         assert [float(v) for v in one[2]] == [float(v) for v in got[2]]
         np.testing.assert_array_equal(one[3], got[3])
         np.testing.assert_array_equal(one[4], got[4])
+
+
+@pytest.mark.parametrize("case", ["mixture", "grid_ties", "duplicates"])
+def test_sweep_shared_knn_equals_single_mst(case):
+    """fh_mst_sweep_device (one tree, one kNN of width max(min_samples)+2)
+    gives, value by value, the single-MST outputs; ties in a row's shared
+    columns send it back to the query at its own width (integer grids and
+    duplicated points make many); two values against the oracle."""
+    import torch
+
+    from fast_hdbscan_amd.hdbscan import mst_device
+    from fast_hdbscan_amd.layers import mst_sweep_device
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(7)
+    if case == "mixture":
+        X = workloads.anisotropic_mixture(40_000, 2, 15, 0.1, seed=8)
+    elif case == "grid_ties":
+        g = np.stack(np.meshgrid(np.arange(150), np.arange(120)), -1).reshape(-1, 2).astype(np.float32)
+        X = g[rng.permutation(g.shape[0])]
+    else:
+        X = workloads.anisotropic_mixture(20_000, 3, 10, 0.05, seed=9)
+        X[10_000:12_000] = X[:2000]
+    values = [2, 3, 5, 10, 17, 33, 64]
+    print(f"sweep case {case} n={X.shape[0]} d={X.shape[1]}", flush=True)
+    Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
+    st = {}
+    src, dst, w, core = mst_sweep_device(Xd, values, stats=st)
+    for i, m in enumerate(values):
+        s1, d1, w1, c1, _ = mst_device(Xd, m)
+        np.testing.assert_array_equal(src[i].cpu().numpy(), s1.cpu().numpy())
+        np.testing.assert_array_equal(dst[i].cpu().numpy(), d1.cpu().numpy())
+        np.testing.assert_array_equal(w[i].cpu().numpy(), w1.cpu().numpy())
+        np.testing.assert_array_equal(core[i].cpu().numpy(), c1.cpu().numpy())
+    tree = O.build_kdtree(np.ascontiguousarray(X, dtype=np.float32))
+    for i in (1, 4):
+        ref = O.boruvka(tree, values[i])
+        np.testing.assert_array_equal(src[i].cpu().numpy(), ref.src)
+        np.testing.assert_array_equal(w[i].cpu().numpy(), ref.w_sq)
+        np.testing.assert_array_equal(core[i].cpu().numpy(), ref.core_sq)
