@@ -816,11 +816,16 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     struct {
         int seed_rounds = 1 << 30;
         bool sampled = false, cert = true;
+        int small_round = fh::small_round_limit();
     } tune;
     tune.sampled = !qa.wave;
     if (const char *m = std::getenv("FH_SEED_ROUNDS")) tune.seed_rounds = std::atoi(m);
     if (const char *m = std::getenv("FH_SAMPLED")) tune.sampled = std::atoi(m) != 0;
     if (const char *m = std::getenv("FH_CERT")) tune.cert = std::atoi(m) != 0;
+    // FH_SMALL_ROUND caps the component count below which a round's
+    // selection, cycle skip, emission and unions run in one work-group
+    if (const char *m = std::getenv("FH_SMALL_ROUND"))
+        tune.small_round = std::min(std::atoi(m), fh::small_round_limit());
     if (seed) {
         FH_TRY(ensure(ctx, ctx->seed_int, n));
         FH_HIP(hipMemsetAsync(ctx->seed_int.p, 0, (size_t)n, st));
@@ -886,16 +891,38 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
             FH_HIP(mark(ctx, 9));
         }
         FH_HIP(mark(ctx, 6));
-        FH_HIP(fh::launch_select(C, ni, ctx->roots.p, ctx->best.p, ctx->partner.p, ctx->posof.p, ctx->perm.p,
-                                 ctx->comp_p.p, ctx->rootslot.p, ctx->succ.p, ctx->ei.p, ctx->ej.p, ctx->ew.p, st));
-        FH_HIP(fh::launch_cycle_keep(C, ctx->succ.p, cb, ctx->keep.p, st));
-        FH_HIP(fh::launch_excl_scan(C, ctx->keep.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 0, st));
-        FH_HIP(fh::launch_round_emit_union(C, base, ctx->keep.p, ctx->offs.p, ctx->ei.p, ctx->ej.p, ctx->ew.p,
-                                           ctx->roots.p, ctx->succ.p, ctx->parent.p, src_out, dst_out, w_out, st));
-        FH_HIP(fh::launch_relabel_and_flags(ni, C, ctx->roots.p, ctx->parent.p, ctx->comp_p.p, ctx->flag.p, st));
-        FH_HIP(fh::launch_excl_scan(C, ctx->flag.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 1, st));
-        FH_HIP(fh::launch_roots_compact(C, ctx->roots.p, ctx->flag.p, ctx->offs.p, ctx->roots2.p, ctx->rootslot.p,
-                                        st));
+        if (C <= tune.small_round) {
+            fh::SmallRoundArgs sr{};
+            sr.c = C;
+            sr.n = ni;
+            sr.base = (int)base;
+            sr.roots = ctx->roots.p;
+            sr.best = ctx->best.p;
+            sr.partner = ctx->partner.p;
+            sr.posof = ctx->posof.p;
+            sr.perm = ctx->perm.p;
+            sr.comp_p = ctx->comp_p.p;
+            sr.rootslot = ctx->rootslot.p;
+            sr.parent = ctx->parent.p;
+            sr.roots_out = ctx->roots2.p;
+            sr.src_out = src_out;
+            sr.dst_out = dst_out;
+            sr.w_out = w_out;
+            sr.small = ctx->small.p;
+            FH_HIP(fh::launch_round_small(sr, st));
+            FH_HIP(fh::launch_relabel(ni, ctx->parent.p, ctx->comp_p.p, st));
+        } else {
+            FH_HIP(fh::launch_select(C, ni, ctx->roots.p, ctx->best.p, ctx->partner.p, ctx->posof.p, ctx->perm.p,
+                                     ctx->comp_p.p, ctx->rootslot.p, ctx->succ.p, ctx->ei.p, ctx->ej.p, ctx->ew.p, st));
+            FH_HIP(fh::launch_cycle_keep(C, ctx->succ.p, cb, ctx->keep.p, st));
+            FH_HIP(fh::launch_excl_scan(C, ctx->keep.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 0, st));
+            FH_HIP(fh::launch_round_emit_union(C, base, ctx->keep.p, ctx->offs.p, ctx->ei.p, ctx->ej.p, ctx->ew.p,
+                                               ctx->roots.p, ctx->succ.p, ctx->parent.p, src_out, dst_out, w_out, st));
+            FH_HIP(fh::launch_relabel_and_flags(ni, C, ctx->roots.p, ctx->parent.p, ctx->comp_p.p, ctx->flag.p, st));
+            FH_HIP(fh::launch_excl_scan(C, ctx->flag.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 1, st));
+            FH_HIP(fh::launch_roots_compact(C, ctx->roots.p, ctx->flag.p, ctx->offs.p, ctx->roots2.p, ctx->rootslot.p,
+                                            st));
+        }
         FH_TRY(read_small(ctx, 2));
         if (brute) {
             FH_HIP(hipMemcpyAsync(ctx->h_small + 4, be.row_count, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -76,6 +76,23 @@ struct CertArgs {
     int d;
 };
 
+// The post-search work of a round with few components, in one work-group.
+struct SmallRoundArgs {
+    int c;                               // components (<= small_round_limit())
+    int n;
+    int base;                            // first output row of the round
+    const int *roots;                    // [c] component label per slot
+    const unsigned long long *best;      // packed best edge per label
+    const int *partner;                  // per position
+    const int *posof, *perm, *comp_p;
+    int *rootslot;                       // label -> slot (read, then rewritten for the next round)
+    int *parent;                         // union-find over labels
+    int *roots_out;                      // [c] surviving labels, compacted
+    int *src_out, *dst_out;
+    float *w_out;
+    int *small;                          // [0] rows emitted, [1] components left
+};
+
 struct CycleBufs {
     int *S0, *M0, *S1, *M1;
 };
@@ -123,6 +140,9 @@ hipError_t launch_round_emit_union(int c, int base, const int *keep, const int *
                                    int *dst_out, float *w_out, hipStream_t st);
 hipError_t launch_relabel_and_flags(int n, int c, const int *roots, const int *parent, int *comp_p, int *flag,
                                     hipStream_t st);
+int small_round_limit();
+hipError_t launch_round_small(const SmallRoundArgs &a, hipStream_t st);
+hipError_t launch_relabel(int n, const int *parent, int *comp_p, hipStream_t st);
 hipError_t launch_roots_compact(int c, const int *roots, const int *flag, const int *offs, int *roots_out,
                                 int *rootslot, hipStream_t st);
 

@@ -1098,6 +1098,116 @@ __global__ void roots_compact_kernel(int c, const int *__restrict__ roots, const
     rootslot[r] = offs[s];
 }
 
+// A round's work after the search when at most kSmallRound components are
+// left, in one work-group (one thread per component slot s): the selection
+// of select_kernel, the cycle skip by pointer doubling in LDS (dbl_*), the
+// emission in slot order (round_emit_kernel), the unions and root
+// compression (round_union_kernel, compress_roots_kernel), the surviving
+// roots and their compaction (still_root_kernel, roots_compact_kernel):
+// the same arithmetic as those kernels, with work-group barriers in place
+// of kernel boundaries.  small[0] = rows emitted, small[1] = components
+// left.  The relabelling of the points stays a grid-wide kernel.
+constexpr int kSmallRound = 1024;
+
+__device__ __forceinline__ int block_excl_scan_1024(int v, int &total, int *wsum) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int inc = wave_incl_scan(v);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kSmallRound / 64; ++w) {
+        const int x = wsum[w];
+        off += (w < wave) ? x : 0;
+        tot += x;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(kSmallRound) void round_small_kernel(SmallRoundArgs a) {
+    __shared__ int S[2][kSmallRound], M[2][kSmallRound];
+    __shared__ int wsum[kSmallRound / 64];
+    const int s = threadIdx.x;
+    const int c = a.c;
+    const bool on = s < c;
+    // selection (select_kernel)
+    int i_star = -1, j_star = -1, succ = -1, r = -1;
+    float w = kInf;
+    if (on) {
+        r = a.roots[s];
+        const unsigned long long key = a.best[r];
+        int tgt;
+        if (key == ~0ull) {
+            i_star = r;
+            tgt = a.comp_p[a.posof[a.n - 1]];
+        } else {
+            i_star = (int)(unsigned int)(key & 0xffffffffull);
+            w = __uint_as_float((unsigned int)(key >> 32));
+            const int pj = a.partner[a.posof[i_star]];
+            j_star = a.perm[pj];
+            tgt = a.comp_p[pj];
+        }
+        succ = a.rootslot[tgt];
+    }
+    // cycle skip (dbl_init, dbl_step, dbl_keep)
+    S[0][s] = succ;
+    M[0][s] = s;
+    __syncthreads();
+    int cur = 0;
+    for (int span = 1; span < c; span <<= 1) {
+        if (on) {
+            const int sv = S[cur][s], mv = M[cur][s];
+            if (sv < 0) {
+                S[cur ^ 1][s] = -1;
+                M[cur ^ 1][s] = mv;
+            } else {
+                S[cur ^ 1][s] = S[cur][sv];
+                const int ms = M[cur][sv];
+                M[cur ^ 1][s] = mv > ms ? mv : ms;
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    int keep = 0;
+    if (on) {
+        const int sv = S[cur][s];
+        const bool skip = sv >= 0 && M[cur][sv] == s;
+        keep = (succ >= 0 && !skip) ? 1 : 0;
+    }
+    // emission in slot order (round_emit_kernel)
+    int emitted;
+    const int off = block_excl_scan_1024(keep, emitted, wsum);
+    if (keep) {
+        const int row = a.base + off;
+        a.src_out[row] = i_star;
+        a.dst_out[row] = j_star;
+        a.w_out[row] = w;
+        // S[cur] is no longer read: the slots' roots for the unions
+    }
+    __syncthreads();
+    if (on) S[cur][s] = r;
+    __syncthreads();
+    if (keep) uf_unite(a.parent, r, S[cur][succ]);
+    __syncthreads();
+    if (on) a.parent[r] = uf_find(a.parent, r);
+    __syncthreads();
+    // surviving roots, compacted (still_root_kernel, roots_compact_kernel)
+    const int flag = (on && ld_relaxed(a.parent + r) == r) ? 1 : 0;
+    int left;
+    const int off2 = block_excl_scan_1024(flag, left, wsum);
+    if (flag) {
+        a.roots_out[off2] = r;
+        a.rootslot[r] = off2;
+    }
+    if (s == 0) {
+        a.small[0] = emitted;
+        a.small[1] = left;
+    }
+}
+
 __global__ void posof_kernel(int n, const int *__restrict__ perm, int *posof) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -1358,6 +1468,19 @@ hipError_t launch_relabel_and_flags(int n, int c, const int *roots, const int *p
                                     hipStream_t st) {
     hipLaunchKernelGGL(relabel_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, parent, comp_p);
     hipLaunchKernelGGL(still_root_kernel, dim3(blocks(c, 256)), dim3(256), 0, st, c, roots, parent, flag);
+    return hipGetLastError();
+}
+
+int small_round_limit() { return kSmallRound; }
+
+hipError_t launch_round_small(const SmallRoundArgs &a, hipStream_t st) {
+    if (a.c < 1 || a.c > kSmallRound) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(round_small_kernel, dim3(1), dim3(kSmallRound), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_relabel(int n, const int *parent, int *comp_p, hipStream_t st) {
+    hipLaunchKernelGGL(relabel_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, parent, comp_p);
     return hipGetLastError();
 }
 

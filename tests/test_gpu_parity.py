@@ -291,14 +291,16 @@ def test_sample_weight_seam_contract():
     assert e.dtype == np.float64 and nb.dtype == np.int32 and core.dtype == np.float32
 
 
-@pytest.mark.parametrize("form", ["lane", "unpruned", "knn_point_rows", "sampled"])
+@pytest.mark.parametrize("form", ["lane", "unpruned", "knn_point_rows", "sampled", "grid_rounds"])
 def test_search_kernel_forms(form, monkeypatch):
     """The main search exists as one lane per point (FH_QUERY_KERNEL=lane)
     and as one wavefront per 64 positions (default for d <= 8), and the
     unpruned single pass (FH_SEARCH_MODE=0, read when a context is made:
     run in a thread of its own); the kNN rows kept in point order
     (FH_KNN_ROWS=point) or in tree order (default); the sampled pre-pass on
-    or off (FH_SAMPLED).  Each equals the oracle."""
+    or off (FH_SAMPLED); the post-search work of every round as grid-wide
+    kernels (FH_SMALL_ROUND=0) instead of one work-group once at most 1024
+    components are left.  Each equals the oracle."""
     import threading
 
     cases = [workloads.scale_2d(120_000, seed=41), workloads.anisotropic_mixture(60_000, 3, 20, 0.05, seed=42)]
@@ -310,6 +312,8 @@ def test_search_kernel_forms(form, monkeypatch):
         monkeypatch.setenv("FH_KNN_ROWS", "point")
     elif form == "sampled":
         monkeypatch.setenv("FH_SAMPLED", "1")
+    elif form == "grid_rounds":
+        monkeypatch.setenv("FH_SMALL_ROUND", "0")
     else:
         monkeypatch.setenv("FH_SEARCH_MODE", "0")
     errors = []
