@@ -285,8 +285,19 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
     // to come down to the LDS range, so the first batch covers most levels
     // in one check; a step whose nodes are all finished launches no-ops
     constexpr int kFirstBatch = 12, kBatch = 4;
-    FH_HIP(fh::tree_identity_keys(ni, a.A, st));
-    for (int l = 0; l < L; ++l) {
+    // the first level whose nodes fit one work-group's LDS: from there on
+    // one launch builds every level (FH_TREE_SUBTREE=0 keeps the
+    // level-by-level kernels down to the leaves)
+    int l_sub = L;
+    const char *sub_env = std::getenv("FH_TREE_SUBTREE");
+    const int sub_cap = (sub_env && std::atoi(sub_env) == 0) ? 0 : fh::tree_subtree_cap(d);
+    for (int l = 0; sub_cap > 0 && l < L; ++l)
+        if (ctx->geom_maxsize[l] <= sub_cap) {
+            l_sub = l;
+            break;
+        }
+    if (l_sub > 0) FH_HIP(fh::tree_identity_keys(ni, a.A, st));
+    for (int l = 0; l < l_sub; ++l) {
         const int count = 1 << l;
         const int maxs = ctx->geom_maxsize[l];
         const bool internal = l < L - 1;
@@ -324,7 +335,10 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
         }
         FH_HIP(fh::tree_small_select(a, count, maxs, st));
     }
-    FH_HIP(fh::tree_finalize(a, ni, ctx->perm.p, ctx->posof.p, ctx->xp.p, st));
+    if (l_sub < L)
+        FH_HIP(fh::tree_subtrees(a, l_sub, L, ctx->perm.p, ctx->posof.p, ctx->xp.p, st));
+    else
+        FH_HIP(fh::tree_finalize(a, ni, ctx->perm.p, ctx->posof.p, ctx->xp.p, st));
     ctx->tree_n = n;
     if (levels_out) *levels_out = L;
     if (nodes_out) *nodes_out = N;

@@ -54,6 +54,15 @@ hipError_t tree_medium_select(const TreeBuildArgs &a, int count, hipStream_t st)
 // max_size: the level's largest node (the segments left are no longer)
 hipError_t tree_small_select(const TreeBuildArgs &a, int count, int max_size, hipStream_t st);
 hipError_t tree_sort_range(uint64_t *A, int L, int R, hipStream_t st);
+// Largest subtree root (points) that one work-group builds in LDS for d
+// dimensions; 0 when d has no such kernel.
+int tree_subtree_cap(int d);
+// Every level from level0 to levels - 1 of the 2^level0 subtrees rooted at
+// level0 (each at most tree_subtree_cap(d) points), one work-group each,
+// then the tree-order outputs (as tree_finalize); a.A holds the keys left
+// by the selects of level0 - 1 (not read when level0 == 0).
+hipError_t tree_subtrees(const TreeBuildArgs &a, int level0, int levels, int *perm, int *posof, float *xp,
+                         hipStream_t st);
 hipError_t tree_finalize(const TreeBuildArgs &a, int n, int *perm, int *posof, float *xp, hipStream_t st);
 
 }  // namespace fh

@@ -66,6 +66,11 @@ def _tree_case(name):
     if name == "organ_pipe":
         v = np.concatenate([np.arange(40_000), np.arange(40_000)[::-1]]).astype(np.float32)
         return np.stack([v, np.zeros_like(v)], 1)
+    if name.startswith("dim"):
+        # d-dimensional mixtures large enough that the subtree kernel starts
+        # below the root (its LDS capacity shrinks with d)
+        d = int(name[3:])
+        return workloads.anisotropic_mixture(60_000 if d > 1 else 200_000, d, 12, 0.05, seed=40 + d)
     if name == "constant_axis":
         X = rng.standard_normal((60_000, 2)).astype(np.float32)
         X[:, 1] = 3.0
@@ -74,8 +79,21 @@ def _tree_case(name):
 
 
 @pytest.mark.parametrize("name", ["blobs", "scale2d_2M", "scale3d_300k", "d8", "dups", "grid", "sorted",
-                                  "reversed", "organ_pipe", "constant_axis"])
+                                  "reversed", "organ_pipe", "constant_axis", "dim1", "dim4", "dim5", "dim6",
+                                  "dim7"])
 def test_device_tree_matches_oracle(name):
+    _tree_check(name)
+
+
+@pytest.mark.parametrize("name", ["blobs", "scale3d_300k", "d8", "dups", "grid", "organ_pipe"])
+def test_device_tree_levels_only(name, monkeypatch):
+    """The level-by-level kernels down to the leaves (FH_TREE_SUBTREE=0)
+    instead of one work-group per subtree once a node fits in LDS."""
+    monkeypatch.setenv("FH_TREE_SUBTREE", "0")
+    _tree_check(name)
+
+
+def _tree_check(name):
     from fast_hdbscan_amd.hdbscan import build_kdtree
     from oracle import oracle as O
 
