@@ -36,8 +36,8 @@ HBM_PEAK_GBS = 8000.0
 # copy) on this workload; command lines and raw counts in
 # profiles/pmc_2Mx2D_traffic.md.
 PMC_TRAFFIC_BYTES = {
-    "boruvka_wave_kernel<2, 2>": None,
-    "knn_kernel<2>": None,
+    "boruvka_wave_kernel<2, 2>": 85.12e6,
+    "knn_kernel<2, false>": 258.35e6,
 }
 
 
@@ -56,7 +56,12 @@ def _parse():
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 256 MiB device copy (calibrates HBM byte counters) and exit")
     p.add_argument("--no-forms", action="store_true",
-                   help="skip the learned / high-dimensional / 10M x 3-D / PLSCAN-sweep side measurements")
+                   help="skip the learned / high-dimensional / 10M x 3-D / PLSCAN-sweep / batch side measurements")
+    p.add_argument("--forms-only", default=None,
+                   help="comma-separated side measurements to run alone (" + ",".join(FORMS) +
+                        "), printing their JSON object instead of the bench line (for profiling one form)")
+    p.add_argument("--batch-one-call", type=int, default=None,
+                   help="with --forms-only batched: one call on the 512 datasets with this many workers")
     return p.parse_args()
 
 
@@ -76,6 +81,20 @@ def copy_probe():
 
 def main():
     args = _parse()
+    if args.forms_only:
+        names = [f for f in args.forms_only.split(",") if f]
+        bad = [f for f in names if f not in FORMS]
+        if bad:
+            raise SystemExit(f"unknown forms {bad}; choose from {list(FORMS)}")
+        import torch
+
+        from fast_hdbscan_amd import workloads
+
+        Xd = None
+        if "learned" in names:
+            Xd = torch.from_numpy(workloads.scale_2d(args.n, seed=1)).cuda()
+        print(json.dumps(side_forms(Xd, args.min_samples, names, args.batch_one_call)))
+        return
     if args.copy_probe:
         copy_probe()
         return
@@ -140,7 +159,7 @@ def main():
     knn_bytes = n * (4 * d) + n * (k - 1) * 8 + 4 * n
     q_bytes = n * (4 * d + 16)
     if knn_ms >= q_ms_total:
-        dom, dur_ms, alg = "knn_kernel<2>", knn_ms, knn_bytes
+        dom, dur_ms, alg = "knn_kernel<2, false>", knn_ms, knn_bytes
     else:
         dom, dur_ms, alg = "boruvka_wave_kernel<2, 2>", q_ms, q_bytes
     achieved = alg / (dur_ms * 1e-3) / 1e9
@@ -295,43 +314,42 @@ def pcie_inclusive(X, ms, mcs, work, steps):
     return {"value": X.shape[0] * steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps}
 
 
-def side_forms(Xd, ms):
-    """The other forms of the path, each run once after the timed steps
-    (not part of `value`): learned search radii on the same 2M x 2-D batch,
-    the MFMA high-dimensional path at 1M x 64, the 10M x 3-D mixture and a
-    PLSCAN min_samples sweep on 500k x 2-D and a batch of 512 small
-    independent datasets."""
-    import numpy as np
+FORMS = ("learned", "high_dim", "scale_3d", "plscan_sweep", "batched")
+
+
+def _mst_ms(X, m, reps=3, **kw):
+    """Best wall time (ms) of `reps` device MSTs, with that run's stats."""
     import torch
 
-    from fast_hdbscan_amd import _lib, workloads
     from fast_hdbscan_amd.hdbscan import mst_device
 
-    out = {}
+    best, st_best = None, None
+    for _ in range(reps):
+        st = {}
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        mst_device(X, m, stats=st, **kw)
+        torch.cuda.synchronize()
+        el = 1e3 * (time.perf_counter() - t)
+        if best is None or el < best:
+            best, st_best = el, st
+    return best, st_best
+
+
+def form_learned(Xd, ms):
+    """Learned search radii on the bench's own batch: plain, trained on
+    every call, trained once and reused."""
+    from fast_hdbscan_amd import _lib
+
     ctx = _lib.context(0)
     L = _lib.load()
-
-    def mst_ms(X, m, reps=3, **kw):
-        best, st_best = None, None
-        for _ in range(reps):
-            st = {}
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            mst_device(X, m, stats=st, **kw)
-            torch.cuda.synchronize()
-            el = 1e3 * (time.perf_counter() - t)
-            if best is None or el < best:
-                best, st_best = el, st
-        return best, st_best
-
-    # learned search radii: plain, train-every-call, train-once-reuse
-    plain_ms, pst = mst_ms(Xd, ms)
+    plain_ms, pst = _mst_ms(Xd, ms)
     res = {"plain": {"mst_ms": plain_ms, "knn_ms": pst["t_knn_ms"]}}
     for mode, name in ((1, "train_each_call"), (2, "trained_once")):
         _lib.check(ctx, L.fh_set_learned_radius(ctx, mode))
         if mode == 2:
-            mst_ms(Xd, ms, reps=1)  # the one training
-        t, st = mst_ms(Xd, ms)
+            _mst_ms(Xd, ms, reps=1)  # the one training
+        t, st = _mst_ms(Xd, ms)
         res[name] = {"mst_ms": t, "knn_ms": st["t_knn_ms"], "train_ms": st["t_learn_ms"],
                      "predict_ms": st["t_predict_ms"],
                      "ball_hit_rate": 1.0 - st["ball_misses"] / Xd.shape[0],
@@ -339,18 +357,25 @@ def side_forms(Xd, ms):
     _lib.check(ctx, L.fh_set_learned_radius(ctx, 0))
     res["note"] = ("exact by construction (ball misses and ties re-run the plain query); the plain kNN already "
                    "prunes after its first leaf, so the ball bound does not pay for itself here")
-    out["learned"] = res
+    return res
 
-    # high-dimensional MFMA path: 1M x 64, 100 Gaussians, min_samples 10
+
+def form_high_dim():
+    """The MFMA high-dimensional path: 1M x 64, 100 Gaussians, min_samples
+    10."""
+    import torch
+
+    from fast_hdbscan_amd import workloads
+
     n5, d5 = 1_000_000, 64
     X5 = torch.from_numpy(workloads.gaussian_highdim(n5, d5, 100, seed=3)).cuda()
-    t5, st5 = mst_ms(X5, 10, reps=1)
+    t5, st5 = _mst_ms(X5, 10, reps=1)
     # algorithmic flops of the round scan: 2*d per scored pair, over the
     # 128 x 128 tile pairs the scans actually scored (pairs of tiles inside
     # one component are skipped), per launch
     scan_ms = st5["t_query_main_ms"] / max(st5["rounds"], 1)
     flops = 2.0 * d5 * 128.0 * 128.0 * st5["scored_tile_pairs"] / max(st5["rounds"], 1)
-    out["high_dim"] = {
+    return {
         "workload": "1M x 64, 100 Gaussians (means N(0,4I), within N(0,I)), seed 3, min_samples 10",
         "mst_ms": t5, "points_per_s": n5 / (t5 * 1e-3), "rounds": st5["rounds"],
         "knn_ms": st5["t_knn_ms"], "knn_tree_rows": st5["knn_tree_rows"], "exact_rows": st5["exact_rows"],
@@ -359,30 +384,43 @@ def side_forms(Xd, ms):
                      "peak": 157.3, "unit": "TFLOP/s", "frac": flops / (scan_ms * 1e-3) / 1e12 / 157.3,
                      "avg_launch_ms": scan_ms, "alg_flops_per_launch": flops},
     }
-    del X5
 
-    # 10M x 3-D mixture, min_samples 15: plain, then the learned-radius form
-    # (trained once on a 1% sample, then reused)
+
+def form_scale_3d():
+    """10M x 3-D mixture, min_samples 15: plain, then the learned-radius
+    form (trained once on a 1% sample, then reused)."""
+    import torch
+
+    from fast_hdbscan_amd import _lib, workloads
+
+    ctx = _lib.context(0)
+    L = _lib.load()
     X3 = torch.from_numpy(workloads.scale_3d()).cuda()
-    t3, st3 = mst_ms(X3, 15, reps=2)
-    out["scale_3d"] = {"workload": "10M x 3-D, 200 anisotropic Gaussians + 5% noise, seed 2, min_samples 15",
-                       "mst_ms": t3, "points_per_s": X3.shape[0] / (t3 * 1e-3), "rounds": st3["rounds"],
-                       "knn_ms": st3["t_knn_ms"], "rounds_ms": st3["t_rounds_ms"], "tree_ms": st3["t_tree_ms"]}
+    t3, st3 = _mst_ms(X3, 15, reps=2)
+    out = {"workload": "10M x 3-D, 200 anisotropic Gaussians + 5% noise, seed 2, min_samples 15",
+           "mst_ms": t3, "points_per_s": X3.shape[0] / (t3 * 1e-3), "rounds": st3["rounds"],
+           "knn_ms": st3["t_knn_ms"], "rounds_ms": st3["t_rounds_ms"], "tree_ms": st3["t_tree_ms"]}
     _lib.check(ctx, L.fh_set_learned_radius(ctx, 2))
-    mst_ms(X3, 15, reps=1)  # the one training
-    t3l, st3l = mst_ms(X3, 15, reps=2)
+    _mst_ms(X3, 15, reps=1)  # the one training
+    t3l, st3l = _mst_ms(X3, 15, reps=2)
     _lib.check(ctx, L.fh_set_learned_radius(ctx, 0))
-    out["scale_3d"]["learned_trained_once"] = {
+    out["learned_trained_once"] = {
         "mst_ms": t3l, "knn_ms": st3l["t_knn_ms"], "predict_ms": st3l["t_predict_ms"],
         "sample_and_features_ms": st3l["t_learn_ms"],
         "ball_hit_rate": 1.0 - st3l["ball_misses"] / X3.shape[0], "speedup_vs_plain": t3 / t3l}
-    del X3
+    return out
 
-    # PLSCAN sweep: min_samples 2..64 on 500k x 2-D, layers for each; first
-    # one at a time, then pipelined (device MSTs back to back, layer stages
-    # in 8 host threads)
-    from fast_hdbscan_amd.hdbscan import _edges_f64
-    from fast_hdbscan_amd.layers import layers_from_mst, plscan_sweep
+
+def form_plscan_sweep():
+    """PLSCAN min_samples 2..64 on 500k x 2-D, layers for each: one at a
+    time, the shared-kNN sweep of all MSTs, then pipelined (device MSTs back
+    to back, layer stages in 8 host threads)."""
+    import numpy as np
+    import torch
+
+    from fast_hdbscan_amd import workloads
+    from fast_hdbscan_amd.hdbscan import _edges_f64, mst_device
+    from fast_hdbscan_amd.layers import layers_from_mst, mst_sweep_device, plscan_sweep
 
     Xs = torch.from_numpy(workloads.anisotropic_mixture(500_000, 2, 50, 0.1, seed=4)).cuda()
     t0 = time.perf_counter()
@@ -396,8 +434,6 @@ def side_forms(Xd, ms):
         t_mst += time.perf_counter() - a
         n_layers.append(len(layers_from_mst(e, c, Xs.shape[0], base_min_cluster_size=5)[0]))
     total = time.perf_counter() - t0
-    from fast_hdbscan_amd.layers import mst_sweep_device
-
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     mst_sweep_device(Xs, list(range(2, 65)))
@@ -407,21 +443,31 @@ def side_forms(Xd, ms):
     sw = plscan_sweep(Xs, range(2, 65), base_min_cluster_size=5, workers=8)
     total_p = time.perf_counter() - t0
     same = [len(sw[m][0]) for m in range(2, 65)] == n_layers
-    out["plscan_sweep"] = {"workload": "500k x 2-D, 50 anisotropic Gaussians + 10% noise, seed 4; min_samples 2..64, "
-                                       "base_min_cluster_size 5",
-                           "serial_total_s": total, "serial_mst_and_copy_s": t_mst,
-                           "serial_layer_stage_s": total - t_mst,
-                           "shared_knn_all_63_msts_s": sweep_mst_s, "pipelined_total_s": total_p,
-                           "pipelined_workers": 8, "pipelined_same_layer_counts": same,
-                           "layers_per_min_samples": n_layers}
+    return {"workload": "500k x 2-D, 50 anisotropic Gaussians + 10% noise, seed 4; min_samples 2..64, "
+                        "base_min_cluster_size 5",
+            "serial_total_s": total, "serial_mst_and_copy_s": t_mst, "serial_layer_stage_s": total - t_mst,
+            "shared_knn_all_63_msts_s": sweep_mst_s, "pipelined_total_s": total_p, "pipelined_workers": 8,
+            "pipelined_same_layer_counts": same, "layers_per_min_samples": n_layers}
 
-    # many independent small datasets through the native batch scheduler
+
+def form_batched(one_call_workers=None):
+    """Many independent small datasets through the native batch scheduler
+    (with one_call_workers: a single call on the 512 datasets, for
+    profiling)."""
+    from fast_hdbscan_amd import workloads
     from fast_hdbscan_amd.batch import hdbscan_batch
 
     data = workloads.batch_datasets(512)
     pts = sum(x.shape[0] for x in data)
     res = {"workload": "512 datasets, n log-uniform 500..50000, d in {2,3,5,8}, 3-20 Gaussians + 5% noise, "
                        "seeds 1000..1511; min_samples 5, min_cluster_size 20", "points": pts}
+    if one_call_workers:
+        w = int(one_call_workers)
+        t0 = time.perf_counter()
+        hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w)
+        el = time.perf_counter() - t0
+        res[f"workers_{w}"] = {"s": el, "datasets_per_s": len(data) / el, "points_per_s": pts / el}
+        return res
     for w in (1, 8, 16):
         t0 = time.perf_counter()
         hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w)
@@ -434,7 +480,27 @@ def side_forms(Xd, ms):
     hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=8)
     el = time.perf_counter() - t0
     res["all_4096_workers_8"] = {"points": pts, "s": el, "datasets_per_s": len(data) / el, "points_per_s": pts / el}
-    out["batched"] = res
+    return res
+
+
+def side_forms(Xd, ms, names=FORMS, batch_one_call=None):
+    """The other forms of the path, each run once after the timed steps
+    (not part of `value`): learned search radii on the same 2M x 2-D batch,
+    the MFMA high-dimensional path at 1M x 64, the 10M x 3-D mixture, a
+    PLSCAN min_samples sweep on 500k x 2-D and a batch of 512 small
+    independent datasets."""
+    out = {}
+    for name in names:
+        if name == "learned":
+            out[name] = form_learned(Xd, ms)
+        elif name == "high_dim":
+            out[name] = form_high_dim()
+        elif name == "scale_3d":
+            out[name] = form_scale_3d()
+        elif name == "plscan_sweep":
+            out[name] = form_plscan_sweep()
+        elif name == "batched":
+            out[name] = form_batched(batch_one_call)
     return out
 
 
