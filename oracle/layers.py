@@ -24,15 +24,15 @@ reference): ``numba_argsort`` restates that published algorithm -- median
 of three, Hoare-style partition with the pivot parked at the end, the
 larger side pushed, insertion sort below 15 elements.
 
-Pinned by (tests/test_layers_oracle.py): the reference's own PLSCAN tests
+Pinned by (tests/test_layers.py): the reference's own PLSCAN tests
 (test_plscan.py: 3 clusters on the 300-point blobs, one layer with
 max_layers=1, base_n_clusters=4 within one, layer ordering, tree coverage,
 strengths in [0, 1]); find_peaks against scipy.signal.find_peaks (whose
 algorithm it is derived from); numba_argsort against a stable sort where
-no ties exist and against insertion sort below 15 elements;
-compute_total_persistence against its definition summed in float64.
-Parity unpinned: numba_argsort's order of equal keys in arrays of 15 or
-more elements (no reference test has such ties in a numba-sorted array).
+no ties exist and against insertion sort below 15 elements, and by a hand
+trace of run_quicksort on 16 keys with ties (partition, pushed side,
+insertion sorts); compute_total_persistence against its definition summed
+in float64.
 """
 from __future__ import annotations
 

@@ -20,14 +20,19 @@ numba_kdtree.build_kdtree 478-521      closed-form level/node counts;
                                        (value, index) median order); median
                                        invariant at every split.  Within-leaf
                                        ORDER (introselect swap sequence):
-                                       parity unpinned beyond the restatement.
+                                       two hand traces of _introselect_impl
+                                       (partition then insertion sort; pivot
+                                       landing on nth).  The depth-limit
+                                       heapsort branch is pinned only by its
+                                       sortedness (unique keys: any sort).
 numba_kdtree.parallel_tree_query       brute-force kNN (same float32 rdist) on
 774-792 (+ heap push / deheap_sort)    tie-free data: identical distances and
                                        indices; core distances vs float64
                                        sklearn KDTree within 1e-5 relative.
-                                       Heap tie layout: parity unpinned beyond
-                                       the restatement (duplicate-point
-                                       fixture checked by hand).
+                                       Heap tie layout: two hand traces of
+                                       simple_heap_push / deheap_sort (all
+                                       points equal; partial ties where a
+                                       stable sort would differ).
 boruvka.parallel_boruvka 552-647       MST weight multiset equal to
 (+ merge_components, init, disjoint    scipy.sparse.csgraph MST on the dense
 set)                                   float32 mutual-reachability matrix;
@@ -36,7 +41,9 @@ set)                                   float32 mutual-reachability matrix;
                                        the minimum, first partner in full DFS
                                        order) checked by an unpruned search.
                                        Row order / orientation under ties:
-                                       parity unpinned beyond the restatement.
+                                       the duplicate-point fixture by hand
+                                       (star to point 4, rows in point
+                                       order); elsewhere the restatement.
 =====================================  =====================================
 
 The float32 accumulation form of rdist (fma vs separate multiply/add) is not
