@@ -581,137 +581,12 @@ __global__ __launch_bounds__(NT) void small_select_kernel(int count, SegState *s
     seg[k].flag = SEG_DONE;
 }
 
-// ------------------------------------- one work-group per node ----
-
-constexpr int kMTB = 1024;  // threads of the one-work-group-per-node steps
-constexpr int kMB = 8;      // keys each thread loads at once
-
-// One partition step of introselect on keys[L, R) (pivot of numba_kdtree.py:
-// 280-299, the two-pointer exchanges of 313-350 as one parallel exchange,
-// the pivot move of 349) by one work-group of kMTB threads, keys in global
-// memory, each thread over a contiguous slice read kMB keys at a time;
-// gtpos / ltpos hold the exchange partners (indices L .. R-1 of the scratch
-// arrays the multi-block steps use).  Returns the pivot's final position P.
-__device__ int wg_select_step(uint64_t *keys, int *gtpos, int *ltpos, int L, int R, int *ws, int *sh) {
-    if (threadIdx.x == 0) median3_and_move(keys, L, R);
-    __syncthreads();
-    const uint64_t pivot = keys[R - 1];
-    const int len = R - 1 - L;
-    const int per = (len + kMTB - 1) / kMTB;
-    const int b = L + threadIdx.x * per;
-    const int e = min(b + per, R - 1);
-    int v = 0;
-    for (int p0 = b; p0 < e; p0 += kMB) {
-        uint64_t kv[kMB];
-#pragma unroll
-        for (int j = 0; j < kMB; ++j) kv[j] = p0 + j < e ? keys[p0 + j] : ~0ull;  // ~0 exceeds every key
-#pragma unroll
-        for (int j = 0; j < kMB; ++j) v += kv[j] < pivot ? 1 : 0;
-    }
-    int S;
-    int before = block_excl_nt<kMTB>(v, S, ws);
-    const int P = L + S;
-    if (threadIdx.x == 0) sh[0] = 0;
-    __syncthreads();
-    for (int p0 = b; p0 < e; p0 += kMB) {
-        uint64_t kv[kMB];
-#pragma unroll
-        for (int j = 0; j < kMB; ++j) kv[j] = p0 + j < e ? keys[p0 + j] : ~0ull;
-#pragma unroll
-        for (int j = 0; j < kMB; ++j) {
-            const int p = p0 + j;
-            if (p < e) {
-                const bool lt = kv[j] < pivot;
-                if (p == P) sh[0] = (P - L) - before;
-                if (p < P) {
-                    if (!lt) gtpos[L + (p - L) - before] = p;
-                } else if (lt) {
-                    ltpos[L + (S - 1 - before)] = p;
-                }
-                before += lt ? 1 : 0;
-            }
-        }
-    }
-    __syncthreads();
-    const int M = sh[0];
-    for (int m = threadIdx.x; m < M; m += kMTB) swap64(keys, gtpos[L + m], ltpos[L + m]);
-    __syncthreads();
-    if (threadIdx.x == 0) swap64(keys, P, R - 1);
-    __syncthreads();
-    return P;
-}
-
-// One block per node of a level whose nodes are at most kMedium long: the
-// steps over global memory while the active range is longer than kSmall
-// (a range reaching the depth limit first is sorted in place, as the
-// reference heap-sorts it); small_select_kernel then finishes in LDS.
-__global__ __launch_bounds__(kMTB) void medium_select_kernel(int count, SegState *seg, uint64_t *A, int *gtpos,
-                                                             int *ltpos) {
-    __shared__ int ws[kMTB / 64];
-    __shared__ int sh[4];
-    const int k = blockIdx.x;
-    if (k >= count) return;
-    SegState st = seg[k];
-    if (st.flag != SEG_LARGE) return;
-    int L = st.L, R = st.R, depth = st.depth;
-    const int nth = st.nth;
-    bool done = false;
-    while (R - L > kSmall) {
-        if (depth == 0) {
-            lds_sort(A + L, R - L);  // the same network, here over global memory
-            done = true;
-            break;
-        }
-        --depth;
-        const int P = wg_select_step(A, gtpos, ltpos, L, R, ws, sh);
-        if (nth < P) {
-            R = P;
-        } else if (nth > P) {
-            L = P + 1;
-        } else {
-            done = true;
-            break;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (done) {
-            seg[k].flag = SEG_DONE;
-        } else {
-            seg[k].L = L;
-            seg[k].R = R;
-            seg[k].depth = depth;
-            seg[k].flag = SEG_SMALL;
-        }
-    }
-}
-
-// ---------------------------------------- one work-group per subtree ----
+// ------------------------------------------- partition passes ----
 //
-// All levels from level0 down of a subtree whose root holds at most
-// sub_cap<D>() points, in one work-group over LDS: the points' rows are
-// gathered once (slot order), every key (ordered coordinate << 32 | point
-// index) moves together with its slot through the selects, and each level
-// runs what the level-synchronous kernels above run over global memory --
-// per-node bounds (bounds_kernel), split axis and box (axis_kernel), keys
-// (keys_kernel) and the exact introselect of every internal node
-// (small_select_kernel), here one wave per node.  Levels with fewer nodes
-// than waves share each node's bounds and keys among several waves (bounds
-// merged with LDS atomics); from the level with one node per wave on, each
-// wave builds the subtrees of its own nodes with no work-group barrier.
-// The work-group ends with the tree-order outputs of its positions
-// (finalize_kernel).  LDS holds 12 + 4D bytes per point, sized so that two
-// work-groups share a CU.
-
-constexpr int kSubNT = 512;
-constexpr int kSubW = kSubNT / 64;
-constexpr int kSubLogW = 3;
-static_assert((1 << kSubLogW) == kSubW, "waves per subtree work-group");
-
-template <int D>
-constexpr int sub_cap() {
-    return ((81920 - 512) / (12 + 4 * D)) / 64 * 64;
-}
+// Helpers shared by the one-work-group-per-node steps (keys in global
+// memory) and the subtree kernel (keys in LDS): a wave owns a contiguous
+// run of positions and reads it 64 keys at a time, four loads in flight;
+// ranks come from ballots.
 
 __device__ __forceinline__ void sub_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -801,7 +676,8 @@ __device__ void sub_sort(uint64_t *sk, uint16_t *sl, int L, int n) {
 // of P at part[R - 2 - rank from the right].  lts enters as the "< pivot"
 // count before b and the return value is the ">= pivot" count left of P in
 // [b, e).
-__device__ __forceinline__ int sub_rank_pass(const uint64_t *sk, uint16_t *part, uint64_t pivot, int L, int R,
+template <typename T>
+__device__ __forceinline__ int sub_rank_pass(const uint64_t *sk, T *part, uint64_t pivot, int L, int R,
                                              int S, int b, int e, int lts) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -823,9 +699,9 @@ __device__ __forceinline__ int sub_rank_pass(const uint64_t *sk, uint16_t *part,
             const int before = lts + __popcll(ltm & below);
             if (in) {
                 if (p < P) {
-                    if (!lt) part[L + (p - L) - before] = (uint16_t)p;
+                    if (!lt) part[L + (p - L) - before] = (T)p;
                 } else if (lt) {
-                    part[R - 2 - (S - 1 - before)] = (uint16_t)p;
+                    part[R - 2 - (S - 1 - before)] = (T)p;
                 }
             }
             lts += __popcll(ltm);
@@ -850,6 +726,144 @@ __device__ __forceinline__ int sub_count_pass(const uint64_t *sk, uint64_t pivot
         for (int u = 0; u < 4; ++u) S += __popcll(__ballot(kv[u] < pivot));
     }
     return S;
+}
+
+// ------------------------------------- one work-group per node ----
+
+constexpr int kMTB = 1024;  // threads of the one-work-group-per-node steps
+constexpr int kMW = kMTB / 64;
+
+// One partition step of introselect on keys[L, R) (pivot of numba_kdtree.py:
+// 280-299, the two-pointer exchanges of 313-350 as one parallel exchange,
+// the pivot move of 349) by one work-group of kMTB threads, keys in global
+// memory: every thread reads the median-of-three candidates, each wave
+// counts and ranks one contiguous sixteenth of [L, R - 1) with coalesced
+// loads, and part[] (the scratch array of the multi-block steps) holds the
+// exchange partners: the m-th ">= pivot" key left of P at L + m, the m-th
+// "< pivot" key from the right at R - 2 - m.  Returns the pivot's final
+// position P.
+__device__ int wg_select_step(uint64_t *keys, int *part, int L, int R, int *ws, int *ws2) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int mid = (L + R - 1) / 2;
+    uint64_t k0 = keys[L], k1 = keys[mid], k2 = keys[R - 1];
+    if (k0 > k1) {
+        const uint64_t t = k0;
+        k0 = k1;
+        k1 = t;
+    }
+    if (k1 > k2) {
+        const uint64_t t = k1;
+        k1 = k2;
+        k2 = t;
+        if (k0 > k1) {
+            const uint64_t u = k0;
+            k0 = k1;
+            k1 = u;
+        }
+    }
+    // the median moves to R - 1 and becomes the pivot
+    const uint64_t pivot = k1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        keys[L] = k0;
+        keys[mid] = k2;
+        keys[R - 1] = k1;
+    }
+    __syncthreads();
+    const int len = R - 1 - L;
+    const int b = L + (int)((int64_t)len * w / kMW), e = L + (int)((int64_t)len * (w + 1) / kMW);
+    const int c = sub_count_pass(keys, pivot, b, e);
+    if (lane == 0) ws[w] = c;
+    __syncthreads();
+    int S = 0, pre = 0;
+#pragma unroll
+    for (int i = 0; i < kMW; ++i) {
+        const int v = ws[i];
+        S += v;
+        pre += i < w ? v : 0;
+    }
+    const int P = L + S;
+    const int mw = sub_rank_pass(keys, part, pivot, L, R, S, b, e, pre);
+    if (lane == 0) ws2[w] = mw;
+    __syncthreads();
+    int M = 0;
+#pragma unroll
+    for (int i = 0; i < kMW; ++i) M += ws2[i];
+    for (int m = threadIdx.x; m < M; m += kMTB) swap64(keys, part[L + m], part[R - 2 - m]);
+    __syncthreads();
+    if (threadIdx.x == 0) swap64(keys, P, R - 1);
+    __syncthreads();
+    return P;
+}
+
+// One block per node of a level whose nodes are at most kMedium long: the
+// steps over global memory while the active range is longer than kSmall
+// (a range reaching the depth limit first is sorted in place, as the
+// reference heap-sorts it); small_select_kernel then finishes in LDS.
+__global__ __launch_bounds__(kMTB) void medium_select_kernel(int count, SegState *seg, uint64_t *A, int *part) {
+    __shared__ int ws[kMW], ws2[kMW];
+    const int k = blockIdx.x;
+    if (k >= count) return;
+    SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    int L = st.L, R = st.R, depth = st.depth;
+    const int nth = st.nth;
+    bool done = false;
+    while (R - L > kSmall) {
+        if (depth == 0) {
+            lds_sort(A + L, R - L);  // the same network, here over global memory
+            done = true;
+            break;
+        }
+        --depth;
+        const int P = wg_select_step(A, part, L, R, ws, ws2);
+        if (nth < P) {
+            R = P;
+        } else if (nth > P) {
+            L = P + 1;
+        } else {
+            done = true;
+            break;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (done) {
+            seg[k].flag = SEG_DONE;
+        } else {
+            seg[k].L = L;
+            seg[k].R = R;
+            seg[k].depth = depth;
+            seg[k].flag = SEG_SMALL;
+        }
+    }
+}
+
+// ---------------------------------------- one work-group per subtree ----
+//
+// All levels from level0 down of a subtree whose root holds at most
+// sub_cap<D>() points, in one work-group over LDS: the points' rows are
+// gathered once (slot order), every key (ordered coordinate << 32 | point
+// index) moves together with its slot through the selects, and each level
+// runs what the level-synchronous kernels above run over global memory --
+// per-node bounds (bounds_kernel), split axis and box (axis_kernel), keys
+// (keys_kernel) and the exact introselect of every internal node
+// (small_select_kernel), here one wave per node.  Levels with fewer nodes
+// than waves share each node's bounds and keys among several waves (bounds
+// merged with LDS atomics); from the level with one node per wave on, each
+// wave builds the subtrees of its own nodes with no work-group barrier.
+// The work-group ends with the tree-order outputs of its positions
+// (finalize_kernel).  LDS holds 12 + 4D bytes per point, sized so that two
+// work-groups share a CU.
+
+constexpr int kSubNT = 512;
+constexpr int kSubW = kSubNT / 64;
+constexpr int kSubLogW = 3;
+static_assert((1 << kSubLogW) == kSubW, "waves per subtree work-group");
+
+template <int D>
+constexpr int sub_cap() {
+    return ((81920 - 512) / (12 + 4 * D)) / 64 * 64;
 }
 
 // The introselect of node [L, R) towards nth (small_select_kernel's loop)
@@ -1218,8 +1232,7 @@ hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStr
 }
 
 hipError_t tree_medium_select(const TreeBuildArgs &a, int count, hipStream_t st) {
-    hipLaunchKernelGGL(medium_select_kernel, dim3((unsigned)count), dim3(kMTB), 0, st, count, a.seg, a.A, a.gtpos,
-                       a.ltpos);
+    hipLaunchKernelGGL(medium_select_kernel, dim3((unsigned)count), dim3(kMTB), 0, st, count, a.seg, a.A, a.gtpos);
     return hipGetLastError();
 }
 
