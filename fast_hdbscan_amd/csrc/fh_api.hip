@@ -257,13 +257,14 @@ int build_tree_device(fh_ctx *ctx, const float *Xd, int64_t n, int d, int *level
     FH_TRY(ensure(ctx, ctx->obox, (size_t)N * 2 * d));
     FH_TRY(ensure(ctx, ctx->taxis, last_count));
     FH_TRY(ensure(ctx, ctx->tseg, last_count));
-    FH_TRY(ensure(ctx, ctx->tcnt, (size_t)n / fh::tree_chunk() + 2 * last_count + 1));
+    // per (node, chunk, wave of the chunk's work-group) counts
+    FH_TRY(ensure(ctx, ctx->tcnt, 4 * ((size_t)n / fh::tree_chunk() + 2 * last_count + 1)));
     FH_TRY(ensure(ctx, ctx->gtpos, n));
     FH_TRY(ensure(ctx, ctx->ltpos, n));
     // step counters (a batch's steps use slots from 0) and the sort count
     constexpr int kSortSlot = 15;
     FH_TRY(ensure(ctx, ctx->tactive, kSortSlot + 1));
-    FH_TRY(ensure(ctx, ctx->taux, (size_t)3 * last_count));
+    FH_TRY(ensure(ctx, ctx->taux, (size_t)4 * last_count));
     fh::TreeBuildArgs a{};
     a.X = Xd;
     a.d = d;

@@ -28,9 +28,9 @@ struct TreeBuildArgs {
     float *box;          // N * 2d bounds (min[d], max[d]) per node
     int *axis;           // per node of the current level
     SegState *seg;       // per node of the current level
-    int *cnt;            // per (node, chunk) counts
-    int *gtpos, *ltpos;  // n each: exchange partners
-    int *aux;            // 3 per node of the level: (S, P, M) of the current step
+    int *cnt;            // per (node, chunk, wave) counts
+    int *gtpos, *ltpos;  // n each: exchange partners (the multi-block and one-work-group steps use gtpos)
+    int *aux;            // 4 per node of the level: (S, P, M, unused) of the current step
     int *active;         // step counters
     int *sorts;          // nodes of the level left to the sort (depth limit)
 };

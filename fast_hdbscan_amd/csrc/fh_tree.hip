@@ -266,321 +266,6 @@ __global__ __launch_bounds__(kTB) void keys_kernel(LevelRef lv, const float *__r
     }
 }
 
-// ------------------------------------------- multi-block introselect ----
-
-__device__ __forceinline__ void swap64(uint64_t *A, int i, int j) {
-    uint64_t t = A[i];
-    A[i] = A[j];
-    A[j] = t;
-}
-
-// numba_kdtree.py:280-299 and the pivot move of 313-318.
-__device__ __forceinline__ void median3_and_move(uint64_t *A, int L, int R) {
-    const int mid = (L + R - 1) / 2;
-    if (A[L] > A[mid]) swap64(A, L, mid);
-    if (A[mid] > A[R - 1]) {
-        swap64(A, mid, R - 1);
-        if (A[L] > A[mid]) swap64(A, L, mid);
-    }
-    swap64(A, mid, R - 1);
-}
-
-// Pivot of the next step of a large segment (numba_kdtree.py:301-318):
-// ranges of at most kSmall go to the in-LDS finish, ranges at the depth
-// limit to the sort (counted in *sorts).  A segment left SEG_LARGE has its
-// pivot moved to R-1 and counts itself in *active.
-__device__ __forceinline__ void pivot_step(SegState &st, uint64_t *A, int *active, int *sorts) {
-    const int size = st.R - st.L;
-    if (size <= kSmall) {
-        st.flag = SEG_SMALL;
-        return;
-    }
-    if (st.depth == 0) {
-        st.flag = SEG_SORT;
-        atomicAdd(sorts, 1);
-        return;
-    }
-    st.depth -= 1;
-    median3_and_move(A, st.L, st.R);
-    st.pivot = A[st.R - 1];
-    atomicAdd(active, 1);
-}
-
-// First pivot of every large segment of a level.
-__global__ void big_pivot_kernel(int count, SegState *seg, uint64_t *A, int *active, int *sorts) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    SegState st = seg[k];
-    if (st.flag != SEG_LARGE) return;
-    pivot_step(st, A, active, sorts);
-    seg[k] = st;
-}
-
-// Blocks over (segment, chunk of [L, R-1)): count of keys < pivot.
-__global__ __launch_bounds__(kTB) void big_count_kernel(int count, int cps, const SegState *__restrict__ seg,
-                                                        const uint64_t *__restrict__ A, int *cnt) {
-    __shared__ int ws[kTB / 64];
-    const int k = blockIdx.x / cps, c = blockIdx.x % cps;
-    if (k >= count) return;
-    const SegState st = seg[k];
-    if (st.flag != SEG_LARGE) return;
-    const int s = st.L + c * kChunk;
-    const int e = min(st.R - 1, s + kChunk);
-    int v = 0;
-    for (int p = s + threadIdx.x; p < e; p += kTB) v += A[p] < st.pivot ? 1 : 0;
-    int tot;
-    block_excl(v, tot, ws);
-    if (threadIdx.x == 0) cnt[(size_t)k * cps + c] = s < e ? tot : 0;
-}
-
-// Sums over the nch chunk counts of one segment: those of chunks before c
-// (returned) and all of them (total).
-__device__ __forceinline__ int chunk_prefix(const int *__restrict__ cnt, int nch, int c, int &total, int *ws) {
-    int pre = 0, all = 0;
-    for (int t = threadIdx.x; t < nch; t += kTB) {
-        const int v = cnt[t];
-        all += v;
-        pre += t < c ? v : 0;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        pre += __shfl_xor(pre, o, 64);
-        all += __shfl_xor(all, o, 64);
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        ws[2 * w] = pre;
-        ws[2 * w + 1] = all;
-    }
-    __syncthreads();
-    pre = 0;
-    all = 0;
-#pragma unroll
-    for (int i = 0; i < kTB / 64; ++i) {
-        pre += ws[2 * i];
-        all += ws[2 * i + 1];
-    }
-    __syncthreads();
-    total = all;
-    return pre;
-}
-
-// Blocks over (segment, chunk): the pivot's final position P = L + S (S
-// keys < pivot) and the ranks of misplaced keys (the partners of the
-// reference's two-pointer exchanges, numba_kdtree.py:320-348).  aux holds
-// (S, P, M) per segment, M = number of exchanges: written by the block
-// holding position P, or by chunk 0 when P = R-1 (then M = 0).
-__global__ __launch_bounds__(kTB) void big_rank_kernel(int count, int cps, const SegState *__restrict__ seg,
-                                                       const uint64_t *__restrict__ A, const int *__restrict__ cnt,
-                                                       int *gtpos, int *ltpos, int *aux) {
-    __shared__ int ws[kTB / 64];
-    __shared__ int ws2[2 * (kTB / 64)];
-    const int k = blockIdx.x / cps, c = blockIdx.x % cps;
-    if (k >= count) return;
-    const SegState st = seg[k];
-    if (st.flag != SEG_LARGE) return;
-    const int s = st.L + c * kChunk;
-    const int e = min(st.R - 1, s + kChunk);
-    if (s >= e) return;
-    const int nch = (st.R - 1 - st.L + kChunk - 1) / kChunk;
-    int S;
-    const int pre = chunk_prefix(cnt + (size_t)k * cps, nch, c, S, ws2);
-    const int P = st.L + S;
-    if (c == 0 && threadIdx.x == 0) {
-        aux[3 * k] = S;
-        aux[3 * k + 1] = P;
-        if (P == st.R - 1) aux[3 * k + 2] = 0;
-    }
-    // each thread owns kPer consecutive elements
-    const int base = s + threadIdx.x * kPer;
-    uint32_t ltmask = 0;
-    int v = 0;
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        const int p = base + j;
-        const bool lt = p < e && A[p] < st.pivot;
-        ltmask |= (lt ? 1u : 0u) << j;
-        v += lt ? 1 : 0;
-    }
-    int tot;
-    int before = pre + block_excl(v, tot, ws);
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        const int p = base + j;
-        if (p >= e) break;
-        const bool lt = (ltmask >> j) & 1u;
-        if (p == P) aux[3 * k + 2] = (P - st.L) - before;
-        if (p < P) {
-            if (!lt) gtpos[st.L + (p - st.L) - before] = p;
-        } else if (lt) {
-            ltpos[st.L + (S - 1 - before)] = p;
-        }
-        before += lt ? 1 : 0;
-    }
-}
-
-// Blocks over (segment, chunk of pairs): the exchanges.
-__global__ __launch_bounds__(kTB) void big_swap_kernel(int count, int cps, const SegState *__restrict__ seg,
-                                                       uint64_t *A, const int *__restrict__ gtpos,
-                                                       const int *__restrict__ ltpos, const int *__restrict__ aux) {
-    const int k = blockIdx.x / cps, c = blockIdx.x % cps;
-    if (k >= count) return;
-    const SegState st = seg[k];
-    if (st.flag != SEG_LARGE) return;
-    const int m0 = c * kChunk;
-    const int m1 = min(aux[3 * k + 2], m0 + kChunk);
-    for (int m = m0 + threadIdx.x; m < m1; m += kTB) swap64(A, gtpos[st.L + m], ltpos[st.L + m]);
-}
-
-// Pivot into place (numba_kdtree.py:349), narrowing (376-383), then the
-// pivot of the segment's next step.
-__global__ void big_finish_kernel(int count, SegState *seg, uint64_t *A, const int *__restrict__ aux, int *active,
-                                  int *sorts) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    SegState st = seg[k];
-    if (st.flag != SEG_LARGE) return;
-    const int P = aux[3 * k + 1];
-    swap64(A, P, st.R - 1);
-    if (st.nth < P) {
-        st.R = P;
-    } else if (st.nth > P) {
-        st.L = P + 1;
-    } else {
-        st.flag = SEG_DONE;
-        seg[k] = st;
-        return;
-    }
-    pivot_step(st, A, active, sorts);
-    seg[k] = st;
-}
-
-// ------------------------------------------------- in-LDS introselect ----
-
-// Ascending sort of n keys in LDS with a bitonic network whose merge steps
-// all point the same way (the first step of each merge compares mirrored
-// positions), so positions >= n behave as +inf and need no storage.
-__device__ void lds_sort(uint64_t *sk, int n) {
-    int m = 1;
-    while (m < n) m <<= 1;
-    for (int size = 2; size <= m; size <<= 1) {
-        for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
-            const int half = size >> 1;
-            const int blk = t / half, off = t % half;
-            const int lo = blk * size + off;
-            const int hi = blk * size + size - 1 - off;
-            if (hi < n && sk[lo] > sk[hi]) {
-                const uint64_t x = sk[lo];
-                sk[lo] = sk[hi];
-                sk[hi] = x;
-            }
-        }
-        __syncthreads();
-        for (int stride = size >> 2; stride > 0; stride >>= 1) {
-            for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
-                const int lo = 2 * stride * (t / stride) + (t % stride);
-                const int hi = lo + stride;
-                if (hi < n && sk[lo] > sk[hi]) {
-                    const uint64_t x = sk[lo];
-                    sk[lo] = sk[hi];
-                    sk[hi] = x;
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// One block of NT threads per node of the level: finish the introselect of
-// segments that are (or became) at most CAP <= kSmall long.  Deep levels,
-// whose nodes hold a few dozen keys, use a small CAP and NT so that many
-// blocks share a CU.
-template <int CAP, int NT>
-__global__ __launch_bounds__(NT) void small_select_kernel(int count, SegState *seg, uint64_t *A) {
-    __shared__ uint64_t sk[CAP];
-    __shared__ int gt_s[CAP / 2], lt_s[CAP / 2];
-    __shared__ int ws[NT / 64];
-    __shared__ int sh[4];
-    const int k = blockIdx.x;
-    if (k >= count) return;
-    SegState st = seg[k];
-    if (st.flag != SEG_SMALL) return;
-    const int L0 = st.L, R0 = st.R;
-    const int n0 = R0 - L0;
-    for (int p = threadIdx.x; p < n0; p += NT) sk[p] = A[L0 + p];
-    __syncthreads();
-    // local coordinates
-    int L = 0, R = n0, nth = st.nth - L0, depth = st.depth;
-    bool sort_rest = false;
-    for (;;) {
-        if (R - L <= 16) {
-            sort_rest = true;
-            break;
-        }
-        if (depth == 0) {
-            sort_rest = true;
-            break;
-        }
-        --depth;
-        if (threadIdx.x == 0) median3_and_move(sk, L, R);
-        __syncthreads();
-        const uint64_t pivot = sk[R - 1];
-        const int len = R - 1 - L;
-        const int per = (len + NT - 1) / NT;
-        const int b = L + threadIdx.x * per;
-        int v = 0;
-        for (int j = 0; j < per; ++j) {
-            const int p = b + j;
-            v += (p < R - 1 && sk[p] < pivot) ? 1 : 0;
-        }
-        int S;
-        int before = block_excl_nt<NT>(v, S, ws);
-        const int P = L + S;
-        if (threadIdx.x == 0) sh[0] = 0;
-        __syncthreads();
-        for (int j = 0; j < per; ++j) {
-            const int p = b + j;
-            if (p >= R - 1) break;
-            const bool lt = sk[p] < pivot;
-            if (p == P) sh[0] = (P - L) - before;
-            if (p < P) {
-                if (!lt) gt_s[(p - L) - before] = p;
-            } else if (lt) {
-                lt_s[S - 1 - before] = p;
-            }
-            before += lt ? 1 : 0;
-        }
-        __syncthreads();
-        const int M = sh[0];
-        for (int m = threadIdx.x; m < M; m += NT) {
-            const int a = gt_s[m], c = lt_s[m];
-            const uint64_t t = sk[a];
-            sk[a] = sk[c];
-            sk[c] = t;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint64_t t = sk[P];
-            sk[P] = sk[R - 1];
-            sk[R - 1] = t;
-        }
-        __syncthreads();
-        if (nth < P)
-            R = P;
-        else if (nth > P)
-            L = P + 1;
-        else
-            break;
-    }
-    if (sort_rest) {
-        // insertion sort (<= 16) or heapsort at the depth limit: both leave
-        // [L, R) ascending
-        lds_sort(sk + L, R - L);
-    }
-    for (int p = threadIdx.x; p < n0; p += NT) A[L0 + p] = sk[p];
-    seg[k].flag = SEG_DONE;
-}
-
 // ------------------------------------------- partition passes ----
 //
 // Helpers shared by the one-work-group-per-node steps (keys in global
@@ -726,6 +411,312 @@ __device__ __forceinline__ int sub_count_pass(const uint64_t *sk, uint64_t pivot
         for (int u = 0; u < 4; ++u) S += __popcll(__ballot(kv[u] < pivot));
     }
     return S;
+}
+
+// ------------------------------------------- multi-block introselect ----
+
+__device__ __forceinline__ void swap64(uint64_t *A, int i, int j) {
+    uint64_t t = A[i];
+    A[i] = A[j];
+    A[j] = t;
+}
+
+// numba_kdtree.py:280-299 and the pivot move of 313-318.
+__device__ __forceinline__ void median3_and_move(uint64_t *A, int L, int R) {
+    const int mid = (L + R - 1) / 2;
+    if (A[L] > A[mid]) swap64(A, L, mid);
+    if (A[mid] > A[R - 1]) {
+        swap64(A, mid, R - 1);
+        if (A[L] > A[mid]) swap64(A, L, mid);
+    }
+    swap64(A, mid, R - 1);
+}
+
+// Pivot of the next step of a large segment (numba_kdtree.py:301-318):
+// ranges of at most kSmall go to the in-LDS finish, ranges at the depth
+// limit to the sort (counted in *sorts).  A segment left SEG_LARGE has its
+// pivot moved to R-1 and counts itself in *active.
+__device__ __forceinline__ void pivot_step(SegState &st, uint64_t *A, int *active, int *sorts) {
+    const int size = st.R - st.L;
+    if (size <= kSmall) {
+        st.flag = SEG_SMALL;
+        return;
+    }
+    if (st.depth == 0) {
+        st.flag = SEG_SORT;
+        atomicAdd(sorts, 1);
+        return;
+    }
+    st.depth -= 1;
+    median3_and_move(A, st.L, st.R);
+    st.pivot = A[st.R - 1];
+    atomicAdd(active, 1);
+}
+
+// First pivot of every large segment of a level; the node's exchange count
+// starts at zero.
+__global__ void big_pivot_kernel(int count, SegState *seg, uint64_t *A, int *aux, int *active, int *sorts) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    aux[4 * k + 2] = 0;
+    SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    pivot_step(st, A, active, sorts);
+    seg[k] = st;
+}
+
+constexpr int kWavesTB = kTB / 64;
+constexpr int kWaveChunk = kChunk / kWavesTB;  // positions per wave of a chunk
+
+// Blocks over (segment, chunk of [L, R - 1)): keys < pivot, counted per wave
+// over a contiguous quarter of the chunk (coalesced, four loads in flight).
+__global__ __launch_bounds__(kTB) void big_count_kernel(int count, int cps, const SegState *__restrict__ seg,
+                                                        const uint64_t *__restrict__ A, int *cnt) {
+    const int k = blockIdx.x / cps, c = blockIdx.x % cps;
+    if (k >= count) return;
+    const SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    const int w = threadIdx.x >> 6;
+    const int s = st.L + c * kChunk;
+    const int e = min(st.R - 1, s + kChunk);
+    const int b = min(e, s + w * kWaveChunk), f = min(e, b + kWaveChunk);
+    const int v = sub_count_pass(A, st.pivot, b, f);
+    if ((threadIdx.x & 63) == 0) cnt[((size_t)k * cps + c) * kWavesTB + w] = v;
+}
+
+// Sums of the per-wave counts of one segment: those before flat index t
+// (the same t for the whole block; returned) and all nch of them (total).
+__device__ __forceinline__ int wave_count_prefix(const int *__restrict__ cnt, int nch, int t, int &total, int *ws) {
+    int pre = 0, all = 0;
+    for (int i = threadIdx.x; i < nch; i += kTB) {
+        const int v = cnt[i];
+        all += v;
+        pre += i < t ? v : 0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        pre += __shfl_xor(pre, o, 64);
+        all += __shfl_xor(all, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        ws[2 * w] = pre;
+        ws[2 * w + 1] = all;
+    }
+    __syncthreads();
+    pre = 0;
+    all = 0;
+#pragma unroll
+    for (int i = 0; i < kTB / 64; ++i) {
+        pre += ws[2 * i];
+        all += ws[2 * i + 1];
+    }
+    __syncthreads();
+    total = all;
+    return pre;
+}
+
+// Blocks over (segment, chunk): S = keys < pivot, the pivot's final place
+// P = L + S, and the exchange partners of the reference's two-pointer scan
+// (numba_kdtree.py:320-348) in part[]: the m-th ">= pivot" key left of P at
+// L + m, the m-th "< pivot" key counted from the right at R - 2 - m.  aux
+// holds (S, P, M) per segment; M (the exchange count) is summed per block.
+__global__ __launch_bounds__(kTB) void big_rank_kernel(int count, int cps, const SegState *__restrict__ seg,
+                                                       const uint64_t *__restrict__ A, const int *__restrict__ cnt,
+                                                       int *part, int *aux) {
+    __shared__ int ws[2 * (kTB / 64)];
+    __shared__ int wm[kTB / 64];
+    const int k = blockIdx.x / cps, c = blockIdx.x % cps;
+    if (k >= count) return;
+    const SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    const int s = st.L + c * kChunk;
+    const int e = min(st.R - 1, s + kChunk);
+    if (s >= e) return;
+    const int w = threadIdx.x >> 6;
+    const int nch = (st.R - 1 - st.L + kChunk - 1) / kChunk;
+    int S;
+    const int *cn = cnt + (size_t)k * cps * kWavesTB;
+    int pre = wave_count_prefix(cn, nch * kWavesTB, c * kWavesTB, S, ws);
+    for (int i = 0; i < w; ++i) pre += cn[c * kWavesTB + i];
+    const int P = st.L + S;
+    if (c == 0 && threadIdx.x == 0) {
+        aux[4 * k] = S;
+        aux[4 * k + 1] = P;
+    }
+    const int b = min(e, s + w * kWaveChunk), f = min(e, b + kWaveChunk);
+    const int m = sub_rank_pass(A, part, st.pivot, st.L, st.R, S, b, f, pre);
+    if ((threadIdx.x & 63) == 0) wm[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int mb = 0;
+#pragma unroll
+        for (int i = 0; i < kTB / 64; ++i) mb += wm[i];
+        if (mb) atomicAdd(aux + 4 * k + 2, mb);
+    }
+}
+
+// Blocks over (segment, chunk of exchange pairs): the exchanges.
+__global__ __launch_bounds__(kTB) void big_swap_kernel(int count, int cps, const SegState *__restrict__ seg,
+                                                       uint64_t *A, const int *__restrict__ part,
+                                                       const int *__restrict__ aux) {
+    const int k = blockIdx.x / cps, c = blockIdx.x % cps;
+    if (k >= count) return;
+    const SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    const int m0 = c * kChunk;
+    const int m1 = min(aux[4 * k + 2], m0 + kChunk);
+    for (int m = m0 + threadIdx.x; m < m1; m += kTB) swap64(A, part[st.L + m], part[st.R - 2 - m]);
+}
+
+// Per segment: pivot into place (numba_kdtree.py:349), narrowing (376-383),
+// then the pivot of the segment's next step; the exchange count restarts
+// at zero.
+__global__ void big_finish_kernel(int count, SegState *seg, uint64_t *A, int *aux, int *active, int *sorts) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    SegState st = seg[k];
+    if (st.flag != SEG_LARGE) return;
+    const int P = aux[4 * k + 1];
+    swap64(A, P, st.R - 1);
+    aux[4 * k + 2] = 0;
+    if (st.nth < P) {
+        st.R = P;
+    } else if (st.nth > P) {
+        st.L = P + 1;
+    } else {
+        st.flag = SEG_DONE;
+        seg[k] = st;
+        return;
+    }
+    pivot_step(st, A, active, sorts);
+    seg[k] = st;
+}
+
+// ------------------------------------------------- in-LDS introselect ----
+
+// Ascending sort of n keys in LDS with a bitonic network whose merge steps
+// all point the same way (the first step of each merge compares mirrored
+// positions), so positions >= n behave as +inf and need no storage.
+__device__ void lds_sort(uint64_t *sk, int n) {
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int size = 2; size <= m; size <<= 1) {
+        for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
+            const int half = size >> 1;
+            const int blk = t / half, off = t % half;
+            const int lo = blk * size + off;
+            const int hi = blk * size + size - 1 - off;
+            if (hi < n && sk[lo] > sk[hi]) {
+                const uint64_t x = sk[lo];
+                sk[lo] = sk[hi];
+                sk[hi] = x;
+            }
+        }
+        __syncthreads();
+        for (int stride = size >> 2; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
+                const int lo = 2 * stride * (t / stride) + (t % stride);
+                const int hi = lo + stride;
+                if (hi < n && sk[lo] > sk[hi]) {
+                    const uint64_t x = sk[lo];
+                    sk[lo] = sk[hi];
+                    sk[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// One block of NT threads per node of the level: finish the introselect of
+// segments that are (or became) at most CAP <= kSmall long.  Deep levels,
+// whose nodes hold a few dozen keys, use a small CAP and NT so that many
+// blocks share a CU.
+template <int CAP, int NT>
+__global__ __launch_bounds__(NT) void small_select_kernel(int count, SegState *seg, uint64_t *A) {
+    __shared__ uint64_t sk[CAP];
+    __shared__ int gt_s[CAP / 2], lt_s[CAP / 2];
+    __shared__ int ws[NT / 64];
+    __shared__ int sh[4];
+    const int k = blockIdx.x;
+    if (k >= count) return;
+    SegState st = seg[k];
+    if (st.flag != SEG_SMALL) return;
+    const int L0 = st.L, R0 = st.R;
+    const int n0 = R0 - L0;
+    for (int p = threadIdx.x; p < n0; p += NT) sk[p] = A[L0 + p];
+    __syncthreads();
+    // local coordinates
+    int L = 0, R = n0, nth = st.nth - L0, depth = st.depth;
+    bool sort_rest = false;
+    for (;;) {
+        if (R - L <= 16) {
+            sort_rest = true;
+            break;
+        }
+        if (depth == 0) {
+            sort_rest = true;
+            break;
+        }
+        --depth;
+        if (threadIdx.x == 0) median3_and_move(sk, L, R);
+        __syncthreads();
+        const uint64_t pivot = sk[R - 1];
+        const int len = R - 1 - L;
+        const int per = (len + NT - 1) / NT;
+        const int b = L + threadIdx.x * per;
+        int v = 0;
+        for (int j = 0; j < per; ++j) {
+            const int p = b + j;
+            v += (p < R - 1 && sk[p] < pivot) ? 1 : 0;
+        }
+        int S;
+        int before = block_excl_nt<NT>(v, S, ws);
+        const int P = L + S;
+        if (threadIdx.x == 0) sh[0] = 0;
+        __syncthreads();
+        for (int j = 0; j < per; ++j) {
+            const int p = b + j;
+            if (p >= R - 1) break;
+            const bool lt = sk[p] < pivot;
+            if (p == P) sh[0] = (P - L) - before;
+            if (p < P) {
+                if (!lt) gt_s[(p - L) - before] = p;
+            } else if (lt) {
+                lt_s[S - 1 - before] = p;
+            }
+            before += lt ? 1 : 0;
+        }
+        __syncthreads();
+        const int M = sh[0];
+        for (int m = threadIdx.x; m < M; m += NT) {
+            const int a = gt_s[m], c = lt_s[m];
+            const uint64_t t = sk[a];
+            sk[a] = sk[c];
+            sk[c] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint64_t t = sk[P];
+            sk[P] = sk[R - 1];
+            sk[R - 1] = t;
+        }
+        __syncthreads();
+        if (nth < P)
+            R = P;
+        else if (nth > P)
+            L = P + 1;
+        else
+            break;
+    }
+    if (sort_rest) {
+        // insertion sort (<= 16) or heapsort at the depth limit: both leave
+        // [L, R) ascending
+        lds_sort(sk + L, R - L);
+    }
+    for (int p = threadIdx.x; p < n0; p += NT) A[L0 + p] = sk[p];
+    seg[k].flag = SEG_DONE;
 }
 
 // ------------------------------------- one work-group per node ----
@@ -1213,7 +1204,7 @@ hipError_t tree_level_keys(const TreeBuildArgs &a, int level, int max_size, int 
 }
 
 hipError_t tree_big_begin(const TreeBuildArgs &a, int count, hipStream_t st) {
-    hipLaunchKernelGGL(big_pivot_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.active,
+    hipLaunchKernelGGL(big_pivot_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.aux, a.active,
                        a.sorts);
     return hipGetLastError();
 }
@@ -1223,11 +1214,11 @@ hipError_t tree_big_step(const TreeBuildArgs &a, int count, int max_size, hipStr
     hipLaunchKernelGGL(big_count_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
                        a.cnt);
     hipLaunchKernelGGL(big_rank_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
-                       a.cnt, a.gtpos, a.ltpos, a.aux);
+                       a.cnt, a.gtpos, a.aux);
     hipLaunchKernelGGL(big_swap_kernel, dim3((unsigned)(count * cps)), dim3(kTB), 0, st, count, cps, a.seg, a.A,
-                       a.gtpos, a.ltpos, a.aux);
-    hipLaunchKernelGGL(big_finish_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.aux,
-                       a.active, a.sorts);
+                       a.gtpos, a.aux);
+    hipLaunchKernelGGL(big_finish_kernel, dim3(nblk(count, 256)), dim3(256), 0, st, count, a.seg, a.A, a.aux, a.active,
+                       a.sorts);
     return hipGetLastError();
 }
 
