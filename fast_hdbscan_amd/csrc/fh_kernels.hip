@@ -313,30 +313,54 @@ __global__ __launch_bounds__(256) void boruvka_query_kernel(QueryArgs a) {
 // neighbours all share its component keeps that property for the rest of
 // the call: it is marked interior (interior[p] = 1) and skipped from then
 // on.
+// atomicMin(base + c, v) for every lane with has set, reduced first over
+// the wave's lanes that share c (lanes of consecutive tree positions mostly
+// share their component): one atomic per distinct c in the wave.  Every lane
+// of the wave must call it.
+template <typename T>
+__device__ __forceinline__ void wave_atomic_min_by_key(T *base, int c, T v, bool has) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(has);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int c0 = __shfl(c, leader, 64);
+        const bool mine = has && c == c0;
+        T m = mine ? v : static_cast<T>(~static_cast<T>(0));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const T other = __shfl_xor(m, o, 64);
+            m = other < m ? other : m;
+        }
+        if (lane == leader) atomicMin(base + c0, m);
+        pending &= ~__ballot(mine);
+    }
+}
+
 __global__ void seed_bounds_kernel(int64_t n, int kn, const int *__restrict__ nb_pos, const float *__restrict__ nb_d,
                                    const float *__restrict__ core_p, const int *__restrict__ comp_p, float *bound,
                                    unsigned char *interior) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || interior[p]) return;
-    const int c = comp_p[p];
-    const float ci = core_p[p];
+    const bool act = p < n && !interior[p];
+    int c = -1;
     float w = kInf;
-    bool cross = false;
-    for (int j = 0; j < kn; ++j) {
-        const int q = nb_pos[p * kn + j];
-        if (q < 0 || comp_p[q] == c) continue;
-        cross = true;
-        float e = nb_d[p * kn + j];
-        e = e > ci ? e : ci;
-        const float cq = core_p[q];
-        e = e > cq ? e : cq;
-        w = e < w ? e : w;
+    if (act) {
+        c = comp_p[p];
+        const float ci = core_p[p];
+        bool cross = false;
+        for (int j = 0; j < kn; ++j) {
+            const int q = nb_pos[p * kn + j];
+            if (q < 0 || comp_p[q] == c) continue;
+            cross = true;
+            float e = nb_d[p * kn + j];
+            e = e > ci ? e : ci;
+            const float cq = core_p[q];
+            e = e > cq ? e : cq;
+            w = e < w ? e : w;
+        }
+        if (!cross) interior[p] = 1;
     }
-    if (!cross) {
-        interior[p] = 1;
-        return;
-    }
-    if (w < kInf) atomicMin(reinterpret_cast<unsigned int *>(bound + c), __float_as_uint(w));
+    // non-negative floats order as their bit patterns
+    wave_atomic_min_by_key(reinterpret_cast<unsigned int *>(bound), c, __float_as_uint(w), act && w < kInf);
 }
 
 // Which certification node (in-level index at `level`) holds each position,
@@ -633,11 +657,13 @@ __global__ __launch_bounds__(256) void boruvka_wave_kernel(QueryArgs a) {
             entering = false;
         }
     }
-    if (act && best_j >= 0) {
-        const unsigned long long key = ((unsigned long long)__float_as_uint(best) << 32) | (unsigned int)a.perm[pos];
-        atomicMin(a.best + comp, key);
+    const bool found = act && best_j >= 0;
+    unsigned long long key = ~0ull;
+    if (found) {
+        key = ((unsigned long long)__float_as_uint(best) << 32) | (unsigned int)a.perm[pos];
         a.partner[pos] = best_j;
     }
+    wave_atomic_min_by_key(a.best, comp, key, found);
 }
 
 // sample_weight_core_distance (boruvka.py:527-539): the first sorted
