@@ -403,18 +403,32 @@ __device__ __forceinline__ float box_box_lb(const float *__restrict__ a, const f
     return r;
 }
 
-// One wave per certification node P of a single component C: the tree is
-// split into the subtrees 6 levels below the root, one per lane, and each
-// lane walks its subtree with the box-to-box bound; the node fails as soon
-// as some point j outside C could give a weight max(d(box P, x_j), core_j,
-// min core of P) <= B_C (an existence test, so the split does not change
-// the answer).  A certified node's points cannot reach B_C (nor, therefore,
-// the component minimum), so the main search skips them.
+// Orders the LDS accesses of one wavefront's lanes (a wave executes in
+// lockstep; this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wave per certification node P of a single component C: the node
+// fails as soon as some point j outside C could give a weight
+// max(d(box P, x_j), core_j, min core of P) <= B_C (an existence test, so
+// the visiting order does not change the answer).  The wave shares the
+// walk: every lane holds one tree node, descends into its left child when
+// the box-to-box bound and the single-component test let it pass and
+// queues the right child in LDS; lanes without a node take queued ones.  A
+// queue overflow leaves the node uncertified (always allowed: the main
+// search then simply visits its points).  A certified node's points cannot
+// reach B_C (nor, therefore, the component minimum), so the main search
+// skips them.
 template <int D>
 __global__ __launch_bounds__(64) void cert_kernel(CertArgs a) {
-    __shared__ int fail_s;
+    constexpr int kQ = 1024;
+    __shared__ int q[kQ];
     const int k = blockIdx.x;
     const int lane = threadIdx.x;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const int dd = D > 0 ? D : a.d;
     const int stride = 2 * dd;
     const int P = (1 << a.level) - 1 + k;
@@ -429,8 +443,6 @@ __global__ __launch_bounds__(64) void cert_kernel(CertArgs a) {
         if (lane == 0) a.cert[k] = mc > B ? 1 : 0;
         return;
     }
-    if (lane == 0) fail_s = 0;
-    __syncthreads();
     float pb[2 * (D > 0 ? D : 1)];
     const float *boxP = a.box + (size_t)P * stride;
     if constexpr (D > 0) {
@@ -438,52 +450,63 @@ __global__ __launch_bounds__(64) void cert_kernel(CertArgs a) {
         for (int j = 0; j < 2 * D; ++j) pb[j] = boxP[j];
     }
     const float *bp = D > 0 ? pb : boxP;
-    const int sd = a.levels - 1 < 6 ? a.levels - 1 : 6;
-    const int nsub = 1 << sd;
-    if (lane < nsub) {
-        const int root = nsub - 1 + lane;
-        int node = root, depth = sd;
-        bool entering = true, fail = false;
-        for (;;) {
-            if (entering) {
-                if (*(volatile int *)&fail_s) break;
-                const float lb = box_box_lb<D>(bp, a.box + (size_t)node * stride, a.d);
-                if (!(lb > B) && a.node_comp[node] != C) {
-                    if (depth == a.levels - 1) {
-                        const int2 r = a.range[node];
-                        for (int j = r.x; j < r.y; ++j) {
-                            if (a.comp_p[j] == C) continue;
-                            const float cj = a.core_p[j];
-                            if (!(cj <= B)) continue;
-                            float w = box_lb<D>(a.xp + (size_t)j * dd, bp, a.d);
-                            w = w > cj ? w : cj;
-                            w = w > mc ? w : mc;
-                            if (!(w > B)) {
-                                fail = true;
-                                break;
-                            }
+    const int first_leaf = (1 << (a.levels - 1)) - 1;
+    int node = lane == 0 ? 0 : -1;
+    int qn = 0;  // queued nodes (the same on every lane)
+    bool fail = false, failed = false, overflow = false;
+    for (;;) {
+        bool push = false;
+        int child = -1;
+        if (node >= 0) {
+            const float lb = box_box_lb<D>(bp, a.box + (size_t)node * stride, a.d);
+            if (!(lb > B) && a.node_comp[node] != C) {
+                if (node >= first_leaf) {
+                    const int2 r = a.range[node];
+                    for (int j = r.x; j < r.y; ++j) {
+                        if (a.comp_p[j] == C) continue;
+                        const float cj = a.core_p[j];
+                        if (!(cj <= B)) continue;
+                        float w = box_lb<D>(a.xp + (size_t)j * dd, bp, a.d);
+                        w = w > cj ? w : cj;
+                        w = w > mc ? w : mc;
+                        if (!(w > B)) {
+                            fail = true;
+                            break;
                         }
-                    } else {
-                        node = 2 * node + 1;
-                        ++depth;
-                        continue;
                     }
+                    node = -1;
+                } else {
+                    child = 2 * node + 2;
+                    push = true;
+                    node = 2 * node + 1;
                 }
-            }
-            if (fail || node == root) break;
-            if (node & 1) {  // left child done: go to its sibling
-                node += 1;
-                entering = true;
             } else {
-                node = (node - 1) >> 1;
-                --depth;
-                entering = false;
+                node = -1;
             }
         }
-        if (fail) fail_s = 1;
+        if (__ballot(fail) != 0) {
+            failed = true;  // (on every lane: the ballot is the wave's)
+            break;
+        }
+        const unsigned long long pm = __ballot(push);
+        const int npush = __popcll(pm);
+        if (qn + npush > kQ) {
+            overflow = true;
+            break;
+        }
+        if (push) q[qn + __popcll(pm & below)] = child;
+        qn += npush;
+        wave_sync();
+        const unsigned long long im = __ballot(node < 0);
+        const int idle = __popcll(im);
+        const int take = idle < qn ? idle : qn;
+        const int ir = __popcll(im & below);
+        if (node < 0 && ir < take) node = q[qn - 1 - ir];
+        qn -= take;
+        wave_sync();
+        if (__ballot(node >= 0) == 0) break;
     }
-    __syncthreads();
-    if (lane == 0) a.cert[k] = fail_s ? 0 : 1;
+    if (lane == 0) a.cert[k] = (failed || overflow) ? 0 : 1;
 }
 
 // ------------------------------------------ wave-cooperative search ----
@@ -508,12 +531,6 @@ __global__ __launch_bounds__(64) void cert_kernel(CertArgs a) {
 // first such candidate in that order, as the recursion records it.
 // Lanes of larger weight report values >= w* that cannot win the packed
 // atomicMin unless they also achieve w*.
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // True when position j1 comes before j2 in the depth-first visiting order of
 // the query for point x (the recursion of boruvka.py:228-285).
