@@ -75,6 +75,7 @@ class FHStats(ctypes.Structure):
         ("t_learn_ms", ctypes.c_double),
         ("t_predict_ms", ctypes.c_double),
         ("scored_tile_pairs", ctypes.c_int64),
+        ("tie_cycle_nodes", ctypes.c_int64),
     ]
 
     def as_dict(self):

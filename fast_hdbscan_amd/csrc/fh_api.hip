@@ -766,11 +766,29 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     fh::CycleBufs cb{ctx->S0.p, ctx->M0.p, ctx->S1.p, ctx->M1.p};
     if (tree_rows)
         FH_HIP(fh::launch_init_edges_pos(ni, kq, ctx->nb_pos.p, ms <= 1 ? ctx->nb_d.p : nullptr, ctx->core_p.p,
-                                         ctx->perm.p, ctx->posof.p, ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p, st));
+                                         ctx->perm.p, ctx->posof.p, ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p,
+                                         ctx->ew.p, st));
     else
         FH_HIP(fh::launch_init_edges(ni, kq, ctx->knn_i.p, knn_d_rows ? ctx->knn_d.p : nullptr, ctx->core_o.p,
                                      ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p, st));
-    FH_HIP(fh::launch_cycle_keep(ni, ctx->succ.p, cb, ctx->keep.p, st));
+    // Cycles of the forest's functional graph: 2-cycles settled at once,
+    // pointer doubling only over points whose successor has an equal core
+    // distance (FH_TIE_CYCLES: from how many points; flag, offs and parent
+    // are free until the emission)
+    int tie_cycles = 4096;
+    if (const char *m = std::getenv("FH_TIE_CYCLES")) tie_cycles = std::atoi(m);
+    if (ni >= tie_cycles) {
+        FH_HIP(fh::launch_cycle_classify(ni, ctx->succ.p, tree_rows ? ctx->ew.p : ctx->core_o.p, ctx->keep.p,
+                                         ctx->flag.p, st));
+        FH_HIP(fh::launch_excl_scan(ni, ctx->flag.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 2, st));
+        FH_HIP(hipMemcpyAsync(ctx->h_small + 2, ctx->small.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+        FH_HIP(hipStreamSynchronize(st));
+        S.tie_cycle_nodes += ctx->h_small[2];
+        FH_HIP(fh::launch_cycle_ties(ni, ctx->h_small[2], ctx->succ.p, ctx->flag.p, ctx->offs.p, cb, ctx->parent.p,
+                                     ctx->keep.p, st));
+    } else {
+        FH_HIP(fh::launch_cycle_keep(ni, ctx->succ.p, cb, ctx->keep.p, st));
+    }
     FH_HIP(fh::launch_excl_scan(ni, ctx->keep.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 0, st));
     FH_HIP(fh::launch_init_emit_union(ni, ctx->keep.p, ctx->offs.p, ctx->succ.p, ctx->dst_raw.p, ctx->w_init.p,
                                       ctx->parent.p, src_out, dst_out, w_out, st));
@@ -832,7 +850,9 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
         int seed_rounds = 1 << 30;
         bool sampled = false, cert = true;
         int small_round = fh::small_round_limit();
+        int tie_cycles = 1 << 17;  // below this a round's doubling costs less than the host read
     } tune;
+    int64_t tie_nodes = 0;
     tune.sampled = !qa.wave;
     if (const char *m = std::getenv("FH_SEED_ROUNDS")) tune.seed_rounds = std::atoi(m);
     if (const char *m = std::getenv("FH_SAMPLED")) tune.sampled = std::atoi(m) != 0;
@@ -841,6 +861,10 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     // selection, cycle skip, emission and unions run in one work-group
     if (const char *m = std::getenv("FH_SMALL_ROUND"))
         tune.small_round = std::min(std::atoi(m), fh::small_round_limit());
+    // FH_TIE_CYCLES: the component count from which a round settles its
+    // 2-cycles directly and doubles only over equal-weight chains (a large
+    // value keeps the doubling over every component)
+    if (const char *m = std::getenv("FH_TIE_CYCLES")) tune.tie_cycles = std::atoi(m);
     if (seed) {
         FH_TRY(ensure(ctx, ctx->seed_int, n));
         FH_HIP(hipMemsetAsync(ctx->seed_int.p, 0, (size_t)n, st));
@@ -929,7 +953,20 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
         } else {
             FH_HIP(fh::launch_select(C, ni, ctx->roots.p, ctx->best.p, ctx->partner.p, ctx->posof.p, ctx->perm.p,
                                      ctx->comp_p.p, ctx->rootslot.p, ctx->succ.p, ctx->ei.p, ctx->ej.p, ctx->ew.p, st));
-            FH_HIP(fh::launch_cycle_keep(C, ctx->succ.p, cb, ctx->keep.p, st));
+            if (C >= tune.tie_cycles) {
+                // 2-cycles settled at once; doubling only over the nodes with
+                // an equal-weight successor (flag, offs and dst_raw are free
+                // until the emission below)
+                FH_HIP(fh::launch_cycle_classify(C, ctx->succ.p, ctx->ew.p, ctx->keep.p, ctx->flag.p, st));
+                FH_HIP(fh::launch_excl_scan(C, ctx->flag.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 2, st));
+                FH_HIP(hipMemcpyAsync(ctx->h_small + 2, ctx->small.p + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+                FH_HIP(hipStreamSynchronize(st));
+                tie_nodes += ctx->h_small[2];
+                FH_HIP(fh::launch_cycle_ties(C, ctx->h_small[2], ctx->succ.p, ctx->flag.p, ctx->offs.p, cb,
+                                             ctx->dst_raw.p, ctx->keep.p, st));
+            } else {
+                FH_HIP(fh::launch_cycle_keep(C, ctx->succ.p, cb, ctx->keep.p, st));
+            }
             FH_HIP(fh::launch_excl_scan(C, ctx->keep.p, ctx->offs.p, ctx->tiles.p, ctx->small.p + 0, st));
             FH_HIP(fh::launch_round_emit_union(C, base, ctx->keep.p, ctx->offs.p, ctx->ei.p, ctx->ej.p, ctx->ew.p,
                                                ctx->roots.p, ctx->succ.p, ctx->parent.p, src_out, dst_out, w_out, st));
@@ -980,6 +1017,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     FH_HIP(ctx->timing ? hipEventSynchronize(ctx->ev[7]) : hipStreamSynchronize(st));
 
     S.rounds = rounds;
+    S.tie_cycle_nodes += tie_nodes;
     S.exact_rows = exact_rows;
     S.n_edges = base;
     S.t_tree_ms = span(ctx, 0, 1);

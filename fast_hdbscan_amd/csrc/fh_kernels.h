@@ -98,8 +98,10 @@ struct CycleBufs {
 };
 
 hipError_t launch_knn(const KnnArgs &a, hipStream_t st);
+// core_pt [n]: each point's core distance, by point index
 hipError_t launch_init_edges_pos(int n, int k, const int *nb_pos, const float *nb_d, const float *core_p,
-                                 const int *perm, const int *posof, int *succ, int *dst_raw, float *w, hipStream_t st);
+                                 const int *perm, const int *posof, int *succ, int *dst_raw, float *w, float *core_pt,
+                                 hipStream_t st);
 hipError_t launch_sweep_slice(int64_t n, int K, int ms, const int *sw_pos, const float *sw_d, int *nb_pos,
                               float *nb_d, float *core_p, int *tie_list, int *tie_count, hipStream_t st);
 hipError_t launch_outputs_by_point(int64_t n, int k, const int *posof, const int *perm, const float *core_p,
@@ -120,6 +122,13 @@ hipError_t launch_node_ranges(int64_t N, int n, int2 *range, hipStream_t st);
 int scan_tiles(int m);
 hipError_t launch_excl_scan(int m, const int *in, int *out, int *tile_sum, int *total, hipStream_t st);
 hipError_t launch_cycle_keep(int m, const int *succ, CycleBufs &b, int *keep, hipStream_t st);
+// Keyed form (w never increases along succ): keep[] for every node not
+// flagged, tflag[] = 1 for the nodes that may lie on a cycle longer than 2;
+// then, with tpos = exclusive scan of tflag and t flagged nodes, keep[] for
+// those (pointer doubling over the t of them; tnode [m] scratch).
+hipError_t launch_cycle_classify(int m, const int *succ, const float *w, int *keep, int *tflag, hipStream_t st);
+hipError_t launch_cycle_ties(int m, int t, const int *succ, const int *tflag, const int *tpos, CycleBufs &b,
+                             int *tnode, int *keep, hipStream_t st);
 
 hipError_t launch_init_edges(int n, int k, const int *knn_i, const float *knn_d, const float *core_o, int *succ,
                              int *dst_raw, float *w, hipStream_t st);

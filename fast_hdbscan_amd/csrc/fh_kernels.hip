@@ -738,7 +738,8 @@ __global__ void init_edges_kernel(int n, int k, const int *__restrict__ knn_i, c
 // are written by point index for the emission in point order.
 __global__ void init_edges_pos_kernel(int n, int k, const int *__restrict__ nb_pos, const float *__restrict__ nb_d,
                                       const float *__restrict__ core_p, const int *__restrict__ perm,
-                                      const int *__restrict__ posof, int *succ, int *dst_raw, float *w) {
+                                      const int *__restrict__ posof, int *succ, int *dst_raw, float *w,
+                                      float *core_pt) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const float ci = core_p[p];
@@ -761,6 +762,7 @@ __global__ void init_edges_pos_kernel(int n, int k, const int *__restrict__ nb_p
     succ[i] = s;
     dst_raw[i] = draw;
     w[i] = wi;
+    core_pt[i] = ci;
 }
 
 // One min_samples value of a sweep from shared tree-order kNN rows of width
@@ -844,6 +846,55 @@ __global__ void dbl_keep_kernel(int m, const int *__restrict__ succ, const int *
     const int s = S[v];
     const bool skip = s >= 0 && M[s] == v;
     keep[v] = (succ[v] >= 0 && !skip) ? 1 : 0;
+}
+
+// A functional graph whose key w never increases along succ: in a round, a
+// component's successor never has a heavier best edge (its own best is at
+// most the edge the two share); in the kNN-seeded forest, a point's
+// successor never has a larger core distance (boruvka.py:492-503).  So a
+// cycle longer than two has equal keys all round, and each node on it has a
+// successor of equal key that does not point back.  cyc_classify settles
+// every other node at once -- the two nodes of a 2-cycle keep the edge of
+// the smaller one (the larger key is the one the reference skips), any
+// other node is on no cycle -- and flags the rest (tflag) for pointer
+// doubling over just those (cyc_tinit, dbl_step, cyc_tkeep).
+__global__ void cyc_classify_kernel(int m, const int *__restrict__ succ, const float *__restrict__ w, int *keep,
+                                    int *tflag) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m) return;
+    const int s = succ[v];
+    int k = 0, t = 0;
+    if (s >= 0) {
+        if (succ[s] == v) {
+            k = v < s ? 1 : 0;
+        } else {
+            k = 1;
+            t = w[s] == w[v] ? 1 : 0;
+        }
+    }
+    keep[v] = k;
+    tflag[v] = t;
+}
+
+// Flagged node v becomes entry tpos[v]: its successor's entry when that is
+// flagged too (else -1: the path leaves the flagged set, so v is on no
+// cycle), and its own id as the running maximum.
+__global__ void cyc_tinit_kernel(int m, const int *__restrict__ succ, const int *__restrict__ tflag,
+                                 const int *__restrict__ tpos, int *S, int *M, int *tnode) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m || !tflag[v]) return;
+    const int i = tpos[v], s = succ[v];
+    S[i] = tflag[s] ? tpos[s] : -1;
+    M[i] = v;
+    tnode[i] = v;
+}
+
+__global__ void cyc_tkeep_kernel(int t, const int *__restrict__ S, const int *__restrict__ M,
+                                 const int *__restrict__ tnode, int *keep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t) return;
+    const int v = tnode[i], s = S[i];
+    keep[v] = (s >= 0 && M[s] == v) ? 0 : 1;
 }
 
 // ------------------------------------------------------------ scans ----
@@ -1414,10 +1465,34 @@ hipError_t launch_cycle_keep(int m, const int *succ, CycleBufs &b, int *keep, hi
     return hipGetLastError();
 }
 
+hipError_t launch_cycle_classify(int m, const int *succ, const float *w, int *keep, int *tflag, hipStream_t st) {
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(cyc_classify_kernel, dim3(blocks(m, 256)), dim3(256), 0, st, m, succ, w, keep, tflag);
+    return hipGetLastError();
+}
+
+hipError_t launch_cycle_ties(int m, int t, const int *succ, const int *tflag, const int *tpos, CycleBufs &b,
+                             int *tnode, int *keep, hipStream_t st) {
+    if (t == 0) return hipSuccess;
+    hipLaunchKernelGGL(cyc_tinit_kernel, dim3(blocks(m, 256)), dim3(256), 0, st, m, succ, tflag, tpos, b.S0, b.M0,
+                       tnode);
+    int steps = 0;
+    while ((int64_t(1) << steps) < t) ++steps;
+    int *S = b.S0, *M = b.M0, *S2 = b.S1, *M2 = b.M1;
+    for (int i = 0; i < steps; ++i) {
+        hipLaunchKernelGGL(dbl_step_kernel, dim3(blocks(t, 256)), dim3(256), 0, st, t, S, M, S2, M2);
+        std::swap(S, S2);
+        std::swap(M, M2);
+    }
+    hipLaunchKernelGGL(cyc_tkeep_kernel, dim3(blocks(t, 256)), dim3(256), 0, st, t, S, M, tnode, keep);
+    return hipGetLastError();
+}
+
 hipError_t launch_init_edges_pos(int n, int k, const int *nb_pos, const float *nb_d, const float *core_p,
-                                 const int *perm, const int *posof, int *succ, int *dst_raw, float *w, hipStream_t st) {
+                                 const int *perm, const int *posof, int *succ, int *dst_raw, float *w, float *core_pt,
+                                 hipStream_t st) {
     hipLaunchKernelGGL(init_edges_pos_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, n, k, nb_pos, nb_d, core_p, perm,
-                       posof, succ, dst_raw, w);
+                       posof, succ, dst_raw, w, core_pt);
     return hipGetLastError();
 }
 

@@ -69,6 +69,7 @@ typedef struct {
     double t_learn_ms;        /* learned: features, sample kNN, training       */
     double t_predict_ms;      /* learned: inference + radii                    */
     int64_t scored_tile_pairs; /* high_dim: 128 x 128 tile pairs the Boruvka scans scored, summed */
+    int64_t tie_cycle_nodes;   /* components whose cycle test needed pointer doubling, summed over rounds */
 } fh_stats;
 
 /* Create a context on HIP device `device`.  Returns NULL on failure (no

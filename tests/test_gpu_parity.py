@@ -309,7 +309,8 @@ def test_sample_weight_seam_contract():
     assert e.dtype == np.float64 and nb.dtype == np.int32 and core.dtype == np.float32
 
 
-@pytest.mark.parametrize("form", ["lane", "unpruned", "knn_point_rows", "sampled", "grid_rounds"])
+@pytest.mark.parametrize("form", ["lane", "unpruned", "knn_point_rows", "sampled", "grid_rounds", "tie_cycles",
+                                  "doubling_only"])
 def test_search_kernel_forms(form, monkeypatch):
     """The main search exists as one lane per point (FH_QUERY_KERNEL=lane)
     and as one wavefront per 64 positions (default for d <= 8), and the
@@ -318,7 +319,10 @@ def test_search_kernel_forms(form, monkeypatch):
     (FH_KNN_ROWS=point) or in tree order (default); the sampled pre-pass on
     or off (FH_SAMPLED); the post-search work of every round as grid-wide
     kernels (FH_SMALL_ROUND=0) instead of one work-group once at most 1024
-    components are left.  Each equals the oracle."""
+    components are left; the cycle skip with 2-cycles settled directly and
+    doubling over equal-weight chains only, from any component count
+    (FH_TIE_CYCLES=0), or pointer doubling over every component
+    (doubling_only).  Each equals the oracle."""
     import threading
 
     cases = [workloads.scale_2d(120_000, seed=41), workloads.anisotropic_mixture(60_000, 3, 20, 0.05, seed=42)]
@@ -332,6 +336,11 @@ def test_search_kernel_forms(form, monkeypatch):
         monkeypatch.setenv("FH_SAMPLED", "1")
     elif form == "grid_rounds":
         monkeypatch.setenv("FH_SMALL_ROUND", "0")
+    elif form == "tie_cycles":
+        monkeypatch.setenv("FH_SMALL_ROUND", "0")
+        monkeypatch.setenv("FH_TIE_CYCLES", "0")
+    elif form == "doubling_only":
+        monkeypatch.setenv("FH_TIE_CYCLES", str(1 << 30))
     else:
         monkeypatch.setenv("FH_SEARCH_MODE", "0")
     errors = []
