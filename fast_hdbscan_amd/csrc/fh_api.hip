@@ -86,7 +86,7 @@ struct fh_ctx {
     DevBuf<int> bx_cand, bx_cnt, bx_flags, bx_rowarg, bx_rows, bx_small;
     DevBuf<unsigned int> bx_nmax;
     DevBuf<int> bx_tilecomp;
-    DevBuf<unsigned long long> bx_scored;
+    DevBuf<unsigned long long> bx_scored, bx_rowkey;
     int brute_min_dim = 17;  // d from which the MFMA scans replace the tree searches
     // learned search radii
     DevBuf<float> ml_feat, ml_w, ml_g, ml_m, ml_v, ml_mu, ml_sd, ml_scratch, ml_tgt, ml_pred, ml_bound, ml_stats;
@@ -136,7 +136,7 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->sw_pos, c->sw_ties, c->sw_d, c->nb_d,
         c->mincore,
         c->cert, c->seed_int, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
-        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
+        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->bx_rowkey, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
         c->ml_v, c->ml_mu, c->ml_sd, c->ml_scratch, c->ml_tgt, c->ml_pred, c->ml_bound, c->ml_stats,
         c->ml_rows, c->ml_miss, c->kr_keys, c->kr_keys_s, c->kr_vals, c->kr_vals_s, c->kr_src, c->kr_dst,
         c->kr_w, c->kr_core, c->sk1, c->sk1s, c->sk2, c->sk2s, c->sv1, c->sv1s,
@@ -468,6 +468,13 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
     FH_HIP(hipMemsetAsync(ctx->bx_scored.p, 0, sizeof(unsigned long long), st));
     bsa.tile_comp = ctx->bx_tilecomp.p;
     bsa.scored_tiles = ctx->bx_scored.p;
+    // each tile pair once in the rounds' scans (FH_SYM_SCAN=0: every row
+    // tile against every column tile)
+    const char *sym_env = std::getenv("FH_SYM_SCAN");
+    if (!(sym_env && std::atoi(sym_env) == 0)) {
+        FH_TRY(ensure(ctx, ctx->bx_rowkey, n));
+        bsa.row_key = ctx->bx_rowkey.p;
+    }
     // thresholds: MODE 2 writes them where MODE 0 reads them
     fh::BruteScanArgs kth = bsa;
     kth.row_min = ctx->bx_thr.p;
@@ -899,7 +906,9 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
             // component, exact re-scan of the rows that can reach it
             FH_HIP(fh::launch_tile_comp(n, bsa.npad, ctx->comp_p.p, ctx->bx_tilecomp.p, st));
             FH_HIP(mark(ctx, 8));
+            if (bsa.row_key) FH_HIP(hipMemsetAsync(bsa.row_key, 0xff, sizeof(unsigned long long) * n, st));
             FH_HIP(fh::launch_scan_boruvka(bsa, st));
+            if (bsa.row_key) FH_HIP(fh::launch_unpack_row_keys(n, bsa.row_key, bsa.row_min, bsa.row_arg, st));
             FH_HIP(mark(ctx, 9));
             FH_HIP(fh::launch_boruvka_upper(be, st));
             FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));

@@ -36,6 +36,12 @@ struct BruteScanArgs {
     // the tile pairs scored are added to *scored_tiles
     const int *tile_comp;
     unsigned long long *scored_tiles;
+    // Boruvka, symmetric form (row_key != nullptr): a block scans only the
+    // column tiles from its own on, and each tile pair's column minima go
+    // to the column rows too; per row the packed (value bits << 32 | arg)
+    // minimum, set to ~0 before the scan and unpacked into row_min /
+    // row_arg after it (launch_unpack_row_keys)
+    unsigned long long *row_key;
 };
 
 hipError_t launch_tile_comp(int64_t n, int64_t npad, const int *comp_p, int *tile_comp, hipStream_t st);
@@ -46,6 +52,8 @@ hipError_t launch_center_pack(int64_t n, int d, int dp, int64_t npad, const floa
 hipError_t launch_scan_kth(const BruteScanArgs &a, hipStream_t st);
 hipError_t launch_scan_knn(const BruteScanArgs &a, hipStream_t st);
 hipError_t launch_scan_boruvka(const BruteScanArgs &a, hipStream_t st);
+hipError_t launch_unpack_row_keys(int64_t n, const unsigned long long *row_key, float *row_min, int *row_arg,
+                                  hipStream_t st);
 
 struct KnnRescoreArgs {
     const float *xp;

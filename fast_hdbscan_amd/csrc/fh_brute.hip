@@ -106,6 +106,10 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     __shared__ float col_core[2][128];
     __shared__ int col_comp[2][128];
     __shared__ int cnt_l[128];
+    // MODE 1, symmetric: per wave and column, the column's minimum over the
+    // wave's 32 rows (double-buffered like the tiles)
+    __shared__ float cmin_s[MODE == 1 ? 2 : 1][4][128];
+    __shared__ int carg_s[MODE == 1 ? 2 : 1][4][128];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, c16 = lane & 15;
@@ -122,7 +126,8 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     }
     // the 8 output rows this lane holds: rbase + rb*16 + 4g + r
     float rn[2][4], rv[2][4], rcore[2][4];
-    int ri[2][4], rcomp[2][4];
+    int ri[2][4], rcomp[2][4];  // MODE 1: rcomp -2 marks a padding row
+    const bool sym = MODE == 1 && a.row_key != nullptr;
     // MODE 2: per row the lane's 4 smallest values, t[..][0] <= ... <= t[..][3]
     float t4[2][4][4];
 #pragma unroll
@@ -192,7 +197,8 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     };
 
     // the tiles scanned, in order: t (in buffer cur), then t1, then t2
-    int t = next_tile(0);
+    // (symmetric: from this block's own tile on)
+    int t = next_tile(sym ? (int)blockIdx.x : 0);
     if (t < ntiles) {
         gload(t);
         lstore(0);
@@ -263,6 +269,8 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
             } else {
                 const float cc = col_core[cur][cl];
                 const int ck = col_comp[cur][cl];
+                float cv = kInfB;  // this column's minimum over the lane's rows
+                int ca = -1;
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -270,11 +278,33 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                         const float approx = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
                         float m = rcore[rb][r] > cc ? rcore[rb][r] : cc;
                         m = approx > m ? approx : m;
-                        if (valid && ck != rcomp[rb][r] && m < rv[rb][r]) {
+                        const bool cand = valid && ck != rcomp[rb][r];
+                        if (cand && m < rv[rb][r]) {
                             rv[rb][r] = m;
                             ri[rb][r] = (int)col;
                         }
+                        // rows in increasing order: the first equal one is kept
+                        if (sym && cand && rcomp[rb][r] != -2 && m < cv) {
+                            cv = m;
+                            ca = (int)(rbase + rb * 16 + 4 * g + r);
+                        }
                     }
+                if (sym) {
+                    // the 4 lanes holding this column's other rows of the wave
+#pragma unroll
+                    for (int o = 16; o < 64; o <<= 1) {
+                        const float ov = __shfl_xor(cv, o, 64);
+                        const int oa = __shfl_xor(ca, o, 64);
+                        if (ov < cv || (ov == cv && (unsigned)oa < (unsigned)ca)) {
+                            cv = ov;
+                            ca = oa;
+                        }
+                    }
+                    if (g == 0) {
+                        cmin_s[cur][wave][cl] = cv;
+                        carg_s[cur][wave][cl] = ca;
+                    }
+                }
             }
         }
         // tile t1 into the other buffer (last read in the previous
@@ -286,6 +316,25 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
             if (t2 < ntiles) gload(t2);
         }
         __syncthreads();
+        if (sym && t != (int)blockIdx.x && tid < 128) {
+            // tile t's column minima over this block's 128 rows, to the
+            // column rows (the pair (t, this tile) is scanned here only);
+            // buffer cur is written again only after the next barrier
+            float v = cmin_s[cur][0][tid];
+            int i = carg_s[cur][0][tid];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) {
+                const float ov = cmin_s[cur][w][tid];
+                const int oi = carg_s[cur][w][tid];
+                if (ov < v || (ov == v && (unsigned)oi < (unsigned)i)) {
+                    v = ov;
+                    i = oi;
+                }
+            }
+            const int64_t col = (int64_t)t * 128 + tid;
+            if (col < n && i >= 0)
+                atomicMin(a.row_key + col, ((unsigned long long)__float_as_uint(v) << 32) | (unsigned int)i);
+        }
         t = t1;
         t1 = t2;
     }
@@ -345,8 +394,13 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                 }
                 const int64_t row = rbase + rb * 16 + 4 * g + r;
                 if (c16 == 0 && row < n) {
-                    a.row_min[row] = v;
-                    a.row_arg[row] = i;
+                    if (sym) {
+                        if (i >= 0)
+                            atomicMin(a.row_key + row, ((unsigned long long)__float_as_uint(v) << 32) | (unsigned int)i);
+                    } else {
+                        a.row_min[row] = v;
+                        a.row_arg[row] = i;
+                    }
                 }
             }
     }
@@ -626,6 +680,27 @@ static hipError_t launch_scan(const BruteScanArgs &a, hipStream_t st) {
 hipError_t launch_scan_knn(const BruteScanArgs &a, hipStream_t st) { return launch_scan<0>(a, st); }
 hipError_t launch_scan_kth(const BruteScanArgs &a, hipStream_t st) { return launch_scan<2>(a, st); }
 hipError_t launch_scan_boruvka(const BruteScanArgs &a, hipStream_t st) { return launch_scan<1>(a, st); }
+
+__global__ void unpack_row_keys_kernel(int64_t n, const unsigned long long *__restrict__ row_key, float *row_min,
+                                       int *row_arg) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long key = row_key[p];
+    if (key == ~0ull) {
+        row_min[p] = kInfB;
+        row_arg[p] = -1;
+    } else {
+        row_min[p] = __uint_as_float((unsigned int)(key >> 32));
+        row_arg[p] = (int)(unsigned int)(key & 0xffffffffull);
+    }
+}
+
+hipError_t launch_unpack_row_keys(int64_t n, const unsigned long long *row_key, float *row_min, int *row_arg,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL(unpack_row_keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, row_key,
+                       row_min, row_arg);
+    return hipGetLastError();
+}
 
 // Component of every 128-row tile whose rows (below n) share one, else -1.
 __global__ __launch_bounds__(128) void tile_comp_kernel(int64_t n, const int *__restrict__ comp_p, int *tile_comp) {
