@@ -475,12 +475,17 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
         FH_TRY(ensure(ctx, ctx->bx_rowkey, n));
         bsa.row_key = ctx->bx_rowkey.p;
     }
-    // thresholds: MODE 2 writes them where MODE 0 reads them
+    // thresholds: MODE 2 writes them where MODE 0 reads them; the
+    // candidates are collected once per tile pair, in both directions
+    // (FH_SYM_SCAN=0: every pair from both sides)
     fh::BruteScanArgs kth = bsa;
     kth.row_min = ctx->bx_thr.p;
     kth.cap = kq;
     FH_HIP(fh::launch_scan_kth(kth, st));
-    FH_HIP(fh::launch_scan_knn(bsa, st));
+    fh::BruteScanArgs kc = bsa;
+    kc.sym_knn = bsa.row_key != nullptr ? 1 : 0;
+    if (kc.sym_knn) FH_HIP(hipMemsetAsync(ctx->bx_cnt.p, 0, sizeof(int) * n, st));
+    FH_HIP(fh::launch_scan_knn(kc, st));
     fh::KnnRescoreArgs ra{};
     ra.xp = ctx->xp.p;
     ra.perm = ctx->perm.p;

@@ -42,6 +42,10 @@ struct BruteScanArgs {
     // minimum, set to ~0 before the scan and unpacked into row_min /
     // row_arg after it (launch_unpack_row_keys)
     unsigned long long *row_key;
+    // kNN candidates (MODE 0), symmetric form (sym_knn): a block scans only
+    // the column tiles from its own on and appends in both directions,
+    // counting in cand_cnt (zeroed before the scan)
+    int sym_knn;
 };
 
 hipError_t launch_tile_comp(int64_t n, int64_t npad, const int *comp_p, int *tile_comp, hipStream_t st);

@@ -105,6 +105,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     __shared__ float col_nrm[2][128];
     __shared__ float col_core[2][128];
     __shared__ int col_comp[2][128];
+    __shared__ float col_thr[2][128];
     __shared__ int cnt_l[128];
     // MODE 1, symmetric: per wave and column, the column's minimum over the
     // wave's 32 rows (double-buffered like the tiles)
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     // the 8 output rows this lane holds: rbase + rb*16 + 4g + r
     float rn[2][4], rv[2][4], rcore[2][4];
     int ri[2][4], rcomp[2][4];  // MODE 1: rcomp -2 marks a padding row
-    const bool sym = MODE == 1 && a.row_key != nullptr;
+    const bool sym = (MODE == 1 && a.row_key != nullptr) || (MODE == 0 && a.sym_knn != 0);
     // MODE 2: per row the lane's 4 smallest values, t[..][0] <= ... <= t[..][3]
     float t4[2][4][4];
 #pragma unroll
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
             if constexpr (MODE == 0) {
                 rv[rb][r] = ok ? a.thr[row] : -kInfB;
                 rcore[rb][r] = 0.0f;
-                rcomp[rb][r] = 0;
+                rcomp[rb][r] = ok ? 0 : -2;
             } else if constexpr (MODE == 2) {
                 rv[rb][r] = kInfB;
                 rcore[rb][r] = 0.0f;
@@ -161,6 +162,8 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     if (MODE == 0 && tid < 128) cnt_l[tid] = 0;
 
     const int ntiles = (int)(a.npad / 128);
+    // the column tiles scanned: [tstart, ntiles)
+    const int tstart = sym ? (int)blockIdx.x : 0;
     // MODE 1: column tiles whose rows all share this tile's single
     // component hold no candidate (boruvka.py:199-201) and are skipped
     const int my_tc = (MODE == 1 && a.tile_comp) ? a.tile_comp[blockIdx.x] : -1;
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
         return t;
     };
     floatx4 pre[NV];
-    float pn = 0.0f, pc = 0.0f;
+    float pn = 0.0f, pc = 0.0f, pt = 0.0f;
     int pk = -1;
     auto gload = [&](int t) {
         const floatx4 *src = reinterpret_cast<const floatx4 *>(a.xc + (int64_t)t * 128 * DP);
@@ -183,6 +186,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                 pc = col < n ? a.core_p[col] : 0.0f;
                 pk = col < n ? a.comp_p[col] : -3;
             }
+            if constexpr (MODE == 0) pt = (sym && col < n) ? a.thr[col] : -kInfB;
         }
     };
     auto lstore = [&](int buf) {
@@ -193,12 +197,13 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
             col_nrm[buf][tid] = pn;
             col_core[buf][tid] = pc;
             col_comp[buf][tid] = pk;
+            col_thr[buf][tid] = pt;
         }
     };
 
     // the tiles scanned, in order: t (in buffer cur), then t1, then t2
     // (symmetric: from this block's own tile on)
-    int t = next_tile(sym ? (int)blockIdx.x : 0);
+    int t = next_tile(tstart);
     if (t < ntiles) {
         gload(t);
         lstore(0);
@@ -255,15 +260,29 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                         }
                     }
             } else if constexpr (MODE == 0) {
+                const float ct = col_thr[cur][cl];
+                const bool other = t != (int)blockIdx.x;
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const float approx = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
+                        const int rl = wave * 32 + rb * 16 + 4 * g + r;
                         if (valid && approx <= rv[rb][r]) {
-                            const int rl = wave * 32 + rb * 16 + 4 * g + r;
-                            const int slot = atomicAdd(&cnt_l[rl], 1);
-                            if (slot < a.cap) a.cand[(blk_row + rl) * a.cap + slot] = (int)col;
+                            if (sym) {
+                                const int slot = atomicAdd(a.cand_cnt + blk_row + rl, 1);
+                                if (slot < a.cap) a.cand[(blk_row + rl) * a.cap + slot] = (int)col;
+                            } else {
+                                const int slot = atomicAdd(&cnt_l[rl], 1);
+                                if (slot < a.cap) a.cand[(blk_row + rl) * a.cap + slot] = (int)col;
+                            }
+                        }
+                        // the same pair for the column's row (tiles other
+                        // than the block's own; the own tile gives both
+                        // orders on the row side)
+                        if (sym && other && valid && rcomp[rb][r] != -2 && approx <= ct) {
+                            const int slot = atomicAdd(a.cand_cnt + col, 1);
+                            if (slot < a.cap) a.cand[col * a.cap + slot] = (int)(blk_row + rl);
                         }
                     }
             } else {
@@ -342,7 +361,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
 
     if constexpr (MODE == 0) {
         __syncthreads();
-        if (tid < 128 && blk_row + tid < n) a.cand_cnt[blk_row + tid] = cnt_l[tid];
+        if (!sym && tid < 128 && blk_row + tid < n) a.cand_cnt[blk_row + tid] = cnt_l[tid];
     } else if constexpr (MODE == 2) {
         // gather the 16 lanes' lists per row, then one thread per row picks
         // the k-th smallest of its 64 values

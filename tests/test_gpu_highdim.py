@@ -42,6 +42,16 @@ def test_highdim_matches_oracle(d, n, ms):
     print(st["knn_tree_rows"], st["exact_rows"], st["rounds"], flush=True)
 
 
+@pytest.mark.parametrize("sym", ["1", "0"])
+def test_highdim_tile_pairs_once_or_twice(monkeypatch, sym):
+    """The candidate and round scans take each 128 x 128 tile pair once
+    (default: row and column minima / candidates from one product) or from
+    both sides (FH_SYM_SCAN=0): both equal the oracle."""
+    monkeypatch.setenv("FH_SYM_SCAN", sym)
+    X = workloads.gaussian_highdim(7000, 40, n_clusters=12, seed=17)
+    _check(X, 7)
+
+
 def test_highdim_duplicates():
     # exact ties: duplicated rows send their kNN rows to the tree search and
     # make mutual-reachability ties that the depth-first order resolves
