@@ -111,6 +111,10 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     // wave's 32 rows (double-buffered like the tiles)
     __shared__ float cmin_s[MODE == 1 ? 2 : 1][4][128];
     __shared__ int carg_s[MODE == 1 ? 2 : 1][4][128];
+    // MODE 1: the block's rows' core distances and components (read per
+    // product from LDS rather than held in registers)
+    __shared__ float row_core_s[MODE == 1 ? 128 : 1];
+    __shared__ int row_comp_s[MODE == 1 ? 128 : 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, c16 = lane & 15;
@@ -155,11 +159,16 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                 rcomp[rb][r] = 0;
             } else {
                 rv[rb][r] = kInfB;
-                rcore[rb][r] = ok ? a.core_p[row] : 0.0f;
-                rcomp[rb][r] = ok ? a.comp_p[row] : -2;
+                rcore[rb][r] = 0.0f;
+                rcomp[rb][r] = 0;
             }
         }
     if (MODE == 0 && tid < 128) cnt_l[tid] = 0;
+    if (MODE == 1 && tid < 128) {
+        const int64_t row = blk_row + tid;
+        row_core_s[tid] = row < n ? a.core_p[row] : 0.0f;
+        row_comp_s[tid] = row < n ? a.comp_p[row] : -2;
+    }
 
     const int ntiles = (int)(a.npad / 128);
     // the column tiles scanned: [tstart, ntiles)
@@ -295,15 +304,18 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const float approx = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
-                        float m = rcore[rb][r] > cc ? rcore[rb][r] : cc;
+                        const int rl = wave * 32 + rb * 16 + 4 * g + r;
+                        const float rc = row_core_s[rl];
+                        const int rk = row_comp_s[rl];
+                        float m = rc > cc ? rc : cc;
                         m = approx > m ? approx : m;
-                        const bool cand = valid && ck != rcomp[rb][r];
+                        const bool cand = valid && ck != rk;
                         if (cand && m < rv[rb][r]) {
                             rv[rb][r] = m;
                             ri[rb][r] = (int)col;
                         }
                         // rows in increasing order: the first equal one is kept
-                        if (sym && cand && rcomp[rb][r] != -2 && m < cv) {
+                        if (sym && cand && rk != -2 && m < cv) {
                             cv = m;
                             ca = (int)(rbase + rb * 16 + 4 * g + r);
                         }
