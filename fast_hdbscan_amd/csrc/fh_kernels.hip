@@ -106,23 +106,41 @@ struct KnnVisitor {
 
     __device__ __forceinline__ bool enter(int, float lb) const { return !(lb > hmax); }
 
+    __device__ __forceinline__ void consider(float dist, int i, bool bounded) {
+        if (dist < hmax) {
+            if (bounded && h.P(0) < kInf && h.k > 1) {
+                float c = h.P(1);
+                if (h.k > 2) c = h.P(2) > c ? h.P(2) : c;
+                if (c == h.P(0)) tie = 1;  // which equal element leaves depends on the layout
+            }
+            h.push(dist, i);
+            hmax = h.P(0) < bound ? h.P(0) : bound;
+        } else if (bounded && dist == hmax && hmax == h.P(0)) {
+            tie = 1;
+        }
+    }
+
+    // The leaf's points in position order; for a compile-time dimension
+    // their rows are loaded kLeafBatch at a time ahead of the distances, so
+    // the loads overlap (the pushes still follow position order).
     __device__ __forceinline__ void leaf(int node) {
         const int2 r = range[node];
-        const int dd = D > 0 ? D : d;
         const bool bounded = bound < kInf;
-        for (int i = r.x; i < r.y; ++i) {
-            const float dist = rdist<D>(x.p(), xp + (size_t)i * dd, d);
-            if (dist < hmax) {
-                if (bounded && h.P(0) < kInf && h.k > 1) {
-                    float c = h.P(1);
-                    if (h.k > 2) c = h.P(2) > c ? h.P(2) : c;
-                    if (c == h.P(0)) tie = 1;  // which equal element leaves depends on the layout
-                }
-                h.push(dist, i);
-                hmax = h.P(0) < bound ? h.P(0) : bound;
-            } else if (bounded && dist == hmax && hmax == h.P(0)) {
-                tie = 1;
+        if constexpr (D > 0) {
+            constexpr int kLeafBatch = 8;
+            int i0 = r.x;
+            for (; i0 + kLeafBatch <= r.y; i0 += kLeafBatch) {
+                float pt[kLeafBatch][D];
+#pragma unroll
+                for (int u = 0; u < kLeafBatch; ++u)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) pt[u][j] = xp[(size_t)(i0 + u) * D + j];
+#pragma unroll
+                for (int u = 0; u < kLeafBatch; ++u) consider(rdist<D>(x.p(), pt[u], D), i0 + u, bounded);
             }
+            for (int i = i0; i < r.y; ++i) consider(rdist<D>(x.p(), xp + (size_t)i * D, D), i, bounded);
+        } else {
+            for (int i = r.x; i < r.y; ++i) consider(rdist<D>(x.p(), xp + (size_t)i * d, d), i, bounded);
         }
     }
 };
