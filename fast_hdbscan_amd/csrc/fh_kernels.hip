@@ -138,6 +138,16 @@ struct KnnVisitor {
 #pragma unroll
                 for (int u = 0; u < kLeafBatch; ++u) consider(rdist<D>(x.p(), pt[u], D), i0 + u, bounded);
             }
+            if (i0 + 4 <= r.y) {
+                float pt[4][D];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) pt[u][j] = xp[(size_t)(i0 + u) * D + j];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) consider(rdist<D>(x.p(), pt[u], D), i0 + u, bounded);
+                i0 += 4;
+            }
             for (int i = i0; i < r.y; ++i) consider(rdist<D>(x.p(), xp + (size_t)i * D, D), i, bounded);
         } else {
             for (int i = r.x; i < r.y; ++i) consider(rdist<D>(x.p(), xp + (size_t)i * d, d), i, bounded);
