@@ -375,15 +375,30 @@ __global__ void seed_bounds_kernel(int64_t n, int kn, const int *__restrict__ nb
         c = comp_p[p];
         const float ci = core_p[p];
         bool cross = false;
-        for (int j = 0; j < kn; ++j) {
-            const int q = nb_pos[p * kn + j];
-            if (q < 0 || comp_p[q] == c) continue;
-            cross = true;
-            float e = nb_d[p * kn + j];
-            e = e > ci ? e : ci;
-            const float cq = core_p[q];
-            e = e > cq ? e : cq;
-            w = e < w ? e : w;
+        // neighbours four at a time: positions first, then their labels and
+        // core distances together
+        for (int j0 = 0; j0 < kn; j0 += 4) {
+            int q[4], cq[4];
+            float dq[4], kq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                q[u] = j0 + u < kn ? nb_pos[p * kn + j0 + u] : -1;
+                dq[u] = j0 + u < kn ? nb_d[p * kn + j0 + u] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                cq[u] = q[u] >= 0 ? comp_p[q[u]] : c;
+                kq[u] = q[u] >= 0 ? core_p[q[u]] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (q[u] < 0 || cq[u] == c) continue;
+                cross = true;
+                float e = dq[u];
+                e = e > ci ? e : ci;
+                e = e > kq[u] ? e : kq[u];
+                w = e < w ? e : w;
+            }
         }
         if (!cross) interior[p] = 1;
     }
@@ -657,11 +672,11 @@ __global__ __launch_bounds__(256) void boruvka_wave_kernel(QueryArgs a) {
                     wave_sync();
                     if (t) {
                         B = ld_relaxed(a.bound + comp);
-                        for (int q = 0; q < cnt; ++q) {
-                            if (sk[q] == comp) continue;
-                            const float cj = sc[q];
-                            if (!(cj <= B)) continue;
-                            float dist = rdist<D>(x, sx + q * D, D);
+                        // staged points read four at a time ahead of the tests
+                        // (still taken in position order)
+                        auto scan = [&](int q, int kq, float cj, const float *y) {
+                            if (kq == comp || !(cj <= B)) return;
+                            float dist = rdist<D>(x, y, D);
                             dist = dist > core_i ? dist : core_i;
                             dist = dist > cj ? dist : cj;
                             if (dist < best) {
@@ -675,7 +690,22 @@ __global__ __launch_bounds__(256) void boruvka_wave_kernel(QueryArgs a) {
                                        dfs_first<D>(x, r.x + q, best_j, a.range, a.box, levels)) {
                                 best_j = r.x + q;
                             }
+                        };
+                        int q = 0;
+                        for (; q + 4 <= cnt; q += 4) {
+                            int kq[4];
+                            float cq[4], yq[4][D];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                kq[u] = sk[q + u];
+                                cq[u] = sc[q + u];
+#pragma unroll
+                                for (int j = 0; j < D; ++j) yq[u][j] = sx[(q + u) * D + j];
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) scan(q + u, kq[u], cq[u], yq[u]);
                         }
+                        for (; q < cnt; ++q) scan(q, sk[q], sc[q], sx + q * D);
                     }
                     wave_sync();
                 } else {
