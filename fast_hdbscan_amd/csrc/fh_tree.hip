@@ -140,15 +140,32 @@ __global__ __launch_bounds__(kTB) void bounds_kernel(LevelRef lv, const float *_
             mn[j] = 0xffffffffu;
             mx[j] = 0u;
         }
-        for (int p = s + threadIdx.x; p < e; p += kTB) {
-            const uint32_t idx = identity ? (uint32_t)p : key_idx(A[p]);
-            const float *row = X + (size_t)idx * d;
+        // kBatch positions per lane at a time: their keys, then their rows,
+        // so that the dependent loads overlap
+        constexpr int kBatch = 8;
+        for (int p0 = s + threadIdx.x; p0 < e; p0 += kBatch * kTB) {
+            uint32_t idx[kBatch];
 #pragma unroll
-            for (int j = 0; j < kRegDims; ++j) {
-                if (j < d) {
-                    const uint32_t o = f2o(row[j]);
-                    mn[j] = min(mn[j], o);
-                    mx[j] = max(mx[j], o);
+            for (int u = 0; u < kBatch; ++u) {
+                const int p = p0 + u * kTB;
+                idx[u] = p < e ? (identity ? (uint32_t)p : key_idx(A[p])) : 0xffffffffu;
+            }
+            float v[kBatch][kRegDims];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+#pragma unroll
+                for (int j = 0; j < kRegDims; ++j)
+                    if (j < d && idx[u] != 0xffffffffu) v[u][j] = X[(size_t)idx[u] * d + j];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                if (idx[u] == 0xffffffffu) continue;
+#pragma unroll
+                for (int j = 0; j < kRegDims; ++j) {
+                    if (j < d) {
+                        const uint32_t o = f2o(v[u][j]);
+                        mn[j] = min(mn[j], o);
+                        mx[j] = max(mx[j], o);
+                    }
                 }
             }
         }
@@ -260,9 +277,23 @@ __global__ __launch_bounds__(kTB) void keys_kernel(LevelRef lv, const float *__r
     if (s >= r.y) return;
     const int e = min(r.y, s + kChunk);
     const int ax = axis[k];
-    for (int p = s + threadIdx.x; p < e; p += kTB) {
-        const uint32_t idx = identity ? (uint32_t)p : key_idx(A[p]);
-        A[p] = make_key(X[(size_t)idx * d + ax], idx);
+    // all of the lane's positions at once: keys, then coordinates, then stores
+    uint32_t idx[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int p = s + threadIdx.x + u * kTB;
+        idx[u] = p < e ? (identity ? (uint32_t)p : key_idx(A[p])) : 0u;
+    }
+    float v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int p = s + threadIdx.x + u * kTB;
+        v[u] = p < e ? X[(size_t)idx[u] * d + ax] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int p = s + threadIdx.x + u * kTB;
+        if (p < e) A[p] = make_key(v[u], idx[u]);
     }
 }
 
