@@ -33,6 +33,12 @@ constexpr float kInf = __builtin_huge_valf();
 // per depth remembers which child went first; on the way back the sibling's
 // bound is recomputed with the same arithmetic, so the walk makes exactly the
 // reference recursion's sequence of enter/prune decisions.
+//
+// Each lane walks on its own until it reaches the next leaf it enters (or
+// ends); the lanes of a wavefront then scan their leaves together.  Lanes
+// whose walks drift apart would otherwise run the leaf scan in almost every
+// step of the loop, a few lanes at a time.  A lane's own sequence of
+// decisions is unchanged: its bound only changes in its leaf scans.
 template <int D, class V>
 __device__ __forceinline__ void walk(const float *__restrict__ box, int levels, int d, V &v) {
     const float *x = v.x.p();
@@ -42,10 +48,10 @@ __device__ __forceinline__ void walk(const float *__restrict__ box, int levels, 
     float lb = box_lb<D>(x, box, d);
     bool entering = true;
     for (;;) {
-        if (entering && v.enter(node, lb)) {
-            if (depth == levels - 1) {
-                v.leaf(node);
-            } else {
+        bool done = false;
+        for (;;) {
+            if (entering && v.enter(node, lb)) {
+                if (depth == levels - 1) break;
                 const int l = 2 * node + 1;
                 const float lbl = box_lb<D>(x, box + (size_t)l * stride, d);
                 const float lbr = box_lb<D>(x, box + (size_t)(l + 1) * stride, d);
@@ -56,19 +62,25 @@ __device__ __forceinline__ void walk(const float *__restrict__ box, int levels, 
                 lb = rf ? lbr : lbl;
                 continue;
             }
+            if (depth == 0) {
+                done = true;
+                break;
+            }
+            const bool rf = (first_right >> depth) & 1u;
+            const bool is_right = (node & 1) == 0;
+            if (is_right == rf) {
+                node = is_right ? node - 1 : node + 1;
+                lb = box_lb<D>(x, box + (size_t)node * stride, d);
+                entering = true;
+            } else {
+                node = (node - 1) >> 1;
+                --depth;
+                entering = false;
+            }
         }
-        if (depth == 0) break;
-        const bool rf = (first_right >> depth) & 1u;
-        const bool is_right = (node & 1) == 0;
-        if (is_right == rf) {
-            node = is_right ? node - 1 : node + 1;
-            lb = box_lb<D>(x, box + (size_t)node * stride, d);
-            entering = true;
-        } else {
-            node = (node - 1) >> 1;
-            --depth;
-            entering = false;
-        }
+        if (done) break;
+        v.leaf(node);
+        entering = false;  // then on as from a finished subtree
     }
 }
 
