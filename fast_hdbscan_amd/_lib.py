@@ -48,6 +48,7 @@ EXPORTED_SYMBOLS = (
     "fh_core_graph",
     "fh_csr_boruvka",
     "fh_hdbscan_batch",
+    "fh_mst_batch",
 )
 
 
@@ -218,6 +219,8 @@ def load() -> ctypes.CDLL:
         L.fh_csr_boruvka.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fh_hdbscan_batch.restype = ctypes.c_int
         L.fh_hdbscan_batch.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _i64, _i32, _vp, _vp, _vp]
+        L.fh_mst_batch.restype = ctypes.c_int
+        L.fh_mst_batch.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp]
         L.fh_sort_mst_device.restype = ctypes.c_int
         L.fh_sort_mst_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fh_edges_to_tree_order.restype = ctypes.c_int

@@ -3,9 +3,15 @@
 ``hdbscan_batch`` runs, for every dataset, what fast_hdbscan(X_i,
 min_samples, min_cluster_size) returns for Euclidean data (labels and
 membership strengths, hdbscan.py:199-336), through the native batch
-scheduler (``fh_hdbscan_batch``): worker threads with one context and HIP
-stream each, so the datasets' MST kernels overlap on the GPU and their
-label stages on the host.
+scheduler (``fh_hdbscan_batch``): datasets of up to a few thousand points
+run in one persistent launch per dimension (a work-group per dataset, taken
+from a device-side queue), the others on worker threads with one context
+and HIP stream each, and the label stages on the host threads.
+
+``mst_batch`` returns the MSTs alone (``fh_mst_batch``): what
+compute_minimum_spanning_tree (hdbscan.py:339-494) computes per dataset,
+as squared-weight rows in the reference's order plus squared core
+distances, and which datasets the persistent launch served.
 """
 from __future__ import annotations
 
@@ -40,3 +46,29 @@ def hdbscan_batch(datasets, min_samples=5, min_cluster_size=20, workers=8, devic
     if times is not None:
         times.extend(ms[:count].tolist())
     return list(zip(labels, probs))
+
+
+def mst_batch(datasets, min_samples=5, device=0):
+    """Returns a list of (src int32[n-1], dst int32[n-1], w_sq float32[n-1],
+    core_sq float32[n], persistent bool) per dataset (n >= 1)."""
+    Xs = [np.ascontiguousarray(X, dtype=np.float32) for X in datasets]
+    count = len(Xs)
+    for X in Xs:
+        if X.ndim != 2 or X.shape[0] < 1:
+            raise ValueError("every dataset must be a 2-D array with at least one row")
+    m = [max(X.shape[0] - 1, 0) for X in Xs]
+    src = [np.empty(max(k, 1), np.int32) for k in m]
+    dst = [np.empty(max(k, 1), np.int32) for k in m]
+    w = [np.empty(max(k, 1), np.float32) for k in m]
+    core = [np.empty(X.shape[0], np.float32) for X in Xs]
+    arr = lambda a: (ctypes.c_void_p * max(count, 1))(*[x.ctypes.data for x in a])
+    ns = np.array([X.shape[0] for X in Xs], np.int64)
+    ds = np.array([X.shape[1] for X in Xs], np.int32)
+    pers = np.zeros(max(count, 1), np.int32)
+    code = _lib.load().fh_mst_batch(device, count, ctypes.cast(arr(Xs), ctypes.c_void_p), _lib.ptr(ns),
+                                    _lib.ptr(ds), int(min_samples), ctypes.cast(arr(src), ctypes.c_void_p),
+                                    ctypes.cast(arr(dst), ctypes.c_void_p), ctypes.cast(arr(w), ctypes.c_void_p),
+                                    ctypes.cast(arr(core), ctypes.c_void_p), _lib.ptr(pers))
+    if code != 0:
+        raise _lib.FHError(f"fh_mst_batch failed with code {code}")
+    return [(src[i][:m[i]], dst[i][:m[i]], w[i][:m[i]], core[i], bool(pers[i])) for i in range(count)]

@@ -17,6 +17,7 @@
 #include <thread>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -27,6 +28,7 @@
 #include <vector>
 
 #include "../../include/fh.h"
+#include "fh_batch.h"
 #include "fh_brute.h"
 #include "fh_csr.h"
 #include "fh_kernels.h"
@@ -107,6 +109,14 @@ struct fh_ctx {
     DevBuf<int> pc_parent, pc_vals, pc_vals_s, pc_counts, pc_runs;
     DevBuf<unsigned char> pc_temp;
     DevBuf<unsigned char> stemp;
+    // persistent batch kernel: packed rows, jobs, outputs, per-work-group
+    // scratch; a pinned host buffer stages the rows up and the outputs down
+    DevBuf<float> bt_x, bt_w, bt_core;
+    DevBuf<int> bt_src, bt_dst, bt_status, bt_next;
+    DevBuf<fh::BatchJob> bt_jobs;
+    DevBuf<unsigned char> bt_scratch;
+    unsigned char *h_bt = nullptr;
+    size_t h_bt_cap = 0;
     int64_t geom_n = -1;            // n the uploaded node ranges belong to
     int64_t tree_n = -1;            // n of the last tree built (perm, posof)
     int geom_maxsize[32] = {};      // largest node per level
@@ -141,7 +151,8 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->ml_rows, c->ml_miss, c->kr_keys, c->kr_keys_s, c->kr_vals, c->kr_vals_s, c->kr_src, c->kr_dst,
         c->kr_w, c->kr_core, c->sk1, c->sk1s, c->sk2, c->sk2s, c->sv1, c->sv1s,
         c->pc_keys, c->pc_keys_s, c->pc_parent, c->pc_vals, c->pc_vals_s, c->pc_counts, c->pc_runs, c->pc_temp,
-        c->stemp);
+        c->stemp, c->bt_x, c->bt_w, c->bt_core, c->bt_src, c->bt_dst, c->bt_status, c->bt_next, c->bt_jobs,
+        c->bt_scratch);
 }
 
 }  // namespace
@@ -1048,6 +1059,169 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     return FH_OK;
 }
 
+// ---------------------------------------------- persistent batch ----
+//
+// Datasets small enough for one work-group (fh_batch.hip) run in one
+// persistent launch per dimension: their rows are packed into a pinned
+// buffer, uploaded with one copy, and the launch's work-groups take them
+// from a device-side queue, costliest first; the rows of every MST, the
+// core distances and a status per dataset come back with one copy each.
+
+// Whether the persistent kernel takes a dataset (FH_BATCH_PERSISTENT=0:
+// none): at most one work-group's tree in LDS, min_samples + 1 within its
+// heap size, and a tree whose leaves all sit on the last level.
+bool batch_eligible(int64_t n, int d, int ms) {
+    if (const char *m = std::getenv("FH_BATCH_PERSISTENT"))
+        if (std::atoi(m) == 0) return false;
+    if (d < 1 || n < 2 || n > fh::batch_max_points(d)) return false;
+    const int kq = ms > 1 ? ms + 1 : 2;
+    if (kq > fh::batch_max_kq()) return false;
+    const int levels = fh::tree_levels(n, kLeafSize);
+    return !(levels >= 2 && (n >> (levels - 2)) < 2);
+}
+
+// One persistent launch: the datasets idx (all of dimension d) and where
+// their results land in the context's pinned buffer.
+struct BatchGroup {
+    fh_ctx *ctx = nullptr;
+    int d = 0;
+    std::vector<int> idx;             // dataset indices in queue order
+    std::vector<fh::BatchJob> jobs;
+    int64_t total_n = 0, total_rows = 0;
+    int grid = 0, max_n = 0, max_nodes = 0;
+    int64_t slice = 0;
+    const int *src = nullptr, *dst = nullptr, *status = nullptr;
+    const float *w = nullptr, *core = nullptr;
+};
+
+// Lays the group out and sizes its buffers (pinned and device): done for
+// every group before any launch, since freeing a buffer to grow it waits
+// for the whole device.
+int batch_group_prepare(BatchGroup &g, const int64_t *n, int ms) {
+    fh_ctx *ctx = g.ctx;
+    int cus = 0;
+    FH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int d = g.d;
+    const int kq = ms > 1 ? ms + 1 : 2;
+    std::stable_sort(g.idx.begin(), g.idx.end(), [&](int a, int b) { return n[a] > n[b]; });
+    const int m = (int)g.idx.size();
+    g.jobs.resize(m);
+    int64_t xo = 0, ro = 0, po = 0;
+    int max_n = 0, max_nodes = 1;
+    for (int j = 0; j < m; ++j) {
+        const int64_t ni = n[g.idx[j]];
+        fh::BatchJob &b = g.jobs[j];
+        b.x_off = xo;
+        b.row_off = ro;
+        b.pt_off = po;
+        b.n = (int32_t)ni;
+        b.d = d;
+        b.levels = fh::tree_levels(ni, kLeafSize);
+        b.pad = 0;
+        xo += ni * d;
+        ro += ni - 1;
+        po += ni;
+        max_n = std::max(max_n, (int)ni);
+        max_nodes = std::max(max_nodes, (1 << b.levels) - 1);
+    }
+    g.total_n = po;
+    g.total_rows = ro;
+    // pinned layout: rows up; src, dst, w, core, status down
+    const size_t up = sizeof(float) * xo;
+    const size_t down = sizeof(int) * 2 * ro + sizeof(float) * ro + sizeof(float) * po + sizeof(int) * m;
+    const size_t need = up + down + 64;
+    if (ctx->h_bt_cap < need) {
+        if (ctx->h_bt) FH_HIP(hipHostFree(ctx->h_bt));
+        ctx->h_bt = nullptr;
+        ctx->h_bt_cap = 0;
+        const size_t want = need + need / 4;
+        FH_HIP(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_bt), want));
+        ctx->h_bt_cap = want;
+    }
+    g.grid = fh::batch_grid(m, cus);
+    g.slice = fh::batch_slice_bytes(max_n, max_nodes, d, kq);
+    g.max_n = max_n;
+    g.max_nodes = max_nodes;
+    FH_TRY(ensure(ctx, ctx->bt_x, (size_t)xo));
+    FH_TRY(ensure(ctx, ctx->bt_jobs, (size_t)m));
+    // (one row past the last dataset's: a round that emits more rows than
+    // its components allow writes at most there before it is caught)
+    FH_TRY(ensure(ctx, ctx->bt_src, (size_t)ro + 1));
+    FH_TRY(ensure(ctx, ctx->bt_dst, (size_t)ro + 1));
+    FH_TRY(ensure(ctx, ctx->bt_w, (size_t)ro + 1));
+    FH_TRY(ensure(ctx, ctx->bt_core, (size_t)po));
+    FH_TRY(ensure(ctx, ctx->bt_status, (size_t)m));
+    FH_TRY(ensure(ctx, ctx->bt_next, 1));
+    FH_TRY(ensure(ctx, ctx->bt_scratch, (size_t)(g.slice * g.grid)));
+    return FH_OK;
+}
+
+// Packs, uploads and launches; the outputs' copies to the pinned buffer are
+// queued behind the kernel on the context's stream (batch_group_wait).
+int batch_group_launch(BatchGroup &g, const float *const *X, int ms) {
+    fh_ctx *ctx = g.ctx;
+    hipStream_t st = ctx->stream;
+    const int d = g.d;
+    const int kq = ms > 1 ? ms + 1 : 2;
+    const int m = (int)g.idx.size();
+    const int64_t xo = g.total_n * d, ro = g.total_rows, po = g.total_n;
+    const size_t up = sizeof(float) * xo;
+    float *hx = reinterpret_cast<float *>(ctx->h_bt);
+    for (int j = 0; j < m; ++j)
+        std::memcpy(hx + g.jobs[j].x_off, X[g.idx[j]], sizeof(float) * (size_t)g.jobs[j].n * d);
+    unsigned char *hd = ctx->h_bt + ((up + 15) / 16) * 16;
+    int *hsrc = reinterpret_cast<int *>(hd);
+    int *hdst = hsrc + ro;
+    float *hw = reinterpret_cast<float *>(hdst + ro);
+    float *hcore = hw + ro;
+    int *hstat = reinterpret_cast<int *>(hcore + po);
+    g.src = hsrc;
+    g.dst = hdst;
+    g.w = hw;
+    g.core = hcore;
+    g.status = hstat;
+
+    FH_HIP(hipMemcpyAsync(ctx->bt_x.p, hx, up, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(ctx->bt_jobs.p, g.jobs.data(), sizeof(fh::BatchJob) * m, hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemsetAsync(ctx->bt_next.p, 0, sizeof(int), st));
+    fh::BatchArgs a{};
+    a.X = ctx->bt_x.p;
+    a.jobs = ctx->bt_jobs.p;
+    a.count = m;
+    a.ms = ms;
+    a.kq = kq;
+    a.next = ctx->bt_next.p;
+    a.scratch = ctx->bt_scratch.p;
+    a.slice_bytes = g.slice;
+    a.max_n = g.max_n;
+    a.max_nodes = g.max_nodes;
+    a.max_d = d;
+    a.src = ctx->bt_src.p;
+    a.dst = ctx->bt_dst.p;
+    a.w = ctx->bt_w.p;
+    a.core = ctx->bt_core.p;
+    a.status = ctx->bt_status.p;
+    FH_HIP(fh::launch_batch(a, g.grid, st));
+    FH_HIP(hipMemcpyAsync(hsrc, ctx->bt_src.p, sizeof(int) * ro, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(hdst, ctx->bt_dst.p, sizeof(int) * ro, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(hw, ctx->bt_w.p, sizeof(float) * ro, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(hcore, ctx->bt_core.p, sizeof(float) * po, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(hstat, ctx->bt_status.p, sizeof(int) * m, hipMemcpyDeviceToHost, st));
+    return FH_OK;
+}
+
+// Waits for the launch and its copies; every dataset must report n - 1 rows.
+int batch_group_wait(BatchGroup &g) {
+    fh_ctx *ctx = g.ctx;
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t j = 0; j < g.jobs.size(); ++j)
+        if (g.status[j] != g.jobs[j].n - 1)
+            return fail(ctx, FH_EDEVICE, "persistent batch: dataset " + std::to_string(g.idx[j]) + " gave " +
+                                             std::to_string(g.status[j]) + " rows for " +
+                                             std::to_string(g.jobs[j].n) + " points");
+    return FH_OK;
+}
+
 int check_args(fh_ctx *ctx, const void *X, int64_t n, int32_t d, int32_t ms) {
     if (!ctx) return FH_EINVAL;
     if (!X) return fail(ctx, FH_EINVAL, "X is null");
@@ -1166,6 +1340,7 @@ void fh_destroy(fh_ctx *c) {
     if (c->stream) e = hipStreamSynchronize(c->stream);
     if (!c->arena) for_each_buf(c, [](auto &b) { release(b); });
     if (c->h_small) e = hipHostFree(c->h_small);
+    if (c->h_bt) e = hipHostFree(c->h_bt);
     for (auto &ev : c->ev)
         if (ev) e = hipEventDestroy(ev);
     if (c->stream) e = hipStreamDestroy(c->stream);
@@ -1478,37 +1653,133 @@ int fh_sort_mst_device(fh_ctx *ctx, int64_t m, const int32_t *src, const int32_t
 // counter and runs the MST on the device, sorts the rows on the host and
 // runs the host label stage, so different datasets' kernels overlap on the
 // GPU and their label stages on the host cores.
+// Per device: the batch workers' contexts and the persistent launches'
+// contexts (one per dimension in flight), kept for the process's lifetime
+// and reused by later calls, so no stream or buffer is created or destroyed
+// while other kernels run and a call starts with buffers already sized.
+// One batch call per device at a time.
+namespace {
+std::mutex batch_mu[kNoAllocDevices];
+std::vector<fh_ctx *> batch_ctx[kNoAllocDevices];
+std::vector<fh_ctx *> batch_group_ctx[kNoAllocDevices];
+
+int grow_pool(std::vector<fh_ctx *> &pool, int device, int size) {
+    while ((int)pool.size() < size) {
+        fh_ctx *c = fh_create(device);
+        if (!c) return FH_EDEVICE;
+        pool.push_back(c);
+    }
+    return FH_OK;
+}
+
+// The datasets the persistent kernel takes, one group per dimension, each
+// launched on its own context's stream; `rest` gets the others.
+int batch_start(int device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
+                int32_t ms, std::vector<BatchGroup> &groups, std::vector<int32_t> &rest) {
+    std::vector<int> by_d[FH_MAX_DIM + 1];
+    for (int32_t i = 0; i < count; ++i) {
+        if (n[i] >= 2 && d[i] >= 1 && d[i] <= FH_MAX_DIM && X[i] && batch_eligible(n[i], d[i], ms))
+            by_d[d[i]].push_back(i);
+        else
+            rest.push_back(i);
+    }
+    int ng = 0;
+    for (int dd = 1; dd <= FH_MAX_DIM; ++dd) ng += by_d[dd].empty() ? 0 : 1;
+    FH_TRY(grow_pool(batch_group_ctx[device], device, ng));
+    int gi = 0;
+    for (int dd = 1; dd <= FH_MAX_DIM; ++dd) {
+        if (by_d[dd].empty()) continue;
+        BatchGroup g;
+        g.ctx = batch_group_ctx[device][gi++];
+        g.d = dd;
+        g.idx = by_d[dd];
+        groups.push_back(std::move(g));
+    }
+    for (auto &g : groups) FH_TRY(batch_group_prepare(g, n, ms));
+    for (size_t k = 0; k < groups.size(); ++k) {
+        const int rc = batch_group_launch(groups[k], X, ms);
+        if (rc != FH_OK) {
+            // nothing of this call may still run when it returns
+            for (size_t j = 0; j < k; ++j) (void)hipStreamSynchronize(groups[j].ctx->stream);
+            return rc;
+        }
+    }
+    return FH_OK;
+}
+}  // namespace
+
+int fh_mst_batch(int32_t device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
+                 int32_t min_samples, int32_t *const *src, int32_t *const *dst, float *const *w_sq,
+                 float *const *core_sq, int32_t *persistent) {
+    if (count < 0 || (count > 0 && (!X || !n || !d || !src || !dst || !w_sq)) || min_samples < 1) return FH_EINVAL;
+    if (device < 0 || device >= kNoAllocDevices) return FH_EINVAL;
+    std::lock_guard<std::mutex> batch_lock(batch_mu[device]);
+    if (hipSetDevice(device) != hipSuccess) return FH_EDEVICE;
+    std::vector<BatchGroup> groups;
+    std::vector<int32_t> rest;
+    FH_TRY(batch_start(device, count, X, n, d, min_samples, groups, rest));
+    if (persistent)
+        for (int32_t i = 0; i < count; ++i) persistent[i] = 0;
+    // the others one at a time on a worker context, beside the launches
+    int rc = grow_pool(batch_ctx[device], device, 1);
+    for (size_t r = 0; rc == FH_OK && r < rest.size(); ++r) {
+        const int32_t i = rest[r];
+        rc = (!X[i] || n[i] < 1) ? FH_EINVAL
+                                 : fh_mst_euclidean(batch_ctx[device][0], X[i], n[i], d[i], min_samples, nullptr,
+                                                    core_sq ? core_sq[i] : nullptr, nullptr, src[i], dst[i], w_sq[i],
+                                                    nullptr);
+    }
+    for (auto &g : groups) {
+        const int wrc = batch_group_wait(g);
+        if (rc == FH_OK) rc = wrc;
+    }
+    if (rc != FH_OK) return rc;
+    for (auto &g : groups) {
+        for (size_t j = 0; j < g.idx.size(); ++j) {
+            const int i = g.idx[j];
+            const fh::BatchJob &b = g.jobs[j];
+            std::memcpy(src[i], g.src + b.row_off, sizeof(int32_t) * (b.n - 1));
+            std::memcpy(dst[i], g.dst + b.row_off, sizeof(int32_t) * (b.n - 1));
+            std::memcpy(w_sq[i], g.w + b.row_off, sizeof(float) * (b.n - 1));
+            if (core_sq && core_sq[i]) std::memcpy(core_sq[i], g.core + b.pt_off, sizeof(float) * b.n);
+            if (persistent) persistent[i] = 1;
+        }
+    }
+    return FH_OK;
+}
+
 int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
                      int32_t min_samples, int64_t min_cluster_size, int32_t workers, int64_t *const *labels,
                      float *const *probabilities, double *ms_out) {
     if (count < 0 || !X || !n || !d || !labels || !probabilities || min_samples < 1 || min_cluster_size < 1)
         return FH_EINVAL;
     const int W = std::max(1, std::min<int>(workers, 16));
-    // Largest first: a worker's buffers reach their final size on its first
-    // dataset instead of growing (hipFree waits for the whole device) as
-    // larger ones arrive, and the long jobs do not trail at the end.
-    std::vector<int32_t> queue(std::max(count, 0));
-    for (int32_t i = 0; i < count; ++i) queue[i] = i;
+    if (device < 0 || device >= kNoAllocDevices) return FH_EINVAL;
+    std::lock_guard<std::mutex> batch_lock(batch_mu[device]);
+    if (hipSetDevice(device) != hipSuccess) return FH_EDEVICE;
+    // small datasets: persistent launches, started before the workers
+    std::vector<BatchGroup> groups;
+    std::vector<int32_t> queue;
+    FH_TRY(batch_start(device, count, X, n, d, min_samples, groups, queue));
+    // The rest one at a time on the workers' streams, largest first: a
+    // worker's buffers reach their final size on its first dataset instead
+    // of growing (hipFree waits for the whole device) as larger ones arrive,
+    // and the long jobs do not trail at the end.
     std::stable_sort(queue.begin(), queue.end(), [&](int32_t a, int32_t b) {
         return n[a] * std::max(d[a], 1) > n[b] * std::max(d[b], 1);
     });
-    if (device < 0 || device >= kNoAllocDevices) return FH_EINVAL;
-    // The workers' contexts (stream, events, pinned slots, device buffers)
-    // are kept for the process's lifetime and reused by later calls: no
-    // stream or buffer is created or destroyed while other workers' kernels
-    // run, and a call starts with buffers already sized.  One batch call per
-    // device at a time.
-    static std::mutex batch_mu[kNoAllocDevices];
-    static std::vector<fh_ctx *> batch_ctx[kNoAllocDevices];
-    std::lock_guard<std::mutex> batch_lock(batch_mu[device]);
+    const int nq = (int)queue.size();
+    // the persistent launches' datasets, for the label stage once they are back
+    std::vector<std::pair<int, int>> small;  // (group, job)
+    for (int gi = 0; gi < (int)groups.size(); ++gi)
+        for (int j = 0; j < (int)groups[gi].idx.size(); ++j) small.emplace_back(gi, j);
     std::vector<fh_ctx *> &pool = batch_ctx[device];
-    while ((int)pool.size() < W) {
-        fh_ctx *c = fh_create(device);
-        if (!c) return FH_EDEVICE;
-        pool.push_back(c);
-    }
-    std::atomic<int> next{0};
+    FH_TRY(grow_pool(pool, device, W));
+    std::atomic<int> next{0}, next_small{0};
     std::atomic<int> err{0};
+    std::mutex ready_mu;
+    std::condition_variable ready_cv;
+    bool ready = false;
     auto work = [&](int wi) {
         fh_ctx *ctx = pool[wi];
         if (hipSetDevice(device) != hipSuccess) {
@@ -1519,9 +1790,30 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
         std::vector<float> w, w2;
         std::vector<int32_t> order;
         fh_label_params prm{min_cluster_size, std::numeric_limits<double>::infinity(), 0.0, 0, 0};
+        // hdbscan.py:511-519: rows by (weight, source, target), then the labels
+        auto label = [&](int i, int64_t ni, const int32_t *rs, const int32_t *rd, const float *rw) -> int {
+            const int64_t m = ni - 1;
+            order.resize(m);
+            for (int64_t r = 0; r < m; ++r) order[r] = (int32_t)r;
+            std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+                if (rw[a] != rw[b]) return rw[a] < rw[b];
+                if (rs[a] != rs[b]) return rs[a] < rs[b];
+                return rd[a] < rd[b];
+            });
+            s2.resize(m);
+            d2.resize(m);
+            w2.resize(m);
+            for (int64_t r = 0; r < m; ++r) {
+                s2[r] = rs[order[r]];
+                d2[r] = rd[order[r]];
+                w2[r] = rw[order[r]];
+            }
+            return fh_labels_from_sorted_mst(ni, s2.data(), d2.data(), w2.data(), nullptr, nullptr, &prm, labels[i],
+                                             probabilities[i], nullptr, nullptr, nullptr, nullptr);
+        };
         for (;;) {
             const int qi = next.fetch_add(1);
-            if (qi >= count || err.load() != 0) break;
+            if (qi >= nq || err.load() != 0) break;
             const int i = queue[qi];
             const auto t0 = std::chrono::steady_clock::now();
             const int64_t ni = n[i];
@@ -1539,28 +1831,7 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
                 w.resize(m);
                 int rc = fh_mst_euclidean(ctx, X[i], ni, d[i], min_samples, nullptr, nullptr, nullptr, src.data(),
                                           dst.data(), w.data(), nullptr);
-                if (rc != FH_OK) {
-                    err = rc;
-                    break;
-                }
-                // hdbscan.py:511-519: rows by (weight, source, target)
-                order.resize(m);
-                for (int64_t r = 0; r < m; ++r) order[r] = (int32_t)r;
-                std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-                    if (w[a] != w[b]) return w[a] < w[b];
-                    if (src[a] != src[b]) return src[a] < src[b];
-                    return dst[a] < dst[b];
-                });
-                s2.resize(m);
-                d2.resize(m);
-                w2.resize(m);
-                for (int64_t r = 0; r < m; ++r) {
-                    s2[r] = src[order[r]];
-                    d2[r] = dst[order[r]];
-                    w2[r] = w[order[r]];
-                }
-                rc = fh_labels_from_sorted_mst(ni, s2.data(), d2.data(), w2.data(), nullptr, nullptr, &prm, labels[i],
-                                               probabilities[i], nullptr, nullptr, nullptr, nullptr);
+                if (rc == FH_OK) rc = label(i, ni, src.data(), dst.data(), w.data());
                 if (rc != FH_OK) {
                     err = rc;
                     break;
@@ -1569,9 +1840,42 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
             if (ms_out)
                 ms_out[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
+        {
+            std::unique_lock<std::mutex> lk(ready_mu);
+            ready_cv.wait(lk, [&] { return ready; });
+        }
+        for (;;) {
+            const int si = next_small.fetch_add(1);
+            if (si >= (int)small.size() || err.load() != 0) break;
+            const BatchGroup &g = groups[small[si].first];
+            const int j = small[si].second;
+            const int i = g.idx[j];
+            const fh::BatchJob &b = g.jobs[j];
+            if (!labels[i] || !probabilities[i]) {
+                err = FH_EINVAL;
+                break;
+            }
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = label(i, b.n, g.src + b.row_off, g.dst + b.row_off, g.w + b.row_off);
+            if (rc != FH_OK) {
+                err = rc;
+                break;
+            }
+            if (ms_out)
+                ms_out[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
     };
     std::vector<std::thread> th;
     for (int t = 0; t < W; ++t) th.emplace_back(work, t);
+    for (auto &g : groups) {
+        const int rc = batch_group_wait(g);
+        if (rc != FH_OK) err = rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(ready_mu);
+        ready = true;
+    }
+    ready_cv.notify_all();
     for (auto &x : th) x.join();
     return err.load();
 }

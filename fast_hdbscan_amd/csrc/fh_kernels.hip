@@ -20,157 +20,14 @@
 #include "fh_kernels.h"
 
 #include "fh_device.h"
+#include "fh_search.h"
 
 namespace fh {
 
 namespace {
 
-constexpr float kInf = __builtin_huge_valf();
+// (the walk and visitors: fh_search.h)
 
-// Depth-first walk of the implicit tree without a stack, visiting children
-// in the reference's order: the child with the smaller box lower bound
-// first, left on ties (numba_kdtree.py:747-760, boruvka.py:228-285).  A bit
-// per depth remembers which child went first; on the way back the sibling's
-// bound is recomputed with the same arithmetic, so the walk makes exactly the
-// reference recursion's sequence of enter/prune decisions.
-//
-// Each lane walks on its own until it reaches the next leaf it enters (or
-// ends); the lanes of a wavefront then scan their leaves together.  Lanes
-// whose walks drift apart would otherwise run the leaf scan in almost every
-// step of the loop, a few lanes at a time.  A lane's own sequence of
-// decisions is unchanged: its bound only changes in its leaf scans.
-template <int D, class V>
-__device__ __forceinline__ void walk(const float *__restrict__ box, int levels, int d, V &v) {
-    const float *x = v.x.p();
-    const int stride = 2 * (D > 0 ? D : d);
-    int node = 0, depth = 0;
-    uint32_t first_right = 0;
-    float lb = box_lb<D>(x, box, d);
-    bool entering = true;
-    for (;;) {
-        bool done = false;
-        for (;;) {
-            if (entering && v.enter(node, lb)) {
-                if (depth == levels - 1) break;
-                const int l = 2 * node + 1;
-                const float lbl = box_lb<D>(x, box + (size_t)l * stride, d);
-                const float lbr = box_lb<D>(x, box + (size_t)(l + 1) * stride, d);
-                const bool rf = !(lbl <= lbr);
-                ++depth;
-                first_right = rf ? (first_right | (1u << depth)) : (first_right & ~(1u << depth));
-                node = rf ? l + 1 : l;
-                lb = rf ? lbr : lbl;
-                continue;
-            }
-            if (depth == 0) {
-                done = true;
-                break;
-            }
-            const bool rf = (first_right >> depth) & 1u;
-            const bool is_right = (node & 1) == 0;
-            if (is_right == rf) {
-                node = is_right ? node - 1 : node + 1;
-                lb = box_lb<D>(x, box + (size_t)node * stride, d);
-                entering = true;
-            } else {
-                node = (node - 1) >> 1;
-                --depth;
-                entering = false;
-            }
-        }
-        if (done) break;
-        v.leaf(node);
-        entering = false;  // then on as from a finished subtree
-    }
-}
-
-// The query point: held in registers for a compile-time dimension, read
-// through a pointer otherwise.
-template <int D>
-struct QueryPoint {
-    float v[D];
-    __device__ __forceinline__ void load(const float *row, int) {
-#pragma unroll
-        for (int j = 0; j < D; ++j) v[j] = row[j];
-    }
-    __device__ __forceinline__ const float *p() const { return v; }
-};
-
-template <>
-struct QueryPoint<0> {
-    const float *ptr;
-    __device__ __forceinline__ void load(const float *row, int) { ptr = row; }
-    __device__ __forceinline__ const float *p() const { return ptr; }
-};
-
-// ---------------------------------------------------------------- kNN ----
-
-template <int D>
-struct KnnVisitor {
-    QueryPoint<D> x;
-    const float *__restrict__ xp;
-    const int2 *__restrict__ range;
-    LdsHeap h;
-    float hmax;
-    float bound;  // +inf: the reference's unbounded query
-    int tie;      // bounded query only: an equality that the heap layout decides
-    int d;
-
-    __device__ __forceinline__ bool enter(int, float lb) const { return !(lb > hmax); }
-
-    __device__ __forceinline__ void consider(float dist, int i, bool bounded) {
-        if (dist < hmax) {
-            if (bounded && h.P(0) < kInf && h.k > 1) {
-                float c = h.P(1);
-                if (h.k > 2) c = h.P(2) > c ? h.P(2) : c;
-                if (c == h.P(0)) tie = 1;  // which equal element leaves depends on the layout
-            }
-            h.push(dist, i);
-            hmax = h.P(0) < bound ? h.P(0) : bound;
-        } else if (bounded && dist == hmax && hmax == h.P(0)) {
-            tie = 1;
-        }
-    }
-
-    // The leaf's points in position order; for a compile-time dimension
-    // their rows are loaded kLeafBatch at a time ahead of the distances, so
-    // the loads overlap (the pushes still follow position order).
-    __device__ __forceinline__ void leaf(int node) {
-        const int2 r = range[node];
-        const bool bounded = bound < kInf;
-        if constexpr (D > 0) {
-            constexpr int kLeafBatch = 8;
-            int i0 = r.x;
-            for (; i0 + kLeafBatch <= r.y; i0 += kLeafBatch) {
-                float pt[kLeafBatch][D];
-#pragma unroll
-                for (int u = 0; u < kLeafBatch; ++u)
-#pragma unroll
-                    for (int j = 0; j < D; ++j) pt[u][j] = xp[(size_t)(i0 + u) * D + j];
-#pragma unroll
-                for (int u = 0; u < kLeafBatch; ++u) consider(rdist<D>(x.p(), pt[u], D), i0 + u, bounded);
-            }
-            if (i0 + 4 <= r.y) {
-                float pt[4][D];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int j = 0; j < D; ++j) pt[u][j] = xp[(size_t)(i0 + u) * D + j];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) consider(rdist<D>(x.p(), pt[u], D), i0 + u, bounded);
-                i0 += 4;
-            }
-            for (int i = i0; i < r.y; ++i) consider(rdist<D>(x.p(), xp + (size_t)i * D, D), i, bounded);
-        } else {
-            for (int i = r.x; i < r.y; ++i) consider(rdist<D>(x.p(), xp + (size_t)i * d, d), i, bounded);
-        }
-    }
-};
-
-// One lane per query row (queries in tree order for the self-query so a
-// wavefront's lanes walk nearby paths).  Heap in LDS, or (GLOBAL_HEAP) in a
-// global scratch slice for large k: two instantiations, so that the LDS
-// heap's accesses compile to LDS instructions rather than generic ones.
 template <int D, bool GLOBAL_HEAP>
 __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     extern __shared__ unsigned char smem_raw[];
@@ -244,63 +101,6 @@ __global__ __launch_bounds__(256) void knn_kernel(KnnArgs a) {
     }
 }
 
-// ----------------------------------------------------- Boruvka query ----
-
-template <int D>
-struct BoruvkaVisitor {
-    QueryPoint<D> x;
-    const float *__restrict__ xp;
-    const int2 *__restrict__ range;
-    const float *__restrict__ core_p;
-    const int *__restrict__ comp_p;
-    const int *__restrict__ node_comp;
-    float *bound_c;
-    float core_i;
-    int comp;
-    int d;
-    float best;
-    int best_j;
-    float B;
-
-    // boruvka.py:176-192: prune on the point's own best, on the component
-    // bound (strict comparisons) and on single-component nodes.
-    // B is the component bound as last read (at the start and on entering
-    // each leaf): it only ever decreases, so a stale value prunes less but
-    // never wrongly.
-    __device__ __forceinline__ bool enter(int node, float lb) {
-        if (lb > best) return false;
-        if (lb > B || core_i > B) return false;
-        if (node_comp[node] == comp) return false;
-        return true;
-    }
-
-    // boruvka.py:196-212.  The candidate filter uses core_j <= B (the
-    // reference's sequential run has core_j < B because the bound it sees is
-    // still above the component minimum when the winning point runs; with
-    // concurrent lanes the bound may already equal it).
-    __device__ __forceinline__ void leaf(int node) {
-        const int2 r = range[node];
-        const int dd = D > 0 ? D : d;
-        B = ld_relaxed(bound_c);
-        for (int i = r.x; i < r.y; ++i) {
-            if (comp_p[i] == comp) continue;
-            const float cj = core_p[i];
-            if (!(cj <= B)) continue;
-            float dist = rdist<D>(x.p(), xp + (size_t)i * dd, d);
-            dist = dist > core_i ? dist : core_i;
-            dist = dist > cj ? dist : cj;
-            if (dist < best) {
-                best = dist;
-                best_j = i;
-                if (dist < B) {
-                    atomicMin(reinterpret_cast<unsigned int *>(bound_c), __float_as_uint(dist));
-                    B = dist;
-                }
-            }
-        }
-    }
-};
-
 // PASS 0: every point; 1: one position per group of sample_mask+1;
 // 2: the others, except those inside certified subtrees.
 template <int D, int PASS>
@@ -353,69 +153,11 @@ __global__ __launch_bounds__(256) void boruvka_query_kernel(QueryArgs a) {
 // neighbours all share its component keeps that property for the rest of
 // the call: it is marked interior (interior[p] = 1) and skipped from then
 // on.
-// atomicMin(base + c, v) for every lane with has set, reduced first over
-// the wave's lanes that share c (lanes of consecutive tree positions mostly
-// share their component): one atomic per distinct c in the wave.  Every lane
-// of the wave must call it.
-template <typename T>
-__device__ __forceinline__ void wave_atomic_min_by_key(T *base, int c, T v, bool has) {
-    const int lane = threadIdx.x & 63;
-    unsigned long long pending = __ballot(has);
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int c0 = __shfl(c, leader, 64);
-        const bool mine = has && c == c0;
-        T m = mine ? v : static_cast<T>(~static_cast<T>(0));
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const T other = __shfl_xor(m, o, 64);
-            m = other < m ? other : m;
-        }
-        if (lane == leader) atomicMin(base + c0, m);
-        pending &= ~__ballot(mine);
-    }
-}
-
 __global__ void seed_bounds_kernel(int64_t n, int kn, const int *__restrict__ nb_pos, const float *__restrict__ nb_d,
                                    const float *__restrict__ core_p, const int *__restrict__ comp_p, float *bound,
                                    unsigned char *interior) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = p < n && !interior[p];
-    int c = -1;
-    float w = kInf;
-    if (act) {
-        c = comp_p[p];
-        const float ci = core_p[p];
-        bool cross = false;
-        // neighbours four at a time: positions first, then their labels and
-        // core distances together
-        for (int j0 = 0; j0 < kn; j0 += 4) {
-            int q[4], cq[4];
-            float dq[4], kq[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                q[u] = j0 + u < kn ? nb_pos[p * kn + j0 + u] : -1;
-                dq[u] = j0 + u < kn ? nb_d[p * kn + j0 + u] : 0.0f;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                cq[u] = q[u] >= 0 ? comp_p[q[u]] : c;
-                kq[u] = q[u] >= 0 ? core_p[q[u]] : 0.0f;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (q[u] < 0 || cq[u] == c) continue;
-                cross = true;
-                float e = dq[u];
-                e = e > ci ? e : ci;
-                e = e > kq[u] ? e : kq[u];
-                w = e < w ? e : w;
-            }
-        }
-        if (!cross) interior[p] = 1;
-    }
-    // non-negative floats order as their bit patterns
-    wave_atomic_min_by_key(reinterpret_cast<unsigned int *>(bound), c, __float_as_uint(w), act && w < kInf);
+    seed_bounds_at(p, p < n, kn, nb_pos, nb_d, core_p, comp_p, bound, interior);
 }
 
 // Which certification node (in-level index at `level`) holds each position,
@@ -438,32 +180,6 @@ __global__ __launch_bounds__(64) void cert_setup_kernel(int level, const int2 *_
         __syncthreads();
     }
     if (threadIdx.x == 0) mincore[k] = sm[0];
-}
-
-// Lower bound of the squared distance between two boxes, in the same
-// sequential fused form as rdist so it never exceeds rdist of any pair of
-// points inside them.
-template <int D>
-__device__ __forceinline__ float box_box_lb(const float *__restrict__ a, const float *__restrict__ b, int d) {
-    const int dd = D > 0 ? D : d;
-    float r = 0.0f;
-#pragma unroll
-    for (int j = 0; j < dd; ++j) {
-        const float alo = a[j], ahi = a[dd + j], blo = b[j], bhi = b[dd + j];
-        const float g1 = blo > ahi ? blo - ahi : 0.0f;
-        const float g2 = alo > bhi ? alo - bhi : 0.0f;
-        const float g = g1 + g2;
-        r = __builtin_fmaf(g, g, r);
-    }
-    return r;
-}
-
-// Orders the LDS accesses of one wavefront's lanes (a wave executes in
-// lockstep; this keeps the compiler from moving them across).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // One wave per certification node P of a single component C: the node
@@ -565,192 +281,15 @@ __global__ __launch_bounds__(64) void cert_kernel(CertArgs a) {
 }
 
 // ------------------------------------------ wave-cooperative search ----
-//
-// The main nearest-edge search with one wavefront per 64 consecutive tree
-// positions (spatially adjacent query points).  The wave walks the tree once
-// for all its lanes, depth first, the child nearer to the lanes' bounding
-// box first; it enters a node when at least one lane passes the per-point
-// tests of boruvka.py:176-192 there (lower bound against the lane's own best
-// and its component bound, core against the bound, single-component node),
-// stages each leaf it reaches in LDS with coalesced loads (points, core
-// distances, component labels), and every lane passing the tests at that
-// leaf scans the staged points (boruvka.py:196-212).
-//
-// Exactness (the selection rule of the one-thread reference run): only the
-// lanes whose point achieves its component's minimum w* matter, and for such
-// a lane every candidate j of weight w* lies in nodes whose lower bound is
-// <= w* <= best, <= w* <= B, with core_j <= w* <= B: no test prunes it, so
-// the lane scans every candidate of weight w*.  Equal weights are resolved
-// by the reference's depth-first order for that point (closer child first,
-// left on equal bounds, position order within a leaf), so the lane keeps the
-// first such candidate in that order, as the recursion records it.
-// Lanes of larger weight report values >= w* that cannot win the packed
-// atomicMin unless they also achieve w*.
 
-// True when position j1 comes before j2 in the depth-first visiting order of
-// the query for point x (the recursion of boruvka.py:228-285).
-template <int D>
-__device__ bool dfs_first(const float *x, int j1, int j2, const int2 *__restrict__ range,
-                          const float *__restrict__ box, int levels) {
-    int node = 0;
-    for (int depth = 0; depth < levels - 1; ++depth) {
-        const int l = 2 * node + 1;
-        const int split = range[l].y;
-        const bool a1 = j1 < split, a2 = j2 < split;
-        if (a1 == a2) {
-            node = a1 ? l : l + 1;
-            continue;
-        }
-        const float lbl = box_lb<D>(x, box + (size_t)l * 2 * D, D);
-        const float lbr = box_lb<D>(x, box + (size_t)(l + 1) * 2 * D, D);
-        return a1 == (lbl <= lbr);
-    }
-    return j1 < j2;
-}
-
-constexpr int kWaveStage = 128;  // staged points per leaf (leaves hold < 2 * leaf_size = 80)
-
+// (the search itself: wave_search in fh_search.h)
 template <int D, int PASS>
 __global__ __launch_bounds__(256) void boruvka_wave_kernel(QueryArgs a) {
     __shared__ float s_x[4][kWaveStage * D];
     __shared__ float s_c[4][kWaveStage];
     __shared__ int s_k[4][kWaveStage];
     const int wv = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int64_t pos = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    bool act = pos < a.n;
-    if constexpr (PASS == 2) {
-        if (act) {
-            const bool sampled = (pos & a.sample_mask) == (a.sample_mask >> 1);
-            act = !sampled && !(a.cert && a.cert[a.cert_of_pos[pos]]);
-        }
-    }
-    if (__ballot(act) == 0) return;  // the whole wave has nothing to search
-    float x[D];
-    float core_i = kInf, B = 0.0f;
-    int comp = -1;
-    if (act) {
-#pragma unroll
-        for (int j = 0; j < D; ++j) x[j] = a.xp[pos * D + j];
-        comp = a.comp_p[pos];
-        core_i = a.core_p[pos];
-        B = ld_relaxed(a.bound + comp);
-    } else {
-#pragma unroll
-        for (int j = 0; j < D; ++j) x[j] = 0.0f;
-    }
-    // the active lanes' bounding box
-    float gb[2 * D];
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-        float lo = act ? x[j] : kInf, hi = act ? x[j] : -kInf;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const float l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
-            lo = l2 < lo ? l2 : lo;
-            hi = h2 > hi ? h2 : hi;
-        }
-        gb[j] = lo;
-        gb[D + j] = hi;
-    }
-    float best = kInf;
-    int best_j = -1;
-    float *sx = s_x[wv];
-    float *sc = s_c[wv];
-    int *sk = s_k[wv];
-    const int levels = a.levels;
-    int node = 0, depth = 0;
-    uint32_t first_right = 0;
-    bool entering = true;
-    for (;;) {
-        // the walk is the wave's: its node index lives in a scalar register,
-        // so the node's box, label and range come through scalar loads
-        node = __builtin_amdgcn_readfirstlane(node);
-        if (entering) {
-            const float *nb = a.box + (size_t)node * 2 * D;
-            const float lb = box_lb<D>(x, nb, D);
-            const bool t = act && !(core_i > B) && !(lb > best) && !(lb > B) && a.node_comp[node] != comp;
-            if (__ballot(t) != 0) {
-                if (depth == levels - 1) {
-                    const int2 r = a.range[node];
-                    const int cnt = r.y - r.x;
-                    for (int q = lane; q < cnt; q += 64) {
-                        const int64_t g = r.x + q;
-#pragma unroll
-                        for (int j = 0; j < D; ++j) sx[q * D + j] = a.xp[g * D + j];
-                        sc[q] = a.core_p[g];
-                        sk[q] = a.comp_p[g];
-                    }
-                    wave_sync();
-                    if (t) {
-                        B = ld_relaxed(a.bound + comp);
-                        // staged points read four at a time ahead of the tests
-                        // (still taken in position order)
-                        auto scan = [&](int q, int kq, float cj, const float *y) {
-                            if (kq == comp || !(cj <= B)) return;
-                            float dist = rdist<D>(x, y, D);
-                            dist = dist > core_i ? dist : core_i;
-                            dist = dist > cj ? dist : cj;
-                            if (dist < best) {
-                                best = dist;
-                                best_j = r.x + q;
-                                if (dist < B) {
-                                    atomicMin(reinterpret_cast<unsigned int *>(a.bound + comp), __float_as_uint(dist));
-                                    B = dist;
-                                }
-                            } else if (dist == best && best_j >= 0 &&
-                                       dfs_first<D>(x, r.x + q, best_j, a.range, a.box, levels)) {
-                                best_j = r.x + q;
-                            }
-                        };
-                        int q = 0;
-                        for (; q + 4 <= cnt; q += 4) {
-                            int kq[4];
-                            float cq[4], yq[4][D];
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                kq[u] = sk[q + u];
-                                cq[u] = sc[q + u];
-#pragma unroll
-                                for (int j = 0; j < D; ++j) yq[u][j] = sx[(q + u) * D + j];
-                            }
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) scan(q + u, kq[u], cq[u], yq[u]);
-                        }
-                        for (; q < cnt; ++q) scan(q, sk[q], sc[q], sx + q * D);
-                    }
-                    wave_sync();
-                } else {
-                    const int l = 2 * node + 1;
-                    const float lbl = box_box_lb<D>(gb, a.box + (size_t)l * 2 * D, D);
-                    const float lbr = box_box_lb<D>(gb, a.box + (size_t)(l + 1) * 2 * D, D);
-                    const bool rf = !(lbl <= lbr);
-                    ++depth;
-                    first_right = rf ? (first_right | (1u << depth)) : (first_right & ~(1u << depth));
-                    node = rf ? l + 1 : l;
-                    continue;
-                }
-            }
-        }
-        if (depth == 0) break;
-        const bool rf = (first_right >> depth) & 1u;
-        const bool is_right = (node & 1) == 0;
-        if (is_right == rf) {
-            node = is_right ? node - 1 : node + 1;
-            entering = true;
-        } else {
-            node = (node - 1) >> 1;
-            --depth;
-            entering = false;
-        }
-    }
-    const bool found = act && best_j >= 0;
-    unsigned long long key = ~0ull;
-    if (found) {
-        key = ((unsigned long long)__float_as_uint(best) << 32) | (unsigned int)a.perm[pos];
-        a.partner[pos] = best_j;
-    }
-    wave_atomic_min_by_key(a.best, comp, key, found);
+    wave_search<D, PASS>(a, (int64_t)blockIdx.x * 256 + threadIdx.x, s_x[wv], s_c[wv], s_k[wv]);
 }
 
 // sample_weight_core_distance (boruvka.py:527-539): the first sorted
@@ -812,27 +351,12 @@ __global__ void init_edges_pos_kernel(int n, int k, const int *__restrict__ nb_p
                                       float *core_pt) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const float ci = core_p[p];
-    const int C = k - 1;
-    int s = -1, draw = -1;
-    float wi = 0.0f;
-    for (int j = 0; j < C; ++j) {
-        const int q = nb_pos[(int64_t)p * C + j];
-        const int kw = q < 0 ? n - 1 : perm[q];
-        const int qp = q < 0 ? posof[n - 1] : q;
-        if (ci >= core_p[qp]) {
-            const float dj = nb_d != nullptr ? nb_d[(int64_t)p * C + j] : ci;
-            s = kw;
-            draw = q < 0 ? -1 : kw;
-            wi = ci > dj ? ci : dj;
-            break;
-        }
-    }
+    const ForestEdge e = forest_edge_at(p, n, k, nb_pos, nb_d, core_p, perm, posof);
     const int i = perm[p];
-    succ[i] = s;
-    dst_raw[i] = draw;
-    w[i] = wi;
-    core_pt[i] = ci;
+    succ[i] = e.succ;
+    dst_raw[i] = e.dst_raw;
+    w[i] = e.w;
+    core_pt[i] = core_p[p];
 }
 
 // One min_samples value of a sweep from shared tree-order kNN rows of width
@@ -1044,33 +568,6 @@ __global__ __launch_bounds__(kScanBlock) void scan_add_kernel(int m, int *out, c
         if (base + j < m) out[base + j] += add;
 }
 
-// ------------------------------------------------------- union-find ----
-//
-// Lock-free union-find over component labels: roots are linked larger
-// under smaller with a CAS, so every root is the lowest original point index
-// of its set (the label convention above).
-
-__device__ __forceinline__ int uf_find(int *parent, int x) {
-    for (;;) {
-        const int p = ld_relaxed(parent + x);
-        if (p == x) return x;
-        x = p;
-    }
-}
-
-__device__ __forceinline__ void uf_unite(int *parent, int a, int b) {
-    for (;;) {
-        a = uf_find(parent, a);
-        b = uf_find(parent, b);
-        if (a == b) return;
-        if (a < b) {
-            const int t = a;
-            a = b;
-            b = t;
-        }
-        if (atomicCAS(parent + a, a, b) == a) return;
-    }
-}
 
 __global__ void iota_kernel(int m, int *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1182,37 +679,19 @@ __global__ __launch_bounds__(256) void purity_kernel(int level, int steps, int l
     }
 }
 
-// Per component: decode the packed best (weight, lowest point), look up the
-// partner, map the target to its slot.  A component without any candidate
-// (only when every core distance is +inf, i.e. min_samples >= n) keeps the
-// reference's entry (lowest point, -1, inf) whose target resolves like
-// ds_find(-1) = the component of point n-1.
+// Per component slot: its round edge (round_edge_of), the target component
+// mapped to its slot.
 __global__ void select_kernel(int c, int n, const int *__restrict__ roots, const unsigned long long *__restrict__ best,
                               const int *__restrict__ partner, const int *__restrict__ posof,
                               const int *__restrict__ perm, const int *__restrict__ comp_p,
                               const int *__restrict__ rootslot, int *succ, int *ei, int *ej, float *ew) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= c) return;
-    const int r = roots[s];
-    const unsigned long long key = best[r];
-    int i_star, j_star, tgt;
-    float w;
-    if (key == ~0ull) {
-        i_star = r;
-        j_star = -1;
-        w = kInf;
-        tgt = comp_p[posof[n - 1]];
-    } else {
-        i_star = (int)(unsigned int)(key & 0xffffffffull);
-        w = __uint_as_float((unsigned int)(key >> 32));
-        const int pj = partner[posof[i_star]];
-        j_star = perm[pj];
-        tgt = comp_p[pj];
-    }
-    succ[s] = rootslot[tgt];
-    ei[s] = i_star;
-    ej[s] = j_star;
-    ew[s] = w;
+    const RoundEdge e = round_edge_of(roots[s], n, best, partner, posof, perm, comp_p);
+    succ[s] = rootslot[e.tgt];
+    ei[s] = e.i;
+    ej[s] = e.j;
+    ew[s] = e.w;
 }
 
 __global__ void round_emit_kernel(int c, int base, const int *__restrict__ keep, const int *__restrict__ offs,
@@ -1301,19 +780,11 @@ __global__ __launch_bounds__(kSmallRound) void round_small_kernel(SmallRoundArgs
     float w = kInf;
     if (on) {
         r = a.roots[s];
-        const unsigned long long key = a.best[r];
-        int tgt;
-        if (key == ~0ull) {
-            i_star = r;
-            tgt = a.comp_p[a.posof[a.n - 1]];
-        } else {
-            i_star = (int)(unsigned int)(key & 0xffffffffull);
-            w = __uint_as_float((unsigned int)(key >> 32));
-            const int pj = a.partner[a.posof[i_star]];
-            j_star = a.perm[pj];
-            tgt = a.comp_p[pj];
-        }
-        succ = a.rootslot[tgt];
+        const RoundEdge e = round_edge_of(r, a.n, a.best, a.partner, a.posof, a.perm, a.comp_p);
+        i_star = e.i;
+        j_star = e.j;
+        w = e.w;
+        succ = a.rootslot[e.tgt];
     }
     // cycle skip (dbl_init, dbl_step, dbl_keep)
     S[0][s] = succ;
@@ -1492,16 +963,7 @@ int scan_tiles(int m) { return (m + kScanTile - 1) / kScanTile; }
 __global__ void node_range_kernel(int64_t N, int n, int2 *range) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    int level = 0;
-    while (((int64_t(1) << (level + 1)) - 1) <= i) ++level;
-    const int64_t k = i - ((int64_t(1) << level) - 1);  // index within the level
-    int s = 0, e = n;
-    for (int b = level - 1; b >= 0; --b) {
-        const int m = s + (e - s) / 2;
-        if ((k >> b) & 1) s = m;
-        else e = m;
-    }
-    range[i] = make_int2(s, e);
+    range[i] = node_range_of(i, n);
 }
 
 hipError_t launch_node_ranges(int64_t N, int n, int2 *range, hipStream_t st) {

@@ -223,16 +223,39 @@ int fh_mst_kruskal_knn(fh_ctx *ctx, const float *X, int64_t n, int32_t d, int32_
 /* ------------------------------------------------------------------ */
 
 /*
+ * The exact MSTs of count independent datasets (fh_mst_euclidean per
+ * dataset, no sample weights).
+ *   Replaces: a loop of compute_minimum_spanning_tree (hdbscan.py:339-494)
+ *             calls, one per dataset (no counterpart in one reference call).
+ * Datasets of at most a few thousand points (the KD-tree of one
+ * work-group's LDS: 4032 points at d = 2, 3392 at d = 3, 2496 at d = 5,
+ * 1792 at d = 8; min_samples <= 15) run in one persistent launch per
+ * dimension: every work-group takes the next dataset from a device-side
+ * queue and computes its tree, kNN, forest and every Boruvka round before
+ * taking another.  The others run one at a time.  Rows, weights and core
+ * distances equal fh_mst_euclidean's bit for bit.  HOST pointers: X[i]
+ * [n[i]*d[i]]; outputs src[i], dst[i], w_sq[i] [n[i]-1] (the reference's
+ * row order, squared weights), core_sq[i] [n[i]] or core_sq / core_sq[i]
+ * NULL; persistent [count] (1: served by the persistent launch) or NULL.
+ * Calls on one device run one at a time.
+ */
+int fh_mst_batch(int32_t device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
+                 int32_t min_samples, int32_t *const *src, int32_t *const *dst, float *const *w_sq,
+                 float *const *core_sq, int32_t *persistent);
+
+/*
  * Labels and membership strengths for count independent datasets (the
  * reference's fast_hdbscan(X_i, min_samples, min_cluster_size) per
  * dataset: Euclidean Boruvka MST, lexsort of the rows, EOM labels;
- * hdbscan.py:199-336).  workers (1..16) threads, each with its own
- * context on `device`, take datasets largest first.  HOST pointers:
- * X[i] [n[i]*d[i]], labels[i] [n[i]] int64, probabilities[i] [n[i]]
- * float32; ms_out [count] per-dataset wall times or NULL.  No context
- * argument: the workers' contexts are created on first use and kept for
- * later calls (up to 16 per device, released at process exit); calls on one
- * device run one at a time.
+ * hdbscan.py:199-336).  The datasets fh_mst_batch takes in its persistent
+ * launches run there; workers (1..16) threads, each with its own context
+ * on `device`, take the others largest first, then the label stages of
+ * the persistent launches' datasets.  HOST pointers: X[i] [n[i]*d[i]],
+ * labels[i] [n[i]] int64, probabilities[i] [n[i]] float32; ms_out [count]
+ * per-dataset wall times (the label stage alone for the persistent
+ * launches' datasets) or NULL.  No context argument: the contexts are
+ * created on first use and kept for later calls (released at process
+ * exit); calls on one device run one at a time.
  */
 int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
                      int32_t min_samples, int64_t min_cluster_size, int32_t workers, int64_t *const *labels,

@@ -47,3 +47,79 @@ def test_consecutive_batch_calls_reuse_workers():
             l1, p1 = fast_hdbscan(X, min_samples=5, min_cluster_size=20)
             np.testing.assert_array_equal(lab, l1)
             np.testing.assert_array_equal(pr, p1)
+
+
+def _batch_cases(rng):
+    data = []
+    for n, d in [(2, 2), (3, 3), (40, 2), (41, 2), (81, 2), (82, 3), (500, 1), (1000, 2), (4032, 2), (3392, 3),
+                 (2496, 5), (1792, 8), (2000, 4), (700, 6), (900, 7), (5000, 2), (2600, 5)]:
+        data.append(rng.standard_normal((n, d)).astype(np.float32))
+    # exact ties: an integer lattice and repeated points
+    g = np.stack(np.meshgrid(np.arange(30), np.arange(30)), -1).reshape(-1, 2).astype(np.float32)
+    data.append(g)
+    data.append(np.repeat(rng.standard_normal((60, 3)).astype(np.float32), 5, axis=0))
+    data.append(np.repeat(rng.integers(0, 4, size=(50, 2)).astype(np.float32), 3, axis=0))
+    data += workloads.batch_datasets(16, n_hi=6000)
+    return data
+
+
+@pytest.mark.parametrize("ms", [1, 5, 10, 15])
+def test_mst_batch_matches_per_call_path(ms):
+    # the persistent launch's rows, weights and cores against the per-call
+    # device path, bit for bit, over sizes up to one work-group's tree and
+    # past it, every dimension 1-8, ties and duplicates
+    from fast_hdbscan_amd.batch import mst_batch
+    from fast_hdbscan_amd.hdbscan import mst_arrays
+
+    data = _batch_cases(np.random.default_rng(ms))
+    print(f"mst_batch: {len(data)} datasets, min_samples={ms}", flush=True)
+    out = mst_batch(data, min_samples=ms)
+    served = 0
+    for i, (X, (src, dst, w, core, pers)) in enumerate(zip(data, out)):
+        n, d = X.shape
+        kq = ms + 1 if ms > 1 else 2
+        levels = int(np.log2(max(1, (n - 1) / 40)) + 1)
+        geometry = not (levels >= 2 and (n >> (levels - 2)) < 2)
+        caps = {1: 5056, 2: 4032, 3: 3392, 4: 2880, 5: 2496, 6: 2240, 7: 1984, 8: 1792}
+        assert pers == (n >= 2 and n <= caps[d] and kq <= 16 and geometry), (i, n, d)
+        served += pers
+        s1, d1, w1, c1, _ = mst_arrays(X, ms)
+        np.testing.assert_array_equal(core, c1, err_msg=f"dataset {i} n={n} d={d}")
+        np.testing.assert_array_equal(w, w1, err_msg=f"dataset {i} n={n} d={d}")
+        np.testing.assert_array_equal(src, s1, err_msg=f"dataset {i} n={n} d={d}")
+        np.testing.assert_array_equal(dst, d1, err_msg=f"dataset {i} n={n} d={d}")
+    assert served >= 10
+
+
+def test_mst_batch_matches_oracle():
+    from fast_hdbscan_amd.batch import mst_batch
+    from oracle import oracle as O
+
+    data = _batch_cases(np.random.default_rng(7))[:20:2]
+    out = mst_batch(data, min_samples=5)
+    for X, (src, dst, w, core, pers) in zip(data, out):
+        print(f"oracle case n={X.shape[0]} d={X.shape[1]} persistent={pers}", flush=True)
+        ref = O.boruvka(O.build_kdtree(X), 5)
+        np.testing.assert_array_equal(core, ref.core_sq)
+        np.testing.assert_array_equal(w, ref.w_sq)
+        np.testing.assert_array_equal(src, ref.src)
+        np.testing.assert_array_equal(dst, ref.dst)
+
+
+def test_mst_batch_consecutive_calls():
+    # the launches' buffers live across calls: smaller, larger and mixed
+    # batches after each other give the per-call path's rows
+    from fast_hdbscan_amd.batch import mst_batch
+    from fast_hdbscan_amd.hdbscan import mst_arrays
+
+    rng = np.random.default_rng(3)
+    small = [rng.standard_normal((int(n), 2)).astype(np.float32) for n in rng.integers(50, 400, 40)]
+    large = workloads.batch_datasets(12, n_hi=4000)
+    for data in (small, large, small[:3], large + small):
+        print(f"mst_batch call: {len(data)} datasets", flush=True)
+        for X, (src, dst, w, core, _) in zip(data, mst_batch(data, min_samples=5)):
+            s1, d1, w1, c1, _ = mst_arrays(X, 5)
+            np.testing.assert_array_equal(core, c1)
+            np.testing.assert_array_equal(w, w1)
+            np.testing.assert_array_equal(src, s1)
+            np.testing.assert_array_equal(dst, d1)
