@@ -451,35 +451,62 @@ def form_plscan_sweep():
 
 
 def form_batched(one_call_workers=None):
-    """Many independent small datasets through the native batch scheduler
-    (with one_call_workers: a single call on the 512 datasets, for
-    profiling)."""
+    """Many independent small datasets through the native batch scheduler:
+    datasets up to one work-group's tree in one persistent launch per
+    dimension, the others on worker streams (with one_call_workers: a single
+    call on the 512 datasets, for profiling)."""
+    import os
+
     from fast_hdbscan_amd import workloads
-    from fast_hdbscan_amd.batch import hdbscan_batch
+    from fast_hdbscan_amd.batch import hdbscan_batch, mst_batch
 
     data = workloads.batch_datasets(512)
     pts = sum(x.shape[0] for x in data)
     res = {"workload": "512 datasets, n log-uniform 500..50000, d in {2,3,5,8}, 3-20 Gaussians + 5% noise, "
                        "seeds 1000..1511; min_samples 5, min_cluster_size 20", "points": pts}
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
     if one_call_workers:
         w = int(one_call_workers)
-        t0 = time.perf_counter()
-        hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w)
-        el = time.perf_counter() - t0
+        el = timed(lambda: hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w))
         res[f"workers_{w}"] = {"s": el, "datasets_per_s": len(data) / el, "points_per_s": pts / el}
         return res
+    hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=8)  # contexts and buffers sized
     for w in (1, 8, 16):
-        t0 = time.perf_counter()
-        hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w)
-        el = time.perf_counter() - t0
+        el = timed(lambda: hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w))
         res[f"workers_{w}"] = {"s": el, "datasets_per_s": len(data) / el, "points_per_s": pts / el}
-    # the whole 4,096-dataset workload once, 8 workers
+    # the same with every dataset on the worker streams (no persistent launch)
+    os.environ["FH_BATCH_PERSISTENT"] = "0"
+    try:
+        hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=8)
+        el = timed(lambda: hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=8))
+    finally:
+        del os.environ["FH_BATCH_PERSISTENT"]
+    res["workers_8_streams_only"] = {"s": el, "datasets_per_s": len(data) / el, "points_per_s": pts / el}
+    # MSTs alone of the datasets the persistent launch takes, one call
+    out = mst_batch(data, min_samples=5)
+    small = [x for x, o in zip(data, out) if o[4]]
+    sp = sum(x.shape[0] for x in small)
+    el = min(timed(lambda: mst_batch(small, min_samples=5)) for _ in range(3))
+    os.environ["FH_BATCH_PERSISTENT"] = "0"
+    try:
+        el1 = timed(lambda: mst_batch(small, min_samples=5))
+    finally:
+        del os.environ["FH_BATCH_PERSISTENT"]
+    res["persistent_msts"] = {"datasets": len(small), "points": sp, "s": el, "datasets_per_s": len(small) / el,
+                              "points_per_s": sp / el, "one_at_a_time_s": el1,
+                              "speedup_vs_one_at_a_time": el1 / el}
+    # the whole 4,096-dataset workload once, 8 and 16 workers
     data = workloads.batch_datasets(4096)
     pts = sum(x.shape[0] for x in data)
-    t0 = time.perf_counter()
-    hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=8)
-    el = time.perf_counter() - t0
-    res["all_4096_workers_8"] = {"points": pts, "s": el, "datasets_per_s": len(data) / el, "points_per_s": pts / el}
+    for w in (8, 16):
+        el = timed(lambda: hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=w))
+        res[f"all_4096_workers_{w}"] = {"points": pts, "s": el, "datasets_per_s": len(data) / el,
+                                        "points_per_s": pts / el}
     return res
 
 

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <limits>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -113,6 +114,7 @@ struct fh_ctx {
     // scratch; a pinned host buffer stages the rows up and the outputs down
     DevBuf<float> bt_x, bt_w, bt_core;
     DevBuf<int> bt_src, bt_dst, bt_status, bt_next;
+    DevBuf<uint64_t> bt_ticks;
     DevBuf<fh::BatchJob> bt_jobs;
     DevBuf<unsigned char> bt_scratch;
     unsigned char *h_bt = nullptr;
@@ -152,7 +154,7 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->kr_w, c->kr_core, c->sk1, c->sk1s, c->sk2, c->sk2s, c->sv1, c->sv1s,
         c->pc_keys, c->pc_keys_s, c->pc_parent, c->pc_vals, c->pc_vals_s, c->pc_counts, c->pc_runs, c->pc_temp,
         c->stemp, c->bt_x, c->bt_w, c->bt_core, c->bt_src, c->bt_dst, c->bt_status, c->bt_next, c->bt_jobs,
-        c->bt_scratch);
+        c->bt_scratch, c->bt_ticks);
 }
 
 }  // namespace
@@ -1067,12 +1069,10 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
 // from a device-side queue, costliest first; the rows of every MST, the
 // core distances and a status per dataset come back with one copy each.
 
-// Whether the persistent kernel takes a dataset (FH_BATCH_PERSISTENT=0:
-// none): at most one work-group's tree in LDS, min_samples + 1 within its
-// heap size, and a tree whose leaves all sit on the last level.
+// Whether the persistent kernel can take a dataset: within its size limit,
+// min_samples + 1 within its heap size, and a tree whose leaves all sit on
+// the last level.  Which of these it is given is the router's choice.
 bool batch_eligible(int64_t n, int d, int ms) {
-    if (const char *m = std::getenv("FH_BATCH_PERSISTENT"))
-        if (std::atoi(m) == 0) return false;
     if (d < 1 || n < 2 || n > fh::batch_max_points(d)) return false;
     const int kq = ms > 1 ? ms + 1 : 2;
     if (kq > fh::batch_max_kq()) return false;
@@ -1090,6 +1090,7 @@ struct BatchGroup {
     int64_t total_n = 0, total_rows = 0;
     int grid = 0, max_n = 0, max_nodes = 0;
     int64_t slice = 0;
+    bool phases = false;
     const int *src = nullptr, *dst = nullptr, *status = nullptr;
     const float *w = nullptr, *core = nullptr;
 };
@@ -1201,6 +1202,13 @@ int batch_group_launch(BatchGroup &g, const float *const *X, int ms) {
     a.w = ctx->bt_w.p;
     a.core = ctx->bt_core.p;
     a.status = ctx->bt_status.p;
+    // FH_BATCH_PHASES=1: per-phase device time, printed once the launch ends
+    g.phases = std::getenv("FH_BATCH_PHASES") && std::atoi(std::getenv("FH_BATCH_PHASES")) != 0;
+    if (g.phases) {
+        FH_TRY(ensure(ctx, ctx->bt_ticks, 5));
+        FH_HIP(hipMemsetAsync(ctx->bt_ticks.p, 0, sizeof(uint64_t) * 5, st));
+        a.phase_ticks = ctx->bt_ticks.p;
+    }
     FH_HIP(fh::launch_batch(a, g.grid, st));
     FH_HIP(hipMemcpyAsync(hsrc, ctx->bt_src.p, sizeof(int) * ro, hipMemcpyDeviceToHost, st));
     FH_HIP(hipMemcpyAsync(hdst, ctx->bt_dst.p, sizeof(int) * ro, hipMemcpyDeviceToHost, st));
@@ -1214,6 +1222,18 @@ int batch_group_launch(BatchGroup &g, const float *const *X, int ms) {
 int batch_group_wait(BatchGroup &g) {
     fh_ctx *ctx = g.ctx;
     FH_HIP(hipStreamSynchronize(ctx->stream));
+    if (g.phases) {
+        uint64_t t[5];
+        FH_HIP(hipMemcpy(t, ctx->bt_ticks.p, sizeof(t), hipMemcpyDeviceToHost));
+        int rate = 0;  // kHz of the wall clock
+        FH_HIP(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, ctx->device));
+        const double ms_per_tick = rate > 0 ? 1.0 / rate : 0.0;
+        std::fprintf(stderr,
+                     "batch phases d=%d datasets=%zu grid=%d (work-group ms summed): tree %.1f knn %.1f forest %.1f "
+                     "searches %.1f rest %.1f\n",
+                     g.d, g.jobs.size(), g.grid, t[0] * ms_per_tick, t[1] * ms_per_tick, t[2] * ms_per_tick,
+                     t[3] * ms_per_tick, t[4] * ms_per_tick);
+    }
     for (size_t j = 0; j < g.jobs.size(); ++j)
         if (g.status[j] != g.jobs[j].n - 1)
             return fail(ctx, FH_EDEVICE, "persistent batch: dataset " + std::to_string(g.idx[j]) + " gave " +
@@ -1672,17 +1692,146 @@ int grow_pool(std::vector<fh_ctx *> &pool, int device, int size) {
     return FH_OK;
 }
 
-// The datasets the persistent kernel takes, one group per dimension, each
-// launched on its own context's stream; `rest` gets the others.
-int batch_start(int device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
-                int32_t ms, std::vector<BatchGroup> &groups, std::vector<int32_t> &rest) {
-    std::vector<int> by_d[FH_MAX_DIM + 1];
+// ---------------------------------------------------- the router ----
+//
+// Which datasets of a batch go to the persistent launches and which to the
+// per-call path on the worker streams, from a cost model whose constants
+// are measured on the device (once per device, dimension and min_samples):
+//   wg[d]: seconds per point for one work-group of the persistent kernel,
+//          from a launch of 2 x CUs synthetic 4000-point datasets;
+//   st[d]: seconds per point of one per-call MST alone (host arrays in and
+//          out), from a 32000-point synthetic dataset.
+// The persistent launches finish after max(sum of their work-group times /
+// resident work-groups, their longest work-group time); the streams after
+// the sum of their datasets' times over the streams that overlap (at most
+// the 4 hardware queues).  Starting with every dataset persistent, the
+// longest work-group jobs move to the streams while that shortens the
+// larger of the two.  FH_BATCH_MAX_N (points) replaces the model with a
+// fixed cap; FH_BATCH_PERSISTENT=0 sends everything to the streams.
+struct BatchCost {
+    int ms = -1;  // min_samples of the measurement (-1: none yet)
+    double wg = 0.0, st = 0.0;
+    int slots = 0;  // resident work-groups of the launch
+};
+BatchCost batch_cost[kNoAllocDevices][FH_MAX_DIM + 1];
+
+// count Gaussian-mixture datasets of n points in d dimensions (5 centres in
+// [-10, 10]^d, unit spread), deterministic
+std::vector<std::vector<float>> calibration_data(int count, int n, int d, uint64_t seed) {
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<float> cen(-10.0f, 10.0f);
+    std::normal_distribution<float> nrm(0.0f, 1.0f);
+    std::vector<std::vector<float>> out(count, std::vector<float>((size_t)n * d));
+    for (auto &X : out) {
+        float c[5][FH_MAX_DIM];
+        for (auto &cc : c)
+            for (int j = 0; j < d; ++j) cc[j] = cen(rng);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < d; ++j) X[(size_t)i * d + j] = c[i % 5][j] + nrm(rng);
+    }
+    return out;
+}
+
+int batch_calibrate(int device, int d, int ms, fh_ctx *gctx, fh_ctx *wctx) {
+    BatchCost &c = batch_cost[device][d];
+    if (c.ms == ms) return FH_OK;
+    fh_ctx *ctx = gctx;
+    int cus = 0;
+    FH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const int res = std::max(1, fh::batch_resident_per_cu(d));
+    const int n_wg = std::min(4000, fh::batch_subtree_points(d));
+    const int K = fh::batch_grid(1 << 30, cus);
+    auto data = calibration_data(K, n_wg, d, 0x5eed0000ull + d);
+    std::vector<const float *> xp(K);
+    std::vector<int64_t> ns(K, n_wg);
+    for (int i = 0; i < K; ++i) xp[i] = data[i].data();
+    double t = 0.0;
+    for (int rep = 0; rep < 2; ++rep) {  // the first sizes the buffers
+        BatchGroup g;
+        g.ctx = gctx;
+        g.d = d;
+        for (int i = 0; i < K; ++i) g.idx.push_back(i);
+        FH_TRY(batch_group_prepare(g, ns.data(), ms));
+        const Timer tm;
+        FH_TRY(batch_group_launch(g, xp.data(), ms));
+        FH_TRY(batch_group_wait(g));
+        t = tm.ms() * 1e-3;
+    }
+    const double waves = std::ceil(K / double(res * cus));
+    c.wg = t / (waves * n_wg);
+    c.slots = res * cus;
+    const int n_st = 32000;
+    auto one = calibration_data(1, n_st, d, 0x5eed1000ull + d);
+    std::vector<int32_t> src(n_st), dst(n_st);
+    std::vector<float> w(n_st);
+    for (int rep = 0; rep < 2; ++rep) {
+        const Timer tm;
+        FH_TRY(fh_mst_euclidean(wctx, one[0].data(), n_st, d, ms, nullptr, nullptr, nullptr, src.data(), dst.data(),
+                                w.data(), nullptr));
+        t = tm.ms() * 1e-3;
+    }
+    c.st = t / n_st;
+    c.ms = ms;
+    return FH_OK;
+}
+
+// Splits the batch: by_d[dd] the persistent launches' datasets, rest the
+// streams'.  streams: how many worker streams overlap.
+int batch_route(int device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d, int32_t ms,
+                int streams, std::vector<int> *by_d, std::vector<int32_t> &rest) {
+    std::vector<int32_t> cand;
     for (int32_t i = 0; i < count; ++i) {
         if (n[i] >= 2 && d[i] >= 1 && d[i] <= FH_MAX_DIM && X[i] && batch_eligible(n[i], d[i], ms))
-            by_d[d[i]].push_back(i);
+            cand.push_back(i);
         else
             rest.push_back(i);
     }
+    if (const char *m = std::getenv("FH_BATCH_PERSISTENT"))
+        if (std::atoi(m) == 0) {
+            rest.insert(rest.end(), cand.begin(), cand.end());
+            return FH_OK;
+        }
+    if (const char *m = std::getenv("FH_BATCH_MAX_N")) {
+        const int64_t cap = std::atoll(m);
+        for (int32_t i : cand) (n[i] <= cap ? by_d[d[i]].push_back(i) : rest.push_back(i));
+        return FH_OK;
+    }
+    // the model's constants for every dimension present
+    bool present[FH_MAX_DIM + 1] = {};
+    for (int32_t i : cand) present[d[i]] = true;
+    FH_TRY(grow_pool(batch_group_ctx[device], device, 1));
+    FH_TRY(grow_pool(batch_ctx[device], device, 1));
+    for (int dd = 1; dd <= FH_MAX_DIM; ++dd)
+        if (present[dd]) FH_TRY(batch_calibrate(device, dd, ms, batch_group_ctx[device][0], batch_ctx[device][0]));
+    const BatchCost *cost = batch_cost[device];
+    std::stable_sort(cand.begin(), cand.end(), [&](int32_t a, int32_t b) {
+        return n[a] * cost[d[a]].wg > n[b] * cost[d[b]].wg;
+    });
+    const double q = std::max(1, std::min(streams, 4));
+    double tp_sum = 0.0, ts = 0.0;
+    for (int32_t i : cand) tp_sum += n[i] * cost[d[i]].wg / cost[d[i]].slots;
+    size_t moved = 0;
+    for (; moved < cand.size(); ++moved) {
+        const int32_t i = cand[moved];
+        const double job = n[i] * cost[d[i]].wg;
+        const double now = std::max({tp_sum, job, ts});
+        const double next_job = moved + 1 < cand.size() ? n[cand[moved + 1]] * cost[d[cand[moved + 1]]].wg : 0.0;
+        const double tp2 = tp_sum - job / cost[d[i]].slots;
+        const double ts2 = ts + (n[i] * cost[d[i]].st + 3e-4) / q;
+        if (std::max({tp2, next_job, ts2}) >= now) break;
+        tp_sum = tp2;
+        ts = ts2;
+    }
+    for (size_t k = 0; k < cand.size(); ++k) (k < moved ? rest.push_back(cand[k]) : by_d[d[cand[k]]].push_back(cand[k]));
+    return FH_OK;
+}
+
+// The datasets the persistent kernel takes, one group per dimension, each
+// launched on its own context's stream; `rest` gets the others.
+int batch_start(int device, int32_t count, const float *const *X, const int64_t *n, const int32_t *d,
+                int32_t ms, int streams, std::vector<BatchGroup> &groups, std::vector<int32_t> &rest) {
+    std::vector<int> by_d[FH_MAX_DIM + 1];
+    FH_TRY(batch_route(device, count, X, n, d, ms, streams, by_d, rest));
     int ng = 0;
     for (int dd = 1; dd <= FH_MAX_DIM; ++dd) ng += by_d[dd].empty() ? 0 : 1;
     FH_TRY(grow_pool(batch_group_ctx[device], device, ng));
@@ -1717,7 +1866,7 @@ int fh_mst_batch(int32_t device, int32_t count, const float *const *X, const int
     if (hipSetDevice(device) != hipSuccess) return FH_EDEVICE;
     std::vector<BatchGroup> groups;
     std::vector<int32_t> rest;
-    FH_TRY(batch_start(device, count, X, n, d, min_samples, groups, rest));
+    FH_TRY(batch_start(device, count, X, n, d, min_samples, 1, groups, rest));
     if (persistent)
         for (int32_t i = 0; i < count; ++i) persistent[i] = 0;
     // the others one at a time on a worker context, beside the launches
@@ -1760,7 +1909,7 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
     // small datasets: persistent launches, started before the workers
     std::vector<BatchGroup> groups;
     std::vector<int32_t> queue;
-    FH_TRY(batch_start(device, count, X, n, d, min_samples, groups, queue));
+    FH_TRY(batch_start(device, count, X, n, d, min_samples, W, groups, queue));
     // The rest one at a time on the workers' streams, largest first: a
     // worker's buffers reach their final size on its first dataset instead
     // of growing (hipFree waits for the whole device) as larger ones arrive,

@@ -35,13 +35,20 @@ struct BatchArgs {
     float *w;                 // their squared weights
     float *core;              // squared core distance by point
     int *status;              // per job: rows written (n - 1), or -1 when a round made no progress
+    uint64_t *phase_ticks;    // optional [5]: wall-clock ticks of tree, kNN, forest, searches, rest of rounds
 };
 
 // Largest dataset (points) a work-group of the batch kernel takes in d
-// dimensions: its whole KD-tree is built in LDS; 0 when d is not handled.
+// dimensions; 0 when d is not handled.
 int batch_max_points(int d);
+// Largest dataset whose whole KD-tree one subtree build makes in LDS (the
+// levels above are built node by node over global memory).
+int batch_subtree_points(int d);
 // Largest neighbour count per query (min_samples + 1) the kernel takes.
 int batch_max_kq();
+// Work-groups of the d-dimensional kernel resident on one CU (registers
+// and LDS permitting); 0 when unknown.
+int batch_resident_per_cu(int d);
 // Scratch bytes of one work-group's slice.
 int64_t batch_slice_bytes(int max_n, int max_nodes, int max_d, int kq);
 // Work-groups of one launch: resident on every CU (two per CU), at most one

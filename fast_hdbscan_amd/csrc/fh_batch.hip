@@ -34,6 +34,7 @@ namespace {
 constexpr int kBT = kSubNT;  // threads per work-group: the subtree build's
 constexpr int kBW = kBT / 64;
 constexpr int kMaxKq = 16;
+constexpr int kMaxBatchN = 1 << 16;  // largest dataset a work-group takes
 // Two work-groups per CU: 80 KB each, less the 256 bytes the work-group
 // reductions of the subtree build (__syncthreads_or) take.
 constexpr int kLds = 81920 - 256;
@@ -53,6 +54,8 @@ static_assert(kMaxKq * kBT * 8 <= kLdsWork, "kNN heaps fit the work-group's LDS"
 #define FH_SLICE_ARRAYS(X)                 \
     X(int2, range, N)                      \
     X(float, box, N * 2 * d)               \
+    X(uint64_t, keys, n)                   \
+    X(int, part, n)                        \
     X(float, xp, n * d)                    \
     X(int, perm, n)                        \
     X(int, posof, n)                       \
@@ -189,6 +192,95 @@ __device__ void wg_cycle_keep(int m, const int *succ, int *S0, int *M0, int *S1,
     __syncthreads();
 }
 
+// Levels 0 .. l_sub - 1 of a dataset larger than one subtree, node after
+// node by the whole work-group over global memory: the node's bounds
+// (numba_kdtree.py:96-105, as ordered integers), split axis (first strictly
+// largest spread, 133-152), keys, and the exact introselect of 353-407
+// (wg_select_step's partition steps; a range of at most 16, or one at the
+// depth limit, sorted).  Leaves s.keys as the subtree builds of level l_sub
+// read them.  lred: kBW * 2 * D words of LDS; ws, ws2: kBW ints each.
+template <int D>
+__device__ void build_top_levels(const float *__restrict__ X, int l_sub, const Slice &s, uint32_t *lred, int *ws,
+                                 int *ws2) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int l = 0; l < l_sub; ++l) {
+        for (int k = 0; k < (1 << l); ++k) {
+            const int node = (1 << l) - 1 + k;
+            const int2 r = s.range[node];
+            const int size = r.y - r.x;
+            uint32_t mn[D], mx[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                mn[j] = 0xffffffffu;
+                mx[j] = 0u;
+            }
+            for (int p = r.x + tid; p < r.y; p += kBT) {
+                const uint32_t idx = l == 0 ? (uint32_t)p : key_idx(s.keys[p]);
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    const uint32_t o = f2o(X[(size_t)idx * D + j]);
+                    mn[j] = min(mn[j], o);
+                    mx[j] = max(mx[j], o);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    mn[j] = min(mn[j], (uint32_t)__shfl_xor((int)mn[j], o, 64));
+                    mx[j] = max(mx[j], (uint32_t)__shfl_xor((int)mx[j], o, 64));
+                }
+                if (lane == 0) {
+                    lred[wv * 2 * D + j] = mn[j];
+                    lred[wv * 2 * D + D + j] = mx[j];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                mn[j] = lred[j];
+                mx[j] = lred[D + j];
+                for (int w = 1; w < kBW; ++w) {
+                    mn[j] = min(mn[j], lred[w * 2 * D + j]);
+                    mx[j] = max(mx[j], lred[w * 2 * D + D + j]);
+                }
+            }
+            if (tid < 2 * D) {
+                uint32_t v = lred[tid];
+                for (int w = 1; w < kBW; ++w) {
+                    const uint32_t x = lred[w * 2 * D + tid];
+                    v = tid < D ? min(v, x) : max(v, x);
+                }
+                s.box[(size_t)node * 2 * D + tid] = o2f(v);
+            }
+            __syncthreads();
+            const int ax = sub_axis<D>(mn, mx);
+            for (int p = r.x + tid; p < r.y; p += kBT) {
+                const uint32_t idx = l == 0 ? (uint32_t)p : key_idx(s.keys[p]);
+                s.keys[p] = make_key(X[(size_t)idx * D + ax], idx);
+            }
+            __syncthreads();
+            int lo = r.x, hi = r.y, depth = size > 16 ? 2 * floor_log2(size) : 0;
+            const int nth = r.x + size / 2;
+            for (;;) {
+                if (hi - lo <= 16 || depth == 0) {
+                    lds_sort(s.keys + lo, hi - lo);  // the same network, here over global memory
+                    break;
+                }
+                --depth;
+                const int P = wg_select_step<kBT>(s.keys, s.part, lo, hi, ws, ws2);
+                if (nth < P)
+                    hi = P;
+                else if (nth > P)
+                    lo = P + 1;
+                else
+                    break;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // The exact MST of one dataset by the calling work-group.
 template <int D>
 __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, const Slice &s, unsigned char *lds,
@@ -197,6 +289,16 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
     const int n = job.n, L = job.levels;
     const int N = (1 << L) - 1;
     const int kq = a.kq, kn = kq - 1, ms = a.ms;
+    // optional per-phase time (a.phase_ticks: tree, kNN, forest, searches,
+    // the rest of the rounds; wall-clock ticks summed over datasets)
+    uint64_t t_mark = a.phase_ticks ? wall_clock64() : 0;
+    auto phase = [&](int k) {
+        if (a.phase_ticks && tid == 0) {
+            const uint64_t t = wall_clock64();
+            atomicAdd(reinterpret_cast<unsigned long long *>(a.phase_ticks + k), (unsigned long long)(t - t_mark));
+            t_mark = t;
+        }
+    };
     int *src = a.src + job.row_off;
     int *dst = a.dst + job.row_off;
     float *wout = a.w + job.row_off;
@@ -206,21 +308,32 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
     for (int i = tid; i < N; i += kBT) s.range[i] = node_range_of(i, n);
     __syncthreads();
     {
+        // the first level whose nodes fit one subtree build
+        int l_sub = 0;
+        while (((n + (1 << l_sub) - 1) >> l_sub) > sub_cap<D>()) ++l_sub;
+        if (l_sub > 0) {
+            uint32_t *lred = reinterpret_cast<uint32_t *>(lds + 64);
+            int *tws = reinterpret_cast<int *>(lred + kBW * 2 * D);
+            build_top_levels<D>(a.X + job.x_off, l_sub, s, lred, tws, tws + kBW);
+        }
         SubtreeArgs t{};
         t.X = a.X + job.x_off;
         t.range = s.range;
-        t.A = nullptr;
-        t.identity = 1;
-        t.level0 = 0;
+        t.A = s.keys;
+        t.identity = l_sub == 0 ? 1 : 0;
+        t.level0 = l_sub;
         t.levels = L;
         t.box = s.box;
         t.perm = s.perm;
         t.posof = s.posof;
         t.xp = s.xp;
-        subtree_build<D>(t, 0, lds);
+        for (int k0 = 0; k0 < (1 << l_sub); ++k0) {
+            subtree_build<D>(t, k0, lds);
+            __syncthreads();
+        }
     }
-    __syncthreads();
 
+    phase(0);
     // -- kNN of every tree position (numba_kdtree.py:765-792), rows in tree
     // order: columns 1..k-1 as positions and distances, the core by position
     {
@@ -251,6 +364,7 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
     }
     __syncthreads();
 
+    phase(1);
     // -- kNN-seeded forest (boruvka.py:474-524), edges by point index
     for (int p = tid; p < n; p += kBT) {
         const ForestEdge e = forest_edge_at(p, n, kq, s.nb_pos, ms <= 1 ? s.nb_d : nullptr, s.core_p, s.perm, s.posof);
@@ -287,6 +401,7 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
     }
     __syncthreads();
 
+    phase(2);
     // -- Boruvka rounds (boruvka.py:604-640)
     QueryArgs qa{};
     qa.xp = s.xp;
@@ -346,8 +461,10 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
         __syncthreads();
         // nearest out-of-component edge of every point, one wavefront per 64
         // consecutive tree positions
+        phase(4);
         for (int p0 = wv * 64; p0 < n; p0 += kBT) wave_search<D, 0>(qa, p0 + lane, sx, sc, sk);
         __syncthreads();
+        phase(3);
         // merge_components (boruvka.py:34-78): each component's edge, the
         // cycle skip, rows in slot (= lowest point) order, unions
         for (int t = tid; t < C; t += kBT) {
@@ -398,6 +515,7 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
         roots = roots2;
         roots2 = t;
     }
+    phase(4);
     for (int i = tid; i < n; i += kBT) a.core[job.pt_off + i] = s.core_p[s.posof[i]];
     if (tid == 0) a.status[ji] = ok ? base : -1;
     __syncthreads();
@@ -424,7 +542,9 @@ __global__ __launch_bounds__(kBT) void batch_kernel(BatchArgs a) {
 
 }  // namespace
 
-int batch_max_points(int d) {
+int batch_max_points(int d) { return d >= 1 && d <= 8 ? kMaxBatchN : 0; }
+
+int batch_subtree_points(int d) {
     switch (d) {
         case 1: return sub_cap<1>();
         case 2: return sub_cap<2>();
@@ -439,6 +559,23 @@ int batch_max_points(int d) {
 }
 
 int batch_max_kq() { return kMaxKq; }
+
+int batch_resident_per_cu(int d) {
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (d) {
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<1>, kBT, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<2>, kBT, 0); break;
+        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<3>, kBT, 0); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<4>, kBT, 0); break;
+        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<5>, kBT, 0); break;
+        case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<6>, kBT, 0); break;
+        case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<7>, kBT, 0); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<8>, kBT, 0); break;
+        default: break;
+    }
+    return e == hipSuccess ? nb : 0;
+}
 
 int64_t batch_slice_bytes(int max_n, int max_nodes, int max_d, int kq) {
     return slice_size(max_n, max_nodes, max_d, kq);

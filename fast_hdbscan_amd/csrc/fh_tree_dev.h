@@ -183,6 +183,111 @@ __device__ __forceinline__ int sub_count_pass(const uint64_t *sk, uint64_t pivot
     return S;
 }
 
+__device__ __forceinline__ void swap64(uint64_t *A, int i, int j) {
+    uint64_t t = A[i];
+    A[i] = A[j];
+    A[j] = t;
+}
+
+// Ascending sort of n keys in LDS with a bitonic network whose merge steps
+// all point the same way (the first step of each merge compares mirrored
+// positions), so positions >= n behave as +inf and need no storage.
+__device__ void lds_sort(uint64_t *sk, int n) {
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int size = 2; size <= m; size <<= 1) {
+        for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
+            const int half = size >> 1;
+            const int blk = t / half, off = t % half;
+            const int lo = blk * size + off;
+            const int hi = blk * size + size - 1 - off;
+            if (hi < n && sk[lo] > sk[hi]) {
+                const uint64_t x = sk[lo];
+                sk[lo] = sk[hi];
+                sk[hi] = x;
+            }
+        }
+        __syncthreads();
+        for (int stride = size >> 2; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
+                const int lo = 2 * stride * (t / stride) + (t % stride);
+                const int hi = lo + stride;
+                if (hi < n && sk[lo] > sk[hi]) {
+                    const uint64_t x = sk[lo];
+                    sk[lo] = sk[hi];
+                    sk[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// One partition step of introselect on keys[L, R) (pivot of numba_kdtree.py:
+// 280-299, the two-pointer exchanges of 313-350 as one parallel exchange,
+// the pivot move of 349) by one work-group of NT threads, keys in global
+// memory: every thread reads the median-of-three candidates, each wave
+// counts and ranks one contiguous (NT / 64)-th of [L, R - 1) with coalesced
+// loads, and part[] (the scratch array of the multi-block steps) holds the
+// exchange partners: the m-th ">= pivot" key left of P at L + m, the m-th
+// "< pivot" key from the right at R - 2 - m.  Returns the pivot's final
+// position P.
+template <int NT>
+__device__ int wg_select_step(uint64_t *keys, int *part, int L, int R, int *ws, int *ws2) {
+    constexpr int kMW = NT / 64;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int mid = (L + R - 1) / 2;
+    uint64_t k0 = keys[L], k1 = keys[mid], k2 = keys[R - 1];
+    if (k0 > k1) {
+        const uint64_t t = k0;
+        k0 = k1;
+        k1 = t;
+    }
+    if (k1 > k2) {
+        const uint64_t t = k1;
+        k1 = k2;
+        k2 = t;
+        if (k0 > k1) {
+            const uint64_t u = k0;
+            k0 = k1;
+            k1 = u;
+        }
+    }
+    // the median moves to R - 1 and becomes the pivot
+    const uint64_t pivot = k1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        keys[L] = k0;
+        keys[mid] = k2;
+        keys[R - 1] = k1;
+    }
+    __syncthreads();
+    const int len = R - 1 - L;
+    const int b = L + (int)((int64_t)len * w / kMW), e = L + (int)((int64_t)len * (w + 1) / kMW);
+    const int c = sub_count_pass(keys, pivot, b, e);
+    if (lane == 0) ws[w] = c;
+    __syncthreads();
+    int S = 0, pre = 0;
+#pragma unroll
+    for (int i = 0; i < kMW; ++i) {
+        const int v = ws[i];
+        S += v;
+        pre += i < w ? v : 0;
+    }
+    const int P = L + S;
+    const int mw = sub_rank_pass(keys, part, pivot, L, R, S, b, e, pre);
+    if (lane == 0) ws2[w] = mw;
+    __syncthreads();
+    int M = 0;
+#pragma unroll
+    for (int i = 0; i < kMW; ++i) M += ws2[i];
+    for (int m = threadIdx.x; m < M; m += NT) swap64(keys, part[L + m], part[R - 2 - m]);
+    __syncthreads();
+    if (threadIdx.x == 0) swap64(keys, P, R - 1);
+    __syncthreads();
+    return P;
+}
+
 // ---------------------------------------- one work-group per subtree ----
 //
 // All levels from level0 down of a subtree whose root holds at most

@@ -64,13 +64,15 @@ def _batch_cases(rng):
 
 
 @pytest.mark.parametrize("ms", [1, 5, 10, 15])
-def test_mst_batch_matches_per_call_path(ms):
+def test_mst_batch_matches_per_call_path(ms, monkeypatch):
     # the persistent launch's rows, weights and cores against the per-call
     # device path, bit for bit, over sizes up to one work-group's tree and
-    # past it, every dimension 1-8, ties and duplicates
+    # past it, every dimension 1-8, ties and duplicates (a fixed routing
+    # cap: every dataset the kernel can take goes to it)
     from fast_hdbscan_amd.batch import mst_batch
     from fast_hdbscan_amd.hdbscan import mst_arrays
 
+    monkeypatch.setenv("FH_BATCH_MAX_N", "65536")
     data = _batch_cases(np.random.default_rng(ms))
     print(f"mst_batch: {len(data)} datasets, min_samples={ms}", flush=True)
     out = mst_batch(data, min_samples=ms)
@@ -80,8 +82,7 @@ def test_mst_batch_matches_per_call_path(ms):
         kq = ms + 1 if ms > 1 else 2
         levels = int(np.log2(max(1, (n - 1) / 40)) + 1)
         geometry = not (levels >= 2 and (n >> (levels - 2)) < 2)
-        caps = {1: 5056, 2: 4032, 3: 3392, 4: 2880, 5: 2496, 6: 2240, 7: 1984, 8: 1792}
-        assert pers == (n >= 2 and n <= caps[d] and kq <= 16 and geometry), (i, n, d)
+        assert pers == (n >= 2 and n <= 65536 and kq <= 16 and geometry), (i, n, d)
         served += pers
         s1, d1, w1, c1, _ = mst_arrays(X, ms)
         np.testing.assert_array_equal(core, c1, err_msg=f"dataset {i} n={n} d={d}")
@@ -91,9 +92,11 @@ def test_mst_batch_matches_per_call_path(ms):
     assert served >= 10
 
 
-def test_mst_batch_matches_oracle():
+def test_mst_batch_matches_oracle(monkeypatch):
     from fast_hdbscan_amd.batch import mst_batch
     from oracle import oracle as O
+
+    monkeypatch.setenv("FH_BATCH_MAX_N", "4000")
 
     data = _batch_cases(np.random.default_rng(7))[:20:2]
     out = mst_batch(data, min_samples=5)
@@ -123,3 +126,51 @@ def test_mst_batch_consecutive_calls():
             np.testing.assert_array_equal(w, w1)
             np.testing.assert_array_equal(src, s1)
             np.testing.assert_array_equal(dst, d1)
+
+
+def test_mst_batch_large_datasets_in_the_persistent_kernel(monkeypatch):
+    # datasets above one subtree build: the top tree levels built node by
+    # node by the work-group, then the subtrees (FH_BATCH_MAX_N raises the
+    # routing cap for this call)
+    from fast_hdbscan_amd.batch import mst_batch
+    from fast_hdbscan_amd.hdbscan import mst_arrays
+
+    monkeypatch.setenv("FH_BATCH_MAX_N", "65536")
+    rng = np.random.default_rng(11)
+    data = [rng.standard_normal((n, d)).astype(np.float32)
+            for n, d in [(4033, 2), (9000, 2), (20000, 3), (65536, 2), (12000, 5), (30000, 8), (6000, 1), (8193, 4)]]
+    data += workloads.batch_datasets(8, n_lo=5000, n_hi=40000)
+    data.append(np.repeat(rng.integers(0, 30, size=(3000, 2)).astype(np.float32), 3, axis=0))
+    for ms in (5, 10):
+        print(f"large mst_batch: {len(data)} datasets, min_samples={ms}", flush=True)
+        out = mst_batch(data, min_samples=ms)
+        for i, (X, (src, dst, w, core, pers)) in enumerate(zip(data, out)):
+            assert pers, (i, X.shape)
+            s1, d1, w1, c1, _ = mst_arrays(X, ms)
+            np.testing.assert_array_equal(core, c1, err_msg=f"dataset {i} {X.shape}")
+            np.testing.assert_array_equal(w, w1, err_msg=f"dataset {i} {X.shape}")
+            np.testing.assert_array_equal(src, s1, err_msg=f"dataset {i} {X.shape}")
+            np.testing.assert_array_equal(dst, d1, err_msg=f"dataset {i} {X.shape}")
+
+
+def test_router_splits_change_nothing(monkeypatch):
+    # the measured cost model's split, everything on the streams, and
+    # everything the kernel can take in it: the same rows
+    from fast_hdbscan_amd.batch import mst_batch
+
+    data = workloads.batch_datasets(24, n_hi=30000) + [np.random.default_rng(5).standard_normal((60000, 2)).astype(np.float32)]
+    base = mst_batch(data, min_samples=5)
+    print("model split: persistent", sum(o[4] for o in base), "of", len(data), flush=True)
+    for env in ({"FH_BATCH_PERSISTENT": "0"}, {"FH_BATCH_MAX_N": "65536"}):
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        out = mst_batch(data, min_samples=5)
+        for k in env:
+            monkeypatch.delenv(k)
+        if "FH_BATCH_PERSISTENT" in env:
+            assert not any(o[4] for o in out)
+        else:
+            assert all(o[4] for o in out)
+        for a, b in zip(base, out):
+            for x, y in zip(a[:4], b[:4]):
+                np.testing.assert_array_equal(x, y)
