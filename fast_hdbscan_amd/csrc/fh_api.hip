@@ -1698,7 +1698,8 @@ int grow_pool(std::vector<fh_ctx *> &pool, int device, int size) {
 // per-call path on the worker streams, from a cost model whose constants
 // are measured on the device (once per device, dimension and min_samples):
 //   wg[d]: seconds per point for one work-group of the persistent kernel,
-//          from a launch of 2 x CUs synthetic 4000-point datasets;
+//          from launches of one synthetic dataset per resident work-group
+//          at 2000 and at 16000 points (a power law in between and beyond);
 //   st[d]: seconds per point of one per-call MST alone (host arrays in and
 //          out), from a 32000-point synthetic dataset.
 // The persistent launches finish after max(sum of their work-group times /
@@ -1710,8 +1711,11 @@ int grow_pool(std::vector<fh_ctx *> &pool, int device, int size) {
 // fixed cap; FH_BATCH_PERSISTENT=0 sends everything to the streams.
 struct BatchCost {
     int ms = -1;  // min_samples of the measurement (-1: none yet)
-    double wg = 0.0, st = 0.0;
+    double wg1 = 0.0, n1 = 1.0, p = 0.0;  // work-group seconds per point at n1 points, growing as (n / n1)^p
+    double st = 0.0;
     int slots = 0;  // resident work-groups of the launch
+    // one work-group's time for a dataset of n points
+    double wg(int64_t n) const { return n * wg1 * std::pow(std::max(1.0, double(n) / n1), p); }
 };
 BatchCost batch_cost[kNoAllocDevices][FH_MAX_DIM + 1];
 
@@ -1732,6 +1736,31 @@ std::vector<std::vector<float>> calibration_data(int count, int n, int d, uint64
     return out;
 }
 
+// Seconds per point of one work-group on datasets of n points: one launch
+// of one dataset per resident work-group (the second of two runs).
+int batch_time_wg(int d, int ms, int n, int count, fh_ctx *gctx, double &out) {
+    fh_ctx *ctx = gctx;
+    auto data = calibration_data(count, n, d, 0x5eed0000ull + 131 * d + n);
+    std::vector<const float *> xp(count);
+    std::vector<int64_t> ns(count, n);
+    for (int i = 0; i < count; ++i) xp[i] = data[i].data();
+    double t = 0.0;
+    for (int rep = 0; rep < 2; ++rep) {
+        BatchGroup g;
+        g.ctx = gctx;
+        g.d = d;
+        for (int i = 0; i < count; ++i) g.idx.push_back(i);
+        FH_TRY(batch_group_prepare(g, ns.data(), ms));
+        const Timer tm;
+        FH_TRY(batch_group_launch(g, xp.data(), ms));
+        FH_TRY(batch_group_wait(g));
+        t = tm.ms() * 1e-3;
+    }
+    out = t / n;
+    (void)ctx;
+    return FH_OK;
+}
+
 int batch_calibrate(int device, int d, int ms, fh_ctx *gctx, fh_ctx *wctx) {
     BatchCost &c = batch_cost[device][d];
     if (c.ms == ms) return FH_OK;
@@ -1739,31 +1768,20 @@ int batch_calibrate(int device, int d, int ms, fh_ctx *gctx, fh_ctx *wctx) {
     int cus = 0;
     FH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     const int res = std::max(1, fh::batch_resident_per_cu(d));
-    const int n_wg = std::min(4000, fh::batch_subtree_points(d));
-    const int K = fh::batch_grid(1 << 30, cus);
-    auto data = calibration_data(K, n_wg, d, 0x5eed0000ull + d);
-    std::vector<const float *> xp(K);
-    std::vector<int64_t> ns(K, n_wg);
-    for (int i = 0; i < K; ++i) xp[i] = data[i].data();
-    double t = 0.0;
-    for (int rep = 0; rep < 2; ++rep) {  // the first sizes the buffers
-        BatchGroup g;
-        g.ctx = gctx;
-        g.d = d;
-        for (int i = 0; i < K; ++i) g.idx.push_back(i);
-        FH_TRY(batch_group_prepare(g, ns.data(), ms));
-        const Timer tm;
-        FH_TRY(batch_group_launch(g, xp.data(), ms));
-        FH_TRY(batch_group_wait(g));
-        t = tm.ms() * 1e-3;
-    }
-    const double waves = std::ceil(K / double(res * cus));
-    c.wg = t / (waves * n_wg);
-    c.slots = res * cus;
+    const int slots = std::max(1, std::min(res * cus, fh::batch_grid(1 << 30, cus)));
+    const int n1 = 2000, n2 = 16000;
+    double c1 = 0.0, c2 = 0.0;
+    FH_TRY(batch_time_wg(d, ms, n1, slots, gctx, c1));
+    FH_TRY(batch_time_wg(d, ms, n2, slots, gctx, c2));
+    c.wg1 = c1;
+    c.n1 = n1;
+    c.p = std::min(1.0, std::max(0.0, std::log(c2 / c1) / std::log(double(n2) / n1)));
+    c.slots = slots;
     const int n_st = 32000;
     auto one = calibration_data(1, n_st, d, 0x5eed1000ull + d);
     std::vector<int32_t> src(n_st), dst(n_st);
     std::vector<float> w(n_st);
+    double t = 0.0;
     for (int rep = 0; rep < 2; ++rep) {
         const Timer tm;
         FH_TRY(fh_mst_euclidean(wctx, one[0].data(), n_st, d, ms, nullptr, nullptr, nullptr, src.data(), dst.data(),
@@ -1772,6 +1790,9 @@ int batch_calibrate(int device, int d, int ms, fh_ctx *gctx, fh_ctx *wctx) {
     }
     c.st = t / n_st;
     c.ms = ms;
+    if (std::getenv("FH_BATCH_PHASES") && std::atoi(std::getenv("FH_BATCH_PHASES")) != 0)
+        std::fprintf(stderr, "batch cost d=%d: work-group %.3g s/pt at %d points (exponent %.2f), alone %.3g s/pt, %d slots\n",
+                     d, c.wg1, n1, c.p, c.st, c.slots);
     return FH_OK;
 }
 
@@ -1804,18 +1825,17 @@ int batch_route(int device, int32_t count, const float *const *X, const int64_t 
     for (int dd = 1; dd <= FH_MAX_DIM; ++dd)
         if (present[dd]) FH_TRY(batch_calibrate(device, dd, ms, batch_group_ctx[device][0], batch_ctx[device][0]));
     const BatchCost *cost = batch_cost[device];
-    std::stable_sort(cand.begin(), cand.end(), [&](int32_t a, int32_t b) {
-        return n[a] * cost[d[a]].wg > n[b] * cost[d[b]].wg;
-    });
+    std::stable_sort(cand.begin(), cand.end(),
+                     [&](int32_t a, int32_t b) { return cost[d[a]].wg(n[a]) > cost[d[b]].wg(n[b]); });
     const double q = std::max(1, std::min(streams, 4));
     double tp_sum = 0.0, ts = 0.0;
-    for (int32_t i : cand) tp_sum += n[i] * cost[d[i]].wg / cost[d[i]].slots;
+    for (int32_t i : cand) tp_sum += cost[d[i]].wg(n[i]) / cost[d[i]].slots;
     size_t moved = 0;
     for (; moved < cand.size(); ++moved) {
         const int32_t i = cand[moved];
-        const double job = n[i] * cost[d[i]].wg;
+        const double job = cost[d[i]].wg(n[i]);
         const double now = std::max({tp_sum, job, ts});
-        const double next_job = moved + 1 < cand.size() ? n[cand[moved + 1]] * cost[d[cand[moved + 1]]].wg : 0.0;
+        const double next_job = moved + 1 < cand.size() ? cost[d[cand[moved + 1]]].wg(n[cand[moved + 1]]) : 0.0;
         const double tp2 = tp_sum - job / cost[d[i]].slots;
         const double ts2 = ts + (n[i] * cost[d[i]].st + 3e-4) / q;
         if (std::max({tp2, next_job, ts2}) >= now) break;
@@ -1919,16 +1939,19 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
     });
     const int nq = (int)queue.size();
     // the persistent launches' datasets, for the label stage once they are back
-    std::vector<std::pair<int, int>> small;  // (group, job)
-    for (int gi = 0; gi < (int)groups.size(); ++gi)
-        for (int j = 0; j < (int)groups[gi].idx.size(); ++j) small.emplace_back(gi, j);
+    // (group, job), published group by group as the launches complete
+    std::vector<std::pair<int, int>> small;
+    size_t n_small = 0;
+    for (const auto &g : groups) n_small += g.idx.size();
+    small.reserve(n_small);
     std::vector<fh_ctx *> &pool = batch_ctx[device];
     FH_TRY(grow_pool(pool, device, W));
     std::atomic<int> next{0}, next_small{0};
     std::atomic<int> err{0};
     std::mutex ready_mu;
     std::condition_variable ready_cv;
-    bool ready = false;
+    int published = 0;
+    bool all_done = false;
     auto work = [&](int wi) {
         fh_ctx *ctx = pool[wi];
         if (hipSetDevice(device) != hipSuccess) {
@@ -1989,13 +2012,14 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
             if (ms_out)
                 ms_out[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
-        {
-            std::unique_lock<std::mutex> lk(ready_mu);
-            ready_cv.wait(lk, [&] { return ready; });
-        }
         for (;;) {
             const int si = next_small.fetch_add(1);
-            if (si >= (int)small.size() || err.load() != 0) break;
+            {
+                std::unique_lock<std::mutex> lk(ready_mu);
+                ready_cv.wait(lk, [&] { return si < published || all_done; });
+                if (si >= published) break;  // every launch is back and its jobs taken
+            }
+            if (err.load() != 0) break;
             const BatchGroup &g = groups[small[si].first];
             const int j = small[si].second;
             const int i = g.idx[j];
@@ -2016,13 +2040,33 @@ int fh_hdbscan_batch(int32_t device, int32_t count, const float *const *X, const
     };
     std::vector<std::thread> th;
     for (int t = 0; t < W; ++t) th.emplace_back(work, t);
-    for (auto &g : groups) {
-        const int rc = batch_group_wait(g);
-        if (rc != FH_OK) err = rc;
+    // each launch's label stages start as soon as it is back
+    std::vector<int> pending;
+    for (int gi = 0; gi < (int)groups.size(); ++gi) pending.push_back(gi);
+    while (!pending.empty()) {
+        bool any = false;
+        for (size_t k = 0; k < pending.size();) {
+            BatchGroup &g = groups[pending[k]];
+            if (hipStreamQuery(g.ctx->stream) == hipErrorNotReady) {
+                ++k;
+                continue;
+            }
+            const int rc = batch_group_wait(g);
+            if (rc != FH_OK) err = rc;
+            {
+                std::lock_guard<std::mutex> lk(ready_mu);
+                for (int j = 0; j < (int)g.idx.size(); ++j) small.emplace_back(pending[k], j);
+                published = (int)small.size();
+            }
+            ready_cv.notify_all();
+            pending.erase(pending.begin() + k);
+            any = true;
+        }
+        if (!any) std::this_thread::sleep_for(std::chrono::microseconds(200));
     }
     {
         std::lock_guard<std::mutex> lk(ready_mu);
-        ready = true;
+        all_done = true;
     }
     ready_cv.notify_all();
     for (auto &x : th) x.join();
