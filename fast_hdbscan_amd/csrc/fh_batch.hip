@@ -35,6 +35,10 @@ constexpr int kBT = kSubNT;  // threads per work-group: the subtree build's
 constexpr int kBW = kBT / 64;
 constexpr int kMaxKq = 16;
 constexpr int kMaxBatchN = 1 << 16;  // largest dataset a work-group takes
+// rows read ahead in the kNN leaves and the search's staged leaves (4 and 2
+// remove the register spills but measured no faster)
+constexpr int kBatchLeafBatch = 8;
+constexpr int kBatchStageBatch = 4;
 // Two work-groups per CU: 80 KB each, less the 256 bytes the work-group
 // reductions of the subtree build (__syncthreads_or) take.
 constexpr int kLds = 81920 - 256;
@@ -344,7 +348,7 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
         h.k = kq;
         for (int q = tid; q < n; q += kBT) {
             h.init();
-            KnnVisitor<D> v;
+            KnnVisitor<D, kBatchLeafBatch> v;
             v.x.load(s.xp + (size_t)q * D, D);
             v.xp = s.xp;
             v.range = s.range;
@@ -462,7 +466,7 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
         // nearest out-of-component edge of every point, one wavefront per 64
         // consecutive tree positions
         phase(4);
-        for (int p0 = wv * 64; p0 < n; p0 += kBT) wave_search<D, 0>(qa, p0 + lane, sx, sc, sk);
+        for (int p0 = wv * 64; p0 < n; p0 += kBT) wave_search<D, 0, kBatchStageBatch>(qa, p0 + lane, sx, sc, sk);
         __syncthreads();
         phase(3);
         // merge_components (boruvka.py:34-78): each component's edge, the

@@ -111,7 +111,8 @@ struct QueryPoint<0> {
 
 // ---------------------------------------------------------------- kNN ----
 
-template <int D>
+// LB: leaf rows loaded ahead of their distances (registers: LB * D).
+template <int D, int LB = 8>
 struct KnnVisitor {
     QueryPoint<D> x;
     const float *__restrict__ xp;
@@ -139,13 +140,13 @@ struct KnnVisitor {
     }
 
     // The leaf's points in position order; for a compile-time dimension
-    // their rows are loaded kLeafBatch at a time ahead of the distances, so
+    // their rows are loaded LB at a time ahead of the distances, so
     // the loads overlap (the pushes still follow position order).
     __device__ __forceinline__ void leaf(int node) {
         const int2 r = range[node];
         const bool bounded = bound < kInf;
         if constexpr (D > 0) {
-            constexpr int kLeafBatch = 8;
+            constexpr int kLeafBatch = LB;
             int i0 = r.x;
             for (; i0 + kLeafBatch <= r.y; i0 += kLeafBatch) {
                 float pt[kLeafBatch][D];
@@ -156,7 +157,7 @@ struct KnnVisitor {
 #pragma unroll
                 for (int u = 0; u < kLeafBatch; ++u) consider(rdist<D>(x.p(), pt[u], D), i0 + u, bounded);
             }
-            if (i0 + 4 <= r.y) {
+            if (LB > 4 && i0 + 4 <= r.y) {
                 float pt[4][D];
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
@@ -440,7 +441,8 @@ constexpr int kWaveStage = 128;  // staged points per leaf (leaves hold < 2 * le
 // The search of one wavefront whose lane l holds tree position pos (its
 // wave's positions are consecutive); sx, sc, sk: this wave's LDS staging
 // (kWaveStage points).
-template <int D, int PASS>
+// SB: staged points read ahead of their tests (registers: SB * (D + 2)).
+template <int D, int PASS, int SB = 4>
 __device__ __forceinline__ void wave_search(const QueryArgs &a, int64_t pos, float *sx, float *sc, int *sk) {
     const int lane = threadIdx.x & 63;
     bool act = pos < a.n;
@@ -506,7 +508,7 @@ __device__ __forceinline__ void wave_search(const QueryArgs &a, int64_t pos, flo
                     wave_sync();
                     if (t) {
                         B = ld_relaxed(a.bound + comp);
-                        // staged points read four at a time ahead of the tests
+                        // staged points read SB at a time ahead of the tests
                         // (still taken in position order)
                         auto scan = [&](int q, int kq, float cj, const float *y) {
                             if (kq == comp || !(cj <= B)) return;
@@ -526,18 +528,18 @@ __device__ __forceinline__ void wave_search(const QueryArgs &a, int64_t pos, flo
                             }
                         };
                         int q = 0;
-                        for (; q + 4 <= cnt; q += 4) {
-                            int kq[4];
-                            float cq[4], yq[4][D];
+                        for (; q + SB <= cnt; q += SB) {
+                            int kq[SB];
+                            float cq[SB], yq[SB][D];
 #pragma unroll
-                            for (int u = 0; u < 4; ++u) {
+                            for (int u = 0; u < SB; ++u) {
                                 kq[u] = sk[q + u];
                                 cq[u] = sc[q + u];
 #pragma unroll
                                 for (int j = 0; j < D; ++j) yq[u][j] = sx[(q + u) * D + j];
                             }
 #pragma unroll
-                            for (int u = 0; u < 4; ++u) scan(q + u, kq[u], cq[u], yq[u]);
+                            for (int u = 0; u < SB; ++u) scan(q + u, kq[u], cq[u], yq[u]);
                         }
                         for (; q < cnt; ++q) scan(q, sk[q], sc[q], sx + q * D);
                     }
