@@ -91,7 +91,7 @@ def main():
         from fast_hdbscan_amd import workloads
 
         Xd = None
-        if "learned" in names:
+        if "learned" in names or "serving" in names:
             Xd = torch.from_numpy(workloads.scale_2d(args.n, seed=1)).cuda()
         print(json.dumps(side_forms(Xd, args.min_samples, names, args.batch_one_call)))
         return
@@ -314,7 +314,7 @@ def pcie_inclusive(X, ms, mcs, work, steps):
     return {"value": X.shape[0] * steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps}
 
 
-FORMS = ("learned", "high_dim", "scale_3d", "plscan_sweep", "batched")
+FORMS = ("learned", "high_dim", "scale_3d", "plscan_sweep", "batched", "serving")
 
 
 def _mst_ms(X, m, reps=3, **kw):
@@ -510,12 +510,50 @@ def form_batched(one_call_workers=None):
     return res
 
 
+def form_serving(Xd, ms, mcs=100, requests=10):
+    """A stream of requests on the bench's 2M points, as a server would run
+    them: each request's host label stage overlaps the next request's device
+    stage (two sets of buffers, one host thread for the label stages).  Same
+    work per request as a timed step; reported beside `value`, which runs
+    the requests one after the other."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    import torch
+
+    from fast_hdbscan_amd.estimator import device_stage, host_stage, hdbscan_device
+
+    works = [{}, {}]
+    for wk in works:
+        host_stage(device_stage(Xd, ms, work=wk), mcs, work=wk)
+    torch.cuda.synchronize()
+    ex = ThreadPoolExecutor(1)
+    pending, last = None, None
+    t0 = time.perf_counter()
+    for i in range(requests):
+        wk = works[i % 2]
+        staged = device_stage(Xd, ms, work=wk)
+        if pending is not None:
+            pending.result()
+        pending = ex.submit(host_stage, staged, mcs, work=wk)
+    last = pending.result()
+    el = time.perf_counter() - t0
+    ex.shutdown()
+    ref = hdbscan_device(Xd, ms, mcs)
+    same = bool(np.array_equal(last[0], ref[0]) and np.array_equal(last[1], ref[1]))
+    n = Xd.shape[0]
+    return {"requests": requests, "ms_per_request": 1e3 * el / requests, "points_per_s": n * requests / el,
+            "labels_identical": same,
+            "note": "request i's host label stage runs while request i+1's MST runs on the device"}
+
+
 def side_forms(Xd, ms, names=FORMS, batch_one_call=None):
     """The other forms of the path, each run once after the timed steps
     (not part of `value`): learned search radii on the same 2M x 2-D batch,
     the MFMA high-dimensional path at 1M x 64, the 10M x 3-D mixture, a
-    PLSCAN min_samples sweep on 500k x 2-D and a batch of 512 small
-    independent datasets."""
+    PLSCAN min_samples sweep on 500k x 2-D, a batch of 512 small
+    independent datasets, and a stream of requests on the same 2M points
+    served with their host stages overlapped."""
     out = {}
     for name in names:
         if name == "learned":
@@ -528,6 +566,8 @@ def side_forms(Xd, ms, names=FORMS, batch_one_call=None):
             out[name] = form_plscan_sweep()
         elif name == "batched":
             out[name] = form_batched(batch_one_call)
+        elif name == "serving":
+            out[name] = form_serving(Xd, ms)
     return out
 
 

@@ -1,16 +1,21 @@
 // C-ABI entry points (include/fh.h) and the host driver of the MST path.
 //
-// Per call:
-//   1. KD-tree built on the host exactly as numba_kdtree.build_kdtree, rows
-//      uploaded in tree order.
-//   2. knn kernel: exact (min_samples+1)-NN of every point, core distances.
+// Per call (run_mst):
+//   1. KD-tree built on the device exactly as numba_kdtree.build_kdtree
+//      (fh_tree.hip), rows kept in tree order.
+//   2. knn kernel: exact (min_samples+1)-NN of every point, core distances
+//      (the MFMA candidate scans of fh_brute.hip in high dimension).
 //   3. kNN-seeded forest (boruvka.py:486-524): edge choice, cycle skip by
 //      pointer doubling, emit in point order, lock-free union.
 //   4. Boruvka rounds (boruvka.py:610-640) until one component is left:
-//      node purity, nearest out-of-component search with packed atomicMin per
-//      component, per-component selection, cycle skip, emit in order of the
-//      component's lowest point, union + relabel, root compaction.  The host
-//      reads back two integers per round (edges emitted, components left).
+//      node purity, bound seeding, nearest out-of-component search with
+//      packed atomicMin per component, per-component selection, cycle skip,
+//      emit in order of the component's lowest point, union + relabel, root
+//      compaction.  The host reads back two integers per round (edges
+//      emitted, components left).
+// Many datasets (fh_mst_batch, fh_hdbscan_batch): a measured router splits
+// them between persistent work-queue launches (fh_batch.hip: every step of a
+// dataset in one work-group) and run_mst on worker streams.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

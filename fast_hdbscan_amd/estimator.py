@@ -205,6 +205,18 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
     host.  ``work`` (a dict, kept by the caller between calls) holds the
     device, pinned and output buffers; with it the returned arrays are
     reused by the next call."""
+    w = work if work is not None else {}
+    staged = device_stage(X, min_samples, stats=stats, work=w)
+    return host_stage(staged, min_cluster_size, cluster_selection_method, max_cluster_size, allow_single_cluster,
+                      cluster_selection_epsilon, stats=stats, work=w, reuse=work is not None)
+
+
+def device_stage(X, min_samples=10, stats: dict | None = None, work: dict | None = None):
+    """hdbscan_device's device part: MST, row sort and row groups on the
+    device, the sorted rows in pinned host memory of ``work`` on return.
+    The returned handle is host_stage's input; the device is free for the
+    next request meanwhile (a caller with two ``work`` dicts can overlap one
+    request's host stage with the next one's device stage)."""
     import ctypes
     import time
 
@@ -258,6 +270,22 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
                                           ctypes.byref(ng)))
         groups = (k, ro[:k], gs[:ng.value])
     torch.cuda.current_stream(dev).synchronize()
+    if stats is not None:
+        stats["t_sort_copy_ms"] = 1e3 * (time.perf_counter() - t0)
+    return n, m, groups
+
+
+def host_stage(staged, min_cluster_size=10, cluster_selection_method="eom", max_cluster_size=np.inf,
+               allow_single_cluster=False, cluster_selection_epsilon=0.0, stats: dict | None = None,
+               work: dict | None = None, reuse: bool = True):
+    """hdbscan_device's host part on device_stage's result (the same
+    ``work``): the label stage.  Returns (labels int64, strengths float32)."""
+    import time
+
+    n, m, groups = staged
+    w = work
+    hs, hd, hw = w["host"]
+    oh = w["order"][1]
     t1 = time.perf_counter()
     params = label_params(min_cluster_size, cluster_selection_method, max_cluster_size, allow_single_cluster,
                           cluster_selection_epsilon)
@@ -268,10 +296,9 @@ def hdbscan_device(X, min_samples=10, min_cluster_size=10, cluster_selection_met
         w["labels"] = (np.empty(n, np.int64), np.empty(n, np.float32))
     labels, strengths, _, _ = labels_from_sorted_arrays(n, hs[:m].numpy(), hd[:m].numpy(), hw[:m].numpy(), params,
                                                         phases=phases, point_order=oh[:n].numpy(), groups=groups,
-                                                        out=w["labels"] if work is not None else None)
+                                                        out=w["labels"] if reuse else None)
     t2 = time.perf_counter()
     if stats is not None:
-        stats["t_sort_copy_ms"] = 1e3 * (t1 - t0)
         stats["t_labels_ms"] = 1e3 * (t2 - t1)
         stats.update(phases)
     return labels, strengths

@@ -234,6 +234,25 @@ def test_repeatable():
         np.testing.assert_array_equal(u, v)
 
 
+def test_reproducible_flag_gives_the_one_thread_result():
+    # boruvka_tree_query_reproducible (boruvka.py:334-471) is the reference's
+    # blocked form of the same search; with one thread both equal the
+    # sequential run, which is what the device path computes either way
+    from fast_hdbscan_amd.hdbscan import compute_minimum_spanning_tree
+    from oracle import oracle as O
+
+    X = workloads.blobs_small(4000, seed=3)
+    e0, n0, c0 = compute_minimum_spanning_tree(X, min_samples=7, reproducible=False)
+    e1, n1, c1 = compute_minimum_spanning_tree(X, min_samples=7, reproducible=True)
+    np.testing.assert_array_equal(e0, e1)
+    np.testing.assert_array_equal(n0, n1)
+    np.testing.assert_array_equal(c0, c1)
+    er, nr, cr = O.compute_minimum_spanning_tree(X, 7)
+    np.testing.assert_array_equal(e1, er)
+    np.testing.assert_array_equal(n1, nr)
+    np.testing.assert_array_equal(c1, cr)
+
+
 def test_threads_get_their_own_contexts():
     """Two Python threads calling the library at once (ctypes releases the
     GIL) each use their own context and stream; results equal the
