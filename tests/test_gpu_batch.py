@@ -174,3 +174,31 @@ def test_router_splits_change_nothing(monkeypatch):
         for a, b in zip(base, out):
             for x, y in zip(a[:4], b[:4]):
                 np.testing.assert_array_equal(x, y)
+
+
+def test_mst_batch_edge_cases(monkeypatch):
+    # an empty list; a single point; dimensions above the kernel's (the
+    # per-call path, MFMA scans included); min_samples above its heap --
+    # each dataset's rows as the per-call path gives them, and the kernel
+    # takes only what it can
+    from fast_hdbscan_amd.batch import mst_batch
+    from fast_hdbscan_amd.hdbscan import mst_arrays
+
+    assert mst_batch([], min_samples=5) == []
+    monkeypatch.setenv("FH_BATCH_MAX_N", "65536")
+    rng = np.random.default_rng(21)
+    data = [rng.standard_normal((1, 2)).astype(np.float32), rng.standard_normal((500, 12)).astype(np.float32),
+            rng.standard_normal((3000, 20)).astype(np.float32), rng.standard_normal((700, 3)).astype(np.float32)]
+    for ms, expect in ((5, [False, False, False, True]), (20, [False, False, False, False])):
+        out = mst_batch(data, min_samples=ms)
+        for X, (src, dst, w, core, pers), e in zip(data, out, expect):
+            print(f"edge case n={X.shape[0]} d={X.shape[1]} ms={ms} persistent={pers}", flush=True)
+            assert pers == e
+            if X.shape[0] < 2:
+                assert src.shape == (0,)
+                continue
+            s1, d1, w1, c1, _ = mst_arrays(X, ms)
+            np.testing.assert_array_equal(core, c1)
+            np.testing.assert_array_equal(w, w1)
+            np.testing.assert_array_equal(src, s1)
+            np.testing.assert_array_equal(dst, d1)
