@@ -34,6 +34,7 @@ namespace {
 constexpr int kBT = kSubNT;  // threads per work-group: the subtree build's
 constexpr int kBW = kBT / 64;
 constexpr int kMaxKq = 16;
+constexpr int kCertDepth = 4;  // certification nodes this many levels above the leaves
 constexpr int kMaxBatchN = 1 << 16;  // largest dataset a work-group takes
 // rows read ahead in the kNN leaves and the search's staged leaves (4 and 2
 // remove the register spills but measured no faster)
@@ -52,6 +53,7 @@ static_assert(lds_fits<1>() && lds_fits<2>() && lds_fits<3>() && lds_fits<4>() &
                   lds_fits<7>() && lds_fits<8>(),
               "every phase fits the work-group's LDS");
 static_assert(kMaxKq * kBT * 8 <= kLdsWork, "kNN heaps fit the work-group's LDS");
+static_assert(kBW * kCertQueue * 4 <= kLdsWork, "certification queues fit the work-group's LDS");
 
 // A work-group's scratch arrays (n: points, N: tree nodes, d: dimensions,
 // kq: neighbours per query), each 256-byte aligned: type, name, count.
@@ -88,7 +90,10 @@ static_assert(kMaxKq * kBT * 8 <= kLdsWork, "kNN heaps fit the work-group's LDS"
     X(int, ei, n)                          \
     X(int, ej, n)                          \
     X(float, ew, n)                        \
-    X(unsigned char, interior, n)
+    X(unsigned char, interior, n)          \
+    X(int, cert_of_pos, n)                 \
+    X(float, mincore, N)                   \
+    X(unsigned char, cert, N)
 
 struct Slice {
 #define FH_SLICE_FIELD(T, name, count) T *name;
@@ -368,6 +373,11 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
     }
     __syncthreads();
 
+    // certification nodes kCertDepth levels above the leaves: which node
+    // holds each position and the smallest core inside each (a wave per node)
+    const int cert_level = L - 1 - kCertDepth > 0 ? L - 1 - kCertDepth : 0;
+    for (int k = wv; k < (1 << cert_level); k += kBW)
+        cert_setup_node(cert_level, s.range, s.core_p, s.cert_of_pos, s.mincore, k);
     phase(1);
     // -- kNN-seeded forest (boruvka.py:474-524), edges by point index
     for (int p = tid; p < n; p += kBT) {
@@ -422,8 +432,23 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
     qa.d = D;
     qa.levels = L;
     qa.pass = 0;
-    qa.sample_mask = -1;
+    qa.sample_mask = -1;  // no position counts as sampled
+    qa.cert_of_pos = s.cert_of_pos;
+    qa.cert = s.cert;
     qa.wave = 1;
+    CertArgs ca{};
+    ca.xp = s.xp;
+    ca.range = s.range;
+    ca.box = s.box;
+    ca.core_p = s.core_p;
+    ca.comp_p = s.comp_p;
+    ca.node_comp = s.node_comp;
+    ca.bound = s.bound;
+    ca.mincore = s.mincore;
+    ca.cert = s.cert;
+    ca.level = cert_level;
+    ca.levels = L;
+    ca.d = D;
     float *sx = reinterpret_cast<float *>(lds) + wv * kWaveStage * D;
     float *sc = reinterpret_cast<float *>(lds) + kBW * kWaveStage * D + wv * kWaveStage;
     int *sk = reinterpret_cast<int *>(lds) + kBW * kWaveStage * (D + 1) + wv * kWaveStage;
@@ -463,10 +488,15 @@ __device__ void run_dataset(const BatchArgs &a, const BatchJob &job, int ji, con
         for (int p0 = 0; p0 < n; p0 += kBT)
             seed_bounds_at(p0 + tid, p0 + tid < n, kn, s.nb_pos, s.nb_d, s.core_p, s.comp_p, s.bound, s.interior);
         __syncthreads();
-        // nearest out-of-component edge of every point, one wavefront per 64
-        // consecutive tree positions
+        // subtrees whose points cannot reach their component's bound
+        // (cert_node: a wave per certification node, its walk queue in LDS)
+        for (int k = wv; k < (1 << cert_level); k += kBW)
+            cert_node<D>(ca, k, reinterpret_cast<int *>(lds) + wv * kCertQueue);
+        __syncthreads();
+        // nearest out-of-component edge of every other point, one wavefront
+        // per 64 consecutive tree positions
         phase(4);
-        for (int p0 = wv * 64; p0 < n; p0 += kBT) wave_search<D, 0, kBatchStageBatch>(qa, p0 + lane, sx, sc, sk);
+        for (int p0 = wv * 64; p0 < n; p0 += kBT) wave_search<D, 2, kBatchStageBatch>(qa, p0 + lane, sx, sc, sk);
         __syncthreads();
         phase(3);
         // merge_components (boruvka.py:34-78): each component's edge, the

@@ -392,6 +392,124 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Which certification node (in-level index at `level`) holds each position,
+// and the smallest core distance inside node k, by one wave.
+__device__ __forceinline__ void cert_setup_node(int level, const int2 *__restrict__ range,
+                                                const float *__restrict__ core_p, int *cert_of_pos, float *mincore,
+                                                int k) {
+    const int lane = threadIdx.x & 63;
+    const int2 r = range[(1 << level) - 1 + k];
+    float m = kInf;
+    for (int p = r.x + lane; p < r.y; p += 64) {
+        cert_of_pos[p] = k;
+        m = core_p[p] < m ? core_p[p] : m;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float t = __shfl_xor(m, o, 64);
+        m = t < m ? t : m;
+    }
+    if (lane == 0) mincore[k] = m;
+}
+
+constexpr int kCertQueue = 1024;  // queued tree nodes of one certification walk
+
+// One wave per certification node P of a single component C: the node
+// fails as soon as some point j outside C could give a weight
+// max(d(box P, x_j), core_j, min core of P) <= B_C (an existence test, so
+// the visiting order does not change the answer).  The wave shares the
+// walk: every lane holds one tree node, descends into its left child when
+// the box-to-box bound and the single-component test let it pass and
+// queues the right child in LDS; lanes without a node take queued ones.  A
+// queue overflow leaves the node uncertified (always allowed: the main
+// search then simply visits its points).  A certified node's points cannot
+// reach B_C (nor, therefore, the component minimum), so the main search
+// skips them.
+template <int D>
+__device__ __forceinline__ void cert_node(const CertArgs &a, int k, int *q) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int dd = D > 0 ? D : a.d;
+    const int stride = 2 * dd;
+    const int P = (1 << a.level) - 1 + k;
+    const int C = a.node_comp[P];
+    if (C < 0) {
+        if (lane == 0) a.cert[k] = 0;
+        return;
+    }
+    const float B = ld_relaxed(a.bound + C);
+    const float mc = a.mincore[k];
+    if (mc > B || !(B < kInf)) {
+        if (lane == 0) a.cert[k] = mc > B ? 1 : 0;
+        return;
+    }
+    float pb[2 * (D > 0 ? D : 1)];
+    const float *boxP = a.box + (size_t)P * stride;
+    if constexpr (D > 0) {
+#pragma unroll
+        for (int j = 0; j < 2 * D; ++j) pb[j] = boxP[j];
+    }
+    const float *bp = D > 0 ? pb : boxP;
+    const int first_leaf = (1 << (a.levels - 1)) - 1;
+    int node = lane == 0 ? 0 : -1;
+    int qn = 0;  // queued nodes (the same on every lane)
+    bool fail = false, failed = false, overflow = false;
+    for (;;) {
+        bool push = false;
+        int child = -1;
+        if (node >= 0) {
+            const float lb = box_box_lb<D>(bp, a.box + (size_t)node * stride, a.d);
+            if (!(lb > B) && a.node_comp[node] != C) {
+                if (node >= first_leaf) {
+                    const int2 r = a.range[node];
+                    for (int j = r.x; j < r.y; ++j) {
+                        if (a.comp_p[j] == C) continue;
+                        const float cj = a.core_p[j];
+                        if (!(cj <= B)) continue;
+                        float w = box_lb<D>(a.xp + (size_t)j * dd, bp, a.d);
+                        w = w > cj ? w : cj;
+                        w = w > mc ? w : mc;
+                        if (!(w > B)) {
+                            fail = true;
+                            break;
+                        }
+                    }
+                    node = -1;
+                } else {
+                    child = 2 * node + 2;
+                    push = true;
+                    node = 2 * node + 1;
+                }
+            } else {
+                node = -1;
+            }
+        }
+        if (__ballot(fail) != 0) {
+            failed = true;  // (on every lane: the ballot is the wave's)
+            break;
+        }
+        const unsigned long long pm = __ballot(push);
+        const int npush = __popcll(pm);
+        if (qn + npush > kCertQueue) {
+            overflow = true;
+            break;
+        }
+        if (push) q[qn + __popcll(pm & below)] = child;
+        qn += npush;
+        wave_sync();
+        const unsigned long long im = __ballot(node < 0);
+        const int idle = __popcll(im);
+        const int take = idle < qn ? idle : qn;
+        const int ir = __popcll(im & below);
+        if (node < 0 && ir < take) node = q[qn - 1 - ir];
+        qn -= take;
+        wave_sync();
+        if (__ballot(node >= 0) == 0) break;
+    }
+    if (lane == 0) a.cert[k] = (failed || overflow) ? 0 : 1;
+}
+
+
 // ------------------------------------------ wave-cooperative search ----
 //
 // The main nearest-edge search with one wavefront per 64 consecutive tree
