@@ -202,3 +202,23 @@ def test_mst_batch_edge_cases(monkeypatch):
             np.testing.assert_array_equal(w, w1)
             np.testing.assert_array_equal(src, s1)
             np.testing.assert_array_equal(dst, d1)
+
+
+def test_batch_full_workload_sampled():
+    # the whole 4,096-dataset workload (sizes log-uniform 500-50,000, d in
+    # {2, 3, 5, 8}) in one call; a sample of 48 datasets against
+    # one-at-a-time calls, every dataset's labels in range
+    from fast_hdbscan_amd.batch import hdbscan_batch
+    from fast_hdbscan_amd.estimator import fast_hdbscan
+
+    data = workloads.batch_datasets(4096)
+    print("full batch workload: 4096 datasets", flush=True)
+    out = hdbscan_batch(data, min_samples=5, min_cluster_size=20, workers=16)
+    for X, (lab, pr) in zip(data, out):
+        assert lab.shape == (X.shape[0],) and lab.min() >= -1
+        assert np.all((pr >= 0) & (pr <= 1))
+    rng = np.random.default_rng(0)
+    for i in rng.choice(len(data), 48, replace=False):
+        l1, p1 = fast_hdbscan(data[i], min_samples=5, min_cluster_size=20)
+        np.testing.assert_array_equal(out[i][0], l1, err_msg=f"dataset {i} {data[i].shape}")
+        np.testing.assert_array_equal(out[i][1], p1, err_msg=f"dataset {i} {data[i].shape}")
