@@ -50,6 +50,7 @@ namespace {
 constexpr int64_t kLeafSize = 40;  // numba_kdtree.py:478 default, used by hdbscan.py:466
 constexpr int kCertDepth = 4;      // certification nodes sit this many levels above the leaves
 constexpr int kSampleMask = 255;   // one sampled search per 256 tree positions
+constexpr int kOrderRound = 1;     // MFMA rounds: rows in component order from this round on
 
 template <class T>
 struct DevBuf {
@@ -94,6 +95,11 @@ struct fh_ctx {
     DevBuf<int> bx_cand, bx_cnt, bx_flags, bx_rowarg, bx_rows, bx_small;
     DevBuf<unsigned int> bx_nmax;
     DevBuf<int> bx_tilecomp;
+    // rounds in component order with tile-pair bounds
+    DevBuf<float> bx_nrm2, bx_core2, bx_tmin, bx_tminc, bx_tmaxc, bx_tU;
+    DevBuf<int> bx_comp2, bx_order, bx_ovals;
+    DevBuf<uint32_t> bx_okeys, bx_okeys_s;
+    DevBuf<unsigned char> bx_otemp;
     DevBuf<unsigned long long> bx_scored, bx_rowkey;
     int brute_min_dim = 17;  // d from which the MFMA scans replace the tree searches
     // learned search radii
@@ -153,7 +159,7 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->sw_pos, c->sw_ties, c->sw_d, c->nb_d,
         c->mincore,
         c->cert, c->seed_int, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
-        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->bx_rowkey, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
+        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->bx_rowkey, c->bx_nrm2, c->bx_core2, c->bx_tmin, c->bx_tminc, c->bx_tmaxc, c->bx_tU, c->bx_comp2, c->bx_order, c->bx_ovals, c->bx_okeys, c->bx_okeys_s, c->bx_otemp, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
         c->ml_v, c->ml_mu, c->ml_sd, c->ml_scratch, c->ml_tgt, c->ml_pred, c->ml_bound, c->ml_stats,
         c->ml_rows, c->ml_miss, c->kr_keys, c->kr_keys_s, c->kr_vals, c->kr_vals_s, c->kr_src, c->kr_dst,
         c->kr_w, c->kr_core, c->sk1, c->sk1s, c->sk2, c->sk2s, c->sv1, c->sv1s,
@@ -920,18 +926,76 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     be.d = d;
     be.levels = levels;
     int64_t exact_rows = 0;
+    // rows in component order with tile-pair bounds in the MFMA rounds
+    // (symmetric scans only; FH_TILE_BOUNDS=0 keeps tree order)
+    bool tile_bounds = brute && bsa.row_key != nullptr;
+    if (const char *m = std::getenv("FH_TILE_BOUNDS")) tile_bounds = tile_bounds && std::atoi(m) != 0;
     while (C > 1) {
         FH_HIP(fh::launch_round_reset(C, ctx->roots.p, ctx->bound.p, ctx->best.p, st));
         if (!brute) FH_HIP(fh::launch_purity(levels, ctx->range.p, ctx->comp_p.p, ctx->node_comp.p, st));
         FH_HIP(mark(ctx, 5));
         if (brute) {
             // approximate row minima on MFMA, exact upper bound per
-            // component, exact re-scan of the rows that can reach it
-            FH_HIP(fh::launch_tile_comp(n, bsa.npad, ctx->comp_p.p, ctx->bx_tilecomp.p, st));
+            // component, exact re-scan of the rows that can reach it.  From
+            // round kOrderRound on (symmetric scans) the scans take the rows
+            // in component order, fixed at that round (components only
+            // merge, so its single-component tiles stay so), and that
+            // round's scan records every tile pair's smallest approximate
+            // distance; later rounds bound their components from those
+            // minima and skip the tile pairs that no row of either tile can
+            // use (FH_TILE_BOUNDS=0: every round in tree order, no skips).
+            const int64_t npad = bsa.npad, ntl = npad / 128;
+            const bool order_rows = tile_bounds && rounds >= kOrderRound;
+            if (order_rows && rounds == kOrderRound) {
+                FH_TRY(ensure(ctx, ctx->bx_order, npad));
+                FH_TRY(ensure(ctx, ctx->bx_okeys, n));
+                FH_TRY(ensure(ctx, ctx->bx_okeys_s, n));
+                FH_TRY(ensure(ctx, ctx->bx_ovals, n));
+                const size_t tb = fh::sort_u32_pairs_temp_bytes(ni);
+                FH_TRY(ensure(ctx, ctx->bx_otemp, tb));
+                FH_TRY(ensure(ctx, ctx->bx_nrm2, npad));
+                FH_TRY(ensure(ctx, ctx->bx_core2, npad));
+                FH_TRY(ensure(ctx, ctx->bx_comp2, npad));
+                FH_TRY(ensure(ctx, ctx->bx_tminc, ntl));
+                FH_TRY(ensure(ctx, ctx->bx_tmaxc, ntl));
+                FH_TRY(ensure(ctx, ctx->bx_tU, ntl));
+                FH_TRY(ensure(ctx, ctx->bx_tmin, (size_t)(ntl * ntl)));
+                FH_HIP(fh::launch_order_keys(n, ctx->comp_p.p, ctx->bx_okeys.p, ctx->bx_ovals.p, st));
+                FH_HIP(fh::sort_u32_pairs(ni, ctx->bx_okeys.p, ctx->bx_okeys_s.p, ctx->bx_ovals.p, ctx->bx_order.p,
+                                          ctx->bx_otemp.p, tb, st));
+                // the packed rows of the kNN scans are not read again
+                FH_HIP(fh::launch_order_pack(n, d, bsa.dp, npad, ctx->bx_order.p, ctx->xp.p, ctx->box.p, ctx->core_p.p,
+                                             ctx->bx_xc.p, ctx->bx_nrm2.p, ctx->bx_core2.p, st));
+                FH_HIP(fh::launch_tile_core_range(n, npad, ctx->bx_core2.p, ctx->bx_tminc.p, ctx->bx_tmaxc.p, st));
+                FH_HIP(hipMemsetAsync(ctx->bx_tmin.p, 0, sizeof(float) * ntl * ntl, st));
+            }
+            fh::BruteScanArgs rs = bsa;
+            if (order_rows) {
+                FH_HIP(fh::launch_gather_comp(n, npad, ctx->bx_order.p, ctx->comp_p.p, ctx->bx_comp2.p, st));
+                FH_HIP(fh::launch_tile_comp(n, npad, ctx->bx_comp2.p, ctx->bx_tilecomp.p, st));
+                rs.nrm = ctx->bx_nrm2.p;
+                rs.core_p = ctx->bx_core2.p;
+                rs.comp_p = ctx->bx_comp2.p;
+                if (rounds == kOrderRound) {
+                    rs.tmin_out = ctx->bx_tmin.p;
+                } else {
+                    FH_HIP(fh::launch_tile_bounds((int)ntl, ctx->bx_tilecomp.p, ctx->bx_tmin.p, ctx->bx_tmaxc.p,
+                                                  bsa.margin_c, bsa.nrm_max_bits, ctx->bound.p, st));
+                    FH_HIP(fh::launch_tile_umax(n, npad, ctx->bx_comp2.p, ctx->bound.p, ctx->bx_tU.p, st));
+                    rs.tmin_in = ctx->bx_tmin.p;
+                    rs.tminc = ctx->bx_tminc.p;
+                    rs.tU = ctx->bx_tU.p;
+                }
+            } else {
+                FH_HIP(fh::launch_tile_comp(n, npad, ctx->comp_p.p, ctx->bx_tilecomp.p, st));
+            }
             FH_HIP(mark(ctx, 8));
-            if (bsa.row_key) FH_HIP(hipMemsetAsync(bsa.row_key, 0xff, sizeof(unsigned long long) * n, st));
-            FH_HIP(fh::launch_scan_boruvka(bsa, st));
-            if (bsa.row_key) FH_HIP(fh::launch_unpack_row_keys(n, bsa.row_key, bsa.row_min, bsa.row_arg, st));
+            if (rs.row_key) FH_HIP(hipMemsetAsync(rs.row_key, 0xff, sizeof(unsigned long long) * n, st));
+            FH_HIP(fh::launch_scan_boruvka(rs, st));
+            if (rs.row_key && order_rows)
+                FH_HIP(fh::launch_unpack_row_keys_ordered(n, rs.row_key, ctx->bx_order.p, rs.row_min, rs.row_arg, st));
+            else if (rs.row_key)
+                FH_HIP(fh::launch_unpack_row_keys(n, rs.row_key, rs.row_min, rs.row_arg, st));
             FH_HIP(mark(ctx, 9));
             FH_HIP(fh::launch_boruvka_upper(be, st));
             FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));

@@ -46,7 +46,47 @@ struct BruteScanArgs {
     // the column tiles from its own on and appends in both directions,
     // counting in cand_cnt (zeroed before the scan)
     int sym_knn;
+    // Boruvka, symmetric form, tile-pair bounds (rows in component order):
+    // tmin_out (non-null): per tile pair scanned, the smallest approximate
+    // squared distance over its valid row-column pairs, at [A * ntiles + B]
+    // (A the row tile, B >= A); tmin_in (non-null): a column tile B is
+    // skipped when max(tmin_in[A * ntiles + B] - m, tminc[A], tminc[B]) >
+    // max(tU[A], tU[B]), with m = 2 margin_c u max|y'|^2 the largest error
+    // of any pair: a lower bound of every mutual-reachability value of the
+    // pair above every upper bound of the components with rows in either
+    // tile
+    float *tmin_out;
+    const float *tmin_in;
+    const float *tminc;
+    const float *tU;
 };
+
+// keys[p] = comp_p[p], vals[p] = p (the sort into component order)
+hipError_t launch_order_keys(int64_t n, const int *comp_p, uint32_t *keys, int *vals, hipStream_t st);
+// Rows in another order for the Boruvka scans (order[p2] = tree position,
+// -1 for the padding rows p2 >= n): xc, nrm and core2 in that order.
+hipError_t launch_order_pack(int64_t n, int d, int dp, int64_t npad, const int *order, const float *xp,
+                             const float *root_box, const float *core_p, float *xc, float *nrm, float *core2,
+                             hipStream_t st);
+// comp2[p2] = comp_p[order[p2]] (-3 for the padding rows)
+hipError_t launch_gather_comp(int64_t n, int64_t npad, const int *order, const int *comp_p, int *comp2,
+                              hipStream_t st);
+// per tile: smallest and largest core distance of its valid rows
+hipError_t launch_tile_core_range(int64_t n, int64_t npad, const float *core2, float *tminc, float *tmaxc,
+                                  hipStream_t st);
+// Upper bounds of the components' minimum outgoing weight from the
+// tile-pair minima: for tiles A < B whose rows all lie in components a != b,
+// some pair (i in A, j in B) has exact squared distance <= tmin + m (m as
+// above) and cores <= tmaxc, so max(tmin + m, tmaxc[A], tmaxc[B]) bounds
+// both components' minimum (float-bits atomicMin into bound[a], bound[b]).
+hipError_t launch_tile_bounds(int ntiles, const int *tile_comp, const float *tmin, const float *tmaxc, float margin_c,
+                              const unsigned int *nrm_max_bits, float *bound, hipStream_t st);
+// per tile: the largest bound[comp2[row]] over its valid rows
+hipError_t launch_tile_umax(int64_t n, int64_t npad, const int *comp2, const float *bound, float *tU,
+                            hipStream_t st);
+// row_key (scan order) into row_min / row_arg by tree position
+hipError_t launch_unpack_row_keys_ordered(int64_t n, const unsigned long long *row_key, const int *order,
+                                          float *row_min, int *row_arg, hipStream_t st);
 
 hipError_t launch_tile_comp(int64_t n, int64_t npad, const int *comp_p, int *tile_comp, hipStream_t st);
 

@@ -115,6 +115,9 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     // product from LDS rather than held in registers)
     __shared__ float row_core_s[MODE == 1 ? 128 : 1];
     __shared__ int row_comp_s[MODE == 1 ? 128 : 1];
+    // MODE 1 with tmin_out: per wave the tile pair's smallest approximate
+    // distance (double-buffered like the tiles)
+    __shared__ float tmin_s[MODE == 1 ? 2 : 1][4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, c16 = lane & 15;
@@ -176,9 +179,25 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     // MODE 1: column tiles whose rows all share this tile's single
     // component hold no candidate (boruvka.py:199-201) and are skipped
     const int my_tc = (MODE == 1 && a.tile_comp) ? a.tile_comp[blockIdx.x] : -1;
+    // MODE 1 with tile-pair bounds: column tiles that no row of either tile
+    // can use are skipped too
+    const bool prune = MODE == 1 && a.tmin_in != nullptr;
+    const float my_minc = prune ? a.tminc[blockIdx.x] : 0.0f;
+    const float my_u = prune ? a.tU[blockIdx.x] : 0.0f;
+    const float *tmin_row = prune ? a.tmin_in + (int64_t)blockIdx.x * ntiles : nullptr;
+    const float tmargin = prune ? 2.0f * a.margin_c * kU * __uint_as_float(*a.nrm_max_bits) : 0.0f;
     auto next_tile = [&](int t) {
-        if (my_tc >= 0)
-            while (t < ntiles && a.tile_comp[t] == my_tc) ++t;
+        for (; t < ntiles; ++t) {
+            if (my_tc >= 0 && a.tile_comp[t] == my_tc) continue;
+            if (prune) {
+                float lb = tmin_row[t] - tmargin;
+                const float mc = my_minc > a.tminc[t] ? my_minc : a.tminc[t];
+                lb = lb > mc ? lb : mc;
+                const float ub = my_u > a.tU[t] ? my_u : a.tU[t];
+                if (lb > ub) continue;
+            }
+            break;
+        }
         return t;
     };
     floatx4 pre[NV];
@@ -240,6 +259,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
             }
         }
         const int64_t col0 = (int64_t)t * 128;
+        float tmv = kInfB;  // MODE 1, tmin_out: the lane's smallest approximate distance of the pair
 #pragma unroll
         for (int cb = 0; cb < 8; ++cb) {
             const int cl = cb * 16 + c16;
@@ -307,6 +327,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                         const int rl = wave * 32 + rb * 16 + 4 * g + r;
                         const float rc = row_core_s[rl];
                         const int rk = row_comp_s[rl];
+                        if (a.tmin_out && valid && rk != -2) tmv = approx < tmv ? approx : tmv;
                         float m = rc > cc ? rc : cc;
                         m = approx > m ? approx : m;
                         const bool cand = valid && ck != rk;
@@ -338,6 +359,14 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                 }
             }
         }
+        if (MODE == 1 && a.tmin_out) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const float ov = __shfl_xor(tmv, o, 64);
+                tmv = ov < tmv ? ov : tmv;
+            }
+            if (lane == 0) tmin_s[cur][wave] = tmv;
+        }
         // tile t1 into the other buffer (last read in the previous
         // iteration, before the previous barrier), prefetch the one after,
         // then one barrier
@@ -347,6 +376,12 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
             if (t2 < ntiles) gload(t2);
         }
         __syncthreads();
+        if (MODE == 1 && a.tmin_out && tid == 0) {
+            float v = tmin_s[cur][0];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) v = tmin_s[cur][w] < v ? tmin_s[cur][w] : v;
+            a.tmin_out[(int64_t)blockIdx.x * ntiles + t] = v;
+        }
         if (sym && t != (int)blockIdx.x && tid < 128) {
             // tile t's column minima over this block's 128 rows, to the
             // column rows (the pair (t, this tile) is scanned here only);
@@ -744,6 +779,158 @@ __global__ __launch_bounds__(128) void tile_comp_kernel(int64_t n, const int *__
     if (row < n && comp_p[row] != c0) mixed = 1;
     __syncthreads();
     if (threadIdx.x == 0) tile_comp[blockIdx.x] = mixed ? -1 : c0;
+}
+
+// ------------------------------------------ rows in component order ----
+
+// center_pack_kernel for the rows in `order` (scan position p2 holds tree
+// position order[p2]; padding rows p2 >= n are zero), plus their cores.
+__global__ void order_pack_kernel(int64_t n, int d, int dp, int64_t npad, const int *__restrict__ order,
+                                  const float *__restrict__ xp, const float *__restrict__ root_box,
+                                  const float *__restrict__ core_p, float *__restrict__ xc, float *__restrict__ nrm,
+                                  float *__restrict__ core2) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npad) return;
+    const int64_t q = p < n ? order[p] : -1;
+    float *tile = xc + (p / 128) * 128 * dp + (p % 128) * 4;
+    float s = 0.0f;
+    for (int k = 0; k < dp; ++k) {
+        float v = 0.0f;
+        if (q >= 0 && k < d) v = xp[q * d + k] - 0.5f * (root_box[k] + root_box[d + k]);
+        tile[(k >> 2) * 512 + (k & 3)] = v;
+        s = __builtin_fmaf(v, v, s);
+    }
+    nrm[p] = s;
+    core2[p] = q >= 0 ? core_p[q] : 0.0f;
+}
+
+__global__ void order_keys_kernel(int64_t n, const int *__restrict__ comp_p, uint32_t *keys, int *vals) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    keys[p] = (uint32_t)comp_p[p];
+    vals[p] = (int)p;
+}
+
+__global__ void gather_comp_kernel(int64_t n, int64_t npad, const int *__restrict__ order,
+                                   const int *__restrict__ comp_p, int *comp2) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npad) return;
+    comp2[p] = p < n ? comp_p[order[p]] : -3;
+}
+
+__global__ __launch_bounds__(128) void tile_core_range_kernel(int64_t n, const float *__restrict__ core2,
+                                                              float *tminc, float *tmaxc) {
+    __shared__ float lo_s[2], hi_s[2];
+    const int64_t row = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    float lo = row < n ? core2[row] : kInfB, hi = row < n ? core2[row] : 0.0f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        lo_s[threadIdx.x >> 6] = lo;
+        hi_s[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        tminc[blockIdx.x] = lo_s[0] < lo_s[1] ? lo_s[0] : lo_s[1];
+        tmaxc[blockIdx.x] = hi_s[0] > hi_s[1] ? hi_s[0] : hi_s[1];
+    }
+}
+
+// One thread per tile pair (A, B), B > A.
+__global__ void tile_bounds_kernel(int ntiles, const int *__restrict__ tile_comp, const float *__restrict__ tmin,
+                                   const float *__restrict__ tmaxc, float margin_c,
+                                   const unsigned int *__restrict__ nrm_max_bits, float *bound) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float tmargin = 2.0f * margin_c * kU * __uint_as_float(*nrm_max_bits);
+    const int A = (int)(e / ntiles), B = (int)(e % ntiles);
+    if (A >= ntiles || B <= A) return;
+    const int ca = tile_comp[A], cb = tile_comp[B];
+    if (ca < 0 || cb < 0 || ca == cb) return;
+    float u = tmin[e] + tmargin;
+    u = u > tmaxc[A] ? u : tmaxc[A];
+    u = u > tmaxc[B] ? u : tmaxc[B];
+    atomicMin(reinterpret_cast<unsigned int *>(bound + ca), __float_as_uint(u));
+    atomicMin(reinterpret_cast<unsigned int *>(bound + cb), __float_as_uint(u));
+}
+
+__global__ __launch_bounds__(128) void tile_umax_kernel(int64_t n, const int *__restrict__ comp2,
+                                                        const float *__restrict__ bound, float *tU) {
+    __shared__ float hi_s[2];
+    const int64_t row = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    float hi = row < n ? bound[comp2[row]] : 0.0f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float b = __shfl_xor(hi, o, 64);
+        hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63) == 0) hi_s[threadIdx.x >> 6] = hi;
+    __syncthreads();
+    if (threadIdx.x == 0) tU[blockIdx.x] = hi_s[0] > hi_s[1] ? hi_s[0] : hi_s[1];
+}
+
+__global__ void unpack_row_keys_ordered_kernel(int64_t n, const unsigned long long *__restrict__ row_key,
+                                               const int *__restrict__ order, float *row_min, int *row_arg) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long key = row_key[p];
+    const int q = order[p];
+    if (key == ~0ull) {
+        row_min[q] = kInfB;
+        row_arg[q] = -1;
+    } else {
+        row_min[q] = __uint_as_float((unsigned int)(key >> 32));
+        row_arg[q] = order[(int)(unsigned int)(key & 0xffffffffull)];
+    }
+}
+
+hipError_t launch_order_pack(int64_t n, int d, int dp, int64_t npad, const int *order, const float *xp,
+                             const float *root_box, const float *core_p, float *xc, float *nrm, float *core2,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(order_pack_kernel, dim3(nblocks(npad, 256)), dim3(256), 0, st, n, d, dp, npad, order, xp,
+                       root_box, core_p, xc, nrm, core2);
+    return hipGetLastError();
+}
+
+hipError_t launch_order_keys(int64_t n, const int *comp_p, uint32_t *keys, int *vals, hipStream_t st) {
+    hipLaunchKernelGGL(order_keys_kernel, dim3((unsigned)nblocks(n, 256)), dim3(256), 0, st, n, comp_p, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_comp(int64_t n, int64_t npad, const int *order, const int *comp_p, int *comp2,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(gather_comp_kernel, dim3(nblocks(npad, 256)), dim3(256), 0, st, n, npad, order, comp_p, comp2);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_core_range(int64_t n, int64_t npad, const float *core2, float *tminc, float *tmaxc,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL(tile_core_range_kernel, dim3((unsigned)(npad / 128)), dim3(128), 0, st, n, core2, tminc, tmaxc);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_bounds(int ntiles, const int *tile_comp, const float *tmin, const float *tmaxc, float margin_c,
+                              const unsigned int *nrm_max_bits, float *bound, hipStream_t st) {
+    const int64_t pairs = (int64_t)ntiles * ntiles;
+    hipLaunchKernelGGL(tile_bounds_kernel, dim3((unsigned)nblocks(pairs, 256)), dim3(256), 0, st, ntiles, tile_comp,
+                       tmin, tmaxc, margin_c, nrm_max_bits, bound);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_umax(int64_t n, int64_t npad, const int *comp2, const float *bound, float *tU,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(tile_umax_kernel, dim3((unsigned)(npad / 128)), dim3(128), 0, st, n, comp2, bound, tU);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_row_keys_ordered(int64_t n, const unsigned long long *row_key, const int *order,
+                                          float *row_min, int *row_arg, hipStream_t st) {
+    hipLaunchKernelGGL(unpack_row_keys_ordered_kernel, dim3((unsigned)nblocks(n, 256)), dim3(256), 0, st, n, row_key,
+                       order, row_min, row_arg);
+    return hipGetLastError();
 }
 
 hipError_t launch_tile_comp(int64_t n, int64_t npad, const int *comp_p, int *tile_comp, hipStream_t st) {

@@ -73,6 +73,11 @@ struct PrefixComponentsArgs {
 size_t prefix_components_temp_bytes(int k);
 hipError_t prefix_components(const PrefixComponentsArgs &a, hipStream_t st);
 
+// Stable sort of m (32-bit key, int value) pairs by key.
+size_t sort_u32_pairs_temp_bytes(int m);
+hipError_t sort_u32_pairs(int m, const uint32_t *keys, uint32_t *keys_s, const int *vals, int *vals_s, void *temp,
+                          size_t temp_bytes, hipStream_t st);
+
 // pos_src[i] = posof[src[i]], pos_dst[i] = posof[dst[i]] (negative ids kept)
 hipError_t tree_positions(int m, const int *src, const int *dst, const int *posof, int *pos_src, int *pos_dst,
                           hipStream_t st);

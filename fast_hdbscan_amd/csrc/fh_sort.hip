@@ -95,6 +95,21 @@ __global__ void knn_edge_gather_kernel(int64_t m, int kq, const int *__restrict_
 
 }  // namespace
 
+size_t sort_u32_pairs_temp_bytes(int m) {
+    size_t b = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<uint32_t *>(nullptr),
+                                                      static_cast<uint32_t *>(nullptr), static_cast<int *>(nullptr),
+                                                      static_cast<int *>(nullptr), m);
+    return e == hipSuccess ? b : 0;
+}
+
+hipError_t sort_u32_pairs(int m, const uint32_t *keys, uint32_t *keys_s, const int *vals, int *vals_s, void *temp,
+                          size_t temp_bytes, hipStream_t st) {
+    if (m <= 0) return hipSuccess;
+    size_t tb = temp_bytes;
+    return hipcub::DeviceRadixSort::SortPairs(temp, tb, keys, keys_s, vals, vals_s, m, 0, 32, st);
+}
+
 size_t knn_edges_temp_bytes(int64_t m) {
     size_t b = 0;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<uint64_t *>(nullptr),
