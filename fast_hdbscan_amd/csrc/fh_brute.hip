@@ -840,21 +840,66 @@ __global__ __launch_bounds__(128) void tile_core_range_kernel(int64_t n, const f
     }
 }
 
-// One thread per tile pair (A, B), B > A.
-__global__ void tile_bounds_kernel(int ntiles, const int *__restrict__ tile_comp, const float *__restrict__ tmin,
-                                   const float *__restrict__ tmaxc, float margin_c,
-                                   const unsigned int *__restrict__ nrm_max_bits, float *bound) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// One work-group per row tile A, over the tile pairs (A, B), B > A, whose
+// two tiles lie in two different single components: the pair's bound goes
+// to both components.  Row A's side is reduced in registers over the whole
+// row (one atomic per tile); the column side is reduced per run of equal
+// components within each wavefront (the tiles are in component order, so
+// the runs are long) before its atomic -- one atomic per pair onto a few
+// hundred component slots serialises in L2.
+__global__ __launch_bounds__(256) void tile_bounds_kernel(int ntiles, const int *__restrict__ tile_comp,
+                                                          const float *__restrict__ tmin,
+                                                          const float *__restrict__ tmaxc, float margin_c,
+                                                          const unsigned int *__restrict__ nrm_max_bits,
+                                                          float *bound) {
+    __shared__ float ua_s[4];
+    const int A = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int ca = tile_comp[A];
+    if (ca < 0) return;  // uniform over the work-group
     const float tmargin = 2.0f * margin_c * kU * __uint_as_float(*nrm_max_bits);
-    const int A = (int)(e / ntiles), B = (int)(e % ntiles);
-    if (A >= ntiles || B <= A) return;
-    const int ca = tile_comp[A], cb = tile_comp[B];
-    if (ca < 0 || cb < 0 || ca == cb) return;
-    float u = tmin[e] + tmargin;
-    u = u > tmaxc[A] ? u : tmaxc[A];
-    u = u > tmaxc[B] ? u : tmaxc[B];
-    atomicMin(reinterpret_cast<unsigned int *>(bound + ca), __float_as_uint(u));
-    atomicMin(reinterpret_cast<unsigned int *>(bound + cb), __float_as_uint(u));
+    const float ta = tmaxc[A];
+    const float *row = tmin + (int64_t)A * ntiles;
+    float ua = kInfB;
+    for (int B0 = A + 1; B0 < ntiles; B0 += 256) {
+        const int B = B0 + tid;
+        int key = -1;
+        float u = kInfB;
+        if (B < ntiles) {
+            const int cb = tile_comp[B];
+            if (cb >= 0 && cb != ca) {
+                u = row[B] + tmargin;
+                u = u > ta ? u : ta;
+                const float tb = tmaxc[B];
+                u = u > tb ? u : tb;
+                key = cb;
+            }
+        }
+        ua = u < ua ? u : ua;
+        // segmented inclusive min over lanes of equal key: the last lane of
+        // each run then holds its run's minimum
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const float ov = __shfl_up(u, o, 64);
+            const int ok = __shfl_up(key, o, 64);
+            if (lane >= o && ok == key) u = ov < u ? ov : u;
+        }
+        const int nk = __shfl_down(key, 1, 64);
+        if (key >= 0 && (lane == 63 || nk != key))
+            atomicMin(reinterpret_cast<unsigned int *>(bound + key), __float_as_uint(u));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(ua, o, 64);
+        ua = ov < ua ? ov : ua;
+    }
+    if (lane == 0) ua_s[tid >> 6] = ua;
+    __syncthreads();
+    if (tid == 0) {
+        float v = ua_s[0];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) v = ua_s[w] < v ? ua_s[w] : v;
+        if (v < kInfB) atomicMin(reinterpret_cast<unsigned int *>(bound + ca), __float_as_uint(v));
+    }
 }
 
 __global__ __launch_bounds__(128) void tile_umax_kernel(int64_t n, const int *__restrict__ comp2,
@@ -914,9 +959,9 @@ hipError_t launch_tile_core_range(int64_t n, int64_t npad, const float *core2, f
 
 hipError_t launch_tile_bounds(int ntiles, const int *tile_comp, const float *tmin, const float *tmaxc, float margin_c,
                               const unsigned int *nrm_max_bits, float *bound, hipStream_t st) {
-    const int64_t pairs = (int64_t)ntiles * ntiles;
-    hipLaunchKernelGGL(tile_bounds_kernel, dim3((unsigned)nblocks(pairs, 256)), dim3(256), 0, st, ntiles, tile_comp,
-                       tmin, tmaxc, margin_c, nrm_max_bits, bound);
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tile_bounds_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, ntiles, tile_comp, tmin, tmaxc,
+                       margin_c, nrm_max_bits, bound);
     return hipGetLastError();
 }
 

@@ -91,3 +91,21 @@ def test_highdim_matches_tree_path_50k(monkeypatch):
     assert st["high_dim"] == 0
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+def test_tile_bounds_prune_and_agree(monkeypatch):
+    """From the second round the scans take the rows in component order and
+    skip the tile pairs that the first round's tile-pair minima rule out:
+    fewer tile pairs are scored and the MST is the one every round in tree
+    order (FH_TILE_BOUNDS=0) gives, bit for bit."""
+    X = workloads.gaussian_highdim(40000, 48, n_clusters=60, seed=9)
+    st_on = {}
+    a = mst_arrays(X, 10, stats=st_on)
+    monkeypatch.setenv("FH_TILE_BOUNDS", "0")
+    st_off = {}
+    b = mst_arrays(X, 10, stats=st_off)
+    print(st_on["rounds"], st_on["scored_tile_pairs"], st_off["scored_tile_pairs"], flush=True)
+    assert st_on["rounds"] >= 3
+    assert st_on["scored_tile_pairs"] < st_off["scored_tile_pairs"]
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
