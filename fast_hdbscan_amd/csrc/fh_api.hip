@@ -450,6 +450,9 @@ bool use_brute(const fh_ctx *ctx, int64_t n, int d, int kq, bool weighted) {
     return !weighted && fh::brute_padded_dim(d) > 0 && (d >= from || small) && kq <= 64 && n >= 256;
 }
 
+constexpr int kSampleOrder = 8;     // order statistic of the sampled thresholds
+constexpr int kSampleRetry = 2048;  // flagged rows above which the kNN runs again unsampled
+
 int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_knn, fh::BruteScanArgs &bsa,
           fh_stats &S) {
     hipStream_t st = ctx->stream;
@@ -501,15 +504,22 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
     }
     // thresholds: MODE 2 writes them where MODE 0 reads them; the
     // candidates are collected once per tile pair, in both directions
-    // (FH_SYM_SCAN=0: every pair from both sides)
+    // (FH_SYM_SCAN=0: every pair from both sides).  From 256 tiles on (and
+    // k + 1 <= 16) the thresholds come from a sample: the row tile and
+    // every 16th tile from it, 8th smallest value (about the 128th nearest
+    // neighbour in expectation), one sixteenth of the full scan; the
+    // re-scoring checks each row against its threshold and a row that may
+    // have missed a neighbour goes to the tree search, or, when more than
+    // kSampleRetry rows do, the whole kNN runs again with full thresholds
+    // (FH_KNN_SAMPLE: the tile step, 1 = full thresholds; FH_KNN_SAMPLE_RETRY:
+    // that row count).
+    const int64_t ntl = npad / 128;
+    int step = (kq <= 16 && ntl >= 256) ? 16 : 1;
+    if (const char *m = std::getenv("FH_KNN_SAMPLE")) step = std::max(1, std::atoi(m));
     fh::BruteScanArgs kth = bsa;
     kth.row_min = ctx->bx_thr.p;
-    kth.cap = kq;
-    FH_HIP(fh::launch_scan_kth(kth, st));
     fh::BruteScanArgs kc = bsa;
     kc.sym_knn = bsa.row_key != nullptr ? 1 : 0;
-    if (kc.sym_knn) FH_HIP(hipMemsetAsync(ctx->bx_cnt.p, 0, sizeof(int) * n, st));
-    FH_HIP(fh::launch_scan_knn(kc, st));
     fh::KnnRescoreArgs ra{};
     ra.xp = ctx->xp.p;
     ra.perm = ctx->perm.p;
@@ -526,10 +536,27 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
     ra.core_p = ctx->core_p.p;
     ra.flag_list = ctx->bx_flags.p;
     ra.flag_count = ctx->bx_small.p;
-    FH_HIP(fh::launch_knn_rescore(ra, st));
-    FH_HIP(hipMemcpyAsync(ctx->h_small, ctx->bx_small.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
-    const int nflag = ctx->h_small[0];
+    ra.thr = ctx->bx_thr.p;
+    ra.nrm = ctx->bx_nrm.p;
+    ra.nrm_max_bits = ctx->bx_nmax.p;
+    ra.margin_c = margin_c;
+    int retry = kSampleRetry;
+    if (const char *m = std::getenv("FH_KNN_SAMPLE_RETRY")) retry = std::atoi(m);
+    int nflag = 0;
+    for (;;) {
+        kth.tile_step = step;
+        kth.cap = step > 1 ? kSampleOrder : kq;
+        FH_HIP(fh::launch_scan_kth(kth, st));
+        if (kc.sym_knn) FH_HIP(hipMemsetAsync(ctx->bx_cnt.p, 0, sizeof(int) * n, st));
+        FH_HIP(fh::launch_scan_knn(kc, st));
+        FH_HIP(hipMemsetAsync(ctx->bx_small.p, 0, sizeof(int), st));
+        FH_HIP(fh::launch_knn_rescore(ra, st));
+        FH_HIP(hipMemcpyAsync(ctx->h_small, ctx->bx_small.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        FH_HIP(hipStreamSynchronize(st));
+        nflag = ctx->h_small[0];
+        if (step == 1 || nflag <= retry) break;
+        step = 1;
+    }
     S.knn_tree_rows = nflag;
     if (nflag > 0) {
         fh::KnnWaveArgs wa{};

@@ -26,6 +26,12 @@ struct BruteScanArgs {
     // k-th distance bounds (MODE 2): the k-th smallest is taken with k = cap
     const unsigned int *nrm_max_bits;
     float margin_c;
+    // MODE 2: scan only the column tiles t with (t - row tile) a multiple
+    // of tile_step (the row tile itself and every tile_step-th from it;
+    // <= 1: every tile).  The k-th smallest over that sample is then no
+    // bound of the k-th nearest distance but a guess of a threshold, which
+    // the re-scoring checks row by row (KnnRescoreArgs::thr)
+    int tile_step;
     // Boruvka row minimum
     const float *core_p;  // [n]
     const int *comp_p;    // [n]
@@ -114,6 +120,14 @@ struct KnnRescoreArgs {
     float *core_p;
     int *flag_list;   // positions whose result must come from the tree search
     int *flag_count;
+    // the candidate thresholds of the scan (thr) with the margin
+    // margin_c u (nrm[p] + max nrm): a row whose k-th smallest exact value
+    // plus that margin exceeds its threshold may have missed a neighbour
+    // and is listed for the tree search as well
+    const float *thr;
+    const float *nrm;
+    const unsigned int *nrm_max_bits;
+    float margin_c;
 };
 hipError_t launch_knn_rescore(const KnnRescoreArgs &a, hipStream_t st);
 

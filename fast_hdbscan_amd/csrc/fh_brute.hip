@@ -187,6 +187,11 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     const float *tmin_row = prune ? a.tmin_in + (int64_t)blockIdx.x * ntiles : nullptr;
     const float tmargin = prune ? 2.0f * a.margin_c * kU * __uint_as_float(*a.nrm_max_bits) : 0.0f;
     auto next_tile = [&](int t) {
+        if (MODE == 2 && a.tile_step > 1) {
+            // the next sampled tile at or after t
+            const int r = ((int)blockIdx.x - t) % a.tile_step;
+            return t + (r < 0 ? r + a.tile_step : r);
+        }
         for (; t < ntiles; ++t) {
             if (my_tc >= 0 && a.tile_comp[t] == my_tc) continue;
             if (prune) {
@@ -510,6 +515,12 @@ __global__ __launch_bounds__(64) void knn_rescore_kernel(KnnRescoreArgs a) {
         }
         for (int t = 0; t < a.k; ++t)
             if (!(v[t * 64] < v[(t + 1) * 64])) flag = true;
+        // every column whose exact value is at most the k-th smallest found
+        // has an approximate value at most that plus the margin; below the
+        // threshold it was collected, so the k + 1 smallest found are the
+        // k + 1 smallest of all columns (ties at the k-th included)
+        const float marg = a.margin_c * kU * (a.nrm[p] + __uint_as_float(*a.nrm_max_bits));
+        if (!(v[(a.k - 1) * 64] + marg <= a.thr[p])) flag = true;
     }
     if (flag) {
         const int slot = atomicAdd(a.flag_count, 1);

@@ -109,3 +109,36 @@ def test_tile_bounds_prune_and_agree(monkeypatch):
     assert st_on["scored_tile_pairs"] < st_off["scored_tile_pairs"]
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+@pytest.mark.parametrize("step,retry", [("1", None), ("4", None), ("16", None), ("16", "0")])
+def test_knn_sampled_thresholds(monkeypatch, step, retry):
+    """kNN thresholds from every step-th tile (the 8th smallest value there):
+    rows whose threshold could have missed a neighbour go to the tree search
+    (knn_tree_rows), or every row runs again with full thresholds when more
+    than the retry count do; the cores and neighbours equal the oracle's."""
+    monkeypatch.setenv("FH_KNN_SAMPLE", step)
+    if retry is not None:
+        monkeypatch.setenv("FH_KNN_SAMPLE_RETRY", retry)
+    X = workloads.gaussian_highdim(6000, 48, n_clusters=15, seed=21)
+    st = _check(X, 10)
+    print(step, retry, st["knn_tree_rows"], flush=True)
+
+
+def test_knn_sampled_thresholds_small_clusters(monkeypatch):
+    # clusters of ~12 points: a sample of whole tiles misses most of a
+    # row's neighbours, so many thresholds are too large (overflowing
+    # lists) or too small; every such row is caught by the check
+    monkeypatch.setenv("FH_KNN_SAMPLE", "8")
+    X = workloads.gaussian_highdim(5000, 32, n_clusters=400, seed=22)
+    st = _check(X, 6)
+    print(st["knn_tree_rows"], flush=True)
+
+
+def test_knn_sampled_thresholds_duplicates(monkeypatch):
+    monkeypatch.setenv("FH_KNN_SAMPLE", "4")
+    rng = np.random.default_rng(23)
+    X = workloads.gaussian_highdim(3000, 24, n_clusters=10, seed=23)
+    X = np.vstack([X, X[rng.choice(3000, 900, replace=False)], X[:40], X[:40]])
+    X = X[rng.permutation(X.shape[0])]
+    _check(X, 5)
