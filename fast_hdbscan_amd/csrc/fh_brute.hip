@@ -150,7 +150,9 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
         for (int r = 0; r < 4; ++r) {
             const int64_t row = rbase + rb * 16 + 4 * g + r;
             const bool ok = row < n;
-            rn[rb][r] = a.nrm[row];
+            // padding rows and columns get +inf norms: every approximate
+            // value of their pairs is +inf, which no comparison below takes
+            rn[rb][r] = ok ? a.nrm[row] : kInfB;
             ri[rb][r] = -1;
             if constexpr (MODE == 0) {
                 rv[rb][r] = ok ? a.thr[row] : -kInfB;
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
         for (int v = 0; v < NV; ++v) pre[v] = src[tid + v * 256];
         if (tid < 128) {
             const int64_t col = (int64_t)t * 128 + tid;
-            pn = a.nrm[col];
+            pn = col < n ? a.nrm[col] : kInfB;
             if constexpr (MODE == 1) {
                 pc = col < n ? a.core_p[col] : 0.0f;
                 pk = col < n ? a.comp_p[col] : -3;
@@ -294,32 +296,49 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                         }
                     }
             } else if constexpr (MODE == 0) {
-                const float ct = col_thr[cur][cl];
+                // hits are rare: the compares first, the appends only when
+                // some lane of the wave has one (the exact conditions are
+                // checked again there)
                 const bool other = t != (int)blockIdx.x;
+                const float ct = (sym && other) ? col_thr[cur][cl] : -kInfB;
+                float ap[2][4];
+                bool hit = false;
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const float approx = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
-                        const int rl = wave * 32 + rb * 16 + 4 * g + r;
-                        if (valid && approx <= rv[rb][r]) {
-                            if (sym) {
-                                const int slot = atomicAdd(a.cand_cnt + blk_row + rl, 1);
-                                if (slot < a.cap) a.cand[(blk_row + rl) * a.cap + slot] = (int)col;
-                            } else {
-                                const int slot = atomicAdd(&cnt_l[rl], 1);
-                                if (slot < a.cap) a.cand[(blk_row + rl) * a.cap + slot] = (int)col;
+                        ap[rb][r] = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
+                        hit = hit || ap[rb][r] <= rv[rb][r] || ap[rb][r] <= ct;
+                    }
+                if (__builtin_amdgcn_ballot_w64(hit) != 0) {
+#pragma unroll
+                    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const float approx = ap[rb][r];
+                            const int rl = wave * 32 + rb * 16 + 4 * g + r;
+                            if (valid && approx <= rv[rb][r]) {
+                                if (sym) {
+                                    const int slot = atomicAdd(a.cand_cnt + blk_row + rl, 1);
+                                    if (slot < a.cap) a.cand[(blk_row + rl) * a.cap + slot] = (int)col;
+                                } else {
+                                    const int slot = atomicAdd(&cnt_l[rl], 1);
+                                    if (slot < a.cap) a.cand[(blk_row + rl) * a.cap + slot] = (int)col;
+                                }
+                            }
+                            // the same pair for the column's row (tiles other
+                            // than the block's own; the own tile gives both
+                            // orders on the row side)
+                            if (sym && other && valid && rcomp[rb][r] != -2 && approx <= ct) {
+                                const int slot = atomicAdd(a.cand_cnt + col, 1);
+                                if (slot < a.cap) a.cand[col * a.cap + slot] = (int)(blk_row + rl);
                             }
                         }
-                        // the same pair for the column's row (tiles other
-                        // than the block's own; the own tile gives both
-                        // orders on the row side)
-                        if (sym && other && valid && rcomp[rb][r] != -2 && approx <= ct) {
-                            const int slot = atomicAdd(a.cand_cnt + col, 1);
-                            if (slot < a.cap) a.cand[col * a.cap + slot] = (int)(blk_row + rl);
-                        }
-                    }
+                }
             } else {
+                // padding rows and columns give +inf values (their norms),
+                // pairs inside one component +inf by selection; the minima
+                // by compare and select, first index kept on equal values
                 const float cc = col_core[cur][cl];
                 const int ck = col_comp[cur][cl];
                 float cv = kInfB;  // this column's minimum over the lane's rows
@@ -332,19 +351,18 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                         const int rl = wave * 32 + rb * 16 + 4 * g + r;
                         const float rc = row_core_s[rl];
                         const int rk = row_comp_s[rl];
-                        if (a.tmin_out && valid && rk != -2) tmv = approx < tmv ? approx : tmv;
-                        float m = rc > cc ? rc : cc;
-                        m = approx > m ? approx : m;
-                        const bool cand = valid && ck != rk;
-                        if (cand && m < rv[rb][r]) {
-                            rv[rb][r] = m;
-                            ri[rb][r] = (int)col;
-                        }
+                        // no value here is NaN: fminf / fmaxf (single
+                        // instructions) equal the compare-and-select forms
+                        tmv = __builtin_fminf(approx, tmv);
+                        float m = __builtin_fmaxf(__builtin_fmaxf(rc, cc), approx);
+                        m = ck != rk ? m : kInfB;
+                        const bool lt = m < rv[rb][r];
+                        rv[rb][r] = lt ? m : rv[rb][r];
+                        ri[rb][r] = lt ? (int)col : ri[rb][r];
                         // rows in increasing order: the first equal one is kept
-                        if (sym && cand && rk != -2 && m < cv) {
-                            cv = m;
-                            ca = (int)(rbase + rb * 16 + 4 * g + r);
-                        }
+                        const bool lc = m < cv;
+                        cv = lc ? m : cv;
+                        ca = lc ? (int)(rbase + rb * 16 + 4 * g + r) : ca;
                     }
                 if (sym) {
                     // the 4 lanes holding this column's other rows of the wave
