@@ -60,22 +60,29 @@ int brute_padded_dim(int d) {
 
 // ------------------------------------------------------------- packing ----
 
-// Centre on the root box's midpoint, zero-pad to dp dimensions and lay the
-// rows out in tiles of 128: inside a tile, dimension k of row r sits at
-// [k / 4][r][k % 4], which is both the order the LDS tile is read in (one
-// wave reads 64 consecutive floats per MFMA operand) and a straight copy.
+// Rows are laid out in tiles of 128 (a tile is copied to LDS as it is):
+// inside a tile, dimension k of row r sits at
+//     (k / 4) * 512 + r * 4 + ((k % 4) ^ swz(r)),   swz(r) = 2 if r & 8,
+// so one wave reads 64 consecutive floats per MFMA operand, and each half
+// wave (lanes of g = k % 4 in {0, 1} or {2, 3}, rows r .. r + 15) hits 32
+// distinct banks: without the swizzle rows r and r + 8 share a bank.
+__device__ inline int tile_slot(int r, int k) { return (k >> 2) * 512 + r * 4 + ((k & 3) ^ ((r >> 2) & 2)); }
+
+// Centre on the root box's midpoint, zero-pad to dp dimensions and pack the
+// rows into tiles (tile_slot).
 __global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const float *__restrict__ xp,
                                    const float *__restrict__ root_box, float *__restrict__ xc, float *__restrict__ nrm,
                                    unsigned int *nrm_max_bits) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npad) return;
     // tile layout: 128 rows per tile, inside a tile [k / 4][row % 128][k % 4]
-    float *tile = xc + (p / 128) * 128 * dp + (p % 128) * 4;
+    float *tile = xc + (p / 128) * 128 * dp;
+    const int r = (int)(p % 128);
     float s = 0.0f;
     for (int k = 0; k < dp; ++k) {
         float v = 0.0f;
         if (p < n && k < d) v = xp[p * d + k] - 0.5f * (root_box[k] + root_box[d + k]);
-        tile[(k >> 2) * 512 + (k & 3)] = v;
+        tile[tile_slot(r, k)] = v;
         s = __builtin_fmaf(v, v, s);
     }
     nrm[p] = s;
@@ -128,7 +135,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     float A[2][DP4];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
-        const float *src = a.xc + blk_row * DP + (wave * 32 + rb * 16 + c16) * 4 + g;
+        const float *src = a.xc + blk_row * DP + (wave * 32 + rb * 16 + c16) * 4 + (g ^ ((c16 >> 2) & 2));
 #pragma unroll
         for (int s = 0; s < DP4; ++s) A[rb][s] = src[s * 512];
     }
@@ -251,24 +258,21 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
         ++scored;
         const float *bt = bs + cur * TILE;
         floatx4 acc[2][8];
+        const int64_t col0 = (int64_t)t * 128;
+        float tmv = kInfB;  // MODE 1, tmin_out: the lane's smallest approximate distance of the pair
+        // column block by column block: its 2 x DP4 products, then its
+        // results, so that one block's compares overlap the next block's
+        // products
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int cb = 0; cb < 8; ++cb) {
+            acc[0][cb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            acc[1][cb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = 0; s < DP4; ++s) {
-            const float *brow = bt + s * 512 + c16 * 4 + g;
-#pragma unroll
-            for (int cb = 0; cb < 8; ++cb) {
-                const float b = brow[cb * 64];
+            for (int s = 0; s < DP4; ++s) {
+                const float b = bt[s * 512 + cb * 64 + c16 * 4 + (g ^ ((c16 >> 2) & 2))];
                 acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[0][s], b, acc[0][cb], 0, 0, 0);
                 acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[1][s], b, acc[1][cb], 0, 0, 0);
             }
-        }
-        const int64_t col0 = (int64_t)t * 128;
-        float tmv = kInfB;  // MODE 1, tmin_out: the lane's smallest approximate distance of the pair
-#pragma unroll
-        for (int cb = 0; cb < 8; ++cb) {
             const int cl = cb * 16 + c16;
             const int64_t col = col0 + cl;
             const bool valid = col < n;
@@ -821,12 +825,13 @@ __global__ void order_pack_kernel(int64_t n, int d, int dp, int64_t npad, const 
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npad) return;
     const int64_t q = p < n ? order[p] : -1;
-    float *tile = xc + (p / 128) * 128 * dp + (p % 128) * 4;
+    float *tile = xc + (p / 128) * 128 * dp;
+    const int r = (int)(p % 128);
     float s = 0.0f;
     for (int k = 0; k < dp; ++k) {
         float v = 0.0f;
         if (q >= 0 && k < d) v = xp[q * d + k] - 0.5f * (root_box[k] + root_box[d + k]);
-        tile[(k >> 2) * 512 + (k & 3)] = v;
+        tile[tile_slot(r, k)] = v;
         s = __builtin_fmaf(v, v, s);
     }
     nrm[p] = s;
