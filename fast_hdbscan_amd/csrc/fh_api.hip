@@ -456,7 +456,33 @@ constexpr int kSampleRetry = 2048;  // flagged rows above which the kNN runs aga
 int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_knn, fh::BruteScanArgs &bsa,
           fh_stats &S) {
     hipStream_t st = ctx->stream;
-    const int dp = fh::brute_padded_dim(d);
+    int dp = fh::brute_padded_dim(d);
+    // Products on the FP16 matrix cores from a hi / lo split of the centred
+    // rows scaled by s = 2^(14 - e), where 2^e exceeds half the root box's
+    // largest extent (so |x' s| < 2^14 and x' s is exact); dp 32 or 64.
+    // The split keeps 22 bits of x' s, the hi.lo and lo.hi products are
+    // summed before hi.hi, and the scale is exact, so the products' error
+    // stays within the FP32 scans' margin (header of fh_brute.hip).
+    // FH_MFMA_SPLIT=0: FP32 products (v_mfma_f32_16x16x4f32) throughout.
+    bool split = true;
+    if (const char *m = std::getenv("FH_MFMA_SPLIT")) split = std::atoi(m) != 0;
+    float pack_scale = 0.0f, dot_scale = -2.0f;
+    if (split) {
+        std::vector<float> hb(2 * (size_t)d);
+        FH_HIP(hipMemcpyAsync(hb.data(), ctx->box.p, sizeof(float) * 2 * d, hipMemcpyDeviceToHost, st));
+        FH_HIP(hipStreamSynchronize(st));
+        float half_ext = 0.0f;
+        for (int k = 0; k < d; ++k) half_ext = std::max(half_ext, 0.5f * std::fabs(hb[d + k] - hb[k]));
+        int e = 0;
+        if (half_ext > 0.0f) std::frexp(half_ext, &e);  // half_ext < 2^e
+        if (!std::isfinite(half_ext) || e < -40 || e > 60) {
+            split = false;
+        } else {
+            pack_scale = std::ldexp(1.0f, 14 - e);
+            dot_scale = -std::ldexp(1.0f, 2 * e - 27);  // -2 / s^2
+            dp = d <= 32 ? 32 : 64;
+        }
+    }
     const float margin_c = 8.0f * dp + 64.0f;
     const int64_t npad = (n + 127) / 128 * 128;
     S.high_dim = 1;
@@ -474,7 +500,10 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
     FH_HIP(hipMemsetAsync(ctx->bx_nmax.p, 0, sizeof(unsigned int), st));
     FH_HIP(hipMemsetAsync(ctx->bx_small.p, 0, 4 * sizeof(int), st));
     FH_HIP(fh::launch_center_pack(n, d, dp, npad, ctx->xp.p, ctx->box.p, ctx->bx_xc.p, ctx->bx_nrm.p,
-                                  ctx->bx_nmax.p, st));
+                                  ctx->bx_nmax.p, pack_scale, st));
+    bsa.split = split ? 1 : 0;
+    bsa.dot_scale = dot_scale;
+    bsa.pack_scale = pack_scale;
     bsa.xc = ctx->bx_xc.p;
     bsa.nrm = ctx->bx_nrm.p;
     bsa.n = n;
@@ -992,7 +1021,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
                                           ctx->bx_otemp.p, tb, st));
                 // the packed rows of the kNN scans are not read again
                 FH_HIP(fh::launch_order_pack(n, d, bsa.dp, npad, ctx->bx_order.p, ctx->xp.p, ctx->box.p, ctx->core_p.p,
-                                             ctx->bx_xc.p, ctx->bx_nrm2.p, ctx->bx_core2.p, st));
+                                             ctx->bx_xc.p, ctx->bx_nrm2.p, ctx->bx_core2.p, bsa.pack_scale, st));
                 FH_HIP(fh::launch_tile_core_range(n, npad, ctx->bx_core2.p, ctx->bx_tminc.p, ctx->bx_tmaxc.p, st));
                 FH_HIP(hipMemsetAsync(ctx->bx_tmin.p, 0, sizeof(float) * ntl * ntl, st));
             }

@@ -14,10 +14,16 @@ namespace fh {
 int brute_padded_dim(int d);
 
 struct BruteScanArgs {
-    const float *xc;      // [npad * dp] centred points, tree order, dims grouped by (k mod 4)
+    const float *xc;      // [npad * dp] centred points, tree order, in tiles of 128 rows
     const float *nrm;     // [npad] squared norms of the centred rows (approximate)
     int64_t n, npad;
     int dp;
+    // split != 0: xc holds the FP16 hi / lo split of the centred rows scaled
+    // by a power of two s (launch_center_pack with scale s; dp 32 or 64),
+    // multiplied on the 16x16x32 FP16 MFMA, and dot_scale = -2 / s^2
+    int split;
+    float dot_scale;
+    float pack_scale;  // s (0 without the split), for re-packing the rows
     // kNN candidate collection
     const float *thr;     // [n] row thresholds (approximate squared distance)
     int *cand;            // [n * cap] candidate positions
@@ -73,7 +79,7 @@ hipError_t launch_order_keys(int64_t n, const int *comp_p, uint32_t *keys, int *
 // -1 for the padding rows p2 >= n): xc, nrm and core2 in that order.
 hipError_t launch_order_pack(int64_t n, int d, int dp, int64_t npad, const int *order, const float *xp,
                              const float *root_box, const float *core_p, float *xc, float *nrm, float *core2,
-                             hipStream_t st);
+                             float scale, hipStream_t st);
 // comp2[p2] = comp_p[order[p2]] (-3 for the padding rows)
 hipError_t launch_gather_comp(int64_t n, int64_t npad, const int *order, const int *comp_p, int *comp2,
                               hipStream_t st);
@@ -96,8 +102,10 @@ hipError_t launch_unpack_row_keys_ordered(int64_t n, const unsigned long long *r
 
 hipError_t launch_tile_comp(int64_t n, int64_t npad, const int *comp_p, int *tile_comp, hipStream_t st);
 
+// scale 0: FP32 tiles; scale > 0 (a power of two with |x - centre| s <
+// 2^14, dp 32 or 64): the FP16 hi / lo split of (x - centre) s
 hipError_t launch_center_pack(int64_t n, int d, int dp, int64_t npad, const float *xp, const float *root_box,
-                              float *xc, float *nrm, unsigned int *nrm_max_bits, hipStream_t st);
+                              float *xc, float *nrm, unsigned int *nrm_max_bits, float scale, hipStream_t st);
 // row thresholds into row_min (k = cap), then candidate collection
 hipError_t launch_scan_kth(const BruteScanArgs &a, hipStream_t st);
 hipError_t launch_scan_knn(const BruteScanArgs &a, hipStream_t st);

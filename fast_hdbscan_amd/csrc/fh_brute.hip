@@ -1,19 +1,34 @@
-// High-dimensional candidate search on the FP32 matrix cores.
+// High-dimensional candidate search on the matrix cores.
 //
 // In high dimension the KD-tree's box bounds stop pruning and every query
 // degenerates into a scan of all leaves, so the kNN and the Boruvka rounds
-// are done as blocked all-pairs scans instead.  The scans run on MFMA
-// (v_mfma_f32_16x16x4f32) with the Gram expansion
+// are done as blocked all-pairs scans instead.  The scans run on MFMA with
+// the Gram expansion
 //     ||x - y||^2 ~= ||x'||^2 + ||y'||^2 - 2 x'.y'     (x' = x - mu)
 // which only *selects candidates*: every pair that could matter under a
 // certified error bound is re-scored with the reference's own float32
 // difference-sum (rdist, sequential FMA), so all outputs stay exact.
 //
+// Products.  FP32 form: v_mfma_f32_16x16x4f32 on x' itself.  Split form
+// (the default; dp 32 or 64): x' is scaled by a power of two s with
+// |x' s| < 2^14 (exact) and stored as the FP16 pair h = fp16(x' s),
+// l = fp16(x' s - h) (x' s - h is exact in FP32); v_mfma_f32_16x16x32_f16
+// sums h.l' and l.h' of every chunk first, then h.h' (each product exact in
+// FP32), and -2 / s^2 (exact) scales the sum.  5.3x fewer matrix cycles
+// than the FP32 form for the same margin.
+//
 // Error bound.  With u = 2^-24, |approx - rdist| <= (4d + 6) u (|x'|^2 +
 // |y'|^2) covers the dot-product and norm sums (gamma_d each), the final
-// combination, the centring roundings and rdist's own rounding; the code
-// uses margin_c * u * (|x'|^2 + max|y'|^2) with margin_c = 8 dp + 64, twice
-// that.
+// combination, the centring roundings and rdist's own rounding (FP32
+// form).  Split form: x' s = h + l + e with |e| <= 2^-22 |x' s| + 2^-25 (the
+// second term from FP16 subnormals, or 2^-14 if the hardware flushes them:
+// relative to s max|y'| either is below u).  The dropped l.l' and the e
+// terms are at most 3 * 2^-22 = 12 u per |x'_i y'_i|; the small products'
+// partial sums stay below 2^-9 sum|x'_i y'_i| s^2, so their roundings add at
+// most 2 dp 2^-9 u; h.h' accumulates like the FP32 form (gamma_dp).  The
+// dot product's error is thus gamma_dp + 13 u, and the total stays below
+// (4 dp + 19) u (|x'|^2 + |y'|^2).  Both forms use margin_c * u * (|x'|^2 +
+// max|y'|^2) with margin_c = 8 dp + 64, at least twice their bound.
 //
 // kNN (numba_kdtree.py:765-792): a first MFMA scan bounds each row's k-th
 // distance from above (the k-th smallest of per-lane top-4 lists: k real
@@ -46,6 +61,7 @@ constexpr float kInfB = __builtin_huge_valf();
 constexpr float kU = 5.9604644775390625e-08f;  // 2^-24
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 
 inline int64_t nblocks(int64_t n, int b) { return (n + b - 1) / b; }
 
@@ -68,11 +84,39 @@ int brute_padded_dim(int d) {
 // distinct banks: without the swizzle rows r and r + 8 share a bank.
 __device__ inline int tile_slot(int r, int k) { return (k >> 2) * 512 + r * 4 + ((k & 3) ^ ((r >> 2) & 2)); }
 
+// The FP16 split form of a tile (SPLIT scans, dp = 32 or 64): dimension
+// k = 32 c + 8 q + j of row r, plane p (0: hi, 1: lo), as a half at
+//     ((c * 2 + p) * 4 + q) * 1024 + r * 8 + j
+// (halves from the tile's start; a tile holds 128 dp floats' bytes).  The
+// lane of the 16x16x32 MFMA that supplies row (or column) r, k = 8 q .. 8 q
+// + 7 of chunk c reads one 16-byte vector; the 16 lanes of one q read 256
+// consecutive bytes.
+__device__ inline int split_slot(int r, int k) {
+    return (((k >> 5) * 2) * 4 + ((k >> 3) & 3)) * 1024 + r * 8 + (k & 7);
+}
+
+// Store centred value v of row r, dimension k into a tile: FP32 (tile_slot)
+// or, with scale > 0, the FP16 pair hi = fp16(v s), lo = fp16(v s - hi)
+// (split_slot; s a power of two with |v s| < 2^14, so v s is exact).
+__device__ inline void pack_store(float *tile, int r, int k, float v, float scale) {
+    if (scale > 0.0f) {
+        _Float16 *h = reinterpret_cast<_Float16 *>(tile);
+        const float vs = v * scale;
+        const _Float16 hi = (_Float16)vs;
+        const _Float16 lo = (_Float16)(vs - (float)hi);
+        const int slot = split_slot(r, k);
+        h[slot] = hi;
+        h[slot + 4096] = lo;  // plane 1: 4 blocks of 1024 halves further
+    } else {
+        tile[tile_slot(r, k)] = v;
+    }
+}
+
 // Centre on the root box's midpoint, zero-pad to dp dimensions and pack the
-// rows into tiles (tile_slot).
+// rows into tiles (tile_slot, or split_slot when scale > 0).
 __global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const float *__restrict__ xp,
                                    const float *__restrict__ root_box, float *__restrict__ xc, float *__restrict__ nrm,
-                                   unsigned int *nrm_max_bits) {
+                                   unsigned int *nrm_max_bits, float scale) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npad) return;
     // tile layout: 128 rows per tile, inside a tile [k / 4][row % 128][k % 4]
@@ -82,7 +126,7 @@ __global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const
     for (int k = 0; k < dp; ++k) {
         float v = 0.0f;
         if (p < n && k < d) v = xp[p * d + k] - 0.5f * (root_box[k] + root_box[d + k]);
-        tile[tile_slot(r, k)] = v;
+        pack_store(tile, r, k, v, scale);
         s = __builtin_fmaf(v, v, s);
     }
     nrm[p] = s;
@@ -103,9 +147,12 @@ __global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const
 //         (64 values from 16 lanes, all real columns) plus one margin
 //         bounds the true k-th exact distance, plus a second margin bounds
 //         the approximate value of every true k nearest neighbour.
-template <int DP4, int MODE>
+template <int DP4, int MODE, bool SPLIT>
 __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     constexpr int DP = 4 * DP4;
+    // SPLIT: FP16 hi / lo planes (split_slot), NC chunks of 32 dimensions
+    constexpr int NC = SPLIT ? DP / 32 : 1;
+    static_assert(!SPLIT || DP % 32 == 0, "split tiles hold whole 32-dimension chunks");
     constexpr int TILE = 128 * DP;  // floats per tile
     constexpr int NV = DP / 8;      // float4 loads per thread per tile
     __shared__ float bs[2 * TILE > 128 * 64 ? 2 * TILE : 128 * 64];
@@ -132,13 +179,27 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     const int64_t rbase = blk_row + wave * 32;
     const int64_t n = a.n;
 
-    float A[2][DP4];
+    float A[2][SPLIT ? 1 : DP4];
+    half8 AH[2][NC], AL[2][NC];  // SPLIT: the row operands, hi and lo planes
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
-        const float *src = a.xc + blk_row * DP + (wave * 32 + rb * 16 + c16) * 4 + (g ^ ((c16 >> 2) & 2));
+        if constexpr (SPLIT) {
+            const _Float16 *src = reinterpret_cast<const _Float16 *>(a.xc + blk_row * DP) +
+                                  (wave * 32 + rb * 16 + c16) * 8;
 #pragma unroll
-        for (int s = 0; s < DP4; ++s) A[rb][s] = src[s * 512];
+            for (int c = 0; c < NC; ++c) {
+                AH[rb][c] = *reinterpret_cast<const half8 *>(src + ((c * 2) * 4 + g) * 1024);
+                AL[rb][c] = *reinterpret_cast<const half8 *>(src + ((c * 2 + 1) * 4 + g) * 1024);
+            }
+        } else {
+            const float *src = a.xc + blk_row * DP + (wave * 32 + rb * 16 + c16) * 4 + (g ^ ((c16 >> 2) & 2));
+#pragma unroll
+            for (int s = 0; s < DP4; ++s) A[rb][s] = src[s * 512];
+        }
     }
+    // approx = |x'|^2 + |y'|^2 + c2 (x'.y') with the products' scale folded
+    // in: -2 (FP32), -2 / s^2 (SPLIT: exact, s a power of two)
+    const float c2 = SPLIT ? a.dot_scale : -2.0f;
     // the 8 output rows this lane holds: rbase + rb*16 + 4g + r
     float rn[2][4], rv[2][4], rcore[2][4];
     int ri[2][4], rcomp[2][4];  // MODE 1: rcomp -2 marks a padding row
@@ -267,11 +328,36 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
         for (int cb = 0; cb < 8; ++cb) {
             acc[0][cb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
             acc[1][cb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            if constexpr (SPLIT) {
+                // hi.lo and lo.hi of every chunk first, hi.hi last: the
+                // partial sums of the small products stay ~2^-10 of the
+                // result, so their roundings add almost nothing
+                const _Float16 *bh = reinterpret_cast<const _Float16 *>(bt) + (cb * 16 + c16) * 8;
+                half8 BH[NC], BL[NC];
 #pragma unroll
-            for (int s = 0; s < DP4; ++s) {
-                const float b = bt[s * 512 + cb * 64 + c16 * 4 + (g ^ ((c16 >> 2) & 2))];
-                acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[0][s], b, acc[0][cb], 0, 0, 0);
-                acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[1][s], b, acc[1][cb], 0, 0, 0);
+                for (int c = 0; c < NC; ++c) {
+                    BH[c] = *reinterpret_cast<const half8 *>(bh + ((c * 2) * 4 + g) * 1024);
+                    BL[c] = *reinterpret_cast<const half8 *>(bh + ((c * 2 + 1) * 4 + g) * 1024);
+                }
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int rb = 0; rb < 2; ++rb) {
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(AH[rb][c], BL[c], acc[rb][cb], 0, 0, 0);
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(AL[rb][c], BH[c], acc[rb][cb], 0, 0, 0);
+                    }
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int rb = 0; rb < 2; ++rb)
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(AH[rb][c], BH[c], acc[rb][cb], 0, 0, 0);
+            } else {
+#pragma unroll
+                for (int s = 0; s < DP4; ++s) {
+                    const float b = bt[s * 512 + cb * 64 + c16 * 4 + (g ^ ((c16 >> 2) & 2))];
+                    acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[0][s], b, acc[0][cb], 0, 0, 0);
+                    acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[1][s], b, acc[1][cb], 0, 0, 0);
+                }
             }
             const int cl = cb * 16 + c16;
             const int64_t col = col0 + cl;
@@ -282,7 +368,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const float approx = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
+                        const float approx = __builtin_fmaf(c2, acc[rb][cb][r], rn[rb][r] + cn);
                         if (valid && approx < t4[rb][r][3]) {
                             float v = approx;
                             float *t = t4[rb][r];
@@ -311,7 +397,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        ap[rb][r] = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
+                        ap[rb][r] = __builtin_fmaf(c2, acc[rb][cb][r], rn[rb][r] + cn);
                         hit = hit || ap[rb][r] <= rv[rb][r] || ap[rb][r] <= ct;
                     }
                 if (__builtin_amdgcn_ballot_w64(hit) != 0) {
@@ -351,7 +437,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const float approx = __builtin_fmaf(-2.0f, acc[rb][cb][r], rn[rb][r] + cn);
+                        const float approx = __builtin_fmaf(c2, acc[rb][cb][r], rn[rb][r] + cn);
                         const int rl = wave * 32 + rb * 16 + 4 * g + r;
                         const float rc = row_core_s[rl];
                         const int rk = row_comp_s[rl];
@@ -756,21 +842,30 @@ __global__ __launch_bounds__(256) void boruvka_row_scan_kernel(BoruvkaExactArgs 
 // ------------------------------------------------------------ launches ----
 
 hipError_t launch_center_pack(int64_t n, int d, int dp, int64_t npad, const float *xp, const float *root_box,
-                              float *xc, float *nrm, unsigned int *nrm_max_bits, hipStream_t st) {
+                              float *xc, float *nrm, unsigned int *nrm_max_bits, float scale, hipStream_t st) {
+    if (scale > 0.0f && dp != 32 && dp != 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(center_pack_kernel, dim3(nblocks(npad, 256)), dim3(256), 0, st, n, d, dp, npad, xp, root_box,
-                       xc, nrm, nrm_max_bits);
+                       xc, nrm, nrm_max_bits, scale);
     return hipGetLastError();
 }
 
 template <int MODE>
 static hipError_t launch_scan(const BruteScanArgs &a, hipStream_t st) {
     const dim3 g(a.npad / 128), b(256);
+    if (a.split) {
+        switch (a.dp) {
+            case 32: hipLaunchKernelGGL((brute_scan_kernel<8, MODE, true>), g, b, 0, st, a); break;
+            case 64: hipLaunchKernelGGL((brute_scan_kernel<16, MODE, true>), g, b, 0, st, a); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (a.dp) {
-        case 16: hipLaunchKernelGGL((brute_scan_kernel<4, MODE>), g, b, 0, st, a); break;
-        case 24: hipLaunchKernelGGL((brute_scan_kernel<6, MODE>), g, b, 0, st, a); break;
-        case 32: hipLaunchKernelGGL((brute_scan_kernel<8, MODE>), g, b, 0, st, a); break;
-        case 48: hipLaunchKernelGGL((brute_scan_kernel<12, MODE>), g, b, 0, st, a); break;
-        case 64: hipLaunchKernelGGL((brute_scan_kernel<16, MODE>), g, b, 0, st, a); break;
+        case 16: hipLaunchKernelGGL((brute_scan_kernel<4, MODE, false>), g, b, 0, st, a); break;
+        case 24: hipLaunchKernelGGL((brute_scan_kernel<6, MODE, false>), g, b, 0, st, a); break;
+        case 32: hipLaunchKernelGGL((brute_scan_kernel<8, MODE, false>), g, b, 0, st, a); break;
+        case 48: hipLaunchKernelGGL((brute_scan_kernel<12, MODE, false>), g, b, 0, st, a); break;
+        case 64: hipLaunchKernelGGL((brute_scan_kernel<16, MODE, false>), g, b, 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -821,7 +916,7 @@ __global__ __launch_bounds__(128) void tile_comp_kernel(int64_t n, const int *__
 __global__ void order_pack_kernel(int64_t n, int d, int dp, int64_t npad, const int *__restrict__ order,
                                   const float *__restrict__ xp, const float *__restrict__ root_box,
                                   const float *__restrict__ core_p, float *__restrict__ xc, float *__restrict__ nrm,
-                                  float *__restrict__ core2) {
+                                  float *__restrict__ core2, float scale) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npad) return;
     const int64_t q = p < n ? order[p] : -1;
@@ -831,7 +926,7 @@ __global__ void order_pack_kernel(int64_t n, int d, int dp, int64_t npad, const 
     for (int k = 0; k < dp; ++k) {
         float v = 0.0f;
         if (q >= 0 && k < d) v = xp[q * d + k] - 0.5f * (root_box[k] + root_box[d + k]);
-        tile[tile_slot(r, k)] = v;
+        pack_store(tile, r, k, v, scale);
         s = __builtin_fmaf(v, v, s);
     }
     nrm[p] = s;
@@ -968,9 +1063,10 @@ __global__ void unpack_row_keys_ordered_kernel(int64_t n, const unsigned long lo
 
 hipError_t launch_order_pack(int64_t n, int d, int dp, int64_t npad, const int *order, const float *xp,
                              const float *root_box, const float *core_p, float *xc, float *nrm, float *core2,
-                             hipStream_t st) {
+                             float scale, hipStream_t st) {
+    if (scale > 0.0f && dp != 32 && dp != 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(order_pack_kernel, dim3(nblocks(npad, 256)), dim3(256), 0, st, n, d, dp, npad, order, xp,
-                       root_box, core_p, xc, nrm, core2);
+                       root_box, core_p, xc, nrm, core2, scale);
     return hipGetLastError();
 }
 
