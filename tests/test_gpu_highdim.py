@@ -142,3 +142,35 @@ def test_knn_sampled_thresholds_duplicates(monkeypatch):
     X = np.vstack([X, X[rng.choice(3000, 900, replace=False)], X[:40], X[:40]])
     X = X[rng.permutation(X.shape[0])]
     _check(X, 5)
+
+
+@pytest.mark.parametrize("scale", [1e-12, 1e-3, 1e6, 1e15])
+def test_split_products_any_scale(scale):
+    """The FP16 split scales the centred rows by a power of two chosen from
+    the root box; results equal the oracle's whatever the data's scale."""
+    X = (workloads.gaussian_highdim(4000, 40, n_clusters=12, seed=31) * scale).astype(np.float32)
+    _check(X, 8)
+
+
+def test_split_products_mixed_magnitudes():
+    # dimensions 1e8 apart in scale and a large offset: the small ones fall
+    # to FP16 subnormals after scaling, which the margin covers
+    X = workloads.gaussian_highdim(4000, 33, n_clusters=12, seed=32).astype(np.float64)
+    X[:, :10] *= 1e-5
+    X[:, 10:20] *= 1e3
+    X += 1e4
+    _check(X.astype(np.float32), 7)
+
+
+@pytest.mark.parametrize("split", ["1", "0"])
+def test_split_and_fp32_products_agree_40k(monkeypatch, split):
+    """FP16 split products (default) and FP32 products
+    (FH_MFMA_SPLIT=0) select candidates differently but give the same
+    exact MST."""
+    X = workloads.gaussian_highdim(40000, 64, n_clusters=50, seed=33)
+    monkeypatch.setenv("FH_MFMA_SPLIT", "0")
+    ref = mst_arrays(X, 10)
+    monkeypatch.setenv("FH_MFMA_SPLIT", split)
+    got = mst_arrays(X, 10)
+    for u, v in zip(got, ref):
+        np.testing.assert_array_equal(u, v)
