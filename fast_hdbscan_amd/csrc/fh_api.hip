@@ -101,6 +101,9 @@ struct fh_ctx {
     DevBuf<uint32_t> bx_okeys, bx_okeys_s;
     DevBuf<unsigned char> bx_otemp;
     DevBuf<unsigned long long> bx_scored, bx_rowkey;
+    // tiled exact re-scan: per listed row and work-group, the best weight and position
+    DevBuf<float> bx_slotw;
+    DevBuf<int> bx_slotj;
     int brute_min_dim = 17;  // d from which the MFMA scans replace the tree searches
     // learned search radii
     DevBuf<float> ml_feat, ml_w, ml_g, ml_m, ml_v, ml_mu, ml_sd, ml_scratch, ml_tgt, ml_pred, ml_bound, ml_stats;
@@ -159,7 +162,7 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->sw_pos, c->sw_ties, c->sw_d, c->nb_d,
         c->mincore,
         c->cert, c->seed_int, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
-        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->bx_rowkey, c->bx_nrm2, c->bx_core2, c->bx_tmin, c->bx_tminc, c->bx_tmaxc, c->bx_tU, c->bx_comp2, c->bx_order, c->bx_ovals, c->bx_okeys, c->bx_okeys_s, c->bx_otemp, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
+        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->bx_rowkey, c->bx_nrm2, c->bx_core2, c->bx_tmin, c->bx_tminc, c->bx_tmaxc, c->bx_tU, c->bx_comp2, c->bx_order, c->bx_ovals, c->bx_okeys, c->bx_okeys_s, c->bx_otemp, c->bx_slotw, c->bx_slotj, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
         c->ml_v, c->ml_mu, c->ml_sd, c->ml_scratch, c->ml_tgt, c->ml_pred, c->ml_bound, c->ml_stats,
         c->ml_rows, c->ml_miss, c->kr_keys, c->kr_keys_s, c->kr_vals, c->kr_vals_s, c->kr_src, c->kr_dst,
         c->kr_w, c->kr_core, c->sk1, c->sk1s, c->sk2, c->sk2s, c->sv1, c->sv1s,
@@ -451,6 +454,7 @@ bool use_brute(const fh_ctx *ctx, int64_t n, int d, int kq, bool weighted) {
 }
 
 constexpr int kSampleOrder = 8;     // order statistic of the sampled thresholds
+constexpr int kRescanBatch = 4096;  // listed rows per tiled exact re-scan launch
 constexpr int kSampleRetry = 2048;  // flagged rows above which the kNN runs again unsampled
 
 int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_knn, fh::BruteScanArgs &bsa,
@@ -982,6 +986,10 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     be.d = d;
     be.levels = levels;
     int64_t exact_rows = 0;
+    // exact re-scan of the listed rows: tiled (64 rows per pass over the
+    // positions), or one work-group per row (FH_TILED_RESCAN=0)
+    bool tiled_rescan = true;
+    if (const char *m = std::getenv("FH_TILED_RESCAN")) tiled_rescan = std::atoi(m) != 0;
     // rows in component order with tile-pair bounds in the MFMA rounds
     // (symmetric scans only; FH_TILE_BOUNDS=0 keeps tree order)
     bool tile_bounds = brute && bsa.row_key != nullptr;
@@ -1056,7 +1064,24 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
             FH_HIP(fh::launch_boruvka_upper(be, st));
             FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));
             FH_HIP(fh::launch_boruvka_rows(be, st));
-            FH_HIP(fh::launch_boruvka_row_scan(be, st));
+            if (tiled_rescan) {
+                // the listed rows 64 at a time against every position, in
+                // batches of kRescanBatch rows (their slots reused)
+                FH_HIP(hipMemcpyAsync(ctx->h_small + 4, be.row_count, sizeof(int), hipMemcpyDeviceToHost, st));
+                FH_HIP(hipStreamSynchronize(st));
+                const int R = ctx->h_small[4];
+                const int RB = std::min(R, kRescanBatch);
+                if (RB > 0) {
+                    const size_t ns = (size_t)fh::boruvka_rows_tiled_slots(n, RB);
+                    FH_TRY(ensure(ctx, ctx->bx_slotw, ns));
+                    FH_TRY(ensure(ctx, ctx->bx_slotj, ns));
+                }
+                for (int r0 = 0; r0 < R; r0 += RB)
+                    FH_HIP(fh::launch_boruvka_rows_tiled(be, r0, std::min(RB, R - r0), ctx->bx_slotw.p,
+                                                         ctx->bx_slotj.p, st));
+            } else {
+                FH_HIP(fh::launch_boruvka_row_scan(be, st));
+            }
         } else if (pruned) {
             // bounds from kNN cross edges, then a sampled search, then
             // certification of whole subtrees, then everyone else

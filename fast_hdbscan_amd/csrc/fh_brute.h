@@ -178,5 +178,11 @@ struct BoruvkaExactArgs {
 hipError_t launch_boruvka_upper(const BoruvkaExactArgs &a, hipStream_t st);
 hipError_t launch_boruvka_rows(const BoruvkaExactArgs &a, hipStream_t st);
 hipError_t launch_boruvka_row_scan(const BoruvkaExactArgs &a, hipStream_t st);
+// The same re-scan of rows [row0, row0 + nrows) of the list, tiled (64 rows
+// against 2,048 positions per work-group); slot_w / slot_j hold
+// boruvka_rows_tiled_slots(n, nrows) entries.
+int64_t boruvka_rows_tiled_slots(int64_t n, int max_rows);
+hipError_t launch_boruvka_rows_tiled(const BoruvkaExactArgs &a, int row0, int nrows, float *slot_w, int *slot_j,
+                                     hipStream_t st);
 
 }  // namespace fh

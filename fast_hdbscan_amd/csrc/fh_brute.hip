@@ -65,6 +65,12 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 
 inline int64_t nblocks(int64_t n, int b) { return (n + b - 1) / b; }
 
+#define FH_BRUTE_CHECK(x)                    \
+    do {                                     \
+        const hipError_t e_ = (x);           \
+        if (e_ != hipSuccess) return e_;     \
+    } while (0)
+
 }  // namespace
 
 int brute_padded_dim(int d) {
@@ -839,6 +845,145 @@ __global__ __launch_bounds__(256) void boruvka_row_scan_kernel(BoruvkaExactArgs 
     }
 }
 
+// The same re-scan, tiled: a work-group takes 64 listed rows (coordinates
+// in LDS) against 2,048 positions (8 x 256, each thread's coordinates in
+// registers, zero-padded to DP: the padded terms add +0 to rdist's sum), so
+// a position is read once per 64 rows instead of once per row.  Per row the
+// work-group's best (weight, position) under the same conditions and tie
+// order goes to slot [row][blockIdx.x]; boruvka_slots_kernel reduces the
+// slots.
+template <int DP>
+__global__ __launch_bounds__(256) void boruvka_rows_tiled_kernel(BoruvkaExactArgs a, int row0, int nrows, int ncb,
+                                                                 float *slot_w, int *slot_j) {
+    __shared__ float rs[64 * DP];
+    __shared__ float rci[64], rB[64];
+    __shared__ int rcomp[64];
+    __shared__ float bw_s[4][64];
+    __shared__ int bj_s[4][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g0 = blockIdx.y * 64;
+    if (g0 >= nrows) return;  // uniform over the work-group
+    const int nr = nrows - g0 < 64 ? nrows - g0 : 64;
+    const int d = a.d;
+    for (int i = tid; i < 64 * DP; i += 256) {
+        const int r = i / DP, k = i % DP;
+        float v = 0.0f;
+        if (r < nr && k < d) v = a.xp[(int64_t)a.rows[row0 + g0 + r] * d + k];
+        rs[i] = v;
+    }
+    if (tid < 64) {
+        if (tid < nr) {
+            const int p = a.rows[row0 + g0 + tid];
+            const int c = a.comp_p[p];
+            rcomp[tid] = c;
+            rci[tid] = a.core_p[p];
+            rB[tid] = a.bound[c];
+        }
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            bw_s[w][tid] = kInfB;
+            bj_s[w][tid] = -1;
+        }
+    }
+    __syncthreads();
+    const int64_t cbase = (int64_t)blockIdx.x * 2048;
+    for (int sc = 0; sc < 8; ++sc) {
+        const int64_t j = cbase + sc * 256 + tid;
+        const bool ok = j < a.n;
+        float y[DP];
+#pragma unroll
+        for (int k = 0; k < DP; ++k) y[k] = (ok && k < d) ? a.xp[j * d + k] : 0.0f;
+        const float cj = ok ? a.core_p[j] : kInfB;
+        const int kj = ok ? a.comp_p[j] : -1;
+        for (int r = 0; r < nr; ++r) {
+            float w = kInfB;
+            const float B = rB[r];
+            if (ok && kj != rcomp[r] && !(cj > B)) {
+                const float *x = rs + r * DP;
+                float acc = 0.0f;
+#pragma unroll
+                for (int k = 0; k < DP; ++k) {
+                    const float df = x[k] - y[k];
+                    acc = __builtin_fmaf(df, df, acc);
+                }
+                const float ci = rci[r];
+                acc = acc > ci ? acc : ci;
+                acc = acc > cj ? acc : cj;
+                if (!(acc > B)) w = acc;
+            }
+            float wm = w;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const float ow = __shfl_xor(wm, o, 64);
+                wm = ow < wm ? ow : wm;
+            }
+            if (!(wm < kInfB)) continue;  // uniform: no candidate in this wave
+            uint64_t ball = __builtin_amdgcn_ballot_w64(w == wm);
+            int jb = __shfl((int)j, __builtin_ctzll(ball), 64);
+            if (lane == 0) {
+                // equal weights: the first in the row's depth-first order
+                ball &= ball - 1;
+                while (ball) {
+                    const int l = __builtin_ctzll(ball);
+                    ball &= ball - 1;
+                    const int jl = (int)(cbase + sc * 256 + wave * 64 + l);
+                    if (dfs_before(rs + r * DP, jl, jb, a.range, a.box, a.levels, d)) jb = jl;
+                }
+                const float ow = bw_s[wave][r];
+                const int oj = bj_s[wave][r];
+                if (wm < ow || oj < 0 || (wm == ow && dfs_before(rs + r * DP, jb, oj, a.range, a.box, a.levels, d))) {
+                    bw_s[wave][r] = wm;
+                    bj_s[wave][r] = jb;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < nr) {
+        float bw = bw_s[0][tid];
+        int bj = bj_s[0][tid];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            const float ow = bw_s[w][tid];
+            const int oj = bj_s[w][tid];
+            if (oj < 0) continue;
+            if (bj < 0 || ow < bw || (ow == bw && dfs_before(rs + tid * DP, oj, bj, a.range, a.box, a.levels, d))) {
+                bw = ow;
+                bj = oj;
+            }
+        }
+        slot_w[(int64_t)(g0 + tid) * ncb + blockIdx.x] = bw;
+        slot_j[(int64_t)(g0 + tid) * ncb + blockIdx.x] = bj;
+    }
+}
+
+// Per listed row: the best of its slots (depth-first order on equal
+// weights), then the component's packed minimum and the row's partner, as
+// boruvka_row_scan_kernel writes them.
+__global__ void boruvka_slots_kernel(BoruvkaExactArgs a, int row0, int nrows, int ncb, const float *slot_w,
+                                     const int *slot_j) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    const int p = a.rows[row0 + i];
+    const float *x = a.xp + (int64_t)p * a.d;
+    float bw = kInfB;
+    int bj = -1;
+    for (int b = 0; b < ncb; ++b) {
+        const int oj = slot_j[(int64_t)i * ncb + b];
+        if (oj < 0) continue;
+        const float ow = slot_w[(int64_t)i * ncb + b];
+        if (bj < 0 || ow < bw || (ow == bw && dfs_before(x, oj, bj, a.range, a.box, a.levels, a.d))) {
+            bw = ow;
+            bj = oj;
+        }
+    }
+    if (bj >= 0) {
+        const unsigned long long key = ((unsigned long long)__float_as_uint(bw) << 32) | (unsigned int)a.perm[p];
+        atomicMin(a.best + a.comp_p[p], key);
+        a.partner[p] = bj;
+    }
+}
+
 // ------------------------------------------------------------ launches ----
 
 hipError_t launch_center_pack(int64_t n, int d, int dp, int64_t npad, const float *xp, const float *root_box,
@@ -1138,6 +1283,27 @@ hipError_t launch_boruvka_rows(const BoruvkaExactArgs &a, hipStream_t st) {
 
 hipError_t launch_boruvka_row_scan(const BoruvkaExactArgs &a, hipStream_t st) {
     hipLaunchKernelGGL(boruvka_row_scan_kernel, dim3(2048), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+int64_t boruvka_rows_tiled_slots(int64_t n, int max_rows) { return (int64_t)max_rows * ((n + 2047) / 2048); }
+
+hipError_t launch_boruvka_rows_tiled(const BoruvkaExactArgs &a, int row0, int nrows, float *slot_w, int *slot_j,
+                                     hipStream_t st) {
+    if (nrows <= 0) return hipSuccess;
+    const int ncb = (int)((a.n + 2047) / 2048);
+    const dim3 g((unsigned)ncb, (unsigned)((nrows + 63) / 64)), b(256);
+    switch (brute_padded_dim(a.d)) {
+        case 16: hipLaunchKernelGGL(boruvka_rows_tiled_kernel<16>, g, b, 0, st, a, row0, nrows, ncb, slot_w, slot_j); break;
+        case 24: hipLaunchKernelGGL(boruvka_rows_tiled_kernel<24>, g, b, 0, st, a, row0, nrows, ncb, slot_w, slot_j); break;
+        case 32: hipLaunchKernelGGL(boruvka_rows_tiled_kernel<32>, g, b, 0, st, a, row0, nrows, ncb, slot_w, slot_j); break;
+        case 48: hipLaunchKernelGGL(boruvka_rows_tiled_kernel<48>, g, b, 0, st, a, row0, nrows, ncb, slot_w, slot_j); break;
+        case 64: hipLaunchKernelGGL(boruvka_rows_tiled_kernel<64>, g, b, 0, st, a, row0, nrows, ncb, slot_w, slot_j); break;
+        default: return hipErrorInvalidValue;
+    }
+    FH_BRUTE_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(boruvka_slots_kernel, dim3((unsigned)((nrows + 63) / 64)), dim3(64), 0, st, a, row0, nrows, ncb,
+                       slot_w, slot_j);
     return hipGetLastError();
 }
 

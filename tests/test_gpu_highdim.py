@@ -174,3 +174,17 @@ def test_split_and_fp32_products_agree_40k(monkeypatch, split):
     got = mst_arrays(X, 10)
     for u, v in zip(got, ref):
         np.testing.assert_array_equal(u, v)
+
+
+def test_tiled_exact_rescan_agrees_40k(monkeypatch):
+    """The exact re-scan of the rows near their component's bound, tiled
+    (64 rows against 2,048 positions per work-group, the default) or one
+    work-group per row (FH_TILED_RESCAN=0): the same MST."""
+    X = workloads.gaussian_highdim(40000, 48, n_clusters=40, seed=34)
+    st = {}
+    a = mst_arrays(X, 10, stats=st)
+    monkeypatch.setenv("FH_TILED_RESCAN", "0")
+    b = mst_arrays(X, 10)
+    assert st["exact_rows"] > 0
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
