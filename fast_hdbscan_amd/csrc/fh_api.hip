@@ -472,8 +472,8 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
     if (const char *m = std::getenv("FH_MFMA_SPLIT")) split = std::atoi(m) != 0;
     float pack_scale = 0.0f, dot_scale = -2.0f;
     if (split) {
-        std::vector<float> hb(2 * (size_t)d);
-        FH_HIP(hipMemcpyAsync(hb.data(), ctx->box.p, sizeof(float) * 2 * d, hipMemcpyDeviceToHost, st));
+        float hb[2 * FH_MAX_DIM];  // the root box (d <= 64 on this path)
+        FH_HIP(hipMemcpyAsync(hb, ctx->box.p, sizeof(float) * 2 * d, hipMemcpyDeviceToHost, st));
         FH_HIP(hipStreamSynchronize(st));
         float half_ext = 0.0f;
         for (int k = 0; k < d; ++k) half_ext = std::max(half_ext, 0.5f * std::fabs(hb[d + k] - hb[k]));
