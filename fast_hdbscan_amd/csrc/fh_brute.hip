@@ -296,6 +296,16 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                 pk = col < n ? a.comp_p[col] : -3;
             }
             if constexpr (MODE == 0) pt = (sym && col < n) ? a.thr[col] : -kInfB;
+            if constexpr (MODE == 1) {
+                // symmetric: the column's best so far (any earlier value
+                // of its packed key; one ulp above, so that equal values
+                // still reach the packed minimum, which orders them)
+                pt = -kInfB;
+                if (sym && col < n) {
+                    const unsigned long long key = a.row_key[col];
+                    pt = key == ~0ull ? kInfB : __uint_as_float((unsigned int)(key >> 32) + 1u);
+                }
+            }
         }
     };
     auto lstore = [&](int buf) {
@@ -433,42 +443,59 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
                 }
             } else {
                 // padding rows and columns give +inf values (their norms),
-                // pairs inside one component +inf by selection; the minima
-                // by compare and select, first index kept on equal values
+                // pairs inside one component +inf by selection.  A value
+                // matters only below the row's best so far or the
+                // column's (symmetric: seeded from its packed key), so the
+                // minima are updated only when some lane of the wave has
+                // such a value; then by compare and select, first index
+                // kept on equal values
                 const float cc = col_core[cur][cl];
                 const int ck = col_comp[cur][cl];
-                float cv = kInfB;  // this column's minimum over the lane's rows
-                int ca = -1;
+                const float cv0 = col_thr[cur][cl];
+                float mm[2][4];
+                bool hit = false;
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const float approx = __builtin_fmaf(c2, acc[rb][cb][r], rn[rb][r] + cn);
                         const int rl = wave * 32 + rb * 16 + 4 * g + r;
-                        const float rc = row_core_s[rl];
-                        const int rk = row_comp_s[rl];
                         // no value here is NaN: fminf / fmaxf (single
                         // instructions) equal the compare-and-select forms
                         tmv = __builtin_fminf(approx, tmv);
-                        float m = __builtin_fmaxf(__builtin_fmaxf(rc, cc), approx);
-                        m = ck != rk ? m : kInfB;
-                        const bool lt = m < rv[rb][r];
-                        rv[rb][r] = lt ? m : rv[rb][r];
-                        ri[rb][r] = lt ? (int)col : ri[rb][r];
-                        // rows in increasing order: the first equal one is kept
-                        const bool lc = m < cv;
-                        cv = lc ? m : cv;
-                        ca = lc ? (int)(rbase + rb * 16 + 4 * g + r) : ca;
+                        float m = __builtin_fmaxf(__builtin_fmaxf(row_core_s[rl], cc), approx);
+                        m = ck != row_comp_s[rl] ? m : kInfB;
+                        mm[rb][r] = m;
+                        hit = hit || m < __builtin_fmaxf(rv[rb][r], cv0);
                     }
-                if (sym) {
-                    // the 4 lanes holding this column's other rows of the wave
+                float cv = cv0;  // this column's minimum over the lane's rows (below cv0)
+                int ca = -1;
+                if (__builtin_amdgcn_ballot_w64(hit) != 0) {
 #pragma unroll
-                    for (int o = 16; o < 64; o <<= 1) {
-                        const float ov = __shfl_xor(cv, o, 64);
-                        const int oa = __shfl_xor(ca, o, 64);
-                        if (ov < cv || (ov == cv && (unsigned)oa < (unsigned)ca)) {
-                            cv = ov;
-                            ca = oa;
+                    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const float m = mm[rb][r];
+                            const bool lt = m < rv[rb][r];
+                            rv[rb][r] = lt ? m : rv[rb][r];
+                            ri[rb][r] = lt ? (int)col : ri[rb][r];
+                            // rows in increasing order: the first equal one is kept
+                            const bool lc = m < cv;
+                            cv = lc ? m : cv;
+                            ca = lc ? (int)(rbase + rb * 16 + 4 * g + r) : ca;
+                        }
+                }
+                if (sym) {
+                    if (__builtin_amdgcn_ballot_w64(ca >= 0) != 0) {
+                        // the 4 lanes holding this column's other rows of the wave
+#pragma unroll
+                        for (int o = 16; o < 64; o <<= 1) {
+                            const float ov = __shfl_xor(cv, o, 64);
+                            const int oa = __shfl_xor(ca, o, 64);
+                            if (oa >= 0 && (ca < 0 || ov < cv || (ov == cv && (unsigned)oa < (unsigned)ca))) {
+                                cv = ov;
+                                ca = oa;
+                            }
                         }
                     }
                     if (g == 0) {
