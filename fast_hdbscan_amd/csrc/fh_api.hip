@@ -606,7 +606,12 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
         wa.out_i = ctx->knn_i.p;
         wa.core_o = ctx->core_o.p;
         wa.core_p = ctx->core_p.p;
-        FH_HIP(fh::launch_knn_wave(wa, nflag, st));
+        // a work-group per row (FH_KNN_TREE_FORM=0: a wavefront walking the tree)
+        const char *form = std::getenv("FH_KNN_TREE_FORM");
+        if (form && std::atoi(form) == 0)
+            FH_HIP(fh::launch_knn_wave(wa, nflag, st));
+        else
+            FH_HIP(fh::launch_knn_block(wa, nflag, n, st));
     }
     return FH_OK;
 }

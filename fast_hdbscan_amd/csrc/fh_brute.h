@@ -154,6 +154,10 @@ struct KnnWaveArgs {
     float *core_p;
 };
 hipError_t launch_knn_wave(const KnnWaveArgs &a, int count, hipStream_t st);
+// The same results, a work-group per listed position (the depth-first leaf
+// order precomputed, leaves taken several at a time); trees of more than
+// 19 levels or leaves above 256 points go to launch_knn_wave.
+hipError_t launch_knn_block(const KnnWaveArgs &a, int count, int64_t n, hipStream_t st);
 
 struct BoruvkaExactArgs {
     const float *xp;

@@ -766,6 +766,99 @@ __global__ __launch_bounds__(64) void knn_wave_kernel(KnnWaveArgs a) {
     }
 }
 
+// The same query, one work-group per row.  Pruning never changes which
+// points the reference's heap receives: every point of a pruned subtree has
+// rdist >= box_lb > top (both are FMA chains of the same shape over
+// per-dimension gaps, the box's no larger, and rounding is monotone), and
+// the top only falls.  So the heap receives, in the query's depth-first
+// leaf order (closer child first, left on equal bounds) and in position
+// order within a leaf, every point whose value is below its top at that
+// moment.  The work-group first sets every internal node's child order (a
+// bit in LDS), then takes the leaves in that order, 256 / leaf_cap at a
+// time (leaf sizes differ by at most one, so leaf_cap = ceil(n / leaves)
+// holds any leaf): a thread per point computes rdist, and one wavefront
+// pushes the values in order (a ballot against the top, then lane 0
+// re-checks and pushes each candidate in turn).
+constexpr int kKnnBlockLevels = 19;  // child-order bits for up to 2^18 - 1 internal nodes
+
+__global__ __launch_bounds__(256) void knn_block_kernel(KnnWaveArgs a, int leaf_cap) {
+    __shared__ float xs[64];
+    __shared__ float hp[64];
+    __shared__ int hi[64];
+    __shared__ float pv[256];
+    __shared__ int pi[256];
+    __shared__ unsigned int rfb[(1 << (kKnnBlockLevels - 1)) / 32];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int pos = a.query_pos[blockIdx.x];
+    const int d = a.d, k = a.k, L = a.levels;
+    if (tid < 64) xs[tid] = tid < d ? a.xp[(int64_t)pos * d + tid] : 0.0f;
+    if (tid < k) {
+        hp[tid] = kInfB;
+        hi[tid] = -1;
+    }
+    const int ninternal = (1 << (L - 1)) - 1;
+    for (int w = tid; w < (ninternal + 31) / 32; w += 256) rfb[w] = 0u;
+    __syncthreads();
+    const size_t stride = (size_t)2 * d;
+    for (int node = tid; node < ninternal; node += 256) {
+        const int l = 2 * node + 1;
+        const float lbl = box_lb<0>(xs, a.box + (size_t)l * stride, d);
+        const float lbr = box_lb<0>(xs, a.box + (size_t)(l + 1) * stride, d);
+        if (!(lbl <= lbr)) atomicOr(&rfb[node >> 5], 1u << (node & 31));
+    }
+    __syncthreads();
+    const int nleaves = 1 << (L - 1);
+    const int lps = 256 / leaf_cap;  // leaves per step
+    const int li = tid / leaf_cap, lo = tid % leaf_cap;
+    LdsHeap h{hp, hi, 1, k};
+    for (int r0 = 0; r0 < nleaves; r0 += lps) {
+        // this thread's point: offset lo of the leaf of depth-first rank r0 + li
+        float dist = kInfB;
+        int pidx = -1;
+        const int r = r0 + li;
+        if (li < lps && r < nleaves) {
+            int node = 0;
+            for (int t = L - 2; t >= 0; --t) {
+                const int rf = (rfb[node >> 5] >> (node & 31)) & 1;
+                node = 2 * node + 1 + (((r >> t) & 1) ^ rf);
+            }
+            const int2 rg = a.range[node];
+            if (rg.x + lo < rg.y) {
+                pidx = rg.x + lo;
+                dist = rdist<0>(xs, a.xp + (int64_t)pidx * d, d);
+            }
+        }
+        pv[tid] = dist;
+        pi[tid] = pidx;
+        __syncthreads();
+        if (tid < 64) {
+            for (int c = 0; c < 256; c += 64) {
+                uint64_t ball = __builtin_amdgcn_ballot_w64(pi[c + lane] >= 0 && pv[c + lane] < hp[0]);
+                while (ball) {
+                    const int l = __builtin_ctzll(ball);
+                    ball &= ball - 1;
+                    if (lane == 0) {
+                        const float dv = pv[c + l];
+                        if (dv < hp[0]) h.push(dv, pi[c + l]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        h.sort();
+        const int64_t row = a.perm[pos];
+        for (int j = 0; j < k; ++j) {
+            a.out_d[row * k + j] = hp[j];
+            a.out_i[row * k + j] = hi[j] >= 0 ? a.perm[hi[j]] : -1;
+        }
+        const float c = a.core_from_knn ? hp[k - 1] : 0.0f;
+        a.core_o[row] = c;
+        a.core_p[pos] = c;
+    }
+}
+
 // ------------------------------------------------ exact Boruvka steps ----
 
 __global__ void boruvka_upper_kernel(BoruvkaExactArgs a) {
@@ -1295,6 +1388,16 @@ hipError_t launch_knn_wave(const KnnWaveArgs &a, int count, hipStream_t st) {
     if (a.k > 64 || a.d > 64) return hipErrorInvalidValue;
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(knn_wave_kernel, dim3(count), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_block(const KnnWaveArgs &a, int count, int64_t n, hipStream_t st) {
+    if (a.k > 64 || a.d > 64) return hipErrorInvalidValue;
+    if (count <= 0) return hipSuccess;
+    const int64_t nleaves = int64_t(1) << (a.levels - 1);
+    const int64_t leaf_cap = (n + nleaves - 1) / nleaves;
+    if (a.levels < 1 || a.levels > kKnnBlockLevels || leaf_cap > 256) return launch_knn_wave(a, count, st);
+    hipLaunchKernelGGL(knn_block_kernel, dim3(count), dim3(256), 0, st, a, (int)(leaf_cap < 1 ? 1 : leaf_cap));
     return hipGetLastError();
 }
 

@@ -188,3 +188,21 @@ def test_tiled_exact_rescan_agrees_40k(monkeypatch):
     assert st["exact_rows"] > 0
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+def test_tree_knn_forms_agree_duplicates_30k(monkeypatch):
+    """Rows whose k + 1 nearest include ties take the reference's tree
+    query: a work-group per row over the precomputed depth-first leaf order
+    (default) or a wavefront walking the tree (FH_KNN_TREE_FORM=0) give the
+    same neighbours, cores and MST."""
+    rng = np.random.default_rng(35)
+    X = workloads.gaussian_highdim(24000, 40, n_clusters=30, seed=35)
+    X = np.vstack([X, X[rng.choice(24000, 6000, replace=False)]])
+    X = X[rng.permutation(X.shape[0])]
+    st = {}
+    a = mst_arrays(X, 8, stats=st)
+    assert st["knn_tree_rows"] > 100
+    monkeypatch.setenv("FH_KNN_TREE_FORM", "0")
+    b = mst_arrays(X, 8)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
