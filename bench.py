@@ -36,8 +36,8 @@ HBM_PEAK_GBS = 8000.0
 # copy) on this workload; command lines and raw counts in
 # profiles/pmc_2Mx2D_traffic.md.
 PMC_TRAFFIC_BYTES = {
-    "boruvka_wave_kernel<2, 2>": 85.12e6,
-    "knn_kernel<2, false>": 258.35e6,
+    "boruvka_wave_kernel<2, 2>": 77.62e6,
+    "knn_kernel<2, false>": 260.93e6,
 }
 
 
