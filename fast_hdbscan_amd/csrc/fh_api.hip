@@ -104,7 +104,7 @@ struct fh_ctx {
     // tiled exact re-scan: per listed row and work-group, the best weight and position
     DevBuf<float> bx_slotw;
     DevBuf<int> bx_slotj;
-    int brute_min_dim = 17;  // d from which the MFMA scans replace the tree searches
+    int brute_min_dim = 9;   // d from which the MFMA scans replace the tree searches (use_brute)
     // learned search radii
     DevBuf<float> ml_feat, ml_w, ml_g, ml_m, ml_v, ml_mu, ml_sd, ml_scratch, ml_tgt, ml_pred, ml_bound, ml_stats;
     DevBuf<int> ml_rows, ml_miss;
@@ -436,21 +436,31 @@ int64_t weighted_k(const float *w, int64_t n, int ms) {
 // kNN of every point by the MFMA candidate scan (fh_brute.hip) with exact
 // re-scoring; rows that need the reference's heap order for ties go to the
 // tree search.  Leaves the packed points and norms in ctx for the rounds.
-// Whether the MFMA candidate scans replace the tree searches: from
-// brute_min_dim dimensions on, and for small inputs from 5 dimensions on,
-// where a tree search visits a large share of the leaves anyway and the
-// per-thread traversal of a few thousand points leaves most of the chip idle
-// (measured crossover n ~ 3000 d).  FH_HIGH_DIM_FROM, read per call,
-// replaces both rules with one switch-over dimension.  Both paths give the
-// same result.
+// Whether the MFMA candidate scans replace the tree searches (both give the
+// same result).  Measured on one MI355X (Gaussian mixtures, min_samples
+// 10): the tree kernels are compiled per dimension up to 8 and degrade
+// quickly above, while the scans cost the same for any d <= 32 (1.31 s at
+// 1M points, 5 ms at 20k).  So:
+//   - d >= 16: the scans (16-D at 200k: 54 ms against 1,172 ms);
+//   - brute_min_dim (9) <= d < 16: the scans up to n = 4e5 * 2.5^(d - 9)
+//     (9-D: 76 against 148 ms at 200k, 1,312 against 878 ms at 1M; 10-D:
+//     about even at 1M; 12-D at 100k: 22 against 214 ms);
+//   - small inputs from 5 dimensions on (n <= 3000 d), where a tree
+//     search visits a large share of the leaves anyway and the per-thread
+//     traversal of a few thousand points leaves most of the chip idle.
+// FH_HIGH_DIM_FROM, read per call, replaces the rules with one switch-over
+// dimension.
 bool use_brute(const fh_ctx *ctx, int64_t n, int d, int kq, bool weighted) {
     int from = ctx->brute_min_dim;
     bool small = d >= 5 && n <= 3000 * (int64_t)d;
+    double limit = d >= 16 ? 1e300 : 4e5 * std::pow(2.5, d - from);
     if (const char *m = std::getenv("FH_HIGH_DIM_FROM")) {
         from = std::atoi(m);
         small = false;
+        limit = 1e300;
     }
-    return !weighted && fh::brute_padded_dim(d) > 0 && (d >= from || small) && kq <= 64 && n >= 256;
+    const bool large = d >= from && (double)n <= limit;
+    return !weighted && fh::brute_padded_dim(d) > 0 && (large || small) && kq <= 64 && n >= 256;
 }
 
 constexpr int kSampleOrder = 8;     // order statistic of the sampled thresholds

@@ -101,7 +101,13 @@ def plscan_sweep(data, min_samples_values, base_min_cluster_size=10, base_n_clus
     out = {}
     shared = n > 1 and all(2 <= m <= 126 for m in values)
     if shared:
-        msts = mst_sweep_device(Xd, values)
+        try:
+            msts = mst_sweep_device(Xd, values)
+        except NotImplementedError:
+            # the shared sweep runs the tree kNN; inputs that the MFMA scans
+            # take (high dimension, small inputs from 5 dimensions on) run
+            # value by value
+            shared = False
     with cf.ThreadPoolExecutor(max_workers=max(1, int(workers))) as pool:
         futures = {}
         for i, m in enumerate(values):

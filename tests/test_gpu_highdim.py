@@ -206,3 +206,18 @@ def test_tree_knn_forms_agree_duplicates_30k(monkeypatch):
     b = mst_arrays(X, 8)
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+@pytest.mark.parametrize("d,n", [(9, 60000), (12, 120000), (16, 250000)])
+def test_mid_dimension_scans_match_tree_path(monkeypatch, d, n):
+    """From 9 dimensions on the MFMA scans take inputs the tree path took
+    before (up to n = 4e5 * 2.5^(d - 9); every n from 16 on): same exact
+    MST as the tree path."""
+    X = workloads.gaussian_highdim(n, d, n_clusters=max(3, n // 4000), seed=d + 40)
+    st = {}
+    a = mst_arrays(X, 10, stats=st)
+    assert st["high_dim"] == 1
+    monkeypatch.setenv("FH_HIGH_DIM_FROM", "1000")
+    b = mst_arrays(X, 10)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)

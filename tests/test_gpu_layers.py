@@ -59,6 +59,26 @@ def test_plscan_sweep_matches_one_at_a_time():
         np.testing.assert_array_equal(one[4], got[4])
 
 
+def test_plscan_sweep_high_dimension_runs_value_by_value():
+    """Inputs that the MFMA scans take (here 12-D) are not served by the
+    shared tree kNN of the sweep: plscan_sweep runs them value by value,
+    with the same layers as one call per value."""
+    from fast_hdbscan_amd import compute_minimum_spanning_tree
+    from fast_hdbscan_amd import layers as FL
+
+    X = workloads.gaussian_highdim(6000, 12, n_clusters=8, seed=41).astype(np.float32)
+    values = [3, 9]
+    sweep = FL.plscan_sweep(X, values, base_min_cluster_size=5, workers=2)
+    for m in values:
+        e, _, core = compute_minimum_spanning_tree(X, min_samples=m)
+        one = FL.layers_from_mst(e, core, X.shape[0], base_min_cluster_size=5)
+        got = sweep[m]
+        assert len(one[0]) == len(got[0])
+        for a, b in zip(one[0] + one[1], got[0] + got[1]):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(one[4], got[4])
+
+
 @pytest.mark.parametrize("case", ["mixture", "grid_ties", "duplicates"])
 def test_sweep_shared_knn_equals_single_mst(case):
     """fh_mst_sweep_device (one tree, one kNN of width max(min_samples)+2)
