@@ -154,18 +154,22 @@ __global__ void center_pack_kernel(int64_t n, int d, int dp, int64_t npad, const
 //         bounds the true k-th exact distance, plus a second margin bounds
 //         the approximate value of every true k nearest neighbour.
 template <int DP4, int MODE, bool SPLIT>
-__global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
+__global__ __launch_bounds__(256, MODE == 1 ? 2 : 3) void brute_scan_kernel(BruteScanArgs a) {
     constexpr int DP = 4 * DP4;
+    // column tiles in LDS: double-buffered in the round scan (one barrier
+    // per tile; its registers allow two wavefronts per SIMD anyway), single
+    // in the kNN scans (two barriers per tile, three work-groups per CU)
+    constexpr int NBUF = MODE == 1 ? 2 : 1;
     // SPLIT: FP16 hi / lo planes (split_slot), NC chunks of 32 dimensions
     constexpr int NC = SPLIT ? DP / 32 : 1;
     static_assert(!SPLIT || DP % 32 == 0, "split tiles hold whole 32-dimension chunks");
     constexpr int TILE = 128 * DP;  // floats per tile
     constexpr int NV = DP / 8;      // float4 loads per thread per tile
-    __shared__ float bs[2 * TILE > 128 * 64 ? 2 * TILE : 128 * 64];
-    __shared__ float col_nrm[2][128];
-    __shared__ float col_core[2][128];
-    __shared__ int col_comp[2][128];
-    __shared__ float col_thr[2][128];
+    __shared__ __align__(16) float bs[NBUF * TILE > 128 * 64 ? NBUF * TILE : 128 * 64];
+    __shared__ float col_nrm[NBUF][128];
+    __shared__ float col_core[NBUF][128];
+    __shared__ int col_comp[NBUF][128];
+    __shared__ float col_thr[NBUF][128];
     __shared__ int cnt_l[128];
     // MODE 1, symmetric: per wave and column, the column's minimum over the
     // wave's 32 rows (double-buffered like the tiles)
@@ -331,7 +335,7 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
     int t1 = t < ntiles ? next_tile(t + 1) : ntiles;
     if (t1 < ntiles) gload(t1);
     int scored = 0;
-    for (int cur = 0; t < ntiles; cur ^= 1) {
+    for (int cur = 0; t < ntiles; cur = NBUF == 2 ? cur ^ 1 : 0) {
         ++scored;
         const float *bt = bs + cur * TILE;
         floatx4 acc[2][8];
@@ -517,8 +521,9 @@ __global__ __launch_bounds__(256, 2) void brute_scan_kernel(BruteScanArgs a) {
         // iteration, before the previous barrier), prefetch the one after,
         // then one barrier
         const int t2 = t1 < ntiles ? next_tile(t1 + 1) : ntiles;
+        if constexpr (NBUF == 1) __syncthreads();  // every wave is done with the tile
         if (t1 < ntiles) {
-            lstore(cur ^ 1);
+            lstore(NBUF == 2 ? cur ^ 1 : 0);
             if (t2 < ntiles) gload(t2);
         }
         __syncthreads();
