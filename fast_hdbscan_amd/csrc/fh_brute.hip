@@ -464,10 +464,16 @@ __global__ __launch_bounds__(256, MODE == 1 ? 2 : 3) void brute_scan_kernel(Brut
                     for (int r = 0; r < 4; ++r) {
                         const float approx = __builtin_fmaf(c2, acc[rb][cb][r], rn[rb][r] + cn);
                         const int rl = wave * 32 + rb * 16 + 4 * g + r;
-                        // no value here is NaN: fminf / fmaxf (single
-                        // instructions) equal the compare-and-select forms
+                        // no value here is NaN; the minimum only bounds
+                        // values, so fminf (one instruction) serves
                         tmv = __builtin_fminf(approx, tmv);
-                        float m = __builtin_fmaxf(__builtin_fmaxf(row_core_s[rl], cc), approx);
+                        // max(core_i, core_j, approx) on the signed bit
+                        // patterns (one instruction): the cores are +0 or
+                        // positive, so a negative or -0 approx loses and the
+                        // result is never -0, whose sign bit would misorder
+                        // the packed (value bits, index) keys
+                        float m = __int_as_float(max(max(__float_as_int(row_core_s[rl]), __float_as_int(cc)),
+                                                     __float_as_int(approx)));
                         m = ck != row_comp_s[rl] ? m : kInfB;
                         mm[rb][r] = m;
                         hit = hit || m < __builtin_fmaxf(rv[rb][r], cv0);

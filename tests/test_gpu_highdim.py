@@ -221,3 +221,17 @@ def test_mid_dimension_scans_match_tree_path(monkeypatch, d, n):
     b = mst_arrays(X, 10)
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+@pytest.mark.parametrize("split", ["1", "0"])
+def test_zero_cores_from_duplicates(monkeypatch, split):
+    """Every point has min_samples + 2 exact copies: core distances are 0,
+    the mutual-reachability weights of copies are 0 and their approximate
+    values can come out as -0; the packed (value, index) minima must still
+    order them first (weights equal the oracle's, bit for bit)."""
+    monkeypatch.setenv("FH_MFMA_SPLIT", split)
+    rng = np.random.default_rng(42)
+    base = workloads.gaussian_highdim(600, 48, n_clusters=6, seed=42)
+    X = np.repeat(base, 7, axis=0)
+    X = X[rng.permutation(X.shape[0])]
+    _check(X, 5)
