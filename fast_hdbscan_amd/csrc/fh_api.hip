@@ -104,6 +104,7 @@ struct fh_ctx {
     // tiled exact re-scan: per listed row and work-group, the best weight and position
     DevBuf<float> bx_slotw;
     DevBuf<int> bx_slotj;
+    DevBuf<unsigned char> bx_cert;  // first round from the candidate lists: settled rows
     int brute_min_dim = 9;   // d from which the MFMA scans replace the tree searches (use_brute)
     // learned search radii
     DevBuf<float> ml_feat, ml_w, ml_g, ml_m, ml_v, ml_mu, ml_sd, ml_scratch, ml_tgt, ml_pred, ml_bound, ml_stats;
@@ -162,7 +163,7 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->sw_pos, c->sw_ties, c->sw_d, c->nb_d,
         c->mincore,
         c->cert, c->seed_int, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
-        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->bx_rowkey, c->bx_nrm2, c->bx_core2, c->bx_tmin, c->bx_tminc, c->bx_tmaxc, c->bx_tU, c->bx_comp2, c->bx_order, c->bx_ovals, c->bx_okeys, c->bx_okeys_s, c->bx_otemp, c->bx_slotw, c->bx_slotj, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
+        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->bx_rowkey, c->bx_nrm2, c->bx_core2, c->bx_tmin, c->bx_tminc, c->bx_tmaxc, c->bx_tU, c->bx_comp2, c->bx_order, c->bx_ovals, c->bx_okeys, c->bx_okeys_s, c->bx_otemp, c->bx_slotw, c->bx_slotj, c->bx_cert, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
         c->ml_v, c->ml_mu, c->ml_sd, c->ml_scratch, c->ml_tgt, c->ml_pred, c->ml_bound, c->ml_stats,
         c->ml_rows, c->ml_miss, c->kr_keys, c->kr_keys_s, c->kr_vals, c->kr_vals_s, c->kr_src, c->kr_dst,
         c->kr_w, c->kr_core, c->sk1, c->sk1s, c->sk2, c->sk2s, c->sv1, c->sv1s,
@@ -465,6 +466,7 @@ bool use_brute(const fh_ctx *ctx, int64_t n, int d, int kq, bool weighted) {
 
 constexpr int kSampleOrder = 8;     // order statistic of the sampled thresholds
 constexpr int kRescanBatch = 4096;  // listed rows per tiled exact re-scan launch
+constexpr int kCandRoundRows = 65536;  // first round from the candidate lists: most rows left for the exact re-scan
 constexpr int kSampleRetry = 2048;  // flagged rows above which the kNN runs again unsampled
 
 int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_knn, fh::BruteScanArgs &bsa,
@@ -1005,6 +1007,9 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     // positions), or one work-group per row (FH_TILED_RESCAN=0)
     bool tiled_rescan = true;
     if (const char *m = std::getenv("FH_TILED_RESCAN")) tiled_rescan = std::atoi(m) != 0;
+    // first round from the kNN candidate lists (FH_CAND_ROUND=0: the MFMA scan)
+    bool cand_round = brute;
+    if (const char *m = std::getenv("FH_CAND_ROUND")) cand_round = cand_round && std::atoi(m) != 0;
     // rows in component order with tile-pair bounds in the MFMA rounds
     // (symmetric scans only; FH_TILE_BOUNDS=0 keeps tree order)
     bool tile_bounds = brute && bsa.row_key != nullptr;
@@ -1048,37 +1053,80 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
                 FH_HIP(fh::launch_tile_core_range(n, npad, ctx->bx_core2.p, ctx->bx_tminc.p, ctx->bx_tmaxc.p, st));
                 FH_HIP(hipMemsetAsync(ctx->bx_tmin.p, 0, sizeof(float) * ntl * ntl, st));
             }
-            fh::BruteScanArgs rs = bsa;
-            if (order_rows) {
-                FH_HIP(fh::launch_gather_comp(n, npad, ctx->bx_order.p, ctx->comp_p.p, ctx->bx_comp2.p, st));
-                FH_HIP(fh::launch_tile_comp(n, npad, ctx->bx_comp2.p, ctx->bx_tilecomp.p, st));
-                rs.nrm = ctx->bx_nrm2.p;
-                rs.core_p = ctx->bx_core2.p;
-                rs.comp_p = ctx->bx_comp2.p;
-                if (rounds == kOrderRound) {
-                    rs.tmin_out = ctx->bx_tmin.p;
-                } else {
-                    FH_HIP(fh::launch_tile_bounds((int)ntl, ctx->bx_tilecomp.p, ctx->bx_tmin.p, ctx->bx_tmaxc.p,
-                                                  bsa.margin_c, bsa.nrm_max_bits, ctx->bound.p, st));
-                    FH_HIP(fh::launch_tile_umax(n, npad, ctx->bx_comp2.p, ctx->bound.p, ctx->bx_tU.p, st));
-                    rs.tmin_in = ctx->bx_tmin.p;
-                    rs.tminc = ctx->bx_tminc.p;
-                    rs.tU = ctx->bx_tU.p;
-                }
-            } else {
-                FH_HIP(fh::launch_tile_comp(n, npad, ctx->comp_p.p, ctx->bx_tilecomp.p, st));
+            // the first round from the kNN candidate lists: rows settled
+            // there enter their component's minimum at once, the others
+            // that can reach its bound are listed for the exact re-scan;
+            // with more than kCandRoundRows of those, the MFMA scan lists
+            // the rows instead (the settled rows' entries stand)
+            bool from_cands = false;
+            if (rounds == 0 && cand_round) {
+                FH_TRY(ensure(ctx, ctx->bx_cert, n));
+                fh::CandRoundArgs cr{};
+                cr.xp = ctx->xp.p;
+                cr.range = ctx->range.p;
+                cr.box = ctx->box.p;
+                cr.levels = levels;
+                cr.d = d;
+                cr.n = n;
+                cr.perm = ctx->perm.p;
+                cr.core_p = ctx->core_p.p;
+                cr.comp_p = ctx->comp_p.p;
+                cr.cand = bsa.cand;
+                cr.cand_cnt = bsa.cand_cnt;
+                cr.cap = bsa.cap;
+                cr.thr = bsa.thr;
+                cr.nrm = bsa.nrm;
+                cr.nrm_max_bits = bsa.nrm_max_bits;
+                cr.margin_c = bsa.margin_c;
+                cr.bound = ctx->bound.p;
+                cr.best = ctx->best.p;
+                cr.partner = ctx->partner.p;
+                cr.cert = ctx->bx_cert.p;
+                cr.rows = be.rows;
+                cr.row_count = be.row_count;
+                FH_HIP(mark(ctx, 8));
+                FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));
+                FH_HIP(fh::launch_cand_round(cr, st));
+                FH_HIP(mark(ctx, 9));
+                FH_HIP(hipMemcpyAsync(ctx->h_small + 4, be.row_count, sizeof(int), hipMemcpyDeviceToHost, st));
+                FH_HIP(hipStreamSynchronize(st));
+                int cap_rows = kCandRoundRows;
+                if (const char *m = std::getenv("FH_CAND_ROUND_ROWS")) cap_rows = std::atoi(m);
+                from_cands = ctx->h_small[4] <= cap_rows;
             }
-            FH_HIP(mark(ctx, 8));
-            if (rs.row_key) FH_HIP(hipMemsetAsync(rs.row_key, 0xff, sizeof(unsigned long long) * n, st));
-            FH_HIP(fh::launch_scan_boruvka(rs, st));
-            if (rs.row_key && order_rows)
-                FH_HIP(fh::launch_unpack_row_keys_ordered(n, rs.row_key, ctx->bx_order.p, rs.row_min, rs.row_arg, st));
-            else if (rs.row_key)
-                FH_HIP(fh::launch_unpack_row_keys(n, rs.row_key, rs.row_min, rs.row_arg, st));
-            FH_HIP(mark(ctx, 9));
-            FH_HIP(fh::launch_boruvka_upper(be, st));
-            FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));
-            FH_HIP(fh::launch_boruvka_rows(be, st));
+            if (!from_cands) {
+                fh::BruteScanArgs rs = bsa;
+                if (order_rows) {
+                    FH_HIP(fh::launch_gather_comp(n, npad, ctx->bx_order.p, ctx->comp_p.p, ctx->bx_comp2.p, st));
+                    FH_HIP(fh::launch_tile_comp(n, npad, ctx->bx_comp2.p, ctx->bx_tilecomp.p, st));
+                    rs.nrm = ctx->bx_nrm2.p;
+                    rs.core_p = ctx->bx_core2.p;
+                    rs.comp_p = ctx->bx_comp2.p;
+                    if (rounds == kOrderRound) {
+                        rs.tmin_out = ctx->bx_tmin.p;
+                    } else {
+                        FH_HIP(fh::launch_tile_bounds((int)ntl, ctx->bx_tilecomp.p, ctx->bx_tmin.p, ctx->bx_tmaxc.p,
+                                                      bsa.margin_c, bsa.nrm_max_bits, ctx->bound.p, st));
+                        FH_HIP(fh::launch_tile_umax(n, npad, ctx->bx_comp2.p, ctx->bound.p, ctx->bx_tU.p, st));
+                        rs.tmin_in = ctx->bx_tmin.p;
+                        rs.tminc = ctx->bx_tminc.p;
+                        rs.tU = ctx->bx_tU.p;
+                    }
+                } else {
+                    FH_HIP(fh::launch_tile_comp(n, npad, ctx->comp_p.p, ctx->bx_tilecomp.p, st));
+                }
+                FH_HIP(mark(ctx, 8));
+                if (rs.row_key) FH_HIP(hipMemsetAsync(rs.row_key, 0xff, sizeof(unsigned long long) * n, st));
+                FH_HIP(fh::launch_scan_boruvka(rs, st));
+                if (rs.row_key && order_rows)
+                    FH_HIP(fh::launch_unpack_row_keys_ordered(n, rs.row_key, ctx->bx_order.p, rs.row_min, rs.row_arg, st));
+                else if (rs.row_key)
+                    FH_HIP(fh::launch_unpack_row_keys(n, rs.row_key, rs.row_min, rs.row_arg, st));
+                FH_HIP(mark(ctx, 9));
+                FH_HIP(fh::launch_boruvka_upper(be, st));
+                FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));
+                FH_HIP(fh::launch_boruvka_rows(be, st));
+            }
             if (tiled_rescan) {
                 // the listed rows 64 at a time against every position, in
                 // batches of kRescanBatch rows (their slots reused)

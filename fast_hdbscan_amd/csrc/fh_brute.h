@@ -182,6 +182,36 @@ struct BoruvkaExactArgs {
 hipError_t launch_boruvka_upper(const BoruvkaExactArgs &a, hipStream_t st);
 hipError_t launch_boruvka_rows(const BoruvkaExactArgs &a, hipStream_t st);
 hipError_t launch_boruvka_row_scan(const BoruvkaExactArgs &a, hipStream_t st);
+// First Boruvka round from the kNN candidate lists (all columns with
+// approximate distance <= thr[p] per row): rows whose best candidate in
+// another component weighs at most thr[p] - margin_p are settled exactly
+// (best, partner, cert[p] = 1); every candidate weight lowers bound; the
+// other rows whose lower bound reaches their component's bound are listed
+// in rows / row_count for the exact re-scan.
+struct CandRoundArgs {
+    const float *xp;
+    const int2 *range;
+    const float *box;
+    int levels, d;
+    int64_t n;
+    const int *perm;
+    const float *core_p;
+    const int *comp_p;
+    const int *cand;
+    const int *cand_cnt;
+    int cap;
+    const float *thr;
+    const float *nrm;
+    const unsigned int *nrm_max_bits;
+    float margin_c;
+    float *bound;
+    unsigned long long *best;
+    int *partner;
+    unsigned char *cert;
+    int *rows;
+    int *row_count;  // zero before the launch
+};
+hipError_t launch_cand_round(const CandRoundArgs &a, hipStream_t st);
 // The same re-scan of rows [row0, row0 + nrows) of the list, tiled (64 rows
 // against 2,048 positions per work-group); slot_w / slot_j hold
 // boruvka_rows_tiled_slots(n, nrows) entries.

@@ -1088,6 +1088,97 @@ __global__ __launch_bounds__(256) void boruvka_rows_tiled_kernel(BoruvkaExactArg
     }
 }
 
+// First Boruvka round from the kNN candidate lists.  Row p's list holds
+// every column whose approximate distance is at most thr[p] (unless it
+// overflowed), so every other column j has exact rdist(p, j) > L_p = thr[p]
+// - margin_p, and so does its mutual-reachability weight.  A wavefront per
+// row takes the exact weight of each candidate in another component (one
+// lane per candidate; the same operations as the exact re-scans) and keeps
+// the smallest, equal weights in the row's depth-first order.  When that
+// weight w is at most L_p, no other column reaches it: (w, partner) is the
+// row's exact answer over all columns, and the row enters its component's
+// packed minimum and writes its partner, as the exact re-scan would.  Any
+// candidate weight bounds its component's minimum from above (bound).
+// cand_round_list_kernel then lists the rows not settled so whose lower
+// bound L_p is at most their component's bound, and every row whose list
+// overflowed (it has no such bound), for the exact re-scan; the other
+// rows cannot reach the component's minimum.
+__global__ __launch_bounds__(256) void cand_round_kernel(CandRoundArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= a.n) return;  // uniform over the wavefront
+    const int comp = a.comp_p[p];
+    const float ci = a.core_p[p];
+    const float *x = a.xp + p * a.d;
+    const int cnt = a.cand_cnt[p];
+    const int m = cnt < a.cap ? cnt : a.cap;
+    float bw = kInfB;
+    int bj = -1;
+    for (int t0 = 0; t0 < m; t0 += 64) {
+        const int t = t0 + lane;
+        float w = kInfB;
+        int j = -1;
+        if (t < m) {
+            j = a.cand[p * a.cap + t];
+            if (a.comp_p[j] != comp) {
+                float r = rdist<0>(x, a.xp + (int64_t)j * a.d, a.d);
+                const float cj = a.core_p[j];
+                r = r > ci ? r : ci;
+                w = r > cj ? r : cj;
+            } else {
+                j = -1;
+            }
+        }
+        float wm = w;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ow = __shfl_xor(wm, o, 64);
+            wm = ow < wm ? ow : wm;
+        }
+        if (!(wm < kInfB) || wm > bw) continue;  // uniform
+        uint64_t ball = __builtin_amdgcn_ballot_w64(j >= 0 && w == wm);
+        int jb = __shfl(j, __builtin_ctzll(ball), 64);
+        if (lane == 0) {
+            ball &= ball - 1;
+            while (ball) {
+                const int l = __builtin_ctzll(ball);
+                ball &= ball - 1;
+                const int jl = a.cand[p * a.cap + t0 + l];
+                if (dfs_before(x, jl, jb, a.range, a.box, a.levels, a.d)) jb = jl;
+            }
+            if (wm < bw || bj < 0 || dfs_before(x, jb, bj, a.range, a.box, a.levels, a.d)) {
+                bw = wm;
+                bj = jb;
+            }
+        }
+        bw = __shfl(bw, 0, 64);
+        bj = __shfl(bj, 0, 64);
+    }
+    if (lane == 0) {
+        const float marg = a.margin_c * kU * (a.nrm[p] + __uint_as_float(*a.nrm_max_bits));
+        const float L = a.thr[p] - marg;
+        const bool settled = cnt <= a.cap && bj >= 0 && bw <= L;
+        a.cert[p] = settled ? 1 : 0;
+        if (bj >= 0) atomicMin(reinterpret_cast<unsigned int *>(a.bound + comp), __float_as_uint(bw));
+        if (settled) {
+            atomicMin(a.best + comp, ((unsigned long long)__float_as_uint(bw) << 32) | (unsigned int)a.perm[p]);
+            a.partner[p] = bj;
+        }
+    }
+}
+
+__global__ void cand_round_list_kernel(CandRoundArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n || a.cert[p]) return;
+    const float marg = a.margin_c * kU * (a.nrm[p] + __uint_as_float(*a.nrm_max_bits));
+    const float L = a.thr[p] - marg;
+    // an overflowed list lost columns under the threshold: no lower bound
+    if (a.cand_cnt[p] > a.cap || L <= a.bound[a.comp_p[p]]) {
+        const int slot = atomicAdd(a.row_count, 1);
+        a.rows[slot] = (int)p;
+    }
+}
+
 // Per listed row: the best of its slots (depth-first order on equal
 // weights), then the component's packed minimum and the row's partner, as
 // boruvka_row_scan_kernel writes them.
@@ -1428,6 +1519,15 @@ hipError_t launch_boruvka_row_scan(const BoruvkaExactArgs &a, hipStream_t st) {
 }
 
 int64_t boruvka_rows_tiled_slots(int64_t n, int max_rows) { return (int64_t)max_rows * ((n + 2047) / 2048); }
+
+hipError_t launch_cand_round(const CandRoundArgs &a, hipStream_t st) {
+    if (a.n <= 0) return hipSuccess;
+    if (a.d > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cand_round_kernel, dim3((unsigned)nblocks(a.n, 4)), dim3(256), 0, st, a);
+    FH_BRUTE_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(cand_round_list_kernel, dim3((unsigned)nblocks(a.n, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_boruvka_rows_tiled(const BoruvkaExactArgs &a, int row0, int nrows, float *slot_w, int *slot_j,
                                      hipStream_t st) {

@@ -235,3 +235,25 @@ def test_zero_cores_from_duplicates(monkeypatch, split):
     X = np.repeat(base, 7, axis=0)
     X = X[rng.permutation(X.shape[0])]
     _check(X, 5)
+
+
+@pytest.mark.parametrize("form", ["lists", "fallback", "scan"])
+def test_first_round_from_candidate_lists(monkeypatch, form):
+    """The first round from the kNN candidate lists (default), the same
+    with its listed rows handed to the MFMA scan (FH_CAND_ROUND_ROWS=0),
+    and the MFMA scan alone (FH_CAND_ROUND=0): one exact MST."""
+    X = workloads.gaussian_highdim(30000, 40, n_clusters=40, seed=44)
+    ref = mst_arrays(X, 10) if form == "lists" else None
+    if form == "fallback":
+        monkeypatch.setenv("FH_CAND_ROUND_ROWS", "0")
+    if form == "scan":
+        monkeypatch.setenv("FH_CAND_ROUND", "0")
+    got = mst_arrays(X, 10)
+    monkeypatch.delenv("FH_CAND_ROUND_ROWS", raising=False)
+    monkeypatch.setenv("FH_CAND_ROUND", "0")
+    want = mst_arrays(X, 10)
+    for u, v in zip(got, want):
+        np.testing.assert_array_equal(u, v)
+    if ref is not None:
+        for u, v in zip(ref, want):
+            np.testing.assert_array_equal(u, v)
