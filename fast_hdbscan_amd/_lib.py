@@ -77,6 +77,8 @@ class FHStats(ctypes.Structure):
         ("t_predict_ms", ctypes.c_double),
         ("scored_tile_pairs", ctypes.c_int64),
         ("tie_cycle_nodes", ctypes.c_int64),
+        ("scan_rounds", ctypes.c_int32),
+        ("pad2", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -1003,6 +1003,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     be.d = d;
     be.levels = levels;
     int64_t exact_rows = 0;
+    int scan_rounds = 0;
     // exact re-scan of the listed rows: tiled (64 rows per pass over the
     // positions), or one work-group per row (FH_TILED_RESCAN=0)
     bool tiled_rescan = true;
@@ -1084,10 +1085,11 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
                 cr.cert = ctx->bx_cert.p;
                 cr.rows = be.rows;
                 cr.row_count = be.row_count;
+                // the main-pass span times the MFMA round scans only
                 FH_HIP(mark(ctx, 8));
+                FH_HIP(mark(ctx, 9));
                 FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));
                 FH_HIP(fh::launch_cand_round(cr, st));
-                FH_HIP(mark(ctx, 9));
                 FH_HIP(hipMemcpyAsync(ctx->h_small + 4, be.row_count, sizeof(int), hipMemcpyDeviceToHost, st));
                 FH_HIP(hipStreamSynchronize(st));
                 int cap_rows = kCandRoundRows;
@@ -1095,6 +1097,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
                 from_cands = ctx->h_small[4] <= cap_rows;
             }
             if (!from_cands) {
+                ++scan_rounds;
                 fh::BruteScanArgs rs = bsa;
                 if (order_rows) {
                     FH_HIP(fh::launch_gather_comp(n, npad, ctx->bx_order.p, ctx->comp_p.p, ctx->bx_comp2.p, st));
@@ -1259,6 +1262,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
     S.rounds = rounds;
     S.tie_cycle_nodes += tie_nodes;
     S.exact_rows = exact_rows;
+    S.scan_rounds = scan_rounds;
     S.n_edges = base;
     S.t_tree_ms = span(ctx, 0, 1);
     S.t_knn_ms = span(ctx, 1, 2);

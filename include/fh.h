@@ -70,6 +70,10 @@ typedef struct {
     double t_predict_ms;      /* learned: inference + radii                    */
     int64_t scored_tile_pairs; /* high_dim: 128 x 128 tile pairs the Boruvka scans scored, summed */
     int64_t tie_cycle_nodes;   /* components whose cycle test needed pointer doubling, summed over rounds */
+    int32_t scan_rounds;       /* high_dim: rounds that ran the MFMA round scan (the first
+                                  round may come from the kNN candidate lists); their
+                                  main-pass time is t_query_main_ms */
+    int32_t pad2;
 } fh_stats;
 
 /* Create a context on HIP device `device`.  Returns NULL on failure (no

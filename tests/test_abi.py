@@ -65,3 +65,32 @@ def test_label_stage_rejects_bad_params():
     p = label_params(0)
     with pytest.raises(ValueError):
         labels_from_sorted_arrays(3, np.array([0, 1]), np.array([1, 2]), np.array([1.0, 2.0], np.float32), p)
+
+
+@pytest.mark.parametrize("struct", ["FHStats", "FHLabelParams", "FHLabelOrder", "FHCondensedTree", "FHLayerParams",
+                                    "FHLayers"])
+def test_ctypes_structs_match_the_header(tmp_path, struct):
+    """Every ctypes mirror has the C struct's size and field offsets
+    (gcc compiles a probe against include/fh.h)."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    cls = getattr(_lib, struct)
+    cname = {"FHStats": "fh_stats", "FHLabelParams": "fh_label_params", "FHLabelOrder": "fh_label_order",
+             "FHCondensedTree": "fh_condensed_tree", "FHLayerParams": "fh_layer_params",
+             "FHLayers": "fh_layers"}[struct]
+    fields = [f for f, _ in cls._fields_]
+    src = tmp_path / "probe.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fh.h"', "int main(void) {",
+             f'  printf("%zu\\n", sizeof({cname}));']
+    lines += [f'  printf("%zu\\n", offsetof({cname}, {f}));' for f in fields]
+    lines += ["  return 0;", "}"]
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(cls)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(cls, f).offset == off, f
