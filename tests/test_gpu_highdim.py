@@ -257,3 +257,19 @@ def test_first_round_from_candidate_lists(monkeypatch, form):
     if ref is not None:
         for u, v in zip(ref, want):
             np.testing.assert_array_equal(u, v)
+
+
+@pytest.mark.parametrize("n,ms", [(40000, 1), (40111, 15), (33333, 16)])
+def test_sampled_thresholds_and_first_round_edges(monkeypatch, n, ms):
+    """Sampled kNN thresholds (from 256 tiles on, k + 1 <= 16) with the
+    first round from the candidate lists at min_samples 1, at the largest
+    sampled width (k + 1 = 16), one above it (full thresholds) and sizes
+    that are not multiples of 128: the tree path's MST, bit for bit."""
+    X = workloads.gaussian_highdim(n, 24, n_clusters=30, seed=n % 1000 + ms)
+    st = {}
+    a = mst_arrays(X, ms, stats=st)
+    assert st["high_dim"] == 1
+    monkeypatch.setenv("FH_HIGH_DIM_FROM", "1000")
+    b = mst_arrays(X, ms)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
