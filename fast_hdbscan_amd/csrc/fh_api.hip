@@ -105,6 +105,7 @@ struct fh_ctx {
     DevBuf<float> bx_slotw;
     DevBuf<int> bx_slotj;
     DevBuf<unsigned char> bx_cert;  // first round from the candidate lists: settled rows
+    DevBuf<float> bx_cand_d;        // the kNN candidates' exact values, for the first round
     int brute_min_dim = 9;   // d from which the MFMA scans replace the tree searches (use_brute)
     // learned search radii
     DevBuf<float> ml_feat, ml_w, ml_g, ml_m, ml_v, ml_mu, ml_sd, ml_scratch, ml_tgt, ml_pred, ml_bound, ml_stats;
@@ -163,7 +164,7 @@ void for_each_buf(fh_ctx *c, F &&f) {
         c->ltpos, c->tactive, c->taux, c->tseg, c->nb_pos, c->cert_of_pos, c->sw_pos, c->sw_ties, c->sw_d, c->nb_d,
         c->mincore,
         c->cert, c->seed_int, c->bx_xc, c->bx_nrm, c->bx_thr, c->bx_rowmin, c->bx_cand, c->bx_cnt, c->bx_flags,
-        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->bx_rowkey, c->bx_nrm2, c->bx_core2, c->bx_tmin, c->bx_tminc, c->bx_tmaxc, c->bx_tU, c->bx_comp2, c->bx_order, c->bx_ovals, c->bx_okeys, c->bx_okeys_s, c->bx_otemp, c->bx_slotw, c->bx_slotj, c->bx_cert, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
+        c->bx_rowarg, c->bx_rows, c->bx_small, c->bx_nmax, c->bx_tilecomp, c->bx_scored, c->bx_rowkey, c->bx_nrm2, c->bx_core2, c->bx_tmin, c->bx_tminc, c->bx_tmaxc, c->bx_tU, c->bx_comp2, c->bx_order, c->bx_ovals, c->bx_okeys, c->bx_okeys_s, c->bx_otemp, c->bx_slotw, c->bx_slotj, c->bx_cert, c->bx_cand_d, c->ml_feat, c->ml_w, c->ml_g, c->ml_m,
         c->ml_v, c->ml_mu, c->ml_sd, c->ml_scratch, c->ml_tgt, c->ml_pred, c->ml_bound, c->ml_stats,
         c->ml_rows, c->ml_miss, c->kr_keys, c->kr_keys_s, c->kr_vals, c->kr_vals_s, c->kr_src, c->kr_dst,
         c->kr_w, c->kr_core, c->sk1, c->sk1s, c->sk2, c->sk2s, c->sv1, c->sv1s,
@@ -571,6 +572,8 @@ int brute_knn(fh_ctx *ctx, int64_t n, int d, int levels, int kq, int core_from_k
     ra.cand = ctx->bx_cand.p;
     ra.cand_cnt = ctx->bx_cnt.p;
     ra.cap = ctx->brute_cap;
+    FH_TRY(ensure(ctx, ctx->bx_cand_d, (size_t)n * ctx->brute_cap));
+    ra.cand_d = ctx->bx_cand_d.p;
     ra.n = n;
     ra.d = d;
     ra.k = kq;
@@ -1074,6 +1077,7 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
                 cr.comp_p = ctx->comp_p.p;
                 cr.cand = bsa.cand;
                 cr.cand_cnt = bsa.cand_cnt;
+                cr.cand_d = ctx->bx_cand_d.p;
                 cr.cap = bsa.cap;
                 cr.thr = bsa.thr;
                 cr.nrm = bsa.nrm;

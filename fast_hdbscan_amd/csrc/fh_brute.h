@@ -136,6 +136,7 @@ struct KnnRescoreArgs {
     const float *nrm;
     const unsigned int *nrm_max_bits;
     float margin_c;
+    float *cand_d;  // optional [n * cap]: the candidates' exact values, in list order
 };
 hipError_t launch_knn_rescore(const KnnRescoreArgs &a, hipStream_t st);
 
@@ -199,6 +200,7 @@ struct CandRoundArgs {
     const int *comp_p;
     const int *cand;
     const int *cand_cnt;
+    const float *cand_d;  // optional: the candidates' exact values (knn_rescore), else recomputed
     int cap;
     const float *thr;
     const float *nrm;

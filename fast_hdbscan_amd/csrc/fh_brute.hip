@@ -634,45 +634,62 @@ __global__ __launch_bounds__(256, MODE == 1 ? 2 : 3) void brute_scan_kernel(Brut
 // Exact rdist of every collected candidate, the k+1 smallest kept sorted in
 // LDS; rows with a tie among them, an overflowed list or fewer than k
 // candidates go to the tree search.
-__global__ __launch_bounds__(64) void knn_rescore_kernel(KnnRescoreArgs a) {
-    __shared__ float sv[65 * 64];
-    __shared__ int si[65 * 64];
-    const int tid = threadIdx.x;
-    const int64_t p = (int64_t)blockIdx.x * 64 + tid;
-    if (p >= a.n) return;
+// A wavefront per row (four per work-group): the lanes compute the
+// candidates' exact values in parallel (into LDS, and into cand_d for the
+// first Boruvka round), then lane 0 inserts them in list order into the
+// sorted k + 1 smallest, as one thread per row did.
+constexpr int kRescoreCap = 512;  // candidates per row held in LDS
+
+__global__ __launch_bounds__(256) void knn_rescore_kernel(KnnRescoreArgs a) {
+    __shared__ float cd[4][kRescoreCap];
+    __shared__ float sv[4][65];
+    __shared__ int si[4][65];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)blockIdx.x * 4 + w;
+    const bool live = p < a.n;
     const int K1 = a.k + 1;
-    float *v = sv + tid;
-    int *id = si + tid;
-    for (int t = 0; t < K1; ++t) {
-        v[t * 64] = kInfB;
-        id[t * 64] = -1;
-    }
-    const int cnt = a.cand_cnt[p];
+    const int cnt = live ? a.cand_cnt[p] : 0;
     bool flag = cnt > a.cap;
-    if (!flag) {
+    const int m = flag ? a.cap : cnt;  // the kept candidates (all of them, overflowed or not)
+    if (live) {
         const float *x = a.xp + p * a.d;
-        for (int t = 0; t < cnt; ++t) {
+        for (int t = lane; t < m; t += 64) {
             const int j = a.cand[p * a.cap + t];
             const float r = rdist<0>(x, a.xp + (int64_t)j * a.d, a.d);
-            if (r < v[(K1 - 1) * 64]) {
+            cd[w][t] = r;
+            if (a.cand_d) a.cand_d[p * a.cap + t] = r;
+        }
+    }
+    __syncthreads();
+    if (!live || lane != 0) return;
+    float *v = sv[w];
+    int *id = si[w];
+    for (int t = 0; t < K1; ++t) {
+        v[t] = kInfB;
+        id[t] = -1;
+    }
+    if (!flag) {
+        for (int t = 0; t < cnt; ++t) {
+            const float r = cd[w][t];
+            if (r < v[K1 - 1]) {
                 int i = K1 - 1;
-                while (i > 0 && v[(i - 1) * 64] > r) {
-                    v[i * 64] = v[(i - 1) * 64];
-                    id[i * 64] = id[(i - 1) * 64];
+                while (i > 0 && v[i - 1] > r) {
+                    v[i] = v[i - 1];
+                    id[i] = id[i - 1];
                     --i;
                 }
-                v[i * 64] = r;
-                id[i * 64] = j;
+                v[i] = r;
+                id[i] = a.cand[p * a.cap + t];
             }
         }
         for (int t = 0; t < a.k; ++t)
-            if (!(v[t * 64] < v[(t + 1) * 64])) flag = true;
+            if (!(v[t] < v[t + 1])) flag = true;
         // every column whose exact value is at most the k-th smallest found
         // has an approximate value at most that plus the margin; below the
         // threshold it was collected, so the k + 1 smallest found are the
         // k + 1 smallest of all columns (ties at the k-th included)
         const float marg = a.margin_c * kU * (a.nrm[p] + __uint_as_float(*a.nrm_max_bits));
-        if (!(v[(a.k - 1) * 64] + marg <= a.thr[p])) flag = true;
+        if (!(v[a.k - 1] + marg <= a.thr[p])) flag = true;
     }
     if (flag) {
         const int slot = atomicAdd(a.flag_count, 1);
@@ -681,10 +698,10 @@ __global__ __launch_bounds__(64) void knn_rescore_kernel(KnnRescoreArgs a) {
     }
     const int64_t row = a.perm[p];
     for (int t = 0; t < a.k; ++t) {
-        a.out_d[row * a.k + t] = v[t * 64];
-        a.out_i[row * a.k + t] = a.perm[id[t * 64]];
+        a.out_d[row * a.k + t] = v[t];
+        a.out_i[row * a.k + t] = a.perm[id[t]];
     }
-    const float c = a.core_from_knn ? v[(a.k - 1) * 64] : 0.0f;
+    const float c = a.core_from_knn ? v[a.k - 1] : 0.0f;
     a.core_o[row] = c;
     a.core_p[p] = c;
 }
@@ -1121,7 +1138,7 @@ __global__ __launch_bounds__(256) void cand_round_kernel(CandRoundArgs a) {
         if (t < m) {
             j = a.cand[p * a.cap + t];
             if (a.comp_p[j] != comp) {
-                float r = rdist<0>(x, a.xp + (int64_t)j * a.d, a.d);
+                float r = a.cand_d ? a.cand_d[p * a.cap + t] : rdist<0>(x, a.xp + (int64_t)j * a.d, a.d);
                 const float cj = a.core_p[j];
                 r = r > ci ? r : ci;
                 w = r > cj ? r : cj;
@@ -1481,8 +1498,8 @@ hipError_t launch_tile_comp(int64_t n, int64_t npad, const int *comp_p, int *til
 }
 
 hipError_t launch_knn_rescore(const KnnRescoreArgs &a, hipStream_t st) {
-    if (a.k + 1 > 65) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(knn_rescore_kernel, dim3(nblocks(a.n, 64)), dim3(64), 0, st, a);
+    if (a.k + 1 > 65 || a.cap > kRescoreCap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(knn_rescore_kernel, dim3(nblocks(a.n, 4)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
