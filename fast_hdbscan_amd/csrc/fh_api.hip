@@ -1130,7 +1130,13 @@ int run_mst(fh_ctx *ctx, const float *Xd, int64_t n, int d, int ms, const float 
                 else if (rs.row_key)
                     FH_HIP(fh::launch_unpack_row_keys(n, rs.row_key, rs.row_min, rs.row_arg, st));
                 FH_HIP(mark(ctx, 9));
-                FH_HIP(fh::launch_boruvka_upper(be, st));
+                // components' upper bounds from the approximate minima plus
+                // the margin (FH_EXACT_UPPER=1: exact weights of the row minima)
+                const char *eu = std::getenv("FH_EXACT_UPPER");
+                if (eu && std::atoi(eu) != 0)
+                    FH_HIP(fh::launch_boruvka_upper(be, st));
+                else
+                    FH_HIP(fh::launch_boruvka_upper_approx(be, st));
                 FH_HIP(hipMemsetAsync(be.row_count, 0, sizeof(int), st));
                 FH_HIP(fh::launch_boruvka_rows(be, st));
             }

@@ -181,6 +181,9 @@ struct BoruvkaExactArgs {
     int d, levels;
 };
 hipError_t launch_boruvka_upper(const BoruvkaExactArgs &a, hipStream_t st);
+// The same upper bound from the approximate row minima plus the margin
+// (no exact weights): bound[c] <= min over c's rows of row_min + margin.
+hipError_t launch_boruvka_upper_approx(const BoruvkaExactArgs &a, hipStream_t st);
 hipError_t launch_boruvka_rows(const BoruvkaExactArgs &a, hipStream_t st);
 hipError_t launch_boruvka_row_scan(const BoruvkaExactArgs &a, hipStream_t st);
 // First Boruvka round from the kNN candidate lists (all columns with

@@ -901,6 +901,35 @@ __global__ void boruvka_upper_kernel(BoruvkaExactArgs a) {
     atomicMin(reinterpret_cast<unsigned int *>(a.bound + a.comp_p[p]), __float_as_uint(w));
 }
 
+// The same bound without the exact weights: a row's approximate value
+// m_i = max(approx, core_i, core_j) is within the margin of the exact
+// weight of (i, j0), so m_i + margin bounds it from above (the margin's
+// factor two covers this sum's rounding).  No gather of j0's row.
+// One atomic per run of equal components within a wavefront (rows are in
+// tree order, where later rounds' components are long runs): a million
+// atomics onto a few hundred slots serialise in L2.
+__global__ void boruvka_upper_approx_kernel(BoruvkaExactArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int key = -1;
+    float u = kInfB;
+    if (p < a.n && a.row_arg[p] >= 0) {
+        key = a.comp_p[p];
+        u = a.row_min[p] + a.margin_c * kU * (a.nrm[p] + __uint_as_float(*a.nrm_max_bits));
+    }
+    // segmented inclusive minimum over lanes of equal key: the last lane of
+    // each run then holds its run's minimum
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const float ou = __shfl_up(u, o, 64);
+        const int ok = __shfl_up(key, o, 64);
+        if (lane >= o && ok == key) u = ou < u ? ou : u;
+    }
+    const int nk = __shfl_down(key, 1, 64);
+    if (key >= 0 && (lane == 63 || nk != key))
+        atomicMin(reinterpret_cast<unsigned int *>(a.bound + key), __float_as_uint(u));
+}
+
 __global__ void boruvka_rows_kernel(BoruvkaExactArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
@@ -1522,6 +1551,11 @@ hipError_t launch_knn_block(const KnnWaveArgs &a, int count, int64_t n, hipStrea
 
 hipError_t launch_boruvka_upper(const BoruvkaExactArgs &a, hipStream_t st) {
     hipLaunchKernelGGL(boruvka_upper_kernel, dim3(nblocks(a.n, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_boruvka_upper_approx(const BoruvkaExactArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(boruvka_upper_approx_kernel, dim3(nblocks(a.n, 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

@@ -273,3 +273,17 @@ def test_sampled_thresholds_and_first_round_edges(monkeypatch, n, ms):
     b = mst_arrays(X, ms)
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+@pytest.mark.parametrize("exact_upper", ["0", "1"])
+def test_round_upper_bounds_approx_or_exact(monkeypatch, exact_upper):
+    """Components' upper bounds from the approximate row minima plus the
+    margin (default) or from the exact weights of the row minima
+    (FH_EXACT_UPPER=1): the same MST as the tree path."""
+    monkeypatch.setenv("FH_EXACT_UPPER", exact_upper)
+    X = workloads.gaussian_highdim(20000, 48, n_clusters=25, seed=46)
+    a = mst_arrays(X, 10)
+    monkeypatch.setenv("FH_HIGH_DIM_FROM", "1000")
+    b = mst_arrays(X, 10)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
